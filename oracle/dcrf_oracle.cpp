@@ -1,0 +1,1295 @@
+// CPU oracle for the dense-CRF hot path.  TEST INFRASTRUCTURE ONLY.
+//
+// This file restates, in plain single-threaded C++, the algorithm of the
+// reference (thwjoy/densecrf_ho) for the permutohedral-lattice filter, the
+// ordered ("upper minus lower") filter, the Frank-Wolfe QP, the proximal LP and
+// the Pn higher-order terms.  It exists to CHECK the HIP engine: only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+// product path never calls it.
+//
+// Two instantiations:
+//   * float  - follows the reference's float arithmetic step by step (SSE2
+//              lattice semantics, Eigen 3.2.7 SSE2 reduction order for the
+//              short column sums), so it can be pinned against the compiled
+//              reference;
+//   * double - the same control flow with every field in FP64 on the same
+//              (float-built) lattice structure: the FP64 oracle.
+//
+// Each function cites the reference file:line it restates.  Pinning status of
+// every function is listed in the header comment of tests/test_oracle.py.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// Reduction order of Eigen 3.2.7 on SSE2 (include/Eigen/src/Core/Redux.h:193-245,
+// arch/SSE/PacketMath.h:423-431): two packet accumulators over the aligned
+// span, a horizontal add (a0+a2)+(a1+a3), then the unaligned head and tail.
+// `astart` is the index of the first 16-byte-aligned element.
+// ---------------------------------------------------------------------------
+template <class F>
+static float eigen_sum_f(int size, int astart, F val) {
+  const int ps = 4;
+  if (astart > size) astart = size;
+  const int as2 = ((size - astart) / (2 * ps)) * (2 * ps);
+  const int as1 = ((size - astart) / ps) * ps;
+  const int ae2 = astart + as2, ae = astart + as1;
+  float res;
+  if (as1) {
+    float p0[4], p1[4];
+    for (int q = 0; q < 4; ++q) p0[q] = val(astart + q);
+    if (as1 > ps) {
+      for (int q = 0; q < 4; ++q) p1[q] = val(astart + ps + q);
+      for (int i = astart + 2 * ps; i < ae2; i += 2 * ps)
+        for (int q = 0; q < 4; ++q) {
+          p0[q] = p0[q] + val(i + q);
+          p1[q] = p1[q] + val(i + ps + q);
+        }
+      for (int q = 0; q < 4; ++q) p0[q] = p0[q] + p1[q];
+      if (ae > ae2)
+        for (int q = 0; q < 4; ++q) p0[q] = p0[q] + val(ae2 + q);
+    }
+    float t0 = p0[0] + p0[2], t1 = p0[1] + p0[3];
+    res = t0 + t1;
+    for (int i = 0; i < astart; ++i) res = res + val(i);
+    for (int i = ae; i < size; ++i) res = res + val(i);
+  } else {
+    res = val(0);
+    for (int i = 1; i < size; ++i) res = res + val(i);
+  }
+  return res;
+}
+
+template <class F>
+static double eigen_sum_d(long size, F val) {
+  const long ps = 2;
+  const long as2 = (size / (2 * ps)) * (2 * ps);
+  const long as1 = (size / ps) * ps;
+  double res;
+  if (size == 0) return 0.0;
+  if (as1) {
+    double p0[2] = {val(0), val(1)}, p1[2];
+    if (as1 > ps) {
+      p1[0] = val(2);
+      p1[1] = val(3);
+      for (long i = 2 * ps; i < as2; i += 2 * ps) {
+        p0[0] += val(i);
+        p0[1] += val(i + 1);
+        p1[0] += val(i + 2);
+        p1[1] += val(i + 3);
+      }
+      p0[0] += p1[0];
+      p0[1] += p1[1];
+      if (as1 > as2) {
+        p0[0] += val(as2);
+        p0[1] += val(as2 + 1);
+      }
+    }
+    res = p0[0] + p0[1];
+    for (long i = as1; i < size; ++i) res += val(i);
+  } else {
+    res = val(0);
+    for (long i = 1; i < size; ++i) res += val(i);
+  }
+  return res;
+}
+
+// first aligned element of column `col` of an L-row float matrix whose storage
+// is 16-byte aligned (include/Eigen/src/Core/util/Memory.h:467-489).
+static inline int col_astart(long col, int L) { return (int)((4 - ((col * L) & 3)) & 3); }
+
+// Sum of a column: float mimics Eigen, double is plain.
+template <class T> struct ColSum;
+template <> struct ColSum<float> {
+  static float run(const float* c, int L, long col) {
+    return eigen_sum_f(L, std::min(col_astart(col, L), L), [&](int i) { return c[i]; });
+  }
+  static float run_aligned(const float* c, int L) {
+    return eigen_sum_f(L, 0, [&](int i) { return c[i]; });
+  }
+  static float dot_aligned(const float* a, const float* b, int L) {
+    return eigen_sum_f(L, 0, [&](int i) { return a[i] * b[i]; });
+  }
+};
+template <> struct ColSum<double> {
+  static double run(const double* c, int L, long) {
+    double s = 0;
+    for (int i = 0; i < L; ++i) s += c[i];
+    return s;
+  }
+  static double run_aligned(const double* c, int L) { return run(c, L, 0); }
+  static double dot_aligned(const double* a, const double* b, int L) {
+    double s = 0;
+    for (int i = 0; i < L; ++i) s += a[i] * b[i];
+    return s;
+  }
+};
+
+// Sum of a long aligned vector (qp_values.sum(), z.sum()).
+static float vec_sum(const float* v, long n) {
+  if (n == 0) return 0.f;
+  return eigen_sum_f((int)n, 0, [&](int i) { return v[i]; });
+}
+static double vec_sum(const double* v, long n) {
+  double s = 0;
+  for (long i = 0; i < n; ++i) s += v[i];
+  return s;
+}
+
+// dotProduct (src/eigen_utils.cpp:68-74): cast both to double, multiply, sum.
+template <class T>
+static double dot(const T* a, const T* b, long n) {
+  return eigen_sum_d(n, [&](long i) { return (double)a[i] * (double)b[i]; });
+}
+
+// ---------------------------------------------------------------------------
+// Permutohedral lattice (src/permutohedral.cpp:56-333, SSE2 path).
+// ---------------------------------------------------------------------------
+struct KeyHash {
+  int d = 0;
+  std::vector<int16_t> keys;  // id-major
+  std::vector<int> table;
+  size_t mask = 0;
+  int count = 0;
+  void reset(int dd, size_t expect) {
+    d = dd;
+    size_t cap = 16;
+    while (cap < 2 * expect + 16) cap <<= 1;
+    table.assign(cap, -1);
+    mask = cap - 1;
+    keys.clear();
+    keys.reserve(expect * d);
+    count = 0;
+  }
+  uint64_t h(const int16_t* k) const {
+    uint64_t x = 0x9E3779B97F4A7C15ull;
+    for (int i = 0; i < d; ++i) {
+      x ^= (uint16_t)k[i];
+      x *= 0xff51afd7ed558ccdull;
+      x ^= x >> 29;
+    }
+    return x;
+  }
+  void grow() {
+    std::vector<int> nt(table.size() * 2, -1);
+    mask = nt.size() - 1;
+    for (int e = 0; e < count; ++e) {
+      size_t s = h(&keys[(size_t)e * d]) & mask;
+      while (nt[s] >= 0) s = (s + 1) & mask;
+      nt[s] = e;
+    }
+    table.swap(nt);
+  }
+  // ids are assigned in insertion order, as in HashTable::find (:100-128)
+  int find(const int16_t* k, bool create) {
+    if (create && 2 * (size_t)(count + 1) >= table.size()) grow();
+    size_t s = h(k) & mask;
+    while (true) {
+      int e = table[s];
+      if (e < 0) {
+        if (!create) return -1;
+        keys.insert(keys.end(), k, k + d);
+        table[s] = count;
+        return count++;
+      }
+      if (std::memcmp(&keys[(size_t)e * d], k, sizeof(int16_t) * d) == 0) return e;
+      s = (s + 1) & mask;
+    }
+  }
+};
+
+struct Lattice {
+  int N = 0, d = 0, M = 0, Npad = 0;
+  std::vector<int> offset, rank;  // Npad*(d+1)
+  std::vector<float> bary;        // Npad*(d+1)
+  std::vector<int16_t> keys;      // M*d
+  std::vector<int> nbr;           // ((j*M+i)*2 + {0:n1, 1:n2})
+
+  // Permutohedral::init, SSE2 path (src/permutohedral.cpp:152-333).  Pixels are
+  // processed in blocks of 4 lanes; a lane past N sees a zero feature vector and
+  // still inserts its d+1 keys (:203-208, :273-288).
+  void init(const float* feat, int N_, int d_) {
+    N = N_;
+    d = d_;
+    Npad = (N + 3) / 4 * 4;
+    const int d1 = d + 1;
+    offset.assign((size_t)Npad * d1, 0);
+    rank.assign((size_t)Npad * d1, 0);
+    bary.assign((size_t)Npad * d1, 0.f);
+    std::vector<int16_t> canonical(d1 * d1);
+    for (int i = 0; i <= d; ++i) {  // :183-188
+      for (int j = 0; j <= d - i; ++j) canonical[i * d1 + j] = (int16_t)i;
+      for (int j = d - i + 1; j <= d; ++j) canonical[i * d1 + j] = (int16_t)(i - d1);
+    }
+    const float inv_std = (float)(std::sqrt(2.0 / 3.0) * d1);  // :191
+    std::vector<float> scale(d);
+    for (int i = 0; i < d; ++i)  // :193-194
+      scale[i] = (float)(1.0 / std::sqrt((double)((i + 2) * (i + 1))) * (double)inv_std);
+    const float invdp1 = 1.0f / d1, dp1 = (float)d1;
+    KeyHash ht;
+    ht.reset(d, (size_t)N * d1 / 4 + 16);
+    std::vector<float> f(d), elev(d1), rem0(d1), rk(d1), bc(d + 2);
+    std::vector<int16_t> key(d1);
+    for (int k = 0; k < Npad; ++k) {
+      for (int j = 0; j < d; ++j) f[j] = k < N ? feat[(size_t)k * d + j] : 0.0f;
+      float sm = 0.f;  // elevate (:213-219)
+      for (int j = d; j > 0; --j) {
+        float cf = f[j - 1] * scale[j - 1];
+        float jc = (float)j * cf;
+        elev[j] = sm - jc;
+        sm = sm + cf;
+      }
+      elev[0] = sm;
+      float sum = 0.f;  // round to the 0-coloured simplex (:222-232)
+      for (int i = 0; i <= d; ++i) {
+        float v = invdp1 * elev[i];
+        v = std::nearbyint(v);
+        rem0[i] = v * dp1;
+        sum = sum + v;
+      }
+      for (int i = 0; i <= d; ++i) rk[i] = 0.f;  // rank (:235-245)
+      for (int i = 0; i < d; ++i) {
+        float di = elev[i] - rem0[i];
+        for (int j = i + 1; j <= d; ++j) {
+          float dj = elev[j] - rem0[j];
+          float c = (di < dj) ? 1.f : 0.f;
+          rk[i] = rk[i] + c;
+          rk[j] = rk[j] + (1.f - c);
+        }
+      }
+      for (int i = 0; i <= d; ++i) {  // wrap (:248-254)
+        rk[i] = rk[i] + sum;
+        float add = rk[i] < 0.f ? dp1 : 0.f;
+        float sub = rk[i] >= dp1 ? dp1 : 0.f;
+        float as = add - sub;
+        rk[i] = rk[i] + as;
+        rem0[i] = rem0[i] + as;
+      }
+      for (int i = 0; i < d + 2; ++i) bc[i] = 0.f;  // barycentric (:257-275)
+      for (int i = 0; i <= d; ++i) {
+        float v = (elev[i] - rem0[i]) * invdp1;
+        int p = (int)((float)d - rk[i]);
+        bc[p] = bc[p] + v;
+        bc[p + 1] = bc[p + 1] - v;
+      }
+      bc[0] = bc[0] + (1.f + bc[d + 1]);
+      for (int r = 0; r <= d; ++r) {  // keys (:280-287)
+        for (int i = 0; i < d; ++i)
+          key[i] = (int16_t)(int)(rem0[i] + (float)canonical[r * d1 + (int)rk[i]]);
+        offset[(size_t)k * d1 + r] = ht.find(key.data(), true);
+        rank[(size_t)k * d1 + r] = (int)rk[r];
+        bary[(size_t)k * d1 + r] = bc[r];
+      }
+    }
+    M = ht.count;
+    keys = ht.keys;
+    nbr.assign((size_t)d1 * M * 2, -1);  // neighbours (:317-330)
+    std::vector<int16_t> n1(d1), n2(d1);
+    for (int j = 0; j <= d; ++j)
+      for (int i = 0; i < M; ++i) {
+        const int16_t* kk = &keys[(size_t)i * d];
+        for (int q = 0; q < d; ++q) {
+          n1[q] = (int16_t)(kk[q] - 1);
+          n2[q] = (int16_t)(kk[q] + 1);
+        }
+        if (j < d) {
+          n1[j] = (int16_t)(kk[j] + d);
+          n2[j] = (int16_t)(kk[j] - d);
+        }
+        nbr[((size_t)j * M + i) * 2 + 0] = ht.find(n1.data(), false);
+        nbr[((size_t)j * M + i) * 2 + 1] = ht.find(n2.data(), false);
+      }
+  }
+
+  // Permutohedral::compute (src/permutohedral.cpp:1079-1087): V<=2 takes
+  // seqCompute (:488-540), otherwise sseCompute (:998-1058).  Row 0 of the
+  // value buffer is the zero pad for absent neighbours.
+  void filter_f(const float* in, float* out, int V, bool reverse) const {
+    const int d1 = d + 1;
+    std::vector<float> a((size_t)(M + 2) * V, 0.f), b((size_t)(M + 2) * V, 0.f);
+    float* values = a.data();
+    float* nv = b.data();
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j <= d; ++j) {
+        int o = offset[(size_t)i * d1 + j] + 1;
+        float w = bary[(size_t)i * d1 + j];
+        for (int k = 0; k < V; ++k) values[(size_t)o * V + k] = values[(size_t)o * V + k] + w * in[(size_t)i * V + k];
+      }
+    for (int jj = 0; jj <= d; ++jj) {
+      int j = reverse ? d - jj : jj;
+      for (int i = 0; i < M; ++i) {
+        const float* ov = values + (size_t)(i + 1) * V;
+        float* nw = nv + (size_t)(i + 1) * V;
+        int n1 = nbr[((size_t)j * M + i) * 2] + 1, n2 = nbr[((size_t)j * M + i) * 2 + 1] + 1;
+        const float* v1 = values + (size_t)n1 * V;
+        const float* v2 = values + (size_t)n2 * V;
+        for (int k = 0; k < V; ++k) {
+          float s = v1[k] + v2[k];
+          if (V <= 2)
+            nw[k] = (float)((double)ov[k] + 0.5 * (double)s);  // :518 (double expression)
+          else
+            nw[k] = ov[k] + 0.5f * s;  // :1035
+        }
+      }
+      std::swap(values, nv);
+    }
+    const float alpha = 1.0f / (1 + std::pow(2.0f, (float)-d));
+    for (int i = 0; i < N; ++i) {
+      float* o_ = out + (size_t)i * V;
+      std::vector<float> acc(V, 0.f);
+      for (int j = 0; j <= d; ++j) {
+        int o = offset[(size_t)i * d1 + j] + 1;
+        float w = bary[(size_t)i * d1 + j];
+        if (V <= 2) {
+          for (int k = 0; k < V; ++k) acc[k] = acc[k] + (w * values[(size_t)o * V + k]) * alpha;  // :533
+        } else {
+          float wa = w * alpha;  // :1048
+          for (int k = 0; k < V; ++k) acc[k] = acc[k] + wa * values[(size_t)o * V + k];
+        }
+      }
+      for (int k = 0; k < V; ++k) o_[k] = acc[k];
+    }
+  }
+
+  // FP64 filter on the same lattice structure.
+  void filter_d(const double* in, double* out, int V, bool reverse) const {
+    const int d1 = d + 1;
+    std::vector<double> a((size_t)(M + 2) * V, 0.0), b((size_t)(M + 2) * V, 0.0);
+    double* values = a.data();
+    double* nv = b.data();
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j <= d; ++j) {
+        int o = offset[(size_t)i * d1 + j] + 1;
+        double w = bary[(size_t)i * d1 + j];
+        for (int k = 0; k < V; ++k) values[(size_t)o * V + k] += w * in[(size_t)i * V + k];
+      }
+    for (int jj = 0; jj <= d; ++jj) {
+      int j = reverse ? d - jj : jj;
+      for (int i = 0; i < M; ++i) {
+        int n1 = nbr[((size_t)j * M + i) * 2] + 1, n2 = nbr[((size_t)j * M + i) * 2 + 1] + 1;
+        for (int k = 0; k < V; ++k)
+          nv[(size_t)(i + 1) * V + k] =
+              values[(size_t)(i + 1) * V + k] + 0.5 * (values[(size_t)n1 * V + k] + values[(size_t)n2 * V + k]);
+      }
+      std::swap(values, nv);
+    }
+    const double alpha = 1.0 / (1.0 + std::pow(2.0, -d));
+    for (int i = 0; i < N; ++i) {
+      for (int k = 0; k < V; ++k) out[(size_t)i * V + k] = 0;
+      for (int j = 0; j <= d; ++j) {
+        int o = offset[(size_t)i * d1 + j] + 1;
+        double wa = (double)bary[(size_t)i * d1 + j] * alpha;
+        for (int k = 0; k < V; ++k) out[(size_t)i * V + k] += wa * values[(size_t)o * V + k];
+      }
+    }
+  }
+
+  // Bin rules of the ordered filter (src/permutohedral.cpp:548-589):
+  // lower bin  = min(floor(float(q*10)), 9)
+  // upper bin  = max(floor((double(q) - 1e-9) * 10), 0)
+  static int bin_lo(float q) { return std::min((int)std::floor(q * 10), 9); }
+  static int bin_up(float q) { return std::max((int)std::floor(((double)q - 1e-9) * 10), 0); }
+
+  // seqCompute_upper_minus_lower_ord (src/permutohedral.cpp:590-701); T is the
+  // accumulation type, the bins always come from the float q.
+  template <class T>
+  void ordered(const float* q, T* out, int V) const {
+    const int d1 = d + 1, R = 10;
+    const T alpha = (T)(1.0f / (1 + std::pow(2.0f, (float)-d)));
+    const T alpha_d = std::is_same<T, double>::value ? (T)(1.0 / (1.0 + std::pow(2.0, -d))) : alpha;
+    std::vector<T> a((size_t)(M + 2) * V * R), b((size_t)(M + 2) * V * R);
+    for (int pass = 0; pass < 2; ++pass) {
+      const bool upper = pass == 1;
+      std::fill(a.begin(), a.end(), (T)0);
+      std::fill(b.begin(), b.end(), (T)0);
+      T* values = a.data();
+      T* nv = b.data();
+      for (int i = 0; i < N; ++i)  // splat (:609-618, :657-666)
+        for (int j = 0; j <= d; ++j) {
+          int o = offset[(size_t)i * d1 + j] + 1;
+          T w = (T)bary[(size_t)i * d1 + j];
+          for (int k = 0; k < V; ++k) {
+            float qq = q[(size_t)i * V + k];
+            T* s = values + ((size_t)o * V + k) * R;
+            if (upper) {
+              for (int c = bin_up(qq); c < R; ++c) s[c] = s[c] + w;
+            } else {
+              for (int c = 0, e = bin_lo(qq); c <= e; ++c) s[c] = s[c] + w;
+            }
+          }
+        }
+      for (int j = 0; j <= d; ++j) {  // blur (:620-636)
+        for (int i = 0; i < M; ++i) {
+          int n1 = nbr[((size_t)j * M + i) * 2] + 1, n2 = nbr[((size_t)j * M + i) * 2 + 1] + 1;
+          for (int k = 0; k < V; ++k) {
+            const T* ov = values + ((size_t)(i + 1) * V + k) * R;
+            const T* v1 = values + ((size_t)n1 * V + k) * R;
+            const T* v2 = values + ((size_t)n2 * V + k) * R;
+            T* nw = nv + ((size_t)(i + 1) * V + k) * R;
+            for (int c = 0; c < R; ++c) nw[c] = ov[c] + (T)0.5 * (v1[c] + v2[c]);
+          }
+        }
+        std::swap(values, nv);
+      }
+      for (int i = 0; i < N; ++i) {  // slice (:638-650, :686-696)
+        if (!upper)
+          for (int k = 0; k < V; ++k) out[(size_t)i * V + k] = 0;
+        for (int j = 0; j <= d; ++j) {
+          int o = offset[(size_t)i * d1 + j] + 1;
+          float wf = bary[(size_t)i * d1 + j];
+          T coef;
+          if (std::is_same<T, float>::value)
+            coef = upper ? (T)(wf * (float)alpha) : (T)(-wf * (float)alpha);
+          else
+            coef = upper ? (T)wf * alpha_d : -(T)wf * alpha_d;
+          for (int k = 0; k < V; ++k) {
+            float qq = q[(size_t)i * V + k];
+            int c = upper ? bin_up(qq) : bin_lo(qq);
+            out[(size_t)i * V + k] = out[(size_t)i * V + k] + values[((size_t)o * V + k) * R + c] * coef;
+          }
+        }
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Dense CRF with Potts pairwise terms and Pn superpixel layers.
+// ---------------------------------------------------------------------------
+struct Pairwise {
+  Lattice lat;
+  std::vector<float> feat;
+  int d = 0;
+  float w = 1.f;
+  int ntype = 3;               // NormalizationType: 0 none, 1 before, 2 after, 3 symmetric
+  std::vector<float> norm_f;   // float norm (DenseKernel::initLattice, src/pairwise.cpp:40-57)
+  std::vector<double> norm_d;  // same rounding then widened
+};
+
+template <class T>
+struct CRF {
+  int N, L;
+  std::vector<T> U;  // L*N, label-contiguous
+  std::vector<Pairwise*> pw;
+  // superpixels (global region index across layers, in layer order)
+  int R = 0;
+  std::vector<std::vector<int>> members;
+  std::vector<T> e_r;  // exp_of_superpixels_
+  std::vector<T> c_r;  // constants_
+  std::vector<T> z_labels;  // L*R after QP
+
+  CRF(int n, int l) : N(n), L(l), U((size_t)n * l, (T)0) {}
+
+  // DenseKernel::filter (src/pairwise.cpp:58-75) + PottsCompatibility::apply
+  // (src/labelcompatibility.cpp:49-52).  `no_norm` selects the NO_NORMALIZATION
+  // copy built by addPairwiseEnergy (src/densecrf.cpp:164-168).
+  void apply(int k, const T* in, T* out, bool no_norm = false, bool transpose = false) const {
+    const Pairwise& p = *pw[k];
+    const size_t NL = (size_t)N * L;
+    std::vector<T> tmp(NL);
+    const int nt = no_norm ? 0 : p.ntype;
+    const bool pre = nt == 3 || (nt == 1 && !transpose) || (nt == 2 && transpose);
+    const bool post = nt == 3 || (nt == 1 && transpose) || (nt == 2 && !transpose);
+    for (int i = 0; i < N; ++i)
+      for (int l = 0; l < L; ++l) {
+        T x = in[(size_t)i * L + l];
+        tmp[(size_t)i * L + l] = pre ? x * nrm(p, i) : x;
+      }
+    filt(p.lat, tmp.data(), tmp.data(), L, transpose);
+    for (int i = 0; i < N; ++i)
+      for (int l = 0; l < L; ++l) {
+        T x = tmp[(size_t)i * L + l];
+        if (post) x = x * nrm(p, i);
+        out[(size_t)i * L + l] = -(T)p.w * x;
+      }
+  }
+  static T nrm(const Pairwise& p, int i);
+  static void filt(const Lattice& lat, const T* in, T* out, int V, bool rev);
+
+  // PairwisePotential::apply_upper_minus_lower_ord (src/pairwise.cpp:214-228)
+  void apply_ord(int k, const float* q01, T* out) const {
+    for (size_t i = 0; i < (size_t)N * L; ++i)
+      if (q01[i] > 1 || q01[i] < 0) throw std::runtime_error("Value out of bounds");
+    pw[k]->lat.ordered<T>(q01, out, L);
+    for (size_t i = 0; i < (size_t)N * L; ++i) out[i] = -(T)pw[k]->w * out[i];
+  }
+
+  // ---- superpixel helpers, bug-compatible (src/densecrf.cpp:58-102) --------
+  // pixels -> regions: res(l,r) = sum_{pixel < |r|} p(l,pixel)  (loop index!)
+  std::vector<T> sp_to_regions(const std::vector<T>& p) const {
+    std::vector<T> res((size_t)L * R, (T)0);
+    for (int r = 0; r < R; ++r)
+      for (int l = 0; l < L; ++l)
+        for (size_t pix = 0; pix < members[r].size(); ++pix)
+          res[(size_t)r * L + l] = res[(size_t)r * L + l] + p[pix * L + l];
+    return res;
+  }
+  // regions -> pixels: res(l,pixel) += p(l,r) for pixel < |r|
+  std::vector<T> sp_to_pixels(const std::vector<T>& p) const {
+    std::vector<T> res((size_t)L * N, (T)0);
+    for (int r = 0; r < R; ++r)
+      for (int l = 0; l < L; ++l)
+        for (size_t pix = 0; pix < members[r].size(); ++pix)
+          res[pix * L + l] = res[pix * L + l] + p[(size_t)r * L + l];
+    return res;
+  }
+  // scalar form (:58-72), accumulated in T
+  T sp_scalar(const std::vector<T>& p1, const std::vector<T>& p2) const {
+    T res = 0;
+    for (int r = 0; r < R; ++r)
+      for (size_t pix = 0; pix < members[r].size(); ++pix)
+        for (int l = 0; l < L; ++l) res = res + p1[(size_t)r * L + l] * p2[pix * L + l];
+    return res;
+  }
+
+  // ---- QP (src/densecrf.cpp:340-442) ---------------------------------------
+  struct QPStats {
+    std::vector<double> energy, num, denom, step;
+    int iters = 0;
+  };
+  std::vector<T> qp(const std::vector<T>& init, int max_iter, bool ref_stop, QPStats* st) {
+    const size_t NL = (size_t)N * L, LR = (size_t)L * R;
+    std::vector<T> Q = init, grad_y(NL), tmp(NL), s(NL), sx(NL), psisx(NL);
+    std::vector<T> constant(LR), grad_z(LR), cgz(LR, 0), sx_z(LR), z(LR, (T)1);
+    for (int r = 0; r < R; ++r)
+      for (int l = 0; l < L; ++l) constant[(size_t)r * L + l] = e_r[r];
+    auto Qm1 = [&]() {
+      std::vector<T> v(NL);
+      for (size_t i = 0; i < NL; ++i) v[i] = Q[i] - (T)1;
+      return v;
+    };
+    auto descent_z = [&]() {  // src/qp.cpp:17-25
+      for (size_t i = 0; i < LR; ++i) cgz[i] = grad_z[i] >= 0 ? 0 : 1;
+    };
+    {  // :373-375
+      std::vector<T> nr = sp_to_regions(Qm1());
+      for (size_t i = 0; i < LR; ++i) grad_z[i] = ((T)1 + nr[i]) * constant[i];
+      descent_z();
+    }
+    grad_y = U;  // :377-382
+    for (size_t k = 0; k < pw.size(); ++k) {
+      apply((int)k, Q.data(), tmp.data());
+      for (size_t i = 0; i < NL; ++i) grad_y[i] = grad_y[i] + (T)2 * tmp[i];
+    }
+    auto ho_bcast = [&]() {  // multiplySuperPixels((z - 1) .* constant)
+      std::vector<T> t(LR);
+      for (size_t i = 0; i < LR; ++i) t[i] = (z[i] - (T)1) * constant[i];
+      return sp_to_pixels(t);
+    };
+    auto ho_energy = [&]() -> double {  // :387 / :435
+      std::vector<T> zc(LR), omz(LR);
+      for (size_t i = 0; i < LR; ++i) {
+        zc[i] = z[i] * constant[i];
+        omz[i] = ((T)1 - z[i]) * constant[i];
+      }
+      T zs = vec_sum(zc.data(), (long)LR);
+      T sc = sp_scalar(omz, Qm1());
+      return (double)(T)(zs + sc);
+    };
+    {
+      std::vector<T> hb = ho_bcast();
+      for (size_t i = 0; i < NL; ++i) grad_y[i] = grad_y[i] + hb[i];
+    }
+    double old_energy = std::numeric_limits<double>::max();
+    double energy = 0;  // compute_LR_QP_value(Q, 0) (:940-957)
+    {
+      energy += dot(U.data(), Q.data(), NL);
+      energy -= 0.0;
+      for (size_t k = 0; k < pw.size(); ++k) {
+        apply((int)k, Q.data(), tmp.data());
+        energy += dot(Q.data(), tmp.data(), NL);
+      }
+      energy += 0.0;
+      energy += ho_energy();
+    }
+    if (st) st->energy.push_back(energy);
+    int it = 0;
+    while (ref_stop ? (old_energy - energy) > 100.0 : it < max_iter) {
+      if (ref_stop && max_iter > 0 && it >= max_iter) break;
+      old_energy = energy;
+      ++it;
+      for (int i = 0; i < N; ++i) {  // descent_direction (src/qp.cpp:5-15)
+        int m = 0;
+        for (int l = 1; l < L; ++l)
+          if (grad_y[(size_t)i * L + l] < grad_y[(size_t)i * L + m]) m = l;
+        for (int l = 0; l < L; ++l) s[(size_t)i * L + l] = l == m ? 1 : 0;
+      }
+      descent_z();
+      for (size_t i = 0; i < NL; ++i) sx[i] = s[i] - Q[i];
+      for (size_t i = 0; i < LR; ++i) sx_z[i] = cgz[i] - z[i];
+      std::fill(psisx.begin(), psisx.end(), (T)0);
+      for (size_t k = 0; k < pw.size(); ++k) {
+        apply((int)k, sx.data(), tmp.data());
+        for (size_t i = 0; i < NL; ++i) psisx[i] = psisx[i] + tmp[i];
+      }
+      std::vector<T> szc(LR), cgm1(LR);
+      for (size_t i = 0; i < LR; ++i) {
+        szc[i] = sx_z[i] * constant[i];
+        cgm1[i] = (cgz[i] - (T)1) * constant[i];
+      }
+      double a = (double)sp_scalar(szc, Qm1());
+      double b = (double)sp_scalar(cgm1, sx);
+      double num = dot(U.data(), sx.data(), NL) + 2 * dot(Q.data(), psisx.data(), NL) + a + b +
+                   (double)vec_sum(sx_z.data(), (long)LR);
+      double denom = dot(sx.data(), psisx.data(), NL) + (double)sp_scalar(szc, sx);
+      double step = -num / (2 * denom);
+      if (st) {
+        st->num.push_back(num);
+        st->denom.push_back(denom);
+      }
+      if (denom == 0 || num == 0) break;
+      if (denom < 0) step = 1;
+      if (st) st->step.push_back(step);
+      const T stp = (T)step, stp2 = (T)(2 * step);
+      for (size_t i = 0; i < NL; ++i) Q[i] = Q[i] + stp * sx[i];
+      for (size_t i = 0; i < LR; ++i) z[i] = z[i] + stp * sx_z[i];
+      {
+        std::vector<T> hb = ho_bcast();
+        for (size_t i = 0; i < NL; ++i) grad_y[i] = grad_y[i] + (stp2 * psisx[i] + stp * hb[i]);
+      }
+      {
+        std::vector<T> nr = sp_to_regions(Qm1());
+        for (size_t i = 0; i < LR; ++i) grad_z[i] = constant[i] + nr[i] * constant[i];
+      }
+      std::vector<T> gu(NL);
+      for (size_t i = 0; i < NL; ++i) gu[i] = grad_y[i] + U[i];
+      energy = dot(Q.data(), gu.data(), NL);
+      energy += ho_energy();
+      if (st) st->energy.push_back(energy);
+    }
+    if (st) st->iters = it;
+    z_labels = z;
+    return Q;
+  }
+
+  // ---- energies ------------------------------------------------------------
+  // currentMap (src/densecrf.cpp:929-938): first maximum per pixel
+  std::vector<int> current_map(const std::vector<T>& Q) const {
+    std::vector<int> lab(N);
+    for (int i = 0; i < N; ++i) {
+      int m = 0;
+      for (int l = 1; l < L; ++l)
+        if (Q[(size_t)i * L + l] > Q[(size_t)i * L + m]) m = l;
+      lab[i] = m;
+    }
+    return lab;
+  }
+  // unaryEnergy + pairwise_energy_true summed per pixel (:873-927)
+  std::vector<T> pixel_energy(const std::vector<int>& lab) const {
+    const size_t NL = (size_t)N * L;
+    std::vector<T> r(N, (T)0), q(NL), o(NL);
+    for (size_t k = 0; k < pw.size(); ++k) {
+      for (int i = 0; i < N; ++i)
+        for (int l = 0; l < L; ++l) q[(size_t)i * L + l] = lab[i] == l ? 0 : 1;
+      apply((int)k, q.data(), o.data(), /*no_norm=*/true);
+      for (int i = 0; i < N; ++i) {
+        T v = (lab[i] >= 0 && lab[i] < L) ? -o[(size_t)i * L + lab[i]] : (T)0;
+        r[i] = r[i] + v;
+      }
+    }
+    std::vector<T> tot(N);
+    for (int i = 0; i < N; ++i) {
+      T u = (lab[i] >= 0 && lab[i] < L) ? U[(size_t)i * L + lab[i]] : (T)0;
+      tot[i] = u + r[i];
+    }
+    return tot;
+  }
+  double assignment_energy_true(const std::vector<int>& lab) const {
+    std::vector<T> t = pixel_energy(lab);
+    double e = 0;
+    for (int i = 0; i < N; ++i) e += (double)t[i];
+    return e;
+  }
+  // assignment_energy_higher_order (:844-870), bug-compatible: compares the
+  // label at loop index `pix` with the label of the region's first member and
+  // always adds (R-1)*e_r per region.
+  double assignment_energy_ho(const std::vector<int>& lab) const {
+    double e = assignment_energy_true(lab);
+    double ho = 0.0;
+    for (int r = 0; r < R; ++r) {
+      int prev = (int16_t)lab[members[r][0]];
+      for (size_t pix = 0; pix < members[r].size(); ++pix)
+        if (prev != (int16_t)lab[pix]) {
+          e += (double)e_r[r];
+          break;
+        }
+      ho += (double)(T)((T)(R - 1) * e_r[r]);  // int*float product (:866)
+    }
+    return e + ho;
+  }
+  // rescale (src/eigen_utils.cpp:120-140): global min/max to [0,1]
+  static std::vector<T> rescale(const std::vector<T>& Q) {
+    std::vector<T> out = Q;
+    T mn = *std::min_element(out.begin(), out.end());
+    for (auto& x : out) x = x - mn;
+    T mx = *std::max_element(out.begin(), out.end());
+    if (mx < 0) throw std::runtime_error("Maxval < 0, skipping");
+    if (mx > 0)
+      for (auto& x : out) x = x / mx;
+    return out;
+  }
+  // compute_energy_LP_higher_order (:982-1037)
+  double lp_energy(const std::vector<T>& Q) const {
+    const size_t NL = (size_t)N * L;
+    double e = dot(U.data(), Q.data(), NL);
+    std::vector<T> rq = rescale(Q);
+    std::vector<float> rqf(rq.begin(), rq.end());
+    std::vector<T> tmp(NL);
+    for (size_t k = 0; k < pw.size(); ++k) {
+      apply_ord((int)k, rqf.data(), tmp.data());
+      e -= dot(Q.data(), tmp.data(), NL);
+    }
+    for (int r = 0; r < R; ++r) {
+      T mdl = -std::numeric_limits<T>::infinity();
+      for (int l = 0; l < L; ++l) {
+        T mx = -std::numeric_limits<T>::infinity(), mn = std::numeric_limits<T>::infinity();
+        for (int p : members[r]) {
+          mx = std::max(mx, Q[(size_t)p * L + l]);
+          mn = std::min(mn, Q[(size_t)p * L + l]);
+        }
+        mdl = std::max(mdl, (T)(mx - mn));
+      }
+      e += (double)(T)(e_r[r] * mdl);
+    }
+    return e;
+  }
+  // computeUCondGrad (:803-830): `min`/`max` are ints, so Q is truncated.
+  void ucondgrad(std::vector<T>& Us, const std::vector<T>& Q) const {
+    std::fill(Us.begin(), Us.end(), (T)0);
+    int min_ind = 0, max_ind = 0, min_lab = 0, max_lab = 0;
+    for (int r = 0; r < R; ++r) {
+      int mn = std::numeric_limits<int>::max(), mx = std::numeric_limits<int>::min();
+      for (int l = 0; l < L; ++l)
+        for (int p : members[r]) {
+          T q = Q[(size_t)p * L + l];
+          if ((T)mn > q) {
+            min_ind = p;
+            min_lab = l;
+            mn = (int)q;
+          }
+          if ((T)mx < q) {
+            max_ind = p;
+            max_lab = l;
+            mx = (int)q;
+          }
+        }
+      Us[(size_t)min_ind * L + min_lab] = c_r[r] * e_r[r];
+      Us[(size_t)max_ind * L + max_lab] = -c_r[r] * e_r[r];
+    }
+  }
+
+  // ---- LP (src/densecrf.cpp:526-799) ----------------------------------------
+  struct LPParams {
+    float lambda = 1e-2f, dual_gap_tol = 1e3f, prox_energy_tol = 1e3f;
+    int prox_max_iter = 10, fw_max_iter = 10, qp_max_iter = 1000;
+    float qp_tol = 1e3f, qp_const = 1e-16f;
+    bool best_int = true;
+  };
+  struct LPStats {
+    std::vector<int> qp_sweeps;
+    std::vector<double> gap, delta, int_energy;
+    int outer = 0;
+    double init_lp_energy = 0, init_int_energy = 0;
+  };
+  static void renormalize(std::vector<T>& Q, int L, int N) {  // :444-455
+    const double uniform = 1.0 / L;
+    for (int i = 0; i < N; ++i) {
+      T* c = &Q[(size_t)i * L];
+      double sum = (double)ColSum<T>::run(c, L, i);
+      if (sum == 0 || sum != sum)
+        for (int l = 0; l < L; ++l) c[l] = (T)uniform;
+      else {
+        T s = (T)sum;
+        for (int l = 0; l < L; ++l) c[l] = c[l] / s;
+      }
+    }
+  }
+  std::vector<T> max_rounding(const std::vector<T>& Q) const {  // :832-840
+    std::vector<T> r((size_t)N * L, (T)0);
+    std::vector<int> m = current_map(Q);
+    for (int i = 0; i < N; ++i) r[(size_t)i * L + m[i]] = 1;
+    return r;
+  }
+  // feasible_Q (:470-492) with sortCols (src/eigen_utils.cpp:96-113)
+  void feasible(std::vector<T>& out, const std::vector<T>& Q) const {
+    std::vector<T> col(L);
+    for (int i = 0; i < N; ++i) {
+      const T* c = &Q[(size_t)i * L];
+      for (int l = 0; l < L; ++l) col[l] = c[l];
+      std::sort(col.begin(), col.end(), [](T a, T b) { return a > b; });
+      double sum = (double)(T)(ColSum<T>::run(c, L, i) - (T)1);
+      int K = -1;
+      for (int k = L; k > 0; --k) {
+        double uk = (double)col[k - 1];
+        if (sum / k < uk) {
+          K = k;
+          break;
+        }
+        sum -= uk;
+      }
+      for (int l = 0; l < L; ++l)
+        out[(size_t)i * L + l] = (T)std::min(std::max((double)c[l] - sum / K, 0.0), 1.0);
+    }
+  }
+
+  std::vector<T> lp(const std::vector<T>& init, const LPParams& P, LPStats* st) {
+    const size_t NL = (size_t)N * L;
+    const int M = L;
+    std::vector<T> Q = init;
+    renormalize(Q, L, N);
+    std::vector<T> best_Q = Q;
+    double int_energy = assignment_energy_ho(current_map(Q));
+    double energy = lp_energy(Q);
+    if (st) {
+      st->init_lp_energy = energy;
+      st->init_int_energy = int_energy;
+    }
+    if (energy > int_energy) {
+      Q = max_rounding(Q);
+      energy = lp_energy(Q);
+    }
+    double best_energy = energy, best_int_energy = int_energy;
+    const T lambda = (T)P.lambda;
+    std::vector<T> alpha(NL), u(NL), sa(NL), su(NL), beta_mat(NL), gamma(NL), cur_Q(NL), tmp(NL), Y(NL), posH(NL),
+        negH(NL), qpv(N);
+    std::vector<T> vg(L), vs(L), vt(L), vt2(L), vy(L);
+    const T c_m1_over_M = (T)(-1.0 / (double)(float)M);
+    const T c_2l_over_M = (T)((T)(2 * lambda) / (T)(float)M);
+    const T c_1_over_M = (T)(1.0 / (double)(float)M);
+    const T c_1m2_over_M = (T)(1 - 2.0 / (double)(float)M);
+    const T qp_delta = (T)P.qp_const;
+    int it = 0, count = 0;
+    do {
+      ++it;
+      std::fill(beta_mat.begin(), beta_mat.end(), (T)0);
+      std::fill(gamma.begin(), gamma.end(), (T)1);
+      cur_Q = Q;
+      std::fill(alpha.begin(), alpha.end(), (T)0);
+      std::fill(u.begin(), u.end(), (T)0);
+      int pit = 0;
+      do {
+        ++pit;
+        std::fill(sa.begin(), sa.end(), (T)0);
+        std::fill(su.begin(), su.end(), (T)0);
+        for (size_t i = 0; i < NL; ++i) tmp[i] = (alpha[i] + u[i]) - U[i];  // :627
+        for (int i = 0; i < N; ++i) {  // qp_gamma_multiplyC (:517-523)
+          for (int l = 0; l < L; ++l) vt2[l] = tmp[(size_t)i * L + l];
+          T sum = ColSum<T>::run_aligned(vt2.data(), L);
+          for (int l = 0; l < L; ++l) Y[(size_t)i * L + l] = (c_m1_over_M * sum + vt2[l]) * lambda;
+        }
+        for (size_t i = 0; i < NL; ++i) {
+          Y[i] = Y[i] + cur_Q[i];
+          posH[i] = std::max(-Y[i], (T)0);
+          negH[i] = std::max(Y[i], (T)0);
+        }
+        int qpit = 0;
+        T qp_value = std::numeric_limits<T>::max();
+        if (std::is_same<T, double>::value) qp_value = (T)std::numeric_limits<float>::max();
+        do {  // :650-675
+          for (int i = 0; i < N; ++i) {
+            T* g = &gamma[(size_t)i * L];
+            const T* ph = &posH[(size_t)i * L];
+            const T* nh = &negH[(size_t)i * L];
+            for (int l = 0; l < L; ++l) vg[l] = g[l];
+            T sum = ColSum<T>::run_aligned(vg.data(), L);  // qp_gamma_step (:495-514)
+            for (int l = 0; l < L; ++l) {
+              T t = (sum - vg[l]) * c_2l_over_M;
+              T stp = (t + ph[l]) + qp_delta;
+              T t2 = (c_1_over_M * sum + c_1m2_over_M * vg[l]) * lambda;
+              t2 = (t2 + nh[l]) + qp_delta;
+              vs[l] = stp / t2;
+              vg[l] = vg[l] * vs[l];
+            }
+            for (int l = 0; l < L; ++l) g[l] = vg[l];
+            T s2 = ColSum<T>::run_aligned(vg.data(), L);
+            for (int l = 0; l < L; ++l) {
+              vt[l] = (c_m1_over_M * s2 + vg[l]) * lambda;
+              vy[l] = Y[(size_t)i * L + l];
+            }
+            qpv[i] = (T)(0.5 * (double)ColSum<T>::dot_aligned(vg.data(), vt.data(), L) +
+                         (double)ColSum<T>::dot_aligned(vg.data(), vy.data(), L));
+          }
+          T qp_value1 = vec_sum(qpv.data(), N);
+          ++qpit;
+          if (std::abs(qp_value - qp_value1) < (T)P.qp_tol) break;
+          qp_value = qp_value1;
+        } while (qpit < P.qp_max_iter);
+        if (st) st->qp_sweeps.push_back(qpit);
+        for (size_t i = 0; i < NL; ++i) beta_mat[i] = ((alpha[i] + u[i]) + gamma[i]) - U[i];  // :684-691
+        for (int i = 0; i < N; ++i) {
+          T b = -ColSum<T>::run(&beta_mat[(size_t)i * L], L, i);
+          b = b / (T)M;
+          for (int l = 0; l < L; ++l) beta_mat[(size_t)i * L + l] = b;
+        }
+        for (size_t i = 0; i < NL; ++i)  // :697
+          Q[i] = lambda * ((((alpha[i] + beta_mat[i]) + u[i]) + gamma[i]) - U[i]) + cur_Q[i];
+        std::vector<T> rq = rescale(Q);
+        std::vector<float> rqf(rq.begin(), rq.end());
+        for (size_t k = 0; k < pw.size(); ++k) {  // :708-717
+          apply_ord((int)k, rqf.data(), tmp.data());
+          for (size_t i = 0; i < NL; ++i) sa[i] = sa[i] + tmp[i];
+        }
+        ucondgrad(su, Q);  // :720
+        for (size_t i = 0; i < NL; ++i) tmp[i] = (alpha[i] + u[i]) - (sa[i] + su[i]);  // :725-738
+        double gap = dot(tmp.data(), Q.data(), NL);
+        double tt = dot(tmp.data(), tmp.data(), NL);
+        float delta = (float)(gap / ((double)(float)lambda * tt));
+        delta = std::min(std::max(delta, 0.0f), 1.0f);
+        if (st) {
+          st->gap.push_back(gap);
+          st->delta.push_back(delta);
+        }
+        const T dl = (T)delta;
+        for (size_t i = 0; i < NL; ++i) {
+          alpha[i] = alpha[i] + dl * (sa[i] - alpha[i]);
+          u[i] = u[i] + dl * (su[i] - u[i]);
+        }
+      } while (pit < P.fw_max_iter);
+      feasible(tmp, Q);  // :746-748
+      Q = tmp;
+      renormalize(Q, L, N);
+      double prev_int = int_energy;
+      int_energy = assignment_energy_ho(current_map(Q));
+      if (st) st->int_energy.push_back(int_energy);
+      if (P.best_int) {
+        if (std::abs(int_energy - prev_int) < P.prox_energy_tol)
+          ++count;
+        else
+          count = 0;
+        if (count >= 5) break;
+        if (int_energy < best_int_energy) {
+          best_Q = Q;
+          best_int_energy = int_energy;
+        }
+      } else {
+        double prev = energy;
+        energy = lp_energy(Q);
+        if (std::abs(energy - prev) < P.prox_energy_tol)
+          ++count;
+        else
+          count = 0;
+        if (count >= 5) break;
+        if (energy < best_energy) {
+          best_Q = Q;
+          best_energy = energy;
+        }
+      }
+    } while (it < P.prox_max_iter);
+    if (st) st->outer = it;
+    return best_Q;
+  }
+};
+
+template <> float CRF<float>::nrm(const Pairwise& p, int i) { return p.norm_f[i]; }
+template <> double CRF<double>::nrm(const Pairwise& p, int i) { return p.norm_d[i]; }
+template <> void CRF<float>::filt(const Lattice& lat, const float* in, float* out, int V, bool rev) {
+  lat.filter_f(in, out, V, rev);
+}
+template <> void CRF<double>::filt(const Lattice& lat, const double* in, double* out, int V, bool rev) {
+  lat.filter_d(in, out, V, rev);
+}
+
+// DenseKernel::initLattice (src/pairwise.cpp:40-57)
+static void pairwise_init(Pairwise& p) {
+  p.lat.init(p.feat.data(), (int)(p.feat.size() / p.d), p.d);
+  const int N = p.lat.N;
+  p.norm_f.assign(N, 1.f);
+  p.norm_d.assign(N, 1.0);
+  if (p.ntype != 0) {
+    std::vector<float> ones(N, 1.f), kn(N);
+    p.lat.filter_f(ones.data(), kn.data(), 1, false);
+    std::vector<double> onesd(N, 1.0), knd(N);
+    p.lat.filter_d(onesd.data(), knd.data(), 1, false);
+    for (int i = 0; i < N; ++i) {
+      if (p.ntype == 3) {
+        p.norm_f[i] = (float)(1.0 / std::sqrt((double)kn[i] + 1e-20));
+        p.norm_d[i] = 1.0 / std::sqrt(knd[i] + 1e-20);
+      } else {
+        p.norm_f[i] = (float)(1.0 / ((double)kn[i] + 1e-20));
+        p.norm_d[i] = 1.0 / (knd[i] + 1e-20);
+      }
+    }
+  }
+}
+
+}  // namespace orc
+
+// ===========================================================================
+// C ABI for ctypes (tests, fixture checks and the CPU baseline).
+// ===========================================================================
+using namespace orc;
+
+struct OrcCRF {
+  int N, L;
+  std::vector<Pairwise*> pw;
+  CRF<float>* f;
+  CRF<double>* d;
+};
+
+static thread_local std::string g_err;
+
+extern "C" {
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+void* orc_lattice_new(const float* feat, int N, int d) {
+  Lattice* L = new Lattice();
+  L->init(feat, N, d);
+  return L;
+}
+void orc_lattice_free(void* h) { delete (Lattice*)h; }
+int orc_lattice_M(void* h) { return ((Lattice*)h)->M; }
+int orc_lattice_Npad(void* h) { return ((Lattice*)h)->Npad; }
+void orc_lattice_get(void* h, int* offset, int* rank, float* bary, short* keys, int* nbr) {
+  Lattice* L = (Lattice*)h;
+  if (offset) std::copy(L->offset.begin(), L->offset.end(), offset);
+  if (rank) std::copy(L->rank.begin(), L->rank.end(), rank);
+  if (bary) std::copy(L->bary.begin(), L->bary.end(), bary);
+  if (keys) std::copy(L->keys.begin(), L->keys.end(), keys);
+  if (nbr) std::copy(L->nbr.begin(), L->nbr.end(), nbr);
+}
+void orc_filter_f(void* h, const float* in, float* out, int V, int reverse) {
+  ((Lattice*)h)->filter_f(in, out, V, reverse != 0);
+}
+void orc_filter_d(void* h, const double* in, double* out, int V, int reverse) {
+  ((Lattice*)h)->filter_d(in, out, V, reverse != 0);
+}
+void orc_ordered_f(void* h, const float* q, float* out, int V) { ((Lattice*)h)->ordered<float>(q, out, V); }
+void orc_ordered_d(void* h, const float* q, double* out, int V) { ((Lattice*)h)->ordered<double>(q, out, V); }
+
+void* orc_crf_new(int N, int L) {
+  OrcCRF* c = new OrcCRF{N, L, {}, new CRF<float>(N, L), new CRF<double>(N, L)};
+  return c;
+}
+void orc_crf_free(void* h) {
+  OrcCRF* c = (OrcCRF*)h;
+  for (auto* p : c->pw) delete p;
+  delete c->f;
+  delete c->d;
+  delete c;
+}
+void orc_crf_set_unary(void* h, const float* U) {
+  OrcCRF* c = (OrcCRF*)h;
+  size_t n = (size_t)c->N * c->L;
+  for (size_t i = 0; i < n; ++i) {
+    c->f->U[i] = U[i];
+    c->d->U[i] = U[i];
+  }
+}
+// features pixel-major (feat[i*d+k]); ntype as NormalizationType
+void orc_crf_add_pairwise(void* h, const float* feat, int d, float w, int ntype) {
+  OrcCRF* c = (OrcCRF*)h;
+  Pairwise* p = new Pairwise();
+  p->d = d;
+  p->w = w;
+  p->ntype = ntype;
+  p->feat.assign(feat, feat + (size_t)c->N * d);
+  pairwise_init(*p);
+  c->pw.push_back(p);
+  c->f->pw.push_back(p);
+  c->d->pw.push_back(p);
+}
+int orc_crf_lattice_M(void* h, int k) { return ((OrcCRF*)h)->pw[k]->lat.M; }
+void orc_crf_get_norm(void* h, int k, float* nf, double* nd) {
+  Pairwise* p = ((OrcCRF*)h)->pw[k];
+  if (nf) std::copy(p->norm_f.begin(), p->norm_f.end(), nf);
+  if (nd) std::copy(p->norm_d.begin(), p->norm_d.end(), nd);
+}
+// one superpixel layer: region id per pixel in [0,R); e/c per region
+void orc_crf_add_layer(void* h, const int* region, int R, const float* e, const float* cst) {
+  OrcCRF* c = (OrcCRF*)h;
+  std::vector<std::vector<int>> mem(R);
+  for (int i = 0; i < c->N; ++i) mem[region[i]].push_back(i);
+  int kept = 0;
+  for (int r = 0; r < R; ++r) {
+    if (mem[r].empty()) continue;  // regions without pixels are dropped
+    ++kept;
+    c->f->members.push_back(mem[r]);
+    c->d->members.push_back(mem[r]);
+    c->f->e_r.push_back(e[r]);
+    c->d->e_r.push_back(e[r]);
+    c->f->c_r.push_back(cst[r]);
+    c->d->c_r.push_back(cst[r]);
+  }
+  c->f->R += kept;
+  c->d->R += kept;
+}
+int orc_crf_apply_f(void* h, int k, const float* in, float* out, int no_norm, int transpose) {
+  ((OrcCRF*)h)->f->apply(k, in, out, no_norm != 0, transpose != 0);
+  return 0;
+}
+int orc_crf_apply_d(void* h, int k, const double* in, double* out, int no_norm, int transpose) {
+  ((OrcCRF*)h)->d->apply(k, in, out, no_norm != 0, transpose != 0);
+  return 0;
+}
+int orc_crf_apply_ord_f(void* h, int k, const float* q, float* out) {
+  try {
+    ((OrcCRF*)h)->f->apply_ord(k, q, out);
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+  return 0;
+}
+int orc_crf_apply_ord_d(void* h, int k, const float* q, double* out) {
+  try {
+    ((OrcCRF*)h)->d->apply_ord(k, q, out);
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+  return 0;
+}
+
+// QP.  dbl selects the FP64 oracle; init/out are float (converted).  stats:
+// energies[max_iter+1], num/denom/step[max_iter]; returns iteration count.
+int orc_crf_qp(void* h, int dbl, const float* init, float* out, int max_iter, int ref_stop, double* energies,
+               double* num, double* denom, double* step, int cap) {
+  OrcCRF* c = (OrcCRF*)h;
+  size_t NL = (size_t)c->N * c->L;
+  auto dump = [&](auto& st) {
+    for (int i = 0; i < cap + 1; ++i)
+      if (energies) energies[i] = i < (int)st.energy.size() ? st.energy[i] : NAN;
+    for (size_t i = 0; i < st.num.size() && (int)i < cap; ++i) {
+      if (num) num[i] = st.num[i];
+      if (denom) denom[i] = st.denom[i];
+    }
+    for (size_t i = 0; i < st.step.size() && (int)i < cap; ++i)
+      if (step) step[i] = st.step[i];
+    return st.iters;
+  };
+  if (dbl) {
+    std::vector<double> q0(init, init + NL);
+    CRF<double>::QPStats st;
+    std::vector<double> Q = c->d->qp(q0, max_iter, ref_stop != 0, &st);
+    for (size_t i = 0; i < NL; ++i) out[i] = (float)Q[i];
+    return dump(st);
+  }
+  std::vector<float> q0(init, init + NL);
+  CRF<float>::QPStats st;
+  std::vector<float> Q = c->f->qp(q0, max_iter, ref_stop != 0, &st);
+  std::copy(Q.begin(), Q.end(), out);
+  return dump(st);
+}
+void orc_crf_zlabels(void* h, float* z) {
+  OrcCRF* c = (OrcCRF*)h;
+  std::copy(c->f->z_labels.begin(), c->f->z_labels.end(), z);
+}
+
+// LP.  params: lambda, dual_gap_tol, prox_energy_tol, qp_tol, qp_const as
+// floats; ints prox_max_iter, fw_max_iter, qp_max_iter, best_int.  stats
+// arrays are filled up to `cap` entries; returns outer iteration count or -1.
+int orc_crf_lp(void* h, int dbl, const float* init, float* out, const float* fp, const int* ip, int* qp_sweeps,
+               double* gap, double* delta, double* int_energy, int cap, double* init_energies) {
+  OrcCRF* c = (OrcCRF*)h;
+  size_t NL = (size_t)c->N * c->L;
+  auto fill = [&](auto& P) {
+    P.lambda = fp[0];
+    P.dual_gap_tol = fp[1];
+    P.prox_energy_tol = fp[2];
+    P.qp_tol = fp[3];
+    P.qp_const = fp[4];
+    P.prox_max_iter = ip[0];
+    P.fw_max_iter = ip[1];
+    P.qp_max_iter = ip[2];
+    P.best_int = ip[3] != 0;
+  };
+  auto dump = [&](auto& st) {
+    for (size_t i = 0; i < st.qp_sweeps.size() && (int)i < cap; ++i) {
+      if (qp_sweeps) qp_sweeps[i] = st.qp_sweeps[i];
+      if (gap) gap[i] = st.gap[i];
+      if (delta) delta[i] = st.delta[i];
+    }
+    for (size_t i = 0; i < st.int_energy.size() && (int)i < cap; ++i)
+      if (int_energy) int_energy[i] = st.int_energy[i];
+    if (init_energies) {
+      init_energies[0] = st.init_lp_energy;
+      init_energies[1] = st.init_int_energy;
+    }
+    return st.outer;
+  };
+  try {
+    if (dbl) {
+      CRF<double>::LPParams P;
+      fill(P);
+      std::vector<double> q0(init, init + NL);
+      CRF<double>::LPStats st;
+      std::vector<double> Q = c->d->lp(q0, P, &st);
+      for (size_t i = 0; i < NL; ++i) out[i] = (float)Q[i];
+      return dump(st);
+    }
+    CRF<float>::LPParams P;
+    fill(P);
+    std::vector<float> q0(init, init + NL);
+    CRF<float>::LPStats st;
+    std::vector<float> Q = c->f->lp(q0, P, &st);
+    std::copy(Q.begin(), Q.end(), out);
+    return dump(st);
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// energies of a labelling / of a relaxed Q (float instantiation unless dbl)
+double orc_crf_assignment_energy_true(void* h, int dbl, const int* lab) {
+  OrcCRF* c = (OrcCRF*)h;
+  std::vector<int> l(lab, lab + c->N);
+  return dbl ? c->d->assignment_energy_true(l) : c->f->assignment_energy_true(l);
+}
+double orc_crf_assignment_energy_ho(void* h, int dbl, const int* lab) {
+  OrcCRF* c = (OrcCRF*)h;
+  std::vector<int> l(lab, lab + c->N);
+  return dbl ? c->d->assignment_energy_ho(l) : c->f->assignment_energy_ho(l);
+}
+double orc_crf_lp_energy(void* h, int dbl, const float* Q) {
+  OrcCRF* c = (OrcCRF*)h;
+  size_t NL = (size_t)c->N * c->L;
+  if (dbl) return c->d->lp_energy(std::vector<double>(Q, Q + NL));
+  return c->f->lp_energy(std::vector<float>(Q, Q + NL));
+}
+void orc_crf_ucondgrad(void* h, const float* Q, float* Us) {
+  OrcCRF* c = (OrcCRF*)h;
+  size_t NL = (size_t)c->N * c->L;
+  std::vector<float> q(Q, Q + NL), u(NL);
+  c->f->ucondgrad(u, q);
+  std::copy(u.begin(), u.end(), Us);
+}
+void orc_crf_current_map(void* h, const float* Q, int* lab) {
+  OrcCRF* c = (OrcCRF*)h;
+  size_t NL = (size_t)c->N * c->L;
+  std::vector<int> m = c->f->current_map(std::vector<float>(Q, Q + NL));
+  std::copy(m.begin(), m.end(), lab);
+}
+// expAndNormalize(-U) (src/densecrf.cpp:321-336)
+void orc_unary_init(const float* U, float* Q, int N, int L) {
+  std::vector<float> b(L);
+  for (int i = 0; i < N; ++i) {
+    float mx = -U[(size_t)i * L];
+    for (int l = 0; l < L; ++l) mx = std::max(mx, -U[(size_t)i * L + l]);
+    for (int l = 0; l < L; ++l) b[l] = std::exp(-U[(size_t)i * L + l] - mx);
+    float s = ColSum<float>::run_aligned(b.data(), L);
+    for (int l = 0; l < L; ++l) Q[(size_t)i * L + l] = b[l] / s;
+  }
+}
+void orc_feasible(const float* Q, float* out, int N, int L) {
+  CRF<float> c(N, L);
+  std::vector<float> q(Q, Q + (size_t)N * L), o((size_t)N * L);
+  c.feasible(o, q);
+  std::copy(o.begin(), o.end(), out);
+}
+void orc_renormalize(float* Q, int N, int L) {
+  std::vector<float> q(Q, Q + (size_t)N * L);
+  CRF<float>::renormalize(q, L, N);
+  std::copy(q.begin(), q.end(), Q);
+}
+
+}  // extern "C"

@@ -1,0 +1,500 @@
+"""ctypes front-end for the CPU oracle (liboracle.so) and, where it was built,
+the compiled reference (oracle/_ref/libdcrf_ref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+and by bench.py's cpu_baseline leg, never by the engine itself.
+
+Layouts are the reference's: features (N, d) pixel-major, fields (N, L)
+label-contiguous, region maps (N,) int32.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(_HERE, "liboracle.so")
+REF_SO = os.path.join(_HERE, "_ref", "libdcrf_ref.so")
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_i16p = np.ctypeslib.ndpointer(np.int16, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+_i = C.c_int
+_f = C.c_float
+_d = C.c_double
+
+NTYPE = {"none": 0, "before": 1, "after": 2, "symmetric": 3}
+
+
+def _sig(lib, name, res, *args):
+    fn = getattr(lib, name)
+    fn.restype = res
+    fn.argtypes = list(args)
+    return fn
+
+
+_orc = None
+
+
+def lib():
+    """Load liboracle.so (build it with ``make -C oracle``)."""
+    global _orc
+    if _orc is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle library missing: {ORACLE_SO} (run `make -C oracle oracle`)")
+        L = C.CDLL(ORACLE_SO)
+        _sig(L, "orc_last_error", C.c_char_p)
+        _sig(L, "orc_lattice_new", _vp, _f32p, _i, _i)
+        _sig(L, "orc_lattice_free", None, _vp)
+        _sig(L, "orc_lattice_M", _i, _vp)
+        _sig(L, "orc_lattice_Npad", _i, _vp)
+        _sig(L, "orc_lattice_get", None, _vp, _i32p, _i32p, _f32p, _i16p, _i32p)
+        _sig(L, "orc_filter_f", None, _vp, _f32p, _f32p, _i, _i)
+        _sig(L, "orc_filter_d", None, _vp, _f64p, _f64p, _i, _i)
+        _sig(L, "orc_ordered_f", None, _vp, _f32p, _f32p, _i)
+        _sig(L, "orc_ordered_d", None, _vp, _f32p, _f64p, _i)
+        _sig(L, "orc_crf_new", _vp, _i, _i)
+        _sig(L, "orc_crf_free", None, _vp)
+        _sig(L, "orc_crf_set_unary", None, _vp, _f32p)
+        _sig(L, "orc_crf_add_pairwise", None, _vp, _f32p, _i, _f, _i)
+        _sig(L, "orc_crf_lattice_M", _i, _vp, _i)
+        _sig(L, "orc_crf_get_norm", None, _vp, _i, _f32p, _f64p)
+        _sig(L, "orc_crf_add_layer", None, _vp, _i32p, _i, _f32p, _f32p)
+        _sig(L, "orc_crf_apply_f", _i, _vp, _i, _f32p, _f32p, _i, _i)
+        _sig(L, "orc_crf_apply_d", _i, _vp, _i, _f64p, _f64p, _i, _i)
+        _sig(L, "orc_crf_apply_ord_f", _i, _vp, _i, _f32p, _f32p)
+        _sig(L, "orc_crf_apply_ord_d", _i, _vp, _i, _f32p, _f64p)
+        _sig(L, "orc_crf_qp", _i, _vp, _i, _f32p, _f32p, _i, _i, _f64p, _f64p, _f64p, _f64p, _i)
+        _sig(L, "orc_crf_zlabels", None, _vp, _f32p)
+        _sig(L, "orc_crf_lp", _i, _vp, _i, _f32p, _f32p, _f32p, _i32p, _i32p, _f64p, _f64p, _f64p, _i, _f64p)
+        _sig(L, "orc_crf_assignment_energy_true", _d, _vp, _i, _i32p)
+        _sig(L, "orc_crf_assignment_energy_ho", _d, _vp, _i, _i32p)
+        _sig(L, "orc_crf_lp_energy", _d, _vp, _i, _f32p)
+        _sig(L, "orc_crf_ucondgrad", None, _vp, _f32p, _f32p)
+        _sig(L, "orc_crf_current_map", None, _vp, _f32p, _i32p)
+        _sig(L, "orc_unary_init", None, _f32p, _f32p, _i, _i)
+        _sig(L, "orc_feasible", None, _f32p, _f32p, _i, _i)
+        _sig(L, "orc_renormalize", None, _f32p, _i, _i)
+        _orc = L
+    return _orc
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+class Lattice:
+    """Oracle lattice built with the reference's SSE2 semantics."""
+
+    def __init__(self, feat: np.ndarray):
+        feat = _c(feat, np.float32)
+        self.N, self.d = feat.shape
+        self._L = lib()
+        self.h = self._L.orc_lattice_new(feat, self.N, self.d)
+        self.M = self._L.orc_lattice_M(self.h)
+        self.Npad = self._L.orc_lattice_Npad(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._L.orc_lattice_free(self.h)
+            self.h = None
+
+    def arrays(self):
+        d1 = self.d + 1
+        off = np.zeros(self.Npad * d1, np.int32)
+        rk = np.zeros(self.Npad * d1, np.int32)
+        bc = np.zeros(self.Npad * d1, np.float32)
+        keys = np.zeros(max(self.M * self.d, 1), np.int16)
+        nbr = np.zeros(max(d1 * self.M * 2, 1), np.int32)
+        self._L.orc_lattice_get(self.h, off, rk, bc, keys, nbr)
+        return dict(offset=off.reshape(self.Npad, d1), rank=rk.reshape(self.Npad, d1),
+                    bary=bc.reshape(self.Npad, d1), keys=keys[: self.M * self.d].reshape(self.M, self.d),
+                    nbr=nbr[: d1 * self.M * 2].reshape(d1, self.M, 2))
+
+    def filter(self, x: np.ndarray, reverse=False, fp64=False):
+        x = np.asarray(x)
+        V = x.shape[1] if x.ndim == 2 else 1
+        if fp64:
+            xi = _c(x, np.float64)
+            out = np.zeros_like(xi)
+            self._L.orc_filter_d(self.h, xi, out, V, int(reverse))
+        else:
+            xi = _c(x, np.float32)
+            out = np.zeros_like(xi)
+            self._L.orc_filter_f(self.h, xi, out, V, int(reverse))
+        return out
+
+    def ordered(self, q: np.ndarray, fp64=False):
+        q = _c(q, np.float32)
+        V = q.shape[1]
+        out = np.zeros(q.shape, np.float64 if fp64 else np.float32)
+        (self._L.orc_ordered_d if fp64 else self._L.orc_ordered_f)(self.h, q, out, V)
+        return out
+
+
+class LPParams:
+    """``LP_inf_params`` (include/densecrf.h:43-69), defaults of src/densecrf.cpp:287-302."""
+
+    def __init__(self, prox_reg_const=1e-2, dual_gap_tol=1e3, prox_energy_tol=1e3, prox_max_iter=10,
+                 fw_max_iter=10, qp_max_iter=1000, qp_tol=1e3, qp_const=1e-16, best_int=True):
+        self.prox_reg_const = prox_reg_const
+        self.dual_gap_tol = dual_gap_tol
+        self.prox_energy_tol = prox_energy_tol
+        self.prox_max_iter = prox_max_iter
+        self.fw_max_iter = fw_max_iter
+        self.qp_max_iter = qp_max_iter
+        self.qp_tol = qp_tol
+        self.qp_const = qp_const
+        self.best_int = best_int
+
+    def packed(self):
+        fp = np.array([self.prox_reg_const, self.dual_gap_tol, self.prox_energy_tol, self.qp_tol,
+                       self.qp_const], np.float32)
+        ip = np.array([self.prox_max_iter, self.fw_max_iter, self.qp_max_iter, int(self.best_int)], np.int32)
+        return fp, ip
+
+
+class CRF:
+    """Oracle dense CRF (Potts pairwise terms, optional Pn superpixel layers)."""
+
+    def __init__(self, N: int, L: int):
+        self.N, self.L = N, L
+        self._L = lib()
+        self.h = self._L.orc_crf_new(N, L)
+        self.nk = 0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._L.orc_crf_free(self.h)
+            self.h = None
+
+    def set_unary(self, U):
+        self._L.orc_crf_set_unary(self.h, _c(U, np.float32))
+
+    def add_pairwise(self, feat, w, ntype="symmetric"):
+        feat = _c(feat, np.float32)
+        self._L.orc_crf_add_pairwise(self.h, feat, feat.shape[1], float(w), NTYPE[ntype])
+        self.nk += 1
+
+    def add_layer(self, region, e, c):
+        region = _c(region, np.int32)
+        R = int(region.max()) + 1
+        e = np.broadcast_to(np.asarray(e, np.float32), (R,)).copy()
+        c = np.broadcast_to(np.asarray(c, np.float32), (R,)).copy()
+        self._L.orc_crf_add_layer(self.h, region, R, e, c)
+
+    def lattice_M(self, k):
+        return self._L.orc_crf_lattice_M(self.h, k)
+
+    def norm(self, k):
+        nf = np.zeros(self.N, np.float32)
+        nd = np.zeros(self.N, np.float64)
+        self._L.orc_crf_get_norm(self.h, k, nf, nd)
+        return nf, nd
+
+    def apply(self, k, x, fp64=False, no_norm=False, transpose=False):
+        if fp64:
+            xi = _c(x, np.float64)
+            out = np.zeros_like(xi)
+            self._L.orc_crf_apply_d(self.h, k, xi, out, int(no_norm), int(transpose))
+        else:
+            xi = _c(x, np.float32)
+            out = np.zeros_like(xi)
+            self._L.orc_crf_apply_f(self.h, k, xi, out, int(no_norm), int(transpose))
+        return out
+
+    def apply_ord(self, k, q, fp64=False):
+        q = _c(q, np.float32)
+        out = np.zeros(q.shape, np.float64 if fp64 else np.float32)
+        rc = (self._L.orc_crf_apply_ord_d if fp64 else self._L.orc_crf_apply_ord_f)(self.h, k, q, out)
+        if rc != 0:
+            raise RuntimeError(self._L.orc_last_error().decode())
+        return out
+
+    def qp(self, init, max_iter=10, ref_stop=False, fp64=False):
+        init = _c(init, np.float32)
+        out = np.zeros_like(init)
+        cap = max(max_iter, 1) if not ref_stop else 100000
+        cap = min(cap, 100000)
+        en = np.zeros(cap + 1)
+        num = np.zeros(cap)
+        den = np.zeros(cap)
+        st = np.zeros(cap)
+        it = self._L.orc_crf_qp(self.h, int(fp64), init, out, int(max_iter), int(ref_stop), en, num, den, st, cap)
+        en = en[: it + 1]
+        return out, dict(iters=it, energy=en[~np.isnan(en)], num=num[:it], denom=den[:it], step=st[:it])
+
+    def zlabels(self, R):
+        z = np.zeros(R * self.L, np.float32)
+        self._L.orc_crf_zlabels(self.h, z)
+        return z.reshape(R, self.L)
+
+    def lp(self, init, params: LPParams | None = None, fp64=False):
+        params = params or LPParams()
+        fp, ip = params.packed()
+        init = _c(init, np.float32)
+        out = np.zeros_like(init)
+        cap = 4096
+        sw = np.zeros(cap, np.int32)
+        gap = np.zeros(cap)
+        dl = np.zeros(cap)
+        ie = np.zeros(cap)
+        e0 = np.zeros(2)
+        it = self._L.orc_crf_lp(self.h, int(fp64), init, out, fp, ip, sw, gap, dl, ie, cap, e0)
+        if it < 0:
+            raise RuntimeError(self._L.orc_last_error().decode())
+        nin = int(np.count_nonzero(sw))
+        return out, dict(outer=it, qp_sweeps=sw[:nin], gap=gap[:nin], delta=dl[:nin], int_energy=ie[:it],
+                         init_lp_energy=e0[0], init_int_energy=e0[1])
+
+    def energy_true(self, lab, fp64=False):
+        return self._L.orc_crf_assignment_energy_true(self.h, int(fp64), _c(lab, np.int32))
+
+    def energy_ho(self, lab, fp64=False):
+        return self._L.orc_crf_assignment_energy_ho(self.h, int(fp64), _c(lab, np.int32))
+
+    def lp_energy(self, Q, fp64=False):
+        return self._L.orc_crf_lp_energy(self.h, int(fp64), _c(Q, np.float32))
+
+    def ucondgrad(self, Q):
+        Q = _c(Q, np.float32)
+        out = np.zeros_like(Q)
+        self._L.orc_crf_ucondgrad(self.h, Q, out)
+        return out
+
+    def current_map(self, Q):
+        Q = _c(Q, np.float32)
+        lab = np.zeros(self.N, np.int32)
+        self._L.orc_crf_current_map(self.h, Q, lab)
+        return lab
+
+
+def unary_init(U):
+    U = _c(U, np.float32)
+    N, L = U.shape
+    Q = np.zeros_like(U)
+    lib().orc_unary_init(U, Q, N, L)
+    return Q
+
+
+def feasible(Q):
+    Q = _c(Q, np.float32)
+    out = np.zeros_like(Q)
+    lib().orc_feasible(Q, out, Q.shape[0], Q.shape[1])
+    return out
+
+
+def renormalize(Q):
+    Q = _c(Q, np.float32).copy()
+    lib().orc_renormalize(Q, Q.shape[0], Q.shape[1])
+    return Q
+
+
+# ---------------------------------------------------------------------------
+# compiled reference (development container only)
+# ---------------------------------------------------------------------------
+_ref = None
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def reflib():
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError("reference build missing (oracle/_ref is only built where the reference exists)")
+        R = C.CDLL(REF_SO)
+        _sig(R, "ref_last_error", C.c_char_p)
+        _sig(R, "ref_lattice_new", _vp, _f32p, _i, _i)
+        _sig(R, "ref_lattice_free", None, _vp)
+        _sig(R, "ref_lattice_M", _i, _vp)
+        _sig(R, "ref_lattice_get", None, _vp, _i32p, _i32p, _f32p, _i32p, _i)
+        _sig(R, "ref_lattice_compute", None, _vp, _f32p, _f32p, _i, _i, _i)
+        _sig(R, "ref_lattice_ordered", None, _vp, _f32p, _f32p, _i, _i)
+        _sig(R, "ref_crf_new", _vp, _i, _i, _i)
+        _sig(R, "ref_crf_free", None, _vp)
+        _sig(R, "ref_crf_set_unary", None, _vp, _f32p)
+        _sig(R, "ref_crf_add_pairwise", None, _vp, _f32p, _i, _f, _i)
+        _sig(R, "ref_crf_add_gaussian", None, _vp, _f, _f, _f)
+        _sig(R, "ref_crf_add_bilateral", None, _vp, _f, _f, _f, _f, _f, _u8p, _f)
+        _sig(R, "ref_crf_get_features", None, _vp, _i, _f32p)
+        _sig(R, "ref_crf_add_layer", None, _vp, _i32p, _i, _f32p, _f32p)
+        _sig(R, "ref_crf_R", _i, _vp)
+        _sig(R, "ref_crf_apply", None, _vp, _i, _f32p, _f32p, _i)
+        _sig(R, "ref_crf_apply_ord", _i, _vp, _i, _f32p, _f32p)
+        _sig(R, "ref_crf_unary_init", None, _vp, _f32p)
+        _sig(R, "ref_crf_qp", None, _vp, _f32p, _f32p)
+        _sig(R, "ref_crf_zlabels", None, _vp, _f32p)
+        _sig(R, "ref_crf_lp", _i, _vp, _f32p, _f32p, _f32p, _i32p)
+        _sig(R, "ref_crf_energy_true", _d, _vp, _i32p)
+        _sig(R, "ref_crf_energy_ho", _d, _vp, _i32p)
+        _sig(R, "ref_crf_lp_energy", _d, _vp, _f32p)
+        _sig(R, "ref_crf_qp_value", _d, _vp, _f32p)
+        _sig(R, "ref_crf_ucondgrad", None, _vp, _f32p, _f32p)
+        _sig(R, "ref_crf_current_map", None, _vp, _f32p, _i32p)
+        _sig(R, "ref_feasible", None, _f32p, _f32p, _i, _i)
+        _sig(R, "ref_renormalize", None, _f32p, _i, _i)
+        _ref = R
+    return _ref
+
+
+class RefLattice:
+    def __init__(self, feat):
+        feat = _c(feat, np.float32)
+        self.N, self.d = feat.shape
+        self._R = reflib()
+        self.h = self._R.ref_lattice_new(feat, self.N, self.d)
+        self.M = self._R.ref_lattice_M(self.h)
+        self.Npad = (self.N + 3) // 4 * 4
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._R.ref_lattice_free(self.h)
+            self.h = None
+
+    def arrays(self):
+        d1 = self.d + 1
+        off = np.zeros(self.Npad * d1, np.int32)
+        rk = np.zeros(self.Npad * d1, np.int32)
+        bc = np.zeros(self.Npad * d1, np.float32)
+        nbr = np.zeros(max(d1 * self.M * 2, 1), np.int32)
+        self._R.ref_lattice_get(self.h, off, rk, bc, nbr, self.Npad)
+        return dict(offset=off.reshape(self.Npad, d1), rank=rk.reshape(self.Npad, d1),
+                    bary=bc.reshape(self.Npad, d1), nbr=nbr[: d1 * self.M * 2].reshape(d1, self.M, 2))
+
+    def filter(self, x, reverse=False):
+        x = _c(x, np.float32)
+        V = x.shape[1] if x.ndim == 2 else 1
+        out = np.zeros_like(x)
+        self._R.ref_lattice_compute(self.h, x, out, V, self.N, int(reverse))
+        return out
+
+    def ordered(self, q):
+        q = _c(q, np.float32)
+        out = np.zeros_like(q)
+        self._R.ref_lattice_ordered(self.h, q, out, q.shape[1], self.N)
+        return out
+
+
+class RefCRF:
+    def __init__(self, W, H, L):
+        self.W, self.H, self.L, self.N = W, H, L, W * H
+        self._R = reflib()
+        self.h = self._R.ref_crf_new(W, H, L)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._R.ref_crf_free(self.h)
+            self.h = None
+
+    def set_unary(self, U):
+        self._R.ref_crf_set_unary(self.h, _c(U, np.float32))
+
+    def add_pairwise(self, feat, w, ntype="symmetric"):
+        feat = _c(feat, np.float32)
+        self._R.ref_crf_add_pairwise(self.h, feat, feat.shape[1], float(w), NTYPE[ntype])
+
+    def add_gaussian(self, sx, sy, w):
+        self._R.ref_crf_add_gaussian(self.h, sx, sy, w)
+
+    def add_bilateral(self, sx, sy, sr, sg, sb, img, w):
+        self._R.ref_crf_add_bilateral(self.h, sx, sy, sr, sg, sb, _c(img, np.uint8), w)
+
+    def features(self, k, d):
+        out = np.zeros(self.N * d, np.float32)
+        self._R.ref_crf_get_features(self.h, k, out)
+        return out.reshape(self.N, d)
+
+    def add_layer(self, region, e, c):
+        region = _c(region, np.int32)
+        R = int(region.max()) + 1
+        e = np.broadcast_to(np.asarray(e, np.float32), (R,)).copy()
+        c = np.broadcast_to(np.asarray(c, np.float32), (R,)).copy()
+        self._R.ref_crf_add_layer(self.h, region, R, e, c)
+
+    @property
+    def R(self):
+        return self._R.ref_crf_R(self.h)
+
+    def apply(self, k, x, which=0):
+        x = _c(x, np.float32)
+        out = np.zeros_like(x)
+        self._R.ref_crf_apply(self.h, k, x, out, which)
+        return out
+
+    def apply_ord(self, k, q):
+        q = _c(q, np.float32)
+        out = np.zeros_like(q)
+        if self._R.ref_crf_apply_ord(self.h, k, q, out) != 0:
+            raise RuntimeError(self._R.ref_last_error().decode())
+        return out
+
+    def unary_init(self):
+        Q = np.zeros((self.N, self.L), np.float32)
+        self._R.ref_crf_unary_init(self.h, Q)
+        return Q
+
+    def qp(self, init):
+        init = _c(init, np.float32)
+        out = np.zeros_like(init)
+        self._R.ref_crf_qp(self.h, init, out)
+        return out
+
+    def zlabels(self):
+        z = np.zeros(self.R * self.L, np.float32)
+        self._R.ref_crf_zlabels(self.h, z)
+        return z.reshape(self.R, self.L)
+
+    def lp(self, init, params: LPParams | None = None):
+        params = params or LPParams()
+        fp, ip = params.packed()
+        init = _c(init, np.float32)
+        out = np.zeros_like(init)
+        if self._R.ref_crf_lp(self.h, init, out, fp, ip) != 0:
+            raise RuntimeError(self._R.ref_last_error().decode())
+        return out
+
+    def energy_true(self, lab):
+        return self._R.ref_crf_energy_true(self.h, _c(lab, np.int32))
+
+    def energy_ho(self, lab):
+        return self._R.ref_crf_energy_ho(self.h, _c(lab, np.int32))
+
+    def lp_energy(self, Q):
+        return self._R.ref_crf_lp_energy(self.h, _c(Q, np.float32))
+
+    def qp_value(self, Q):
+        return self._R.ref_crf_qp_value(self.h, _c(Q, np.float32))
+
+    def ucondgrad(self, Q):
+        Q = _c(Q, np.float32)
+        out = np.zeros_like(Q)
+        self._R.ref_crf_ucondgrad(self.h, Q, out)
+        return out
+
+    def current_map(self, Q):
+        Q = _c(Q, np.float32)
+        lab = np.zeros(self.N, np.int32)
+        self._R.ref_crf_current_map(self.h, Q, lab)
+        return lab
+
+
+def ref_feasible(Q):
+    Q = _c(Q, np.float32)
+    out = np.zeros_like(Q)
+    reflib().ref_feasible(Q, out, Q.shape[0], Q.shape[1])
+    return out
+
+
+def ref_renormalize(Q):
+    Q = _c(Q, np.float32).copy()
+    reflib().ref_renormalize(Q, Q.shape[0], Q.shape[1])
+    return Q

@@ -1,0 +1,232 @@
+// Glue that exposes the compiled reference core through a C ABI so that the
+// fixture script (tests/golden/make_golden.py) and the oracle tests can drive
+// it with ctypes.  TEST INFRASTRUCTURE ONLY: it is built from the reference
+// sources where they lie (see oracle/Makefile) into oracle/_ref/, and only in
+// the development container where those sources exist.  The reference never
+// travels to the GPU box; only the fixtures written from it do.
+//
+// Protected members of the reference classes are reached by deriving from
+// them; no reference code is restated here.
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "densecrf.h"
+#include "eigen_utils.hpp"
+#include "permutohedral.h"
+
+// free functions of src/densecrf.cpp that the header does not declare
+void renormalize(MatrixXf& Q);
+void feasible_Q(MatrixXf& tmp, MatrixXi& ind, VectorXd& sum, VectorXi& K, const MatrixXf& Q);
+
+namespace {
+
+struct RefLattice : public Permutohedral {
+  int M() const { return M_; }
+  int d() const { return d_; }
+  void get(int* offset, int* rank, float* bary, int* nbr, int npad) const {
+    const int d1 = d_ + 1;
+    for (int i = 0; i < npad * d1; ++i) {
+      if (offset) offset[i] = offset_[i];
+      if (rank) rank[i] = rank_[i];
+      if (bary) bary[i] = barycentric_[i];
+    }
+    if (nbr)
+      for (int i = 0; i < d1 * M_; ++i) {
+        nbr[2 * i] = blur_neighbors_[i].n1;
+        nbr[2 * i + 1] = blur_neighbors_[i].n2;
+      }
+  }
+};
+
+struct RefCRF : public DenseCRF2D {
+  RefCRF(int W, int H, int L) : DenseCRF2D(W, H, L) {}
+  int N() const { return N_; }
+  int L() const { return M_; }
+  PairwisePotential* pw(int k) { return pairwise_[k]; }
+  PairwisePotential* nn(int k) { return no_norm_pairwise_[k]; }
+  void add_layer(const int* region, int R, const float* e, const float* c) {
+    std::vector<std::vector<double>> mem(R);
+    for (int i = 0; i < N_; ++i) mem[region[i]].push_back(i);
+    for (int r = 0; r < R; ++r) {
+      if (mem[r].empty()) continue;
+      super_pixel_container_.push_back(mem[r]);
+      constants_.push_back(c[r]);
+      exp_of_superpixels_.conservativeResize(R_ + 1);
+      exp_of_superpixels_(R_) = e[r];
+      mean_of_superpixels_.conservativeResize(R_ + 1);
+      mean_of_superpixels_(R_) = 0;
+      ++R_;
+    }
+  }
+  void ucond(MatrixXf& Us, const MatrixXf& Q) const { computeUCondGrad(Us, Q); }
+  int R() const { return R_; }
+  const MatrixXf& z() const { return z_labels_; }
+};
+
+thread_local std::string g_err;
+
+MatrixXf as_mat(const float* p, int rows, int cols) {
+  MatrixXf m(rows, cols);
+  std::copy(p, p + (size_t)rows * cols, m.data());
+  return m;
+}
+VectorXs as_lab(const int* l, int n) {
+  VectorXs v(n);
+  for (int i = 0; i < n; ++i) v[i] = (short)l[i];
+  return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ref_last_error() { return g_err.c_str(); }
+
+// ---- Permutohedral ---------------------------------------------------------
+void* ref_lattice_new(const float* feat, int N, int d) {
+  RefLattice* L = new RefLattice();
+  L->init(as_mat(feat, d, N));
+  return L;
+}
+void ref_lattice_free(void* h) { delete (RefLattice*)h; }
+int ref_lattice_M(void* h) { return ((RefLattice*)h)->M(); }
+void ref_lattice_get(void* h, int* offset, int* rank, float* bary, int* nbr, int npad) {
+  ((RefLattice*)h)->get(offset, rank, bary, nbr, npad);
+}
+void ref_lattice_compute(void* h, const float* in, float* out, int V, int N, int reverse) {
+  MatrixXf i = as_mat(in, V, N), o;
+  ((RefLattice*)h)->compute(o, i, reverse != 0);
+  std::copy(o.data(), o.data() + (size_t)V * N, out);
+}
+void ref_lattice_ordered(void* h, const float* q, float* out, int V, int N) {
+  MatrixXf i = as_mat(q, V, N), o(V, N);
+  o.fill(0);
+  ((RefLattice*)h)->compute_upper_minus_lower_ord(o, i);
+  std::copy(o.data(), o.data() + (size_t)V * N, out);
+}
+
+// ---- DenseCRF --------------------------------------------------------------
+void* ref_crf_new(int W, int H, int L) { return new RefCRF(W, H, L); }
+void ref_crf_free(void* h) { delete (RefCRF*)h; }
+void ref_crf_set_unary(void* h, const float* U) {
+  RefCRF* c = (RefCRF*)h;
+  c->setUnaryEnergy(as_mat(U, c->L(), c->N()));
+}
+void ref_crf_add_pairwise(void* h, const float* feat, int d, float w, int ntype) {
+  RefCRF* c = (RefCRF*)h;
+  c->addPairwiseEnergy(as_mat(feat, d, c->N()), new PottsCompatibility(w), DIAG_KERNEL, (NormalizationType)ntype);
+}
+void ref_crf_add_gaussian(void* h, float sx, float sy, float w) {
+  ((RefCRF*)h)->addPairwiseGaussian(sx, sy, new PottsCompatibility(w));
+}
+void ref_crf_add_bilateral(void* h, float sx, float sy, float sr, float sg, float sb, const unsigned char* im,
+                           float w) {
+  ((RefCRF*)h)->addPairwiseBilateral(sx, sy, sr, sg, sb, im, new PottsCompatibility(w));
+}
+void ref_crf_get_features(void* h, int k, float* out) {
+  MatrixXf f = ((RefCRF*)h)->pw(k)->features();
+  std::copy(f.data(), f.data() + f.size(), out);
+}
+void ref_crf_add_layer(void* h, const int* region, int R, const float* e, const float* c) {
+  ((RefCRF*)h)->add_layer(region, R, e, c);
+}
+int ref_crf_R(void* h) { return ((RefCRF*)h)->R(); }
+// which: 0 apply, 1 applyTranspose, 2 no-normalisation copy's apply
+void ref_crf_apply(void* h, int k, const float* in, float* out, int which) {
+  RefCRF* c = (RefCRF*)h;
+  MatrixXf i = as_mat(in, c->L(), c->N()), o;
+  if (which == 0) c->pw(k)->apply(o, i);
+  else if (which == 1) c->pw(k)->applyTranspose(o, i);
+  else c->nn(k)->apply(o, i);
+  std::copy(o.data(), o.data() + o.size(), out);
+}
+int ref_crf_apply_ord(void* h, int k, const float* q, float* out) {
+  RefCRF* c = (RefCRF*)h;
+  MatrixXf i = as_mat(q, c->L(), c->N()), o(c->L(), c->N());
+  try {
+    c->pw(k)->apply_upper_minus_lower_ord(o, i);
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+  std::copy(o.data(), o.data() + o.size(), out);
+  return 0;
+}
+void ref_crf_unary_init(void* h, float* Q) {
+  MatrixXf q = ((RefCRF*)h)->unary_init();
+  std::copy(q.data(), q.data() + q.size(), Q);
+}
+void ref_crf_qp(void* h, const float* init, float* out) {
+  RefCRF* c = (RefCRF*)h;
+  MatrixXf q = c->qp_inference_super_pixels_non_convex(as_mat(init, c->L(), c->N()));
+  std::copy(q.data(), q.data() + q.size(), out);
+}
+void ref_crf_zlabels(void* h, float* z) {
+  const MatrixXf& m = ((RefCRF*)h)->z();
+  std::copy(m.data(), m.data() + m.size(), z);
+}
+int ref_crf_lp(void* h, const float* init, float* out, const float* fp, const int* ip) {
+  RefCRF* c = (RefCRF*)h;
+  LP_inf_params P;
+  P.prox_reg_const = fp[0];
+  P.dual_gap_tol = fp[1];
+  P.prox_energy_tol = fp[2];
+  P.qp_tol = fp[3];
+  P.qp_const = fp[4];
+  P.prox_max_iter = ip[0];
+  P.fw_max_iter = ip[1];
+  P.qp_max_iter = ip[2];
+  P.best_int = ip[3] != 0;
+  try {
+    MatrixXf q = c->lp_inference_prox_super_pixels(as_mat(init, c->L(), c->N()), P);
+    std::copy(q.data(), q.data() + q.size(), out);
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+  return 0;
+}
+double ref_crf_energy_true(void* h, const int* lab) {
+  RefCRF* c = (RefCRF*)h;
+  return c->assignment_energy_true(as_lab(lab, c->N()));
+}
+double ref_crf_energy_ho(void* h, const int* lab) {
+  RefCRF* c = (RefCRF*)h;
+  return c->assignment_energy_higher_order(as_lab(lab, c->N()));
+}
+double ref_crf_lp_energy(void* h, const float* Q) {
+  RefCRF* c = (RefCRF*)h;
+  return c->compute_energy_LP_higher_order(as_mat(Q, c->L(), c->N()));
+}
+double ref_crf_qp_value(void* h, const float* Q) {
+  RefCRF* c = (RefCRF*)h;
+  return c->compute_LR_QP_value(as_mat(Q, c->L(), c->N()), MatrixXf::Zero(c->L(), c->N()));
+}
+void ref_crf_ucondgrad(void* h, const float* Q, float* Us) {
+  RefCRF* c = (RefCRF*)h;
+  MatrixXf u(c->L(), c->N());
+  c->ucond(u, as_mat(Q, c->L(), c->N()));
+  std::copy(u.data(), u.data() + u.size(), Us);
+}
+void ref_crf_current_map(void* h, const float* Q, int* lab) {
+  RefCRF* c = (RefCRF*)h;
+  VectorXs m = c->currentMap(as_mat(Q, c->L(), c->N()));
+  for (int i = 0; i < c->N(); ++i) lab[i] = m[i];
+}
+void ref_feasible(const float* Q, float* out, int N, int L) {
+  MatrixXf q = as_mat(Q, L, N), t(L, N);
+  MatrixXi ind(L, N);
+  VectorXd sum(N);
+  VectorXi K(N);
+  feasible_Q(t, ind, sum, K, q);
+  std::copy(t.data(), t.data() + t.size(), out);
+}
+void ref_renormalize(float* Q, int N, int L) {
+  MatrixXf q = as_mat(Q, L, N);
+  renormalize(q);
+  std::copy(q.data(), q.data() + q.size(), Q);
+}
+
+}  // extern "C"
