@@ -1,0 +1,411 @@
+"""Host-side mirror of the reference's C++ interface (names, argument meaning
+and exceptions of include/permutohedral.h, include/pairwise.h,
+include/labelcompatibility.h and include/densecrf.h), backed by the C ABI.
+
+Arrays are either numpy arrays (host; the call stages them to the device) or
+torch tensors on the GPU (used in place).  Fields are (N, L) float32 - the
+reference's L x N column-major MatrixXf - features (N, d) float32, labellings
+(N,) int32.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+NO_NORMALIZATION, NORMALIZE_BEFORE, NORMALIZE_AFTER, NORMALIZE_SYMMETRIC = 0, 1, 2, 3
+
+
+class EngineError(RuntimeError):
+    """std::runtime_error of the reference (message kept) or an engine failure."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise EngineError(rc, _lib.load().dcrf_last_error().decode())
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+class DeviceArray:
+    """A buffer in device memory allocated by the engine's own HIP runtime
+    (dcrf_malloc): inputs kept resident in HBM without another GPU library."""
+
+    def __init__(self, shape, dtype=np.float32):
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        _check(_lib.load().dcrf_malloc(C.byref(p), self.nbytes))
+        self.ptr = p
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        _check(_lib.load().dcrf_memcpy(d.ptr, a.ctypes.data_as(C.c_void_p), d.nbytes, 0))
+        return d
+
+    def numpy(self):
+        a = np.empty(self.shape, self.dtype)
+        _check(_lib.load().dcrf_memcpy(a.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes, 1))
+        return a
+
+    def copy_from(self, other: "DeviceArray"):
+        _check(_lib.load().dcrf_memcpy(self.ptr, other.ptr, min(self.nbytes, other.nbytes), 2))
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                _lib.load().dcrf_free(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+def synchronize():
+    _check(_lib.load().dcrf_device_synchronize())
+
+
+def profile_enable(on: bool = True):
+    _check(_lib.load().dcrf_profile_enable(int(on)))
+
+
+def profile_report() -> dict:
+    """{kernel name: (launch count, total ms)} since profile_enable()."""
+    L = _lib.load()
+    n = L.dcrf_profile_report(None, 0)
+    buf = C.create_string_buffer(max(n, 1))
+    L.dcrf_profile_report(buf, n)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, cnt, ms = line.rsplit(" ", 2)
+        out[name] = (int(cnt), float(ms))
+    return out
+
+
+class _Arg:
+    """Pointer + on_device flag for an input or output array."""
+
+    def __init__(self, x, dtype, out=False):
+        if isinstance(x, DeviceArray):
+            if x.dtype != np.dtype(dtype):
+                raise TypeError(f"device array must be {np.dtype(dtype)}")
+            self.arr = x
+            self.ptr = x.ptr
+            self.dev = 1
+        elif _is_torch(x):
+            import torch
+            want = {np.float32: torch.float32, np.int32: torch.int32, np.uint8: torch.uint8}[dtype]
+            if x.dtype != want or not x.is_contiguous() or not x.is_cuda:
+                raise TypeError(f"device tensor must be a contiguous {want} CUDA tensor")
+            self.arr = x
+            self.ptr = C.c_void_p(x.data_ptr())
+            self.dev = 1
+        else:
+            a = np.asarray(x)
+            if out:
+                if a.dtype != dtype or not a.flags.c_contiguous:
+                    raise TypeError(f"output array must be a contiguous {np.dtype(dtype)} array")
+            else:
+                a = np.ascontiguousarray(a, dtype=dtype)
+            self.arr = a
+            self.ptr = a.ctypes.data_as(C.c_void_p)
+            self.dev = 0
+
+
+def _empty_like_field(x, shape, dtype=np.float32):
+    if isinstance(x, DeviceArray):
+        return DeviceArray(shape, dtype)
+    if _is_torch(x):
+        import torch
+        tdt = {np.float32: torch.float32, np.int32: torch.int32}[dtype]
+        return torch.empty(shape, dtype=tdt, device=x.device)
+    return np.empty(shape, dtype)
+
+
+class PottsCompatibility:
+    """PottsCompatibility(w): mu(a,b) = -w [a==b] (include/labelcompatibility.h:51-63)."""
+
+    def __init__(self, weight: float = 1.0):
+        self.w = float(weight)
+
+    def parameters(self):
+        return np.array([self.w], np.float32)
+
+
+class Permutohedral:
+    """Permutohedral lattice (include/permutohedral.h:50-88)."""
+
+    def __init__(self):
+        self._h = None
+        self.N = self.d = self.M = 0
+
+    def init(self, features):
+        """Permutohedral::init(features); features (N, d)."""
+        L = _lib.load()
+        self.close()
+        a = _Arg(features, np.float32)
+        shp = tuple(a.arr.shape)
+        if len(shp) != 2:
+            raise ValueError("features must be (N, d)")
+        h = C.c_void_p()
+        _check(L.dcrf_lattice_create(a.ptr, shp[0], shp[1], a.dev, C.byref(h)))
+        self._h = h
+        self.N, self.d = shp
+        M, Np = C.c_int(), C.c_int()
+        _check(L.dcrf_lattice_info(self._h, C.byref(M), C.byref(Np)))
+        self.M, self.Npad = M.value, Np.value
+        return self
+
+    def close(self):
+        if self._h is not None:
+            _lib.load().dcrf_lattice_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def compute(self, x, reverse: bool = False, out=None):
+        """Permutohedral::compute(out, in, reverse); x (N, V)."""
+        V = 1 if len(x.shape) == 1 else x.shape[1]
+        a = _Arg(x, np.float32)
+        if out is None:
+            out = _empty_like_field(x, tuple(x.shape))
+        o = _Arg(out, np.float32, out=True)
+        _check(_lib.load().dcrf_lattice_compute(self._h, a.ptr, o.ptr, V, int(reverse), a.dev))
+        return out
+
+    def compute_upper_minus_lower_ord(self, Q, out=None):
+        """Permutohedral::compute_upper_minus_lower_ord(out, Q); Q (N, V) in [0,1]."""
+        a = _Arg(Q, np.float32)
+        if out is None:
+            out = _empty_like_field(Q, tuple(Q.shape))
+        o = _Arg(out, np.float32, out=True)
+        _check(_lib.load().dcrf_lattice_compute_upper_minus_lower_ord(self._h, a.ptr, o.ptr, Q.shape[1], a.dev))
+        return out
+
+    def tables(self):
+        """offset_, rank_, barycentric_ (Npad, d+1), keys (M, d), neighbours (d+1, M, 2)."""
+        d1 = self.d + 1
+        off = np.zeros((self.Npad, d1), np.int32)
+        rk = np.zeros((self.Npad, d1), np.int32)
+        bc = np.zeros((self.Npad, d1), np.float32)
+        keys = np.zeros((max(self.M, 1), self.d), np.int16)
+        nbr = np.zeros((d1, max(self.M, 1), 2), np.int32)
+        p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+        _check(_lib.load().dcrf_lattice_export(self._h, p(off), p(rk), p(bc), p(keys), p(nbr)))
+        return dict(offset=off, rank=rk, bary=bc, keys=keys[: self.M], nbr=nbr[:, : self.M])
+
+
+class LP_inf_params:
+    """LP_inf_params (include/densecrf.h:43-69); defaults of src/densecrf.cpp:287-302."""
+
+    def __init__(self, prox_reg_const=1e-2, dual_gap_tol=1e3, prox_energy_tol=1e3, prox_max_iter=10,
+                 fw_max_iter=10, qp_max_iter=1000, qp_tol=1e3, qp_const=1e-16, best_int=True, accel_prox=True,
+                 work_set_size=10, approx_fw_iter=10, less_confident_percent=0.0, confidence_tol=0.95):
+        self.__dict__.update(locals())
+        del self.__dict__["self"]
+
+    def _c(self):
+        p = _lib.LPParams()
+        for name, _ in _lib.LPParams._fields_:
+            setattr(p, name, type(getattr(p, name))(getattr(self, name)))
+        return p
+
+
+class DenseCRF:
+    """DenseCRF (include/densecrf.h:73-170) on the GPU engine."""
+
+    def __init__(self, N: int, M: int):
+        L = _lib.load()
+        self.N, self.M = int(N), int(M)
+        h = C.c_void_p()
+        _check(L.dcrf_crf_create(self.N, self.M, C.byref(h)))
+        self._h = h
+        self.npairwise = 0
+        self.R = 0
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            _lib.load().dcrf_crf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- model -------------------------------------------------------------------
+    def setUnaryEnergy(self, unary):
+        a = _Arg(unary, np.float32)
+        _check(_lib.load().dcrf_crf_set_unary(self._h, a.ptr, a.dev))
+
+    def addPairwiseEnergy(self, features, function: PottsCompatibility | None = None, kernel_type=1,
+                          normalization_type=NORMALIZE_SYMMETRIC):
+        w = (function or PottsCompatibility(1.0)).w
+        a = _Arg(features, np.float32)
+        d = a.arr.shape[1]
+        _check(_lib.load().dcrf_crf_add_pairwise(self._h, a.ptr, d, w, int(normalization_type), a.dev))
+        self.npairwise += 1
+
+    def addSuperPixelLayer(self, region, exp_of_superpixels, constants):
+        """Superpixel layer from a region map (the state DenseCRF2D::addSuperPixel builds)."""
+        r = np.ascontiguousarray(region, np.int32)
+        R = int(r.max()) + 1
+        e = np.ascontiguousarray(np.broadcast_to(np.asarray(exp_of_superpixels, np.float32), (R,)))
+        c = np.ascontiguousarray(np.broadcast_to(np.asarray(constants, np.float32), (R,)))
+        p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+        _check(_lib.load().dcrf_crf_add_superpixel_layer(self._h, p(r), R, p(e), p(c)))
+        self.R += int(np.count_nonzero(np.bincount(r, minlength=R)))
+
+    # ---- potentials -------------------------------------------------------------
+    def apply(self, k, Q, transpose=False, no_norm=False, out=None):
+        a = _Arg(Q, np.float32)
+        out = _empty_like_field(Q, (self.N, self.M)) if out is None else out
+        o = _Arg(out, np.float32, out=True)
+        _check(_lib.load().dcrf_crf_apply(self._h, k, a.ptr, o.ptr, int(transpose), int(no_norm), a.dev))
+        return out
+
+    def apply_upper_minus_lower_ord(self, k, Q, out=None):
+        a = _Arg(Q, np.float32)
+        out = _empty_like_field(Q, (self.N, self.M)) if out is None else out
+        o = _Arg(out, np.float32, out=True)
+        _check(_lib.load().dcrf_crf_apply_upper_minus_lower_ord(self._h, k, a.ptr, o.ptr, a.dev))
+        return out
+
+    def unary_init(self, device=None):
+        out = np.empty((self.N, self.M), np.float32) if device is None else _torch_empty((self.N, self.M), device)
+        o = _Arg(out, np.float32, out=True)
+        _check(_lib.load().dcrf_crf_unary_init(self._h, o.ptr, o.dev))
+        return out
+
+    # ---- solvers ---------------------------------------------------------------
+    def qp_inference_super_pixels_non_convex(self, init, max_iter: int = 0, ref_stop: bool | None = None,
+                                             stats: bool = False, out=None):
+        """QP Frank-Wolfe.  Default: the reference's stopping rule; max_iter>0
+        with ref_stop=False runs exactly max_iter iterations."""
+        if ref_stop is None:
+            ref_stop = max_iter <= 0
+        a = _Arg(init, np.float32)
+        out = _empty_like_field(init, (self.N, self.M)) if out is None else out
+        o = _Arg(out, np.float32, out=True)
+        st = None
+        keep = None
+        if stats:
+            cap = max(max_iter, 1) if max_iter > 0 else 4096
+            keep = [np.zeros(cap + 1), np.zeros(cap), np.zeros(cap), np.zeros(cap)]
+            dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+            st = _lib.QPStats(0, 0, cap, dp(keep[0]), dp(keep[1]), dp(keep[2]), dp(keep[3]))
+        _check(_lib.load().dcrf_crf_qp(self._h, a.ptr, o.ptr, int(max_iter), int(ref_stop), a.dev,
+                                       C.byref(st) if st is not None else None))
+        if stats:
+            it, up = st.iterations, st.updates
+            return out, dict(iters=it, updates=up, energy=keep[0][: it + 1], num=keep[1][:it], denom=keep[2][:it],
+                             step=keep[3][:up])
+        return out
+
+    def lp_inference_prox_super_pixels(self, init, params: LP_inf_params | None = None, stats: bool = False,
+                                       out=None):
+        params = params or LP_inf_params()
+        a = _Arg(init, np.float32)
+        out = _empty_like_field(init, (self.N, self.M)) if out is None else out
+        o = _Arg(out, np.float32, out=True)
+        st = None
+        keep = None
+        if stats:
+            cap = 4096
+            keep = [np.zeros(cap, np.int32), np.zeros(cap), np.zeros(cap), np.zeros(cap)]
+            dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+            st = _lib.LPStats(0, 0, cap, keep[0].ctypes.data_as(C.POINTER(C.c_int)), dp(keep[1]), dp(keep[2]),
+                              dp(keep[3]), 0.0, 0.0)
+        _check(_lib.load().dcrf_crf_lp(self._h, a.ptr, o.ptr, C.byref(params._c()), a.dev,
+                                       C.byref(st) if st is not None else None))
+        if stats:
+            n = st.inner
+            return out, dict(outer=st.outer, qp_sweeps=keep[0][:n], gap=keep[1][:n], delta=keep[2][:n],
+                             int_energy=keep[3][: st.outer], init_lp_energy=st.init_lp_energy,
+                             init_int_energy=st.init_int_energy)
+        return out
+
+    # ---- energies ---------------------------------------------------------------
+    def currentMap(self, Q):
+        a = _Arg(Q, np.float32)
+        out = _empty_like_field(Q, (self.N,), np.int32)
+        o = _Arg(out, np.int32, out=True)
+        _check(_lib.load().dcrf_crf_current_map(self._h, a.ptr, o.ptr, a.dev))
+        return out
+
+    def _energy(self, fn, x, dtype):
+        a = _Arg(x, dtype)
+        e = C.c_double()
+        _check(fn(self._h, a.ptr, a.dev, C.byref(e)))
+        return e.value
+
+    def assignment_energy_true(self, labels):
+        return self._energy(_lib.load().dcrf_crf_assignment_energy_true, labels, np.int32)
+
+    def assignment_energy_higher_order(self, labels):
+        return self._energy(_lib.load().dcrf_crf_assignment_energy_higher_order, labels, np.int32)
+
+    def compute_energy_LP_higher_order(self, Q):
+        return self._energy(_lib.load().dcrf_crf_lp_energy, Q, np.float32)
+
+    def computeUCondGrad(self, Q, out=None):
+        a = _Arg(Q, np.float32)
+        out = _empty_like_field(Q, (self.N, self.M)) if out is None else out
+        o = _Arg(out, np.float32, out=True)
+        _check(_lib.load().dcrf_crf_ucondgrad(self._h, a.ptr, o.ptr, a.dev))
+        return out
+
+    def compute_LR_QP_value(self, Q):
+        return self._energy(_lib.load().dcrf_crf_qp_value, Q, np.float32)
+
+
+def _torch_empty(shape, device):
+    import torch
+    return torch.empty(shape, dtype=torch.float32, device=device)
+
+
+class DenseCRF2D(DenseCRF):
+    """DenseCRF2D (include/densecrf.h:172-194): pixel index y*W + x."""
+
+    def __init__(self, W: int, H: int, M: int):
+        super().__init__(W * H, M)
+        self.W, self.H = int(W), int(H)
+
+    def addPairwiseGaussian(self, sx, sy, function: PottsCompatibility | None = None, kernel_type=1,
+                            normalization_type=NORMALIZE_SYMMETRIC):
+        if normalization_type != NORMALIZE_SYMMETRIC:
+            from .synthetic import gaussian_features
+            return self.addPairwiseEnergy(gaussian_features(self.W, self.H, sx, sy), function, kernel_type,
+                                          normalization_type)
+        w = (function or PottsCompatibility(1.0)).w
+        _check(_lib.load().dcrf_crf_add_pairwise_gaussian(self._h, self.W, self.H, float(sx), float(sy), w))
+        self.npairwise += 1
+
+    def addPairwiseBilateral(self, sx, sy, sr, sg, sb, im, function: PottsCompatibility | None = None,
+                             kernel_type=1, normalization_type=NORMALIZE_SYMMETRIC):
+        im = np.ascontiguousarray(im, np.uint8).reshape(-1)
+        if normalization_type != NORMALIZE_SYMMETRIC:
+            from .synthetic import bilateral_features
+            return self.addPairwiseEnergy(bilateral_features(self.W, self.H, sx, sy, sr, sg, sb, im), function,
+                                          kernel_type, normalization_type)
+        w = (function or PottsCompatibility(1.0)).w
+        _check(_lib.load().dcrf_crf_add_pairwise_bilateral(self._h, self.W, self.H, float(sx), float(sy), float(sr),
+                                                           float(sg), float(sb),
+                                                           im.ctypes.data_as(C.c_void_p), w))
+        self.npairwise += 1

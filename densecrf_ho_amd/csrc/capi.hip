@@ -1,0 +1,490 @@
+// C ABI (include/densecrf_ho_amd.h): status codes, host/device staging and
+// the thread-local error message.
+#include <cstring>
+#include <string>
+
+#include "../../include/densecrf_ho_amd.h"
+#include "crf.hpp"
+#include "crf_kernels.hpp"
+
+struct dcrf_lattice_s {
+  hipStream_t s = nullptr;
+  std::unique_ptr<dcrf::Lattice> lat;
+  dcrf::DBuf<float> minmax;
+  ~dcrf_lattice_s() {
+    if (s) {
+      (void)hipStreamSynchronize(s);
+      lat.reset();
+      (void)hipStreamDestroy(s);
+    }
+  }
+};
+struct dcrf_crf_s {
+  std::unique_ptr<dcrf::CRF> crf;
+};
+
+namespace dcrf {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+template <class F>
+static int guarded(F&& f) {
+  try {
+    f();
+    return kOk;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_last_error("out of host memory");
+    return kInternal;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return kInternal;
+  }
+}
+
+// device view of a caller buffer: either the caller's device pointer or a
+// staged copy of a host buffer
+template <class T>
+struct Staged {
+  T* d = nullptr;
+  DBuf<T> own;
+  T* host = nullptr;
+  size_t n = 0;
+  hipStream_t s;
+  Staged(const T* p, size_t count, bool on_device, hipStream_t st, bool copy_in) : n(count), s(st) {
+    if (on_device) {
+      d = const_cast<T*>(p);
+      return;
+    }
+    own.resize(count);
+    d = own.get();
+    host = const_cast<T*>(p);
+    if (copy_in) DCRF_HIP(hipMemcpyAsync(d, p, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void copy_out() {
+    if (host) DCRF_HIP(hipMemcpyAsync(host, d, n * sizeof(T), hipMemcpyDeviceToHost, s));
+  }
+};
+
+namespace {
+__global__ void k_features_2d(float* __restrict__ f, int W, int H, float sx, float sy, const unsigned char* im,
+                              float sr, float sg, float sb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W * H) return;
+  const int x = i % W, y = i / W;
+  const int d = im ? 5 : 2;
+  f[(size_t)i * d + 0] = (float)x / sx;
+  f[(size_t)i * d + 1] = (float)y / sy;
+  if (im) {
+    f[(size_t)i * d + 2] = (float)im[(size_t)i * 3 + 0] / sr;
+    f[(size_t)i * d + 3] = (float)im[(size_t)i * 3 + 1] / sg;
+    f[(size_t)i * d + 4] = (float)im[(size_t)i * 3 + 2] / sb;
+  }
+}
+
+__global__ void k_minmax(const float* __restrict__ a, size_t n, float* __restrict__ parts) {
+  __shared__ float smn[kBlock], smx[kBlock];
+  float mn = INFINITY, mx = -INFINITY;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float v = a[i];
+    mn = fminf(mn, v);
+    mx = fmaxf(mx, v);
+    if (v != v) mn = mx = v;  // NaN propagates
+  }
+  smn[threadIdx.x] = mn;
+  smx[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      smn[threadIdx.x] = fminf(smn[threadIdx.x], smn[threadIdx.x + o]);
+      smx[threadIdx.x] = fmaxf(smx[threadIdx.x], smx[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    parts[2 * blockIdx.x] = smn[0];
+    parts[2 * blockIdx.x + 1] = smx[0];
+  }
+}
+}  // namespace
+
+// host-side [min, max] of a device array (bounds checks of the ordered filter)
+static void minmax_host(const float* d, size_t n, hipStream_t s, float* mn, float* mx) {
+  const int nb = 256;
+  DBuf<float> parts(2 * nb);
+  k_minmax<<<nb, kBlock, 0, s>>>(d, n, parts.get());
+  check_launch("k_minmax");
+  std::vector<float> h(2 * nb);
+  DCRF_HIP(hipMemcpyAsync(h.data(), parts.get(), h.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+  DCRF_HIP(hipStreamSynchronize(s));
+  float a = INFINITY, b = -INFINITY;
+  bool nan = false;
+  for (int i = 0; i < nb; ++i) {
+    if (h[2 * i] != h[2 * i] || h[2 * i + 1] != h[2 * i + 1]) nan = true;
+    a = std::min(a, h[2 * i]);
+    b = std::max(b, h[2 * i + 1]);
+  }
+  *mn = nan ? NAN : a;
+  *mx = nan ? NAN : b;
+}
+
+// PairwisePotential::apply_upper_minus_lower_ord throws when Q leaves [0,1]
+// (src/pairwise.cpp:218-219).  NaN passes the reference's test and is
+// rejected here as out of bounds too.
+void check_unit_interval(const float* d, size_t n, hipStream_t s) {
+  float mn, mx;
+  minmax_host(d, n, s, &mn, &mx);
+  if (!(mx <= 1.f) || !(mn >= 0.f)) throw Error(kOutOfBounds, "Value out of bounds");
+}
+
+}  // namespace dcrf
+
+using namespace dcrf;
+
+extern "C" {
+
+const char* dcrf_last_error(void) { return g_last_error.c_str(); }
+const char* dcrf_version(void) { return "densecrf_ho_amd 0.1 (gfx950)"; }
+
+// ---- lattice ------------------------------------------------------------------
+int dcrf_lattice_create(const float* features, int N, int d, int on_device, dcrf_lattice_t* out) {
+  return guarded([&] {
+    DCRF_REQUIRE(features && out && N > 0 && d > 0, "dcrf_lattice_create: bad arguments");
+    auto h = std::make_unique<dcrf_lattice_s>();
+    DCRF_HIP(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+    h->lat = std::make_unique<Lattice>(h->s);
+    Staged<float> f(features, (size_t)N * d, on_device != 0, h->s, true);
+    h->lat->build(f.d, N, d);
+    h->minmax.resize(2);
+    DCRF_HIP(hipMemsetAsync(h->minmax.get(), 0, 2 * sizeof(float), h->s));
+    DCRF_HIP(hipStreamSynchronize(h->s));
+    *out = h.release();
+  });
+}
+int dcrf_lattice_destroy(dcrf_lattice_t lat) {
+  return guarded([&] { delete lat; });
+}
+int dcrf_lattice_info(dcrf_lattice_t lat, int* M, int* Npad) {
+  return guarded([&] {
+    DCRF_REQUIRE(lat, "dcrf_lattice_info: null handle");
+    if (M) *M = lat->lat->M();
+    if (Npad) *Npad = lat->lat->Npad();
+  });
+}
+int dcrf_lattice_export(dcrf_lattice_t lat, int* offset, int* rank, float* barycentric, short* keys, int* neighbors) {
+  return guarded([&] {
+    DCRF_REQUIRE(lat, "dcrf_lattice_export: null handle");
+    lat->lat->export_tables(offset, rank, barycentric, keys, neighbors);
+  });
+}
+int dcrf_lattice_compute(dcrf_lattice_t lat, const float* in, float* out, int V, int reverse, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(lat && in && out && V > 0, "dcrf_lattice_compute: bad arguments");
+    const size_t n = (size_t)lat->lat->N() * V;
+    hipStream_t s = lat->s;
+    Staged<float> a(in, n, on_device != 0, s, true);
+    Staged<float> b(out, n, on_device != 0, s, false);
+    SplatArgs sa;
+    sa.kind = SplatIn::kPlain;
+    sa.in = a.d;
+    SliceArgs so;
+    so.out = b.d;
+    lat->lat->filter(sa, so, V, reverse != 0);
+    b.copy_out();
+    DCRF_HIP(hipStreamSynchronize(s));
+  });
+}
+int dcrf_lattice_compute_upper_minus_lower_ord(dcrf_lattice_t lat, const float* Q, float* out, int V, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(lat && Q && out && V > 0, "dcrf_lattice_compute_upper_minus_lower_ord: bad arguments");
+    const size_t n = (size_t)lat->lat->N() * V;
+    hipStream_t s = lat->s;
+    Staged<float> a(Q, n, on_device != 0, s, true);
+    Staged<float> b(out, n, on_device != 0, s, false);
+    check_unit_interval(a.d, n, s);
+    SliceArgs so;
+    so.out = b.d;
+    lat->lat->ordered(a.d, lat->minmax.get(), V, so);
+    b.copy_out();
+    DCRF_HIP(hipStreamSynchronize(s));
+  });
+}
+
+// ---- CRF ------------------------------------------------------------------------
+int dcrf_crf_create(int N, int L, dcrf_crf_t* out) {
+  return guarded([&] {
+    DCRF_REQUIRE(out, "dcrf_crf_create: null output");
+    auto h = std::make_unique<dcrf_crf_s>();
+    h->crf = std::make_unique<CRF>(N, L);
+    *out = h.release();
+  });
+}
+int dcrf_crf_destroy(dcrf_crf_t crf) {
+  return guarded([&] { delete crf; });
+}
+int dcrf_crf_set_unary(dcrf_crf_t crf, const float* unary, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && unary, "dcrf_crf_set_unary: bad arguments");
+    CRF& c = *crf->crf;
+    Staged<float> u(unary, (size_t)c.N() * c.L(), on_device != 0, c.stream(), true);
+    c.set_unary(u.d);
+    DCRF_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
+int dcrf_crf_add_pairwise(dcrf_crf_t crf, const float* features, int d, float potts_w, int ntype, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && features && d > 0, "dcrf_crf_add_pairwise: bad arguments");
+    CRF& c = *crf->crf;
+    Staged<float> f(features, (size_t)c.N() * d, on_device != 0, c.stream(), true);
+    c.add_pairwise(f.d, d, potts_w, ntype);
+  });
+}
+int dcrf_crf_add_pairwise_gaussian(dcrf_crf_t crf, int W, int H, float sx, float sy, float potts_w) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf, "dcrf_crf_add_pairwise_gaussian: null handle");
+    CRF& c = *crf->crf;
+    DCRF_REQUIRE(W > 0 && H > 0 && (long)W * H == c.N(), "dcrf_crf_add_pairwise_gaussian: W*H must equal N");
+    DBuf<float> f((size_t)c.N() * 2);
+    k_features_2d<<<grid_for(c.N()), kBlock, 0, c.stream()>>>(f.get(), W, H, sx, sy, nullptr, 1.f, 1.f, 1.f);
+    check_launch("k_features_2d");
+    c.add_pairwise(f.get(), 2, potts_w, kNormSymmetric);
+  });
+}
+int dcrf_crf_add_pairwise_bilateral(dcrf_crf_t crf, int W, int H, float sx, float sy, float sr, float sg, float sb,
+                                    const unsigned char* image, float potts_w) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && image, "dcrf_crf_add_pairwise_bilateral: bad arguments");
+    CRF& c = *crf->crf;
+    DCRF_REQUIRE(W > 0 && H > 0 && (long)W * H == c.N(), "dcrf_crf_add_pairwise_bilateral: W*H must equal N");
+    DBuf<unsigned char> im((size_t)c.N() * 3);
+    DCRF_HIP(hipMemcpyAsync(im.get(), image, (size_t)c.N() * 3, hipMemcpyHostToDevice, c.stream()));
+    DBuf<float> f((size_t)c.N() * 5);
+    k_features_2d<<<grid_for(c.N()), kBlock, 0, c.stream()>>>(f.get(), W, H, sx, sy, im.get(), sr, sg, sb);
+    check_launch("k_features_2d");
+    c.add_pairwise(f.get(), 5, potts_w, kNormSymmetric);
+  });
+}
+int dcrf_crf_add_superpixel_layer(dcrf_crf_t crf, const int* region, int R, const float* e, const float* cst) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && region && e && cst && R > 0, "dcrf_crf_add_superpixel_layer: bad arguments");
+    crf->crf->add_layer(region, R, e, cst);
+  });
+}
+int dcrf_crf_apply(dcrf_crf_t crf, int k, const float* in, float* out, int transpose, int no_norm, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && in && out, "dcrf_crf_apply: bad arguments");
+    CRF& c = *crf->crf;
+    DCRF_REQUIRE(k >= 0 && k < c.num_pairwise(), "dcrf_crf_apply: no such potential");
+    const size_t n = (size_t)c.N() * c.L();
+    Staged<float> a(in, n, on_device != 0, c.stream(), true);
+    Staged<float> b(out, n, on_device != 0, c.stream(), false);
+    c.apply(k, a.d, b.d, transpose != 0, no_norm != 0);
+    b.copy_out();
+    DCRF_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
+int dcrf_crf_apply_upper_minus_lower_ord(dcrf_crf_t crf, int k, const float* Q, float* out, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && Q && out, "dcrf_crf_apply_upper_minus_lower_ord: bad arguments");
+    CRF& c = *crf->crf;
+    DCRF_REQUIRE(k >= 0 && k < c.num_pairwise(), "dcrf_crf_apply_upper_minus_lower_ord: no such potential");
+    const size_t n = (size_t)c.N() * c.L();
+    Staged<float> a(Q, n, on_device != 0, c.stream(), true);
+    Staged<float> b(out, n, on_device != 0, c.stream(), false);
+    c.apply_ord(k, a.d, b.d, true);
+    b.copy_out();
+    DCRF_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
+int dcrf_crf_unary_init(dcrf_crf_t crf, float* Q, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && Q, "dcrf_crf_unary_init: bad arguments");
+    CRF& c = *crf->crf;
+    Staged<float> b(Q, (size_t)c.N() * c.L(), on_device != 0, c.stream(), false);
+    c.unary_init(b.d);
+    b.copy_out();
+    DCRF_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
+int dcrf_crf_qp(dcrf_crf_t crf, const float* init, float* out, int max_iter, int ref_stop, int on_device,
+                dcrf_qp_stats* stats) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && init && out, "dcrf_crf_qp: bad arguments");
+    CRF& c = *crf->crf;
+    const size_t n = (size_t)c.N() * c.L();
+    Staged<float> a(init, n, on_device != 0, c.stream(), true);
+    Staged<float> b(out, n, on_device != 0, c.stream(), false);
+    const int cap = stats ? stats->capacity : 0;
+    QPResult r = c.qp(a.d, b.d, max_iter, ref_stop != 0, cap);
+    b.copy_out();
+    DCRF_HIP(hipStreamSynchronize(c.stream()));
+    if (stats) {
+      stats->iterations = r.iterations;
+      stats->updates = r.updates;
+      for (size_t i = 0; i < r.energy.size() && stats->energy; ++i) stats->energy[i] = r.energy[i];
+      for (size_t i = 0; i < r.num.size(); ++i) {
+        if (stats->num) stats->num[i] = r.num[i];
+        if (stats->denom) stats->denom[i] = r.denom[i];
+      }
+      for (size_t i = 0; i < r.step.size() && stats->step; ++i) stats->step[i] = r.step[i];
+    }
+  });
+}
+void dcrf_lp_params_default(dcrf_lp_params* p) {
+  if (!p) return;
+  p->prox_reg_const = 1e-2f;
+  p->dual_gap_tol = 1e3f;
+  p->prox_energy_tol = 1e3f;
+  p->prox_max_iter = 10;
+  p->fw_max_iter = 10;
+  p->qp_max_iter = 1000;
+  p->qp_tol = 1e3f;
+  p->qp_const = 1e-16f;
+  p->best_int = 1;
+  p->accel_prox = 1;
+  p->work_set_size = 10;
+  p->approx_fw_iter = 10;
+  p->less_confident_percent = 0.f;
+  p->confidence_tol = 0.95f;
+}
+int dcrf_crf_lp(dcrf_crf_t crf, const float* init, float* out, const dcrf_lp_params* params, int on_device,
+                dcrf_lp_stats* stats) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && init && out, "dcrf_crf_lp: bad arguments");
+    CRF& c = *crf->crf;
+    dcrf_lp_params d;
+    dcrf_lp_params_default(&d);
+    const dcrf_lp_params& P = params ? *params : d;
+    DCRF_REQUIRE(P.less_confident_percent <= 0.f, "dcrf_crf_lp: less_confident_percent > 0 is not supported");
+    LPParams lp;
+    lp.prox_reg_const = P.prox_reg_const;
+    lp.dual_gap_tol = P.dual_gap_tol;
+    lp.prox_energy_tol = P.prox_energy_tol;
+    lp.prox_max_iter = P.prox_max_iter;
+    lp.fw_max_iter = P.fw_max_iter;
+    lp.qp_max_iter = P.qp_max_iter;
+    lp.qp_tol = P.qp_tol;
+    lp.qp_const = P.qp_const;
+    lp.best_int = P.best_int != 0;
+    const size_t n = (size_t)c.N() * c.L();
+    Staged<float> a(init, n, on_device != 0, c.stream(), true);
+    Staged<float> b(out, n, on_device != 0, c.stream(), false);
+    LPResult r = c.lp(a.d, b.d, lp, stats ? stats->capacity : 0);
+    b.copy_out();
+    DCRF_HIP(hipStreamSynchronize(c.stream()));
+    if (stats) {
+      stats->outer = r.outer;
+      stats->inner = (int)r.qp_sweeps.size();
+      for (size_t i = 0; i < r.qp_sweeps.size() && (int)i < stats->capacity; ++i) {
+        if (stats->qp_sweeps) stats->qp_sweeps[i] = r.qp_sweeps[i];
+        if (stats->dual_gap) stats->dual_gap[i] = r.gap[i];
+        if (stats->delta) stats->delta[i] = r.delta[i];
+      }
+      for (size_t i = 0; i < r.int_energy.size() && (int)i < stats->capacity; ++i)
+        if (stats->int_energy) stats->int_energy[i] = r.int_energy[i];
+      stats->init_lp_energy = r.init_lp_energy;
+      stats->init_int_energy = r.init_int_energy;
+    }
+  });
+}
+int dcrf_crf_current_map(dcrf_crf_t crf, const float* Q, int* labels, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && Q && labels, "dcrf_crf_current_map: bad arguments");
+    CRF& c = *crf->crf;
+    Staged<float> a(Q, (size_t)c.N() * c.L(), on_device != 0, c.stream(), true);
+    Staged<int> b(labels, (size_t)c.N(), on_device != 0, c.stream(), false);
+    c.current_map(a.d, b.d);
+    b.copy_out();
+    DCRF_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
+int dcrf_crf_assignment_energy_true(dcrf_crf_t crf, const int* labels, int on_device, double* energy) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && labels && energy, "dcrf_crf_assignment_energy_true: bad arguments");
+    CRF& c = *crf->crf;
+    Staged<int> a(labels, (size_t)c.N(), on_device != 0, c.stream(), true);
+    *energy = c.assignment_energy_true(a.d);
+  });
+}
+int dcrf_crf_assignment_energy_higher_order(dcrf_crf_t crf, const int* labels, int on_device, double* energy) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && labels && energy, "dcrf_crf_assignment_energy_higher_order: bad arguments");
+    CRF& c = *crf->crf;
+    Staged<int> a(labels, (size_t)c.N(), on_device != 0, c.stream(), true);
+    *energy = c.assignment_energy_ho(a.d);
+  });
+}
+int dcrf_crf_lp_energy(dcrf_crf_t crf, const float* Q, int on_device, double* energy) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && Q && energy, "dcrf_crf_lp_energy: bad arguments");
+    CRF& c = *crf->crf;
+    Staged<float> a(Q, (size_t)c.N() * c.L(), on_device != 0, c.stream(), true);
+    *energy = c.lp_energy(a.d);
+  });
+}
+int dcrf_crf_ucondgrad(dcrf_crf_t crf, const float* Q, float* Us, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && Q && Us, "dcrf_crf_ucondgrad: bad arguments");
+    CRF& c = *crf->crf;
+    const size_t n = (size_t)c.N() * c.L();
+    Staged<float> a(Q, n, on_device != 0, c.stream(), true);
+    Staged<float> b(Us, n, on_device != 0, c.stream(), false);
+    c.ucondgrad(a.d, b.d);
+    b.copy_out();
+    DCRF_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
+int dcrf_crf_qp_value(dcrf_crf_t crf, const float* Q, int on_device, double* energy) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && Q && energy, "dcrf_crf_qp_value: bad arguments");
+    CRF& c = *crf->crf;
+    Staged<float> a(Q, (size_t)c.N() * c.L(), on_device != 0, c.stream(), true);
+    *energy = c.qp_value(a.d);
+  });
+}
+
+// ---- device memory and timing ------------------------------------------------------
+int dcrf_malloc(void** ptr, size_t bytes) {
+  return guarded([&] {
+    DCRF_REQUIRE(ptr, "dcrf_malloc: null output");
+    *ptr = nullptr;
+    if (bytes) DCRF_HIP(hipMalloc(ptr, bytes));
+  });
+}
+int dcrf_free(void* ptr) {
+  return guarded([&] {
+    if (ptr) DCRF_HIP(hipFree(ptr));
+  });
+}
+int dcrf_memcpy(void* dst, const void* src, size_t bytes, int kind) {
+  return guarded([&] {
+    DCRF_REQUIRE(kind >= 0 && kind <= 2, "dcrf_memcpy: kind must be 0, 1 or 2");
+    const hipMemcpyKind k[3] = {hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice};
+    if (bytes) DCRF_HIP(hipMemcpy(dst, src, bytes, k[kind]));
+  });
+}
+int dcrf_device_synchronize(void) {
+  return guarded([&] { DCRF_HIP(hipDeviceSynchronize()); });
+}
+int dcrf_profile_enable(int on) {
+  return guarded([&] {
+    Profiler::get().clear();
+    Profiler::get().on.store(on != 0);
+  });
+}
+int dcrf_profile_report(char* buf, int len) {
+  std::string r;
+  const int rc = guarded([&] { r = Profiler::get().report(); });
+  if (rc != kOk) return -rc;
+  if (buf && len > 0) {
+    const size_t n = std::min((size_t)len - 1, r.size());
+    std::memcpy(buf, r.data(), n);
+    buf[n] = 0;
+  }
+  return (int)r.size() + 1;
+}
+
+}  // extern "C"

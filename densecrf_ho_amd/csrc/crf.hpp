@@ -1,0 +1,113 @@
+// Dense CRF on the device: unary, Potts pairwise potentials over permutohedral
+// lattices, Pn superpixel layers, and the QP / LP solvers.
+//
+// Reference: DenseCRF (include/densecrf.h:73-194, src/densecrf.cpp).
+// Fields are L x N stored label-contiguous per pixel (Eigen column-major),
+// index pixel*L + label, exactly as the reference's MatrixXf.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "lattice.hpp"
+
+namespace dcrf {
+
+// throws Error(kOutOfBounds, "Value out of bounds") unless all of d[0..n) is in [0,1]
+void check_unit_interval(const float* d, size_t n, hipStream_t s);
+
+enum NormType : int { kNoNorm = 0, kNormBefore = 1, kNormAfter = 2, kNormSymmetric = 3 };
+
+struct Potential {
+  std::unique_ptr<Lattice> lat;
+  DBuf<float> norm;  // per pixel (DenseKernel::norm_)
+  int d = 0;
+  float w = 1.f;     // Potts weight
+  int ntype = kNormSymmetric;
+};
+
+struct QPResult {
+  int iterations = 0, updates = 0;
+  std::vector<double> energy, num, denom, step;
+};
+
+struct LPParams {
+  float prox_reg_const = 1e-2f, dual_gap_tol = 1e3f, prox_energy_tol = 1e3f;
+  int prox_max_iter = 10, fw_max_iter = 10, qp_max_iter = 1000;
+  float qp_tol = 1e3f, qp_const = 1e-16f;
+  bool best_int = true;
+};
+
+struct LPResult {
+  int outer = 0;
+  std::vector<int> qp_sweeps;
+  std::vector<double> gap, delta, int_energy;
+  double init_lp_energy = 0, init_int_energy = 0;
+};
+
+class CRF {
+ public:
+  CRF(int N, int L);
+  ~CRF();
+  int N() const { return N_; }
+  int L() const { return L_; }
+  hipStream_t stream() const { return s_; }
+
+  // ---- model (device pointers) -------------------------------------------
+  void set_unary(const float* dU);                                         // setUnaryEnergy
+  void add_pairwise(const float* d_feat, int d, float w, int ntype);       // addPairwiseEnergy
+  void add_layer(const int* h_region, int R, const float* h_e, const float* h_c);  // superpixel layer
+  int num_pairwise() const { return (int)pw_.size(); }
+  Potential& pairwise(int k) { return *pw_.at(k); }
+  int num_regions() const { return R_; }
+
+  // ---- potentials (device pointers, N x L) ---------------------------------
+  void apply(int k, const float* in, float* out, bool transpose, bool no_norm);    // PairwisePotential::apply
+  void apply_ord(int k, const float* q01, float* out, bool check);                 // apply_upper_minus_lower_ord
+  void unary_init(float* Q);                                                       // DenseCRF::unary_init
+
+  // ---- solvers (device pointers) ---------------------------------------------
+  QPResult qp(const float* init, float* out, int max_iter, bool ref_stop, int stats_cap);
+  LPResult lp(const float* init, float* out, const LPParams& p, int stats_cap);
+
+  // ---- energies ---------------------------------------------------------------
+  void current_map(const float* Q, int* lab);                 // currentMap (device)
+  double assignment_energy_true(const int* lab);              // device labels
+  double assignment_energy_ho(const int* lab);
+  double lp_energy(const float* Q);                           // compute_energy_LP_higher_order
+  double qp_value(const float* Q);                            // compute_LR_QP_value(Q, 0)
+  void ucondgrad(const float* Q, float* Us);                  // computeUCondGrad
+
+  // z_labels_ after the last QP (L x R, region-major), host pointer
+  void z_labels(float* host);
+
+  struct Impl;
+  struct HOState;
+  struct LPState;
+
+ private:
+  // device-side building blocks of the LP and the energies
+  void int_energy_dev(const int* lab, double* dst, double* scratch, hipStream_t s);
+  void lp_energy_dev(const float* Q, double* dst, double* scratch, hipStream_t s);
+  void minmax_dev(const float* Q, size_t n, float* mm, int* fl, hipStream_t s);
+  void ucondgrad_dev(const float* Q, float* Us, hipStream_t s);
+  void ho_maxmin_dev(const float* Q, double* out, hipStream_t s);
+  void ho_assign_dev(const int* lab, double* out, int* scratch_int, hipStream_t s);
+  std::unique_ptr<LPState> lp_;
+  HOState* ho_state();
+  void ho_qp_init(hipStream_t s, const float* Q, float* gy, double* sho);
+  void ho_qp_direction(hipStream_t s, const float* Q, const int* lab, double* sho, int* fl);
+  void ho_qp_zstep(hipStream_t s, double* sc, int* fl);
+  void ho_qp_after(hipStream_t s, const float* Q, double* sho, int* fl);
+  void ho_qp_finish(hipStream_t s);
+  std::unique_ptr<HOState> ho_;
+
+  int N_, L_;
+  int R_ = 0;
+  hipStream_t s_ = nullptr;
+  DBuf<float> U_;
+  std::vector<std::unique_ptr<Potential>> pw_;
+  std::unique_ptr<Impl> im_;
+};
+
+}  // namespace dcrf

@@ -1,0 +1,295 @@
+// Per-pixel and reduction kernels shared by the solvers.  Included once, by
+// the engine's single translation unit.
+#pragma once
+
+#include "common.hpp"
+
+namespace dcrf {
+
+constexpr int kMaxLabels = 64;
+
+namespace {
+
+__global__ void k_fill(float* __restrict__ p, size_t n, float v) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+__global__ void k_fill_int(int* __restrict__ p, int n, int v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_add_scalar(double* dst, const double* src) { dst[0] = dst[0] + src[0]; }
+__global__ void k_sub_scalar(double* dst, const double* src) { dst[0] = dst[0] - src[0]; }
+
+__global__ void k_zero_scalars(double* __restrict__ sc, int ns, int* __restrict__ fl, int nf) {
+  for (int i = threadIdx.x; i < ns; i += blockDim.x) sc[i] = 0.0;
+  for (int i = threadIdx.x; i < nf; i += blockDim.x) fl[i] = 0;
+}
+
+// norm = 1 / sqrt(K1 + 1e-20) or 1 / (K1 + 1e-20), double then float
+__global__ void k_norm_finish(float* __restrict__ n, int N, int symmetric) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double v = (double)n[i] + 1e-20;
+  n[i] = symmetric ? (float)(1.0 / sqrt(v)) : (float)(1.0 / v);
+}
+
+// Reduction order of Eigen 3.2.7 on SSE2 for a short dense vector whose first
+// 16-byte-aligned element is `a` (include/Eigen/src/Core/Redux.h:193-245):
+// two 4-wide accumulators over the aligned span, (p0+p2)+(p1+p3), then the
+// unaligned head and tail in order.  `val(i)` supplies element i.
+template <class F>
+__device__ inline float eigen_sum(int n, int a, F val) {
+  if (a > n) a = n;
+  const int as2 = ((n - a) / 8) * 8, as1 = ((n - a) / 4) * 4;
+  const int ae2 = a + as2, ae = a + as1;
+  float res;
+  if (as1) {
+    float p0 = val(a), p1 = val(a + 1), p2 = val(a + 2), p3 = val(a + 3);
+    if (as1 > 4) {
+      float q0 = val(a + 4), q1 = val(a + 5), q2 = val(a + 6), q3 = val(a + 7);
+      for (int i = a + 8; i < ae2; i += 8) {
+        p0 = p0 + val(i);
+        p1 = p1 + val(i + 1);
+        p2 = p2 + val(i + 2);
+        p3 = p3 + val(i + 3);
+        q0 = q0 + val(i + 4);
+        q1 = q1 + val(i + 5);
+        q2 = q2 + val(i + 6);
+        q3 = q3 + val(i + 7);
+      }
+      p0 = p0 + q0;
+      p1 = p1 + q1;
+      p2 = p2 + q2;
+      p3 = p3 + q3;
+      if (ae > ae2) {
+        p0 = p0 + val(ae2);
+        p1 = p1 + val(ae2 + 1);
+        p2 = p2 + val(ae2 + 2);
+        p3 = p3 + val(ae2 + 3);
+      }
+    }
+    res = (p0 + p2) + (p1 + p3);
+    for (int i = 0; i < a; ++i) res = res + val(i);
+    for (int i = ae; i < n; ++i) res = res + val(i);
+  } else {
+    res = val(0);
+    for (int i = 1; i < n; ++i) res = res + val(i);
+  }
+  return res;
+}
+// first aligned element of column `col` of an L-row matrix with aligned storage
+__device__ inline int col_astart(long col, int L) { return (int)((4 - ((col * L) & 3)) & 3); }
+
+// expAndNormalize(-U) (src/densecrf.cpp:328-336), one thread per pixel
+__global__ void k_softmax_neg(const float* __restrict__ U, float* __restrict__ Q, int N, int L) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const float* u = U + (size_t)i * L;
+  float* q = Q + (size_t)i * L;
+  float mx = -u[0];
+  for (int l = 1; l < L; ++l) mx = fmaxf(mx, -u[l]);
+  for (int l = 0; l < L; ++l) q[l] = expf(-u[l] - mx);
+  const float s = eigen_sum(L, 0, [&](int l) { return q[l]; });
+  for (int l = 0; l < L; ++l) q[l] = q[l] / s;
+}
+
+// Σ a[i]*b[i] in FP64 over n elements: per-block partials, then one block
+__global__ void k_dot(const float* __restrict__ a, const float* __restrict__ b, size_t n, double* __restrict__ parts) {
+  double v[1] = {0.0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    v[0] += (double)a[i] * (double)b[i];
+  block_sum<1>(v);
+  if (threadIdx.x == 0) parts[blockIdx.x] = v[0];
+}
+inline void launch_dot(hipStream_t s, const float* a, const float* b, size_t n, double* parts, double* out) {
+  k_dot<<<kRedBlocks, kBlock, 0, s>>>(a, b, n, parts);
+  check_launch("k_dot");
+  finalize_sums<1><<<1, kBlock, 0, s>>>(parts, kRedBlocks, out);
+  check_launch("finalize_sums");
+}
+
+// grad_y += 2 * tmp (src/densecrf.cpp:380) and partial Σ Q*tmp (:953)
+__global__ void k_gy_init_accum(float* __restrict__ gy, const float* __restrict__ tmp, const float* __restrict__ Q,
+                                size_t n, double* __restrict__ parts) {
+  double v[1] = {0.0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float t = tmp[i];
+    gy[i] = gy[i] + 2.f * t;
+    v[0] += (double)Q[i] * (double)t;
+  }
+  block_sum<1>(v);
+  if (threadIdx.x == 0) parts[blockIdx.x] = v[0];
+}
+
+// slot layout shared with crf.hip
+constexpr int kS_ECUR = 0, kS_EPREV = 1, kS_NUM = 2, kS_DENOM = 3, kS_STEP = 4, kS_DOTS = 5, kS_EPART = 8,
+              kS_HO = 9, kS_TMP = 17;
+constexpr int kF_DONE = 0, kF_ITERS = 1;
+// higher-order scalar slots (relative to kS_HO)
+constexpr int kH_A = 0, kH_B = 1, kH_SXZ = 2, kH_DEN = 3, kH_E = 4;
+
+// E0 = U.Q - 0 + Σ_k Q.psi_k Q + 0 (+ higher order)  (:386-387, :940-957)
+__global__ void k_qp_e0(double* sc, int nk, int has_ho, double* st_energy, int cap) {
+  double e = 0;
+  e += sc[kS_TMP + 0];
+  e -= 0.0;
+  for (int k = 0; k < nk; ++k) e += sc[kS_TMP + 1 + k];
+  e += 0.0;
+  if (has_ho) e += sc[kS_HO + kH_E];
+  sc[kS_ECUR] = e;
+  sc[kS_EPREV] = 1.7976931348623157e308;
+  if (cap > 0) st_energy[0] = e;
+}
+
+// descent_direction (src/qp.cpp:5-15): first minimum of each column
+__global__ void k_argmin(const float* __restrict__ g, int* __restrict__ lab, int N, int L, const int* __restrict__ fl) {
+  if (fl[kF_DONE]) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const float* r = g + (size_t)i * L;
+  int m = 0;
+  float v = r[0];
+  for (int l = 1; l < L; ++l) {
+    const float x = r[l];
+    if (x < v) {
+      v = x;
+      m = l;
+    }
+  }
+  lab[i] = m;
+}
+
+// line-search sums U.sx, Q.psisx, sx.psisx with sx = onehot(lab) - Q (:408-414)
+__global__ void k_qp_dots(const float* __restrict__ U, const float* __restrict__ Q, const int* __restrict__ lab,
+                          const float* __restrict__ psx, size_t n, FastDiv divL, double* __restrict__ parts,
+                          const int* __restrict__ fl) {
+  if (fl[kF_DONE]) return;
+  const int L = (int)divL.d;
+  double v[3] = {0.0, 0.0, 0.0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t pix = divL.div((uint32_t)i);
+    const int l = (int)(i - (size_t)pix * L);
+    const float q = Q[i], p = psx[i];
+    const float sx = (lab[pix] == l ? 1.f : 0.f) - q;
+    v[0] += (double)U[i] * (double)sx;
+    v[1] += (double)q * (double)p;
+    v[2] += (double)sx * (double)p;
+  }
+  block_sum<3>(v);
+  if (threadIdx.x == 0) {
+    parts[blockIdx.x * 3 + 0] = v[0];
+    parts[blockIdx.x * 3 + 1] = v[1];
+    parts[blockIdx.x * 3 + 2] = v[2];
+  }
+}
+
+// step size (:412-421); done = 2 when the reference breaks on denom==0||num==0
+__global__ void k_qp_step(double* sc, int* fl, int has_ho, double* st_num, double* st_denom, double* st_step,
+                          double* st_energy, int cap) {
+  if (fl[kF_DONE]) return;
+  sc[kS_EPREV] = sc[kS_ECUR];
+  double num = sc[kS_DOTS + 0] + 2 * sc[kS_DOTS + 1];
+  double denom = sc[kS_DOTS + 2];
+  if (has_ho) {
+    num = num + sc[kS_HO + kH_A] + sc[kS_HO + kH_B] + sc[kS_HO + kH_SXZ];
+    denom = denom + sc[kS_HO + kH_DEN];
+  }
+  double step = -num / (2 * denom);
+  const int it = fl[kF_ITERS]++;  // iterations entered, as the reference's counter
+  if (it < cap) {
+    st_num[it] = num;
+    st_denom[it] = denom;
+  }
+  sc[kS_NUM] = num;
+  sc[kS_DENOM] = denom;
+  if (denom == 0 || num == 0) {
+    if (it + 1 < cap + 1) st_energy[it + 1] = sc[kS_ECUR];  // no update: energy unchanged
+    fl[kF_DONE] = 2;
+    return;
+  }
+  if (denom < 0) step = 1;
+  sc[kS_STEP] = step;
+  if (it < cap) st_step[it] = step;
+}
+
+// Q += step*sx; grad_y += 2*step*psisx + step*hb; partial Q.(grad_y+U) (:424-434)
+__global__ void k_qp_update(const float* __restrict__ U, float* __restrict__ Q, float* __restrict__ gy,
+                            const float* __restrict__ psx, const int* __restrict__ lab, const float* __restrict__ hb,
+                            int hb_npix, size_t n, FastDiv divL, const double* __restrict__ sc,
+                            double* __restrict__ parts, const int* __restrict__ fl) {
+  if (fl[kF_DONE]) return;
+  const int L = (int)divL.d;
+  const double step = sc[kS_STEP];
+  const float st = (float)step, st2 = (float)(2 * step);
+  double v[1] = {0.0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t pix = divL.div((uint32_t)i);
+    const int l = (int)(i - (size_t)pix * L);
+    const float q0 = Q[i];
+    const float sx = (lab[pix] == l ? 1.f : 0.f) - q0;
+    const float q = q0 + st * sx;
+    const float h = (hb != nullptr && (int)pix < hb_npix) ? hb[i] : 0.f;
+    const float g = gy[i] + (st2 * psx[i] + st * h);
+    Q[i] = q;
+    gy[i] = g;
+    v[0] += (double)q * (double)(g + U[i]);
+  }
+  block_sum<1>(v);
+  if (threadIdx.x == 0) parts[blockIdx.x] = v[0];
+}
+
+// new energy and the reference's stopping rule old - energy > DIFF_ENG=100
+// (:390, :434-435)
+__global__ void k_qp_check(double* sc, int* fl, int ref_stop, int has_ho, double* st_energy, int cap) {
+  if (fl[kF_DONE]) return;
+  double e = sc[kS_EPART];
+  if (has_ho) e += sc[kS_HO + kH_E];
+  sc[kS_ECUR] = e;
+  const int it = fl[kF_ITERS];
+  if (it < cap + 1) st_energy[it] = e;
+  if (ref_stop && !((sc[kS_EPREV] - e) > 100.0)) fl[kF_DONE] = 1;
+}
+
+// currentMap (src/densecrf.cpp:929-938): first maximum of each column
+__global__ void k_argmax(const float* __restrict__ Q, int* __restrict__ lab, int N, int L) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const float* r = Q + (size_t)i * L;
+  int m = 0;
+  float v = r[0];
+  for (int l = 1; l < L; ++l) {
+    const float x = r[l];
+    if (x > v) {
+      v = x;
+      m = l;
+    }
+  }
+  lab[i] = m;
+}
+
+// r[i] += -out(l_i, i) of one no-normalisation potential (:915-925)
+__global__ void k_pairwise_true_accum(const float* __restrict__ o, const int* __restrict__ lab,
+                                      float* __restrict__ r, int N, int L) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int l = lab[i];
+  const float v = (l >= 0 && l < L) ? -o[(size_t)i * L + l] : 0.f;
+  r[i] = r[i] + v;
+}
+
+// Σ_i (unary(l_i, i) + pairwise_i) in FP64 (:873-885)
+__global__ void k_assign_energy(const float* __restrict__ U, const int* __restrict__ lab,
+                                const float* __restrict__ r, int N, int L, double* __restrict__ parts) {
+  double v[1] = {0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    const int l = lab[i];
+    const float u = (l >= 0 && l < L) ? U[(size_t)i * L + l] : 0.f;
+    v[0] += (double)(u + r[i]);
+  }
+  block_sum<1>(v);
+  if (threadIdx.x == 0) parts[blockIdx.x] = v[0];
+}
+
+}  // namespace
+}  // namespace dcrf

@@ -1,0 +1,7 @@
+// Single translation unit of libdensecrf_ho_amd.so (kernels live in
+// anonymous namespaces of the included sources).
+#include "lattice.hip"
+#include "crf.hip"
+#include "ho.hip"
+#include "lp.hip"
+#include "capi.hip"

@@ -1,0 +1,498 @@
+// Pn superpixel (higher-order) terms on the device.
+//
+// QP side, bug-compatible with DenseCRF::multiplySuperPixels
+// (src/densecrf.cpp:58-102): region r of size s_r sums p(l, pixel) over the
+// LOOP INDEX pixel = 0..s_r-1, not over its members, and broadcasts back to
+// columns 0..s_r-1.  With the distinct region sizes sorted ascending
+// (u_0 < u_1 < ...), both become scans over the size groups:
+//   pixels -> regions: NtoR(l, r) = sum_{g <= g(r)} sum_{pix in [u_{g-1}, u_g)} p(l, pix)
+//   regions -> pixels: RtoN(pix, l) = sum_{g : u_g > pix} sum_{r in group g} p(l, r)
+// evaluated in FP64 in a fixed order (the reference accumulates in float).
+//
+// LP side (correct member ids): computeUCondGrad (:803-830),
+// the max-min term of compute_energy_LP_higher_order (:1020-1034) and the
+// region term of assignment_energy_higher_order (:856-867).
+#include <algorithm>
+#include <numeric>
+
+#include "crf.hpp"
+#include "crf_kernels.hpp"
+
+namespace dcrf {
+
+struct CRF::HOState {
+  int R = 0, G = 0, smax = 0, nmem = 0;
+  std::vector<int> h_size;            // per region
+  std::vector<int> h_layer_start;     // first region of each layer (+ end)
+  std::vector<float> h_e, h_c;
+  std::vector<int> h_mstart, h_mpix;  // member CSR (ascending pixel ids)
+  DBuf<float> e, c;
+  DBuf<int> usz, grp_start, grp_reg, reg_grp, mstart, mpix, layer_of;
+  // QP state (L x R) and scratch
+  DBuf<float> z, gz, sxz, cgz;
+  DBuf<double> nrQ, nrS, segB, prefB;
+  DBuf<float> hb;  // smax x L
+  // LP scratch: per region min/max positions
+  DBuf<int> mpos;  // 4 per region: min_pix, min_lab, max_pix, max_lab
+  DBuf<double> parts;
+  bool dirty = true;
+};
+
+namespace {
+
+// one thread per (size group, label): sum of P(pix, l) over pix in
+// [u_{g-1}, u_g); MODE 0: P = Q - 1, MODE 1: P = onehot(lab) - Q
+template <int MODE>
+__global__ void k_ho_segsum(const float* __restrict__ Q, const int* __restrict__ lab, int L,
+                            const int* __restrict__ usz, int G, double* __restrict__ B, const int* __restrict__ fl) {
+  if (fl && fl[kF_DONE]) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= G * L) return;
+  const int g = t / L, l = t - g * L;
+  const int p0 = g ? usz[g - 1] : 0, p1 = usz[g];
+  double s = 0;
+  for (int p = p0; p < p1; ++p) {
+    const float q = Q[(size_t)p * L + l];
+    const float v = MODE == 0 ? q - 1.f : (lab[p] == l ? 1.f : 0.f) - q;
+    s += (double)v;
+  }
+  B[(size_t)g * L + l] = s;
+}
+
+// inclusive prefix over groups (PREFIX) or suffix (!PREFIX), one thread per label
+template <bool PREFIX>
+__global__ void k_ho_scan(const double* __restrict__ B, double* __restrict__ out, int G, int L,
+                          const int* __restrict__ fl) {
+  if (fl && fl[kF_DONE]) return;
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= L) return;
+  double s = 0;
+  if (PREFIX)
+    for (int g = 0; g < G; ++g) out[(size_t)g * L + l] = (s += B[(size_t)g * L + l]);
+  else
+    for (int g = G - 1; g >= 0; --g) out[(size_t)g * L + l] = (s += B[(size_t)g * L + l]);
+}
+
+// per (group, label): sum over the group's regions of p(l, r) where
+// p = (z - 1) * e (the argument of the broadcast in :382 and :431)
+__global__ void k_ho_groupsum_zm1(const float* __restrict__ z, const float* __restrict__ e,
+                                  const int* __restrict__ gstart, const int* __restrict__ greg, int G, int L,
+                                  double* __restrict__ A, const int* __restrict__ fl) {
+  if (fl && fl[kF_DONE]) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= G * L) return;
+  const int g = t / L, l = t - g * L;
+  double s = 0;
+  for (int i = gstart[g]; i < gstart[g + 1]; ++i) {
+    const int r = greg[i];
+    s += (double)((z[(size_t)r * L + l] - 1.f) * e[r]);
+  }
+  A[(size_t)g * L + l] = s;
+}
+
+// hb(pix, l) = suffix(g*) with g* the first group whose size exceeds pix
+__global__ void k_ho_bcast(const double* __restrict__ suf, const int* __restrict__ usz, int G, int smax, int L,
+                           float* __restrict__ hb, const int* __restrict__ fl) {
+  if (fl && fl[kF_DONE]) return;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)smax * L) return;
+  const int pix = (int)(t / L), l = (int)(t - (size_t)pix * L);
+  int lo = 0, hi = G - 1;  // smallest g with usz[g] > pix (exists since pix < smax = usz[G-1])
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (usz[mid] > pix)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  hb[t] = (float)suf[(size_t)lo * L + l];
+}
+
+// grad_y += hb on the first smax pixels (:382)
+__global__ void k_ho_add_hb(float* __restrict__ gy, const float* __restrict__ hb, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) gy[t] = gy[t] + hb[t];
+}
+
+// after NtoR(Q-1): grad_z and the higher-order energy
+// init (:373-375):      grad_z = (1 + NtoR(Q-1)) .* c
+// iteration (:432):     grad_z = c + NtoR(Q-1) .* c
+// energy (:387, :435):  sum(z .* c) + sum((1 - z) .* c .* NtoR(Q-1))
+__global__ void k_ho_gradz_energy(const float* __restrict__ z, const float* __restrict__ e,
+                                  const int* __restrict__ reg_grp, const double* __restrict__ prefQ, int R, int L,
+                                  int init, float* __restrict__ gz, double* __restrict__ nrQ,
+                                  double* __restrict__ sho, const int* __restrict__ fl) {
+  if (fl && fl[kF_DONE]) return;
+  double v[1] = {0.0};
+  for (int t = threadIdx.x; t < R * L; t += blockDim.x) {
+    const int r = t / L, l = t - r * L;
+    const double nr = prefQ[(size_t)reg_grp[r] * L + l];
+    nrQ[t] = nr;
+    const float nrf = (float)nr, c = e[r];
+    gz[t] = init ? (1.f + nrf) * c : c + nrf * c;
+    v[0] += (double)(z[t] * c) + (double)((1.f - z[t]) * c) * nr;
+  }
+  block_sum<1>(v);
+  if (threadIdx.x == 0) sho[kH_E] = v[0];
+}
+
+// direction terms (:396-414): cond_grad_z, sx_z and the four scalars
+__global__ void k_ho_direction(const float* __restrict__ z, const float* __restrict__ gz, const float* __restrict__ e,
+                               const int* __restrict__ reg_grp, const double* __restrict__ prefS,
+                               const double* __restrict__ nrQ, int R, int L, float* __restrict__ cgz,
+                               float* __restrict__ sxz, double* __restrict__ sho, const int* __restrict__ fl) {
+  if (fl[kF_DONE]) return;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int t = threadIdx.x; t < R * L; t += blockDim.x) {
+    const int r = t / L, l = t - r * L;
+    const float cg = gz[t] >= 0 ? 0.f : 1.f;  // descent_direction_z (src/qp.cpp:17-25)
+    const float sz = cg - z[t];
+    cgz[t] = cg;
+    sxz[t] = sz;
+    const float c = e[r];
+    const double nrS = prefS[(size_t)reg_grp[r] * L + l];
+    const float szc = sz * c;
+    v[0] += (double)szc * nrQ[t];              // a
+    v[1] += (double)((cg - 1.f) * c) * nrS;    // b
+    v[2] += (double)sz;                        // sum(sx_z)
+    v[3] += (double)szc * nrS;                 // denominator term
+  }
+  block_sum<4>(v);
+  if (threadIdx.x == 0) {
+    sho[kH_A] = v[0];
+    sho[kH_B] = v[1];
+    sho[kH_SXZ] = v[2];
+    sho[kH_DEN] = v[3];
+  }
+}
+
+// z += step * sx_z (:425)
+__global__ void k_ho_zstep(float* __restrict__ z, const float* __restrict__ sxz, int n, const double* __restrict__ sc,
+                           const int* __restrict__ fl) {
+  if (fl[kF_DONE]) return;
+  const float st = (float)sc[kS_STEP];
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) z[t] = z[t] + st * sxz[t];
+}
+
+// ---- LP side -------------------------------------------------------------------
+// computeUCondGrad: scan (label-major, members in order) with int `min`/`max`.
+// The running int min after element k is trunc(prefix-min of Q); element k
+// takes the min position iff Q_k < trunc(prefix-min before k) (the first
+// element always does).  Likewise for the max.  One wave per region.
+__global__ void k_ucond_scan(const float* __restrict__ Q, int L, const int* __restrict__ mstart,
+                             const int* __restrict__ mpix, int r0, int r1, int* __restrict__ mpos) {
+  const int r = r0 + (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= r1) return;
+  const int b = mstart[r], s = mstart[r + 1] - b;
+  const long n = (long)s * L;
+  float carry_min = INFINITY, carry_max = -INFINITY;  // prefix over earlier chunks
+  long best_min = -1, best_max = -1;
+  for (long base = 0; base < n; base += 64) {
+    const long k = base + lane;
+    float q = 0.f;
+    const bool valid = k < n;
+    if (valid) {
+      const int lab = (int)(k / s), m = (int)(k - (long)lab * s);
+      q = Q[(size_t)mpix[b + m] * L + lab];
+    }
+    // exclusive prefix min/max within the chunk
+    float pmin = valid ? q : INFINITY, pmax = valid ? q : -INFINITY;
+    for (int o = 1; o < 64; o <<= 1) {
+      const float a = __shfl_up(pmin, o, 64), c = __shfl_up(pmax, o, 64);
+      if (lane >= o) {
+        pmin = fminf(pmin, a);
+        pmax = fmaxf(pmax, c);
+      }
+    }
+    float emin = __shfl_up(pmin, 1, 64), emax = __shfl_up(pmax, 1, 64);
+    if (lane == 0) {
+      emin = INFINITY;
+      emax = -INFINITY;
+    }
+    emin = fminf(emin, carry_min);
+    emax = fmaxf(emax, carry_max);
+    if (valid) {
+      // int state before k: INT_MAX / INT_MIN initially, else trunc(prefix)
+      const bool take_min = (emin == INFINITY) ? (2147483648.0f > q) : ((float)(int)emin > q);
+      const bool take_max = (emax == -INFINITY) ? (-2147483648.0f < q) : ((float)(int)emax < q);
+      if (take_min) best_min = k;
+      if (take_max) best_max = k;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      best_min = max(best_min, (long)__shfl_xor(best_min, o, 64));
+      best_max = max(best_max, (long)__shfl_xor(best_max, o, 64));
+    }
+    carry_min = fminf(carry_min, __shfl(pmin, 63, 64));
+    carry_max = fmaxf(carry_max, __shfl(pmax, 63, 64));
+  }
+  if (lane == 0) {
+    const long kmin = best_min < 0 ? 0 : best_min, kmax = best_max < 0 ? 0 : best_max;
+    const int lmin = (int)(kmin / s), lmax = (int)(kmax / s);
+    mpos[4 * r + 0] = mpix[b + (int)(kmin - (long)lmin * s)];
+    mpos[4 * r + 1] = lmin;
+    mpos[4 * r + 2] = mpix[b + (int)(kmax - (long)lmax * s)];
+    mpos[4 * r + 3] = lmax;
+  }
+}
+
+// Us(min) = c*e, then Us(max) = -c*e (the max write wins), regions of one layer
+__global__ void k_ucond_write(const int* __restrict__ mpos, const float* __restrict__ e, const float* __restrict__ c,
+                              int r0, int r1, int L, float* __restrict__ Us) {
+  const int r = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= r1) return;
+  const float v = c[r] * e[r];
+  Us[(size_t)mpos[4 * r + 0] * L + mpos[4 * r + 1]] = v;
+  Us[(size_t)mpos[4 * r + 2] * L + mpos[4 * r + 3]] = -c[r] * e[r];
+}
+
+// sum_r e_r * max_l (max_members Q - min_members Q), one block per region
+__global__ void k_ho_maxmin(const float* __restrict__ Q, int L, const int* __restrict__ mstart,
+                            const int* __restrict__ mpix, const float* __restrict__ e, double* __restrict__ parts) {
+  const int r = blockIdx.x;
+  const int b = mstart[r], s = mstart[r + 1] - b;
+  __shared__ float smax[kMaxLabels][4], smin[kMaxLabels][4];
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int l = 0; l < L; ++l) {
+    float mx = -INFINITY, mn = INFINITY;
+    for (int m = threadIdx.x; m < s; m += blockDim.x) {
+      const float q = Q[(size_t)mpix[b + m] * L + l];
+      mx = fmaxf(mx, q);
+      mn = fminf(mn, q);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+      mn = fminf(mn, __shfl_xor(mn, o, 64));
+    }
+    if (lane == 0) {
+      smax[l][wid] = mx;
+      smin[l][wid] = mn;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float md = -INFINITY;
+    for (int l = 0; l < L; ++l) {
+      float mx = smax[l][0], mn = smin[l][0];
+      for (int w = 1; w < (int)blockDim.x / 64; ++w) {
+        mx = fmaxf(mx, smax[l][w]);
+        mn = fminf(mn, smin[l][w]);
+      }
+      md = fmaxf(md, mx - mn);
+    }
+    parts[r] = (double)(e[r] * md);
+  }
+}
+
+// assignment_energy_higher_order's region term (:856-867), bug-compatible:
+// region r differs iff some loop index pix < s_r has lab[pix] != lab[first member]
+__global__ void k_ho_assign(const int* __restrict__ lab, const int* __restrict__ mstart,
+                            const int* __restrict__ mpix, const float* __restrict__ e, int R,
+                            const int* __restrict__ fd, double* __restrict__ parts) {
+  const int first_diff = *fd;
+  double v[1] = {0.0};
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < R; r += gridDim.x * blockDim.x) {
+    const int s = mstart[r + 1] - mstart[r];
+    const int l0 = (int)(short)lab[mpix[mstart[r]]];
+    const bool differ = l0 != (int)(short)lab[0] || first_diff < s;
+    if (differ) v[0] += (double)e[r];
+    v[0] += (double)((float)(R - 1) * e[r]);
+  }
+  block_sum<1>(v);
+  if (threadIdx.x == 0) parts[blockIdx.x] = v[0];
+}
+
+// first pixel index whose label differs from lab[0] (N if none)
+__global__ void k_first_diff(const int* __restrict__ lab, int N, int* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N && (short)lab[i] != (short)lab[0]) atomicMin(out, i);
+}
+
+}  // namespace
+
+float* ho_hb(CRF::HOState* h) { return h->hb.get(); }
+int ho_smax(CRF::HOState* h) { return h->smax; }
+
+CRF::HOState* CRF::ho_state() {
+  if (R_ == 0) return nullptr;
+  HOState& h = *ho_;
+  if (h.dirty) {
+    // distinct sizes and size groups
+    const int R = h.R;
+    std::vector<int> order(R);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return h.h_size[a] < h.h_size[b]; });
+    std::vector<int> usz, gstart, reg_grp(R);
+    for (int i = 0; i < R; ++i) {
+      const int sz = h.h_size[order[i]];
+      if (usz.empty() || usz.back() != sz) {
+        usz.push_back(sz);
+        gstart.push_back(i);
+      }
+      reg_grp[order[i]] = (int)usz.size() - 1;
+    }
+    gstart.push_back(R);
+    h.G = (int)usz.size();
+    h.smax = usz.back();
+    auto up = [&](DBuf<int>& d, const std::vector<int>& v) {
+      d.resize(std::max<size_t>(v.size(), 1));
+      DCRF_HIP(hipMemcpyAsync(d.get(), v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice, s_));
+    };
+    up(h.usz, usz);
+    up(h.grp_start, gstart);
+    up(h.grp_reg, order);
+    up(h.reg_grp, reg_grp);
+    up(h.mstart, h.h_mstart);
+    up(h.mpix, h.h_mpix);
+    h.e.resize(R);
+    h.c.resize(R);
+    DCRF_HIP(hipMemcpyAsync(h.e.get(), h.h_e.data(), R * sizeof(float), hipMemcpyHostToDevice, s_));
+    DCRF_HIP(hipMemcpyAsync(h.c.get(), h.h_c.data(), R * sizeof(float), hipMemcpyHostToDevice, s_));
+    const size_t LR = (size_t)L_ * R, GL = (size_t)L_ * h.G;
+    h.z.resize(LR);
+    h.gz.resize(LR);
+    h.sxz.resize(LR);
+    h.cgz.resize(LR);
+    h.nrQ.resize(LR);
+    h.nrS.resize(LR);
+    h.segB.resize(GL);
+    h.prefB.resize(GL);
+    h.hb.resize((size_t)h.smax * L_);
+    h.mpos.resize(4 * (size_t)R);
+    h.parts.resize(std::max(R, kRedBlocks));
+    DCRF_HIP(hipStreamSynchronize(s_));
+    h.dirty = false;
+  }
+  return &h;
+}
+
+void CRF::add_layer(const int* h_region, int R, const float* h_e, const float* h_c) {
+  DCRF_REQUIRE(R > 0, "superpixel layer: R must be positive");
+  if (!ho_) ho_ = std::make_unique<HOState>();
+  HOState& h = *ho_;
+  std::vector<std::vector<int>> mem(R);
+  for (int i = 0; i < N_; ++i) {
+    const int r = h_region[i];
+    DCRF_REQUIRE(r >= 0 && r < R, "superpixel layer: region id out of range");
+    mem[r].push_back(i);
+  }
+  if (h.h_mstart.empty()) h.h_mstart.push_back(0);
+  h.h_layer_start.push_back(h.R);
+  for (int r = 0; r < R; ++r) {
+    if (mem[r].empty()) continue;  // regions without pixels are dropped
+    h.h_size.push_back((int)mem[r].size());
+    h.h_e.push_back(h_e[r]);
+    h.h_c.push_back(h_c[r]);
+    h.h_mpix.insert(h.h_mpix.end(), mem[r].begin(), mem[r].end());
+    h.h_mstart.push_back((int)h.h_mpix.size());
+    ++h.R;
+  }
+  h.nmem = (int)h.h_mpix.size();
+  R_ = h.R;
+  h.dirty = true;
+}
+
+// QP start: z = 1, grad_z, cond_grad_z, grad_y += RtoN((z-1).*c), energy term
+void CRF::ho_qp_init(hipStream_t s, const float* Q, float* gy, double* sho) {
+  HOState& h = *ho_;
+  const int G = h.G, L = L_, R = h.R;
+  k_fill<<<grid_for((size_t)R * L), kBlock, 0, s>>>(h.z.get(), (size_t)R * L, 1.f);
+  k_ho_segsum<0><<<grid_for((size_t)G * L), kBlock, 0, s>>>(Q, nullptr, L, h.usz.get(), G, h.segB.get(), nullptr);
+  k_ho_scan<true><<<1, 64, 0, s>>>(h.segB.get(), h.prefB.get(), G, L, nullptr);
+  k_ho_gradz_energy<<<1, kBlock, 0, s>>>(h.z.get(), h.e.get(), h.reg_grp.get(), h.prefB.get(), R, L, 1, h.gz.get(),
+                                         h.nrQ.get(), sho, nullptr);
+  k_ho_groupsum_zm1<<<grid_for((size_t)G * L), kBlock, 0, s>>>(h.z.get(), h.e.get(), h.grp_start.get(),
+                                                               h.grp_reg.get(), G, L, h.segB.get(), nullptr);
+  k_ho_scan<false><<<1, 64, 0, s>>>(h.segB.get(), h.prefB.get(), G, L, nullptr);
+  k_ho_bcast<<<grid_for((size_t)h.smax * L), kBlock, 0, s>>>(h.prefB.get(), h.usz.get(), G, h.smax, L, h.hb.get(),
+                                                             nullptr);
+  k_ho_add_hb<<<grid_for((size_t)h.smax * L), kBlock, 0, s>>>(gy, h.hb.get(), (size_t)h.smax * L);
+  check_launch("ho_qp_init");
+}
+
+// after the argmin: cond_grad_z, sx_z, NtoR(sx) and the line-search terms
+void CRF::ho_qp_direction(hipStream_t s, const float* Q, const int* lab, double* sho, int* fl) {
+  HOState& h = *ho_;
+  const int G = h.G, L = L_;
+  k_ho_segsum<1><<<grid_for((size_t)G * L), kBlock, 0, s>>>(Q, lab, L, h.usz.get(), G, h.segB.get(), fl);
+  k_ho_scan<true><<<1, 64, 0, s>>>(h.segB.get(), h.prefB.get(), G, L, fl);
+  k_ho_direction<<<1, kBlock, 0, s>>>(h.z.get(), h.gz.get(), h.e.get(), h.reg_grp.get(), h.prefB.get(), h.nrQ.get(),
+                                      h.R, L, h.cgz.get(), h.sxz.get(), sho, fl);
+  check_launch("ho_qp_direction");
+}
+
+// after the step: z update and the new broadcast term RtoN((z-1).*c)
+void CRF::ho_qp_zstep(hipStream_t s, double* sc, int* fl) {
+  HOState& h = *ho_;
+  const int G = h.G, L = L_;
+  k_ho_zstep<<<grid_for((size_t)h.R * L), kBlock, 0, s>>>(h.z.get(), h.sxz.get(), h.R * L, sc, fl);
+  k_ho_groupsum_zm1<<<grid_for((size_t)G * L), kBlock, 0, s>>>(h.z.get(), h.e.get(), h.grp_start.get(),
+                                                               h.grp_reg.get(), G, L, h.segB.get(), fl);
+  k_ho_scan<false><<<1, 64, 0, s>>>(h.segB.get(), h.prefB.get(), G, L, fl);
+  k_ho_bcast<<<grid_for((size_t)h.smax * L), kBlock, 0, s>>>(h.prefB.get(), h.usz.get(), G, h.smax, L, h.hb.get(),
+                                                             fl);
+  check_launch("ho_qp_zstep");
+}
+
+// after the update of Q: NtoR(Q-1), grad_z and the energy term
+void CRF::ho_qp_after(hipStream_t s, const float* Q, double* sho, int* fl) {
+  HOState& h = *ho_;
+  const int G = h.G, L = L_;
+  k_ho_segsum<0><<<grid_for((size_t)G * L), kBlock, 0, s>>>(Q, nullptr, L, h.usz.get(), G, h.segB.get(), fl);
+  k_ho_scan<true><<<1, 64, 0, s>>>(h.segB.get(), h.prefB.get(), G, L, fl);
+  k_ho_gradz_energy<<<1, kBlock, 0, s>>>(h.z.get(), h.e.get(), h.reg_grp.get(), h.prefB.get(), h.R, L, 0,
+                                         h.gz.get(), h.nrQ.get(), sho, fl);
+  check_launch("ho_qp_after");
+}
+
+void CRF::ho_qp_finish(hipStream_t) {}
+
+// z_labels_ after the last QP (L x R, region-major)
+void CRF::z_labels(float* host) {
+  HOState* h = ho_state();
+  if (!h) return;
+  DCRF_HIP(hipMemcpyAsync(host, h->z.get(), (size_t)h->R * L_ * sizeof(float), hipMemcpyDeviceToHost, s_));
+  DCRF_HIP(hipStreamSynchronize(s_));
+}
+
+// computeUCondGrad (src/densecrf.cpp:803-830) into a zero-filled Us
+void CRF::ucondgrad_dev(const float* Q, float* Us, hipStream_t s) {
+  HOState& h = *ho_state();
+  k_fill<<<grid_for((size_t)N_ * L_), kBlock, 0, s>>>(Us, (size_t)N_ * L_, 0.f);
+  const int nl = (int)h.h_layer_start.size();
+  k_ucond_scan<<<grid_for((size_t)h.R * 64), kBlock, 0, s>>>(Q, L_, h.mstart.get(), h.mpix.get(), 0, h.R,
+                                                             h.mpos.get());
+  for (int li = 0; li < nl; ++li) {  // later layers overwrite earlier ones, as in region order
+    const int r0 = h.h_layer_start[li], r1 = li + 1 < nl ? h.h_layer_start[li + 1] : h.R;
+    if (r1 > r0)
+      k_ucond_write<<<grid_for(r1 - r0), kBlock, 0, s>>>(h.mpos.get(), h.e.get(), h.c.get(), r0, r1, L_, Us);
+  }
+  check_launch("ucondgrad");
+}
+
+void CRF::ucondgrad(const float* Q, float* Us) {
+  DCRF_REQUIRE(R_ > 0, "computeUCondGrad: no superpixel layer");
+  ucondgrad_dev(Q, Us, s_);
+  DCRF_HIP(hipStreamSynchronize(s_));
+}
+
+// sum_r e_r * max_l(max - min) into *out (device double)
+void CRF::ho_maxmin_dev(const float* Q, double* out, hipStream_t s) {
+  HOState& h = *ho_state();
+  k_ho_maxmin<<<h.R, kBlock, 0, s>>>(Q, L_, h.mstart.get(), h.mpix.get(), h.e.get(), h.parts.get());
+  finalize_sums<1><<<1, kBlock, 0, s>>>(h.parts.get(), h.R, out);
+  check_launch("ho_maxmin");
+}
+
+// region term of assignment_energy_higher_order into *out (device double)
+void CRF::ho_assign_dev(const int* lab, double* out, int* scratch_int, hipStream_t s) {
+  HOState& h = *ho_state();
+  k_fill_int<<<1, 1, 0, s>>>(scratch_int, 1, N_);
+  k_first_diff<<<grid_for(N_), kBlock, 0, s>>>(lab, N_, scratch_int);
+  k_ho_assign<<<kRedBlocks, kBlock, 0, s>>>(lab, h.mstart.get(), h.mpix.get(), h.e.get(), h.R, scratch_int,
+                                            h.parts.get());
+  finalize_sums<1><<<1, kBlock, 0, s>>>(h.parts.get(), kRedBlocks, out);
+  check_launch("ho_assign");
+}
+
+}  // namespace dcrf

@@ -1,0 +1,780 @@
+// Permutohedral lattice on gfx950: build (elevation, hash, compaction,
+// neighbours, splat gather lists) and the plain / ordered filters.
+//
+// Float arithmetic follows the reference operation by operation and the
+// library is compiled with -ffp-contract=off, so the lattice tables are
+// bit-identical to the reference's and the filters are too wherever the
+// splat sums a lattice point in one chunk (sequential pixel order).
+#include <hipcub/hipcub.hpp>
+
+#include "lattice.hpp"
+
+namespace dcrf {
+namespace {
+
+constexpr int kMaxD = 12;
+constexpr int kChunk = 64;  // splat entries summed sequentially per chunk
+
+// ---- packed keys: 4 int16 coordinates per 64-bit word ---------------------
+template <int D>
+struct PKey {
+  static constexpr int W = (D + 3) / 4;
+  uint64_t w[W];
+};
+
+template <int D>
+__device__ inline PKey<D> pack(const int16_t* k) {
+  PKey<D> p;
+#pragma unroll
+  for (int i = 0; i < PKey<D>::W; ++i) p.w[i] = 0;
+#pragma unroll
+  for (int i = 0; i < D; ++i) p.w[i / 4] |= (uint64_t)(uint16_t)k[i] << (16 * (i % 4));
+  return p;
+}
+template <int D>
+__device__ inline void unpack(const PKey<D>& p, int16_t* k) {
+#pragma unroll
+  for (int i = 0; i < D; ++i) k[i] = (int16_t)(uint16_t)(p.w[i / 4] >> (16 * (i % 4)));
+}
+template <int D>
+__device__ inline bool key_eq(const PKey<D>& a, const PKey<D>& b) {
+  bool eq = true;
+#pragma unroll
+  for (int i = 0; i < PKey<D>::W; ++i) eq = eq && (a.w[i] == b.w[i]);
+  return eq;
+}
+template <int D>
+__device__ inline uint32_t key_hash(const PKey<D>& k) {
+  uint64_t x = 0x9E3779B97F4A7C15ull;
+#pragma unroll
+  for (int i = 0; i < PKey<D>::W; ++i) {
+    x ^= k.w[i];
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 31;
+  }
+  x *= 0x94d049bb133111ebull;
+  x ^= x >> 29;
+  return (uint32_t)x;
+}
+
+struct ElevConst {
+  float scale[kMaxD];
+  float invdp1, dp1;
+};
+
+// ---- Permutohedral::init, per pixel (src/permutohedral.cpp:203-288) --------
+// Pixels at or past N (the padding of the last block of 4) take a zero
+// feature vector and still produce keys, like the reference's SSE lanes.
+template <int D>
+__global__ void k_elevate(const float* __restrict__ feat, int N, int Np, ElevConst c, PKey<D>* __restrict__ ekeys,
+                          int* __restrict__ rank_out, float* __restrict__ bary_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Np) return;
+  constexpr int D1 = D + 1;
+  float f[D], elev[D1], rem0[D1], rk[D1], bc[D + 2];
+#pragma unroll
+  for (int j = 0; j < D; ++j) f[j] = i < N ? feat[(size_t)i * D + j] : 0.0f;
+  float sm = 0.f;
+#pragma unroll
+  for (int j = D; j > 0; --j) {
+    const float cf = f[j - 1] * c.scale[j - 1];
+    const float jc = (float)j * cf;
+    elev[j] = sm - jc;
+    sm = sm + cf;
+  }
+  elev[0] = sm;
+  float sum = 0.f;
+#pragma unroll
+  for (int j = 0; j < D1; ++j) {
+    float v = c.invdp1 * elev[j];
+    v = __builtin_rintf(v);  // round half to even, as _mm_cvtps_epi32
+    rem0[j] = v * c.dp1;
+    sum = sum + v;
+  }
+#pragma unroll
+  for (int j = 0; j < D1; ++j) rk[j] = 0.f;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    const float da = elev[a] - rem0[a];
+#pragma unroll
+    for (int b = a + 1; b < D1; ++b) {
+      const float db = elev[b] - rem0[b];
+      const float cc = (da < db) ? 1.f : 0.f;
+      rk[a] = rk[a] + cc;
+      rk[b] = rk[b] + (1.f - cc);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < D1; ++j) {
+    rk[j] = rk[j] + sum;
+    const float add = rk[j] < 0.f ? c.dp1 : 0.f;
+    const float sub = rk[j] >= c.dp1 ? c.dp1 : 0.f;
+    const float as = add - sub;
+    rk[j] = rk[j] + as;
+    rem0[j] = rem0[j] + as;
+  }
+#pragma unroll
+  for (int j = 0; j < D + 2; ++j) bc[j] = 0.f;
+#pragma unroll
+  for (int j = 0; j < D1; ++j) {
+    const float v = (elev[j] - rem0[j]) * c.invdp1;
+    const int p = (int)((float)D - rk[j]);
+    // p is a permutation of 0..D, so each slot sees at most one + and one -
+#pragma unroll
+    for (int q = 0; q < D1; ++q) {
+      if (q == p) bc[q] = bc[q] + v;
+      if (q == p + 1) bc[q] = bc[q] - v;
+    }
+    if (p + 1 == D1) bc[D1] = bc[D1] - v;
+  }
+  bc[0] = bc[0] + (1.f + bc[D + 1]);
+  int rki[D1];
+#pragma unroll
+  for (int j = 0; j < D1; ++j) rki[j] = (int)rk[j];
+#pragma unroll
+  for (int r = 0; r < D1; ++r) {
+    int16_t key[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const int cn = (rki[j] <= D - r) ? r : r - D1;  // canonical simplex (:183-188)
+      key[j] = (int16_t)(int)(rem0[j] + (float)cn);
+    }
+    const size_t e = (size_t)i * D1 + r;
+    ekeys[e] = pack<D>(key);
+    rank_out[e] = rki[r];
+    bary_out[e] = bc[r];
+  }
+}
+
+// Open-addressing insert with CAS.  Every table slot ends up holding the
+// smallest entry index carrying its key (the key's first occurrence).
+template <int D>
+__global__ void k_insert(const PKey<D>* __restrict__ ekeys, int E, int* table, uint32_t mask,
+                         int* __restrict__ slot_of, int* __restrict__ err) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const PKey<D> k = ekeys[e];
+  uint32_t h = key_hash<D>(k) & mask;
+  for (uint32_t probes = 0; probes <= mask; ++probes) {
+    int cur = __hip_atomic_load(&table[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cur < 0) {
+      const int prev = atomicCAS(&table[h], -1, e);
+      if (prev < 0) {
+        slot_of[e] = (int)h;
+        return;
+      }
+      cur = prev;
+    }
+    if (key_eq<D>(ekeys[cur], k)) {
+      atomicMin(&table[h], e);
+      slot_of[e] = (int)h;
+      return;
+    }
+    h = (h + 1) & mask;
+  }
+  atomicExch(err, 1);  // table full: cannot happen with capacity >= 2E
+}
+
+__global__ void k_mark_first(const int* __restrict__ table, const int* __restrict__ slot_of, int E,
+                             int* __restrict__ flag) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  flag[e] = table[slot_of[e]] == e ? 1 : 0;
+}
+
+template <int D>
+__global__ void k_assign(const int* __restrict__ table, const int* __restrict__ slot_of, const int* __restrict__ pos,
+                         const int* __restrict__ flag, const PKey<D>* __restrict__ ekeys, int E,
+                         int* __restrict__ offset, PKey<D>* __restrict__ ukeys) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  offset[e] = pos[table[slot_of[e]]];
+  if (flag[e]) ukeys[pos[e]] = ekeys[e];
+}
+
+template <int D>
+__device__ inline int lookup(const PKey<D>& k, const int* __restrict__ table, uint32_t mask,
+                             const PKey<D>* __restrict__ ekeys, const int* __restrict__ pos) {
+  uint32_t h = key_hash<D>(k) & mask;
+  for (uint32_t probes = 0; probes <= mask; ++probes) {
+    const int cur = table[h];
+    if (cur < 0) return -1;
+    if (key_eq<D>(ekeys[cur], k)) return pos[cur];
+    h = (h + 1) & mask;
+  }
+  return -1;
+}
+
+// neighbours along each of the d+1 lattice axes (src/permutohedral.cpp:317-330)
+template <int D>
+__global__ void k_neighbors(const PKey<D>* __restrict__ ukeys, int M, const int* __restrict__ table, uint32_t mask,
+                            const PKey<D>* __restrict__ ekeys, const int* __restrict__ pos, int* __restrict__ nbr,
+                            int16_t* __restrict__ keys16) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)M * (D + 1)) return;
+  const int j = (int)(t / M), i = (int)(t % M);
+  int16_t key[D], n1[D], n2[D];
+  unpack<D>(ukeys[i], key);
+#pragma unroll
+  for (int q = 0; q < D; ++q) {
+    n1[q] = (int16_t)(key[q] - 1);
+    n2[q] = (int16_t)(key[q] + 1);
+  }
+  if (j < D) {
+#pragma unroll
+    for (int q = 0; q < D; ++q)
+      if (q == j) {
+        n1[q] = (int16_t)(key[q] + D);
+        n2[q] = (int16_t)(key[q] - D);
+      }
+  }
+  nbr[t * 2 + 0] = lookup<D>(pack<D>(n1), table, mask, ekeys, pos);
+  nbr[t * 2 + 1] = lookup<D>(pack<D>(n2), table, mask, ekeys, pos);
+  if (j == 0)
+#pragma unroll
+    for (int q = 0; q < D; ++q) keys16[(size_t)i * D + q] = key[q];
+}
+
+// ---- splat gather lists -------------------------------------------------------
+__global__ void k_count(const int* __restrict__ offset, int E, int* __restrict__ cnt) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < E) atomicAdd(&cnt[offset[e]], 1);
+}
+__global__ void k_iota(int* __restrict__ v, int n) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) v[e] = e;
+}
+__global__ void k_fill_sp(const int* __restrict__ ent, const float* __restrict__ bary, int E, int d1,
+                          int* __restrict__ sp_pix, float* __restrict__ sp_w) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= E) return;
+  const int e = ent[t];
+  sp_pix[t] = e / d1;
+  sp_w[t] = bary[e];
+}
+// chunks per point (at least one so that every row of the value buffer is
+// written, also for points that only padding lanes produced)
+__global__ void k_nchunks(const int* __restrict__ cnt, int M, int* __restrict__ nch, int* __restrict__ multi_flag,
+                          int* __restrict__ pn) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= M) return;
+  int n = (cnt[p] + kChunk - 1) / kChunk;
+  if (n < 1) n = 1;
+  nch[p] = n;
+  multi_flag[p] = n > 1 ? 1 : 0;
+  pn[p] = n > 1 ? n : 0;  // partial-sum slots are only needed for multi-chunk points
+}
+__global__ void k_chpoint(const int* __restrict__ chstart, const int* __restrict__ mpos,
+                          const int* __restrict__ mflag, int M, int* __restrict__ chpoint, int* __restrict__ multi) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= M) return;
+  for (int c = chstart[p]; c < chstart[p + 1]; ++c) chpoint[c] = p;
+  if (mflag[p]) multi[mpos[p]] = p;
+}
+
+// ---- filter kernels ---------------------------------------------------------
+// Threads are mapped so that a wave covers floor(64/V) rows of V values
+// (V<=64) or one row with a stride-64 loop (V>64).
+struct RowMap {
+  int gpw;  // rows per wave
+  __device__ inline void at(int V, long& row, int& k0, int& kstep) const {
+    const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long wave = gt >> 6;
+    const int lane = (int)(gt & 63);
+    if (V <= 64) {
+      const int g = lane / V;
+      row = g < gpw ? wave * gpw + g : -1;
+      k0 = lane - g * V;
+      kstep = V;
+    } else {
+      row = wave;
+      k0 = lane;
+      kstep = 64;
+    }
+  }
+};
+inline RowMap rowmap(int V) { return RowMap{V <= 64 ? 64 / V : 1}; }
+inline unsigned rows_grid(long rows, int V) {
+  const long gpw = V <= 64 ? 64 / V : 1;
+  const long waves = (rows + gpw - 1) / gpw;
+  return grid_for((size_t)waves * 64);
+}
+
+template <SplatIn KIND>
+__device__ inline float splat_in(const SplatArgs& a, int pix, int k, int V) {
+  if (KIND == SplatIn::kPlain) return a.in[(size_t)pix * V + k];
+  if (KIND == SplatIn::kScaled) return a.in[(size_t)pix * V + k] * a.norm[pix];
+  if (KIND == SplatIn::kSxScaled) {
+    const float s = (a.lab[pix] == k ? 1.f : 0.f) - a.in[(size_t)pix * V + k];
+    return s * a.norm[pix];
+  }
+  if (KIND == SplatIn::kSx) return (a.lab[pix] == k ? 1.f : 0.f) - a.in[(size_t)pix * V + k];
+  return a.lab[pix] == k ? 0.f : 1.f;  // kNotOnehot
+}
+
+// values[p] = sum over the point's entries, in pixel order, of w * in(pixel)
+template <SplatIn KIND>
+__global__ void k_splat(SplatArgs a, int V, RowMap rm, const int* __restrict__ sp_pix, const float* __restrict__ sp_w,
+                        const int* __restrict__ seg, const int* __restrict__ chstart, const int* __restrict__ chpoint,
+                        const int* __restrict__ pstart, int nchunks, float* __restrict__ values,
+                        float* __restrict__ part) {
+  long c;
+  int k0, ks;
+  rm.at(V, c, k0, ks);
+  if (c < 0 || c >= nchunks) return;
+  const int p = chpoint[c];
+  const int c0 = chstart[p], nc = chstart[p + 1] - c0;
+  const int b = seg[p] + (int)(c - c0) * kChunk;
+  const int e = min(seg[p + 1], b + kChunk);
+  float* dst = nc == 1 ? values + (size_t)(p + 1) * V : part + (size_t)(pstart[p] + (c - c0)) * V;
+  for (int k = k0; k < V; k += ks) {
+    float acc = 0.f;
+    int t = b;
+    for (; t + 4 <= e; t += 4) {
+      const int p0 = sp_pix[t], p1 = sp_pix[t + 1], p2 = sp_pix[t + 2], p3 = sp_pix[t + 3];
+      const float w0 = sp_w[t], w1 = sp_w[t + 1], w2 = sp_w[t + 2], w3 = sp_w[t + 3];
+      const float x0 = splat_in<KIND>(a, p0, k, V), x1 = splat_in<KIND>(a, p1, k, V);
+      const float x2 = splat_in<KIND>(a, p2, k, V), x3 = splat_in<KIND>(a, p3, k, V);
+      acc = acc + w0 * x0;
+      acc = acc + w1 * x1;
+      acc = acc + w2 * x2;
+      acc = acc + w3 * x3;
+    }
+    for (; t < e; ++t) acc = acc + sp_w[t] * splat_in<KIND>(a, sp_pix[t], k, V);
+    dst[k] = acc;
+  }
+}
+
+// points whose entries span several chunks: sum the chunk partials in order
+__global__ void k_combine(int V, RowMap rm, const int* __restrict__ multi, int nmulti, const int* __restrict__ chstart,
+                          const int* __restrict__ pstart, const float* __restrict__ part, float* __restrict__ values) {
+  long m;
+  int k0, ks;
+  rm.at(V, m, k0, ks);
+  if (m < 0 || m >= nmulti) return;
+  const int p = multi[m];
+  const int c0 = pstart[p], c1 = c0 + chstart[p + 1] - chstart[p];
+  for (int k = k0; k < V; k += ks) {
+    float acc = part[(size_t)c0 * V + k];
+    for (int c = c0 + 1; c < c1; ++c) acc = acc + part[(size_t)c * V + k];
+    values[(size_t)(p + 1) * V + k] = acc;
+  }
+}
+
+// one blur axis: new = old + 0.5 * (n1 + n2); SEQ evaluates the sum in double
+// and rounds once (seqCompute, :518), otherwise float (sseCompute, :1035)
+template <bool SEQ>
+__global__ void k_blur(int V, RowMap rm, int M, const int* __restrict__ nbr, const float* __restrict__ src,
+                       float* __restrict__ dst) {
+  long i;
+  int k0, ks;
+  rm.at(V, i, k0, ks);
+  if (i < 0 || i >= M) return;
+  const int n1 = nbr[2 * i] + 1, n2 = nbr[2 * i + 1] + 1;
+  for (int k = k0; k < V; k += ks) {
+    const float o = src[(size_t)(i + 1) * V + k];
+    const float s = src[(size_t)n1 * V + k] + src[(size_t)n2 * V + k];
+    dst[(size_t)(i + 1) * V + k] = SEQ ? (float)((double)o + 0.5 * (double)s) : o + 0.5f * s;
+  }
+}
+
+// float4 form for wide rows (ordered filter state), same arithmetic
+__global__ void k_blur4(int V4, int M, const int* __restrict__ nbr, const float4* __restrict__ src,
+                        float4* __restrict__ dst) {
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= M) return;
+  const long i = wave;
+  const int n1 = nbr[2 * i] + 1, n2 = nbr[2 * i + 1] + 1;
+  for (int k = lane; k < V4; k += 64) {
+    const float4 o = src[(size_t)(i + 1) * V4 + k];
+    const float4 a = src[(size_t)n1 * V4 + k];
+    const float4 b = src[(size_t)n2 * V4 + k];
+    float4 r;
+    r.x = o.x + 0.5f * (a.x + b.x);
+    r.y = o.y + 0.5f * (a.y + b.y);
+    r.z = o.z + 0.5f * (a.z + b.z);
+    r.w = o.w + 0.5f * (a.w + b.w);
+    dst[(size_t)(i + 1) * V4 + k] = r;
+  }
+}
+
+__device__ inline void slice_store(const SliceArgs& o, int pix, int k, int V, float x) {
+  if (o.norm) x = x * o.norm[pix];
+  if (o.apply_scale) x = o.scale * x;
+  if (o.post_mult != 1.f) x = o.post_mult * x;
+  float* dst = o.out + (size_t)pix * V + k;
+  *dst = o.accumulate ? *dst + x : x;
+}
+
+// barycentric gather: SEQ uses (w*v)*alpha (:533), otherwise (w*alpha)*v (:1048)
+template <bool SEQ>
+__global__ void k_slice(int V, RowMap rm, int N, int d1, const int* __restrict__ offset, const float* __restrict__ bary,
+                        const float* __restrict__ values, float alpha, SliceArgs o) {
+  long pix;
+  int k0, ks;
+  rm.at(V, pix, k0, ks);
+  if (pix < 0 || pix >= N) return;
+  for (int k = k0; k < V; k += ks) {
+    float acc = 0.f;
+    for (int j = 0; j < d1; ++j) {
+      const int off = offset[(size_t)pix * d1 + j] + 1;
+      const float w = bary[(size_t)pix * d1 + j];
+      const float v = values[(size_t)off * V + k];
+      if (SEQ)
+        acc = acc + (w * v) * alpha;
+      else
+        acc = acc + (w * alpha) * v;
+    }
+    slice_store(o, (int)pix, k, V, acc);
+  }
+}
+
+// ---- ordered filter ------------------------------------------------------------
+// bins of src/permutohedral.cpp:548-589
+__device__ inline int bin_lo(float q) { return min((int)floorf(q * 10.f), 9); }
+__device__ inline int bin_up(float q) { return max((int)floor(((double)q - 1e-9) * 10.0), 0); }
+__device__ inline float rescaled(const float* Q, size_t idx, const float* mm) {
+  float q = Q[idx] - mm[0];
+  if (mm[1] > 0.f) q = q / mm[1];
+  return q;
+}
+
+// per (chunk, label): 10 lower bins (+w to bins 0..bin_lo) and 10 upper bins
+// (+w to bins bin_up..9), accumulated in pixel order
+__global__ void k_osplat(const float* __restrict__ Q, const float* __restrict__ mm, int L, RowMap rm,
+                         const int* __restrict__ sp_pix, const float* __restrict__ sp_w, const int* __restrict__ seg,
+                         const int* __restrict__ chstart, const int* __restrict__ chpoint,
+                         const int* __restrict__ pstart, int nchunks, float* __restrict__ values,
+                         float* __restrict__ part) {
+  long c;
+  int k0, ks;
+  rm.at(L, c, k0, ks);
+  if (c < 0 || c >= nchunks) return;
+  const int p = chpoint[c];
+  const int c0 = chstart[p], nc = chstart[p + 1] - c0;
+  const int b = seg[p] + (int)(c - c0) * kChunk;
+  const int e = min(seg[p + 1], b + kChunk);
+  float* dst = nc == 1 ? values + (size_t)(p + 1) * L * 20 : part + (size_t)(pstart[p] + (c - c0)) * L * 20;
+  for (int k = k0; k < L; k += ks) {
+    float acc[20];
+#pragma unroll
+    for (int q = 0; q < 20; ++q) acc[q] = 0.f;
+    for (int t = b; t < e; ++t) {
+      const int pix = sp_pix[t];
+      const float w = sp_w[t];
+      const float q = rescaled(Q, (size_t)pix * L + k, mm);
+      const int lo = bin_lo(q), up = bin_up(q);
+#pragma unroll
+      for (int r = 0; r < 10; ++r) {
+        acc[r] = acc[r] + (r <= lo ? w : 0.f);
+        acc[10 + r] = acc[10 + r] + (r >= up ? w : 0.f);
+      }
+    }
+    float4* d4 = reinterpret_cast<float4*>(dst + (size_t)k * 20);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) d4[q] = make_float4(acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]);
+  }
+}
+
+// lower slice then upper slice (:638-650, :686-696), then out (+)= scale*x
+__global__ void k_oslice(const float* __restrict__ Q, const float* __restrict__ mm, int L, RowMap rm, int N, int d1,
+                         const int* __restrict__ offset, const float* __restrict__ bary,
+                         const float* __restrict__ values, float alpha, SliceArgs o) {
+  long pix;
+  int k0, ks;
+  rm.at(L, pix, k0, ks);
+  if (pix < 0 || pix >= N) return;
+  for (int k = k0; k < L; k += ks) {
+    const float q = rescaled(Q, (size_t)pix * L + k, mm);
+    const int lo = bin_lo(q), up = bin_up(q);
+    float acc = 0.f;
+    for (int j = 0; j < d1; ++j) {
+      const int off = offset[(size_t)pix * d1 + j] + 1;
+      const float w = bary[(size_t)pix * d1 + j];
+      acc = acc + values[((size_t)off * L + k) * 20 + lo] * (-w * alpha);
+    }
+    for (int j = 0; j < d1; ++j) {
+      const int off = offset[(size_t)pix * d1 + j] + 1;
+      const float w = bary[(size_t)pix * d1 + j];
+      acc = acc + values[((size_t)off * L + k) * 20 + 10 + up] * (w * alpha);
+    }
+    slice_store(o, (int)pix, k, L, acc);
+  }
+}
+
+__global__ void k_zero(float* p, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0.f;
+}
+
+float lattice_alpha(int d) { return 1.0f / (1 + powf(2.0f, (float)-d)); }
+
+// ---- host-side build, templated on the feature dimension ---------------------
+struct BuildOut {
+  DBuf<int>* offset;
+  DBuf<int>* rank;
+  DBuf<float>* bary;
+  DBuf<int16_t>* keys;
+  DBuf<int>* nbr;
+  int M;
+};
+
+void scan_excl(const int* in, int* out, int n, hipStream_t s) {
+  size_t tmp = 0;
+  DCRF_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, n, s));
+  DBuf<char> t(tmp + 1);
+  DCRF_HIP(hipcub::DeviceScan::ExclusiveSum(t.get(), tmp, in, out, n, s));
+  DCRF_HIP(hipStreamSynchronize(s));  // t is released on return
+}
+
+template <int D>
+int build_tables(const float* feat, int N, int Np, hipStream_t s, DBuf<int>& offset, DBuf<int>& rank,
+                 DBuf<float>& bary, DBuf<int16_t>& keys, DBuf<int>& nbr) {
+  constexpr int D1 = D + 1;
+  const int E = Np * D1;
+  ElevConst c{};
+  const float inv_std = (float)(std::sqrt(2.0 / 3.0) * D1);  // :191
+  for (int i = 0; i < D; ++i) c.scale[i] = (float)(1.0 / std::sqrt((double)((i + 2) * (i + 1))) * (double)inv_std);
+  c.invdp1 = 1.0f / D1;
+  c.dp1 = (float)D1;
+
+  DBuf<PKey<D>> ekeys(E);
+  offset.resize(E);
+  rank.resize(E);
+  bary.resize(E);
+  k_elevate<D><<<grid_for(Np), kBlock, 0, s>>>(feat, N, Np, c, ekeys.get(), rank.get(), bary.get());
+  check_launch("k_elevate");
+
+  uint32_t cap = 1024;
+  while (cap < 2u * (uint32_t)E) cap <<= 1;
+  DBuf<int> table(cap), slot_of(E), flag(E), pos(E), err(1);
+  DCRF_HIP(hipMemsetAsync(table.get(), 0xff, (size_t)cap * sizeof(int), s));
+  DCRF_HIP(hipMemsetAsync(err.get(), 0, sizeof(int), s));
+  k_insert<D><<<grid_for(E), kBlock, 0, s>>>(ekeys.get(), E, table.get(), cap - 1, slot_of.get(), err.get());
+  check_launch("k_insert");
+  k_mark_first<<<grid_for(E), kBlock, 0, s>>>(table.get(), slot_of.get(), E, flag.get());
+  check_launch("k_mark_first");
+  scan_excl(flag.get(), pos.get(), E, s);
+  int last_pos = 0, last_flag = 0, herr = 0;
+  DCRF_HIP(hipMemcpyAsync(&last_pos, pos.get() + E - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+  DCRF_HIP(hipMemcpyAsync(&last_flag, flag.get() + E - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+  DCRF_HIP(hipMemcpyAsync(&herr, err.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+  DCRF_HIP(hipStreamSynchronize(s));
+  if (herr) throw Error(kInternal, "lattice hash table overflow");
+  const int M = last_pos + last_flag;
+
+  DBuf<PKey<D>> ukeys(M);
+  k_assign<D><<<grid_for(E), kBlock, 0, s>>>(table.get(), slot_of.get(), pos.get(), flag.get(), ekeys.get(), E,
+                                             offset.get(), ukeys.get());
+  check_launch("k_assign");
+  keys.resize((size_t)M * D);
+  nbr.resize((size_t)D1 * M * 2);
+  k_neighbors<D><<<grid_for((size_t)M * D1), kBlock, 0, s>>>(ukeys.get(), M, table.get(), cap - 1, ekeys.get(),
+                                                              pos.get(), nbr.get(), keys.get());
+  check_launch("k_neighbors");
+  DCRF_HIP(hipStreamSynchronize(s));
+  return M;
+}
+
+using BuildFn = int (*)(const float*, int, int, hipStream_t, DBuf<int>&, DBuf<int>&, DBuf<float>&, DBuf<int16_t>&,
+                        DBuf<int>&);
+BuildFn build_fn(int d) {
+  switch (d) {
+    case 1: return build_tables<1>;
+    case 2: return build_tables<2>;
+    case 3: return build_tables<3>;
+    case 4: return build_tables<4>;
+    case 5: return build_tables<5>;
+    case 6: return build_tables<6>;
+    case 7: return build_tables<7>;
+    case 8: return build_tables<8>;
+    case 9: return build_tables<9>;
+    case 10: return build_tables<10>;
+    case 11: return build_tables<11>;
+    case 12: return build_tables<12>;
+    default: return nullptr;
+  }
+}
+
+}  // namespace
+
+Lattice::Lattice(hipStream_t s) : stream_(s) {}
+Lattice::~Lattice() = default;
+
+void Lattice::build(const float* d_feat, int N, int d) {
+  DCRF_REQUIRE(N > 0, "lattice: N must be positive");
+  BuildFn fn = build_fn(d);
+  DCRF_REQUIRE(fn != nullptr, "lattice: feature dimension must be in [1, 12]");
+  N_ = N;
+  d_ = d;
+  Np_ = (N + 3) / 4 * 4;
+  const int d1 = d + 1;
+  hipStream_t s = stream_;
+  M_ = fn(d_feat, N, Np_, s, offset_, rank_, bary_, keys_, nbr_);
+
+  // splat gather lists: entries of real pixels sorted by (point, pixel)
+  const int E = N * d1, M = M_;
+  DBuf<int> cnt(M + 1), iota(E), ids_sorted(E), ent_sorted(E);
+  DCRF_HIP(hipMemsetAsync(cnt.get(), 0, (M + 1) * sizeof(int), s));
+  k_count<<<grid_for(E), kBlock, 0, s>>>(offset_.get(), E, cnt.get());
+  check_launch("k_count");
+  seg_.resize(M + 1);
+  scan_excl(cnt.get(), seg_.get(), M + 1, s);
+  k_iota<<<grid_for(E), kBlock, 0, s>>>(iota.get(), E);
+  check_launch("k_iota");
+  int bits = 1;
+  while ((1 << bits) <= M) ++bits;
+  {
+    size_t tmp = 0;
+    DCRF_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, offset_.get(), ids_sorted.get(), iota.get(),
+                                                ent_sorted.get(), E, 0, bits, s));
+    DBuf<char> t(tmp + 1);
+    DCRF_HIP(hipcub::DeviceRadixSort::SortPairs(t.get(), tmp, offset_.get(), ids_sorted.get(), iota.get(),
+                                                ent_sorted.get(), E, 0, bits, s));
+    DCRF_HIP(hipStreamSynchronize(s));
+  }
+  sp_pix_.resize(E);
+  sp_w_.resize(E);
+  k_fill_sp<<<grid_for(E), kBlock, 0, s>>>(ent_sorted.get(), bary_.get(), E, d1, sp_pix_.get(), sp_w_.get());
+  check_launch("k_fill_sp");
+
+  DBuf<int> nch(M + 1), mflag(M + 1), mpos(M + 1), pn(M + 1);
+  DCRF_HIP(hipMemsetAsync(nch.get(), 0, (M + 1) * sizeof(int), s));
+  DCRF_HIP(hipMemsetAsync(mflag.get(), 0, (M + 1) * sizeof(int), s));
+  DCRF_HIP(hipMemsetAsync(pn.get(), 0, (M + 1) * sizeof(int), s));
+  k_nchunks<<<grid_for(M), kBlock, 0, s>>>(cnt.get(), M, nch.get(), mflag.get(), pn.get());
+  check_launch("k_nchunks");
+  chstart_.resize(M + 1);
+  pstart_.resize(M + 1);
+  scan_excl(nch.get(), chstart_.get(), M + 1, s);
+  scan_excl(mflag.get(), mpos.get(), M + 1, s);
+  scan_excl(pn.get(), pstart_.get(), M + 1, s);
+  DCRF_HIP(hipMemcpyAsync(&nchunks_, chstart_.get() + M, sizeof(int), hipMemcpyDeviceToHost, s));
+  DCRF_HIP(hipMemcpyAsync(&nmulti_, mpos.get() + M, sizeof(int), hipMemcpyDeviceToHost, s));
+  DCRF_HIP(hipMemcpyAsync(&npart_, pstart_.get() + M, sizeof(int), hipMemcpyDeviceToHost, s));
+  DCRF_HIP(hipStreamSynchronize(s));
+  chpoint_.resize(nchunks_);
+  multi_.resize(nmulti_ > 0 ? nmulti_ : 1);
+  k_chpoint<<<grid_for(M), kBlock, 0, s>>>(chstart_.get(), mpos.get(), mflag.get(), M, chpoint_.get(), multi_.get());
+  check_launch("k_chpoint");
+  DCRF_HIP(hipStreamSynchronize(s));
+}
+
+void Lattice::ensure_values(size_t floats_per_row) {
+  const size_t need = (size_t)(M_ + 1) * floats_per_row;
+  if (need > vcap_) {
+    vA_.resize(need);
+    vB_.resize(need);
+    vcap_ = need;
+  }
+  // row 0 stands for absent neighbours and must read as zero; rows 1..M are
+  // fully rewritten by every splat.  A kernel (not a memset) keeps this
+  // capturable in a HIP graph.
+  k_zero<<<grid_for(floats_per_row), kBlock, 0, stream_>>>(vA_.get(), floats_per_row);
+  k_zero<<<grid_for(floats_per_row), kBlock, 0, stream_>>>(vB_.get(), floats_per_row);
+  check_launch("k_zero");
+  if (nmulti_ > 0) part_.resize((size_t)npart_ * floats_per_row);
+}
+
+void Lattice::filter(const SplatArgs& in, const SliceArgs& out, int V, bool reverse) {
+  DCRF_REQUIRE(V > 0, "filter: value size must be positive");
+  ensure_values(V);
+  hipStream_t s = stream_;
+  const RowMap rm = rowmap(V);
+  float* src = vA_.get();
+  float* dst = vB_.get();
+  const std::string tag = prof_on() ? "[d=" + std::to_string(d_) + ",V=" + std::to_string(V) + "]" : std::string();
+  {
+  KTimer tm("splat" + tag, s);
+  switch (in.kind) {
+#define DCRF_SPLAT(K)                                                                                             \
+  case K:                                                                                                         \
+    k_splat<K><<<rows_grid(nchunks_, V), kBlock, 0, s>>>(in, V, rm, sp_pix_.get(), sp_w_.get(), seg_.get(),     \
+                                                         chstart_.get(), chpoint_.get(), pstart_.get(), nchunks_, \
+                                                         src, part_.get());                                     \
+    break;
+    DCRF_SPLAT(SplatIn::kPlain)
+    DCRF_SPLAT(SplatIn::kScaled)
+    DCRF_SPLAT(SplatIn::kSxScaled)
+    DCRF_SPLAT(SplatIn::kNotOnehot)
+    DCRF_SPLAT(SplatIn::kSx)
+#undef DCRF_SPLAT
+  }
+  check_launch("k_splat");
+  if (nmulti_ > 0) {
+    k_combine<<<rows_grid(nmulti_, V), kBlock, 0, s>>>(V, rm, multi_.get(), nmulti_, chstart_.get(), pstart_.get(),
+                                                       part_.get(), src);
+    check_launch("k_combine");
+  }
+  }
+  const bool seq = V <= 2;
+  for (int jj = 0; jj <= d_; ++jj) {
+    const int j = reverse ? d_ - jj : jj;
+    const int* nb = nbr_.get() + (size_t)j * M_ * 2;
+    KTimer tm("blur" + tag, s);
+    if (seq)
+      k_blur<true><<<rows_grid(M_, V), kBlock, 0, s>>>(V, rm, M_, nb, src, dst);
+    else
+      k_blur<false><<<rows_grid(M_, V), kBlock, 0, s>>>(V, rm, M_, nb, src, dst);
+    check_launch("k_blur");
+    std::swap(src, dst);
+  }
+  const float alpha = lattice_alpha(d_);
+  KTimer tm("slice" + tag, s);
+  if (seq)
+    k_slice<true><<<rows_grid(N_, V), kBlock, 0, s>>>(V, rm, N_, d_ + 1, offset_.get(), bary_.get(), src, alpha, out);
+  else
+    k_slice<false><<<rows_grid(N_, V), kBlock, 0, s>>>(V, rm, N_, d_ + 1, offset_.get(), bary_.get(), src, alpha, out);
+  check_launch("k_slice");
+}
+
+void Lattice::ordered(const float* Q, const float* d_minmax, int L, const SliceArgs& out) {
+  DCRF_REQUIRE(L > 0, "ordered filter: label count must be positive");
+  const int row = L * 20;
+  ensure_values(row);
+  hipStream_t s = stream_;
+  const RowMap rm = rowmap(L);
+  float* src = vA_.get();
+  float* dst = vB_.get();
+  const std::string tag = prof_on() ? "[d=" + std::to_string(d_) + ",L=" + std::to_string(L) + "]" : std::string();
+  {
+  KTimer tm("osplat" + tag, s);
+  k_osplat<<<rows_grid(nchunks_, L), kBlock, 0, s>>>(Q, d_minmax, L, rm, sp_pix_.get(), sp_w_.get(), seg_.get(),
+                                                     chstart_.get(), chpoint_.get(), pstart_.get(), nchunks_, src,
+                                                     part_.get());
+  check_launch("k_osplat");
+  if (nmulti_ > 0) {
+    const RowMap rw = rowmap(row);
+    k_combine<<<rows_grid(nmulti_, row), kBlock, 0, s>>>(row, rw, multi_.get(), nmulti_, chstart_.get(),
+                                                         pstart_.get(), part_.get(), src);
+    check_launch("k_combine");
+  }
+  }
+  for (int j = 0; j <= d_; ++j) {  // forward order only (:621, :669)
+    KTimer tm("oblur" + tag, s);
+    k_blur4<<<grid_for((size_t)M_ * 64), kBlock, 0, s>>>(row / 4, M_, nbr_.get() + (size_t)j * M_ * 2,
+                                                         reinterpret_cast<const float4*>(src),
+                                                         reinterpret_cast<float4*>(dst));
+    check_launch("k_blur4");
+    std::swap(src, dst);
+  }
+  KTimer tm("oslice" + tag, s);
+  k_oslice<<<rows_grid(N_, L), kBlock, 0, s>>>(Q, d_minmax, L, rm, N_, d_ + 1, offset_.get(), bary_.get(), src,
+                                               lattice_alpha(d_), out);
+  check_launch("k_oslice");
+}
+
+void Lattice::export_tables(int* offset, int* rank, float* bary, short* keys, int* nbr) const {
+  const size_t E = (size_t)Np_ * (d_ + 1);
+  if (offset) DCRF_HIP(hipMemcpyAsync(offset, offset_.get(), E * sizeof(int), hipMemcpyDeviceToHost, stream_));
+  if (rank) DCRF_HIP(hipMemcpyAsync(rank, rank_.get(), E * sizeof(int), hipMemcpyDeviceToHost, stream_));
+  if (bary) DCRF_HIP(hipMemcpyAsync(bary, bary_.get(), E * sizeof(float), hipMemcpyDeviceToHost, stream_));
+  if (keys)
+    DCRF_HIP(hipMemcpyAsync(keys, keys_.get(), (size_t)M_ * d_ * sizeof(int16_t), hipMemcpyDeviceToHost, stream_));
+  if (nbr)
+    DCRF_HIP(hipMemcpyAsync(nbr, nbr_.get(), (size_t)(d_ + 1) * M_ * 2 * sizeof(int), hipMemcpyDeviceToHost, stream_));
+  DCRF_HIP(hipStreamSynchronize(stream_));
+}
+
+}  // namespace dcrf

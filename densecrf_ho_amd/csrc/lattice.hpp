@@ -1,0 +1,96 @@
+// Permutohedral lattice on the device: build, plain Gaussian filter and the
+// ordered ("upper minus lower") filter.
+//
+// Reference: Permutohedral (include/permutohedral.h:50-88,
+// src/permutohedral.cpp).  The lattice reproduces the reference's point ids
+// exactly: ids are assigned in order of first occurrence of a key in the
+// entry sequence pixel*(d+1)+remainder, which is the insertion order of the
+// reference's HashTable (src/permutohedral.cpp:100-128).
+#pragma once
+
+#include "common.hpp"
+
+namespace dcrf {
+
+// Input of a splat: value `k` of pixel `pix`.  Concrete forms are defined in
+// lattice.hip; the enum selects the one used by a launch.
+enum class SplatIn : int {
+  kPlain = 0,      // in[pix*V+k]
+  kScaled = 1,     // in[pix*V+k] * norm[pix]
+  kSxScaled = 2,   // ((k==s[pix]) - Q[pix*V+k]) * norm[pix]   (QP direction)
+  kNotOnehot = 3,  // (lab[pix]==k ? 0 : 1)                     (pairwise_energy_true)
+  kSx = 4,         // ((k==s[pix]) - Q[pix*V+k])                 (no normalisation)
+};
+
+struct SplatArgs {
+  SplatIn kind = SplatIn::kPlain;
+  const float* in = nullptr;    // kPlain/kScaled/kSx*: N x V field
+  const float* norm = nullptr;  // per-pixel normalisation
+  const int* lab = nullptr;     // kSx*: argmin label; kNotOnehot: labelling
+};
+
+// What the slice does with the filtered value x of (pixel, k).
+struct SliceArgs {
+  float* out = nullptr;
+  const float* norm = nullptr;  // multiply by norm[pix] (post-normalisation) if set
+  float scale = 1.f;            // then multiply by `scale` (Potts: -w) if apply_scale
+  bool apply_scale = false;
+  bool accumulate = false;      // out = out + x instead of out = x
+  float post_mult = 1.f;        // exact multiplier (1 or 2) applied last
+};
+
+class Lattice {
+ public:
+  Lattice(hipStream_t s);
+  ~Lattice();
+  // Permutohedral::init (src/permutohedral.cpp:152-333) from device features
+  // (N x d, feature-contiguous per pixel).
+  void build(const float* d_feat, int N, int d);
+
+  int N() const { return N_; }
+  int d() const { return d_; }
+  int M() const { return M_; }
+  int Npad() const { return Np_; }
+
+  // Permutohedral::compute (src/permutohedral.cpp:1079-1087) with the input
+  // and output transforms of DenseKernel::filter fused in.  V<=2 follows the
+  // reference's seqCompute rounding (:488-540), V>2 its sseCompute (:998-1058).
+  void filter(const SplatArgs& in, const SliceArgs& out, int V, bool reverse);
+
+  // seqCompute_upper_minus_lower_ord (src/permutohedral.cpp:590-701) on the
+  // rescaled field q = (Q - qmin) / qmax (qmax == 0 -> no division), read on
+  // the fly from Q; then out (+)= scale * result.
+  void ordered(const float* Q, const float* d_minmax, int L, const SliceArgs& out);
+
+  // copies of the lattice tables for parity checks (host pointers, may be null)
+  void export_tables(int* offset, int* rank, float* bary, short* keys, int* nbr) const;
+
+  // device tables (read by solver kernels)
+  const int* offset() const { return offset_.get(); }
+  const float* bary() const { return bary_.get(); }
+  hipStream_t stream() const { return stream_; }
+  size_t bytes_values(int V) const { return (size_t)(M_ + 1) * V * sizeof(float); }
+
+ private:
+  void ensure_values(size_t floats);
+  hipStream_t stream_;
+  int N_ = 0, d_ = 0, Np_ = 0, M_ = 0;
+  int nchunks_ = 0, nmulti_ = 0, npart_ = 0;
+  DBuf<int> offset_, rank_;
+  DBuf<float> bary_;
+  DBuf<int16_t> keys_;
+  DBuf<int> nbr_;
+  // splat gather structure: entries sorted by (point id, pixel)
+  DBuf<int> sp_pix_;
+  DBuf<float> sp_w_;
+  DBuf<int> seg_;      // M+1
+  DBuf<int> chstart_;  // M+1: first chunk of each point
+  DBuf<int> chpoint_;  // point of each chunk
+  DBuf<int> multi_;    // points with more than one chunk
+  DBuf<int> pstart_;   // M+1: first partial-sum slot of each multi-chunk point
+  // value buffers (row 0 is the zero row standing for absent neighbours)
+  DBuf<float> vA_, vB_, part_;
+  size_t vcap_ = 0;
+};
+
+}  // namespace dcrf

@@ -1,0 +1,686 @@
+// Proximal LP (DenseCRF::lp_inference_prox_super_pixels, src/densecrf.cpp:526-799)
+// and the LP-side energies on gfx950.
+//
+// Per inner Frank-Wolfe step:
+//   k_lp_Y        alpha,u <- previous step's update (fused), Y = lambda*C(alpha+u-U) + cur_Q
+//   k_lp_sweeps   qp-gamma: B sweeps per pixel in registers, per-sweep partial
+//                 sums of qp_values; k_lp_sweep_decide finds the stopping sweep
+//                 and the replay pass writes gamma after exactly that many sweeps
+//   k_lp_betaQ    beta, prox Q and its global min/max (rescale)
+//   ordered filters -> sa ; computeUCondGrad -> su (superpixels)
+//   k_lp_gap      dual gap and |tmp|^2 in FP64 -> delta
+// Per outer step: k_lp_project (feasible_Q + renormalize + argmax) and the
+// integer energy of the labelling.
+#include <algorithm>
+#include <cmath>
+#include <limits>
+
+#include "crf.hpp"
+#include "crf_kernels.hpp"
+
+namespace dcrf {
+namespace {
+
+// LP scalar slots (doubles) and flags (ints) in LPState::sc / fl
+constexpr int kL_GAP = 0, kL_TT = 1, kL_DELTA = 2, kL_QPV = 3, kL_EINT = 4, kL_EHO = 5, kL_EPAIR = 6, kL_TMP = 8,
+              kL_NSC = 16;
+constexpr int kLF_SWEEP_DONE = 0, kLF_SWEEPS = 1, kLF_ERR = 2, kLF_NFL = 4;
+constexpr int kSweepBatch = 16;
+
+// sum of the first L entries of a register array in Eigen's order for an
+// aligned vector (first aligned element 0): blocks of 8 into two 4-wide
+// accumulators, one more block of 4 if present, (p0+p2)+(p1+p3), then the tail
+template <int MAXL>
+__device__ inline float eigen_sum_reg(const float (&v)[MAXL], int L) {
+  const int as2 = (L / 8) * 8, as1 = (L / 4) * 4;
+  if (as1 == 0) {
+    float r = v[0];
+#pragma unroll
+    for (int i = 1; i < MAXL; ++i)
+      if (i < L) r = r + v[i];
+    return r;
+  }
+  float p0 = v[0], p1 = v[1], p2 = v[2], p3 = v[3];
+  if (as1 > 4) {
+    float q0 = v[4], q1 = v[5], q2 = v[6], q3 = v[7];
+#pragma unroll
+    for (int b = 8; b + 7 < MAXL; b += 8)
+      if (b < as2) {
+        p0 = p0 + v[b];
+        p1 = p1 + v[b + 1];
+        p2 = p2 + v[b + 2];
+        p3 = p3 + v[b + 3];
+        q0 = q0 + v[b + 4];
+        q1 = q1 + v[b + 5];
+        q2 = q2 + v[b + 6];
+        q3 = q3 + v[b + 7];
+      }
+    p0 = p0 + q0;
+    p1 = p1 + q1;
+    p2 = p2 + q2;
+    p3 = p3 + q3;
+    if (as1 > as2) {
+#pragma unroll
+      for (int b = 8; b + 3 < MAXL; b += 8)
+        if (b == as2) {
+          p0 = p0 + v[b];
+          p1 = p1 + v[b + 1];
+          p2 = p2 + v[b + 2];
+          p3 = p3 + v[b + 3];
+        }
+    }
+  }
+  float r = (p0 + p2) + (p1 + p3);
+#pragma unroll
+  for (int i = 4; i < MAXL; ++i)
+    if (i >= as1 && i < L) r = r + v[i];
+  return r;
+}
+template <int MAXL>
+__device__ inline float eigen_dot_reg(const float (&a)[MAXL], const float (&b)[MAXL], int L) {
+  float p[MAXL];
+#pragma unroll
+  for (int i = 0; i < MAXL; ++i) p[i] = a[i] * b[i];
+  return eigen_sum_reg<MAXL>(p, L);
+}
+
+struct LPConst {
+  float lambda, c_m1_over_M, c_2l_over_M, c_1_over_M, c_1m2_over_M, qp_delta;
+};
+
+// alpha/u update of the previous step (:737-738, fused here), then
+// Y = lambda * (tmp - sum(tmp)/M) + cur_Q with tmp = alpha + u - U (:627-635)
+template <int MAXL>
+__global__ void k_lp_Y(int N, int L, LPConst c, float* __restrict__ alpha, float* __restrict__ u,
+                       const float* __restrict__ sa, const float* __restrict__ su, const double* __restrict__ sc,
+                       int apply_update, const float* __restrict__ U, const float* __restrict__ curQ,
+                       float* __restrict__ Y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const size_t o = (size_t)i * L;
+  const float dl = apply_update ? (float)sc[kL_DELTA] : 0.f;
+  float t[MAXL];
+#pragma unroll
+  for (int l = 0; l < MAXL; ++l)
+    if (l < L) {
+      float a = alpha[o + l];
+      float uu = u ? u[o + l] : 0.f;
+      if (apply_update) {
+        a = a + dl * (sa[o + l] - a);
+        alpha[o + l] = a;
+        if (u) {
+          uu = uu + dl * (su[o + l] - uu);
+          u[o + l] = uu;
+        }
+      }
+      t[l] = (a + uu) - U[o + l];
+    } else {
+      t[l] = 0.f;
+    }
+  const float s = eigen_sum_reg<MAXL>(t, L);
+#pragma unroll
+  for (int l = 0; l < MAXL; ++l)
+    if (l < L) Y[o + l] = ((c.c_m1_over_M * s + t[l]) * c.lambda) + curQ[o + l];
+}
+
+// one qp-gamma sweep of a pixel (qp_gamma_step :495-514, qp_gamma_multiplyC
+// :517-523, qp_values :667); returns the pixel's qp value
+template <int MAXL>
+__device__ inline float qp_sweep(float (&g)[MAXL], const float (&y)[MAXL], int L, const LPConst& c) {
+  const float sum = eigen_sum_reg<MAXL>(g, L);
+#pragma unroll
+  for (int l = 0; l < MAXL; ++l)
+    if (l < L) {
+      const float ph = fmaxf(-y[l], 0.f), nh = fmaxf(y[l], 0.f);
+      const float t = (sum - g[l]) * c.c_2l_over_M;
+      const float stp = (t + ph) + c.qp_delta;
+      float t2 = (c.c_1_over_M * sum + c.c_1m2_over_M * g[l]) * c.lambda;
+      t2 = (t2 + nh) + c.qp_delta;
+      g[l] = g[l] * (stp / t2);
+    }
+  const float s2 = eigen_sum_reg<MAXL>(g, L);
+  float vt[MAXL];
+#pragma unroll
+  for (int l = 0; l < MAXL; ++l) vt[l] = l < L ? (c.c_m1_over_M * s2 + g[l]) * c.lambda : 0.f;
+  return (float)(0.5 * (double)eigen_dot_reg<MAXL>(g, vt, L) + (double)eigen_dot_reg<MAXL>(g, y, L));
+}
+
+// pass 1 (write == 0): `nsw` sweeps from the stored gamma, per-sweep block
+// partial sums of qp_values, gamma not written.  pass 2 (write == 1): replays
+// exactly fl[kLF_SWEEPS] sweeps and stores gamma.
+template <int MAXL>
+__global__ void k_lp_sweeps(int N, int L, LPConst c, float* __restrict__ gamma, const float* __restrict__ Y, int nsw,
+                            int write, double* __restrict__ parts, const int* __restrict__ fl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = i < N;
+  if (write && fl[kLF_SWEEPS] == 0) return;
+  const int n = write ? fl[kLF_SWEEPS] : nsw;
+  const size_t o = (size_t)i * L;
+  float g[MAXL], y[MAXL];
+#pragma unroll
+  for (int l = 0; l < MAXL; ++l) {
+    g[l] = (act && l < L) ? gamma[o + l] : 0.f;
+    y[l] = (act && l < L) ? Y[o + l] : 0.f;
+  }
+  for (int sw = 0; sw < n; ++sw) {
+    const float v = act ? qp_sweep<MAXL>(g, y, L, c) : 0.f;
+    if (!write) {
+      double vv[1] = {(double)v};
+      block_sum<1>(vv);
+      if (threadIdx.x == 0) parts[(size_t)sw * gridDim.x + blockIdx.x] = vv[0];
+    }
+  }
+  if (write && act)
+#pragma unroll
+    for (int l = 0; l < MAXL; ++l)
+      if (l < L) gamma[o + l] = g[l];
+}
+
+// stopping sweep of the batch (:670-675): the first sweep whose float total
+// differs from the previous one by less than qp_tol; fl[kLF_SWEEPS] = sweeps
+// to replay, fl[kLF_SWEEP_DONE] = 1 if the loop ends in this batch.
+// sc[kL_QPV] carries the previous total across batches.
+__global__ void k_lp_sweep_decide(const double* __restrict__ parts, int nblocks, int nsw, float qp_tol, int done_before,
+                                  int max_total, double* __restrict__ sc, int* __restrict__ fl) {
+  __shared__ double tot[kSweepBatch];
+  for (int sw = 0; sw < nsw; ++sw) {
+    double v[1] = {0.0};
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) v[0] += parts[(size_t)sw * nblocks + b];
+    block_sum<1>(v);
+    if (threadIdx.x == 0) tot[sw] = v[0];
+  }
+  if (threadIdx.x != 0) return;
+  float prev = done_before == 0 ? 3.402823466e38f : (float)sc[kL_QPV];
+  int stop = nsw, finished = 0;
+  for (int sw = 0; sw < nsw; ++sw) {
+    const float cur = (float)tot[sw];
+    const int total = done_before + sw + 1;
+    if (fabsf(prev - cur) < qp_tol || total >= max_total) {
+      stop = sw + 1;
+      finished = 1;
+      break;
+    }
+    prev = cur;
+  }
+  sc[kL_QPV] = (double)prev;
+  fl[kLF_SWEEPS] = stop;
+  fl[kLF_SWEEP_DONE] = finished;
+}
+
+// beta = -colsum(alpha+u+gamma-U)/M, Q = lambda*(alpha+beta+u+gamma-U) + cur_Q
+// (:684-697); block min/max of Q for rescale (:699)
+__global__ void k_lp_betaQ(int N, int L, float lambda, const float* __restrict__ alpha, const float* __restrict__ u,
+                           const float* __restrict__ gamma, const float* __restrict__ U,
+                           const float* __restrict__ curQ, float* __restrict__ Q, float* __restrict__ mmparts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  float mn = INFINITY, mx = -INFINITY;
+  if (i < N) {
+    const size_t o = (size_t)i * L;
+    auto bm = [&](int l) {
+      const float uu = u ? u[o + l] : 0.f;
+      return ((alpha[o + l] + uu) + gamma[o + l]) - U[o + l];
+    };
+    float b = -eigen_sum(L, col_astart(i, L), bm);
+    b = b / (float)L;
+    for (int l = 0; l < L; ++l) {
+      const float uu = u ? u[o + l] : 0.f;
+      const float q = lambda * ((((alpha[o + l] + b) + uu) + gamma[o + l]) - U[o + l]) + curQ[o + l];
+      Q[o + l] = q;
+      mn = fminf(mn, q);
+      mx = fmaxf(mx, q);
+      if (q != q) mn = mx = q;
+    }
+  }
+  __shared__ float smn[kBlock], smx[kBlock];
+  smn[threadIdx.x] = mn;
+  smx[threadIdx.x] = mx;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const float a = smn[threadIdx.x + s], b2 = smx[threadIdx.x + s];
+      smn[threadIdx.x] = (a != a || smn[threadIdx.x] != smn[threadIdx.x]) ? NAN : fminf(smn[threadIdx.x], a);
+      smx[threadIdx.x] = (b2 != b2 || smx[threadIdx.x] != smx[threadIdx.x]) ? NAN : fmaxf(smx[threadIdx.x], b2);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    mmparts[2 * blockIdx.x] = smn[0];
+    mmparts[2 * blockIdx.x + 1] = smx[0];
+  }
+}
+
+// rescale (src/eigen_utils.cpp:120-140): mm = {min, max(Q - min)}; error when
+// max < 0 (only possible with NaN here)
+__global__ void k_lp_minmax_final(const float* __restrict__ parts, int nb, float* __restrict__ mm,
+                                  int* __restrict__ fl) {
+  if (threadIdx.x != 0) return;
+  float mn = parts[0], mx = parts[1];
+  bool nan = mn != mn || mx != mx;
+  for (int b = 1; b < nb; ++b) {
+    const float a = parts[2 * b], c = parts[2 * b + 1];
+    nan = nan || a != a || c != c;
+    mn = fminf(mn, a);
+    mx = fmaxf(mx, c);
+  }
+  const float maxval = mx - mn;
+  mm[0] = mn;
+  mm[1] = maxval;
+  if (nan || maxval < 0) fl[kLF_ERR] = 1;
+}
+
+// tmp = alpha + u - (sa + su); gap = tmp.Q; tt = tmp.tmp (:725-729)
+__global__ void k_lp_gap(size_t n, const float* __restrict__ alpha, const float* __restrict__ u,
+                         const float* __restrict__ sa, const float* __restrict__ su, const float* __restrict__ Q,
+                         double* __restrict__ parts) {
+  double v[2] = {0.0, 0.0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float uu = u ? u[i] : 0.f, ss = su ? su[i] : 0.f;
+    const float t = (alpha[i] + uu) - (sa[i] + ss);
+    v[0] += (double)t * (double)Q[i];
+    v[1] += (double)t * (double)t;
+  }
+  block_sum<2>(v);
+  if (threadIdx.x == 0) {
+    parts[2 * blockIdx.x] = v[0];
+    parts[2 * blockIdx.x + 1] = v[1];
+  }
+}
+// delta = clamp((float)(gap / (lambda * tt)), 0, 1) (:729-730); stats
+__global__ void k_lp_delta(double* __restrict__ sc, float lambda, double* __restrict__ st_gap,
+                           double* __restrict__ st_delta, int idx, int cap) {
+  const double gap = sc[kL_GAP], tt = sc[kL_TT];
+  float d = (float)(gap / ((double)lambda * tt));
+  d = (d < 0.0f) ? 0.0f : d;  // std::max(d, 0.0f)
+  d = (1.0f < d) ? 1.0f : d;  // std::min(d, 1.0f)
+  sc[kL_DELTA] = d;
+  if (idx < cap) {
+    st_gap[idx] = gap;
+    st_delta[idx] = d;
+  }
+}
+
+// descending sort of a register array (odd-even transposition network)
+template <int MAXL>
+__device__ inline void sort_desc(float (&v)[MAXL]) {
+#pragma unroll
+  for (int pass = 0; pass < MAXL; ++pass)
+#pragma unroll
+    for (int i = pass & 1; i + 1 < MAXL; i += 2) {
+      const float a = v[i], b = v[i + 1];
+      v[i] = fmaxf(a, b);
+      v[i + 1] = fminf(a, b);
+    }
+}
+
+// feasible_Q (:470-492) then renormalize (:444-455) then currentMap (:929-938)
+template <int MAXL>
+__global__ void k_lp_project(int N, int L, const float* __restrict__ Qin, float* __restrict__ Qout,
+                             int* __restrict__ lab, int do_feasible) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const size_t o = (size_t)i * L;
+  float q[MAXL];
+#pragma unroll
+  for (int l = 0; l < MAXL; ++l) q[l] = l < L ? Qin[o + l] : -INFINITY;
+  if (do_feasible) {
+    float srt[MAXL];
+#pragma unroll
+    for (int l = 0; l < MAXL; ++l) srt[l] = q[l];
+    sort_desc<MAXL>(srt);
+    const float cs = eigen_sum(L, col_astart(i, L), [&](int l) { return Qin[o + l]; });
+    double sum = (double)(float)(cs - 1.f);
+    int K = -1;
+    bool found = false;
+#pragma unroll
+    for (int k = MAXL; k > 0; --k)
+      if (k <= L && !found) {
+        const double uk = (double)srt[k - 1];
+        if (sum / k < uk) {
+          K = k;
+          found = true;
+        } else {
+          sum -= uk;
+        }
+      }
+#pragma unroll
+    for (int l = 0; l < MAXL; ++l)
+      if (l < L) q[l] = (float)fmin(fmax((double)q[l] - sum / K, 0.0), 1.0);
+    // renormalize: column sum in Eigen's order for this column's alignment
+#pragma unroll
+    for (int l = 0; l < MAXL; ++l)
+      if (l < L) Qout[o + l] = q[l];
+    const double s = (double)eigen_sum(L, col_astart(i, L), [&](int l) { return Qout[o + l]; });
+    if (s == 0 || s != s) {
+      const float un = (float)(1.0 / L);
+#pragma unroll
+      for (int l = 0; l < MAXL; ++l) q[l] = un;
+    } else {
+      const float sf = (float)s;
+#pragma unroll
+      for (int l = 0; l < MAXL; ++l) q[l] = q[l] / sf;
+    }
+  } else {
+    const double s = (double)eigen_sum(L, col_astart(i, L), [&](int l) { return Qin[o + l]; });
+    if (s == 0 || s != s) {
+      const float un = (float)(1.0 / L);
+#pragma unroll
+      for (int l = 0; l < MAXL; ++l) q[l] = un;
+    } else {
+      const float sf = (float)s;
+#pragma unroll
+      for (int l = 0; l < MAXL; ++l) q[l] = q[l] / sf;
+    }
+  }
+  int m = 0;
+  float best = q[0];
+#pragma unroll
+  for (int l = 0; l < MAXL; ++l)
+    if (l < L) {
+      Qout[o + l] = q[l];
+      if (q[l] > best) {
+        best = q[l];
+        m = l;
+      }
+    }
+  lab[i] = m;
+}
+
+// max_rounding (:832-840)
+__global__ void k_onehot(const int* __restrict__ lab, int N, int L, float* __restrict__ Q) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)N * L) return;
+  const int i = (int)(t / L), l = (int)(t - (size_t)i * L);
+  Q[t] = lab[i] == l ? 1.f : 0.f;
+}
+
+}  // namespace
+
+// dispatch over the register width of per-pixel kernels
+#define DCRF_MAXL_DISPATCH(L, BODY)          \
+  do {                                       \
+    if ((L) <= 8) {                          \
+      constexpr int MAXL = 8;                \
+      BODY;                                  \
+    } else if ((L) <= 16) {                  \
+      constexpr int MAXL = 16;               \
+      BODY;                                  \
+    } else if ((L) <= 24) {                  \
+      constexpr int MAXL = 24;               \
+      BODY;                                  \
+    } else if ((L) <= 32) {                  \
+      constexpr int MAXL = 32;               \
+      BODY;                                  \
+    } else {                                 \
+      constexpr int MAXL = 64;               \
+      BODY;                                  \
+    }                                        \
+  } while (0)
+
+struct CRF::LPState {
+  DBuf<float> alpha, u, sa, su, gamma, Y, Q, curQ, best, tmp;
+  DBuf<int> lab, iscratch;
+  DBuf<double> parts, sc;
+  DBuf<float> mmparts, mm;
+  DBuf<int> fl;
+  DBuf<double> st_gap, st_delta;
+  int cap = 0;
+};
+
+// labelling energy: unary + true pairwise (+ superpixel term), into sc[dst]
+void CRF::int_energy_dev(const int* lab, double* dst, double* scratch, hipStream_t s) {
+  Impl& m = *im_;
+  k_fill<<<grid_for(N_), kBlock, 0, s>>>(m.rpix.get(), (size_t)N_, 0.f);
+  for (int k = 0; k < num_pairwise(); ++k) {
+    Potential& p = pairwise(k);
+    SplatArgs a;
+    a.kind = SplatIn::kNotOnehot;
+    a.lab = lab;
+    SliceArgs o;
+    o.out = m.tmp.get();
+    o.scale = -p.w;
+    o.apply_scale = true;
+    p.lat->filter(a, o, L_, false);
+    k_pairwise_true_accum<<<grid_for(N_), kBlock, 0, s>>>(m.tmp.get(), lab, m.rpix.get(), N_, L_);
+  }
+  k_assign_energy<<<kRedBlocks, kBlock, 0, s>>>(U_.get(), lab, m.rpix.get(), N_, L_, m.parts.get());
+  finalize_sums<1><<<1, kBlock, 0, s>>>(m.parts.get(), kRedBlocks, dst);
+  if (R_ > 0) {
+    ho_assign_dev(lab, scratch, m.iscratch.get(), s);
+    k_add_scalar<<<1, 1, 0, s>>>(dst, scratch);
+  }
+  check_launch("int_energy");
+}
+
+// LP energy (:982-1037) of a device Q into *dst (device double)
+void CRF::lp_energy_dev(const float* Q, double* dst, double* scratch, hipStream_t s) {
+  Impl& m = *im_;
+  const size_t NL = (size_t)N_ * L_;
+  launch_dot(s, U_.get(), Q, NL, m.parts.get(), dst);
+  minmax_dev(Q, NL, m.mm.get(), m.fl.get(), s);
+  for (int k = 0; k < num_pairwise(); ++k) {
+    Potential& p = pairwise(k);
+    SliceArgs o;
+    o.out = m.tmp.get();
+    o.scale = -p.w;
+    o.apply_scale = true;
+    p.lat->ordered(Q, m.mm.get(), L_, o);
+    launch_dot(s, Q, m.tmp.get(), NL, m.parts.get(), scratch);
+    k_sub_scalar<<<1, 1, 0, s>>>(dst, scratch);
+  }
+  if (R_ > 0) {
+    ho_maxmin_dev(Q, scratch, s);
+    k_add_scalar<<<1, 1, 0, s>>>(dst, scratch);
+  }
+  check_launch("lp_energy");
+}
+
+double CRF::lp_energy(const float* Q) {
+  Impl& m = *im_;
+  k_zero_scalars<<<1, 64, 0, s_>>>(m.sc.get(), 32, m.fl.get(), 8);
+  lp_energy_dev(Q, m.sc.get() + kS_TMP, m.sc.get() + kS_TMP + 1, s_);
+  double e = 0;
+  int err = 0;
+  DCRF_HIP(hipMemcpyAsync(&e, m.sc.get() + kS_TMP, sizeof(double), hipMemcpyDeviceToHost, s_));
+  DCRF_HIP(hipMemcpyAsync(&err, m.fl.get() + 2, sizeof(int), hipMemcpyDeviceToHost, s_));
+  DCRF_HIP(hipStreamSynchronize(s_));
+  if (err) throw Error(kMaxvalNegative, "Maxval < 0, skipping");
+  return e;
+}
+
+double CRF::assignment_energy_ho(const int* lab) {
+  Impl& m = *im_;
+  int_energy_dev(lab, m.sc.get() + kS_TMP, m.sc.get() + kS_TMP + 1, s_);
+  double e = 0;
+  DCRF_HIP(hipMemcpyAsync(&e, m.sc.get() + kS_TMP, sizeof(double), hipMemcpyDeviceToHost, s_));
+  DCRF_HIP(hipStreamSynchronize(s_));
+  return e;
+}
+
+LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap) {
+  if (!lp_) lp_ = std::make_unique<LPState>();
+  LPState& st = *lp_;
+  Impl& m = *im_;
+  const size_t NL = (size_t)N_ * L_;
+  const int N = N_, L = L_;
+  hipStream_t s = s_;
+  const bool ho = R_ > 0;
+  if (ho) ho_state();
+  for (DBuf<float>* b : {&st.alpha, &st.sa, &st.gamma, &st.Y, &st.Q, &st.curQ, &st.best, &st.tmp}) b->resize(NL);
+  if (ho) {
+    st.u.resize(NL);
+    st.su.resize(NL);
+  }
+  st.lab.resize(N);
+  st.iscratch.resize(4);
+  const int nbp = (int)grid_for(N);
+  st.parts.resize(std::max<size_t>((size_t)kSweepBatch * nbp, (size_t)2 * kRedBlocks));
+  st.sc.resize(kL_NSC);
+  st.mmparts.resize(2 * (size_t)nbp);
+  st.mm.resize(2);
+  st.fl.resize(kLF_NFL);
+  const int cap = std::max(1, std::min(stats_cap, 1 << 16));
+  if (cap > st.cap) {
+    st.st_gap.resize(cap);
+    st.st_delta.resize(cap);
+    st.cap = cap;
+  }
+  float* u = ho ? st.u.get() : nullptr;
+  float* su = ho ? st.su.get() : nullptr;
+  double* sc = st.sc.get();
+  int* fl = st.fl.get();
+  k_zero_scalars<<<1, 64, 0, s>>>(sc, kL_NSC, fl, kLF_NFL);
+
+  const float lambda = P.prox_reg_const;
+  LPConst c;
+  c.lambda = lambda;
+  c.c_m1_over_M = (float)(-1.0 / (double)(float)L);
+  c.c_2l_over_M = (float)(2 * lambda) / (float)L;
+  c.c_1_over_M = (float)(1.0 / (double)(float)L);
+  c.c_1m2_over_M = (float)(1 - 2.0 / (double)(float)L);
+  c.qp_delta = P.qp_const;
+
+  auto read_d = [&](const double* d) {
+    double v = 0;
+    DCRF_HIP(hipMemcpyAsync(&v, d, sizeof(double), hipMemcpyDeviceToHost, s));
+    DCRF_HIP(hipStreamSynchronize(s));
+    return v;
+  };
+  auto check_err = [&]() {
+    int e = 0;
+    DCRF_HIP(hipMemcpyAsync(&e, fl + kLF_ERR, sizeof(int), hipMemcpyDeviceToHost, s));
+    DCRF_HIP(hipStreamSynchronize(s));
+    if (e) throw Error(kMaxvalNegative, "Maxval < 0, skipping");
+  };
+
+  // ---- initialisation (:534-550) ------------------------------------------------
+  DCRF_MAXL_DISPATCH(L, (k_lp_project<MAXL><<<grid_for(N), kBlock, 0, s>>>(N, L, init, st.Q.get(), st.lab.get(), 0)));
+  DCRF_HIP(hipMemcpyAsync(st.best.get(), st.Q.get(), NL * sizeof(float), hipMemcpyDeviceToDevice, s));
+  int_energy_dev(st.lab.get(), sc + kL_EINT, sc + kL_TMP, s);
+  double int_energy = read_d(sc + kL_EINT);
+  lp_energy_dev(st.Q.get(), sc + kL_TMP + 1, sc + kL_TMP + 2, s);
+  check_err();
+  double energy = read_d(sc + kL_TMP + 1);
+  LPResult res;
+  res.init_lp_energy = energy;
+  res.init_int_energy = int_energy;
+  if (energy > int_energy) {
+    k_onehot<<<grid_for(NL), kBlock, 0, s>>>(st.lab.get(), N, L, st.Q.get());
+    lp_energy_dev(st.Q.get(), sc + kL_TMP + 1, sc + kL_TMP + 2, s);
+    energy = read_d(sc + kL_TMP + 1);
+  }
+  double best_energy = energy, best_int_energy = int_energy;
+  int it = 0, count = 0, inner_idx = 0;
+  do {
+    ++it;
+    k_fill<<<grid_for(NL), kBlock, 0, s>>>(st.gamma.get(), NL, 1.f);
+    DCRF_HIP(hipMemcpyAsync(st.curQ.get(), st.Q.get(), NL * sizeof(float), hipMemcpyDeviceToDevice, s));
+    k_fill<<<grid_for(NL), kBlock, 0, s>>>(st.alpha.get(), NL, 0.f);
+    if (ho) k_fill<<<grid_for(NL), kBlock, 0, s>>>(u, NL, 0.f);
+    int pit = 0;
+    do {
+      ++pit;
+      {
+        KTimer tm("lp_Y", s);
+        DCRF_MAXL_DISPATCH(L, (k_lp_Y<MAXL><<<grid_for(N), kBlock, 0, s>>>(N, L, c, st.alpha.get(), u, st.sa.get(), su,
+                                                                           sc, pit > 1 ? 1 : 0, U_.get(),
+                                                                           st.curQ.get(), st.Y.get())));
+      }
+      // qp-gamma sweeps in batches; the replay leaves gamma after the
+      // reference's number of sweeps
+      int done_sweeps = 0;
+      while (true) {
+        const int nsw = std::min(kSweepBatch, P.qp_max_iter - done_sweeps);
+        {
+          KTimer tm("lp_sweeps", s);
+          DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbp, kBlock, 0, s>>>(N, L, c, st.gamma.get(), st.Y.get(), nsw, 0,
+                                                                         st.parts.get(), fl)));
+        }
+        k_lp_sweep_decide<<<1, kBlock, 0, s>>>(st.parts.get(), nbp, nsw, P.qp_tol, done_sweeps, P.qp_max_iter, sc, fl);
+        {
+          KTimer tm("lp_sweeps_replay", s);
+          DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbp, kBlock, 0, s>>>(N, L, c, st.gamma.get(), st.Y.get(), nsw, 1,
+                                                                         st.parts.get(), fl)));
+        }
+        int hf[2] = {0, 0};
+        DCRF_HIP(hipMemcpyAsync(hf, fl, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+        DCRF_HIP(hipStreamSynchronize(s));
+        done_sweeps += hf[kLF_SWEEPS];
+        if (hf[kLF_SWEEP_DONE] || done_sweeps >= P.qp_max_iter) break;
+      }
+      res.qp_sweeps.push_back(done_sweeps);
+      {
+        KTimer tm("lp_betaQ", s);
+        k_lp_betaQ<<<nbp, kBlock, 0, s>>>(N, L, lambda, st.alpha.get(), u, st.gamma.get(), U_.get(), st.curQ.get(),
+                                          st.Q.get(), st.mmparts.get());
+        k_lp_minmax_final<<<1, 64, 0, s>>>(st.mmparts.get(), nbp, st.mm.get(), fl);
+      }
+      for (int k = 0; k < num_pairwise(); ++k) {
+        Potential& p = pairwise(k);
+        SliceArgs o;
+        o.out = st.sa.get();
+        o.scale = -p.w;
+        o.apply_scale = true;
+        o.accumulate = k > 0;
+        p.lat->ordered(st.Q.get(), st.mm.get(), L, o);
+      }
+      if (num_pairwise() == 0) k_fill<<<grid_for(NL), kBlock, 0, s>>>(st.sa.get(), NL, 0.f);
+      if (ho) ucondgrad_dev(st.Q.get(), su, s);
+      {
+        KTimer tm("lp_gap", s);
+        k_lp_gap<<<kRedBlocks, kBlock, 0, s>>>(NL, st.alpha.get(), u, st.sa.get(), su, st.Q.get(), st.parts.get());
+        finalize_sums<2><<<1, kBlock, 0, s>>>(st.parts.get(), kRedBlocks, sc + kL_GAP);
+        k_lp_delta<<<1, 1, 0, s>>>(sc, lambda, st.st_gap.get(), st.st_delta.get(), inner_idx, cap);
+      }
+      ++inner_idx;
+    } while (pit < P.fw_max_iter);
+    check_err();
+    // project, renormalize, labelling (:746-752)
+    {
+      KTimer tm("lp_project", s);
+      DCRF_MAXL_DISPATCH(L, (k_lp_project<MAXL><<<grid_for(N), kBlock, 0, s>>>(N, L, st.Q.get(), st.Q.get(),
+                                                                                st.lab.get(), 1)));
+    }
+    const double prev_int = int_energy;
+    int_energy_dev(st.lab.get(), sc + kL_EINT, sc + kL_TMP, s);
+    int_energy = read_d(sc + kL_EINT);
+    res.int_energy.push_back(int_energy);
+    if (P.best_int) {
+      if (std::abs(int_energy - prev_int) < P.prox_energy_tol)
+        ++count;
+      else
+        count = 0;
+      if (count >= 5) break;
+      if (int_energy < best_int_energy) {
+        DCRF_HIP(hipMemcpyAsync(st.best.get(), st.Q.get(), NL * sizeof(float), hipMemcpyDeviceToDevice, s));
+        best_int_energy = int_energy;
+      }
+    } else {
+      const double prev = energy;
+      lp_energy_dev(st.Q.get(), sc + kL_TMP + 1, sc + kL_TMP + 2, s);
+      check_err();
+      energy = read_d(sc + kL_TMP + 1);
+      if (std::abs(energy - prev) < P.prox_energy_tol)
+        ++count;
+      else
+        count = 0;
+      if (count >= 5) break;
+      if (energy < best_energy) {
+        DCRF_HIP(hipMemcpyAsync(st.best.get(), st.Q.get(), NL * sizeof(float), hipMemcpyDeviceToDevice, s));
+        best_energy = energy;
+      }
+    }
+  } while (it < P.prox_max_iter);
+  res.outer = it;
+  DCRF_HIP(hipMemcpyAsync(out, st.best.get(), NL * sizeof(float), hipMemcpyDeviceToDevice, s));
+  const int n = std::min(inner_idx, cap);
+  res.gap.resize(n);
+  res.delta.resize(n);
+  if (n) {
+    DCRF_HIP(hipMemcpyAsync(res.gap.data(), st.st_gap.get(), n * sizeof(double), hipMemcpyDeviceToHost, s));
+    DCRF_HIP(hipMemcpyAsync(res.delta.data(), st.st_delta.get(), n * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  DCRF_HIP(hipStreamSynchronize(s));
+  return res;
+}
+
+}  // namespace dcrf

@@ -1,0 +1,191 @@
+/*
+ * densecrf_ho_amd: C ABI of the MI355X (gfx950) dense-CRF engine.
+ *
+ * Drop-in boundary for the hot path of thwjoy/densecrf_ho: the
+ * Permutohedral lattice filter, PairwisePotential, and the DenseCRF QP / LP
+ * solvers with Pn superpixel terms.  Each entry point names the reference
+ * interface it replaces (file:line in the reference repository).
+ *
+ * Conventions
+ *  - Every function returns DCRF_OK (0) or an error code; dcrf_last_error()
+ *    returns the message of the calling thread's last failure.  The messages
+ *    of the reference's exceptions are kept ("Value out of bounds",
+ *    "Maxval < 0, skipping").
+ *  - Layouts are the reference's Eigen column-major ones:
+ *      fields (Q, unaries, filter in/out)  L x N, element [pixel*L + label]
+ *      features                             d x N, element [pixel*d + k]
+ *      labellings                           N int32
+ *      image                                N x 3 uint8 RGB, pixel = y*W + x
+ *  - `on_device` != 0 means the pointers are device (HIP) pointers on the
+ *    current device; otherwise they are host pointers and the call copies.
+ *  - Calls are synchronous: results are complete when the call returns.
+ *  - A handle is used by one thread at a time (like the reference objects,
+ *    whose ordered filter mutates member buffers).
+ */
+#ifndef DENSECRF_HO_AMD_H
+#define DENSECRF_HO_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DCRF_OK 0
+#define DCRF_INVALID_ARGUMENT 1
+#define DCRF_OUT_OF_BOUNDS 2   /* std::runtime_error("Value out of bounds"), src/pairwise.cpp:218-219 */
+#define DCRF_MAXVAL_NEGATIVE 3 /* std::runtime_error("Maxval < 0, skipping"), src/eigen_utils.cpp:138 */
+#define DCRF_HIP_ERROR 4
+#define DCRF_INTERNAL 5
+
+/* NormalizationType, include/pairwise.h:34-39 */
+#define DCRF_NO_NORMALIZATION 0
+#define DCRF_NORMALIZE_BEFORE 1
+#define DCRF_NORMALIZE_AFTER 2
+#define DCRF_NORMALIZE_SYMMETRIC 3
+
+typedef struct dcrf_lattice_s* dcrf_lattice_t;
+typedef struct dcrf_crf_s* dcrf_crf_t;
+
+const char* dcrf_last_error(void);
+/* library version string */
+const char* dcrf_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Permutohedral lattice (include/permutohedral.h:50-88)                     */
+/* ------------------------------------------------------------------------ */
+
+/* Permutohedral::init(const MatrixXf& features), src/permutohedral.cpp:152-333.
+ * Builds the lattice of N feature vectors of dimension d (1 <= d <= 12). */
+int dcrf_lattice_create(const float* features, int N, int d, int on_device, dcrf_lattice_t* out);
+int dcrf_lattice_destroy(dcrf_lattice_t lat);
+/* number of lattice points (Permutohedral::M_) and padded pixel count
+ * (N rounded up to a multiple of 4, the rows of offset/rank/barycentric) */
+int dcrf_lattice_info(dcrf_lattice_t lat, int* M, int* Npad);
+/* copies of offset_, rank_, barycentric_ ((d+1) x Npad each), the lattice
+ * keys (d x M, int16) and blur_neighbors_ ((d+1) x M pairs n1,n2; -1 absent),
+ * src/permutohedral.cpp:65-67,311-330.  Host pointers; any may be NULL. */
+int dcrf_lattice_export(dcrf_lattice_t lat, int* offset, int* rank, float* barycentric, short* keys, int* neighbors);
+/* Permutohedral::compute(MatrixXf& out, const MatrixXf& in, bool reverse),
+ * include/permutohedral.h:80-81, src/permutohedral.cpp:1079-1087.
+ * in/out are V x N; out may alias in. */
+int dcrf_lattice_compute(dcrf_lattice_t lat, const float* in, float* out, int V, int reverse, int on_device);
+/* Permutohedral::compute_upper_minus_lower_ord(MatrixXf& out, const MatrixXf& Q),
+ * include/permutohedral.h:83, src/permutohedral.cpp:590-701 (10-bin ordered
+ * filter, no normalisation, no compatibility).  Q is V x N with values in [0,1]. */
+int dcrf_lattice_compute_upper_minus_lower_ord(dcrf_lattice_t lat, const float* Q, float* out, int V, int on_device);
+
+/* ------------------------------------------------------------------------ */
+/* DenseCRF (include/densecrf.h:73-194)                                      */
+/* ------------------------------------------------------------------------ */
+
+/* DenseCRF::DenseCRF(int N, int M), src/densecrf.cpp:139 (1 <= L <= 64) */
+int dcrf_crf_create(int N, int L, dcrf_crf_t* out);
+int dcrf_crf_destroy(dcrf_crf_t crf);
+/* DenseCRF::setUnaryEnergy(const MatrixXf& unary), src/densecrf.cpp:261-263 */
+int dcrf_crf_set_unary(dcrf_crf_t crf, const float* unary, int on_device);
+/* DenseCRF::addPairwiseEnergy(features, new PottsCompatibility(w), DIAG_KERNEL, ntype),
+ * src/densecrf.cpp:157-169 (also keeps the NO_NORMALIZATION copy used by
+ * the energies; here both share one lattice) */
+int dcrf_crf_add_pairwise(dcrf_crf_t crf, const float* features, int d, float potts_w, int ntype, int on_device);
+/* DenseCRF2D::addPairwiseGaussian(sx, sy, new PottsCompatibility(w)), src/densecrf.cpp:170-178 */
+int dcrf_crf_add_pairwise_gaussian(dcrf_crf_t crf, int W, int H, float sx, float sy, float potts_w);
+/* DenseCRF2D::addPairwiseBilateral(sx, sy, sr, sg, sb, im, new PottsCompatibility(w)),
+ * src/densecrf.cpp:179-190; `image` is a host pointer to H*W*3 RGB bytes */
+int dcrf_crf_add_pairwise_bilateral(dcrf_crf_t crf, int W, int H, float sx, float sy, float sr, float sg, float sb,
+                                    const unsigned char* image, float potts_w);
+/* Superpixel layer from a region map: region[pixel] in [0,R) (host pointer);
+ * exp_of_superpixels_ (e) and constants_ (c) per region.  No reference
+ * counterpart: DenseCRF2D::addSuperPixel (src/densecrf.cpp:193-251) computes
+ * the map with mean-shift and then stores exactly these three things.
+ * Regions with no pixel are dropped. */
+int dcrf_crf_add_superpixel_layer(dcrf_crf_t crf, const int* region, int R, const float* e, const float* c);
+/* PairwisePotential::apply / applyTranspose of potential k, src/pairwise.cpp:203-213;
+ * no_norm != 0 applies the NO_NORMALIZATION copy (src/densecrf.cpp:164-168). */
+int dcrf_crf_apply(dcrf_crf_t crf, int k, const float* in, float* out, int transpose, int no_norm, int on_device);
+/* PairwisePotential::apply_upper_minus_lower_ord(out, Q), src/pairwise.cpp:214-228:
+ * fails with DCRF_OUT_OF_BOUNDS when Q leaves [0,1]. */
+int dcrf_crf_apply_upper_minus_lower_ord(dcrf_crf_t crf, int k, const float* Q, float* out, int on_device);
+/* DenseCRF::unary_init(), src/densecrf.cpp:321-325 */
+int dcrf_crf_unary_init(dcrf_crf_t crf, float* Q, int on_device);
+
+/* Per-iteration record of the QP (host arrays owned by the caller). */
+typedef struct {
+  int iterations;  /* out: iterations entered (the reference's loop counter) */
+  int updates;     /* out: iterations that updated Q (all but a final num==0 || denom==0 stop) */
+  int capacity;    /* in: length of num/denom/step (energy holds capacity+1) */
+  double* energy;  /* energy before the first and after each iteration */
+  double* num;
+  double* denom;
+  double* step;
+} dcrf_qp_stats;
+
+/* DenseCRF::qp_inference_super_pixels_non_convex(init), src/densecrf.cpp:340-442.
+ * ref_stop != 0: the reference's rule (stop when the energy decreases by at
+ * most 100, src/densecrf.cpp:390), with max_iter as an extra cap (0 = none);
+ * ref_stop == 0: exactly max_iter iterations unless the line search breaks
+ * (denom == 0 or num == 0, :419).  stats may be NULL. */
+int dcrf_crf_qp(dcrf_crf_t crf, const float* init, float* out, int max_iter, int ref_stop, int on_device,
+                dcrf_qp_stats* stats);
+
+/* LP_inf_params, include/densecrf.h:43-69 (defaults: dcrf_lp_params_default) */
+typedef struct {
+  float prox_reg_const, dual_gap_tol, prox_energy_tol;
+  int prox_max_iter, fw_max_iter, qp_max_iter;
+  float qp_tol, qp_const;
+  int best_int, accel_prox, work_set_size, approx_fw_iter; /* last three read nowhere by the reference */
+  float less_confident_percent, confidence_tol;
+} dcrf_lp_params;
+/* LP_inf_params::LP_inf_params(), src/densecrf.cpp:287-302 */
+void dcrf_lp_params_default(dcrf_lp_params* p);
+
+typedef struct {
+  int outer;          /* out: proximal iterations */
+  int inner;          /* out: Frank-Wolfe steps recorded */
+  int capacity;       /* in: length of the arrays below */
+  int* qp_sweeps;     /* per FW step */
+  double* dual_gap;   /* per FW step */
+  double* delta;      /* per FW step */
+  double* int_energy; /* per proximal iteration */
+  double init_lp_energy, init_int_energy;
+} dcrf_lp_stats;
+
+/* DenseCRF::lp_inference_prox_super_pixels(init, params), src/densecrf.cpp:526-799 */
+int dcrf_crf_lp(dcrf_crf_t crf, const float* init, float* out, const dcrf_lp_params* params, int on_device,
+                dcrf_lp_stats* stats);
+
+/* DenseCRF::currentMap(Q), src/densecrf.cpp:929-938 */
+int dcrf_crf_current_map(dcrf_crf_t crf, const float* Q, int* labels, int on_device);
+/* DenseCRF::assignment_energy_true(l), src/densecrf.cpp:873-885 */
+int dcrf_crf_assignment_energy_true(dcrf_crf_t crf, const int* labels, int on_device, double* energy);
+/* DenseCRF::assignment_energy_higher_order(l), src/densecrf.cpp:844-870 */
+int dcrf_crf_assignment_energy_higher_order(dcrf_crf_t crf, const int* labels, int on_device, double* energy);
+/* DenseCRF::compute_energy_LP_higher_order(Q), src/densecrf.cpp:982-1037 */
+int dcrf_crf_lp_energy(dcrf_crf_t crf, const float* Q, int on_device, double* energy);
+/* DenseCRF::computeUCondGrad(Us, Q), src/densecrf.cpp:803-830 (protected in the
+ * reference; the LP's superpixel conditional gradient) */
+int dcrf_crf_ucondgrad(dcrf_crf_t crf, const float* Q, float* Us, int on_device);
+/* DenseCRF::compute_LR_QP_value(Q, 0), src/densecrf.cpp:940-957 */
+int dcrf_crf_qp_value(dcrf_crf_t crf, const float* Q, int on_device, double* energy);
+
+/* ------------------------------------------------------------------------ */
+/* Device memory and kernel timing.  No counterpart in the reference: they  */
+/* let a caller without another GPU allocator keep inputs resident in HBM   */
+/* (on_device = 1) and time each kernel on the stream it runs on.           */
+/* ------------------------------------------------------------------------ */
+int dcrf_malloc(void** ptr, size_t bytes);
+int dcrf_free(void* ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device */
+int dcrf_memcpy(void* dst, const void* src, size_t bytes, int kind);
+int dcrf_device_synchronize(void);
+/* enable(1) clears the record and times every kernel launch with HIP events;
+ * report writes lines "name count total_ms" into buf (NUL-terminated) and
+ * returns the length it needs (call with len 0 to size the buffer). */
+int dcrf_profile_enable(int on);
+int dcrf_profile_report(char* buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DENSECRF_HO_AMD_H */

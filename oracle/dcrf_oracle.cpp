@@ -29,6 +29,13 @@
 
 namespace orc {
 
+// Deliberate mistakes for the tests that show the fixtures pin the oracle
+// (orc_set_mutation); 0 = faithful restatement.
+//   1 no padding lanes   2 round half away from zero   3 upper bin without 1e-9
+//   4 float blur for V<=2   5 QP step without the factor 2   6 last argmin
+//   7 superpixel sums over member ids   8 ucondgrad without int truncation
+static int g_mut = 0;
+
 // ---------------------------------------------------------------------------
 // Reduction order of Eigen 3.2.7 on SSE2 (include/Eigen/src/Core/Redux.h:193-245,
 // arch/SSE/PacketMath.h:423-431): two packet accumulators over the aligned
@@ -219,7 +226,7 @@ struct Lattice {
   void init(const float* feat, int N_, int d_) {
     N = N_;
     d = d_;
-    Npad = (N + 3) / 4 * 4;
+    Npad = g_mut == 1 ? N : (N + 3) / 4 * 4;
     const int d1 = d + 1;
     offset.assign((size_t)Npad * d1, 0);
     rank.assign((size_t)Npad * d1, 0);
@@ -251,7 +258,7 @@ struct Lattice {
       float sum = 0.f;  // round to the 0-coloured simplex (:222-232)
       for (int i = 0; i <= d; ++i) {
         float v = invdp1 * elev[i];
-        v = std::nearbyint(v);
+        v = g_mut == 2 ? std::round(v) : std::nearbyint(v);
         rem0[i] = v * dp1;
         sum = sum + v;
       }
@@ -333,7 +340,7 @@ struct Lattice {
         const float* v2 = values + (size_t)n2 * V;
         for (int k = 0; k < V; ++k) {
           float s = v1[k] + v2[k];
-          if (V <= 2)
+          if (V <= 2 && g_mut != 4)
             nw[k] = (float)((double)ov[k] + 0.5 * (double)s);  // :518 (double expression)
           else
             nw[k] = ov[k] + 0.5f * s;  // :1035
@@ -396,7 +403,7 @@ struct Lattice {
   // lower bin  = min(floor(float(q*10)), 9)
   // upper bin  = max(floor((double(q) - 1e-9) * 10), 0)
   static int bin_lo(float q) { return std::min((int)std::floor(q * 10), 9); }
-  static int bin_up(float q) { return std::max((int)std::floor(((double)q - 1e-9) * 10), 0); }
+  static int bin_up(float q) { return std::max((int)std::floor(((double)q - (g_mut == 3 ? 0.0 : 1e-9)) * 10), 0); }
 
   // seqCompute_upper_minus_lower_ord (src/permutohedral.cpp:590-701); T is the
   // accumulation type, the bins always come from the float q.
@@ -528,8 +535,10 @@ struct CRF {
     std::vector<T> res((size_t)L * R, (T)0);
     for (int r = 0; r < R; ++r)
       for (int l = 0; l < L; ++l)
-        for (size_t pix = 0; pix < members[r].size(); ++pix)
-          res[(size_t)r * L + l] = res[(size_t)r * L + l] + p[pix * L + l];
+        for (size_t pix = 0; pix < members[r].size(); ++pix) {
+          const size_t px = g_mut == 7 ? (size_t)members[r][pix] : pix;
+          res[(size_t)r * L + l] = res[(size_t)r * L + l] + p[px * L + l];
+        }
     return res;
   }
   // regions -> pixels: res(l,pixel) += p(l,r) for pixel < |r|
@@ -619,7 +628,9 @@ struct CRF {
       for (int i = 0; i < N; ++i) {  // descent_direction (src/qp.cpp:5-15)
         int m = 0;
         for (int l = 1; l < L; ++l)
-          if (grad_y[(size_t)i * L + l] < grad_y[(size_t)i * L + m]) m = l;
+          if (g_mut == 6 ? grad_y[(size_t)i * L + l] <= grad_y[(size_t)i * L + m]
+                         : grad_y[(size_t)i * L + l] < grad_y[(size_t)i * L + m])
+            m = l;
         for (int l = 0; l < L; ++l) s[(size_t)i * L + l] = l == m ? 1 : 0;
       }
       descent_z();
@@ -640,12 +651,15 @@ struct CRF {
       double num = dot(U.data(), sx.data(), NL) + 2 * dot(Q.data(), psisx.data(), NL) + a + b +
                    (double)vec_sum(sx_z.data(), (long)LR);
       double denom = dot(sx.data(), psisx.data(), NL) + (double)sp_scalar(szc, sx);
-      double step = -num / (2 * denom);
+      double step = g_mut == 5 ? -num / denom : -num / (2 * denom);
       if (st) {
         st->num.push_back(num);
         st->denom.push_back(denom);
       }
-      if (denom == 0 || num == 0) break;
+      if (denom == 0 || num == 0) {
+        if (st) st->energy.push_back(energy);  // no update: the energy is unchanged
+        break;
+      }
       if (denom < 0) step = 1;
       if (st) st->step.push_back(step);
       const T stp = (T)step, stp2 = (T)(2 * step);
@@ -774,6 +788,7 @@ struct CRF {
             min_ind = p;
             min_lab = l;
             mn = (int)q;
+            if (g_mut == 8) mn = (int)std::floor(q * 1e6);
           }
           if ((T)mx < q) {
             max_ind = p;
@@ -1039,6 +1054,7 @@ static thread_local std::string g_err;
 extern "C" {
 
 const char* orc_last_error() { return g_err.c_str(); }
+void orc_set_mutation(int m) { orc::g_mut = m; }
 
 void* orc_lattice_new(const float* feat, int N, int d) {
   Lattice* L = new Lattice();

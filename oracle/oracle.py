@@ -81,6 +81,7 @@ def lib():
         _sig(L, "orc_unary_init", None, _f32p, _f32p, _i, _i)
         _sig(L, "orc_feasible", None, _f32p, _f32p, _i, _i)
         _sig(L, "orc_renormalize", None, _f32p, _i, _i)
+        _sig(L, "orc_set_mutation", None, _i)
         _orc = L
     return _orc
 
@@ -228,7 +229,9 @@ class CRF:
         st = np.zeros(cap)
         it = self._L.orc_crf_qp(self.h, int(fp64), init, out, int(max_iter), int(ref_stop), en, num, den, st, cap)
         en = en[: it + 1]
-        return out, dict(iters=it, energy=en[~np.isnan(en)], num=num[:it], denom=den[:it], step=st[:it])
+        up = it - (1 if it > 0 and (num[it - 1] == 0 or den[it - 1] == 0) else 0)
+        return out, dict(iters=it, updates=up, energy=en[~np.isnan(en)], num=num[:it], denom=den[:it],
+                         step=st[:up])
 
     def zlabels(self, R):
         z = np.zeros(R * self.L, np.float32)
@@ -273,6 +276,11 @@ class CRF:
         lab = np.zeros(self.N, np.int32)
         self._L.orc_crf_current_map(self.h, Q, lab)
         return lab
+
+
+def set_mutation(m: int):
+    """Plant deliberate mistake `m` in the oracle (tests only; 0 restores it)."""
+    lib().orc_set_mutation(int(m))
 
 
 def unary_init(U):

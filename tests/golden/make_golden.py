@@ -1,0 +1,187 @@
+"""Write the golden fixtures of tests/golden/ from the compiled reference.
+
+The reference core is built from its own sources by ``make -C oracle ref``
+into oracle/_ref/ (development container only).  This script drives it
+through ctypes on seeded synthetic inputs and stores inputs and outputs as
+compressed .npz files.  Run from the repository root:
+
+    python tests/golden/make_golden.py
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import oracle as O  # noqa: E402
+from densecrf_ho_amd import synthetic as S  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def edge_q(N, V, seed):
+    """values in [0,1] with exact bin edges k/10, 0 and 1 mixed in"""
+    rng = np.random.default_rng(seed)
+    q = rng.random((N, V)).astype(np.float32)
+    m = rng.random((N, V)) < 0.25
+    q[m] = (rng.integers(0, 11, m.sum()) / np.float32(10)).astype(np.float32)
+    q[0, :] = 0.0
+    q[1, :] = 1.0
+    return q
+
+
+def tie_features(n, seed):
+    """d=1 features whose elevated coordinate is an exact odd integer, so that
+    rounding to the nearest multiple of d+1=2 meets exact .5 ties
+    (src/permutohedral.cpp:224-229)."""
+    rng = np.random.default_rng(seed)
+    s0 = np.float32(1.0 / np.sqrt(2.0) * np.float32(np.sqrt(2.0 / 3.0) * 2))
+    out = []
+    while len(out) < n:
+        m = int(rng.integers(-40, 40)) * 2 + 1
+        f = np.float32(m / float(s0))
+        for _ in range(8):
+            if np.float32(f * s0) == np.float32(m):
+                out.append(f)
+                break
+            f = np.nextafter(f, np.float32(np.inf) if np.float32(f * s0) < m else np.float32(-np.inf))
+        else:
+            out.append(np.float32(rng.standard_normal() * 5))
+    return np.array(out, np.float32).reshape(n, 1)
+
+
+def lattice_case(name, feat, V=5, seed=0):
+    lat = O.RefLattice(feat)
+    A = lat.arrays()
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((lat.N, V)).astype(np.float32)
+    x1 = (rng.random((lat.N, 1)) + 0.5).astype(np.float32)
+    q = edge_q(lat.N, V, seed + 1)
+    np.savez_compressed(os.path.join(OUT, f"lattice_{name}.npz"), feat=feat, M=np.int32(lat.M),
+                        offset=A["offset"], rank=A["rank"], bary=A["bary"], nbr=A["nbr"],
+                        x=x, out=lat.filter(x), out_rev=lat.filter(x, reverse=True), x1=x1, out1=lat.filter(x1),
+                        q=q, ord=lat.ordered(q))
+    print(f"lattice_{name}: N={lat.N} d={feat.shape[1]} M={lat.M}")
+
+
+def crf_case(name, W, H, K, seed, L, layers=None, lp_outer=3):
+    wl = S.Workload(W, H, K, seed, L=L)
+    r = O.RefCRF(W, H, L)
+    r.set_unary(wl.U)
+    r.add_gaussian(3, 3, 3)
+    r.add_bilateral(50, 50, 15, 15, 15, wl.img, 5)
+    regions = []
+    if layers:
+        for i, (reg, _) in enumerate(S.superpixel_layers(W, H, layers, [21 + i for i in range(len(layers))])):
+            r.add_layer(reg, 0.5, 0.2)
+            regions.append(reg)
+    rng = np.random.default_rng(seed + 7)
+    x = rng.random((wl.N, L)).astype(np.float32)
+    d = dict(W=np.int32(W), H=np.int32(H), K=np.int32(K), seed=np.int32(seed), L=np.int32(L), img=wl.img, U=wl.U,
+             x=x)
+    for i, reg in enumerate(regions):
+        d[f"region{i}"] = reg
+    for k in range(2):
+        d[f"feat{k}"] = r.features(k, 2 if k == 0 else 5)
+        d[f"apply{k}"] = r.apply(k, x, 0)
+        d[f"applyT{k}"] = r.apply(k, x, 1)
+        d[f"applyNN{k}"] = r.apply(k, x, 2)
+        q = edge_q(wl.N, L, seed + 11 + k)
+        d[f"ordq{k}"] = q
+        d[f"ord{k}"] = r.apply_ord(k, q)
+    Q0 = r.unary_init()
+    d["Q0"] = Q0
+    Qqp = r.qp(Q0)
+    d["Qqp"] = Qqp
+    if layers:
+        d["zlabels"] = r.zlabels()
+    lab = r.current_map(Qqp)
+    d["lab_qp"] = lab.astype(np.int32)
+    d["e_true"] = np.float64(r.energy_true(lab))
+    d["e_ho"] = np.float64(r.energy_ho(lab))
+    d["e_lp"] = np.float64(r.lp_energy(Qqp))
+    d["e_qpvalue"] = np.float64(r.qp_value(Qqp))
+    Qpert = (Qqp * 1.3 - 0.1).astype(np.float32)
+    d["Qpert"] = Qpert
+    d["feasible"] = O.ref_feasible(Qpert)
+    d["renorm"] = O.ref_renormalize(np.abs(Qpert))
+    d["ucond"] = r.ucondgrad(Qpert)
+    P = O.LPParams(prox_max_iter=lp_outer)
+    d["Qlp"] = r.lp(Qqp, P)
+    d["lp_outer_cap"] = np.int32(lp_outer)
+    np.savez_compressed(os.path.join(OUT, f"crf_{name}.npz"), **d)
+    print(f"crf_{name}: N={wl.N} L={L} R={r.R}")
+
+
+def qp_case(name, W, H, K, seed, L, weights=(3.0, 5.0), dup_label=None):
+    """QP end-to-end on a small image.  Negative Potts weights make the
+    problem convex so the line-search step is not clamped to 1; duplicated
+    unary columns create exact ties for the first-minimum rule."""
+    wl = S.Workload(W, H, K, seed, L=L)
+    U = wl.U.copy()
+    if dup_label is not None:
+        U[:, dup_label[1]] = U[:, dup_label[0]]
+    r = O.RefCRF(W, H, L)
+    r.set_unary(U)
+    r.add_gaussian(3, 3, weights[0])
+    r.add_bilateral(50, 50, 15, 15, 15, wl.img, weights[1])
+    Q0 = r.unary_init()
+    Qqp = r.qp(Q0)
+    np.savez_compressed(os.path.join(OUT, f"qp_{name}.npz"), W=np.int32(W), H=np.int32(H), L=np.int32(L), U=U,
+                        img=wl.img, w=np.array(weights, np.float32), Q0=Q0, Qqp=Qqp,
+                        feat0=r.features(0, 2), feat1=r.features(1, 5))
+    print(f"qp_{name}: N={wl.N} L={L}")
+
+
+def c0_case():
+    """Config 0 (320x213, 40 cells, 21 labels): end-to-end QP with the
+    reference's stopping rule and a 2-iteration LP, energies and labels."""
+    W, H, K, L = 320, 213, 40, 21
+    wl = S.Workload(W, H, K, 0, L=L)
+    r = O.RefCRF(W, H, L)
+    r.set_unary(wl.U)
+    r.add_gaussian(3, 3, 3)
+    r.add_bilateral(50, 50, 15, 15, 15, wl.img, 5)
+    Q0 = r.unary_init()
+    Qqp = r.qp(Q0)
+    lab_qp = r.current_map(Qqp)
+    P = O.LPParams(prox_max_iter=2)
+    Qlp = r.lp(Qqp, P)
+    lab_lp = r.current_map(Qlp)
+    res = dict(e_true_qp=r.energy_true(lab_qp), qp_value=r.qp_value(Qqp), lp_energy_qp=r.lp_energy(Qqp),
+               e_true_lp=r.energy_true(lab_lp), lp_energy_lp=r.lp_energy(Qlp),
+               unary_sum=float(wl.U.astype(np.float64).sum()))
+    np.savez_compressed(os.path.join(OUT, "c0_320x213.npz"), lab_qp=lab_qp.astype(np.uint8),
+                        lab_lp=lab_lp.astype(np.uint8), Qqp_rowsum=Qqp.astype(np.float64).sum(1))
+    with open(os.path.join(OUT, "c0_320x213.json"), "w") as f:
+        json.dump(res, f, indent=1)
+    print("c0:", res)
+
+
+def main():
+    W, H = 32, 24
+    wl = S.Workload(W, H, 10, 0)
+    lattice_case("g2_32x24", wl.kernels[0][0])
+    lattice_case("b5_32x24", wl.kernels[1][0])
+    wl2 = S.Workload(31, 23, 8, 1)
+    lattice_case("b5_31x23", wl2.kernels[1][0], seed=3)
+    rng = np.random.default_rng(5)
+    lattice_case("r3_101", (rng.standard_normal((101, 3)) * 3).astype(np.float32), seed=4)
+    lattice_case("r1_50", (rng.standard_normal((50, 1)) * 2).astype(np.float32), seed=6)
+    lattice_case("ties_d1", tie_features(61, 7), seed=8)
+    crf_case("31x23_L5", 31, 23, 8, 1, 5)
+    crf_case("32x24_L21_ho", 32, 24, 10, 2, 21, layers=(5, 9, 17), lp_outer=2)
+    qp_case("convex_31x23_L5", 31, 23, 8, 1, 5, weights=(-3.0, -5.0))
+    qp_case("ties_31x23_L5", 31, 23, 8, 1, 5, dup_label=(1, 3))
+    c0_case()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,110 @@
+"""GPU potentials, QP and energies against the reference fixtures and the
+FP64 oracle.  Tolerances: potentials 1e-5 relative; QP energies 1e-4
+relative to the FP64 oracle; label agreement >= 99.9%."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import GOLDEN, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_crf(g):
+    import densecrf_ho_amd as D
+    W, H, L = int(g["W"]), int(g["H"]), int(g["L"])
+    c = D.DenseCRF2D(W, H, L)
+    c.setUnaryEnergy(g["U"])
+    w = g["w"] if "w" in g else (3.0, 5.0)
+    c.addPairwiseEnergy(g["feat0"], D.PottsCompatibility(float(w[0])))
+    c.addPairwiseEnergy(g["feat1"], D.PottsCompatibility(float(w[1])))
+    return c
+
+
+def test_potentials_vs_reference(golden):
+    g = golden("crf_31x23_L5.npz")
+    c = gpu_crf(g)
+    for k in range(2):
+        print("potential", k, flush=True)
+        assert rel_err(c.apply(k, g["x"]), g[f"apply{k}"]) < 1e-5
+        assert rel_err(c.apply(k, g["x"], transpose=True), g[f"applyT{k}"]) < 1e-5
+        assert rel_err(c.apply(k, g["x"], no_norm=True), g[f"applyNN{k}"]) < 1e-5
+        assert rel_err(c.apply_upper_minus_lower_ord(k, g[f"ordq{k}"]), g[f"ord{k}"]) < 1e-5
+
+
+def test_unary_init(golden):
+    g = golden("crf_31x23_L5.npz")
+    c = gpu_crf(g)
+    np.testing.assert_allclose(c.unary_init(), g["Q0"], rtol=2e-6, atol=1e-7)
+
+
+@pytest.mark.parametrize("case", ["crf_31x23_L5.npz", "qp_convex_31x23_L5.npz", "qp_ties_31x23_L5.npz"])
+def test_qp_reference_stopping_vs_reference(golden, case):
+    g = golden(case)
+    print("case", case, flush=True)
+    c = gpu_crf(g)
+    Q = c.qp_inference_super_pixels_non_convex(g["Q0"])
+    assert rel_err(Q, g["Qqp"]) < 1e-4
+    assert (Q.argmax(1) == g["Qqp"].argmax(1)).mean() >= 0.999
+
+
+def test_energies_vs_reference(golden):
+    g = golden("crf_31x23_L5.npz")
+    c = gpu_crf(g)
+    lab = c.currentMap(g["Qqp"])
+    np.testing.assert_array_equal(lab, g["lab_qp"])
+    e = c.assignment_energy_true(lab)
+    assert abs(e - float(g["e_true"])) <= 1e-5 * abs(float(g["e_true"]))
+    v = c.compute_LR_QP_value(g["Qqp"])
+    assert abs(v - float(g["e_qpvalue"])) <= 1e-5 * abs(float(g["e_qpvalue"]))
+
+
+def _c0():
+    from densecrf_ho_amd import synthetic as S
+    return S.Workload(320, 213, 40, 0)
+
+
+def test_config0_qp_fixed_iterations_vs_fp64_oracle():
+    """320x213, 21 labels, 10 Frank-Wolfe iterations."""
+    import densecrf_ho_amd as D
+    wl = _c0()
+    orc = O.CRF(wl.N, 21)
+    orc.set_unary(wl.U)
+    c = D.DenseCRF2D(320, 213, 21)
+    c.setUnaryEnergy(wl.U)
+    for f, w in wl.kernels:
+        orc.add_pairwise(f, w)
+        c.addPairwiseEnergy(f, D.PottsCompatibility(w))
+    Q0 = O.unary_init(wl.U)
+    Qd, std = orc.qp(Q0, max_iter=10, fp64=True)
+    Qf, stf = orc.qp(Q0, max_iter=10)
+    Qg, stg = c.qp_inference_super_pixels_non_convex(Q0, max_iter=10, ref_stop=False, stats=True)
+    print("energies gpu", stg["energy"], "fp64", std["energy"], flush=True)
+    assert stg["iters"] == std["iters"] == 10
+    np.testing.assert_allclose(stg["energy"], std["energy"], rtol=1e-4)
+    assert (Qg.argmax(1) == Qd.argmax(1)).mean() >= 0.999
+    assert (Qg.argmax(1) == Qf.argmax(1)).mean() >= 0.999
+    eg = c.assignment_energy_true(c.currentMap(Qg))
+    ed = orc.energy_true(orc.current_map(Qd), fp64=True)
+    assert abs(eg - ed) <= 1e-4 * abs(ed)
+
+
+def test_config0_reference_stopping_rule(golden):
+    import densecrf_ho_amd as D
+    with open(os.path.join(GOLDEN, "c0_320x213.json")) as f:
+        ref = json.load(f)
+    g = golden("c0_320x213.npz")
+    wl = _c0()
+    c = D.DenseCRF2D(320, 213, 21)
+    c.setUnaryEnergy(wl.U)
+    c.addPairwiseGaussian(3, 3, D.PottsCompatibility(3))
+    c.addPairwiseBilateral(50, 50, 15, 15, 15, wl.img, D.PottsCompatibility(5))
+    Q = c.qp_inference_super_pixels_non_convex(c.unary_init())
+    lab = c.currentMap(Q)
+    assert (lab == g["lab_qp"]).mean() >= 0.999
+    e = c.assignment_energy_true(lab)
+    assert abs(e - ref["e_true_qp"]) <= 1e-4 * abs(ref["e_true_qp"])

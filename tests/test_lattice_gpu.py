@@ -1,0 +1,78 @@
+"""GPU lattice and filters against the oracle and the reference fixtures.
+
+Integer tables (offset, rank, neighbours, M) and barycentric weights must be
+bit-identical to the reference; filters are compared within 1e-5 relative
+(they are bit-identical wherever a lattice point's splat fits one chunk)."""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+LATTICES = ["g2_32x24", "b5_32x24", "b5_31x23", "r3_101", "r1_50", "ties_d1"]
+
+
+def gpu_lattice(feat):
+    import densecrf_ho_amd as D
+    return D.Permutohedral().init(feat)
+
+
+@pytest.mark.parametrize("name", LATTICES)
+def test_tables_bit_exact_vs_reference(golden, name):
+    g = golden(f"lattice_{name}.npz")
+    print("case", name, flush=True)
+    lat = gpu_lattice(g["feat"])
+    T = lat.tables()
+    assert lat.M == int(g["M"])
+    np.testing.assert_array_equal(T["offset"], g["offset"])
+    np.testing.assert_array_equal(T["rank"], g["rank"])
+    np.testing.assert_array_equal(T["bary"], g["bary"])
+    np.testing.assert_array_equal(T["nbr"], g["nbr"])
+
+
+@pytest.mark.parametrize("name", LATTICES)
+def test_filters_vs_reference(golden, name):
+    g = golden(f"lattice_{name}.npz")
+    print("case", name, flush=True)
+    lat = gpu_lattice(g["feat"])
+    assert rel_err(lat.compute(g["x"]), g["out"]) < 1e-5
+    assert rel_err(lat.compute(g["x"], reverse=True), g["out_rev"]) < 1e-5
+    assert rel_err(lat.compute(g["x1"]), g["out1"]) < 1e-5
+    assert rel_err(lat.compute_upper_minus_lower_ord(g["q"]), g["ord"]) < 1e-5
+
+
+@pytest.mark.parametrize("W,H,K,d", [(320, 213, 40, 5), (641, 479, 120, 2), (641, 479, 120, 5)])
+def test_image_lattice_vs_oracle(W, H, K, d):
+    """Voronoi images (641x479 has N % 4 == 3: padding lanes)."""
+    from densecrf_ho_amd import synthetic as S
+    print("case", W, H, d, flush=True)
+    wl = S.Workload(W, H, K, 1)
+    feat = wl.kernels[0 if d == 2 else 1][0]
+    ref = O.Lattice(feat)
+    A = ref.arrays()
+    lat = gpu_lattice(feat)
+    T = lat.tables()
+    assert lat.M == ref.M
+    for k in ("offset", "rank", "bary", "nbr", "keys"):
+        np.testing.assert_array_equal(T[k], A[k], err_msg=k)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((wl.N, 21)).astype(np.float32)
+    out = lat.compute(x)
+    assert rel_err(out, ref.filter(x.astype(np.float64), fp64=True)) < 1e-4
+    assert rel_err(out, ref.filter(x)) < 1e-5
+    q = rng.random((wl.N, 21)).astype(np.float32)
+    o2 = lat.compute_upper_minus_lower_ord(q)
+    assert rel_err(o2, ref.ordered(q, fp64=True)) < 1e-4
+
+
+def test_ordered_filter_rejects_out_of_range(golden):
+    import densecrf_ho_amd as D
+    g = golden("lattice_b5_31x23.npz")
+    lat = gpu_lattice(g["feat"])
+    q = g["q"].copy()
+    q[5, 2] = -0.25
+    with pytest.raises(D.EngineError, match="Value out of bounds"):
+        lat.compute_upper_minus_lower_ord(q)

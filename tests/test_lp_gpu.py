@@ -1,0 +1,110 @@
+"""GPU LP solver and superpixel (Pn) terms against the reference fixtures and
+the oracle.  Bars: integer positions (ucondgrad) exact; energies within 1e-4
+relative; label agreement >= 99.9%."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import GOLDEN, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_crf(g, layers=True):
+    import densecrf_ho_amd as D
+    W, H, L = int(g["W"]), int(g["H"]), int(g["L"])
+    c = D.DenseCRF2D(W, H, L)
+    c.setUnaryEnergy(g["U"])
+    c.addPairwiseEnergy(g["feat0"], D.PottsCompatibility(3.0))
+    c.addPairwiseEnergy(g["feat1"], D.PottsCompatibility(5.0))
+    i = 0
+    while layers and f"region{i}" in g:
+        c.addSuperPixelLayer(g[f"region{i}"], 0.5, 0.2)
+        i += 1
+    return c
+
+
+def orc_crf(g):
+    c = O.CRF(int(g["W"]) * int(g["H"]), int(g["L"]))
+    c.set_unary(g["U"])
+    c.add_pairwise(g["feat0"], 3.0)
+    c.add_pairwise(g["feat1"], 5.0)
+    i = 0
+    while f"region{i}" in g:
+        c.add_layer(g[f"region{i}"], 0.5, 0.2)
+        i += 1
+    return c
+
+
+@pytest.mark.parametrize("case", ["crf_31x23_L5.npz", "crf_32x24_L21_ho.npz"])
+def test_lp_vs_reference(golden, case):
+    import densecrf_ho_amd as D
+    g = golden(case)
+    print("case", case, flush=True)
+    c = gpu_crf(g)
+    Q, st = c.lp_inference_prox_super_pixels(g["Qqp"], D.LP_inf_params(prox_max_iter=int(g["lp_outer_cap"])),
+                                             stats=True)
+    o = orc_crf(g)
+    Qo, sto = o.lp(g["Qqp"], O.LPParams(prox_max_iter=int(g["lp_outer_cap"])))
+    print("gpu sweeps", st["qp_sweeps"][:20], "oracle", sto["qp_sweeps"][:20], flush=True)
+    print("gpu int energies", st["int_energy"], "oracle", sto["int_energy"], flush=True)
+    lab, labr = Q.argmax(1), g["Qlp"].argmax(1)
+    assert (lab == labr).mean() >= 0.999
+    e, er = o.energy_ho(lab, fp64=True), o.energy_ho(labr, fp64=True)
+    assert abs(e - er) <= 1e-4 * abs(er)
+    assert rel_err(Q, g["Qlp"]) < 1e-3
+
+
+def test_higher_order_pieces_vs_reference(golden):
+    g = golden("crf_32x24_L21_ho.npz")
+    c = gpu_crf(g)
+    lab = g["lab_qp"]
+    e = c.assignment_energy_higher_order(lab)
+    assert abs(e - float(g["e_ho"])) <= 1e-6 * abs(float(g["e_ho"]))
+    e = c.compute_energy_LP_higher_order(g["Qqp"])
+    assert abs(e - float(g["e_lp"])) <= 1e-5 * abs(float(g["e_lp"]))
+
+
+def test_qp_with_superpixels_vs_reference(golden):
+    g = golden("crf_32x24_L21_ho.npz")
+    c = gpu_crf(g)
+    Q = c.qp_inference_super_pixels_non_convex(g["Q0"])
+    assert rel_err(Q, g["Qqp"]) < 1e-4
+    assert (Q.argmax(1) == g["Qqp"].argmax(1)).mean() >= 0.999
+
+
+def test_ucondgrad_positions_exact(golden):
+    """computeUCondGrad with the reference's int truncation: same positions."""
+    g = golden("crf_32x24_L21_ho.npz")
+    c = gpu_crf(g)
+    np.testing.assert_array_equal(c.computeUCondGrad(g["Qpert"]), g["ucond"])
+    # values spanning several integers exercise the truncating comparisons
+    rng = np.random.default_rng(4)
+    Q = (rng.standard_normal(g["Qpert"].shape) * 2.5).astype(np.float32)
+    np.testing.assert_array_equal(c.computeUCondGrad(Q), orc_crf(g).ucondgrad(Q))
+
+
+def test_config0_lp_vs_reference(golden):
+    """320x213, 21 labels: QP (reference stopping) then the LP with 2 outer steps."""
+    import densecrf_ho_amd as D
+    from densecrf_ho_amd import synthetic as S
+    with open(os.path.join(GOLDEN, "c0_320x213.json")) as f:
+        ref = json.load(f)
+    g = golden("c0_320x213.npz")
+    wl = S.Workload(320, 213, 40, 0)
+    c = D.DenseCRF2D(320, 213, 21)
+    c.setUnaryEnergy(wl.U)
+    for feat, w in wl.kernels:
+        c.addPairwiseEnergy(feat, D.PottsCompatibility(w))
+    Qqp = c.qp_inference_super_pixels_non_convex(c.unary_init())
+    Qlp = c.lp_inference_prox_super_pixels(Qqp, D.LP_inf_params(prox_max_iter=2))
+    lab = c.currentMap(Qlp)
+    agree = (lab == g["lab_lp"]).mean()
+    e = c.assignment_energy_true(lab)
+    print("C0 LP: agreement", agree, "energy", e, "reference", ref["e_true_lp"], flush=True)
+    assert agree >= 0.999
+    assert abs(e - ref["e_true_lp"]) <= 1e-4 * abs(ref["e_true_lp"])
