@@ -1,0 +1,210 @@
+"""Benchmark of the dense-CRF hot path on one MI355X.
+
+One step = one QP solve (10 Frank-Wolfe iterations) followed by one LP solve
+(5 proximal x 10 Frank-Wolfe steps, warm-started from the QP result) of a
+synthetic 1920x1080 Voronoi image with 21 labels, spatial (w=3, sxy=3) and
+bilateral (w=5, sxy=50, srgb=15) Potts kernels.  Unaries, initial Q and the
+lattices are resident in HBM before the timed region starts; the lattice build
+is reported separately.
+
+Prints one JSON line (value = ms per QP+LP solve, lower is better) with:
+  roofline      achieved HBM GB/s of the dominant kernel, from algorithmic
+                bytes per launch over its average duration measured with HIP
+                events on the engine's stream
+  cpu_baseline  the CPU oracle (port of the reference) on host cores, on a
+                bounded sample extrapolated to one solve
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+W, H, K, L = 1920, 1080, 518, 21
+QP_ITERS = 10
+LP_PROX, LP_FW = 5, 10
+PEAK_HBM_GBS = 8000.0
+METRIC = "ms/solve QP & LP (1080p, 21 labels) + filter HBM GB/s; batched images/s"
+
+
+def algorithmic_bytes(name: str, N: int, M: dict, V: int = L, S: int = 4) -> float:
+    """Bytes a kernel launch must move at minimum (one read of every input it
+    needs, one write of every output), see DESIGN.md "Kernels and rooflines"."""
+    kind, tag = name.split("[")
+    d = int(tag.split("d=")[1].split(",")[0])
+    m = M[d]
+    e = N * (d + 1)
+    if kind == "splat":
+        return S * N * V + 8 * e + S * m * V + 4 * N
+    if kind == "blur":
+        return 2 * S * m * V + 8 * m
+    if kind == "slice":
+        return S * m * V + 8 * e + 2 * S * N * V + 4 * N
+    if kind == "osplat":
+        return S * N * V + 8 * e + S * m * 20 * V
+    if kind == "oblur":
+        return 2 * S * m * 20 * V + 8 * m
+    if kind == "oslice":
+        return S * m * 20 * V + 8 * e + S * N * V + 2 * S * N * V
+    return float("nan")
+
+
+def cpu_baseline():
+    """Oracle on the host: QP with 1 and 2 iterations at 1080p and the LP with
+    1x1 and 1x2 steps on a 480x270 crop of the same generator (area 1/16),
+    extrapolated linearly to one 10-iteration QP + 5x10 LP at 1080p."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from densecrf_ho_amd import synthetic as S
+
+    def crf(wl):
+        c = O.CRF(wl.N, L)
+        c.set_unary(wl.U)
+        for f, w in wl.kernels:
+            c.add_pairwise(f, w)
+        return c
+
+    wl = S.Workload(W, H, K, 0)
+    c = crf(wl)
+    Q0 = O.unary_init(wl.U)
+    t0 = time.perf_counter()
+    Q1, _ = c.qp(Q0, max_iter=1)
+    t1 = time.perf_counter()
+    c.qp(Q0, max_iter=2)
+    t2 = time.perf_counter()
+    qp_it = (t2 - t1) - (t1 - t0)
+    qp_ms = ((t1 - t0) + (QP_ITERS - 1) * qp_it) * 1e3
+    del c
+    sw, sh = W // 4, H // 4
+    wl2 = S.Workload(sw, sh, max(1, round(sw * sh / 4000)), 0)
+    c2 = crf(wl2)
+    q0 = O.unary_init(wl2.U)
+    qq, _ = c2.qp(q0, max_iter=QP_ITERS)
+    t3 = time.perf_counter()
+    c2.lp(qq, O.LPParams(prox_max_iter=1, fw_max_iter=1))
+    t4 = time.perf_counter()
+    c2.lp(qq, O.LPParams(prox_max_iter=1, fw_max_iter=2))
+    t5 = time.perf_counter()
+    step = (t5 - t4) - (t4 - t3)
+    lp_ms = ((t4 - t3) + (LP_PROX * LP_FW - 1) * step) * 1e3 * (W * H) / (sw * sh)
+    return {"value": round(qp_ms + lp_ms, 1), "unit": "ms/solve", "cores": 1, "kind": "port",
+            "qp_ms": round(qp_ms, 1), "lp_ms": round(lp_ms, 1),
+            "sample": (f"CPU oracle, 1 thread: QP 1 and 2 iterations at {W}x{H} and LP 1x1 and 1x2 steps at "
+                       f"{sw}x{sh}, extrapolated linearly to a {QP_ITERS}-iteration QP and a {LP_PROX}x{LP_FW} LP "
+                       f"at {W}x{H} (LP scaled by pixel count)")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--qp-only", action="store_true", help="time the QP alone (profiling runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(images would shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import densecrf_ho_amd as D
+    from densecrf_ho_amd import synthetic as S
+
+    wl = S.Workload(W, H, K, 0, L=L)
+    crf = D.DenseCRF2D(W, H, L)
+    crf.setUnaryEnergy(D.DeviceArray.from_numpy(wl.U))
+    t = time.perf_counter()
+    for f, w in wl.kernels:
+        crf.addPairwiseEnergy(D.DeviceArray.from_numpy(f), D.PottsCompatibility(w))
+    D.synchronize()
+    build_ms = (time.perf_counter() - t) * 1e3
+    Ms = {2: None, 5: None}
+    for k, (f, _) in enumerate(wl.kernels):
+        lat = D.Permutohedral().init(f)
+        Ms[f.shape[1]] = lat.M
+        lat.close()
+
+    Q0 = D.DeviceArray.from_numpy(S.softmax_init(wl.U))
+    Qqp = D.DeviceArray((wl.N, L))
+    Qlp = D.DeviceArray((wl.N, L))
+    lp_params = D.LP_inf_params(prox_max_iter=LP_PROX, fw_max_iter=LP_FW)
+
+    def step():
+        t0 = time.perf_counter()
+        crf.qp_inference_super_pixels_non_convex(Q0, max_iter=QP_ITERS, ref_stop=False, out=Qqp)
+        t1 = time.perf_counter()
+        if not args.qp_only:
+            crf.lp_inference_prox_super_pixels(Qqp, lp_params, out=Qlp)
+        D.synchronize()
+        return t1 - t0, time.perf_counter() - t1
+
+    for _ in range(args.warmup):
+        step()
+    D.synchronize()
+    D.profile_enable(True)
+    qp_t = lp_t = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        a, b = step()
+        qp_t += a
+        lp_t += b
+    D.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = D.profile_report()
+    D.profile_enable(False)
+
+    ms_per_step = elapsed / args.steps * 1e3
+    # dominant lattice kernel by total time
+    lat_k = {n: v for n, v in prof.items() if "[" in n}
+    dom = max(lat_k, key=lambda n: lat_k[n][1])
+    cnt, tot = lat_k[dom]
+    avg_ms = tot / cnt
+    ab = algorithmic_bytes(dom, wl.N, Ms)
+    achieved = ab / (avg_ms * 1e-3) / 1e9
+    # whole-filter rate of the plain bilateral filter inside the QP
+    filt = [n for n in lat_k if n.split("[")[0] in ("splat", "blur", "slice") and "d=5" in n]
+    fb = sum(algorithmic_bytes(n, wl.N, Ms) * lat_k[n][0] for n in filt)
+    ft = sum(lat_k[n][1] for n in filt) * 1e-3
+    filter_gbs = fb / ft / 1e9 if ft > 0 else None
+
+    out = {
+        "metric": METRIC,
+        "value": round(ms_per_step, 3),
+        "unit": "ms/solve",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"QP {QP_ITERS} it + LP {LP_PROX}x{LP_FW} on {W}x{H} Voronoi image, {L} labels, "
+                               "spatial w=3 sxy=3 + bilateral w=5 sxy=50 srgb=15 Potts",
+                   "image": f"{W}x{H}", "labels": L, "lattice_points": {"d=2": Ms[2], "d=5": Ms[5]}},
+        "qp_ms_per_solve": round(qp_t / args.steps * 1e3, 3),
+        "lp_ms_per_solve": None if args.qp_only else round(lp_t / args.steps * 1e3, 3),
+        "lattice_build_ms": round(build_ms, 2),
+        "filter_gbps": None if filter_gbs is None else round(filter_gbs, 1),
+        "images_per_s": None,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": int(ab)},
+        "kernels": {n: {"launches": c, "total_ms": round(m, 3)} for n, (c, m) in sorted(prof.items())},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -22,6 +22,20 @@ struct dcrf_lattice_s {
 struct dcrf_crf_s {
   std::unique_ptr<dcrf::CRF> crf;
 };
+struct dcrf_pairwise_s {
+  hipStream_t s = nullptr;
+  dcrf::Potential p;
+  dcrf::DBuf<float> mm0;
+  ~dcrf_pairwise_s() {
+    if (s) {
+      (void)hipStreamSynchronize(s);
+      p.lat.reset();
+      p.norm.release();
+      mm0.release();
+      (void)hipStreamDestroy(s);
+    }
+  }
+};
 
 namespace dcrf {
 
@@ -210,6 +224,48 @@ int dcrf_lattice_compute_upper_minus_lower_ord(dcrf_lattice_t lat, const float* 
     lat->lat->ordered(a.d, lat->minmax.get(), V, so);
     b.copy_out();
     DCRF_HIP(hipStreamSynchronize(s));
+  });
+}
+
+// ---- pairwise potential ---------------------------------------------------------
+int dcrf_pairwise_create(const float* features, int N, int d, float potts_w, int ntype, int on_device,
+                         dcrf_pairwise_t* out) {
+  return guarded([&] {
+    DCRF_REQUIRE(features && out && N > 0 && d > 0, "dcrf_pairwise_create: bad arguments");
+    auto h = std::make_unique<dcrf_pairwise_s>();
+    DCRF_HIP(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+    Staged<float> f(features, (size_t)N * d, on_device != 0, h->s, true);
+    potential_init(h->p, f.d, N, d, potts_w, ntype, h->s);
+    h->mm0.resize(2);
+    DCRF_HIP(hipMemsetAsync(h->mm0.get(), 0, 2 * sizeof(float), h->s));
+    DCRF_HIP(hipStreamSynchronize(h->s));
+    *out = h.release();
+  });
+}
+int dcrf_pairwise_destroy(dcrf_pairwise_t pw) {
+  return guarded([&] { delete pw; });
+}
+int dcrf_pairwise_apply(dcrf_pairwise_t pw, const float* in, float* out, int L, int transpose, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(pw && in && out && L > 0, "dcrf_pairwise_apply: bad arguments");
+    const size_t n = (size_t)pw->p.lat->N() * L;
+    Staged<float> a(in, n, on_device != 0, pw->s, true);
+    Staged<float> b(out, n, on_device != 0, pw->s, false);
+    potential_apply(pw->p, a.d, b.d, L, transpose != 0, false);
+    b.copy_out();
+    DCRF_HIP(hipStreamSynchronize(pw->s));
+  });
+}
+int dcrf_pairwise_apply_upper_minus_lower_ord(dcrf_pairwise_t pw, const float* Q, float* out, int L, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(pw && Q && out && L > 0, "dcrf_pairwise_apply_upper_minus_lower_ord: bad arguments");
+    const size_t n = (size_t)pw->p.lat->N() * L;
+    Staged<float> a(Q, n, on_device != 0, pw->s, true);
+    Staged<float> b(out, n, on_device != 0, pw->s, false);
+    check_unit_interval(a.d, n, pw->s);
+    potential_apply_ord(pw->p, a.d, pw->mm0.get(), b.d, L);
+    b.copy_out();
+    DCRF_HIP(hipStreamSynchronize(pw->s));
   });
 }
 

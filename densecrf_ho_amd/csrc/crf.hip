@@ -81,30 +81,35 @@ void CRF::set_unary(const float* dU) {
 
 // DenseKernel::initLattice (src/pairwise.cpp:40-57): norm = 1/sqrt(K*1 + 1e-20)
 // (symmetric) or 1/(K*1 + 1e-20), evaluated in double and rounded to float.
-void CRF::add_pairwise(const float* d_feat, int d, float w, int ntype) {
+void potential_init(Potential& p, const float* d_feat, int N, int d, float w, int ntype, hipStream_t s) {
   DCRF_REQUIRE(ntype >= 0 && ntype <= 3, "addPairwiseEnergy: bad normalisation type");
-  auto p = std::make_unique<Potential>();
-  p->d = d;
-  p->w = w;
-  p->ntype = ntype;
-  p->lat = std::make_unique<Lattice>(s_);
-  p->lat->build(d_feat, N_, d);
-  p->norm.resize(N_);
+  p.d = d;
+  p.w = w;
+  p.ntype = ntype;
+  p.lat = std::make_unique<Lattice>(s);
+  p.lat->build(d_feat, N, d);
+  p.norm.resize(N);
   if (ntype != kNoNorm) {
-    im_->ones.resize(N_);
-    k_fill<<<grid_for(N_), kBlock, 0, s_>>>(im_->ones.get(), (size_t)N_, 1.f);
+    DBuf<float> ones(N);
+    k_fill<<<grid_for(N), kBlock, 0, s>>>(ones.get(), (size_t)N, 1.f);
     SplatArgs in;
     in.kind = SplatIn::kPlain;
-    in.in = im_->ones.get();
+    in.in = ones.get();
     SliceArgs out;
-    out.out = p->norm.get();
-    p->lat->filter(in, out, 1, false);
-    k_norm_finish<<<grid_for(N_), kBlock, 0, s_>>>(p->norm.get(), N_, ntype == kNormSymmetric ? 1 : 0);
+    out.out = p.norm.get();
+    p.lat->filter(in, out, 1, false);
+    k_norm_finish<<<grid_for(N), kBlock, 0, s>>>(p.norm.get(), N, ntype == kNormSymmetric ? 1 : 0);
     check_launch("k_norm_finish");
+    DCRF_HIP(hipStreamSynchronize(s));
   } else {
-    k_fill<<<grid_for(N_), kBlock, 0, s_>>>(p->norm.get(), (size_t)N_, 1.f);
+    k_fill<<<grid_for(N), kBlock, 0, s>>>(p.norm.get(), (size_t)N, 1.f);
+    DCRF_HIP(hipStreamSynchronize(s));
   }
-  DCRF_HIP(hipStreamSynchronize(s_));
+}
+
+void CRF::add_pairwise(const float* d_feat, int d, float w, int ntype) {
+  auto p = std::make_unique<Potential>();
+  potential_init(*p, d_feat, N_, d, w, ntype, s_);
   pw_.push_back(std::move(p));
 }
 
@@ -120,8 +125,7 @@ static inline bool post_norm(int nt, bool tr) {
 // kernel filter then Potts (out = -w * out, src/labelcompatibility.cpp:49-52).
 // no_norm selects the NO_NORMALIZATION copy that addPairwiseEnergy keeps for
 // energies (src/densecrf.cpp:164-168); both share one lattice here.
-void CRF::apply(int k, const float* in, float* out, bool transpose, bool no_norm) {
-  Potential& p = pairwise(k);
+void potential_apply(Potential& p, const float* in, float* out, int V, bool transpose, bool no_norm) {
   const int nt = no_norm ? kNoNorm : p.ntype;
   SplatArgs a;
   a.kind = pre_norm(nt, transpose) ? SplatIn::kScaled : SplatIn::kPlain;
@@ -132,7 +136,19 @@ void CRF::apply(int k, const float* in, float* out, bool transpose, bool no_norm
   o.norm = post_norm(nt, transpose) ? p.norm.get() : nullptr;
   o.scale = -p.w;
   o.apply_scale = true;
-  p.lat->filter(a, o, L_, transpose);
+  p.lat->filter(a, o, V, transpose);
+}
+
+void potential_apply_ord(Potential& p, const float* q01, const float* d_zero_minmax, float* out, int V) {
+  SliceArgs o;
+  o.out = out;
+  o.scale = -p.w;
+  o.apply_scale = true;
+  p.lat->ordered(q01, d_zero_minmax, V, o);
+}
+
+void CRF::apply(int k, const float* in, float* out, bool transpose, bool no_norm) {
+  potential_apply(pairwise(k), in, out, L_, transpose, no_norm);
 }
 
 void CRF::unary_init(float* Q) {
@@ -285,12 +301,7 @@ namespace dcrf {
 // PairwisePotential::apply_upper_minus_lower_ord (src/pairwise.cpp:214-228)
 void CRF::apply_ord(int k, const float* q01, float* out, bool check) {
   if (check) check_unit_interval(q01, (size_t)N_ * L_, s_);
-  Potential& p = pairwise(k);
-  SliceArgs o;
-  o.out = out;
-  o.scale = -p.w;
-  o.apply_scale = true;
-  p.lat->ordered(q01, im_->mm0.get(), L_, o);
+  potential_apply_ord(pairwise(k), q01, im_->mm0.get(), out, L_);
 }
 
 // {min, max(Q - min)} of a device array into mm (rescale, src/eigen_utils.cpp:120-140);

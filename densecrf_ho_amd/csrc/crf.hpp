@@ -26,6 +26,13 @@ struct Potential {
   int ntype = kNormSymmetric;
 };
 
+// DenseKernel::initLattice: build the lattice and the normalisation
+void potential_init(Potential& p, const float* d_feat, int N, int d, float w, int ntype, hipStream_t s);
+// PairwisePotential::apply / applyTranspose on V values per pixel
+void potential_apply(Potential& p, const float* in, float* out, int V, bool transpose, bool no_norm);
+// PairwisePotential::apply_upper_minus_lower_ord (the caller checks [0,1])
+void potential_apply_ord(Potential& p, const float* q01, const float* d_zero_minmax, float* out, int V);
+
 struct QPResult {
   int iterations = 0, updates = 0;
   std::vector<double> energy, num, denom, step;

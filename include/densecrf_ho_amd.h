@@ -45,6 +45,7 @@ extern "C" {
 #define DCRF_NORMALIZE_SYMMETRIC 3
 
 typedef struct dcrf_lattice_s* dcrf_lattice_t;
+typedef struct dcrf_pairwise_s* dcrf_pairwise_t;
 typedef struct dcrf_crf_s* dcrf_crf_t;
 
 const char* dcrf_last_error(void);
@@ -74,6 +75,20 @@ int dcrf_lattice_compute(dcrf_lattice_t lat, const float* in, float* out, int V,
  * include/permutohedral.h:83, src/permutohedral.cpp:590-701 (10-bin ordered
  * filter, no normalisation, no compatibility).  Q is V x N with values in [0,1]. */
 int dcrf_lattice_compute_upper_minus_lower_ord(dcrf_lattice_t lat, const float* Q, float* out, int V, int on_device);
+
+/* ------------------------------------------------------------------------ */
+/* PairwisePotential with Potts compatibility (include/pairwise.h:63-95)     */
+/* ------------------------------------------------------------------------ */
+
+/* PairwisePotential(features, new PottsCompatibility(w), DIAG_KERNEL, ntype),
+ * src/pairwise.cpp:200-202 with DenseKernel::initLattice (:40-57) */
+int dcrf_pairwise_create(const float* features, int N, int d, float potts_w, int ntype, int on_device,
+                         dcrf_pairwise_t* out);
+int dcrf_pairwise_destroy(dcrf_pairwise_t pw);
+/* PairwisePotential::apply / applyTranspose, src/pairwise.cpp:203-213; in/out are L x N */
+int dcrf_pairwise_apply(dcrf_pairwise_t pw, const float* in, float* out, int L, int transpose, int on_device);
+/* PairwisePotential::apply_upper_minus_lower_ord, src/pairwise.cpp:214-228 */
+int dcrf_pairwise_apply_upper_minus_lower_ord(dcrf_pairwise_t pw, const float* Q, float* out, int L, int on_device);
 
 /* ------------------------------------------------------------------------ */
 /* DenseCRF (include/densecrf.h:73-194)                                      */

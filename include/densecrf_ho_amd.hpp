@@ -1,0 +1,370 @@
+// densecrf_ho_amd C++ interface: the reference's classes (Permutohedral,
+// PairwisePotential, LabelCompatibility/PottsCompatibility, DenseCRF,
+// DenseCRF2D, LP_inf_params) with the same names, argument meaning and
+// exceptions, forwarding to the C ABI of include/densecrf_ho_amd.h.
+//
+// Matrices are column-major float matrices exactly as the reference's
+// Eigen::MatrixXf (fields L x N, features d x N).  Define DCRF_USE_EIGEN
+// before including this header to use Eigen::MatrixXf / VectorXf; otherwise
+// the minimal dcrf_ho::MatrixXf below provides the members the API touches
+// (rows, cols, data, resize, operator()).
+//
+// Reference interfaces: include/permutohedral.h:50-88, include/pairwise.h:33-95,
+// include/labelcompatibility.h:36-63, include/densecrf.h:43-194.
+#pragma once
+
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "densecrf_ho_amd.h"
+
+#ifdef DCRF_USE_EIGEN
+#include <Eigen/Core>
+#endif
+
+namespace dcrf_ho {
+
+#ifdef DCRF_USE_EIGEN
+using MatrixXf = Eigen::MatrixXf;
+using VectorXs = Eigen::Matrix<short, Eigen::Dynamic, 1>;
+#else
+// column-major float matrix (the subset of Eigen::MatrixXf the API uses)
+class MatrixXf {
+ public:
+  MatrixXf() = default;
+  MatrixXf(long r, long c) : r_(r), c_(c), v_((size_t)r * c, 0.f) {}
+  long rows() const { return r_; }
+  long cols() const { return c_; }
+  long size() const { return r_ * c_; }
+  float* data() { return v_.data(); }
+  const float* data() const { return v_.data(); }
+  void resize(long r, long c) {
+    r_ = r;
+    c_ = c;
+    v_.assign((size_t)r * c, 0.f);
+  }
+  float& operator()(long i, long j) { return v_[(size_t)j * r_ + i]; }
+  float operator()(long i, long j) const { return v_[(size_t)j * r_ + i]; }
+  void fill(float x) { std::fill(v_.begin(), v_.end(), x); }
+
+ private:
+  long r_ = 0, c_ = 0;
+  std::vector<float> v_;
+};
+class VectorXs {
+ public:
+  VectorXs() = default;
+  explicit VectorXs(long n) : v_((size_t)n, 0) {}
+  long rows() const { return (long)v_.size(); }
+  long size() const { return (long)v_.size(); }
+  short& operator[](long i) { return v_[(size_t)i]; }
+  short operator[](long i) const { return v_[(size_t)i]; }
+  short* data() { return v_.data(); }
+  const short* data() const { return v_.data(); }
+
+ private:
+  std::vector<short> v_;
+};
+#endif
+
+// the reference's exceptions are std::runtime_error with these messages
+inline void check(int rc) {
+  if (rc != DCRF_OK) throw std::runtime_error(dcrf_last_error());
+}
+
+enum NormalizationType { NO_NORMALIZATION, NORMALIZE_BEFORE, NORMALIZE_AFTER, NORMALIZE_SYMMETRIC };
+enum KernelType { CONST_KERNEL, DIAG_KERNEL, FULL_KERNEL };
+
+// LabelCompatibility (include/labelcompatibility.h:36-49); the engine runs
+// the Potts form mu(a,b) = -w [a==b]
+class LabelCompatibility {
+ public:
+  virtual ~LabelCompatibility() = default;
+  virtual float potts_weight() const = 0;
+};
+class PottsCompatibility : public LabelCompatibility {
+ public:
+  explicit PottsCompatibility(float weight = 1.0f) : w_(weight) {}
+  float potts_weight() const override { return w_; }
+
+ protected:
+  float w_;
+};
+
+// LP_inf_params (include/densecrf.h:43-69), defaults of src/densecrf.cpp:287-302
+class LP_inf_params {
+ public:
+  float prox_reg_const, dual_gap_tol, prox_energy_tol;
+  int prox_max_iter, fw_max_iter, qp_max_iter;
+  float qp_tol, qp_const;
+  bool best_int, accel_prox;
+  int work_set_size, approx_fw_iter;
+  float less_confident_percent, confidence_tol;
+  LP_inf_params() {
+    dcrf_lp_params p;
+    dcrf_lp_params_default(&p);
+    from_c(p);
+  }
+  LP_inf_params(float prox_reg_const_, float dual_gap_tol_, float prox_energy_tol_, int prox_max_iter_,
+                int fw_max_iter_, int qp_max_iter_, float qp_tol_, float qp_const_, bool best_int_, bool accel_prox_,
+                int work_set_size_, int approx_fw_iter_, float less_confident_percent_, float confidence_tol_)
+      : prox_reg_const(prox_reg_const_), dual_gap_tol(dual_gap_tol_), prox_energy_tol(prox_energy_tol_),
+        prox_max_iter(prox_max_iter_), fw_max_iter(fw_max_iter_), qp_max_iter(qp_max_iter_), qp_tol(qp_tol_),
+        qp_const(qp_const_), best_int(best_int_), accel_prox(accel_prox_), work_set_size(work_set_size_),
+        approx_fw_iter(approx_fw_iter_), less_confident_percent(less_confident_percent_),
+        confidence_tol(confidence_tol_) {}
+  dcrf_lp_params to_c() const {
+    dcrf_lp_params p;
+    p.prox_reg_const = prox_reg_const;
+    p.dual_gap_tol = dual_gap_tol;
+    p.prox_energy_tol = prox_energy_tol;
+    p.prox_max_iter = prox_max_iter;
+    p.fw_max_iter = fw_max_iter;
+    p.qp_max_iter = qp_max_iter;
+    p.qp_tol = qp_tol;
+    p.qp_const = qp_const;
+    p.best_int = best_int;
+    p.accel_prox = accel_prox;
+    p.work_set_size = work_set_size;
+    p.approx_fw_iter = approx_fw_iter;
+    p.less_confident_percent = less_confident_percent;
+    p.confidence_tol = confidence_tol;
+    return p;
+  }
+
+ private:
+  void from_c(const dcrf_lp_params& p) {
+    prox_reg_const = p.prox_reg_const;
+    dual_gap_tol = p.dual_gap_tol;
+    prox_energy_tol = p.prox_energy_tol;
+    prox_max_iter = p.prox_max_iter;
+    fw_max_iter = p.fw_max_iter;
+    qp_max_iter = p.qp_max_iter;
+    qp_tol = p.qp_tol;
+    qp_const = p.qp_const;
+    best_int = p.best_int != 0;
+    accel_prox = p.accel_prox != 0;
+    work_set_size = p.work_set_size;
+    approx_fw_iter = p.approx_fw_iter;
+    less_confident_percent = p.less_confident_percent;
+    confidence_tol = p.confidence_tol;
+  }
+};
+
+// Permutohedral (include/permutohedral.h:50-88)
+class Permutohedral {
+ public:
+  Permutohedral() = default;
+  ~Permutohedral() {
+    if (h_) dcrf_lattice_destroy(h_);
+  }
+  Permutohedral(const Permutohedral&) = delete;
+  Permutohedral& operator=(const Permutohedral&) = delete;
+  void init(const MatrixXf& features) {
+    if (h_) dcrf_lattice_destroy(h_);
+    h_ = nullptr;
+    check(dcrf_lattice_create(features.data(), (int)features.cols(), (int)features.rows(), 0, &h_));
+    N_ = (int)features.cols();
+  }
+  void compute(MatrixXf& out, const MatrixXf& in, bool reverse = false) const {
+    if (out.rows() != in.rows() || out.cols() != in.cols()) out.resize(in.rows(), in.cols());
+    check(dcrf_lattice_compute(h_, in.data(), out.data(), (int)in.rows(), reverse ? 1 : 0, 0));
+  }
+  MatrixXf compute(const MatrixXf& in, bool reverse = false) const {
+    MatrixXf r;
+    compute(r, in, reverse);
+    return r;
+  }
+  void compute_upper_minus_lower_ord(MatrixXf& out, const MatrixXf& Q) {
+    if (out.rows() != Q.rows() || out.cols() != Q.cols()) out.resize(Q.rows(), Q.cols());
+    check(dcrf_lattice_compute_upper_minus_lower_ord(h_, Q.data(), out.data(), (int)Q.rows(), 0));
+  }
+  int latticeSize() const {
+    int M = 0;
+    check(dcrf_lattice_info(h_, &M, nullptr));
+    return M;
+  }
+
+ private:
+  dcrf_lattice_t h_ = nullptr;
+  int N_ = 0;
+};
+
+// PairwisePotential (include/pairwise.h:63-95) with a Potts compatibility;
+// takes ownership of `compatibility` as the reference does
+class PairwisePotential {
+ public:
+  PairwisePotential(const MatrixXf& features, LabelCompatibility* compatibility, KernelType ktype = CONST_KERNEL,
+                    NormalizationType ntype = NORMALIZE_SYMMETRIC)
+      : compat_(compatibility), f_(features), ktype_(ktype), ntype_(ntype) {
+    check(dcrf_pairwise_create(features.data(), (int)features.cols(), (int)features.rows(),
+                               compatibility->potts_weight(), (int)ntype, 0, &h_));
+  }
+  ~PairwisePotential() {
+    if (h_) dcrf_pairwise_destroy(h_);
+  }
+  PairwisePotential(const PairwisePotential&) = delete;
+  void apply(MatrixXf& out, const MatrixXf& Q) const {
+    if (out.rows() != Q.rows() || out.cols() != Q.cols()) out.resize(Q.rows(), Q.cols());
+    check(dcrf_pairwise_apply(h_, Q.data(), out.data(), (int)Q.rows(), 0, 0));
+  }
+  void applyTranspose(MatrixXf& out, const MatrixXf& Q) const {
+    if (out.rows() != Q.rows() || out.cols() != Q.cols()) out.resize(Q.rows(), Q.cols());
+    check(dcrf_pairwise_apply(h_, Q.data(), out.data(), (int)Q.rows(), 1, 0));
+  }
+  void apply_upper_minus_lower_ord(MatrixXf& out, const MatrixXf& Q) {
+    if (out.rows() != Q.rows() || out.cols() != Q.cols()) out.resize(Q.rows(), Q.cols());
+    check(dcrf_pairwise_apply_upper_minus_lower_ord(h_, Q.data(), out.data(), (int)Q.rows(), 0));
+  }
+  MatrixXf features() const { return f_; }
+  KernelType ktype() const { return ktype_; }
+  NormalizationType ntype() const { return ntype_; }
+
+ private:
+  std::unique_ptr<LabelCompatibility> compat_;
+  MatrixXf f_;
+  KernelType ktype_;
+  NormalizationType ntype_;
+  dcrf_pairwise_t h_ = nullptr;
+};
+
+// DenseCRF (include/densecrf.h:73-170)
+class DenseCRF {
+ public:
+  DenseCRF(int N, int M) : N_(N), M_(M) { check(dcrf_crf_create(N, M, &h_)); }
+  virtual ~DenseCRF() {
+    if (h_) dcrf_crf_destroy(h_);
+  }
+  DenseCRF(const DenseCRF&) = delete;
+  DenseCRF& operator=(const DenseCRF&) = delete;
+
+  // ownership of `function` is transferred, as in the reference
+  virtual void addPairwiseEnergy(const MatrixXf& features, LabelCompatibility* function,
+                                 KernelType kernel_type = DIAG_KERNEL,
+                                 NormalizationType normalization_type = NORMALIZE_SYMMETRIC) {
+    std::unique_ptr<LabelCompatibility> f(function);
+    check(dcrf_crf_add_pairwise(h_, features.data(), (int)features.rows(), f->potts_weight(),
+                                (int)normalization_type, 0));
+  }
+  void setUnaryEnergy(const MatrixXf& unary) { check(dcrf_crf_set_unary(h_, unary.data(), 0)); }
+  // superpixel layer from a region map (the state addSuperPixel derives from
+  // mean-shift): region[i] in [0,R), exp_of_superpixels_ e and constants_ c
+  void addSuperPixelLayer(const std::vector<int>& region, const std::vector<float>& e,
+                          const std::vector<float>& c) {
+    check(dcrf_crf_add_superpixel_layer(h_, region.data(), (int)e.size(), e.data(), c.data()));
+  }
+  MatrixXf unary_init() const {
+    MatrixXf Q(M_, N_);
+    check(dcrf_crf_unary_init(h_, Q.data(), 0));
+    return Q;
+  }
+  // reference stopping rule (src/densecrf.cpp:390); max_iter > 0 runs a fixed count
+  const MatrixXf qp_inference_super_pixels_non_convex(const MatrixXf& init, int max_iter = 0) {
+    MatrixXf Q(M_, N_);
+    check(dcrf_crf_qp(h_, init.data(), Q.data(), max_iter, max_iter > 0 ? 0 : 1, 0, nullptr));
+    return Q;
+  }
+  const MatrixXf lp_inference_prox_super_pixels(const MatrixXf& init, const LP_inf_params& params) const {
+    MatrixXf Q(M_, N_);
+    const dcrf_lp_params p = params.to_c();
+    check(dcrf_crf_lp(h_, init.data(), Q.data(), &p, 0, nullptr));
+    return Q;
+  }
+  MatrixXf max_rounding(const MatrixXf& estimates) const {
+    VectorXs l = currentMap(estimates);
+    MatrixXf r(estimates.rows(), estimates.cols());
+    r.fill(0.f);
+    for (long i = 0; i < estimates.cols(); ++i) r(l[i], i) = 1.f;
+    return r;
+  }
+  VectorXs currentMap(const MatrixXf& Q) const {
+    std::vector<int> lab((size_t)N_);
+    check(dcrf_crf_current_map(h_, Q.data(), lab.data(), 0));
+    VectorXs r(N_);
+    for (int i = 0; i < N_; ++i) r[i] = (short)lab[(size_t)i];
+    return r;
+  }
+  double assignment_energy_true(const VectorXs& l) const {
+    std::vector<int> lab = labels(l);
+    double e = 0;
+    check(dcrf_crf_assignment_energy_true(h_, lab.data(), 0, &e));
+    return e;
+  }
+  double assignment_energy_higher_order(const VectorXs& l) const {
+    std::vector<int> lab = labels(l);
+    double e = 0;
+    check(dcrf_crf_assignment_energy_higher_order(h_, lab.data(), 0, &e));
+    return e;
+  }
+  double compute_energy_LP_higher_order(const MatrixXf& Q) const {
+    double e = 0;
+    check(dcrf_crf_lp_energy(h_, Q.data(), 0, &e));
+    return e;
+  }
+  // compute_LR_QP_value(Q, diag_dom): the engine evaluates diag_dom == 0
+  double compute_LR_QP_value(const MatrixXf& Q, const MatrixXf& diag_dom) const {
+    for (long i = 0; i < diag_dom.size(); ++i)
+      if (diag_dom.data()[i] != 0.f) throw std::runtime_error("compute_LR_QP_value: diag_dom must be zero");
+    double e = 0;
+    check(dcrf_crf_qp_value(h_, Q.data(), 0, &e));
+    return e;
+  }
+
+ protected:
+  std::vector<int> labels(const VectorXs& l) const {
+    std::vector<int> lab((size_t)N_);
+    for (int i = 0; i < N_; ++i) lab[(size_t)i] = l[i];
+    return lab;
+  }
+  int N_, M_;
+  dcrf_crf_t h_ = nullptr;
+};
+
+// DenseCRF2D (include/densecrf.h:172-194)
+class DenseCRF2D : public DenseCRF {
+ public:
+  DenseCRF2D(int W, int H, int M) : DenseCRF(W * H, M), W_(W), H_(H) {}
+  void addPairwiseGaussian(float sx, float sy, LabelCompatibility* function = nullptr,
+                           KernelType kernel_type = DIAG_KERNEL,
+                           NormalizationType normalization_type = NORMALIZE_SYMMETRIC) {
+    std::unique_ptr<LabelCompatibility> f(function ? function : new PottsCompatibility(1.f));
+    if (normalization_type == NORMALIZE_SYMMETRIC) {
+      check(dcrf_crf_add_pairwise_gaussian(h_, W_, H_, sx, sy, f->potts_weight()));
+      return;
+    }
+    MatrixXf feature(2, N_);
+    for (int j = 0; j < H_; j++)
+      for (int i = 0; i < W_; i++) {
+        feature(0, j * W_ + i) = i / sx;
+        feature(1, j * W_ + i) = j / sy;
+      }
+    addPairwiseEnergy(feature, f.release(), kernel_type, normalization_type);
+  }
+  void addPairwiseBilateral(float sx, float sy, float sr, float sg, float sb, const unsigned char* im,
+                            LabelCompatibility* function = nullptr, KernelType kernel_type = DIAG_KERNEL,
+                            NormalizationType normalization_type = NORMALIZE_SYMMETRIC) {
+    std::unique_ptr<LabelCompatibility> f(function ? function : new PottsCompatibility(1.f));
+    if (normalization_type == NORMALIZE_SYMMETRIC) {
+      check(dcrf_crf_add_pairwise_bilateral(h_, W_, H_, sx, sy, sr, sg, sb, im, f->potts_weight()));
+      return;
+    }
+    MatrixXf feature(5, N_);
+    for (int j = 0; j < H_; j++)
+      for (int i = 0; i < W_; i++) {
+        feature(0, j * W_ + i) = i / sx;
+        feature(1, j * W_ + i) = j / sy;
+        feature(2, j * W_ + i) = im[(i + j * W_) * 3 + 0] / sr;
+        feature(3, j * W_ + i) = im[(i + j * W_) * 3 + 1] / sg;
+        feature(4, j * W_ + i) = im[(i + j * W_) * 3 + 2] / sb;
+      }
+    addPairwiseEnergy(feature, f.release(), kernel_type, normalization_type);
+  }
+
+ protected:
+  int W_, H_;
+};
+
+}  // namespace dcrf_ho

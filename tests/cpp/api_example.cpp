@@ -1,0 +1,72 @@
+// Drives the C++ interface (include/densecrf_ho_amd.hpp) the way the
+// reference's example_inference does (src/example_inference.cpp:15-88):
+// build a DenseCRF2D with a spatial and a bilateral Potts kernel, initialise
+// from the unaries, run the QP and then the LP, print the energies.
+//
+//   api_example W H L unary.f32 image.u8 out_qp.f32 out_lp.f32
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <stdexcept>
+#include <vector>
+
+#include "densecrf_ho_amd.hpp"
+
+using namespace dcrf_ho;
+
+template <class T>
+static std::vector<T> read_file(const char* p, size_t n) {
+  std::vector<T> v(n);
+  std::ifstream f(p, std::ios::binary);
+  if (!f.read(reinterpret_cast<char*>(v.data()), n * sizeof(T))) throw std::runtime_error(std::string("read ") + p);
+  return v;
+}
+static void write_mat(const char* p, const MatrixXf& m) {
+  std::ofstream f(p, std::ios::binary);
+  f.write(reinterpret_cast<const char*>(m.data()), m.size() * sizeof(float));
+}
+
+int main(int argc, char** argv) {
+  if (argc != 8) {
+    std::fprintf(stderr, "usage: %s W H L unary.f32 image.u8 out_qp.f32 out_lp.f32\n", argv[0]);
+    return 2;
+  }
+  const int W = std::atoi(argv[1]), H = std::atoi(argv[2]), L = std::atoi(argv[3]), N = W * H;
+  std::vector<float> u = read_file<float>(argv[4], (size_t)N * L);
+  std::vector<unsigned char> im = read_file<unsigned char>(argv[5], (size_t)N * 3);
+  MatrixXf unary(L, N);
+  std::copy(u.begin(), u.end(), unary.data());
+
+  DenseCRF2D crf(W, H, L);
+  crf.setUnaryEnergy(unary);
+  crf.addPairwiseGaussian(3, 3, new PottsCompatibility(3));
+  crf.addPairwiseBilateral(50, 50, 15, 15, 15, im.data(), new PottsCompatibility(5));
+  MatrixXf Q = crf.unary_init();
+  MatrixXf Qqp = crf.qp_inference_super_pixels_non_convex(Q);
+  LP_inf_params params;
+  params.prox_max_iter = 2;
+  MatrixXf Qlp = crf.lp_inference_prox_super_pixels(Qqp, params);
+  std::printf("qp_energy %.6f\n", crf.assignment_energy_true(crf.currentMap(Qqp)));
+  std::printf("lp_energy %.6f\n", crf.assignment_energy_true(crf.currentMap(Qlp)));
+  std::printf("qp_value %.6f\n", crf.compute_LR_QP_value(Qqp, MatrixXf(L, N)));
+  write_mat(argv[6], Qqp);
+  write_mat(argv[7], Qlp);
+
+  // the reference's exception for an ordered filter input outside [0,1]
+  MatrixXf f(2, N);
+  for (int j = 0; j < H; ++j)
+    for (int i = 0; i < W; ++i) {
+      f(0, j * W + i) = i / 3.f;
+      f(1, j * W + i) = j / 3.f;
+    }
+  PairwisePotential pw(f, new PottsCompatibility(3));
+  MatrixXf bad = Q, out;
+  bad(0, 0) = 1.5f;
+  try {
+    pw.apply_upper_minus_lower_ord(out, bad);
+    std::printf("no exception\n");
+  } catch (const std::runtime_error& e) {
+    std::printf("exception %s\n", e.what());
+  }
+  return 0;
+}

@@ -63,3 +63,19 @@ def test_python_api_raises_engine_error():
     import densecrf_ho_amd as D
     with pytest.raises(D.EngineError):
         D.DenseCRF(0, 5)
+
+
+def test_cpp_interface_compiles():
+    """include/densecrf_ho_amd.hpp compiles on its own, and against the
+    reference's Eigen when that is present (drop-in with Eigen::MatrixXf)."""
+    import subprocess
+    inc = os.path.join(ROOT, "include")
+    src = "#include \"densecrf_ho_amd.hpp\"\nint main(){ dcrf_ho::LP_inf_params p; return p.fw_max_iter == 10 ? 0 : 1; }\n"
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", inc, "-x", "c++", "-"], input=src, text=True,
+                       capture_output=True)
+    assert r.returncode == 0, r.stderr
+    eigen = "/root/reference/include"
+    if os.path.isdir(os.path.join(eigen, "Eigen")):
+        r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-DDCRF_USE_EIGEN", "-I", inc, "-I", eigen,
+                            "-x", "c++", "-"], input=src, text=True, capture_output=True)
+        assert r.returncode == 0, r.stderr
