@@ -192,12 +192,12 @@ QPResult CRF::qp(const float* init, float* out, int max_iter, bool ref_stop, int
   k_qp_e0<<<1, 1, 0, s>>>(sc, num_pairwise(), ho != nullptr, m.st_energy.get(), stats_cap);
   check_launch("k_qp_e0");
 
+  {
+    KTimer tm("qp_argmin", s);
+    k_argmin_rg<<<rg_grid(N_, L_, kRedBlocks), kBlock, 0, s>>>(gy, lab, N_, L_);
+    check_launch("k_argmin_rg");
+  }
   auto iteration = [&]() {
-    {
-      KTimer tm("qp_argmin", s);
-      k_argmin<<<grid_for(N_), kBlock, 0, s>>>(gy, lab, N_, L_, fl);
-      check_launch("k_argmin");
-    }
     if (ho) ho_qp_direction(s, Q, lab, sc + S_HO, fl);
     for (int k = 0; k < num_pairwise(); ++k) {
       Potential& p = pairwise(k);
@@ -226,9 +226,9 @@ QPResult CRF::qp(const float* init, float* out, int max_iter, bool ref_stop, int
     if (ho) ho_qp_zstep(s, sc, fl);
     {
       KTimer tm("qp_update", s);
-      k_qp_update<<<kRedBlocks, kBlock, 0, s>>>(U_.get(), Q, gy, psx, lab, ho ? ho_hb(ho) : nullptr,
-                                                 ho ? ho_smax(ho) : 0, NL, divL, sc, m.parts.get(), fl);
-      check_launch("k_qp_update");
+      k_qp_update_rg<<<kRedBlocks, kBlock, 0, s>>>(U_.get(), Q, gy, psx, lab, ho ? ho_hb(ho) : nullptr,
+                                                    ho ? ho_smax(ho) : 0, N_, L_, sc, m.parts.get(), fl);
+      check_launch("k_qp_update_rg");
     }
     finalize_sums<1><<<1, kBlock, 0, s>>>(m.parts.get(), kRedBlocks, sc + S_EPART);
     if (ho) ho_qp_after(s, Q, sc + S_HO, fl);

@@ -2,6 +2,8 @@
 // the engine's single translation unit.
 #pragma once
 
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace dcrf {
@@ -80,6 +82,38 @@ __device__ inline float eigen_sum(int n, int a, F val) {
 }
 // first aligned element of column `col` of an L-row matrix with aligned storage
 __device__ inline int col_astart(long col, int L) { return (int)((4 - ((col * L) & 3)) & 3); }
+
+// ---------------------------------------------------------------------------
+// Row groups: fields are rows of L <= 64 values per pixel.  A wave covers
+// gpw = floor(64/L) rows, lane -> (group g, value k), so global loads and
+// stores coalesce across lanes.  Per-row reductions go through a wave-private
+// 64-float LDS segment, read in the order the reference sums.  Kernels walk
+// tiles of kBlock/64 waves with a block-uniform grid-stride loop so that
+// __syncthreads() is reached by every thread.
+// ---------------------------------------------------------------------------
+struct RG {
+  int g, k;
+  long row;
+  bool valid;
+};
+__device__ inline RG rg_at(int L, long nrows, long tile) {
+  const int gpw = 64 / L;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  RG r;
+  r.g = lane / L;
+  r.k = lane - r.g * L;
+  r.row = (tile * (kBlock / 64) + wave) * gpw + r.g;
+  r.valid = r.g < gpw && r.row < nrows;
+  return r;
+}
+__host__ __device__ inline long rg_tiles(long nrows, int L) {
+  const long rows_per_tile = (long)(64 / L) * (kBlock / 64);
+  return (nrows + rows_per_tile - 1) / rows_per_tile;
+}
+inline unsigned rg_grid(long nrows, int L, long cap = 1 << 30) {
+  const long t = rg_tiles(nrows, L);
+  return (unsigned)std::max(1L, std::min(t, cap));
+}
 
 // expAndNormalize(-U) (src/densecrf.cpp:328-336), one thread per pixel
 __global__ void k_softmax_neg(const float* __restrict__ U, float* __restrict__ Q, int N, int L) {
@@ -160,6 +194,29 @@ __global__ void k_argmin(const float* __restrict__ g, int* __restrict__ lab, int
   lab[i] = m;
 }
 
+// first minimum of each row (descent_direction, src/qp.cpp:5-15), row groups
+__global__ void k_argmin_rg(const float* __restrict__ g, int* __restrict__ lab, int N, int L) {
+  __shared__ float seg[kBlock];
+  const long ntiles = rg_tiles(N, L);
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const RG r = rg_at(L, N, tile);
+    float* row = seg + (threadIdx.x & ~63) + r.g * L;
+    if (r.valid) row[r.k] = g[(size_t)r.row * L + r.k];
+    __syncthreads();
+    if (r.valid && r.k == 0) {
+      int m = 0;
+      float v = row[0];
+      for (int l = 1; l < L; ++l)
+        if (row[l] < v) {
+          v = row[l];
+          m = l;
+        }
+      lab[r.row] = m;
+    }
+    __syncthreads();
+  }
+}
+
 // line-search sums U.sx, Q.psisx, sx.psisx with sx = onehot(lab) - Q (:408-414)
 __global__ void k_qp_dots(const float* __restrict__ U, const float* __restrict__ Q, const int* __restrict__ lab,
                           const float* __restrict__ psx, size_t n, FastDiv divL, double* __restrict__ parts,
@@ -235,6 +292,52 @@ __global__ void k_qp_update(const float* __restrict__ U, float* __restrict__ Q, 
     gy[i] = g;
     v[0] += (double)q * (double)(g + U[i]);
   }
+  block_sum<1>(v);
+  if (threadIdx.x == 0) parts[blockIdx.x] = v[0];
+}
+
+// Q += step*sx; grad_y += 2*step*psisx + step*hb; partial Q.(grad_y+U)
+// (:424-434), then the first minimum of the new grad_y row: the next
+// iteration's conditional gradient (src/qp.cpp:5-15)
+__global__ void k_qp_update_rg(const float* __restrict__ U, float* __restrict__ Q, float* __restrict__ gy,
+                               const float* __restrict__ psx, int* __restrict__ lab, const float* __restrict__ hb,
+                               int hb_npix, int N, int L, const double* __restrict__ sc, double* __restrict__ parts,
+                               const int* __restrict__ fl) {
+  if (fl[kF_DONE]) return;
+  __shared__ float seg[kBlock];
+  const double step = sc[kS_STEP];
+  const float st = (float)step, st2 = (float)(2 * step);
+  double e = 0.0;
+  const long ntiles = rg_tiles(N, L);
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const RG r = rg_at(L, N, tile);
+    float* row = seg + (threadIdx.x & ~63) + r.g * L;
+    if (r.valid) {
+      const size_t idx = (size_t)r.row * L + r.k;
+      const float q0 = Q[idx];
+      const float sx = (lab[r.row] == r.k ? 1.f : 0.f) - q0;
+      const float q = q0 + st * sx;
+      const float h = (hb != nullptr && r.row < hb_npix) ? hb[idx] : 0.f;
+      const float g = gy[idx] + (st2 * psx[idx] + st * h);
+      Q[idx] = q;
+      gy[idx] = g;
+      e += (double)q * (double)(g + U[idx]);
+      row[r.k] = g;
+    }
+    __syncthreads();
+    if (r.valid && r.k == 0) {
+      int m = 0;
+      float v = row[0];
+      for (int l = 1; l < L; ++l)
+        if (row[l] < v) {
+          v = row[l];
+          m = l;
+        }
+      lab[r.row] = m;
+    }
+    __syncthreads();
+  }
+  double v[1] = {e};
   block_sum<1>(v);
   if (threadIdx.x == 0) parts[blockIdx.x] = v[0];
 }

@@ -430,6 +430,110 @@ __global__ void k_slice(int V, RowMap rm, int N, int d1, const int* __restrict__
   }
 }
 
+// k_combine that does nothing unless *need (the upper pass without edges)
+__global__ void k_combine_if(int V, RowMap rm, const int* __restrict__ multi, int nmulti,
+                             const int* __restrict__ chstart, const int* __restrict__ pstart,
+                             const float* __restrict__ part, float* __restrict__ values, const int* __restrict__ need) {
+  if (need && *need == 0) return;
+  long m;
+  int k0, ks;
+  rm.at(V, m, k0, ks);
+  if (m < 0 || m >= nmulti) return;
+  const int p = multi[m];
+  const int c0 = pstart[p], c1 = c0 + chstart[p + 1] - chstart[p];
+  for (int k = k0; k < V; k += ks) {
+    float acc = part[(size_t)c0 * V + k];
+    for (int c = c0 + 1; c < c1; ++c) acc = acc + part[(size_t)c * V + k];
+    values[(size_t)(p + 1) * V + k] = acc;
+  }
+}
+__global__ void k_fill_f(float* __restrict__ p, size_t n, float v) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+// ---- staged variants (V >= kStageMinV) ------------------------------------------
+// The group's V lanes copy V consecutive splat entries (pixel, weight) into a
+// block-shared LDS slice per round; the entry loop then reads LDS and issues
+// the value gathers back to back.  The summation order (pixel order) is the
+// same as the unstaged loop.
+constexpr int kStageMinV = 8;
+
+template <SplatIn KIND>
+__global__ void k_splat_staged(SplatArgs a, int V, RowMap rm, const int* __restrict__ sp_pix,
+                               const float* __restrict__ sp_w, const int* __restrict__ seg,
+                               const int* __restrict__ chstart, const int* __restrict__ chpoint,
+                               const int* __restrict__ pstart, int nchunks, float* __restrict__ values,
+                               float* __restrict__ part) {
+  __shared__ int spx[kBlock];
+  __shared__ float swt[kBlock];
+  long c;
+  int k0, ks;
+  rm.at(V, c, k0, ks);
+  const bool act = c >= 0 && c < nchunks;
+  const int gb = (threadIdx.x & ~63) + (k0 >= 0 ? ((threadIdx.x & 63) / V) * V : 0);
+  int b = 0, e = 0, p = 0, c0 = 0, nc = 1;
+  if (act) {
+    p = chpoint[c];
+    c0 = chstart[p];
+    nc = chstart[p + 1] - c0;
+    b = seg[p] + (int)(c - c0) * kChunk;
+    e = min(seg[p + 1], b + kChunk);
+  }
+  float acc = 0.f;
+  const int rounds = (kChunk + V - 1) / V;
+  for (int r = 0; r < rounds; ++r) {
+    const int t0 = b + r * V;
+    if (act) {
+      const int t = t0 + k0;
+      spx[gb + k0] = t < e ? sp_pix[t] : 0;
+      swt[gb + k0] = t < e ? sp_w[t] : 0.f;
+    }
+    __syncthreads();
+    if (act) {
+      const int n = min(V, e - t0);
+#pragma unroll 8
+      for (int j = 0; j < n; ++j) acc = acc + swt[gb + j] * splat_in<KIND>(a, spx[gb + j], k0, V);
+    }
+    __syncthreads();
+  }
+  if (act) {
+    float* dst = nc == 1 ? values + (size_t)(p + 1) * V : part + (size_t)(pstart[p] + (c - c0)) * V;
+    dst[k0] = acc;
+  }
+}
+
+// slice with the pixel's d+1 (offset, weight) pairs staged in LDS (V >= d+1)
+template <bool SEQ>
+__global__ void k_slice_staged(int V, RowMap rm, int N, int d1, const int* __restrict__ offset,
+                               const float* __restrict__ bary, const float* __restrict__ values, float alpha,
+                               SliceArgs o) {
+  __shared__ int soff[kBlock];
+  __shared__ float sbw[kBlock];
+  long pix;
+  int k0, ks;
+  rm.at(V, pix, k0, ks);
+  const bool act = pix >= 0 && pix < N;
+  const int gb = (threadIdx.x & ~63) + ((threadIdx.x & 63) / V) * V;
+  if (act && k0 < d1) {
+    soff[gb + k0] = offset[(size_t)pix * d1 + k0] + 1;
+    sbw[gb + k0] = bary[(size_t)pix * d1 + k0];
+  }
+  __syncthreads();
+  if (!act) return;
+  float acc = 0.f;
+#pragma unroll 4
+  for (int j = 0; j < d1; ++j) {
+    const float w = sbw[gb + j];
+    const float v = values[(size_t)soff[gb + j] * V + k0];
+    if (SEQ)
+      acc = acc + (w * v) * alpha;
+    else
+      acc = acc + (w * alpha) * v;
+  }
+  slice_store(o, (int)pix, k0, V, acc);
+}
+
 // ---- ordered filter ------------------------------------------------------------
 // bins of src/permutohedral.cpp:548-589
 __device__ inline int bin_lo(float q) { return min((int)floorf(q * 10.f), 9); }
@@ -440,68 +544,137 @@ __device__ inline float rescaled(const float* Q, size_t idx, const float* mm) {
   return q;
 }
 
-// per (chunk, label): 10 lower bins (+w to bins 0..bin_lo) and 10 upper bins
-// (+w to bins bin_up..9), accumulated in pixel order
-__global__ void k_osplat(const float* __restrict__ Q, const float* __restrict__ mm, int L, RowMap rm,
+// Ordered filter state: LO[point][label][10] holds the lower pass (+w to bins
+// 0..bin_lo, :548-569).  The upper pass (+w to bins bin_up..9) equals
+// T - LO[bin+1] (T = the point's total weight, lo[10] = 0) whenever
+// bin_up == bin_lo for every (pixel, label); the splat raises `edge` when
+// some q sits on a bin edge (bin_up == bin_lo - 1), and then the upper pass
+// is splatted and blurred in full into UP[point][label][10].
+__global__ void k_osplat(const float* __restrict__ Q, const float* __restrict__ mm, int L, RowMap rm, int upper,
                          const int* __restrict__ sp_pix, const float* __restrict__ sp_w, const int* __restrict__ seg,
                          const int* __restrict__ chstart, const int* __restrict__ chpoint,
                          const int* __restrict__ pstart, int nchunks, float* __restrict__ values,
-                         float* __restrict__ part) {
+                         float* __restrict__ part, int* __restrict__ edge) {
+  __shared__ int spx[kBlock];
+  __shared__ float swt[kBlock];
+  if (upper && *edge == 0) return;
   long c;
   int k0, ks;
   rm.at(L, c, k0, ks);
-  if (c < 0 || c >= nchunks) return;
-  const int p = chpoint[c];
-  const int c0 = chstart[p], nc = chstart[p + 1] - c0;
-  const int b = seg[p] + (int)(c - c0) * kChunk;
-  const int e = min(seg[p + 1], b + kChunk);
-  float* dst = nc == 1 ? values + (size_t)(p + 1) * L * 20 : part + (size_t)(pstart[p] + (c - c0)) * L * 20;
-  for (int k = k0; k < L; k += ks) {
-    float acc[20];
+  const bool act = c >= 0 && c < nchunks;
+  const int gb = (threadIdx.x & ~63) + ((threadIdx.x & 63) / L) * L;
+  int b = 0, e = 0, p = 0, c0 = 0, nc = 1;
+  if (act) {
+    p = chpoint[c];
+    c0 = chstart[p];
+    nc = chstart[p + 1] - c0;
+    b = seg[p] + (int)(c - c0) * kChunk;
+    e = min(seg[p + 1], b + kChunk);
+  }
+  float acc[10];
 #pragma unroll
-    for (int q = 0; q < 20; ++q) acc[q] = 0.f;
-    for (int t = b; t < e; ++t) {
-      const int pix = sp_pix[t];
-      const float w = sp_w[t];
-      const float q = rescaled(Q, (size_t)pix * L + k, mm);
-      const int lo = bin_lo(q), up = bin_up(q);
+  for (int q = 0; q < 10; ++q) acc[q] = 0.f;
+  bool any_edge = false;
+  const int rounds = (kChunk + L - 1) / L;
+  for (int r = 0; r < rounds; ++r) {
+    const int t0 = b + r * L;
+    if (act) {
+      const int t = t0 + k0;
+      spx[gb + k0] = t < e ? sp_pix[t] : 0;
+      swt[gb + k0] = t < e ? sp_w[t] : 0.f;
+    }
+    __syncthreads();
+    if (act) {
+      const int n = min(L, e - t0);
+      for (int j = 0; j < n; ++j) {
+        const float w = swt[gb + j];
+        const float q = rescaled(Q, (size_t)spx[gb + j] * L + k0, mm);
+        const int lo = bin_lo(q), up = bin_up(q);
+        any_edge = any_edge || (lo != up);
+        if (upper) {
 #pragma unroll
-      for (int r = 0; r < 10; ++r) {
-        acc[r] = acc[r] + (r <= lo ? w : 0.f);
-        acc[10 + r] = acc[10 + r] + (r >= up ? w : 0.f);
+          for (int r2 = 0; r2 < 10; ++r2) acc[r2] = acc[r2] + (r2 >= up ? w : 0.f);
+        } else {
+#pragma unroll
+          for (int r2 = 0; r2 < 10; ++r2) acc[r2] = acc[r2] + (r2 <= lo ? w : 0.f);
+        }
       }
     }
-    float4* d4 = reinterpret_cast<float4*>(dst + (size_t)k * 20);
+    __syncthreads();
+  }
+  if (!upper && any_edge) atomicOr(edge, 1);
+  if (act) {
+    float* dst = nc == 1 ? values + ((size_t)(p + 1) * L + k0) * 10
+                         : part + ((size_t)(pstart[p] + (c - c0)) * L + k0) * 10;
+    float2* d2 = reinterpret_cast<float2*>(dst);
 #pragma unroll
-    for (int q = 0; q < 5; ++q) d4[q] = make_float4(acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]);
+    for (int q = 0; q < 5; ++q) d2[q] = make_float2(acc[2 * q], acc[2 * q + 1]);
   }
 }
 
-// lower slice then upper slice (:638-650, :686-696), then out (+)= scale*x
+// one blur axis over rows of R floats (R even), float2 wide; skip == 1 makes it a no-op
+__global__ void k_blur2(int R2, int M, const int* __restrict__ nbr, const float2* __restrict__ src,
+                        float2* __restrict__ dst, const int* __restrict__ need) {
+  if (need && *need == 0) return;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)M * R2) return;
+  const int i = (int)(t / R2), k = (int)(t - (size_t)i * R2);
+  const int n1 = nbr[2 * i] + 1, n2 = nbr[2 * i + 1] + 1;
+  const float2 o = src[(size_t)(i + 1) * R2 + k];
+  const float2 a = src[(size_t)n1 * R2 + k];
+  const float2 b = src[(size_t)n2 * R2 + k];
+  float2 r;
+  r.x = o.x + 0.5f * (a.x + b.x);
+  r.y = o.y + 0.5f * (a.y + b.y);
+  dst[(size_t)(i + 1) * R2 + k] = r;
+}
+
+// lower slice then upper slice (:638-650, :686-696), then out (+)= scale*x.
+// Without edges the upper value at bin c is T - LO[c+1].
 __global__ void k_oslice(const float* __restrict__ Q, const float* __restrict__ mm, int L, RowMap rm, int N, int d1,
                          const int* __restrict__ offset, const float* __restrict__ bary,
-                         const float* __restrict__ values, float alpha, SliceArgs o) {
+                         const float* __restrict__ lo_v, const float* __restrict__ up_v, const float* __restrict__ Tb,
+                         const int* __restrict__ edge, float alpha, SliceArgs o) {
+  __shared__ int soff[kBlock];
+  __shared__ float sbw[kBlock];
   long pix;
   int k0, ks;
   rm.at(L, pix, k0, ks);
-  if (pix < 0 || pix >= N) return;
-  for (int k = k0; k < L; k += ks) {
-    const float q = rescaled(Q, (size_t)pix * L + k, mm);
-    const int lo = bin_lo(q), up = bin_up(q);
-    float acc = 0.f;
-    for (int j = 0; j < d1; ++j) {
-      const int off = offset[(size_t)pix * d1 + j] + 1;
-      const float w = bary[(size_t)pix * d1 + j];
-      acc = acc + values[((size_t)off * L + k) * 20 + lo] * (-w * alpha);
-    }
-    for (int j = 0; j < d1; ++j) {
-      const int off = offset[(size_t)pix * d1 + j] + 1;
-      const float w = bary[(size_t)pix * d1 + j];
-      acc = acc + values[((size_t)off * L + k) * 20 + 10 + up] * (w * alpha);
-    }
-    slice_store(o, (int)pix, k, L, acc);
+  const bool act = pix >= 0 && pix < N;
+  const int gb = (threadIdx.x & ~63) + ((threadIdx.x & 63) / L) * L;
+  const bool staged = L >= d1;  // otherwise the group has too few lanes/slots
+  if (staged && act && k0 < d1) {
+    soff[gb + k0] = offset[(size_t)pix * d1 + k0] + 1;
+    sbw[gb + k0] = bary[(size_t)pix * d1 + k0];
   }
+  __syncthreads();
+  if (!act) return;
+  auto voff = [&](int j) { return staged ? soff[gb + j] : offset[(size_t)pix * d1 + j] + 1; };
+  auto vw = [&](int j) { return staged ? sbw[gb + j] : bary[(size_t)pix * d1 + j]; };
+  const bool full = *edge != 0;
+  const float q = rescaled(Q, (size_t)pix * L + k0, mm);
+  const int lo = bin_lo(q), up = bin_up(q);
+  float acc = 0.f;
+#pragma unroll 4
+  for (int j = 0; j < d1; ++j) {
+    const float w = vw(j);
+    acc = acc + lo_v[((size_t)voff(j) * L + k0) * 10 + lo] * (-w * alpha);
+  }
+#pragma unroll 4
+  for (int j = 0; j < d1; ++j) {
+    const int of = voff(j);
+    const float w = vw(j);
+    float u;
+    if (full)
+      u = up_v[((size_t)of * L + k0) * 10 + up];
+    else
+      u = Tb[of] - (up < 9 ? lo_v[((size_t)of * L + k0) * 10 + up + 1] : 0.f);
+    acc = acc + u * (w * alpha);
+  }
+  slice_store(o, (int)pix, k0, L, acc);
 }
+
+__global__ void k_zero_int(int* p) { *p = 0; }
 
 __global__ void k_zero(float* p, size_t n) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -687,12 +860,18 @@ void Lattice::filter(const SplatArgs& in, const SliceArgs& out, int V, bool reve
   const std::string tag = prof_on() ? "[d=" + std::to_string(d_) + ",V=" + std::to_string(V) + "]" : std::string();
   {
   KTimer tm("splat" + tag, s);
+  const bool stage = V >= kStageMinV && V <= 64;
   switch (in.kind) {
 #define DCRF_SPLAT(K)                                                                                             \
   case K:                                                                                                         \
-    k_splat<K><<<rows_grid(nchunks_, V), kBlock, 0, s>>>(in, V, rm, sp_pix_.get(), sp_w_.get(), seg_.get(),     \
-                                                         chstart_.get(), chpoint_.get(), pstart_.get(), nchunks_, \
-                                                         src, part_.get());                                     \
+    if (stage)                                                                                                    \
+      k_splat_staged<K><<<rows_grid(nchunks_, V), kBlock, 0, s>>>(in, V, rm, sp_pix_.get(), sp_w_.get(),        \
+                                                                  seg_.get(), chstart_.get(), chpoint_.get(),    \
+                                                                  pstart_.get(), nchunks_, src, part_.get());    \
+    else                                                                                                          \
+      k_splat<K><<<rows_grid(nchunks_, V), kBlock, 0, s>>>(in, V, rm, sp_pix_.get(), sp_w_.get(), seg_.get(),   \
+                                                           chstart_.get(), chpoint_.get(), pstart_.get(),        \
+                                                           nchunks_, src, part_.get());                          \
     break;
     DCRF_SPLAT(SplatIn::kPlain)
     DCRF_SPLAT(SplatIn::kScaled)
@@ -722,46 +901,102 @@ void Lattice::filter(const SplatArgs& in, const SliceArgs& out, int V, bool reve
   }
   const float alpha = lattice_alpha(d_);
   KTimer tm("slice" + tag, s);
-  if (seq)
+  if (V >= d_ + 1 && V <= 64) {
+    if (seq)
+      k_slice_staged<true><<<rows_grid(N_, V), kBlock, 0, s>>>(V, rm, N_, d_ + 1, offset_.get(), bary_.get(), src,
+                                                               alpha, out);
+    else
+      k_slice_staged<false><<<rows_grid(N_, V), kBlock, 0, s>>>(V, rm, N_, d_ + 1, offset_.get(), bary_.get(), src,
+                                                                alpha, out);
+  } else if (seq)
     k_slice<true><<<rows_grid(N_, V), kBlock, 0, s>>>(V, rm, N_, d_ + 1, offset_.get(), bary_.get(), src, alpha, out);
   else
     k_slice<false><<<rows_grid(N_, V), kBlock, 0, s>>>(V, rm, N_, d_ + 1, offset_.get(), bary_.get(), src, alpha, out);
   check_launch("k_slice");
 }
 
-void Lattice::ordered(const float* Q, const float* d_minmax, int L, const SliceArgs& out) {
-  DCRF_REQUIRE(L > 0, "ordered filter: label count must be positive");
-  const int row = L * 20;
-  ensure_values(row);
+// total splatted weight per point, blurred with the ordered filter's float
+// arithmetic (:631); the upper pass of the ordered filter is T - LO[c+1]
+void Lattice::ensure_total_weight() {
+  if (have_tb_) return;
   hipStream_t s = stream_;
-  const RowMap rm = rowmap(L);
+  ensure_values(1);
+  DBuf<float> ones(N_);
+  k_fill_f<<<grid_for(N_), kBlock, 0, s>>>(ones.get(), (size_t)N_, 1.f);
+  SplatArgs a;
+  a.kind = SplatIn::kPlain;
+  a.in = ones.get();
+  const RowMap rm = rowmap(1);
   float* src = vA_.get();
   float* dst = vB_.get();
-  const std::string tag = prof_on() ? "[d=" + std::to_string(d_) + ",L=" + std::to_string(L) + "]" : std::string();
-  {
-  KTimer tm("osplat" + tag, s);
-  k_osplat<<<rows_grid(nchunks_, L), kBlock, 0, s>>>(Q, d_minmax, L, rm, sp_pix_.get(), sp_w_.get(), seg_.get(),
-                                                     chstart_.get(), chpoint_.get(), pstart_.get(), nchunks_, src,
-                                                     part_.get());
-  check_launch("k_osplat");
-  if (nmulti_ > 0) {
-    const RowMap rw = rowmap(row);
-    k_combine<<<rows_grid(nmulti_, row), kBlock, 0, s>>>(row, rw, multi_.get(), nmulti_, chstart_.get(),
-                                                         pstart_.get(), part_.get(), src);
-    check_launch("k_combine");
-  }
-  }
-  for (int j = 0; j <= d_; ++j) {  // forward order only (:621, :669)
-    KTimer tm("oblur" + tag, s);
-    k_blur4<<<grid_for((size_t)M_ * 64), kBlock, 0, s>>>(row / 4, M_, nbr_.get() + (size_t)j * M_ * 2,
-                                                         reinterpret_cast<const float4*>(src),
-                                                         reinterpret_cast<float4*>(dst));
-    check_launch("k_blur4");
+  k_splat<SplatIn::kPlain><<<rows_grid(nchunks_, 1), kBlock, 0, s>>>(a, 1, rm, sp_pix_.get(), sp_w_.get(), seg_.get(),
+                                                                     chstart_.get(), chpoint_.get(), pstart_.get(),
+                                                                     nchunks_, src, part_.get());
+  if (nmulti_ > 0)
+    k_combine<<<rows_grid(nmulti_, 1), kBlock, 0, s>>>(1, rm, multi_.get(), nmulti_, chstart_.get(), pstart_.get(),
+                                                       part_.get(), src);
+  for (int j = 0; j <= d_; ++j) {
+    k_blur<false><<<rows_grid(M_, 1), kBlock, 0, s>>>(1, rm, M_, nbr_.get() + (size_t)j * M_ * 2, src, dst);
     std::swap(src, dst);
   }
+  tb_.resize(M_ + 1);
+  DCRF_HIP(hipMemcpyAsync(tb_.get(), src, (M_ + 1) * sizeof(float), hipMemcpyDeviceToDevice, s));
+  edge_.resize(1);
+  DCRF_HIP(hipStreamSynchronize(s));
+  check_launch("total weight");
+  have_tb_ = true;
+}
+
+void Lattice::ordered(const float* Q, const float* d_minmax, int L, const SliceArgs& out) {
+  DCRF_REQUIRE(L > 0, "ordered filter: label count must be positive");
+  DCRF_REQUIRE(L <= 64, "ordered filter: at most 64 labels");
+  ensure_total_weight();
+  const int row = L * 10;
+  ensure_values(2 * row);  // LO and UP halves of each buffer
+  const size_t half = (size_t)(M_ + 1) * row;
+  hipStream_t s = stream_;
+  const RowMap rm = rowmap(L);
+  float* srcL = vA_.get();
+  float* dstL = vB_.get();
+  float* srcU = vA_.get() + half;
+  float* dstU = vB_.get() + half;
+  // row 0 of each half is the zero row
+  k_zero<<<grid_for(row), kBlock, 0, s>>>(vA_.get() + half, row);
+  k_zero<<<grid_for(row), kBlock, 0, s>>>(vB_.get() + half, row);
+  int* edge = edge_.get();
+  const std::string tag = prof_on() ? "[d=" + std::to_string(d_) + ",L=" + std::to_string(L) + "]" : std::string();
+  {
+    KTimer tm("osplat" + tag, s);
+    k_zero_int<<<1, 1, 0, s>>>(edge);
+    for (int upper = 0; upper < 2; ++upper) {
+      float* dst_half = upper ? srcU : srcL;
+      k_osplat<<<rows_grid(nchunks_, L), kBlock, 0, s>>>(Q, d_minmax, L, rm, upper, sp_pix_.get(), sp_w_.get(),
+                                                         seg_.get(), chstart_.get(), chpoint_.get(), pstart_.get(),
+                                                         nchunks_, dst_half, part_.get(), edge);
+      if (nmulti_ > 0) {
+        const RowMap rw = rowmap(row);
+        k_combine_if<<<rows_grid(nmulti_, row), kBlock, 0, s>>>(row, rw, multi_.get(), nmulti_, chstart_.get(),
+                                                               pstart_.get(), part_.get(), dst_half,
+                                                               upper ? edge : nullptr);
+      }
+    }
+    check_launch("k_osplat");
+  }
+  const int R2 = row / 2;
+  for (int j = 0; j <= d_; ++j) {  // forward order only (:621, :669)
+    KTimer tm("oblur" + tag, s);
+    const int* nb = nbr_.get() + (size_t)j * M_ * 2;
+    k_blur2<<<grid_for((size_t)M_ * R2), kBlock, 0, s>>>(R2, M_, nb, reinterpret_cast<const float2*>(srcL),
+                                                         reinterpret_cast<float2*>(dstL), nullptr);
+    k_blur2<<<grid_for((size_t)M_ * R2), kBlock, 0, s>>>(R2, M_, nb, reinterpret_cast<const float2*>(srcU),
+                                                         reinterpret_cast<float2*>(dstU), edge);
+    check_launch("k_blur2");
+    std::swap(srcL, dstL);
+    std::swap(srcU, dstU);
+  }
   KTimer tm("oslice" + tag, s);
-  k_oslice<<<rows_grid(N_, L), kBlock, 0, s>>>(Q, d_minmax, L, rm, N_, d_ + 1, offset_.get(), bary_.get(), src,
-                                               lattice_alpha(d_), out);
+  k_oslice<<<rows_grid(N_, L), kBlock, 0, s>>>(Q, d_minmax, L, rm, N_, d_ + 1, offset_.get(), bary_.get(), srcL,
+                                               srcU, tb_.get(), edge, lattice_alpha(d_), out);
   check_launch("k_oslice");
 }
 

@@ -73,6 +73,7 @@ class Lattice {
 
  private:
   void ensure_values(size_t floats);
+  void ensure_total_weight();
   hipStream_t stream_;
   int N_ = 0, d_ = 0, Np_ = 0, M_ = 0;
   int nchunks_ = 0, nmulti_ = 0, npart_ = 0;
@@ -91,6 +92,10 @@ class Lattice {
   // value buffers (row 0 is the zero row standing for absent neighbours)
   DBuf<float> vA_, vB_, part_;
   size_t vcap_ = 0;
+  // ordered filter: blurred total weight per point and the bin-edge flag
+  DBuf<float> tb_;
+  DBuf<int> edge_;
+  bool have_tb_ = false;
 };
 
 }  // namespace dcrf

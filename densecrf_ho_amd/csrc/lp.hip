@@ -89,38 +89,41 @@ struct LPConst {
 };
 
 // alpha/u update of the previous step (:737-738, fused here), then
-// Y = lambda * (tmp - sum(tmp)/M) + cur_Q with tmp = alpha + u - U (:627-635)
-template <int MAXL>
+// Y = lambda * (tmp - sum(tmp)/M) + cur_Q with tmp = alpha + u - U (:627-635).
+// Row groups; sum(tmp) is the sum of an aligned copy (qp_gamma_multiplyC).
 __global__ void k_lp_Y(int N, int L, LPConst c, float* __restrict__ alpha, float* __restrict__ u,
                        const float* __restrict__ sa, const float* __restrict__ su, const double* __restrict__ sc,
                        int apply_update, const float* __restrict__ U, const float* __restrict__ curQ,
                        float* __restrict__ Y) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const size_t o = (size_t)i * L;
+  __shared__ float seg[kBlock];
   const float dl = apply_update ? (float)sc[kL_DELTA] : 0.f;
-  float t[MAXL];
-#pragma unroll
-  for (int l = 0; l < MAXL; ++l)
-    if (l < L) {
-      float a = alpha[o + l];
-      float uu = u ? u[o + l] : 0.f;
+  const long ntiles = rg_tiles(N, L);
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const RG r = rg_at(L, N, tile);
+    float* row = seg + (threadIdx.x & ~63) + r.g * L;
+    const size_t idx = (size_t)r.row * L + r.k;
+    float t = 0.f;
+    if (r.valid) {
+      float a = alpha[idx];
+      float uu = u ? u[idx] : 0.f;
       if (apply_update) {
-        a = a + dl * (sa[o + l] - a);
-        alpha[o + l] = a;
+        a = a + dl * (sa[idx] - a);
+        alpha[idx] = a;
         if (u) {
-          uu = uu + dl * (su[o + l] - uu);
-          u[o + l] = uu;
+          uu = uu + dl * (su[idx] - uu);
+          u[idx] = uu;
         }
       }
-      t[l] = (a + uu) - U[o + l];
-    } else {
-      t[l] = 0.f;
+      t = (a + uu) - U[idx];
+      row[r.k] = t;
     }
-  const float s = eigen_sum_reg<MAXL>(t, L);
-#pragma unroll
-  for (int l = 0; l < MAXL; ++l)
-    if (l < L) Y[o + l] = ((c.c_m1_over_M * s + t[l]) * c.lambda) + curQ[o + l];
+    __syncthreads();
+    if (r.valid) {
+      const float S = eigen_sum(L, 0, [&](int i) { return row[i]; });
+      Y[idx] = ((c.c_m1_over_M * S + t) * c.lambda) + curQ[idx];
+    }
+    __syncthreads();
+  }
 }
 
 // one qp-gamma sweep of a pixel (qp_gamma_step :495-514, qp_gamma_multiplyC
@@ -147,33 +150,46 @@ __device__ inline float qp_sweep(float (&g)[MAXL], const float (&y)[MAXL], int L
 
 // pass 1 (write == 0): `nsw` sweeps from the stored gamma, per-sweep block
 // partial sums of qp_values, gamma not written.  pass 2 (write == 1): replays
-// exactly fl[kLF_SWEEPS] sweeps and stores gamma.
+// exactly fl[kLF_SWEEPS] sweeps and stores gamma.  One thread per pixel holds
+// its row in registers; rows move through LDS so that global accesses of the
+// block's kBlock x L tile coalesce.
 template <int MAXL>
 __global__ void k_lp_sweeps(int N, int L, LPConst c, float* __restrict__ gamma, const float* __restrict__ Y, int nsw,
                             int write, double* __restrict__ parts, const int* __restrict__ fl) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool act = i < N;
+  __shared__ float tile[kBlock * MAXL];
   if (write && fl[kLF_SWEEPS] == 0) return;
   const int n = write ? fl[kLF_SWEEPS] : nsw;
-  const size_t o = (size_t)i * L;
+  const int p0 = blockIdx.x * kBlock;
+  const int np = min(kBlock, N - p0);
+  const size_t base = (size_t)p0 * L;
+  const int t = threadIdx.x;
+  const bool act = t < np;
   float g[MAXL], y[MAXL];
+  for (int i = t; i < np * L; i += kBlock) tile[i] = gamma[base + i];
+  __syncthreads();
 #pragma unroll
-  for (int l = 0; l < MAXL; ++l) {
-    g[l] = (act && l < L) ? gamma[o + l] : 0.f;
-    y[l] = (act && l < L) ? Y[o + l] : 0.f;
-  }
+  for (int l = 0; l < MAXL; ++l) g[l] = (act && l < L) ? tile[t * L + l] : 0.f;
+  __syncthreads();
+  for (int i = t; i < np * L; i += kBlock) tile[i] = Y[base + i];
+  __syncthreads();
+#pragma unroll
+  for (int l = 0; l < MAXL; ++l) y[l] = (act && l < L) ? tile[t * L + l] : 0.f;
   for (int sw = 0; sw < n; ++sw) {
     const float v = act ? qp_sweep<MAXL>(g, y, L, c) : 0.f;
     if (!write) {
       double vv[1] = {(double)v};
       block_sum<1>(vv);
-      if (threadIdx.x == 0) parts[(size_t)sw * gridDim.x + blockIdx.x] = vv[0];
+      if (t == 0) parts[(size_t)sw * gridDim.x + blockIdx.x] = vv[0];
     }
   }
-  if (write && act)
+  if (write) {
+    __syncthreads();
 #pragma unroll
     for (int l = 0; l < MAXL; ++l)
-      if (l < L) gamma[o + l] = g[l];
+      if (act && l < L) tile[t * L + l] = g[l];
+    __syncthreads();
+    for (int i = t; i < np * L; i += kBlock) gamma[base + i] = tile[i];
+  }
 }
 
 // stopping sweep of the batch (:670-675): the first sweep whose float total
@@ -183,12 +199,27 @@ __global__ void k_lp_sweeps(int N, int L, LPConst c, float* __restrict__ gamma, 
 __global__ void k_lp_sweep_decide(const double* __restrict__ parts, int nblocks, int nsw, float qp_tol, int done_before,
                                   int max_total, double* __restrict__ sc, int* __restrict__ fl) {
   __shared__ double tot[kSweepBatch];
-  for (int sw = 0; sw < nsw; ++sw) {
-    double v[1] = {0.0};
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) v[0] += parts[(size_t)sw * nblocks + b];
-    block_sum<1>(v);
-    if (threadIdx.x == 0) tot[sw] = v[0];
+  __shared__ double red[kSweepBatch][1024 / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double acc[kSweepBatch];
+#pragma unroll
+  for (int sw = 0; sw < kSweepBatch; ++sw) acc[sw] = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+#pragma unroll
+    for (int sw = 0; sw < kSweepBatch; ++sw)
+      if (sw < nsw) acc[sw] += parts[(size_t)sw * nblocks + b];
+#pragma unroll
+  for (int sw = 0; sw < kSweepBatch; ++sw) {
+    const double w = wave_sum(acc[sw]);
+    if (lane == 0) red[sw][wid] = w;
   }
+  __syncthreads();
+  if (threadIdx.x < (unsigned)nsw) {
+    double v = 0;
+    for (int w = 0; w < (int)blockDim.x / 64; ++w) v += red[threadIdx.x][w];
+    tot[threadIdx.x] = v;
+  }
+  __syncthreads();
   if (threadIdx.x != 0) return;
   float prev = done_before == 0 ? 3.402823466e38f : (float)sc[kL_QPV];
   int stop = nsw, finished = 0;
@@ -207,39 +238,51 @@ __global__ void k_lp_sweep_decide(const double* __restrict__ parts, int nblocks,
   fl[kLF_SWEEP_DONE] = finished;
 }
 
+// NaN-propagating min / max (Eigen's minCoeff/maxCoeff see NaN as unordered;
+// any NaN makes the rescale fail here)
+__device__ inline float nmin(float a, float b) { return (a != a || b != b) ? NAN : fminf(a, b); }
+__device__ inline float nmax(float a, float b) { return (a != a || b != b) ? NAN : fmaxf(a, b); }
+
 // beta = -colsum(alpha+u+gamma-U)/M, Q = lambda*(alpha+beta+u+gamma-U) + cur_Q
-// (:684-697); block min/max of Q for rescale (:699)
+// (:684-697), row groups; the column sum follows Eigen's order for the
+// column's alignment.  Block min/max of Q for rescale (:699).
 __global__ void k_lp_betaQ(int N, int L, float lambda, const float* __restrict__ alpha, const float* __restrict__ u,
                            const float* __restrict__ gamma, const float* __restrict__ U,
                            const float* __restrict__ curQ, float* __restrict__ Q, float* __restrict__ mmparts) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  float mn = INFINITY, mx = -INFINITY;
-  if (i < N) {
-    const size_t o = (size_t)i * L;
-    auto bm = [&](int l) {
-      const float uu = u ? u[o + l] : 0.f;
-      return ((alpha[o + l] + uu) + gamma[o + l]) - U[o + l];
-    };
-    float b = -eigen_sum(L, col_astart(i, L), bm);
-    b = b / (float)L;
-    for (int l = 0; l < L; ++l) {
-      const float uu = u ? u[o + l] : 0.f;
-      const float q = lambda * ((((alpha[o + l] + b) + uu) + gamma[o + l]) - U[o + l]) + curQ[o + l];
-      Q[o + l] = q;
-      mn = fminf(mn, q);
-      mx = fmaxf(mx, q);
-      if (q != q) mn = mx = q;
-    }
-  }
+  __shared__ float seg[kBlock];
   __shared__ float smn[kBlock], smx[kBlock];
+  float mn = INFINITY, mx = -INFINITY;
+  const long ntiles = rg_tiles(N, L);
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const RG r = rg_at(L, N, tile);
+    float* row = seg + (threadIdx.x & ~63) + r.g * L;
+    const size_t idx = (size_t)r.row * L + r.k;
+    float a = 0.f, uu = 0.f, gm = 0.f, uv = 0.f;
+    if (r.valid) {
+      a = alpha[idx];
+      uu = u ? u[idx] : 0.f;
+      gm = gamma[idx];
+      uv = U[idx];
+      row[r.k] = ((a + uu) + gm) - uv;
+    }
+    __syncthreads();
+    if (r.valid) {
+      float b = -eigen_sum(L, col_astart(r.row, L), [&](int i) { return row[i]; });
+      b = b / (float)L;
+      const float q = lambda * ((((a + b) + uu) + gm) - uv) + curQ[idx];
+      Q[idx] = q;
+      mn = nmin(mn, q);
+      mx = nmax(mx, q);
+    }
+    __syncthreads();
+  }
   smn[threadIdx.x] = mn;
   smx[threadIdx.x] = mx;
   __syncthreads();
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      const float a = smn[threadIdx.x + s], b2 = smx[threadIdx.x + s];
-      smn[threadIdx.x] = (a != a || smn[threadIdx.x] != smn[threadIdx.x]) ? NAN : fminf(smn[threadIdx.x], a);
-      smx[threadIdx.x] = (b2 != b2 || smx[threadIdx.x] != smx[threadIdx.x]) ? NAN : fmaxf(smx[threadIdx.x], b2);
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      smn[threadIdx.x] = nmin(smn[threadIdx.x], smn[threadIdx.x + o]);
+      smx[threadIdx.x] = nmax(smx[threadIdx.x], smx[threadIdx.x + o]);
     }
     __syncthreads();
   }
@@ -250,22 +293,31 @@ __global__ void k_lp_betaQ(int N, int L, float lambda, const float* __restrict__
 }
 
 // rescale (src/eigen_utils.cpp:120-140): mm = {min, max(Q - min)}; error when
-// max < 0 (only possible with NaN here)
+// max < 0 (only possible with NaN here).  One block.
 __global__ void k_lp_minmax_final(const float* __restrict__ parts, int nb, float* __restrict__ mm,
                                   int* __restrict__ fl) {
-  if (threadIdx.x != 0) return;
-  float mn = parts[0], mx = parts[1];
-  bool nan = mn != mn || mx != mx;
-  for (int b = 1; b < nb; ++b) {
-    const float a = parts[2 * b], c = parts[2 * b + 1];
-    nan = nan || a != a || c != c;
-    mn = fminf(mn, a);
-    mx = fmaxf(mx, c);
+  __shared__ float smn[1024], smx[1024];
+  float mn = INFINITY, mx = -INFINITY;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    mn = nmin(mn, parts[2 * b]);
+    mx = nmax(mx, parts[2 * b + 1]);
   }
-  const float maxval = mx - mn;
-  mm[0] = mn;
-  mm[1] = maxval;
-  if (nan || maxval < 0) fl[kLF_ERR] = 1;
+  smn[threadIdx.x] = mn;
+  smx[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      smn[threadIdx.x] = nmin(smn[threadIdx.x], smn[threadIdx.x + o]);
+      smx[threadIdx.x] = nmax(smx[threadIdx.x], smx[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float maxval = smx[0] - smn[0];
+    mm[0] = smn[0];
+    mm[1] = maxval;
+    if (smn[0] != smn[0] || smx[0] != smx[0] || maxval < 0) fl[kLF_ERR] = 1;
+  }
 }
 
 // tmp = alpha + u - (sa + su); gap = tmp.Q; tt = tmp.tmp (:725-729)
@@ -299,90 +351,75 @@ __global__ void k_lp_delta(double* __restrict__ sc, float lambda, double* __rest
   }
 }
 
-// descending sort of a register array (odd-even transposition network)
-template <int MAXL>
-__device__ inline void sort_desc(float (&v)[MAXL]) {
-#pragma unroll
-  for (int pass = 0; pass < MAXL; ++pass)
-#pragma unroll
-    for (int i = pass & 1; i + 1 < MAXL; i += 2) {
-      const float a = v[i], b = v[i + 1];
-      v[i] = fmaxf(a, b);
-      v[i + 1] = fminf(a, b);
-    }
-}
-
-// feasible_Q (:470-492) then renormalize (:444-455) then currentMap (:929-938)
-template <int MAXL>
+// feasible_Q (:470-492), then renormalize (:444-455), then currentMap
+// (:929-938); row groups.  The descending sort of sortCols becomes a rank per
+// value (ties broken by index), the threshold loop runs on the sorted row in
+// LDS, and the column sums follow Eigen's order for the column's alignment.
 __global__ void k_lp_project(int N, int L, const float* __restrict__ Qin, float* __restrict__ Qout,
                              int* __restrict__ lab, int do_feasible) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const size_t o = (size_t)i * L;
-  float q[MAXL];
-#pragma unroll
-  for (int l = 0; l < MAXL; ++l) q[l] = l < L ? Qin[o + l] : -INFINITY;
-  if (do_feasible) {
-    float srt[MAXL];
-#pragma unroll
-    for (int l = 0; l < MAXL; ++l) srt[l] = q[l];
-    sort_desc<MAXL>(srt);
-    const float cs = eigen_sum(L, col_astart(i, L), [&](int l) { return Qin[o + l]; });
-    double sum = (double)(float)(cs - 1.f);
-    int K = -1;
-    bool found = false;
-#pragma unroll
-    for (int k = MAXL; k > 0; --k)
-      if (k <= L && !found) {
-        const double uk = (double)srt[k - 1];
-        if (sum / k < uk) {
-          K = k;
-          found = true;
-        } else {
+  __shared__ float sq[kBlock], ss[kBlock];
+  const long ntiles = rg_tiles(N, L);
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const RG r = rg_at(L, N, tile);
+    const int wb = (threadIdx.x & ~63) + r.g * L;
+    float* rowq = sq + wb;
+    float* rows = ss + wb;
+    const size_t idx = (size_t)r.row * L + r.k;
+    const float q = r.valid ? Qin[idx] : 0.f;
+    if (r.valid) rowq[r.k] = q;
+    __syncthreads();
+    float v = q;
+    if (do_feasible) {
+      float cs = 0.f;
+      if (r.valid) {
+        cs = eigen_sum(L, col_astart(r.row, L), [&](int i) { return rowq[i]; });
+        int rank = 0;
+        for (int j = 0; j < L; ++j) {
+          const float qj = rowq[j];
+          rank += (qj > q) || (qj == q && j < r.k);
+        }
+        rows[rank] = q;
+      }
+      __syncthreads();
+      if (r.valid) {
+        double sum = (double)(float)(cs - 1.f);
+        int K = -1;
+        for (int k = L; k > 0; --k) {
+          const double uk = (double)rows[k - 1];
+          if (sum / k < uk) {
+            K = k;
+            break;
+          }
           sum -= uk;
         }
+        double t = (double)q - sum / K;
+        t = (t < 0.0) ? 0.0 : t;  // std::max(t, 0.0)
+        t = (1.0 < t) ? 1.0 : t;  // std::min(t, 1.0)
+        v = (float)t;
       }
-#pragma unroll
-    for (int l = 0; l < MAXL; ++l)
-      if (l < L) q[l] = (float)fmin(fmax((double)q[l] - sum / K, 0.0), 1.0);
-    // renormalize: column sum in Eigen's order for this column's alignment
-#pragma unroll
-    for (int l = 0; l < MAXL; ++l)
-      if (l < L) Qout[o + l] = q[l];
-    const double s = (double)eigen_sum(L, col_astart(i, L), [&](int l) { return Qout[o + l]; });
-    if (s == 0 || s != s) {
-      const float un = (float)(1.0 / L);
-#pragma unroll
-      for (int l = 0; l < MAXL; ++l) q[l] = un;
-    } else {
-      const float sf = (float)s;
-#pragma unroll
-      for (int l = 0; l < MAXL; ++l) q[l] = q[l] / sf;
+      __syncthreads();
+      if (r.valid) rowq[r.k] = v;
+      __syncthreads();
     }
-  } else {
-    const double s = (double)eigen_sum(L, col_astart(i, L), [&](int l) { return Qin[o + l]; });
-    if (s == 0 || s != s) {
-      const float un = (float)(1.0 / L);
-#pragma unroll
-      for (int l = 0; l < MAXL; ++l) q[l] = un;
-    } else {
-      const float sf = (float)s;
-#pragma unroll
-      for (int l = 0; l < MAXL; ++l) q[l] = q[l] / sf;
+    if (r.valid) {
+      const double sum = (double)eigen_sum(L, col_astart(r.row, L), [&](int i) { return rowq[i]; });
+      const float out = (sum == 0 || sum != sum) ? (float)(1.0 / L) : v / (float)sum;
+      Qout[idx] = out;
+      rows[r.k] = out;
     }
+    __syncthreads();
+    if (r.valid && r.k == 0) {
+      int m = 0;
+      float best = rows[0];
+      for (int l = 1; l < L; ++l)
+        if (rows[l] > best) {
+          best = rows[l];
+          m = l;
+        }
+      lab[r.row] = m;
+    }
+    __syncthreads();
   }
-  int m = 0;
-  float best = q[0];
-#pragma unroll
-  for (int l = 0; l < MAXL; ++l)
-    if (l < L) {
-      Qout[o + l] = q[l];
-      if (q[l] > best) {
-        best = q[l];
-        m = l;
-      }
-    }
-  lab[i] = m;
 }
 
 // max_rounding (:832-840)
@@ -515,7 +552,7 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
   const int nbp = (int)grid_for(N);
   st.parts.resize(std::max<size_t>((size_t)kSweepBatch * nbp, (size_t)2 * kRedBlocks));
   st.sc.resize(kL_NSC);
-  st.mmparts.resize(2 * (size_t)nbp);
+  st.mmparts.resize(2 * (size_t)std::max<long>(nbp, 4096));
   st.mm.resize(2);
   st.fl.resize(kLF_NFL);
   const int cap = std::max(1, std::min(stats_cap, 1 << 16));
@@ -553,7 +590,7 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
   };
 
   // ---- initialisation (:534-550) ------------------------------------------------
-  DCRF_MAXL_DISPATCH(L, (k_lp_project<MAXL><<<grid_for(N), kBlock, 0, s>>>(N, L, init, st.Q.get(), st.lab.get(), 0)));
+  k_lp_project<<<rg_grid(N, L, 8192), kBlock, 0, s>>>(N, L, init, st.Q.get(), st.lab.get(), 0);
   DCRF_HIP(hipMemcpyAsync(st.best.get(), st.Q.get(), NL * sizeof(float), hipMemcpyDeviceToDevice, s));
   int_energy_dev(st.lab.get(), sc + kL_EINT, sc + kL_TMP, s);
   double int_energy = read_d(sc + kL_EINT);
@@ -581,9 +618,8 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
       ++pit;
       {
         KTimer tm("lp_Y", s);
-        DCRF_MAXL_DISPATCH(L, (k_lp_Y<MAXL><<<grid_for(N), kBlock, 0, s>>>(N, L, c, st.alpha.get(), u, st.sa.get(), su,
-                                                                           sc, pit > 1 ? 1 : 0, U_.get(),
-                                                                           st.curQ.get(), st.Y.get())));
+        k_lp_Y<<<rg_grid(N, L, 8192), kBlock, 0, s>>>(N, L, c, st.alpha.get(), u, st.sa.get(), su, sc,
+                                                     pit > 1 ? 1 : 0, U_.get(), st.curQ.get(), st.Y.get());
       }
       // qp-gamma sweeps in batches; the replay leaves gamma after the
       // reference's number of sweeps
@@ -595,7 +631,7 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
           DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbp, kBlock, 0, s>>>(N, L, c, st.gamma.get(), st.Y.get(), nsw, 0,
                                                                          st.parts.get(), fl)));
         }
-        k_lp_sweep_decide<<<1, kBlock, 0, s>>>(st.parts.get(), nbp, nsw, P.qp_tol, done_sweeps, P.qp_max_iter, sc, fl);
+        k_lp_sweep_decide<<<1, 1024, 0, s>>>(st.parts.get(), nbp, nsw, P.qp_tol, done_sweeps, P.qp_max_iter, sc, fl);
         {
           KTimer tm("lp_sweeps_replay", s);
           DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbp, kBlock, 0, s>>>(N, L, c, st.gamma.get(), st.Y.get(), nsw, 1,
@@ -610,9 +646,10 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
       res.qp_sweeps.push_back(done_sweeps);
       {
         KTimer tm("lp_betaQ", s);
-        k_lp_betaQ<<<nbp, kBlock, 0, s>>>(N, L, lambda, st.alpha.get(), u, st.gamma.get(), U_.get(), st.curQ.get(),
+        const unsigned nbq = rg_grid(N, L, 4096);
+        k_lp_betaQ<<<nbq, kBlock, 0, s>>>(N, L, lambda, st.alpha.get(), u, st.gamma.get(), U_.get(), st.curQ.get(),
                                           st.Q.get(), st.mmparts.get());
-        k_lp_minmax_final<<<1, 64, 0, s>>>(st.mmparts.get(), nbp, st.mm.get(), fl);
+        k_lp_minmax_final<<<1, 1024, 0, s>>>(st.mmparts.get(), nbq, st.mm.get(), fl);
       }
       for (int k = 0; k < num_pairwise(); ++k) {
         Potential& p = pairwise(k);
@@ -637,8 +674,7 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
     // project, renormalize, labelling (:746-752)
     {
       KTimer tm("lp_project", s);
-      DCRF_MAXL_DISPATCH(L, (k_lp_project<MAXL><<<grid_for(N), kBlock, 0, s>>>(N, L, st.Q.get(), st.Q.get(),
-                                                                                st.lab.get(), 1)));
+      k_lp_project<<<rg_grid(N, L, 8192), kBlock, 0, s>>>(N, L, st.Q.get(), st.Q.get(), st.lab.get(), 1);
     }
     const double prev_int = int_energy;
     int_energy_dev(st.lab.get(), sc + kL_EINT, sc + kL_TMP, s);
