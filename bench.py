@@ -48,12 +48,16 @@ def algorithmic_bytes(name: str, N: int, M: dict, V: int = L, S: int = 4) -> flo
         return 2 * S * m * V + 8 * m
     if kind == "slice":
         return S * m * V + 8 * e + 2 * S * N * V + 4 * N
+    # ordered filter: bins are one byte per (pixel, label); the lattice state
+    # holds 10 lower bins per (point, label); the upper pass is T - lower
+    if kind == "obin":
+        return S * N * V + N * V
     if kind == "osplat":
-        return S * N * V + 8 * e + S * m * 20 * V
+        return N * V + 8 * e + S * m * 10 * V
     if kind == "oblur":
-        return 2 * S * m * 20 * V + 8 * m
+        return 2 * S * m * 10 * V + 8 * m
     if kind == "oslice":
-        return S * m * 20 * V + 8 * e + S * N * V + 2 * S * N * V
+        return N * V + 8 * e + S * m * 10 * V + S * m + 2 * S * N * V
     return float("nan")
 
 
@@ -137,12 +141,16 @@ def main():
     Qlp = D.DeviceArray((wl.N, L))
     lp_params = D.LP_inf_params(prox_max_iter=LP_PROX, fw_max_iter=LP_FW)
 
+    lp_info = {}
+
     def step():
         t0 = time.perf_counter()
         crf.qp_inference_super_pixels_non_convex(Q0, max_iter=QP_ITERS, ref_stop=False, out=Qqp)
         t1 = time.perf_counter()
         if not args.qp_only:
-            crf.lp_inference_prox_super_pixels(Qqp, lp_params, out=Qlp)
+            _, st = crf.lp_inference_prox_super_pixels(Qqp, lp_params, out=Qlp, stats=True)
+            lp_info.update(outer=int(st["outer"]), inner=int(len(st["qp_sweeps"])),
+                           qp_sweeps=int(np.sum(st["qp_sweeps"])))
         D.synchronize()
         return t1 - t0, time.perf_counter() - t1
 
@@ -194,6 +202,7 @@ def main():
         "qp_ms_per_solve": round(qp_t / args.steps * 1e3, 3),
         "lp_ms_per_solve": None if args.qp_only else round(lp_t / args.steps * 1e3, 3),
         "lattice_build_ms": round(build_ms, 2),
+        "lp_iterations": lp_info or None,
         "filter_gbps": None if filter_gbs is None else round(filter_gbs, 1),
         "images_per_s": None,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,

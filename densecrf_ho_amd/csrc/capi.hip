@@ -84,7 +84,7 @@ struct Staged {
 };
 
 namespace {
-__global__ void k_features_2d(float* __restrict__ f, int W, int H, float sx, float sy, const unsigned char* im,
+__global__ void __launch_bounds__(kBlock) k_features_2d(float* __restrict__ f, int W, int H, float sx, float sy, const unsigned char* im,
                               float sr, float sg, float sb) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= W * H) return;
@@ -99,7 +99,7 @@ __global__ void k_features_2d(float* __restrict__ f, int W, int H, float sx, flo
   }
 }
 
-__global__ void k_minmax(const float* __restrict__ a, size_t n, float* __restrict__ parts) {
+__global__ void __launch_bounds__(kBlock) k_minmax(const float* __restrict__ a, size_t n, float* __restrict__ parts) {
   __shared__ float smn[kBlock], smx[kBlock];
   float mn = INFINITY, mx = -INFINITY;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {

@@ -197,6 +197,19 @@ struct FastDiv {
   }
 };
 
+// XCD-aware block order.  Workgroups are dispatched round-robin over the 8
+// XCDs (each with its own L2); this maps them so that XCD x works on one
+// contiguous range of logical blocks, keeping neighbouring rows (lattice
+// points that share pixels, pixels that share vertices) in one L2.
+constexpr unsigned kNumXcd = 8;
+__device__ inline unsigned xcd_block() {
+  const unsigned g = gridDim.x, b = blockIdx.x;
+  if (g < kNumXcd) return b;
+  const unsigned per = g / kNumXcd, rem = g % kNumXcd;
+  const unsigned x = b % kNumXcd, i = b / kNumXcd;
+  return x * per + min(x, rem) + i;
+}
+
 // Number of blocks used by grid-stride reduction kernels: a fixed count so the
 // combination order of partial sums (and therefore the result) is
 // deterministic for a given problem size.

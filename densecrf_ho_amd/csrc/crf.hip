@@ -306,7 +306,7 @@ void CRF::apply_ord(int k, const float* q01, float* out, bool check) {
 
 // {min, max(Q - min)} of a device array into mm (rescale, src/eigen_utils.cpp:120-140);
 // fl[2] = 1 when the maximum is negative or NaN is met
-__global__ void k_mm_parts(const float* __restrict__ a, size_t n, float* __restrict__ parts) {
+__global__ void __launch_bounds__(kBlock) k_mm_parts(const float* __restrict__ a, size_t n, float* __restrict__ parts) {
   __shared__ float smn[kBlock], smx[kBlock];
   float mn = INFINITY, mx = -INFINITY;
   bool nan = false;

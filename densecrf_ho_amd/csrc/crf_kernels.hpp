@@ -12,7 +12,7 @@ constexpr int kMaxLabels = 64;
 
 namespace {
 
-__global__ void k_fill(float* __restrict__ p, size_t n, float v) {
+__global__ void __launch_bounds__(kBlock) k_fill(float* __restrict__ p, size_t n, float v) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = v;
 }
@@ -29,7 +29,7 @@ __global__ void k_zero_scalars(double* __restrict__ sc, int ns, int* __restrict_
 }
 
 // norm = 1 / sqrt(K1 + 1e-20) or 1 / (K1 + 1e-20), double then float
-__global__ void k_norm_finish(float* __restrict__ n, int N, int symmetric) {
+__global__ void __launch_bounds__(kBlock) k_norm_finish(float* __restrict__ n, int N, int symmetric) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const double v = (double)n[i] + 1e-20;
@@ -116,7 +116,7 @@ inline unsigned rg_grid(long nrows, int L, long cap = 1 << 30) {
 }
 
 // expAndNormalize(-U) (src/densecrf.cpp:328-336), one thread per pixel
-__global__ void k_softmax_neg(const float* __restrict__ U, float* __restrict__ Q, int N, int L) {
+__global__ void __launch_bounds__(kBlock) k_softmax_neg(const float* __restrict__ U, float* __restrict__ Q, int N, int L) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const float* u = U + (size_t)i * L;
@@ -129,7 +129,7 @@ __global__ void k_softmax_neg(const float* __restrict__ U, float* __restrict__ Q
 }
 
 // Σ a[i]*b[i] in FP64 over n elements: per-block partials, then one block
-__global__ void k_dot(const float* __restrict__ a, const float* __restrict__ b, size_t n, double* __restrict__ parts) {
+__global__ void __launch_bounds__(kBlock) k_dot(const float* __restrict__ a, const float* __restrict__ b, size_t n, double* __restrict__ parts) {
   double v[1] = {0.0};
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     v[0] += (double)a[i] * (double)b[i];
@@ -144,7 +144,7 @@ inline void launch_dot(hipStream_t s, const float* a, const float* b, size_t n, 
 }
 
 // grad_y += 2 * tmp (src/densecrf.cpp:380) and partial Σ Q*tmp (:953)
-__global__ void k_gy_init_accum(float* __restrict__ gy, const float* __restrict__ tmp, const float* __restrict__ Q,
+__global__ void __launch_bounds__(kBlock) k_gy_init_accum(float* __restrict__ gy, const float* __restrict__ tmp, const float* __restrict__ Q,
                                 size_t n, double* __restrict__ parts) {
   double v[1] = {0.0};
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -177,7 +177,7 @@ __global__ void k_qp_e0(double* sc, int nk, int has_ho, double* st_energy, int c
 }
 
 // descent_direction (src/qp.cpp:5-15): first minimum of each column
-__global__ void k_argmin(const float* __restrict__ g, int* __restrict__ lab, int N, int L, const int* __restrict__ fl) {
+__global__ void __launch_bounds__(kBlock) k_argmin(const float* __restrict__ g, int* __restrict__ lab, int N, int L, const int* __restrict__ fl) {
   if (fl[kF_DONE]) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -195,7 +195,7 @@ __global__ void k_argmin(const float* __restrict__ g, int* __restrict__ lab, int
 }
 
 // first minimum of each row (descent_direction, src/qp.cpp:5-15), row groups
-__global__ void k_argmin_rg(const float* __restrict__ g, int* __restrict__ lab, int N, int L) {
+__global__ void __launch_bounds__(kBlock) k_argmin_rg(const float* __restrict__ g, int* __restrict__ lab, int N, int L) {
   __shared__ float seg[kBlock];
   const long ntiles = rg_tiles(N, L);
   for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -218,7 +218,7 @@ __global__ void k_argmin_rg(const float* __restrict__ g, int* __restrict__ lab, 
 }
 
 // line-search sums U.sx, Q.psisx, sx.psisx with sx = onehot(lab) - Q (:408-414)
-__global__ void k_qp_dots(const float* __restrict__ U, const float* __restrict__ Q, const int* __restrict__ lab,
+__global__ void __launch_bounds__(kBlock) k_qp_dots(const float* __restrict__ U, const float* __restrict__ Q, const int* __restrict__ lab,
                           const float* __restrict__ psx, size_t n, FastDiv divL, double* __restrict__ parts,
                           const int* __restrict__ fl) {
   if (fl[kF_DONE]) return;
@@ -271,7 +271,7 @@ __global__ void k_qp_step(double* sc, int* fl, int has_ho, double* st_num, doubl
 }
 
 // Q += step*sx; grad_y += 2*step*psisx + step*hb; partial Q.(grad_y+U) (:424-434)
-__global__ void k_qp_update(const float* __restrict__ U, float* __restrict__ Q, float* __restrict__ gy,
+__global__ void __launch_bounds__(kBlock) k_qp_update(const float* __restrict__ U, float* __restrict__ Q, float* __restrict__ gy,
                             const float* __restrict__ psx, const int* __restrict__ lab, const float* __restrict__ hb,
                             int hb_npix, size_t n, FastDiv divL, const double* __restrict__ sc,
                             double* __restrict__ parts, const int* __restrict__ fl) {
@@ -299,7 +299,7 @@ __global__ void k_qp_update(const float* __restrict__ U, float* __restrict__ Q, 
 // Q += step*sx; grad_y += 2*step*psisx + step*hb; partial Q.(grad_y+U)
 // (:424-434), then the first minimum of the new grad_y row: the next
 // iteration's conditional gradient (src/qp.cpp:5-15)
-__global__ void k_qp_update_rg(const float* __restrict__ U, float* __restrict__ Q, float* __restrict__ gy,
+__global__ void __launch_bounds__(kBlock) k_qp_update_rg(const float* __restrict__ U, float* __restrict__ Q, float* __restrict__ gy,
                                const float* __restrict__ psx, int* __restrict__ lab, const float* __restrict__ hb,
                                int hb_npix, int N, int L, const double* __restrict__ sc, double* __restrict__ parts,
                                const int* __restrict__ fl) {
@@ -355,7 +355,7 @@ __global__ void k_qp_check(double* sc, int* fl, int ref_stop, int has_ho, double
 }
 
 // currentMap (src/densecrf.cpp:929-938): first maximum of each column
-__global__ void k_argmax(const float* __restrict__ Q, int* __restrict__ lab, int N, int L) {
+__global__ void __launch_bounds__(kBlock) k_argmax(const float* __restrict__ Q, int* __restrict__ lab, int N, int L) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const float* r = Q + (size_t)i * L;
@@ -372,7 +372,7 @@ __global__ void k_argmax(const float* __restrict__ Q, int* __restrict__ lab, int
 }
 
 // r[i] += -out(l_i, i) of one no-normalisation potential (:915-925)
-__global__ void k_pairwise_true_accum(const float* __restrict__ o, const int* __restrict__ lab,
+__global__ void __launch_bounds__(kBlock) k_pairwise_true_accum(const float* __restrict__ o, const int* __restrict__ lab,
                                       float* __restrict__ r, int N, int L) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -382,7 +382,7 @@ __global__ void k_pairwise_true_accum(const float* __restrict__ o, const int* __
 }
 
 // Σ_i (unary(l_i, i) + pairwise_i) in FP64 (:873-885)
-__global__ void k_assign_energy(const float* __restrict__ U, const int* __restrict__ lab,
+__global__ void __launch_bounds__(kBlock) k_assign_energy(const float* __restrict__ U, const int* __restrict__ lab,
                                 const float* __restrict__ r, int N, int L, double* __restrict__ parts) {
   double v[1] = {0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {

@@ -43,7 +43,7 @@ namespace {
 // one thread per (size group, label): sum of P(pix, l) over pix in
 // [u_{g-1}, u_g); MODE 0: P = Q - 1, MODE 1: P = onehot(lab) - Q
 template <int MODE>
-__global__ void k_ho_segsum(const float* __restrict__ Q, const int* __restrict__ lab, int L,
+__global__ void __launch_bounds__(kBlock) k_ho_segsum(const float* __restrict__ Q, const int* __restrict__ lab, int L,
                             const int* __restrict__ usz, int G, double* __restrict__ B, const int* __restrict__ fl) {
   if (fl && fl[kF_DONE]) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -75,7 +75,7 @@ __global__ void k_ho_scan(const double* __restrict__ B, double* __restrict__ out
 
 // per (group, label): sum over the group's regions of p(l, r) where
 // p = (z - 1) * e (the argument of the broadcast in :382 and :431)
-__global__ void k_ho_groupsum_zm1(const float* __restrict__ z, const float* __restrict__ e,
+__global__ void __launch_bounds__(kBlock) k_ho_groupsum_zm1(const float* __restrict__ z, const float* __restrict__ e,
                                   const int* __restrict__ gstart, const int* __restrict__ greg, int G, int L,
                                   double* __restrict__ A, const int* __restrict__ fl) {
   if (fl && fl[kF_DONE]) return;
@@ -91,7 +91,7 @@ __global__ void k_ho_groupsum_zm1(const float* __restrict__ z, const float* __re
 }
 
 // hb(pix, l) = suffix(g*) with g* the first group whose size exceeds pix
-__global__ void k_ho_bcast(const double* __restrict__ suf, const int* __restrict__ usz, int G, int smax, int L,
+__global__ void __launch_bounds__(kBlock) k_ho_bcast(const double* __restrict__ suf, const int* __restrict__ usz, int G, int smax, int L,
                            float* __restrict__ hb, const int* __restrict__ fl) {
   if (fl && fl[kF_DONE]) return;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -109,7 +109,7 @@ __global__ void k_ho_bcast(const double* __restrict__ suf, const int* __restrict
 }
 
 // grad_y += hb on the first smax pixels (:382)
-__global__ void k_ho_add_hb(float* __restrict__ gy, const float* __restrict__ hb, size_t n) {
+__global__ void __launch_bounds__(kBlock) k_ho_add_hb(float* __restrict__ gy, const float* __restrict__ hb, size_t n) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) gy[t] = gy[t] + hb[t];
 }
@@ -167,7 +167,7 @@ __global__ void k_ho_direction(const float* __restrict__ z, const float* __restr
 }
 
 // z += step * sx_z (:425)
-__global__ void k_ho_zstep(float* __restrict__ z, const float* __restrict__ sxz, int n, const double* __restrict__ sc,
+__global__ void __launch_bounds__(kBlock) k_ho_zstep(float* __restrict__ z, const float* __restrict__ sxz, int n, const double* __restrict__ sc,
                            const int* __restrict__ fl) {
   if (fl[kF_DONE]) return;
   const float st = (float)sc[kS_STEP];
@@ -179,7 +179,7 @@ __global__ void k_ho_zstep(float* __restrict__ z, const float* __restrict__ sxz,
 // The running int min after element k is trunc(prefix-min of Q); element k
 // takes the min position iff Q_k < trunc(prefix-min before k) (the first
 // element always does).  Likewise for the max.  One wave per region.
-__global__ void k_ucond_scan(const float* __restrict__ Q, int L, const int* __restrict__ mstart,
+__global__ void __launch_bounds__(kBlock) k_ucond_scan(const float* __restrict__ Q, int L, const int* __restrict__ mstart,
                              const int* __restrict__ mpix, int r0, int r1, int* __restrict__ mpos) {
   const int r = r0 + (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
   const int lane = threadIdx.x & 63;
@@ -237,7 +237,7 @@ __global__ void k_ucond_scan(const float* __restrict__ Q, int L, const int* __re
 }
 
 // Us(min) = c*e, then Us(max) = -c*e (the max write wins), regions of one layer
-__global__ void k_ucond_write(const int* __restrict__ mpos, const float* __restrict__ e, const float* __restrict__ c,
+__global__ void __launch_bounds__(kBlock) k_ucond_write(const int* __restrict__ mpos, const float* __restrict__ e, const float* __restrict__ c,
                               int r0, int r1, int L, float* __restrict__ Us) {
   const int r = r0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= r1) return;
@@ -247,7 +247,7 @@ __global__ void k_ucond_write(const int* __restrict__ mpos, const float* __restr
 }
 
 // sum_r e_r * max_l (max_members Q - min_members Q), one block per region
-__global__ void k_ho_maxmin(const float* __restrict__ Q, int L, const int* __restrict__ mstart,
+__global__ void __launch_bounds__(kBlock) k_ho_maxmin(const float* __restrict__ Q, int L, const int* __restrict__ mstart,
                             const int* __restrict__ mpix, const float* __restrict__ e, double* __restrict__ parts) {
   const int r = blockIdx.x;
   const int b = mstart[r], s = mstart[r + 1] - b;
@@ -286,7 +286,7 @@ __global__ void k_ho_maxmin(const float* __restrict__ Q, int L, const int* __res
 
 // assignment_energy_higher_order's region term (:856-867), bug-compatible:
 // region r differs iff some loop index pix < s_r has lab[pix] != lab[first member]
-__global__ void k_ho_assign(const int* __restrict__ lab, const int* __restrict__ mstart,
+__global__ void __launch_bounds__(kBlock) k_ho_assign(const int* __restrict__ lab, const int* __restrict__ mstart,
                             const int* __restrict__ mpix, const float* __restrict__ e, int R,
                             const int* __restrict__ fd, double* __restrict__ parts) {
   const int first_diff = *fd;
@@ -303,7 +303,7 @@ __global__ void k_ho_assign(const int* __restrict__ lab, const int* __restrict__
 }
 
 // first pixel index whose label differs from lab[0] (N if none)
-__global__ void k_first_diff(const int* __restrict__ lab, int N, int* __restrict__ out) {
+__global__ void __launch_bounds__(kBlock) k_first_diff(const int* __restrict__ lab, int N, int* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < N && (short)lab[i] != (short)lab[0]) atomicMin(out, i);
 }

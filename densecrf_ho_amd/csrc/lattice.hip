@@ -66,7 +66,7 @@ struct ElevConst {
 // Pixels at or past N (the padding of the last block of 4) take a zero
 // feature vector and still produce keys, like the reference's SSE lanes.
 template <int D>
-__global__ void k_elevate(const float* __restrict__ feat, int N, int Np, ElevConst c, PKey<D>* __restrict__ ekeys,
+__global__ void __launch_bounds__(kBlock) k_elevate(const float* __restrict__ feat, int N, int Np, ElevConst c, PKey<D>* __restrict__ ekeys,
                           int* __restrict__ rank_out, float* __restrict__ bary_out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= Np) return;
@@ -149,7 +149,7 @@ __global__ void k_elevate(const float* __restrict__ feat, int N, int Np, ElevCon
 // Open-addressing insert with CAS.  Every table slot ends up holding the
 // smallest entry index carrying its key (the key's first occurrence).
 template <int D>
-__global__ void k_insert(const PKey<D>* __restrict__ ekeys, int E, int* table, uint32_t mask,
+__global__ void __launch_bounds__(kBlock) k_insert(const PKey<D>* __restrict__ ekeys, int E, int* table, uint32_t mask,
                          int* __restrict__ slot_of, int* __restrict__ err) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
@@ -175,7 +175,7 @@ __global__ void k_insert(const PKey<D>* __restrict__ ekeys, int E, int* table, u
   atomicExch(err, 1);  // table full: cannot happen with capacity >= 2E
 }
 
-__global__ void k_mark_first(const int* __restrict__ table, const int* __restrict__ slot_of, int E,
+__global__ void __launch_bounds__(kBlock) k_mark_first(const int* __restrict__ table, const int* __restrict__ slot_of, int E,
                              int* __restrict__ flag) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
@@ -183,7 +183,7 @@ __global__ void k_mark_first(const int* __restrict__ table, const int* __restric
 }
 
 template <int D>
-__global__ void k_assign(const int* __restrict__ table, const int* __restrict__ slot_of, const int* __restrict__ pos,
+__global__ void __launch_bounds__(kBlock) k_assign(const int* __restrict__ table, const int* __restrict__ slot_of, const int* __restrict__ pos,
                          const int* __restrict__ flag, const PKey<D>* __restrict__ ekeys, int E,
                          int* __restrict__ offset, PKey<D>* __restrict__ ukeys) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -207,7 +207,7 @@ __device__ inline int lookup(const PKey<D>& k, const int* __restrict__ table, ui
 
 // neighbours along each of the d+1 lattice axes (src/permutohedral.cpp:317-330)
 template <int D>
-__global__ void k_neighbors(const PKey<D>* __restrict__ ukeys, int M, const int* __restrict__ table, uint32_t mask,
+__global__ void __launch_bounds__(kBlock) k_neighbors(const PKey<D>* __restrict__ ukeys, int M, const int* __restrict__ table, uint32_t mask,
                             const PKey<D>* __restrict__ ekeys, const int* __restrict__ pos, int* __restrict__ nbr,
                             int16_t* __restrict__ keys16) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -236,15 +236,15 @@ __global__ void k_neighbors(const PKey<D>* __restrict__ ukeys, int M, const int*
 }
 
 // ---- splat gather lists -------------------------------------------------------
-__global__ void k_count(const int* __restrict__ offset, int E, int* __restrict__ cnt) {
+__global__ void __launch_bounds__(kBlock) k_count(const int* __restrict__ offset, int E, int* __restrict__ cnt) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e < E) atomicAdd(&cnt[offset[e]], 1);
 }
-__global__ void k_iota(int* __restrict__ v, int n) {
+__global__ void __launch_bounds__(kBlock) k_iota(int* __restrict__ v, int n) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e < n) v[e] = e;
 }
-__global__ void k_fill_sp(const int* __restrict__ ent, const float* __restrict__ bary, int E, int d1,
+__global__ void __launch_bounds__(kBlock) k_fill_sp(const int* __restrict__ ent, const float* __restrict__ bary, int E, int d1,
                           int* __restrict__ sp_pix, float* __restrict__ sp_w) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= E) return;
@@ -254,7 +254,7 @@ __global__ void k_fill_sp(const int* __restrict__ ent, const float* __restrict__
 }
 // chunks per point (at least one so that every row of the value buffer is
 // written, also for points that only padding lanes produced)
-__global__ void k_nchunks(const int* __restrict__ cnt, int M, int* __restrict__ nch, int* __restrict__ multi_flag,
+__global__ void __launch_bounds__(kBlock) k_nchunks(const int* __restrict__ cnt, int M, int* __restrict__ nch, int* __restrict__ multi_flag,
                           int* __restrict__ pn) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= M) return;
@@ -264,7 +264,7 @@ __global__ void k_nchunks(const int* __restrict__ cnt, int M, int* __restrict__ 
   multi_flag[p] = n > 1 ? 1 : 0;
   pn[p] = n > 1 ? n : 0;  // partial-sum slots are only needed for multi-chunk points
 }
-__global__ void k_chpoint(const int* __restrict__ chstart, const int* __restrict__ mpos,
+__global__ void __launch_bounds__(kBlock) k_chpoint(const int* __restrict__ chstart, const int* __restrict__ mpos,
                           const int* __restrict__ mflag, int M, int* __restrict__ chpoint, int* __restrict__ multi) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= M) return;
@@ -278,7 +278,11 @@ __global__ void k_chpoint(const int* __restrict__ chstart, const int* __restrict
 struct RowMap {
   int gpw;  // rows per wave
   __device__ inline void at(int V, long& row, int& k0, int& kstep) const {
-    const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    at_block(V, (long)xcd_block(), row, k0, kstep);
+  }
+  // same mapping for an explicit (virtual) block index
+  __device__ inline void at_block(int V, long blk, long& row, int& k0, int& kstep) const {
+    const long gt = blk * blockDim.x + threadIdx.x;
     const long wave = gt >> 6;
     const int lane = (int)(gt & 63);
     if (V <= 64) {
@@ -314,7 +318,7 @@ __device__ inline float splat_in(const SplatArgs& a, int pix, int k, int V) {
 
 // values[p] = sum over the point's entries, in pixel order, of w * in(pixel)
 template <SplatIn KIND>
-__global__ void k_splat(SplatArgs a, int V, RowMap rm, const int* __restrict__ sp_pix, const float* __restrict__ sp_w,
+__global__ void __launch_bounds__(kBlock) k_splat(SplatArgs a, int V, RowMap rm, const int* __restrict__ sp_pix, const float* __restrict__ sp_w,
                         const int* __restrict__ seg, const int* __restrict__ chstart, const int* __restrict__ chpoint,
                         const int* __restrict__ pstart, int nchunks, float* __restrict__ values,
                         float* __restrict__ part) {
@@ -346,7 +350,7 @@ __global__ void k_splat(SplatArgs a, int V, RowMap rm, const int* __restrict__ s
 }
 
 // points whose entries span several chunks: sum the chunk partials in order
-__global__ void k_combine(int V, RowMap rm, const int* __restrict__ multi, int nmulti, const int* __restrict__ chstart,
+__global__ void __launch_bounds__(kBlock) k_combine(int V, RowMap rm, const int* __restrict__ multi, int nmulti, const int* __restrict__ chstart,
                           const int* __restrict__ pstart, const float* __restrict__ part, float* __restrict__ values) {
   long m;
   int k0, ks;
@@ -364,7 +368,7 @@ __global__ void k_combine(int V, RowMap rm, const int* __restrict__ multi, int n
 // one blur axis: new = old + 0.5 * (n1 + n2); SEQ evaluates the sum in double
 // and rounds once (seqCompute, :518), otherwise float (sseCompute, :1035)
 template <bool SEQ>
-__global__ void k_blur(int V, RowMap rm, int M, const int* __restrict__ nbr, const float* __restrict__ src,
+__global__ void __launch_bounds__(kBlock) k_blur(int V, RowMap rm, int M, const int* __restrict__ nbr, const float* __restrict__ src,
                        float* __restrict__ dst) {
   long i;
   int k0, ks;
@@ -379,9 +383,9 @@ __global__ void k_blur(int V, RowMap rm, int M, const int* __restrict__ nbr, con
 }
 
 // float4 form for wide rows (ordered filter state), same arithmetic
-__global__ void k_blur4(int V4, int M, const int* __restrict__ nbr, const float4* __restrict__ src,
+__global__ void __launch_bounds__(kBlock) k_blur4(int V4, int M, const int* __restrict__ nbr, const float4* __restrict__ src,
                         float4* __restrict__ dst) {
-  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long wave = ((long)xcd_block() * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (wave >= M) return;
   const long i = wave;
@@ -409,7 +413,7 @@ __device__ inline void slice_store(const SliceArgs& o, int pix, int k, int V, fl
 
 // barycentric gather: SEQ uses (w*v)*alpha (:533), otherwise (w*alpha)*v (:1048)
 template <bool SEQ>
-__global__ void k_slice(int V, RowMap rm, int N, int d1, const int* __restrict__ offset, const float* __restrict__ bary,
+__global__ void __launch_bounds__(kBlock) k_slice(int V, RowMap rm, int N, int d1, const int* __restrict__ offset, const float* __restrict__ bary,
                         const float* __restrict__ values, float alpha, SliceArgs o) {
   long pix;
   int k0, ks;
@@ -431,23 +435,26 @@ __global__ void k_slice(int V, RowMap rm, int N, int d1, const int* __restrict__
 }
 
 // k_combine that does nothing unless *need (the upper pass without edges)
-__global__ void k_combine_if(int V, RowMap rm, const int* __restrict__ multi, int nmulti,
+__global__ void __launch_bounds__(kBlock) k_combine_if(int V, RowMap rm, const int* __restrict__ multi, int nmulti,
                              const int* __restrict__ chstart, const int* __restrict__ pstart,
                              const float* __restrict__ part, float* __restrict__ values, const int* __restrict__ need) {
   if (need && *need == 0) return;
-  long m;
-  int k0, ks;
-  rm.at(V, m, k0, ks);
-  if (m < 0 || m >= nmulti) return;
-  const int p = multi[m];
-  const int c0 = pstart[p], c1 = c0 + chstart[p + 1] - chstart[p];
-  for (int k = k0; k < V; k += ks) {
-    float acc = part[(size_t)c0 * V + k];
-    for (int c = c0 + 1; c < c1; ++c) acc = acc + part[(size_t)c * V + k];
-    values[(size_t)(p + 1) * V + k] = acc;
+  const long nvb = (long)((nmulti + rm.gpw - 1) / rm.gpw * 64 + blockDim.x - 1) / blockDim.x;
+  for (long vb = xcd_block(); vb < nvb; vb += gridDim.x) {
+    long m;
+    int k0, ks;
+    rm.at_block(V, vb, m, k0, ks);
+    if (m < 0 || m >= nmulti) continue;
+    const int p = multi[m];
+    const int c0 = pstart[p], c1 = c0 + chstart[p + 1] - chstart[p];
+    for (int k = k0; k < V; k += ks) {
+      float acc = part[(size_t)c0 * V + k];
+      for (int c = c0 + 1; c < c1; ++c) acc = acc + part[(size_t)c * V + k];
+      values[(size_t)(p + 1) * V + k] = acc;
+    }
   }
 }
-__global__ void k_fill_f(float* __restrict__ p, size_t n, float v) {
+__global__ void __launch_bounds__(kBlock) k_fill_f(float* __restrict__ p, size_t n, float v) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = v;
 }
@@ -460,7 +467,7 @@ __global__ void k_fill_f(float* __restrict__ p, size_t n, float v) {
 constexpr int kStageMinV = 8;
 
 template <SplatIn KIND>
-__global__ void k_splat_staged(SplatArgs a, int V, RowMap rm, const int* __restrict__ sp_pix,
+__global__ void __launch_bounds__(kBlock) k_splat_staged(SplatArgs a, int V, RowMap rm, const int* __restrict__ sp_pix,
                                const float* __restrict__ sp_w, const int* __restrict__ seg,
                                const int* __restrict__ chstart, const int* __restrict__ chpoint,
                                const int* __restrict__ pstart, int nchunks, float* __restrict__ values,
@@ -505,7 +512,7 @@ __global__ void k_splat_staged(SplatArgs a, int V, RowMap rm, const int* __restr
 
 // slice with the pixel's d+1 (offset, weight) pairs staged in LDS (V >= d+1)
 template <bool SEQ>
-__global__ void k_slice_staged(int V, RowMap rm, int N, int d1, const int* __restrict__ offset,
+__global__ void __launch_bounds__(kBlock) k_slice_staged(int V, RowMap rm, int N, int d1, const int* __restrict__ offset,
                                const float* __restrict__ bary, const float* __restrict__ values, float alpha,
                                SliceArgs o) {
   __shared__ int soff[kBlock];
@@ -550,17 +557,21 @@ __device__ inline float rescaled(const float* Q, size_t idx, const float* mm) {
 // bin_up == bin_lo for every (pixel, label); the splat raises `edge` when
 // some q sits on a bin edge (bin_up == bin_lo - 1), and then the upper pass
 // is splatted and blurred in full into UP[point][label][10].
-__global__ void k_osplat(const float* __restrict__ Q, const float* __restrict__ mm, int L, RowMap rm, int upper,
+template <int TAG>  // TAG only names the instantiation (lattice dim, +100 for the upper pass)
+__global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ bins, int L, RowMap rm, long nvblocks,
+                                                   int upper,
                          const int* __restrict__ sp_pix, const float* __restrict__ sp_w, const int* __restrict__ seg,
                          const int* __restrict__ chstart, const int* __restrict__ chpoint,
                          const int* __restrict__ pstart, int nchunks, float* __restrict__ values,
                          float* __restrict__ part, int* __restrict__ edge) {
   __shared__ int spx[kBlock];
   __shared__ float swt[kBlock];
+  __shared__ int maxlen;
   if (upper && *edge == 0) return;
+  for (long vb = xcd_block(); vb < nvblocks; vb += gridDim.x) {
   long c;
   int k0, ks;
-  rm.at(L, c, k0, ks);
+  rm.at_block(L, vb, c, k0, ks);
   const bool act = c >= 0 && c < nchunks;
   const int gb = (threadIdx.x & ~63) + ((threadIdx.x & 63) / L) * L;
   int b = 0, e = 0, p = 0, c0 = 0, nc = 1;
@@ -574,8 +585,11 @@ __global__ void k_osplat(const float* __restrict__ Q, const float* __restrict__ 
   float acc[10];
 #pragma unroll
   for (int q = 0; q < 10; ++q) acc[q] = 0.f;
-  bool any_edge = false;
-  const int rounds = (kChunk + L - 1) / L;
+  if (threadIdx.x == 0) maxlen = 0;
+  __syncthreads();
+  if (act && k0 == 0) atomicMax(&maxlen, e - b);
+  __syncthreads();
+  const int rounds = (maxlen + L - 1) / L;  // block-uniform: every thread reaches each barrier
   for (int r = 0; r < rounds; ++r) {
     const int t0 = b + r * L;
     if (act) {
@@ -586,11 +600,11 @@ __global__ void k_osplat(const float* __restrict__ Q, const float* __restrict__ 
     __syncthreads();
     if (act) {
       const int n = min(L, e - t0);
+#pragma unroll 4
       for (int j = 0; j < n; ++j) {
         const float w = swt[gb + j];
-        const float q = rescaled(Q, (size_t)spx[gb + j] * L + k0, mm);
-        const int lo = bin_lo(q), up = bin_up(q);
-        any_edge = any_edge || (lo != up);
+        const int bb = bins[(size_t)spx[gb + j] * L + k0];
+        const int lo = bb & 15, up = bb >> 4;
         if (upper) {
 #pragma unroll
           for (int r2 = 0; r2 < 10; ++r2) acc[r2] = acc[r2] + (r2 >= up ? w : 0.f);
@@ -602,7 +616,6 @@ __global__ void k_osplat(const float* __restrict__ Q, const float* __restrict__ 
     }
     __syncthreads();
   }
-  if (!upper && any_edge) atomicOr(edge, 1);
   if (act) {
     float* dst = nc == 1 ? values + ((size_t)(p + 1) * L + k0) * 10
                          : part + ((size_t)(pstart[p] + (c - c0)) * L + k0) * 10;
@@ -610,28 +623,52 @@ __global__ void k_osplat(const float* __restrict__ Q, const float* __restrict__ 
 #pragma unroll
     for (int q = 0; q < 5; ++q) d2[q] = make_float2(acc[2 * q], acc[2 * q + 1]);
   }
+  __syncthreads();  // maxlen / staging buffers are reused by the next virtual block
+  }
 }
 
-// one blur axis over rows of R floats (R even), float2 wide; skip == 1 makes it a no-op
-__global__ void k_blur2(int R2, int M, const int* __restrict__ nbr, const float2* __restrict__ src,
+// one blur axis over rows of R2 float2 (ordered state): one wave per lattice
+// point, lanes stride the row; need == 0 makes it a no-op
+template <int TAG>
+__global__ void __launch_bounds__(kBlock) k_blur2(int R2, int M, const int* __restrict__ nbr, const float2* __restrict__ src,
                         float2* __restrict__ dst, const int* __restrict__ need) {
   if (need && *need == 0) return;
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)M * R2) return;
-  const int i = (int)(t / R2), k = (int)(t - (size_t)i * R2);
-  const int n1 = nbr[2 * i] + 1, n2 = nbr[2 * i + 1] + 1;
-  const float2 o = src[(size_t)(i + 1) * R2 + k];
-  const float2 a = src[(size_t)n1 * R2 + k];
-  const float2 b = src[(size_t)n2 * R2 + k];
-  float2 r;
-  r.x = o.x + 0.5f * (a.x + b.x);
-  r.y = o.y + 0.5f * (a.y + b.y);
-  dst[(size_t)(i + 1) * R2 + k] = r;
+  const int lane = threadIdx.x & 63;
+  const long nw = (long)gridDim.x * (blockDim.x >> 6);
+  for (long i = ((long)xcd_block() * blockDim.x + threadIdx.x) >> 6; i < M; i += nw) {
+    const int n1 = nbr[2 * i] + 1, n2 = nbr[2 * i + 1] + 1;
+    const float2* so = src + (size_t)(i + 1) * R2;
+    const float2* s1 = src + (size_t)n1 * R2;
+    const float2* s2 = src + (size_t)n2 * R2;
+    float2* d = dst + (size_t)(i + 1) * R2;
+    for (int k = lane; k < R2; k += 64) {
+      const float2 o = so[k], a = s1[k], b = s2[k];
+      float2 r;
+      r.x = o.x + 0.5f * (a.x + b.x);
+      r.y = o.y + 0.5f * (a.y + b.y);
+      d[k] = r;
+    }
+  }
+}
+
+// bins of every (pixel, label) of the rescaled field, packed lo | up << 4;
+// edge = 1 if some bin_up != bin_lo (then the upper pass is done in full)
+__global__ void __launch_bounds__(kBlock) k_obin(const float* __restrict__ Q, const float* __restrict__ mm, size_t n,
+                       uint8_t* __restrict__ bins, int* __restrict__ edge) {
+  bool e = false;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float q = rescaled(Q, i, mm);
+    const int lo = bin_lo(q), up = bin_up(q);
+    e = e || lo != up;
+    bins[i] = (uint8_t)(lo | (up << 4));
+  }
+  if (__any(e) && (threadIdx.x & 63) == 0) atomicOr(edge, 1);
 }
 
 // lower slice then upper slice (:638-650, :686-696), then out (+)= scale*x.
 // Without edges the upper value at bin c is T - LO[c+1].
-__global__ void k_oslice(const float* __restrict__ Q, const float* __restrict__ mm, int L, RowMap rm, int N, int d1,
+template <int TAG>
+__global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ bins, int L, RowMap rm, int N, int d1,
                          const int* __restrict__ offset, const float* __restrict__ bary,
                          const float* __restrict__ lo_v, const float* __restrict__ up_v, const float* __restrict__ Tb,
                          const int* __restrict__ edge, float alpha, SliceArgs o) {
@@ -652,31 +689,37 @@ __global__ void k_oslice(const float* __restrict__ Q, const float* __restrict__ 
   auto voff = [&](int j) { return staged ? soff[gb + j] : offset[(size_t)pix * d1 + j] + 1; };
   auto vw = [&](int j) { return staged ? sbw[gb + j] : bary[(size_t)pix * d1 + j]; };
   const bool full = *edge != 0;
-  const float q = rescaled(Q, (size_t)pix * L + k0, mm);
-  const int lo = bin_lo(q), up = bin_up(q);
+  const int bb = bins[(size_t)pix * L + k0];
+  const int lo = bb & 15, up = bb >> 4;
+  // one pass over the vertices; the upper terms wait in registers so the sum
+  // keeps the reference's order (all lower terms, then all upper terms)
+  constexpr int kMaxD1 = kMaxD + 1;
+  float ut[kMaxD1];
   float acc = 0.f;
-#pragma unroll 4
-  for (int j = 0; j < d1; ++j) {
-    const float w = vw(j);
-    acc = acc + lo_v[((size_t)voff(j) * L + k0) * 10 + lo] * (-w * alpha);
+#pragma unroll
+  for (int j = 0; j < kMaxD1; ++j) {
+    if (j < d1) {
+      const int of = voff(j);
+      const float w = vw(j);
+      const float* row = lo_v + ((size_t)of * L + k0) * 10;
+      acc = acc + row[lo] * (-w * alpha);
+      float u;
+      if (full)
+        u = up_v[((size_t)of * L + k0) * 10 + up];
+      else
+        u = Tb[of] - (up < 9 ? row[up + 1] : 0.f);
+      ut[j] = u * (w * alpha);
+    }
   }
-#pragma unroll 4
-  for (int j = 0; j < d1; ++j) {
-    const int of = voff(j);
-    const float w = vw(j);
-    float u;
-    if (full)
-      u = up_v[((size_t)of * L + k0) * 10 + up];
-    else
-      u = Tb[of] - (up < 9 ? lo_v[((size_t)of * L + k0) * 10 + up + 1] : 0.f);
-    acc = acc + u * (w * alpha);
-  }
+#pragma unroll
+  for (int j = 0; j < kMaxD1; ++j)
+    if (j < d1) acc = acc + ut[j];
   slice_store(o, (int)pix, k0, L, acc);
 }
 
 __global__ void k_zero_int(int* p) { *p = 0; }
 
-__global__ void k_zero(float* p, size_t n) {
+__global__ void __launch_bounds__(kBlock) k_zero(float* p, size_t n) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = 0.f;
 }
@@ -965,38 +1008,79 @@ void Lattice::ordered(const float* Q, const float* d_minmax, int L, const SliceA
   k_zero<<<grid_for(row), kBlock, 0, s>>>(vB_.get() + half, row);
   int* edge = edge_.get();
   const std::string tag = prof_on() ? "[d=" + std::to_string(d_) + ",L=" + std::to_string(L) + "]" : std::string();
+  const size_t NL = (size_t)N_ * L;
+  bins_.resize(NL);
+  {
+    KTimer tm("obin" + tag, s);
+    k_zero_int<<<1, 1, 0, s>>>(edge);
+    k_obin<<<grid_for(NL, kBlock, 2048), kBlock, 0, s>>>(Q, d_minmax, NL, bins_.get(), edge);
+  }
   {
     KTimer tm("osplat" + tag, s);
-    k_zero_int<<<1, 1, 0, s>>>(edge);
     for (int upper = 0; upper < 2; ++upper) {
       float* dst_half = upper ? srcU : srcL;
-      k_osplat<<<rows_grid(nchunks_, L), kBlock, 0, s>>>(Q, d_minmax, L, rm, upper, sp_pix_.get(), sp_w_.get(),
-                                                         seg_.get(), chstart_.get(), chpoint_.get(), pstart_.get(),
-                                                         nchunks_, dst_half, part_.get(), edge);
+      const long nvb = rows_grid(nchunks_, L);
+      const unsigned grid = upper ? (unsigned)std::min<long>(nvb, 2048) : (unsigned)nvb;
+      auto launch = [&](auto kern) {
+        kern<<<grid, kBlock, 0, s>>>(bins_.get(), L, rm, nvb, upper, sp_pix_.get(), sp_w_.get(), seg_.get(),
+                                     chstart_.get(), chpoint_.get(), pstart_.get(), nchunks_, dst_half, part_.get(),
+                                     edge);
+      };
+      if (d_ == 2)
+        upper ? launch(k_osplat<102>) : launch(k_osplat<2>);
+      else if (d_ == 5)
+        upper ? launch(k_osplat<105>) : launch(k_osplat<5>);
+      else
+        upper ? launch(k_osplat<100>) : launch(k_osplat<0>);
       if (nmulti_ > 0) {
         const RowMap rw = rowmap(row);
-        k_combine_if<<<rows_grid(nmulti_, row), kBlock, 0, s>>>(row, rw, multi_.get(), nmulti_, chstart_.get(),
-                                                               pstart_.get(), part_.get(), dst_half,
-                                                               upper ? edge : nullptr);
+        const unsigned g = upper ? std::min(rows_grid(nmulti_, row), 1024u) : rows_grid(nmulti_, row);
+        k_combine_if<<<g, kBlock, 0, s>>>(row, rw, multi_.get(), nmulti_, chstart_.get(), pstart_.get(), part_.get(),
+                                          dst_half, upper ? edge : nullptr);
       }
     }
     check_launch("k_osplat");
   }
   const int R2 = row / 2;
+  auto blur = [&](auto kern, const int* nb, const float* a, float* b, const int* need) {
+    const unsigned g = need ? std::min(grid_for((size_t)M_ * 64), 2048u) : grid_for((size_t)M_ * 64);
+    kern<<<g, kBlock, 0, s>>>(R2, M_, nb, reinterpret_cast<const float2*>(a), reinterpret_cast<float2*>(b), need);
+  };
   for (int j = 0; j <= d_; ++j) {  // forward order only (:621, :669)
-    KTimer tm("oblur" + tag, s);
     const int* nb = nbr_.get() + (size_t)j * M_ * 2;
-    k_blur2<<<grid_for((size_t)M_ * R2), kBlock, 0, s>>>(R2, M_, nb, reinterpret_cast<const float2*>(srcL),
-                                                         reinterpret_cast<float2*>(dstL), nullptr);
-    k_blur2<<<grid_for((size_t)M_ * R2), kBlock, 0, s>>>(R2, M_, nb, reinterpret_cast<const float2*>(srcU),
-                                                         reinterpret_cast<float2*>(dstU), edge);
+    {
+      KTimer tm("oblur" + tag, s);
+      if (d_ == 2)
+        blur(k_blur2<2>, nb, srcL, dstL, nullptr);
+      else if (d_ == 5)
+        blur(k_blur2<5>, nb, srcL, dstL, nullptr);
+      else
+        blur(k_blur2<0>, nb, srcL, dstL, nullptr);
+    }
+    {
+      KTimer tm("oblur_upper" + tag, s);
+      if (d_ == 2)
+        blur(k_blur2<102>, nb, srcU, dstU, edge);
+      else if (d_ == 5)
+        blur(k_blur2<105>, nb, srcU, dstU, edge);
+      else
+        blur(k_blur2<100>, nb, srcU, dstU, edge);
+    }
     check_launch("k_blur2");
     std::swap(srcL, dstL);
     std::swap(srcU, dstU);
   }
   KTimer tm("oslice" + tag, s);
-  k_oslice<<<rows_grid(N_, L), kBlock, 0, s>>>(Q, d_minmax, L, rm, N_, d_ + 1, offset_.get(), bary_.get(), srcL,
-                                               srcU, tb_.get(), edge, lattice_alpha(d_), out);
+  auto slice = [&](auto kern) {
+    kern<<<rows_grid(N_, L), kBlock, 0, s>>>(bins_.get(), L, rm, N_, d_ + 1, offset_.get(), bary_.get(), srcL, srcU,
+                                             tb_.get(), edge, lattice_alpha(d_), out);
+  };
+  if (d_ == 2)
+    slice(k_oslice<2>);
+  else if (d_ == 5)
+    slice(k_oslice<5>);
+  else
+    slice(k_oslice<0>);
   check_launch("k_oslice");
 }
 

@@ -95,6 +95,7 @@ class Lattice {
   // ordered filter: blurred total weight per point and the bin-edge flag
   DBuf<float> tb_;
   DBuf<int> edge_;
+  DBuf<uint8_t> bins_;
   bool have_tb_ = false;
 };
 

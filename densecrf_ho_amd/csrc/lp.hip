@@ -91,7 +91,7 @@ struct LPConst {
 // alpha/u update of the previous step (:737-738, fused here), then
 // Y = lambda * (tmp - sum(tmp)/M) + cur_Q with tmp = alpha + u - U (:627-635).
 // Row groups; sum(tmp) is the sum of an aligned copy (qp_gamma_multiplyC).
-__global__ void k_lp_Y(int N, int L, LPConst c, float* __restrict__ alpha, float* __restrict__ u,
+__global__ void __launch_bounds__(kBlock) k_lp_Y(int N, int L, LPConst c, float* __restrict__ alpha, float* __restrict__ u,
                        const float* __restrict__ sa, const float* __restrict__ su, const double* __restrict__ sc,
                        int apply_update, const float* __restrict__ U, const float* __restrict__ curQ,
                        float* __restrict__ Y) {
@@ -129,7 +129,8 @@ __global__ void k_lp_Y(int N, int L, LPConst c, float* __restrict__ alpha, float
 // one qp-gamma sweep of a pixel (qp_gamma_step :495-514, qp_gamma_multiplyC
 // :517-523, qp_values :667); returns the pixel's qp value
 template <int MAXL>
-__device__ inline float qp_sweep(float (&g)[MAXL], const float (&y)[MAXL], int L, const LPConst& c) {
+__device__ inline __attribute__((always_inline)) float qp_sweep(float (&g)[MAXL], const float (&y)[MAXL], int L,
+                                                                const LPConst& c) {
   const float sum = eigen_sum_reg<MAXL>(g, L);
 #pragma unroll
   for (int l = 0; l < MAXL; ++l)
@@ -148,24 +149,28 @@ __device__ inline float qp_sweep(float (&g)[MAXL], const float (&y)[MAXL], int L
   return (float)(0.5 * (double)eigen_dot_reg<MAXL>(g, vt, L) + (double)eigen_dot_reg<MAXL>(g, y, L));
 }
 
-// pass 1 (write == 0): `nsw` sweeps from the stored gamma, per-sweep block
-// partial sums of qp_values, gamma not written.  pass 2 (write == 1): replays
-// exactly fl[kLF_SWEEPS] sweeps and stores gamma.  One thread per pixel holds
-// its row in registers; rows move through LDS so that global accesses of the
-// block's kBlock x L tile coalesce.
-template <int MAXL>
-__global__ void k_lp_sweeps(int N, int L, LPConst c, float* __restrict__ gamma, const float* __restrict__ Y, int nsw,
-                            int write, double* __restrict__ parts, const int* __restrict__ fl) {
+// qp-gamma sweeps (:650-675), one thread per pixel holding its row in
+// registers; rows move through LDS so the block's kBlock x L tiles load and
+// store coalesced.
+//   pass 1 (replay == 0): `nsw` sweeps from gin, gamma after nsw sweeps to
+//     gout, per-sweep block partial sums of qp_values to parts
+//   replay (replay == 1): if the loop stopped after s < nsw sweeps
+//     (fl[kLF_SWEEPS]), redo s sweeps from gin into gout; otherwise no-op
+template <int MAXL, int LT = 0>  // LT > 0: the label count is known at compile time
+__global__ void __launch_bounds__(kBlock) k_lp_sweeps(int N, int Lr, LPConst c, const float* __restrict__ gin, float* __restrict__ gout,
+                            const float* __restrict__ Y, int nsw, int replay, double* __restrict__ parts,
+                            const int* __restrict__ fl) {
   __shared__ float tile[kBlock * MAXL];
-  if (write && fl[kLF_SWEEPS] == 0) return;
-  const int n = write ? fl[kLF_SWEEPS] : nsw;
+  const int L = LT > 0 ? LT : Lr;
+  if (replay && fl[kLF_SWEEPS] >= nsw) return;
+  const int n = replay ? fl[kLF_SWEEPS] : nsw;
   const int p0 = blockIdx.x * kBlock;
   const int np = min(kBlock, N - p0);
   const size_t base = (size_t)p0 * L;
   const int t = threadIdx.x;
   const bool act = t < np;
   float g[MAXL], y[MAXL];
-  for (int i = t; i < np * L; i += kBlock) tile[i] = gamma[base + i];
+  for (int i = t; i < np * L; i += kBlock) tile[i] = gin[base + i];
   __syncthreads();
 #pragma unroll
   for (int l = 0; l < MAXL; ++l) g[l] = (act && l < L) ? tile[t * L + l] : 0.f;
@@ -174,22 +179,26 @@ __global__ void k_lp_sweeps(int N, int L, LPConst c, float* __restrict__ gamma, 
   __syncthreads();
 #pragma unroll
   for (int l = 0; l < MAXL; ++l) y[l] = (act && l < L) ? tile[t * L + l] : 0.f;
+  double acc[kSweepBatch];
+#pragma unroll
+  for (int sw = 0; sw < kSweepBatch; ++sw) acc[sw] = 0.0;
   for (int sw = 0; sw < n; ++sw) {
     const float v = act ? qp_sweep<MAXL>(g, y, L, c) : 0.f;
-    if (!write) {
-      double vv[1] = {(double)v};
-      block_sum<1>(vv);
-      if (t == 0) parts[(size_t)sw * gridDim.x + blockIdx.x] = vv[0];
-    }
-  }
-  if (write) {
-    __syncthreads();
 #pragma unroll
-    for (int l = 0; l < MAXL; ++l)
-      if (act && l < L) tile[t * L + l] = g[l];
-    __syncthreads();
-    for (int i = t; i < np * L; i += kBlock) gamma[base + i] = tile[i];
+    for (int q = 0; q < kSweepBatch; ++q)
+      if (q == sw) acc[q] = (double)v;
   }
+  if (!replay) {
+    block_sum<kSweepBatch>(acc);
+    if (t == 0)
+      for (int sw = 0; sw < n; ++sw) parts[(size_t)sw * gridDim.x + blockIdx.x] = acc[sw];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int l = 0; l < MAXL; ++l)
+    if (act && l < L) tile[t * L + l] = g[l];
+  __syncthreads();
+  for (int i = t; i < np * L; i += kBlock) gout[base + i] = tile[i];
 }
 
 // stopping sweep of the batch (:670-675): the first sweep whose float total
@@ -246,7 +255,7 @@ __device__ inline float nmax(float a, float b) { return (a != a || b != b) ? NAN
 // beta = -colsum(alpha+u+gamma-U)/M, Q = lambda*(alpha+beta+u+gamma-U) + cur_Q
 // (:684-697), row groups; the column sum follows Eigen's order for the
 // column's alignment.  Block min/max of Q for rescale (:699).
-__global__ void k_lp_betaQ(int N, int L, float lambda, const float* __restrict__ alpha, const float* __restrict__ u,
+__global__ void __launch_bounds__(kBlock) k_lp_betaQ(int N, int L, float lambda, const float* __restrict__ alpha, const float* __restrict__ u,
                            const float* __restrict__ gamma, const float* __restrict__ U,
                            const float* __restrict__ curQ, float* __restrict__ Q, float* __restrict__ mmparts) {
   __shared__ float seg[kBlock];
@@ -321,7 +330,7 @@ __global__ void k_lp_minmax_final(const float* __restrict__ parts, int nb, float
 }
 
 // tmp = alpha + u - (sa + su); gap = tmp.Q; tt = tmp.tmp (:725-729)
-__global__ void k_lp_gap(size_t n, const float* __restrict__ alpha, const float* __restrict__ u,
+__global__ void __launch_bounds__(kBlock) k_lp_gap(size_t n, const float* __restrict__ alpha, const float* __restrict__ u,
                          const float* __restrict__ sa, const float* __restrict__ su, const float* __restrict__ Q,
                          double* __restrict__ parts) {
   double v[2] = {0.0, 0.0};
@@ -355,7 +364,7 @@ __global__ void k_lp_delta(double* __restrict__ sc, float lambda, double* __rest
 // (:929-938); row groups.  The descending sort of sortCols becomes a rank per
 // value (ties broken by index), the threshold loop runs on the sorted row in
 // LDS, and the column sums follow Eigen's order for the column's alignment.
-__global__ void k_lp_project(int N, int L, const float* __restrict__ Qin, float* __restrict__ Qout,
+__global__ void __launch_bounds__(kBlock) k_lp_project(int N, int L, const float* __restrict__ Qin, float* __restrict__ Qout,
                              int* __restrict__ lab, int do_feasible) {
   __shared__ float sq[kBlock], ss[kBlock];
   const long ntiles = rg_tiles(N, L);
@@ -423,7 +432,7 @@ __global__ void k_lp_project(int N, int L, const float* __restrict__ Qin, float*
 }
 
 // max_rounding (:832-840)
-__global__ void k_onehot(const int* __restrict__ lab, int N, int L, float* __restrict__ Q) {
+__global__ void __launch_bounds__(kBlock) k_onehot(const int* __restrict__ lab, int N, int L, float* __restrict__ Q) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)N * L) return;
   const int i = (int)(t / L), l = (int)(t - (size_t)i * L);
@@ -454,7 +463,7 @@ __global__ void k_onehot(const int* __restrict__ lab, int N, int L, float* __res
   } while (0)
 
 struct CRF::LPState {
-  DBuf<float> alpha, u, sa, su, gamma, Y, Q, curQ, best, tmp;
+  DBuf<float> alpha, u, sa, su, gamma, gamma2, Y, Q, curQ, best, tmp;
   DBuf<int> lab, iscratch;
   DBuf<double> parts, sc;
   DBuf<float> mmparts, mm;
@@ -542,7 +551,8 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
   hipStream_t s = s_;
   const bool ho = R_ > 0;
   if (ho) ho_state();
-  for (DBuf<float>* b : {&st.alpha, &st.sa, &st.gamma, &st.Y, &st.Q, &st.curQ, &st.best, &st.tmp}) b->resize(NL);
+  for (DBuf<float>* b : {&st.alpha, &st.sa, &st.gamma, &st.gamma2, &st.Y, &st.Q, &st.curQ, &st.best, &st.tmp})
+    b->resize(NL);
   if (ho) {
     st.u.resize(NL);
     st.su.resize(NL);
@@ -606,7 +616,7 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
     energy = read_d(sc + kL_TMP + 1);
   }
   double best_energy = energy, best_int_energy = int_energy;
-  int it = 0, count = 0, inner_idx = 0;
+  int it = 0, count = 0, inner_idx = 0, prev_sweeps = kSweepBatch;
   do {
     ++it;
     k_fill<<<grid_for(NL), kBlock, 0, s>>>(st.gamma.get(), NL, 1.f);
@@ -621,28 +631,41 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
         k_lp_Y<<<rg_grid(N, L, 8192), kBlock, 0, s>>>(N, L, c, st.alpha.get(), u, st.sa.get(), su, sc,
                                                      pit > 1 ? 1 : 0, U_.get(), st.curQ.get(), st.Y.get());
       }
-      // qp-gamma sweeps in batches; the replay leaves gamma after the
-      // reference's number of sweeps
+      // qp-gamma sweeps in batches: pass 1 runs `nsw` sweeps gamma -> gamma2,
+      // the decision finds where the reference's loop stops, the replay redoes
+      // that many sweeps only when the stop comes before the end of the batch.
+      // The batch length follows the previous step's sweep count.
       int done_sweeps = 0;
       while (true) {
-        const int nsw = std::min(kSweepBatch, P.qp_max_iter - done_sweeps);
+        const int guess = pit == 1 ? kSweepBatch : std::max(2, std::min(kSweepBatch, prev_sweeps));
+        const int nsw = std::min(guess, P.qp_max_iter - done_sweeps);
         {
           KTimer tm("lp_sweeps", s);
-          DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbp, kBlock, 0, s>>>(N, L, c, st.gamma.get(), st.Y.get(), nsw, 0,
-                                                                         st.parts.get(), fl)));
+          if (L == 21)
+            k_lp_sweeps<21, 21><<<nbp, kBlock, 0, s>>>(N, L, c, st.gamma.get(), st.gamma2.get(), st.Y.get(), nsw, 0,
+                                                       st.parts.get(), fl);
+          else
+            DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbp, kBlock, 0, s>>>(N, L, c, st.gamma.get(), st.gamma2.get(),
+                                                                           st.Y.get(), nsw, 0, st.parts.get(), fl)));
         }
         k_lp_sweep_decide<<<1, 1024, 0, s>>>(st.parts.get(), nbp, nsw, P.qp_tol, done_sweeps, P.qp_max_iter, sc, fl);
         {
           KTimer tm("lp_sweeps_replay", s);
-          DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbp, kBlock, 0, s>>>(N, L, c, st.gamma.get(), st.Y.get(), nsw, 1,
-                                                                         st.parts.get(), fl)));
+          if (L == 21)
+            k_lp_sweeps<21, 21><<<nbp, kBlock, 0, s>>>(N, L, c, st.gamma.get(), st.gamma2.get(), st.Y.get(), nsw, 1,
+                                                       st.parts.get(), fl);
+          else
+            DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbp, kBlock, 0, s>>>(N, L, c, st.gamma.get(), st.gamma2.get(),
+                                                                           st.Y.get(), nsw, 1, st.parts.get(), fl)));
         }
+        std::swap(st.gamma, st.gamma2);
         int hf[2] = {0, 0};
         DCRF_HIP(hipMemcpyAsync(hf, fl, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
         DCRF_HIP(hipStreamSynchronize(s));
         done_sweeps += hf[kLF_SWEEPS];
         if (hf[kLF_SWEEP_DONE] || done_sweeps >= P.qp_max_iter) break;
       }
+      prev_sweeps = done_sweeps;
       res.qp_sweeps.push_back(done_sweeps);
       {
         KTimer tm("lp_betaQ", s);
