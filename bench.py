@@ -170,6 +170,14 @@ def main():
     D.profile_enable(False)
 
     ms_per_step = elapsed / args.steps * 1e3
+    # HBM bytes per launch of each kernel, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes over this same workload (scaled by the
+    # counters' reading of a copy kernel of known size)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic_1080p_L21.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh)
     # dominant lattice kernel by total time
     lat_k = {n: v for n, v in prof.items() if "[" in n}
     dom = max(lat_k, key=lambda n: lat_k[n][1])
@@ -206,7 +214,9 @@ def main():
         "filter_gbps": None if filter_gbs is None else round(filter_gbs, 1),
         "images_per_s": None,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": (traffic or {}).get("bytes_per_launch", {}).get(dom),
+                     "traffic_source": traffic["source"] if traffic else None,
                      "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": int(ab)},
         "kernels": {n: {"launches": c, "total_ms": round(m, 3)} for n, (c, m) in sorted(prof.items())},
     }
