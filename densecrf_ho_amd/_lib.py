@@ -52,6 +52,14 @@ SIGNATURES = {
     "dcrf_crf_destroy": (_i, [_vp]),
     "dcrf_crf_set_unary": (_i, [_vp, _vp, _i]),
     "dcrf_crf_add_pairwise": (_i, [_vp, _vp, _i, _f, _i, _i]),
+    "dcrf_crf_add_pairwise_compat": (_i, [_vp, _vp, _i, _i, _vp, _i, _i]),
+    "dcrf_crf_pairwise_nparams": (_i, [_vp, _i, _ip]),
+    "dcrf_crf_pairwise_parameters": (_i, [_vp, _i, _vp]),
+    "dcrf_crf_pairwise_set_parameters": (_i, [_vp, _i, _vp]),
+    "dcrf_crf_pairwise_gradient": (_i, [_vp, _i, _vp, _vp, _vp, _i]),
+    "dcrf_crf_meanfield": (_i, [_vp, _i, _vp, _i]),
+    "dcrf_crf_meanfield_backward": (_i, [_vp, _vp, _vp, _vp, _i]),
+    "dcrf_crf_cross_entropy": (_i, [_vp, _vp, _vp, _vp, C.POINTER(_d), _i]),
     "dcrf_crf_add_pairwise_gaussian": (_i, [_vp, _i, _i, _f, _f, _f]),
     "dcrf_crf_add_pairwise_bilateral": (_i, [_vp, _i, _i, _f, _f, _f, _f, _f, _vp, _f]),
     "dcrf_crf_add_superpixel_layer": (_i, [_vp, _vp, _i, _vp, _vp]),

@@ -139,8 +139,38 @@ class PottsCompatibility:
     def __init__(self, weight: float = 1.0):
         self.w = float(weight)
 
+    kind = 0
+
+    def params(self):
+        return np.array([self.w], np.float32)
+
     def parameters(self):
         return np.array([self.w], np.float32)
+
+
+class DiagonalCompatibility:
+    """DiagonalCompatibility(v): out = diag(v) Q (src/labelcompatibility.cpp:78-96)."""
+
+    kind = 1
+
+    def __init__(self, v):
+        self.v = np.ascontiguousarray(v, np.float32).reshape(-1)
+
+    def params(self):
+        return self.v
+
+
+class MatrixCompatibility:
+    """MatrixCompatibility(m): out = W Q with W = (m + m^T)/2
+    (src/labelcompatibility.cpp:97-129); m is (L, L), m[i, j] row i column j."""
+
+    kind = 2
+
+    def __init__(self, m):
+        self.m = np.asarray(m, np.float32)
+
+    def params(self):
+        return np.ascontiguousarray(self.m.T).reshape(-1)  # column-major, as Eigen stores it
 
 
 class Permutohedral:
@@ -254,13 +284,79 @@ class DenseCRF:
         a = _Arg(unary, np.float32)
         _check(_lib.load().dcrf_crf_set_unary(self._h, a.ptr, a.dev))
 
-    def addPairwiseEnergy(self, features, function: PottsCompatibility | None = None, kernel_type=1,
-                          normalization_type=NORMALIZE_SYMMETRIC):
-        w = (function or PottsCompatibility(1.0)).w
+    def addPairwiseEnergy(self, features, function=None, kernel_type=1, normalization_type=NORMALIZE_SYMMETRIC):
+        """addPairwiseEnergy(features (N, d), LabelCompatibility, kernel type, normalisation)."""
+        function = function or PottsCompatibility(1.0)
         a = _Arg(features, np.float32)
         d = a.arr.shape[1]
-        _check(_lib.load().dcrf_crf_add_pairwise(self._h, a.ptr, d, w, int(normalization_type), a.dev))
+        if function.kind == 0:
+            _check(_lib.load().dcrf_crf_add_pairwise(self._h, a.ptr, d, function.w, int(normalization_type), a.dev))
+        else:
+            par = np.ascontiguousarray(function.params(), np.float32)
+            _check(_lib.load().dcrf_crf_add_pairwise_compat(self._h, a.ptr, d, function.kind,
+                                                            par.ctypes.data_as(C.c_void_p),
+                                                            int(normalization_type), a.dev))
         self.npairwise += 1
+
+    def pairwise_parameters(self, k):
+        """PairwisePotential::parameters() of potential k."""
+        n = C.c_int()
+        _check(_lib.load().dcrf_crf_pairwise_nparams(self._h, k, C.byref(n)))
+        out = np.zeros(n.value, np.float32)
+        _check(_lib.load().dcrf_crf_pairwise_parameters(self._h, k, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def set_pairwise_parameters(self, k, v):
+        """PairwisePotential::setParameters(v) of potential k."""
+        v = np.ascontiguousarray(v, np.float32)
+        if v.size != self.pairwise_parameters(k).size:
+            raise ValueError("wrong number of parameters")
+        _check(_lib.load().dcrf_crf_pairwise_set_parameters(self._h, k, v.ctypes.data_as(C.c_void_p)))
+
+    def pairwise_gradient(self, k, b, Q):
+        """PairwisePotential::gradient(b, Q) of potential k (parameters() layout)."""
+        g = np.zeros(self.pairwise_parameters(k).size, np.float32)
+        a, q = _Arg(b, np.float32), _Arg(Q, np.float32)
+        if a.dev != q.dev:
+            raise TypeError("b and Q must both be host arrays or both device buffers")
+        _check(_lib.load().dcrf_crf_pairwise_gradient(self._h, k, a.ptr, q.ptr, g.ctypes.data_as(C.c_void_p), a.dev))
+        return g
+
+    # ---- unrolled mean field and its adjoint ------------------------------------
+    def meanfield(self, iterations: int, out=None, device=None):
+        """Q after `iterations` mean-field updates from softmax(-U); keeps the
+        per-iteration checkpoints for meanfield_backward."""
+        if out is None:
+            out = np.empty((self.N, self.M), np.float32) if device is None else _torch_empty((self.N, self.M), device)
+        o = _Arg(out, np.float32, out=True)
+        _check(_lib.load().dcrf_crf_meanfield(self._h, int(iterations), o.ptr, o.dev))
+        return out
+
+    def meanfield_backward(self, dQ, out=None):
+        """dL/dQ_T -> (dL/dU, [dL/dparameters of each potential] (FP64))."""
+        a = _Arg(dQ, np.float32)
+        out = _empty_like_field(dQ, (self.N, self.M)) if out is None else out
+        o = _Arg(out, np.float32, out=True)
+        sizes = [self.pairwise_parameters(k).size for k in range(self.npairwise)]
+        g = np.zeros(max(sum(sizes), 1))
+        _check(_lib.load().dcrf_crf_meanfield_backward(self._h, a.ptr, o.ptr, g.ctypes.data_as(C.c_void_p), a.dev))
+        parts, off = [], 0
+        for n in sizes:
+            parts.append(g[off:off + n].copy())
+            off += n
+        return out, parts
+
+    def cross_entropy(self, Q, labels, out=None):
+        """(-sum_i log Q(y_i, i) over labelled pixels, dloss/dQ)."""
+        a = _Arg(Q, np.float32)
+        lab = _Arg(labels, np.int32)
+        if a.dev != lab.dev:
+            raise TypeError("Q and labels must both be host arrays or both device buffers")
+        out = _empty_like_field(Q, (self.N, self.M)) if out is None else out
+        o = _Arg(out, np.float32, out=True)
+        e = C.c_double()
+        _check(_lib.load().dcrf_crf_cross_entropy(self._h, a.ptr, lab.ptr, o.ptr, C.byref(e), a.dev))
+        return e.value, out
 
     def addSuperPixelLayer(self, region, exp_of_superpixels, constants):
         """Superpixel layer from a region map (the state DenseCRF2D::addSuperPixel builds)."""
@@ -389,7 +485,7 @@ class DenseCRF2D(DenseCRF):
 
     def addPairwiseGaussian(self, sx, sy, function: PottsCompatibility | None = None, kernel_type=1,
                             normalization_type=NORMALIZE_SYMMETRIC):
-        if normalization_type != NORMALIZE_SYMMETRIC:
+        if normalization_type != NORMALIZE_SYMMETRIC or getattr(function, "kind", 0) != 0:
             from .synthetic import gaussian_features
             return self.addPairwiseEnergy(gaussian_features(self.W, self.H, sx, sy), function, kernel_type,
                                           normalization_type)
@@ -400,7 +496,7 @@ class DenseCRF2D(DenseCRF):
     def addPairwiseBilateral(self, sx, sy, sr, sg, sb, im, function: PottsCompatibility | None = None,
                              kernel_type=1, normalization_type=NORMALIZE_SYMMETRIC):
         im = np.ascontiguousarray(im, np.uint8).reshape(-1)
-        if normalization_type != NORMALIZE_SYMMETRIC:
+        if normalization_type != NORMALIZE_SYMMETRIC or getattr(function, "kind", 0) != 0:
             from .synthetic import bilateral_features
             return self.addPairwiseEnergy(bilateral_features(self.W, self.H, sx, sy, sr, sg, sb, im), function,
                                           kernel_type, normalization_type)

@@ -298,6 +298,86 @@ int dcrf_crf_add_pairwise(dcrf_crf_t crf, const float* features, int d, float po
     c.add_pairwise(f.d, d, potts_w, ntype);
   });
 }
+int dcrf_crf_add_pairwise_compat(dcrf_crf_t crf, const float* features, int d, int compat_type,
+                                 const float* params, int ntype, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && features && params && d > 0, "dcrf_crf_add_pairwise_compat: bad arguments");
+    CRF& c = *crf->crf;
+    DCRF_REQUIRE(compat_type >= DCRF_COMPAT_POTTS && compat_type <= DCRF_COMPAT_MATRIX,
+                 "dcrf_crf_add_pairwise_compat: unknown compatibility type");
+    Staged<float> f(features, (size_t)c.N() * d, on_device != 0, c.stream(), true);
+    c.add_pairwise(f.d, d, compat_type == DCRF_COMPAT_POTTS ? params[0] : 1.f, ntype);
+    potential_set_compat(c.pairwise(c.num_pairwise() - 1), compat_type, params, c.L(), c.stream());
+  });
+}
+int dcrf_crf_pairwise_nparams(dcrf_crf_t crf, int k, int* n) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && n && k >= 0 && k < crf->crf->num_pairwise(), "dcrf_crf_pairwise_nparams: bad arguments");
+    *n = potential_nparams(crf->crf->pairwise(k), crf->crf->L());
+  });
+}
+int dcrf_crf_pairwise_parameters(dcrf_crf_t crf, int k, float* params) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && params && k >= 0 && k < crf->crf->num_pairwise(),
+                 "dcrf_crf_pairwise_parameters: bad arguments");
+    potential_parameters(crf->crf->pairwise(k), crf->crf->L(), params);
+  });
+}
+int dcrf_crf_pairwise_set_parameters(dcrf_crf_t crf, int k, const float* params) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && params && k >= 0 && k < crf->crf->num_pairwise(),
+                 "dcrf_crf_pairwise_set_parameters: bad arguments");
+    potential_set_parameters(crf->crf->pairwise(k), crf->crf->L(), params, crf->crf->stream());
+  });
+}
+int dcrf_crf_pairwise_gradient(dcrf_crf_t crf, int k, const float* b, const float* Q, float* grad, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && b && Q && grad && k >= 0 && k < crf->crf->num_pairwise(),
+                 "dcrf_crf_pairwise_gradient: bad arguments");
+    CRF& c = *crf->crf;
+    const size_t n = (size_t)c.N() * c.L();
+    Staged<float> sb(b, n, on_device != 0, c.stream(), true);
+    Staged<float> sq(Q, n, on_device != 0, c.stream(), true);
+    c.pairwise_gradient(k, sb.d, sq.d, grad);
+  });
+}
+int dcrf_crf_meanfield(dcrf_crf_t crf, int iterations, float* Q, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && Q, "dcrf_crf_meanfield: bad arguments");
+    CRF& c = *crf->crf;
+    const size_t n = (size_t)c.N() * c.L();
+    Staged<float> q(Q, n, on_device != 0, c.stream(), false);
+    c.meanfield(iterations, q.d);
+    q.copy_out();
+    DCRF_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
+int dcrf_crf_meanfield_backward(dcrf_crf_t crf, const float* dQ, float* dU, double* dparams, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && dQ && dU, "dcrf_crf_meanfield_backward: bad arguments");
+    CRF& c = *crf->crf;
+    const size_t n = (size_t)c.N() * c.L();
+    Staged<float> g(dQ, n, on_device != 0, c.stream(), true);
+    Staged<float> u(dU, n, on_device != 0, c.stream(), false);
+    c.meanfield_backward(g.d, u.d, dparams);
+    u.copy_out();
+    DCRF_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
+int dcrf_crf_cross_entropy(dcrf_crf_t crf, const float* Q, const int* labels, float* dQ, double* loss,
+                           int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && Q && labels && dQ && loss, "dcrf_crf_cross_entropy: bad arguments");
+    CRF& c = *crf->crf;
+    const size_t n = (size_t)c.N() * c.L();
+    Staged<float> q(Q, n, on_device != 0, c.stream(), true);
+    Staged<int> l(labels, c.N(), on_device != 0, c.stream(), true);
+    Staged<float> g(dQ, n, on_device != 0, c.stream(), false);
+    *loss = c.cross_entropy(q.d, l.d, g.d);
+    g.copy_out();
+    DCRF_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
 int dcrf_crf_add_pairwise_gaussian(dcrf_crf_t crf, int W, int H, float sx, float sy, float potts_w) {
   return guarded([&] {
     DCRF_REQUIRE(crf, "dcrf_crf_add_pairwise_gaussian: null handle");

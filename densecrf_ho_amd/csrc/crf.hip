@@ -85,6 +85,7 @@ void potential_init(Potential& p, const float* d_feat, int N, int d, float w, in
   DCRF_REQUIRE(ntype >= 0 && ntype <= 3, "addPairwiseEnergy: bad normalisation type");
   p.d = d;
   p.w = w;
+  p.nn_w = w;
   p.ntype = ntype;
   p.lat = std::make_unique<Lattice>(s);
   p.lat->build(d_feat, N, d);
@@ -134,17 +135,20 @@ void potential_apply(Potential& p, const float* in, float* out, int V, bool tran
   SliceArgs o;
   o.out = out;
   o.norm = post_norm(nt, transpose) ? p.norm.get() : nullptr;
-  o.scale = -p.w;
-  o.apply_scale = true;
-  p.lat->filter(a, o, V, transpose);
+  if (no_norm) {  // the no-norm copy: Potts with the first parameter
+    o.scale = -p.nn_w;
+    o.apply_scale = true;
+    p.lat->filter(a, o, V, transpose);
+    return;
+  }
+  compat_slice(p, o, V, p.lat->stream(), [&](const SliceArgs& oo) { p.lat->filter(a, oo, V, transpose); });
 }
 
 void potential_apply_ord(Potential& p, const float* q01, const float* d_zero_minmax, float* out, int V) {
   SliceArgs o;
   o.out = out;
-  o.scale = -p.w;
-  o.apply_scale = true;
-  p.lat->ordered(q01, d_zero_minmax, V, o);
+  compat_slice(p, o, V, p.lat->stream(),
+               [&](const SliceArgs& oo) { p.lat->ordered(q01, d_zero_minmax, V, oo); });
 }
 
 void CRF::apply(int k, const float* in, float* out, bool transpose, bool no_norm) {
@@ -209,10 +213,8 @@ QPResult CRF::qp(const float* init, float* out, int max_iter, bool ref_stop, int
       SliceArgs o;
       o.out = psx;
       o.norm = post_norm(p.ntype, false) ? p.norm.get() : nullptr;
-      o.scale = -p.w;
-      o.apply_scale = true;
       o.accumulate = k > 0;
-      p.lat->filter(a, o, L_, false);
+      compat_slice(p, o, L_, s, [&](const SliceArgs& oo) { p.lat->filter(a, oo, L_, false); });
     }
     {
       KTimer tm("qp_dots", s);
@@ -368,7 +370,7 @@ double CRF::assignment_energy_true(const int* lab) {
     a.lab = lab;
     SliceArgs o;
     o.out = m.tmp.get();
-    o.scale = -p.w;
+    o.scale = -p.nn_w;  // the no-norm copy (Potts)
     o.apply_scale = true;
     p.lat->filter(a, o, L_, false);
     k_pairwise_true_accum<<<grid_for(N_), kBlock, 0, s_>>>(m.tmp.get(), lab, m.rpix.get(), N_, L_);

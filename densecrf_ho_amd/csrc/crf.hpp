@@ -6,6 +6,7 @@
 // index pixel*L + label, exactly as the reference's MatrixXf.
 #pragma once
 
+#include <functional>
 #include <memory>
 #include <vector>
 
@@ -18,13 +19,46 @@ void check_unit_interval(const float* d, size_t n, hipStream_t s);
 
 enum NormType : int { kNoNorm = 0, kNormBefore = 1, kNormAfter = 2, kNormSymmetric = 3 };
 
+// LabelCompatibility kinds (include/labelcompatibility.h:51-92)
+enum CompatType : int { kPotts = 0, kDiagonal = 1, kMatrix = 2 };
+
 struct Potential {
   std::unique_ptr<Lattice> lat;
   DBuf<float> norm;  // per pixel (DenseKernel::norm_)
   int d = 0;
   float w = 1.f;     // Potts weight
   int ntype = kNormSymmetric;
+  // Diagonal / Matrix compatibility: cL labels, host and device copies of
+  // v[L] or W[L*L] = (m + m^T)/2 (column-major, symmetric)
+  int ctype = kPotts;
+  int cL = 0;
+  std::vector<float> cparam;
+  DBuf<float> cdev;
+  DBuf<float> ctmp;  // N x L filter output before the compatibility pass
+  // the NO_NORMALIZATION copy used by the true energies is always
+  // PottsCompatibility(parameters()(0)), taken when the term is added
+  // (src/densecrf.cpp:161-168)
+  float nn_w = 1.f;
 };
+
+// Gives a freshly added potential a Diagonal (params v[L]) or Matrix (params
+// m[L*L] column-major, stored as (m + m^T)/2) compatibility, or a Potts weight
+// (params[0]); also fixes the no-norm copy's Potts weight to parameters()(0).
+void potential_set_compat(Potential& p, int ctype, const float* params, int L, hipStream_t s);
+// LabelCompatibility::parameters / setParameters (src/labelcompatibility.cpp:61-68, :88-96, :109-121)
+int potential_nparams(const Potential& p, int L);
+void potential_parameters(const Potential& p, int L, float* out);
+void potential_set_parameters(Potential& p, int L, const float* v, hipStream_t s);
+// Runs `filt` with a SliceArgs that writes the filtered field, then applies the
+// compatibility of p: Potts folds into the slice (x -> -w x); Diagonal and
+// Matrix filter into p.ctmp and a per-pixel pass writes (or adds) C x to o.out.
+void compat_slice(Potential& p, const SliceArgs& o, int V, hipStream_t s,
+                  const std::function<void(const SliceArgs&)>& filt);
+// PairwisePotential::gradient(b, Q) = compatibility.gradient(b, K Q)
+// (src/pairwise.cpp:390-395), device fields, result in the layout of
+// parameters(), accumulated (FP64) into the device array acc.
+void potential_gradient_dev(Potential& p, int L, const float* b, const float* Q, double* acc, double* parts,
+                            hipStream_t s);
 
 // DenseKernel::initLattice: build the lattice and the normalisation
 void potential_init(Potential& p, const float* d_feat, int N, int d, float w, int ntype, hipStream_t s);
@@ -88,6 +122,20 @@ class CRF {
   // z_labels_ after the last QP (L x R, region-major), host pointer
   void z_labels(float* host);
 
+  // ---- unrolled mean field and its adjoint ------------------------------------
+  // T updates from Q0 = softmax(-U): Q <- softmax(-U - sum_k apply_k(Q));
+  // keeps Q_0..Q_T and the filtered fields K_k Q_t for meanfield_backward.
+  void meanfield(int T, float* Qout);
+  // dL/dQ_T -> dL/dU (device N x L) and every potential's parameter gradient,
+  // back to back, into the host array dparams (FP64)
+  void meanfield_backward(const float* dQ, float* dU, double* dparams);
+  int meanfield_iterations() const;
+  // -sum_i log Q(y_i, i) over labelled pixels (y_i in [0,L)) and its gradient
+  double cross_entropy(const float* Q, const int* lab, float* dQ);
+  // PairwisePotential::gradient for potential k, host result (parameters() layout)
+  void pairwise_gradient(int k, const float* b, const float* Q, float* host_out);
+  struct MFState;
+
   struct Impl;
   struct HOState;
   struct LPState;
@@ -108,6 +156,7 @@ class CRF {
   void ho_qp_after(hipStream_t s, const float* Q, double* sho, int* fl);
   void ho_qp_finish(hipStream_t s);
   std::unique_ptr<HOState> ho_;
+  std::unique_ptr<MFState> mf_;
 
   int N_, L_;
   int R_ = 0;

@@ -4,4 +4,5 @@
 #include "crf.hip"
 #include "ho.hip"
 #include "lp.hip"
+#include "adjoint.hip"
 #include "capi.hip"

@@ -483,7 +483,7 @@ void CRF::int_energy_dev(const int* lab, double* dst, double* scratch, hipStream
     a.lab = lab;
     SliceArgs o;
     o.out = m.tmp.get();
-    o.scale = -p.w;
+    o.scale = -p.nn_w;  // the no-norm copy (Potts)
     o.apply_scale = true;
     p.lat->filter(a, o, L_, false);
     k_pairwise_true_accum<<<grid_for(N_), kBlock, 0, s>>>(m.tmp.get(), lab, m.rpix.get(), N_, L_);
@@ -507,9 +507,7 @@ void CRF::lp_energy_dev(const float* Q, double* dst, double* scratch, hipStream_
     Potential& p = pairwise(k);
     SliceArgs o;
     o.out = m.tmp.get();
-    o.scale = -p.w;
-    o.apply_scale = true;
-    p.lat->ordered(Q, m.mm.get(), L_, o);
+    compat_slice(p, o, L_, s, [&](const SliceArgs& oo) { p.lat->ordered(Q, m.mm.get(), L_, oo); });
     launch_dot(s, Q, m.tmp.get(), NL, m.parts.get(), scratch);
     k_sub_scalar<<<1, 1, 0, s>>>(dst, scratch);
   }
@@ -678,10 +676,8 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
         Potential& p = pairwise(k);
         SliceArgs o;
         o.out = st.sa.get();
-        o.scale = -p.w;
-        o.apply_scale = true;
         o.accumulate = k > 0;
-        p.lat->ordered(st.Q.get(), st.mm.get(), L, o);
+        compat_slice(p, o, L, s, [&](const SliceArgs& oo) { p.lat->ordered(st.Q.get(), st.mm.get(), L, oo); });
       }
       if (num_pairwise() == 0) k_fill<<<grid_for(NL), kBlock, 0, s>>>(st.sa.get(), NL, 0.f);
       if (ho) ucondgrad_dev(st.Q.get(), su, s);

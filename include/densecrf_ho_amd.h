@@ -44,6 +44,14 @@ extern "C" {
 #define DCRF_NORMALIZE_AFTER 2
 #define DCRF_NORMALIZE_SYMMETRIC 3
 
+/* LabelCompatibility kinds, include/labelcompatibility.h:51-92.  Parameters:
+ * Potts w (1 value); Diagonal v (L values); Matrix m (L*L, column-major; stored
+ * as (m + m^T)/2 like the MatrixCompatibility constructor,
+ * src/labelcompatibility.cpp:94-97). */
+#define DCRF_COMPAT_POTTS 0
+#define DCRF_COMPAT_DIAGONAL 1
+#define DCRF_COMPAT_MATRIX 2
+
 typedef struct dcrf_lattice_s* dcrf_lattice_t;
 typedef struct dcrf_pairwise_s* dcrf_pairwise_t;
 typedef struct dcrf_crf_s* dcrf_crf_t;
@@ -103,6 +111,23 @@ int dcrf_crf_set_unary(dcrf_crf_t crf, const float* unary, int on_device);
  * src/densecrf.cpp:157-169 (also keeps the NO_NORMALIZATION copy used by
  * the energies; here both share one lattice) */
 int dcrf_crf_add_pairwise(dcrf_crf_t crf, const float* features, int d, float potts_w, int ntype, int on_device);
+/* DenseCRF::addPairwiseEnergy(features, function, DIAG_KERNEL, ntype) with
+ * function a PottsCompatibility, DiagonalCompatibility or MatrixCompatibility
+ * (src/densecrf.cpp:157-169, src/labelcompatibility.cpp:47-129); `params` is a
+ * host array in the layout described at DCRF_COMPAT_POTTS. */
+int dcrf_crf_add_pairwise_compat(dcrf_crf_t crf, const float* features, int d, int compat_type,
+                                 const float* params, int ntype, int on_device);
+/* PairwisePotential::parameters() / setParameters() of potential k
+ * (src/pairwise.cpp:382-387; layouts src/labelcompatibility.cpp:61-68, 88-96,
+ * 109-121: Potts [w], Diagonal v[L], Matrix the upper triangle, row by row,
+ * L*(L+1)/2 values).  `params` is a host array. */
+int dcrf_crf_pairwise_nparams(dcrf_crf_t crf, int k, int* n);
+int dcrf_crf_pairwise_parameters(dcrf_crf_t crf, int k, float* params);
+int dcrf_crf_pairwise_set_parameters(dcrf_crf_t crf, int k, const float* params);
+/* PairwisePotential::gradient(b, Q) of potential k, src/pairwise.cpp:390-395:
+ * the compatibility's gradient of b against the filtered Q, in the layout of
+ * parameters() (host array `grad`, FP64 accumulation rounded to float). */
+int dcrf_crf_pairwise_gradient(dcrf_crf_t crf, int k, const float* b, const float* Q, float* grad, int on_device);
 /* DenseCRF2D::addPairwiseGaussian(sx, sy, new PottsCompatibility(w)), src/densecrf.cpp:170-178 */
 int dcrf_crf_add_pairwise_gaussian(dcrf_crf_t crf, int W, int H, float sx, float sy, float potts_w);
 /* DenseCRF2D::addPairwiseBilateral(sx, sy, sr, sg, sb, im, new PottsCompatibility(w)),
@@ -182,6 +207,27 @@ int dcrf_crf_lp_energy(dcrf_crf_t crf, const float* Q, int on_device, double* en
 int dcrf_crf_ucondgrad(dcrf_crf_t crf, const float* Q, float* Us, int on_device);
 /* DenseCRF::compute_LR_QP_value(Q, 0), src/densecrf.cpp:940-957 */
 int dcrf_crf_qp_value(dcrf_crf_t crf, const float* Q, int on_device, double* energy);
+
+/* ------------------------------------------------------------------------ */
+/* Unrolled mean field and its adjoint.  The fork has no inference loop: the */
+/* update is the classic one built from its pieces, Q0 = expAndNormalize(-U) */
+/* (src/densecrf.cpp:321-336), then `iterations` times                       */
+/*   Q <- softmax(-U - sum_k PairwisePotential::apply_k(Q))                  */
+/* (src/pairwise.cpp:203-208).  The forward keeps Q_0..Q_T and every K_k Q_t */
+/* on the device; the backward runs the hand-written reverse pass through    */
+/* the softmax, the compatibilities and the transposed filters              */
+/* (applyTranspose, src/pairwise.cpp:209-213).                              */
+/* ------------------------------------------------------------------------ */
+int dcrf_crf_meanfield(dcrf_crf_t crf, int iterations, float* Q, int on_device);
+/* dQ = dL/dQ_T of the last dcrf_crf_meanfield -> dU = dL/dU (N x L) and the
+ * gradient of every potential's parameters (parameters() layout, potentials
+ * back to back, host FP64 array; may be null). */
+int dcrf_crf_meanfield_backward(dcrf_crf_t crf, const float* dQ, float* dU, double* dparams, int on_device);
+/* loss = -sum_i log Q(labels[i], i) over pixels with labels[i] in [0,L) (FP64)
+ * and dQ = dloss/dQ.  No reference counterpart: the upstream loss of the
+ * unrolled-inference gradient. */
+int dcrf_crf_cross_entropy(dcrf_crf_t crf, const float* Q, const int* labels, float* dQ, double* loss,
+                           int on_device);
 
 /* ------------------------------------------------------------------------ */
 /* Device memory and kernel timing.  No counterpart in the reference: they  */

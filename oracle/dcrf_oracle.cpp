@@ -479,7 +479,61 @@ struct Pairwise {
   int ntype = 3;               // NormalizationType: 0 none, 1 before, 2 after, 3 symmetric
   std::vector<float> norm_f;   // float norm (DenseKernel::initLattice, src/pairwise.cpp:40-57)
   std::vector<double> norm_d;  // same rounding then widened
+  // label compatibility (include/labelcompatibility.h:51-92): 0 Potts (w),
+  // 1 Diagonal (cp = v[L]), 2 Matrix (cp = W[L*L], already (m + m^T)/2 as the
+  // MatrixCompatibility constructor stores it, src/labelcompatibility.cpp:94-97)
+  int ctype = 0;
+  std::vector<float> cp;
+  // the NO_NORMALIZATION copy always carries PottsCompatibility(parameters()(0))
+  // taken when the term is added (src/densecrf.cpp:161-168)
+  float nn_w = 1.f;
 };
+
+// LabelCompatibility::apply on one pixel's L-vector x -> y
+// (src/labelcompatibility.cpp:49-52 Potts, :81-84 Diagonal, :100-102 Matrix;
+// W is symmetric, so applyTranspose (:103-105) is the same map)
+template <class T>
+static void compat_apply(const Pairwise& p, int L, const T* x, T* y) {
+  if (p.ctype == 0) {
+    for (int l = 0; l < L; ++l) y[l] = -(T)p.w * x[l];
+  } else if (p.ctype == 1) {
+    for (int l = 0; l < L; ++l) y[l] = (T)p.cp[l] * x[l];
+  } else {
+    for (int l = 0; l < L; ++l) {
+      T s = 0;
+      for (int j = 0; j < L; ++j) s += (T)p.cp[(size_t)j * L + l] * x[j];
+      y[l] = s;
+    }
+  }
+}
+
+// number of parameters of a compatibility (LabelCompatibility::parameters,
+// src/labelcompatibility.cpp:61-65, :88-90, :109-115)
+static int compat_nparams(const Pairwise& p, int L) {
+  return p.ctype == 0 ? 1 : p.ctype == 1 ? L : L * (L + 1) / 2;
+}
+
+// LabelCompatibility::gradient(b, Q) accumulated into g (FP64):
+// Potts -sum(b o Q) (:72-76), Diagonal row sums of b o Q (:91-93), Matrix the
+// packed upper triangle of b Q^T with off-diagonal entries doubled (:122-129)
+template <class T>
+static void compat_grad_accum(const Pairwise& p, int N, int L, const T* b, const T* Q, double* g) {
+  if (p.ctype == 0) {
+    double s = 0;
+    for (size_t i = 0; i < (size_t)N * L; ++i) s += (double)b[i] * (double)Q[i];
+    g[0] -= s;
+  } else if (p.ctype == 1) {
+    for (int i = 0; i < N; ++i)
+      for (int l = 0; l < L; ++l) g[l] += (double)b[(size_t)i * L + l] * (double)Q[(size_t)i * L + l];
+  } else {
+    std::vector<double> G((size_t)L * L, 0.0);  // G(a, c) = sum_i b(a,i) Q(c,i), at a*L + c
+    for (int i = 0; i < N; ++i)
+      for (int a = 0; a < L; ++a)
+        for (int c = 0; c < L; ++c) G[(size_t)a * L + c] += (double)b[(size_t)i * L + a] * (double)Q[(size_t)i * L + c];
+    for (int i = 0, k = 0; i < L; ++i)
+      for (int j = i; j < L; ++j, ++k) g[k] += G[(size_t)i * L + j] + (i != j ? G[(size_t)j * L + i] : 0.0);
+  }
+}
 
 template <class T>
 struct CRF {
@@ -495,10 +549,25 @@ struct CRF {
 
   CRF(int n, int l) : N(n), L(l), U((size_t)n * l, (T)0) {}
 
-  // DenseKernel::filter (src/pairwise.cpp:58-75) + PottsCompatibility::apply
-  // (src/labelcompatibility.cpp:49-52).  `no_norm` selects the NO_NORMALIZATION
-  // copy built by addPairwiseEnergy (src/densecrf.cpp:164-168).
+  // DenseKernel::filter (src/pairwise.cpp:58-75) + LabelCompatibility::apply
+  // (src/pairwise.cpp:203-213).  `no_norm` selects the NO_NORMALIZATION copy
+  // built by addPairwiseEnergy (src/densecrf.cpp:164-168).
   void apply(int k, const T* in, T* out, bool no_norm = false, bool transpose = false) const {
+    kernel(k, in, out, no_norm, transpose);
+    const Pairwise& p = *pw[k];
+    if (no_norm) {
+      for (size_t i = 0; i < (size_t)N * L; ++i) out[i] = -(T)p.nn_w * out[i];
+      return;
+    }
+    for (int i = 0; i < N; ++i) {
+      T y[64];
+      compat_apply(p, L, out + (size_t)i * L, y);
+      std::copy(y, y + L, out + (size_t)i * L);
+    }
+  }
+  // DenseKernel::apply / applyTranspose alone: the normalised filter without
+  // the compatibility (src/pairwise.cpp:76-87)
+  void kernel(int k, const T* in, T* out, bool no_norm = false, bool transpose = false) const {
     const Pairwise& p = *pw[k];
     const size_t NL = (size_t)N * L;
     std::vector<T> tmp(NL);
@@ -515,7 +584,7 @@ struct CRF {
       for (int l = 0; l < L; ++l) {
         T x = tmp[(size_t)i * L + l];
         if (post) x = x * nrm(p, i);
-        out[(size_t)i * L + l] = -(T)p.w * x;
+        out[(size_t)i * L + l] = x;
       }
   }
   static T nrm(const Pairwise& p, int i);
@@ -526,7 +595,94 @@ struct CRF {
     for (size_t i = 0; i < (size_t)N * L; ++i)
       if (q01[i] > 1 || q01[i] < 0) throw std::runtime_error("Value out of bounds");
     pw[k]->lat.ordered<T>(q01, out, L);
-    for (size_t i = 0; i < (size_t)N * L; ++i) out[i] = -(T)pw[k]->w * out[i];
+    for (int i = 0; i < N; ++i) {
+      T y[64];
+      compat_apply(*pw[k], L, out + (size_t)i * L, y);
+      std::copy(y, y + L, out + (size_t)i * L);
+    }
+  }
+
+  // PairwisePotential::gradient(b, Q) = compatibility.gradient(b, K Q)
+  // (src/pairwise.cpp:390-395), FP64 accumulation into g
+  void pairwise_gradient(int k, const T* b, const T* Q, double* g) const {
+    std::vector<T> fq((size_t)N * L);
+    kernel(k, Q, fq.data());
+    compat_grad_accum(*pw[k], N, L, b, fq.data(), g);
+  }
+
+  // ---- unrolled mean field and its adjoint ---------------------------------
+  // The fork has no inference loop; this is the classic mean-field update
+  // built from its pieces: Q0 = expAndNormalize(-U) (src/densecrf.cpp:321-336),
+  // then T times  a = -U - sum_k PairwisePotential::apply_k(Q),  Q = softmax(a).
+  static void softmax_row(const T* a, T* q, int L) {
+    T mx = a[0];
+    for (int l = 1; l < L; ++l) mx = std::max(mx, a[l]);
+    T s = 0;
+    for (int l = 0; l < L; ++l) {
+      q[l] = std::exp(a[l] - mx);
+      s += q[l];
+    }
+    for (int l = 0; l < L; ++l) q[l] = q[l] / s;
+  }
+  // Qs receives Q_0..Q_T, Ms the filtered fields K_k Q_t (t < T, k-major
+  // within t); both are what the adjoint reads
+  std::vector<T> meanfield(int T_, std::vector<std::vector<T>>* Qs, std::vector<std::vector<T>>* Ms) const {
+    const size_t NL = (size_t)N * L;
+    std::vector<T> a(NL), Q(NL), m(NL), c(NL);
+    for (size_t i = 0; i < NL; ++i) a[i] = -U[i];
+    for (int i = 0; i < N; ++i) softmax_row(&a[(size_t)i * L], &Q[(size_t)i * L], L);
+    if (Qs) Qs->push_back(Q);
+    for (int t = 0; t < T_; ++t) {
+      for (size_t i = 0; i < NL; ++i) a[i] = -U[i];
+      for (size_t k = 0; k < pw.size(); ++k) {
+        kernel((int)k, Q.data(), m.data());
+        if (Ms) Ms->push_back(m);
+        for (int i = 0; i < N; ++i) compat_apply(*pw[k], L, &m[(size_t)i * L], &c[(size_t)i * L]);
+        for (size_t i = 0; i < NL; ++i) a[i] -= c[i];
+      }
+      for (int i = 0; i < N; ++i) softmax_row(&a[(size_t)i * L], &Q[(size_t)i * L], L);
+      if (Qs) Qs->push_back(Q);
+    }
+    return Q;
+  }
+  // Reverse pass of meanfield(): given dL/dQ_T, dU = dL/dU and, per
+  // potential k, dL/d(parameters of k) in the layout of
+  // LabelCompatibility::parameters (g holds them back to back).
+  //   softmax:  ga = Q o (gQ - <Q, gQ>)   per pixel
+  //   a = -U - sum_k C_k(K_k Q):  dU -= ga;  b_k = -ga;
+  //   dparams_k += C_k.gradient(b_k, K_k Q)   (the reference's gradient);
+  //   gQ_prev = sum_k applyTranspose_k(b_k)    (src/pairwise.cpp:209-213)
+  void meanfield_backward(int T_, const std::vector<std::vector<T>>& Qs, const std::vector<std::vector<T>>& Ms,
+                          const T* dQ, T* dU, double* g) const {
+    const size_t NL = (size_t)N * L;
+    const int K = (int)pw.size();
+    std::vector<T> gQ(dQ, dQ + NL), ga(NL), b(NL), tmp(NL), gprev(NL);
+    std::vector<int> goff(K + 1, 0);
+    for (int k = 0; k < K; ++k) goff[k + 1] = goff[k] + compat_nparams(*pw[k], L);
+    std::fill(dU, dU + NL, (T)0);
+    auto sm_back = [&](const std::vector<T>& Q) {
+      for (int i = 0; i < N; ++i) {
+        T s = 0;
+        for (int l = 0; l < L; ++l) s += Q[(size_t)i * L + l] * gQ[(size_t)i * L + l];
+        for (int l = 0; l < L; ++l) ga[(size_t)i * L + l] = Q[(size_t)i * L + l] * (gQ[(size_t)i * L + l] - s);
+      }
+    };
+    for (int t = T_; t >= 1; --t) {
+      sm_back(Qs[t]);
+      for (size_t i = 0; i < NL; ++i) {
+        dU[i] -= ga[i];
+        b[i] = -ga[i];
+      }
+      std::fill(gprev.begin(), gprev.end(), (T)0);
+      for (int k = 0; k < K; ++k) {
+        compat_grad_accum(*pw[k], N, L, b.data(), Ms[(size_t)(t - 1) * K + k].data(), g + goff[k]);
+        apply(k, b.data(), tmp.data(), false, true);
+        for (size_t i = 0; i < NL; ++i) gprev[i] += tmp[i];
+      }
+      gQ = gprev;
+    }
+    sm_back(Qs[0]);
+    for (size_t i = 0; i < NL; ++i) dU[i] -= ga[i];
   }
 
   // ---- superpixel helpers, bug-compatible (src/densecrf.cpp:58-102) --------
@@ -1106,12 +1262,66 @@ void orc_crf_add_pairwise(void* h, const float* feat, int d, float w, int ntype)
   Pairwise* p = new Pairwise();
   p->d = d;
   p->w = w;
+  p->nn_w = w;
   p->ntype = ntype;
   p->feat.assign(feat, feat + (size_t)c->N * d);
   pairwise_init(*p);
   c->pw.push_back(p);
   c->f->pw.push_back(p);
   c->d->pw.push_back(p);
+}
+// as orc_crf_add_pairwise with a Diagonal (ctype 1, params v[L]) or Matrix
+// (ctype 2, params m[L*L] column-major, stored as (m + m^T)/2) compatibility
+void orc_crf_add_pairwise_compat(void* h, const float* feat, int d, int ctype, const float* params, int ntype) {
+  OrcCRF* c = (OrcCRF*)h;
+  const int L = c->L;
+  orc_crf_add_pairwise(h, feat, d, ctype == 0 ? params[0] : 1.f, ntype);
+  Pairwise* p = c->pw.back();
+  p->ctype = ctype;
+  if (ctype == 1) p->cp.assign(params, params + L);
+  if (ctype == 2) {
+    p->cp.assign((size_t)L * L, 0.f);
+    for (int j = 0; j < L; ++j)
+      for (int i = 0; i < L; ++i)
+        p->cp[(size_t)j * L + i] = 0.5f * (params[(size_t)j * L + i] + params[(size_t)i * L + j]);
+  }
+  if (ctype != 0) p->nn_w = p->cp[0];
+}
+int orc_crf_nparams(void* h, int k) {
+  OrcCRF* c = (OrcCRF*)h;
+  return compat_nparams(*c->pw[k], c->L);
+}
+void orc_crf_pairwise_gradient_f(void* h, int k, const float* b, const float* Q, double* g) {
+  OrcCRF* c = (OrcCRF*)h;
+  std::fill(g, g + compat_nparams(*c->pw[k], c->L), 0.0);
+  c->f->pairwise_gradient(k, b, Q, g);
+}
+void orc_crf_pairwise_gradient_d(void* h, int k, const double* b, const double* Q, double* g) {
+  OrcCRF* c = (OrcCRF*)h;
+  std::fill(g, g + compat_nparams(*c->pw[k], c->L), 0.0);
+  c->d->pairwise_gradient(k, b, Q, g);
+}
+void orc_crf_meanfield_f(void* h, int T, float* Q) {
+  OrcCRF* c = (OrcCRF*)h;
+  std::vector<float> q = c->f->meanfield(T, nullptr, nullptr);
+  std::copy(q.begin(), q.end(), Q);
+}
+// FP64 forward (Q_T into Q, may be null) and adjoint of dL/dQ_T = dQ:
+// dU [N*L] and the parameter gradients of every potential back to back in g
+void orc_crf_meanfield_grad_d(void* h, int T, const double* dQ, double* Q, double* dU, double* g) {
+  OrcCRF* c = (OrcCRF*)h;
+  std::vector<std::vector<double>> Qs, Ms;
+  std::vector<double> q = c->d->meanfield(T, &Qs, &Ms);
+  if (Q) std::copy(q.begin(), q.end(), Q);
+  int np = 0;
+  for (auto* p : c->pw) np += compat_nparams(*p, c->L);
+  std::fill(g, g + np, 0.0);
+  if (dQ) c->d->meanfield_backward(T, Qs, Ms, dQ, dU, g);
+}
+void orc_crf_meanfield_d(void* h, int T, double* Q) {
+  OrcCRF* c = (OrcCRF*)h;
+  std::vector<double> q = c->d->meanfield(T, nullptr, nullptr);
+  std::copy(q.begin(), q.end(), Q);
 }
 int orc_crf_lattice_M(void* h, int k) { return ((OrcCRF*)h)->pw[k]->lat.M; }
 void orc_crf_get_norm(void* h, int k, float* nf, double* nd) {

@@ -30,6 +30,7 @@ _f = C.c_float
 _d = C.c_double
 
 NTYPE = {"none": 0, "before": 1, "after": 2, "symmetric": 3}
+COMPAT = {"potts": 0, "diagonal": 1, "matrix": 2}
 
 
 def _sig(lib, name, res, *args):
@@ -63,6 +64,13 @@ def lib():
         _sig(L, "orc_crf_free", None, _vp)
         _sig(L, "orc_crf_set_unary", None, _vp, _f32p)
         _sig(L, "orc_crf_add_pairwise", None, _vp, _f32p, _i, _f, _i)
+        _sig(L, "orc_crf_add_pairwise_compat", None, _vp, _f32p, _i, _i, _f32p, _i)
+        _sig(L, "orc_crf_nparams", _i, _vp, _i)
+        _sig(L, "orc_crf_pairwise_gradient_f", None, _vp, _i, _f32p, _f32p, _f64p)
+        _sig(L, "orc_crf_pairwise_gradient_d", None, _vp, _i, _f64p, _f64p, _f64p)
+        _sig(L, "orc_crf_meanfield_f", None, _vp, _i, _f32p)
+        _sig(L, "orc_crf_meanfield_d", None, _vp, _i, _f64p)
+        _sig(L, "orc_crf_meanfield_grad_d", None, _vp, _i, _f64p, _f64p, _f64p, _f64p)
         _sig(L, "orc_crf_lattice_M", _i, _vp, _i)
         _sig(L, "orc_crf_get_norm", None, _vp, _i, _f32p, _f64p)
         _sig(L, "orc_crf_add_layer", None, _vp, _i32p, _i, _f32p, _f32p)
@@ -189,6 +197,47 @@ class CRF:
         e = np.broadcast_to(np.asarray(e, np.float32), (R,)).copy()
         c = np.broadcast_to(np.asarray(c, np.float32), (R,)).copy()
         self._L.orc_crf_add_layer(self.h, region, R, e, c)
+
+    def add_pairwise_compat(self, feat, kind, params, ntype="symmetric"):
+        """Pairwise term with a Potts ("potts", [w]), Diagonal ("diagonal", v[L])
+        or Matrix ("matrix", m (L, L), stored as (m + m^T)/2) compatibility."""
+        feat = _c(feat, np.float32)
+        ct = COMPAT[kind]
+        # the matrix is passed column-major (Eigen); m[i, j] is row i, column j
+        par = _c(np.asarray(params, np.float32).T.reshape(-1) if ct == 2 else np.ravel(params), np.float32)
+        self._L.orc_crf_add_pairwise_compat(self.h, feat, feat.shape[1], ct, par, NTYPE[ntype])
+        self.nk += 1
+
+    def nparams(self, k):
+        return self._L.orc_crf_nparams(self.h, k)
+
+    def pairwise_gradient(self, k, b, Q, fp64=False):
+        """PairwisePotential::gradient(b, Q) in the layout of parameters()."""
+        g = np.zeros(self.nparams(k))
+        if fp64:
+            self._L.orc_crf_pairwise_gradient_d(self.h, k, _c(b, np.float64), _c(Q, np.float64), g)
+        else:
+            self._L.orc_crf_pairwise_gradient_f(self.h, k, _c(b, np.float32), _c(Q, np.float32), g)
+        return g
+
+    def meanfield(self, T, fp64=False):
+        """Q after T unrolled mean-field updates from Q0 = softmax(-U)."""
+        out = np.zeros((self.N, self.L), np.float64 if fp64 else np.float32)
+        (self._L.orc_crf_meanfield_d if fp64 else self._L.orc_crf_meanfield_f)(self.h, int(T), out)
+        return out
+
+    def meanfield_grad(self, T, dQ):
+        """FP64 forward and adjoint: (Q_T, dL/dU, [dL/dparams of each term])."""
+        Q = np.zeros((self.N, self.L))
+        dU = np.zeros((self.N, self.L))
+        nps = [self.nparams(k) for k in range(self.nk)]
+        g = np.zeros(max(sum(nps), 1))
+        self._L.orc_crf_meanfield_grad_d(self.h, int(T), _c(dQ, np.float64), Q, dU, g)
+        out, o = [], 0
+        for n in nps:
+            out.append(g[o:o + n].copy())
+            o += n
+        return Q, dU, out
 
     def lattice_M(self, k):
         return self._L.orc_crf_lattice_M(self.h, k)
@@ -331,6 +380,8 @@ def reflib():
         _sig(R, "ref_crf_free", None, _vp)
         _sig(R, "ref_crf_set_unary", None, _vp, _f32p)
         _sig(R, "ref_crf_add_pairwise", None, _vp, _f32p, _i, _f, _i)
+        _sig(R, "ref_crf_add_pairwise_compat", None, _vp, _f32p, _i, _i, _f32p, _i)
+        _sig(R, "ref_crf_pairwise_gradient", _i, _vp, _i, _f32p, _f32p, C.c_void_p)
         _sig(R, "ref_crf_add_gaussian", None, _vp, _f, _f, _f)
         _sig(R, "ref_crf_add_bilateral", None, _vp, _f, _f, _f, _f, _f, _u8p, _f)
         _sig(R, "ref_crf_get_features", None, _vp, _i, _f32p)
@@ -409,6 +460,19 @@ class RefCRF:
     def add_pairwise(self, feat, w, ntype="symmetric"):
         feat = _c(feat, np.float32)
         self._R.ref_crf_add_pairwise(self.h, feat, feat.shape[1], float(w), NTYPE[ntype])
+
+    def add_pairwise_compat(self, feat, kind, params, ntype="symmetric"):
+        feat = _c(feat, np.float32)
+        ct = COMPAT[kind]
+        par = _c(np.asarray(params, np.float32).T.reshape(-1) if ct == 2 else np.ravel(params), np.float32)
+        self._R.ref_crf_add_pairwise_compat(self.h, feat, feat.shape[1], ct, par, NTYPE[ntype])
+
+    def pairwise_gradient(self, k, b, Q):
+        b, Q = _c(b, np.float32), _c(Q, np.float32)
+        n = self._R.ref_crf_pairwise_gradient(self.h, k, b, Q, None)
+        g = np.zeros(n, np.float32)
+        self._R.ref_crf_pairwise_gradient(self.h, k, b, Q, g.ctypes.data_as(C.c_void_p))
+        return g
 
     def add_gaussian(self, sx, sy, w):
         self._R.ref_crf_add_gaussian(self.h, sx, sy, w)

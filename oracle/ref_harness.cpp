@@ -118,6 +118,29 @@ void ref_crf_add_pairwise(void* h, const float* feat, int d, float w, int ntype)
   RefCRF* c = (RefCRF*)h;
   c->addPairwiseEnergy(as_mat(feat, d, c->N()), new PottsCompatibility(w), DIAG_KERNEL, (NormalizationType)ntype);
 }
+// ctype 0 Potts (params[0] = w), 1 Diagonal (v[L]), 2 Matrix (m[L*L] column-major)
+void ref_crf_add_pairwise_compat(void* h, const float* feat, int d, int ctype, const float* params, int ntype) {
+  RefCRF* c = (RefCRF*)h;
+  const int L = c->L();
+  LabelCompatibility* lc;
+  if (ctype == 0) {
+    lc = new PottsCompatibility(params[0]);
+  } else if (ctype == 1) {
+    VectorXf v(L);
+    for (int i = 0; i < L; ++i) v[i] = params[i];
+    lc = new DiagonalCompatibility(v);
+  } else {
+    lc = new MatrixCompatibility(as_mat(params, L, L));
+  }
+  c->addPairwiseEnergy(as_mat(feat, d, c->N()), lc, DIAG_KERNEL, (NormalizationType)ntype);
+}
+// PairwisePotential::gradient(b, Q); returns the number of parameters
+int ref_crf_pairwise_gradient(void* h, int k, const float* b, const float* Q, float* g) {
+  RefCRF* c = (RefCRF*)h;
+  VectorXf r = c->pw(k)->gradient(as_mat(b, c->L(), c->N()), as_mat(Q, c->L(), c->N()));
+  if (g) std::copy(r.data(), r.data() + r.size(), g);
+  return (int)r.size();
+}
 void ref_crf_add_gaussian(void* h, float sx, float sy, float w) {
   ((RefCRF*)h)->addPairwiseGaussian(sx, sy, new PottsCompatibility(w));
 }

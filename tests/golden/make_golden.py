@@ -165,6 +165,41 @@ def c0_case():
     print("c0:", res)
 
 
+def compat_case(name, W, H, K, seed, L):
+    """Potts / Diagonal / Matrix compatibilities on the bilateral kernel:
+    apply, applyTranspose, the no-norm copy and PairwisePotential::gradient;
+    plus a reference-rule QP with a Matrix term."""
+    wl = S.Workload(W, H, K, seed, L=L)
+    rng = np.random.default_rng(seed + 100)
+    v = rng.uniform(-3, 3, L).astype(np.float32)
+    m = rng.normal(0, 2, (L, L)).astype(np.float32)
+    x = rng.random((wl.N, L)).astype(np.float32)
+    b = rng.standard_normal((wl.N, L)).astype(np.float32)
+    feat = wl.kernels[1][0]
+    d = dict(U=wl.U, feat=feat, feat_g=wl.kernels[0][0], v=v, m=m, x=x, b=b)
+    for kind, par in (("potts", [3.0]), ("diagonal", v), ("matrix", m)):
+        r = O.RefCRF(W, H, L)
+        r.set_unary(wl.U)
+        r.add_pairwise_compat(feat, kind, par)
+        d[f"{kind}_apply"] = r.apply(0, x, 0)
+        d[f"{kind}_applyT"] = r.apply(0, x, 1)
+        d[f"{kind}_nn"] = r.apply(0, x, 2)
+        d[f"{kind}_grad"] = r.pairwise_gradient(0, b, x)
+    r = O.RefCRF(W, H, L)
+    r.set_unary(wl.U)
+    r.add_pairwise_compat(wl.kernels[0][0], "potts", [3.0])
+    r.add_pairwise_compat(feat, "matrix", -np.abs(m) - 1.0)
+    q0 = r.unary_init()
+    d["qp_m"] = -np.abs(m) - 1.0
+    d["qp_Q"] = r.qp(q0)
+    lab = np.zeros(wl.N, np.int32)
+    r._R.ref_crf_current_map(r.h, O._c(d["qp_Q"], np.float32), lab)
+    d["qp_lab"] = lab
+    d["qp_energy_true"] = np.float64(r._R.ref_crf_energy_true(r.h, lab))
+    np.savez_compressed(os.path.join(OUT, f"compat_{name}.npz"), **d)
+    print(f"compat_{name}: N={wl.N} L={L}")
+
+
 def main():
     W, H = 32, 24
     wl = S.Workload(W, H, 10, 0)
@@ -180,6 +215,7 @@ def main():
     crf_case("32x24_L21_ho", 32, 24, 10, 2, 21, layers=(5, 9, 17), lp_outer=2)
     qp_case("convex_31x23_L5", 31, 23, 8, 1, 5, weights=(-3.0, -5.0))
     qp_case("ties_31x23_L5", 31, 23, 8, 1, 5, dup_label=(1, 3))
+    compat_case("31x23_L5", 31, 23, 8, 1, 5)
     c0_case()
 
 
