@@ -75,6 +75,19 @@ SIGNATURES = {
     "dcrf_crf_lp_energy": (_i, [_vp, _vp, _i, C.POINTER(_d)]),
     "dcrf_crf_qp_value": (_i, [_vp, _vp, _i, C.POINTER(_d)]),
     "dcrf_crf_ucondgrad": (_i, [_vp, _vp, _vp, _i]),
+    "dcrf_batch_create": (_i, [_i, _i, _vp, C.POINTER(_vp)]),
+    "dcrf_batch_create_2d": (_i, [_i, _i, _vp, _vp, C.POINTER(_vp)]),
+    "dcrf_batch_destroy": (_i, [_vp]),
+    "dcrf_batch_pixel_offsets": (_i, [_vp, _vp]),
+    "dcrf_batch_set_graphs": (_i, [_vp, _i]),
+    "dcrf_batch_set_unary": (_i, [_vp, _i, _vp, _i]),
+    "dcrf_batch_add_pairwise": (_i, [_vp, _vp, _i, _f, _i, _i]),
+    "dcrf_batch_add_pairwise_gaussian": (_i, [_vp, _f, _f, _f]),
+    "dcrf_batch_add_pairwise_bilateral": (_i, [_vp, _f, _f, _f, _f, _f, _vp, _f]),
+    "dcrf_batch_unary_init": (_i, [_vp, _vp, _i]),
+    "dcrf_batch_qp": (_i, [_vp, _vp, _vp, _i, _i, _i, _vp]),
+    "dcrf_batch_lp": (_i, [_vp, _vp, _vp, C.POINTER(LPParams), _i, _vp, _i]),
+    "dcrf_batch_assignment_energy_true": (_i, [_vp, _vp, _i, _vp]),
     "dcrf_malloc": (_i, [C.POINTER(_vp), C.c_size_t]),
     "dcrf_free": (_i, [_vp]),
     "dcrf_memcpy": (_i, [_vp, _vp, C.c_size_t, _i]),

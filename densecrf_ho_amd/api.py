@@ -505,3 +505,109 @@ class DenseCRF2D(DenseCRF):
                                                            float(sg), float(sb),
                                                            im.ctypes.data_as(C.c_void_p), w))
         self.npairwise += 1
+
+
+class DenseCRFBatch:
+    """Many images of uneven sizes solved together (no counterpart in the
+    reference, which runs one DenseCRF per image): each image follows the
+    per-image QP / LP algorithm.  Batch fields are the images' (N_b, L) fields
+    concatenated in image order; ``offsets[b]`` is image b's first pixel."""
+
+    def __init__(self, L: int, sizes):
+        """sizes: list of (W, H) per image."""
+        Lb = _lib.load()
+        self.L = int(L)
+        W = np.array([int(w) for w, _ in sizes], np.int32)
+        H = np.array([int(h) for _, h in sizes], np.int32)
+        self.B = len(W)
+        h = C.c_void_p()
+        _check(Lb.dcrf_batch_create_2d(self.L, self.B, W.ctypes.data_as(C.c_void_p), H.ctypes.data_as(C.c_void_p),
+                                       C.byref(h)))
+        self._h = h
+        off = np.zeros(self.B + 1, np.int64)
+        _check(Lb.dcrf_batch_pixel_offsets(self._h, off.ctypes.data_as(C.c_void_p)))
+        self.offsets = off
+        self.N = int(off[-1])
+        self.sizes = list(zip(W.tolist(), H.tolist()))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            _lib.load().dcrf_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_graphs(self, on: bool):
+        _check(_lib.load().dcrf_batch_set_graphs(self._h, int(on)))
+
+    def setUnaryEnergy(self, b: int, unary):
+        a = _Arg(unary, np.float32)
+        _check(_lib.load().dcrf_batch_set_unary(self._h, int(b), a.ptr, a.dev))
+
+    def addPairwiseGaussian(self, sx, sy, function: PottsCompatibility | None = None):
+        w = (function or PottsCompatibility(1.0)).w
+        _check(_lib.load().dcrf_batch_add_pairwise_gaussian(self._h, float(sx), float(sy), w))
+
+    def addPairwiseBilateral(self, sx, sy, sr, sg, sb, images, function: PottsCompatibility | None = None):
+        w = (function or PottsCompatibility(1.0)).w
+        ims = [np.ascontiguousarray(im, np.uint8).reshape(-1) for im in images]
+        ptrs = (C.c_void_p * self.B)(*[im.ctypes.data_as(C.c_void_p).value for im in ims])
+        _check(_lib.load().dcrf_batch_add_pairwise_bilateral(self._h, float(sx), float(sy), float(sr), float(sg),
+                                                             float(sb), C.cast(ptrs, C.c_void_p), w))
+
+    def addPairwiseEnergy(self, features, function: PottsCompatibility | None = None,
+                          normalization_type=NORMALIZE_SYMMETRIC):
+        """features: one (N_b, d) array per image (host) or one DeviceArray each."""
+        w = (function or PottsCompatibility(1.0)).w
+        args = [_Arg(f, np.float32) for f in features]
+        d = args[0].arr.shape[1]
+        ptrs = (C.c_void_p * self.B)(*[a.ptr.value if isinstance(a.ptr, C.c_void_p) else a.ptr for a in args])
+        _check(_lib.load().dcrf_batch_add_pairwise(self._h, C.cast(ptrs, C.c_void_p), d, w, int(normalization_type),
+                                                   args[0].dev))
+
+    def image(self, field, b):
+        """image b's rows of a batch field"""
+        return field[self.offsets[b]:self.offsets[b + 1]]
+
+    def unary_init(self, out=None):
+        out = np.empty((self.N, self.L), np.float32) if out is None else out
+        o = _Arg(out, np.float32, out=True)
+        _check(_lib.load().dcrf_batch_unary_init(self._h, o.ptr, o.dev))
+        return out
+
+    def qp(self, init=None, max_iter: int = 0, ref_stop: bool | None = None, out=None):
+        """QP of every image; returns (Q, iterations per image)."""
+        if ref_stop is None:
+            ref_stop = max_iter <= 0
+        a = _Arg(init, np.float32) if init is not None else None
+        proto = init if init is not None else out
+        if out is None:
+            out = _empty_like_field(proto, (self.N, self.L)) if proto is not None else \
+                np.empty((self.N, self.L), np.float32)
+        o = _Arg(out, np.float32, out=True)
+        it = np.zeros(self.B, np.int32)
+        _check(_lib.load().dcrf_batch_qp(self._h, a.ptr if a else None, o.ptr, int(max_iter), int(ref_stop), o.dev,
+                                         it.ctypes.data_as(C.c_void_p)))
+        return out, it
+
+    def lp(self, init, params: LP_inf_params | None = None, out=None):
+        """LP of every image; returns (Q, qp-gamma sweeps (B, inner steps))."""
+        params = params or LP_inf_params()
+        a = _Arg(init, np.float32)
+        out = _empty_like_field(init, (self.N, self.L)) if out is None else out
+        o = _Arg(out, np.float32, out=True)
+        cap = int(params.prox_max_iter * params.fw_max_iter)
+        sw = np.zeros((self.B, cap), np.int32)
+        _check(_lib.load().dcrf_batch_lp(self._h, a.ptr, o.ptr, C.byref(params._c()), a.dev,
+                                         sw.ctypes.data_as(C.c_void_p), cap))
+        return out, sw
+
+    def assignment_energy_true(self, labels):
+        a = _Arg(labels, np.int32)
+        e = np.zeros(self.B)
+        _check(_lib.load().dcrf_batch_assignment_energy_true(self._h, a.ptr, a.dev, e.ctypes.data_as(C.c_void_p)))
+        return e

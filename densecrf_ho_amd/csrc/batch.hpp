@@ -1,0 +1,94 @@
+// Batched dense CRF (many images of uneven sizes, one lattice union per
+// pairwise term, per-image solver control on the device); see batch.hip.
+#pragma once
+
+#include <functional>
+#include <map>
+#include <memory>
+#include <set>
+#include <vector>
+
+#include "crf.hpp"
+#include "crf_kernels.hpp"
+
+namespace dcrf {
+
+// blocks of a batched launch: image b owns blocks [first[b], first[b+1])
+struct BlockPlan {
+  int nblocks = 0;
+  std::vector<int> first_h;
+  DBuf<int> img, first;
+  void build(const std::vector<int>& per_image, hipStream_t s);
+  ImageBlocks map(const long* pix) const { return ImageBlocks{img.get(), first.get(), pix}; }
+};
+
+// Fixed launch sequences captured once into HIP graphs and replayed.  A
+// sequence runs eagerly the first time (so every buffer it touches is
+// allocated and every table built), is captured the second time, and from
+// then on replays.  Sequences that read a double-buffered array are keyed by
+// which buffer is current.
+struct GraphCache {
+  std::map<long, hipGraphExec_t> exec;
+  std::set<long> seen;
+  ~GraphCache();
+  void clear();
+  void run(long key, hipStream_t s, bool use, const std::function<void()>& body);
+};
+
+class Batch {
+ public:
+  Batch(int L, const std::vector<int>& npix);
+  ~Batch();
+  int L() const { return L_; }
+  int B() const { return B_; }
+  long N() const { return N_; }
+  const std::vector<long>& pixel_offsets() const { return pix_h_; }
+  hipStream_t stream() const { return s_; }
+  void set_graphs(bool on) {
+    use_graph_ = on;
+    graphs_.clear();
+  }
+
+  void set_unary(int b, const float* dU);                                          // device, N_b x L
+  void add_pairwise(const std::vector<const float*>& feats, int d, float w, int ntype);  // device, per image
+  int num_pairwise() const { return (int)pw_.size(); }
+  Potential& pairwise(int k) { return *pw_.at(k); }
+  void unary_init(float* Q);
+
+  // solvers over the union (device pointers); init may be null (softmax(-U))
+  std::vector<int> qp(const float* init, float* out, int max_iter, bool ref_stop);
+  // returns per image the qp-gamma sweep count of every inner step
+  std::vector<std::vector<int>> lp(const float* init, float* out, const LPParams& P);
+  void assignment_energy_true(const int* lab, double* host);  // per image
+
+ private:
+  static constexpr int kSweepBatchB = 16;
+  static constexpr long kGraphQP = 1, kGraphY = 2, kGraphPost = 10, kGraphSweep = 100;
+  void ensure_state();
+  ImageBlocks red_map() const;
+  ImageBlocks sweep_map() const;
+  void int_energy_dev(const int* lab, double* dst, int stride);
+  void lp_energy_dev(const float* Q, double* dst, double* scratch, int stride);
+
+  int L_, B_;
+  long N_ = 0;
+  hipStream_t s_ = nullptr;
+  std::vector<long> pix_h_;
+  DBuf<long> pix_;
+  BlockPlan red_, sweep_;
+  DBuf<float> U_;
+  std::vector<std::unique_ptr<Potential>> pw_;
+  bool use_graph_ = true;
+  GraphCache graphs_;
+  // QP state
+  DBuf<float> Q_, gy_, psx_, tmp_, rpix_;
+  DBuf<int> lab_;
+  DBuf<double> parts_, sc_;
+  DBuf<int> fl_;
+  // LP state
+  DBuf<float> alpha_, sa_, gamma_, gamma2_, Y_, lQ_, curQ_, best_, mmparts_, mm_;
+  DBuf<double> lsc_, ost_;
+  DBuf<int> lfl_, ofl_;
+};
+
+}  // namespace dcrf

@@ -4,6 +4,7 @@
 #include <string>
 
 #include "../../include/densecrf_ho_amd.h"
+#include "batch.hpp"
 #include "crf.hpp"
 #include "crf_kernels.hpp"
 
@@ -21,6 +22,10 @@ struct dcrf_lattice_s {
 };
 struct dcrf_crf_s {
   std::unique_ptr<dcrf::CRF> crf;
+};
+struct dcrf_batch_s {
+  std::unique_ptr<dcrf::Batch> b;
+  std::vector<int> W, H;  // image sizes when given (2-D kernels)
 };
 struct dcrf_pairwise_s {
   hipStream_t s = nullptr;
@@ -611,6 +616,170 @@ int dcrf_profile_enable(int on) {
     Profiler::get().on.store(on != 0);
   });
 }
+// ---- batches ------------------------------------------------------------------
+int dcrf_batch_create(int L, int B, const int* npix, dcrf_batch_t* out) {
+  return guarded([&] {
+    DCRF_REQUIRE(npix && out && B > 0, "dcrf_batch_create: bad arguments");
+    auto h = std::make_unique<dcrf_batch_s>();
+    h->b = std::make_unique<Batch>(L, std::vector<int>(npix, npix + B));
+    *out = h.release();
+  });
+}
+int dcrf_batch_create_2d(int L, int B, const int* W, const int* H, dcrf_batch_t* out) {
+  return guarded([&] {
+    DCRF_REQUIRE(W && H && out && B > 0, "dcrf_batch_create_2d: bad arguments");
+    std::vector<int> n(B);
+    for (int b = 0; b < B; ++b) {
+      DCRF_REQUIRE(W[b] > 0 && H[b] > 0 && (long)W[b] * H[b] < (1L << 31), "dcrf_batch_create_2d: bad image size");
+      n[b] = W[b] * H[b];
+    }
+    auto h = std::make_unique<dcrf_batch_s>();
+    h->b = std::make_unique<Batch>(L, n);
+    h->W.assign(W, W + B);
+    h->H.assign(H, H + B);
+    *out = h.release();
+  });
+}
+int dcrf_batch_destroy(dcrf_batch_t b) {
+  return guarded([&] { delete b; });
+}
+int dcrf_batch_pixel_offsets(dcrf_batch_t b, long long* offsets) {
+  return guarded([&] {
+    DCRF_REQUIRE(b && offsets, "dcrf_batch_pixel_offsets: bad arguments");
+    const auto& o = b->b->pixel_offsets();
+    for (size_t i = 0; i < o.size(); ++i) offsets[i] = o[i];
+  });
+}
+int dcrf_batch_set_graphs(dcrf_batch_t b, int on) {
+  return guarded([&] {
+    DCRF_REQUIRE(b, "dcrf_batch_set_graphs: null handle");
+    b->b->set_graphs(on != 0);
+  });
+}
+int dcrf_batch_set_unary(dcrf_batch_t b, int img, const float* unary, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(b && unary && img >= 0 && img < b->b->B(), "dcrf_batch_set_unary: bad arguments");
+    Batch& B = *b->b;
+    const auto& o = B.pixel_offsets();
+    Staged<float> u(unary, (size_t)(o[img + 1] - o[img]) * B.L(), on_device != 0, B.stream(), true);
+    B.set_unary(img, u.d);
+    DCRF_HIP(hipStreamSynchronize(B.stream()));
+  });
+}
+int dcrf_batch_add_pairwise(dcrf_batch_t b, const float* const* features, int d, float potts_w, int ntype,
+                            int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(b && features && d > 0, "dcrf_batch_add_pairwise: bad arguments");
+    Batch& B = *b->b;
+    const auto& o = B.pixel_offsets();
+    std::vector<std::unique_ptr<Staged<float>>> st;
+    std::vector<const float*> f;
+    for (int i = 0; i < B.B(); ++i) {
+      DCRF_REQUIRE(features[i], "dcrf_batch_add_pairwise: null feature array");
+      st.push_back(std::make_unique<Staged<float>>(features[i], (size_t)(o[i + 1] - o[i]) * d, on_device != 0,
+                                                   B.stream(), true));
+      f.push_back(st.back()->d);
+    }
+    B.add_pairwise(f, d, potts_w, ntype);
+  });
+}
+static void batch_add_2d(dcrf_batch_t b, float sx, float sy, const unsigned char* const* images, float sr, float sg,
+                         float sbl, float w) {
+  DCRF_REQUIRE(b && !b->W.empty(), "batch: 2-D kernels need a batch made by dcrf_batch_create_2d");
+  Batch& B = *b->b;
+  const int d = images ? 5 : 2;
+  std::vector<DBuf<float>> feats;
+  std::vector<const float*> f;
+  for (int i = 0; i < B.B(); ++i) {
+    const int W = b->W[i], H = b->H[i];
+    DBuf<unsigned char> im;
+    if (images) {
+      DCRF_REQUIRE(images[i], "batch: null image");
+      im.resize((size_t)W * H * 3);
+      DCRF_HIP(hipMemcpyAsync(im.get(), images[i], (size_t)W * H * 3, hipMemcpyHostToDevice, B.stream()));
+    }
+    feats.emplace_back((size_t)W * H * d);
+    k_features_2d<<<grid_for((size_t)W * H), kBlock, 0, B.stream()>>>(feats.back().get(), W, H, sx, sy,
+                                                                       images ? im.get() : nullptr, sr, sg, sbl);
+    check_launch("k_features_2d");
+    DCRF_HIP(hipStreamSynchronize(B.stream()));
+    f.push_back(feats.back().get());
+  }
+  B.add_pairwise(f, d, w, kNormSymmetric);
+}
+int dcrf_batch_add_pairwise_gaussian(dcrf_batch_t b, float sx, float sy, float potts_w) {
+  return guarded([&] { batch_add_2d(b, sx, sy, nullptr, 1.f, 1.f, 1.f, potts_w); });
+}
+int dcrf_batch_add_pairwise_bilateral(dcrf_batch_t b, float sx, float sy, float sr, float sg, float sb,
+                                      const unsigned char* const* images, float potts_w) {
+  return guarded([&] {
+    DCRF_REQUIRE(images, "dcrf_batch_add_pairwise_bilateral: null images");
+    batch_add_2d(b, sx, sy, images, sr, sg, sb, potts_w);
+  });
+}
+int dcrf_batch_unary_init(dcrf_batch_t b, float* Q, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(b && Q, "dcrf_batch_unary_init: bad arguments");
+    Batch& B = *b->b;
+    Staged<float> q(Q, (size_t)B.N() * B.L(), on_device != 0, B.stream(), false);
+    B.unary_init(q.d);
+    q.copy_out();
+    DCRF_HIP(hipStreamSynchronize(B.stream()));
+  });
+}
+int dcrf_batch_qp(dcrf_batch_t b, const float* init, float* out, int max_iter, int ref_stop, int on_device,
+                  int* iterations) {
+  return guarded([&] {
+    DCRF_REQUIRE(b && out, "dcrf_batch_qp: bad arguments");
+    Batch& B = *b->b;
+    const size_t n = (size_t)B.N() * B.L();
+    std::unique_ptr<Staged<float>> a;
+    if (init) a = std::make_unique<Staged<float>>(init, n, on_device != 0, B.stream(), true);
+    Staged<float> o(out, n, on_device != 0, B.stream(), false);
+    std::vector<int> it = B.qp(a ? a->d : nullptr, o.d, max_iter, ref_stop != 0);
+    o.copy_out();
+    DCRF_HIP(hipStreamSynchronize(B.stream()));
+    if (iterations) std::copy(it.begin(), it.end(), iterations);
+  });
+}
+int dcrf_batch_lp(dcrf_batch_t b, const float* init, float* out, const dcrf_lp_params* params, int on_device,
+                  int* sweeps, int cap) {
+  return guarded([&] {
+    DCRF_REQUIRE(b && init && out, "dcrf_batch_lp: bad arguments");
+    Batch& B = *b->b;
+    dcrf_lp_params def;
+    dcrf_lp_params_default(&def);
+    const dcrf_lp_params& p = params ? *params : def;
+    LPParams P;
+    P.prox_reg_const = p.prox_reg_const;
+    P.dual_gap_tol = p.dual_gap_tol;
+    P.prox_energy_tol = p.prox_energy_tol;
+    P.prox_max_iter = p.prox_max_iter;
+    P.fw_max_iter = p.fw_max_iter;
+    P.qp_max_iter = p.qp_max_iter;
+    P.qp_tol = p.qp_tol;
+    P.qp_const = p.qp_const;
+    P.best_int = p.best_int != 0;
+    const size_t n = (size_t)B.N() * B.L();
+    Staged<float> a(init, n, on_device != 0, B.stream(), true);
+    Staged<float> o(out, n, on_device != 0, B.stream(), false);
+    std::vector<std::vector<int>> sw = B.lp(a.d, o.d, P);
+    o.copy_out();
+    DCRF_HIP(hipStreamSynchronize(B.stream()));
+    if (sweeps && cap > 0)
+      for (int i = 0; i < B.B(); ++i)
+        for (int j = 0; j < cap; ++j) sweeps[(size_t)i * cap + j] = j < (int)sw[i].size() ? sw[i][j] : 0;
+  });
+}
+int dcrf_batch_assignment_energy_true(dcrf_batch_t b, const int* labels, int on_device, double* energies) {
+  return guarded([&] {
+    DCRF_REQUIRE(b && labels && energies, "dcrf_batch_assignment_energy_true: bad arguments");
+    Batch& B = *b->b;
+    Staged<int> l(labels, (size_t)B.N(), on_device != 0, B.stream(), true);
+    B.assignment_energy_true(l.d, energies);
+  });
+}
+
 int dcrf_profile_report(char* buf, int len) {
   std::string r;
   const int rc = guarded([&] { r = Profiler::get().report(); });

@@ -19,8 +19,9 @@ float* ho_hb(CRF::HOState* h);
 int ho_smax(CRF::HOState* h);
 
 // scalar slots (kS_*, kF_*) are defined in crf_kernels.hpp
-constexpr int S_ECUR = kS_ECUR, S_DOTS = kS_DOTS, S_EPART = kS_EPART, S_HO = kS_HO, S_TMP = kS_TMP, S_COUNT = 32;
-constexpr int F_DONE = kF_DONE, F_ITERS = kF_ITERS, F_COUNT = 8;
+constexpr int S_ECUR = kS_ECUR, S_DOTS = kS_DOTS, S_EPART = kS_EPART, S_HO = kS_HO, S_TMP = kS_TMP,
+              S_COUNT = kS_COUNT;
+constexpr int F_DONE = kF_DONE, F_ITERS = kF_ITERS, F_COUNT = kF_COUNT;
 
 struct CRF::Impl {
   DBuf<float> Q, gy, psx, tmp, ones, rpix, mm0, mm, mmparts;
@@ -83,12 +84,19 @@ void CRF::set_unary(const float* dU) {
 // (symmetric) or 1/(K*1 + 1e-20), evaluated in double and rounded to float.
 void potential_init(Potential& p, const float* d_feat, int N, int d, float w, int ntype, hipStream_t s) {
   DCRF_REQUIRE(ntype >= 0 && ntype <= 3, "addPairwiseEnergy: bad normalisation type");
-  p.d = d;
+  auto lat = std::make_unique<Lattice>(s);
+  lat->build(d_feat, N, d);
+  potential_init_lattice(p, std::move(lat), w, ntype, s);
+}
+
+void potential_init_lattice(Potential& p, std::unique_ptr<Lattice> lat, float w, int ntype, hipStream_t s) {
+  DCRF_REQUIRE(ntype >= 0 && ntype <= 3, "addPairwiseEnergy: bad normalisation type");
+  const int N = lat->N();
+  p.d = lat->d();
   p.w = w;
   p.nn_w = w;
   p.ntype = ntype;
-  p.lat = std::make_unique<Lattice>(s);
-  p.lat->build(d_feat, N, d);
+  p.lat = std::move(lat);
   p.norm.resize(N);
   if (ntype != kNoNorm) {
     DBuf<float> ones(N);
@@ -188,12 +196,12 @@ QPResult CRF::qp(const float* init, float* out, int max_iter, bool ref_stop, int
   launch_dot(s, U_.get(), Q, NL, m.parts.get(), sc + S_TMP + 0);
   for (int k = 0; k < num_pairwise(); ++k) {
     apply(k, Q, tmp, false, false);
-    k_gy_init_accum<<<kRedBlocks, kBlock, 0, s>>>(gy, tmp, Q, NL, m.parts.get());
+    k_gy_init_accum<<<kRedBlocks, kBlock, 0, s>>>(gy, tmp, Q, 1, OneImage{(long)NL}, m.parts.get());
     check_launch("k_gy_init_accum");
     finalize_sums<1><<<1, kBlock, 0, s>>>(m.parts.get(), kRedBlocks, sc + S_TMP + 1 + k);
   }
   if (ho) ho_qp_init(s, Q, gy, sc + S_HO);
-  k_qp_e0<<<1, 1, 0, s>>>(sc, num_pairwise(), ho != nullptr, m.st_energy.get(), stats_cap);
+  k_qp_e0<<<1, 1, 0, s>>>(sc, num_pairwise(), ho != nullptr, m.st_energy.get(), stats_cap, 1);
   check_launch("k_qp_e0");
 
   {
@@ -218,23 +226,23 @@ QPResult CRF::qp(const float* init, float* out, int max_iter, bool ref_stop, int
     }
     {
       KTimer tm("qp_dots", s);
-      k_qp_dots<<<kRedBlocks, kBlock, 0, s>>>(U_.get(), Q, lab, psx, NL, divL, m.parts.get(), fl);
+      k_qp_dots<<<kRedBlocks, kBlock, 0, s>>>(U_.get(), Q, lab, psx, OneImage{N_}, divL, m.parts.get(), fl);
       check_launch("k_qp_dots");
     }
     finalize_sums<3><<<1, kBlock, 0, s>>>(m.parts.get(), kRedBlocks, sc + S_DOTS);
     k_qp_step<<<1, 1, 0, s>>>(sc, fl, ho != nullptr, m.st_num.get(), m.st_denom.get(), m.st_step.get(),
-                              m.st_energy.get(), stats_cap);
+                              m.st_energy.get(), stats_cap, 1);
     check_launch("k_qp_step");
     if (ho) ho_qp_zstep(s, sc, fl);
     {
       KTimer tm("qp_update", s);
       k_qp_update_rg<<<kRedBlocks, kBlock, 0, s>>>(U_.get(), Q, gy, psx, lab, ho ? ho_hb(ho) : nullptr,
-                                                    ho ? ho_smax(ho) : 0, N_, L_, sc, m.parts.get(), fl);
+                                                    ho ? ho_smax(ho) : 0, OneImage{N_}, L_, sc, m.parts.get(), fl);
       check_launch("k_qp_update_rg");
     }
     finalize_sums<1><<<1, kBlock, 0, s>>>(m.parts.get(), kRedBlocks, sc + S_EPART);
     if (ho) ho_qp_after(s, Q, sc + S_HO, fl);
-    k_qp_check<<<1, 1, 0, s>>>(sc, fl, ref_stop ? 1 : 0, ho != nullptr, m.st_energy.get(), stats_cap);
+    k_qp_check<<<1, 1, 0, s>>>(sc, fl, ref_stop ? 1 : 0, ho != nullptr, m.st_energy.get(), stats_cap, 1);
     check_launch("k_qp_check");
   };
 
@@ -376,7 +384,7 @@ double CRF::assignment_energy_true(const int* lab) {
     k_pairwise_true_accum<<<grid_for(N_), kBlock, 0, s_>>>(m.tmp.get(), lab, m.rpix.get(), N_, L_);
     check_launch("k_pairwise_true_accum");
   }
-  k_assign_energy<<<kRedBlocks, kBlock, 0, s_>>>(U_.get(), lab, m.rpix.get(), N_, L_, m.parts.get());
+  k_assign_energy<<<kRedBlocks, kBlock, 0, s_>>>(U_.get(), lab, m.rpix.get(), OneImage{N_}, L_, m.parts.get());
   check_launch("k_assign_energy");
   finalize_sums<1><<<1, kBlock, 0, s_>>>(m.parts.get(), kRedBlocks, m.sc.get() + S_TMP);
   double e = 0;

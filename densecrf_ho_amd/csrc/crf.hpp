@@ -62,6 +62,8 @@ void potential_gradient_dev(Potential& p, int L, const float* b, const float* Q,
 
 // DenseKernel::initLattice: build the lattice and the normalisation
 void potential_init(Potential& p, const float* d_feat, int N, int d, float w, int ntype, hipStream_t s);
+// the same on an already built lattice (e.g. the union of several images')
+void potential_init_lattice(Potential& p, std::unique_ptr<Lattice> lat, float w, int ntype, hipStream_t s);
 // PairwisePotential::apply / applyTranspose on V values per pixel
 void potential_apply(Potential& p, const float* in, float* out, int V, bool transpose, bool no_norm);
 // PairwisePotential::apply_upper_minus_lower_ord (the caller checks [0,1])

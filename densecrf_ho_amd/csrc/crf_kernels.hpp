@@ -115,6 +115,51 @@ inline unsigned rg_grid(long nrows, int L, long cap = 1 << 30) {
   return (unsigned)std::max(1L, std::min(t, cap));
 }
 
+// ---------------------------------------------------------------------------
+// Work split of a launch over images.  OneImage spreads every block over all
+// N pixels.  ImageBlocks (a batch) gives each image its own contiguous range
+// of blocks, so a block only touches one image's pixels, reads that image's
+// scalars, and per-block partial sums combine per image.
+// ---------------------------------------------------------------------------
+struct SegPos {
+  int img;
+  long p0, n;   // first pixel and pixel count of the block's image
+  long lb, nb;  // the block's index among its image's blocks, and their number
+};
+struct OneImage {
+  long N;
+  __device__ inline SegPos pos() const { return SegPos{0, 0, N, (long)blockIdx.x, (long)gridDim.x}; }
+};
+struct ImageBlocks {
+  const int* blk_img;    // image of each block
+  const int* blk_first;  // first block of each image (images + 1 entries)
+  const long* pix;       // first pixel of each image (images + 1 entries)
+  __device__ inline SegPos pos() const {
+    const int i = blk_img[blockIdx.x];
+    const int f = blk_first[i];
+    return SegPos{i, pix[i], pix[i + 1] - pix[i], (long)blockIdx.x - f, (long)(blk_first[i + 1] - f)};
+  }
+};
+
+// per-image sums of per-block partials: block i of the launch sums the NV
+// partials of blocks [first[i], first[i+1]) (or [0, nparts) when first is
+// null) in block order and writes them to out[i * stride ..]
+template <int NV>
+__global__ void finalize_seg(const double* __restrict__ parts, const int* __restrict__ first, int nparts,
+                             double* __restrict__ out, int stride) {
+  const int b0 = first ? first[blockIdx.x] : 0, b1 = first ? first[blockIdx.x + 1] : nparts;
+  double v[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = 0;
+  for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] += parts[(size_t)b * NV + i];
+  block_sum<NV>(v);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) out[(size_t)blockIdx.x * stride + i] = v[i];
+}
+
 // expAndNormalize(-U) (src/densecrf.cpp:328-336), one thread per pixel
 __global__ void __launch_bounds__(kBlock) k_softmax_neg(const float* __restrict__ U, float* __restrict__ Q, int N, int L) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -129,25 +174,32 @@ __global__ void __launch_bounds__(kBlock) k_softmax_neg(const float* __restrict_
 }
 
 // Σ a[i]*b[i] in FP64 over n elements: per-block partials, then one block
-__global__ void __launch_bounds__(kBlock) k_dot(const float* __restrict__ a, const float* __restrict__ b, size_t n, double* __restrict__ parts) {
+template <class SEG>
+__global__ void __launch_bounds__(kBlock) k_dot(const float* __restrict__ a, const float* __restrict__ b, int L, SEG seg,
+                                                double* __restrict__ parts) {
+  const SegPos q = seg.pos();
   double v[1] = {0.0};
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+  const size_t e = (size_t)(q.p0 + q.n) * L;
+  for (size_t i = (size_t)q.p0 * L + (size_t)q.lb * blockDim.x + threadIdx.x; i < e; i += (size_t)q.nb * blockDim.x)
     v[0] += (double)a[i] * (double)b[i];
   block_sum<1>(v);
   if (threadIdx.x == 0) parts[blockIdx.x] = v[0];
 }
 inline void launch_dot(hipStream_t s, const float* a, const float* b, size_t n, double* parts, double* out) {
-  k_dot<<<kRedBlocks, kBlock, 0, s>>>(a, b, n, parts);
+  k_dot<<<kRedBlocks, kBlock, 0, s>>>(a, b, 1, OneImage{(long)n}, parts);
   check_launch("k_dot");
   finalize_sums<1><<<1, kBlock, 0, s>>>(parts, kRedBlocks, out);
   check_launch("finalize_sums");
 }
 
 // grad_y += 2 * tmp (src/densecrf.cpp:380) and partial Σ Q*tmp (:953)
+template <class SEG>
 __global__ void __launch_bounds__(kBlock) k_gy_init_accum(float* __restrict__ gy, const float* __restrict__ tmp, const float* __restrict__ Q,
-                                size_t n, double* __restrict__ parts) {
+                                int L, SEG seg, double* __restrict__ parts) {
+  const SegPos q = seg.pos();
   double v[1] = {0.0};
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+  const size_t e = (size_t)(q.p0 + q.n) * L;
+  for (size_t i = (size_t)q.p0 * L + (size_t)q.lb * blockDim.x + threadIdx.x; i < e; i += (size_t)q.nb * blockDim.x) {
     const float t = tmp[i];
     gy[i] = gy[i] + 2.f * t;
     v[0] += (double)Q[i] * (double)t;
@@ -160,20 +212,26 @@ __global__ void __launch_bounds__(kBlock) k_gy_init_accum(float* __restrict__ gy
 constexpr int kS_ECUR = 0, kS_EPREV = 1, kS_NUM = 2, kS_DENOM = 3, kS_STEP = 4, kS_DOTS = 5, kS_EPART = 8,
               kS_HO = 9, kS_TMP = 17;
 constexpr int kF_DONE = 0, kF_ITERS = 1;
+// per-image strides of the QP scalar slots and flags
+constexpr int kS_COUNT = 32, kF_COUNT = 8;
 // higher-order scalar slots (relative to kS_HO)
 constexpr int kH_A = 0, kH_B = 1, kH_SXZ = 2, kH_DEN = 3, kH_E = 4;
 
 // E0 = U.Q - 0 + Σ_k Q.psi_k Q + 0 (+ higher order)  (:386-387, :940-957)
-__global__ void k_qp_e0(double* sc, int nk, int has_ho, double* st_energy, int cap) {
-  double e = 0;
-  e += sc[kS_TMP + 0];
-  e -= 0.0;
-  for (int k = 0; k < nk; ++k) e += sc[kS_TMP + 1 + k];
-  e += 0.0;
-  if (has_ho) e += sc[kS_HO + kH_E];
-  sc[kS_ECUR] = e;
-  sc[kS_EPREV] = 1.7976931348623157e308;
-  if (cap > 0) st_energy[0] = e;
+// (one thread per image; scalars of image i at sc + i * kS_COUNT)
+__global__ void k_qp_e0(double* sc_all, int nk, int has_ho, double* st_energy, int cap, int nimg) {
+  for (int img = threadIdx.x; img < nimg; img += blockDim.x) {
+    double* sc = sc_all + (size_t)img * kS_COUNT;
+    double e = 0;
+    e += sc[kS_TMP + 0];
+    e -= 0.0;
+    for (int k = 0; k < nk; ++k) e += sc[kS_TMP + 1 + k];
+    e += 0.0;
+    if (has_ho) e += sc[kS_HO + kH_E];
+    sc[kS_ECUR] = e;
+    sc[kS_EPREV] = 1.7976931348623157e308;
+    if (cap > 0) st_energy[0] = e;
+  }
 }
 
 // descent_direction (src/qp.cpp:5-15): first minimum of each column
@@ -218,13 +276,16 @@ __global__ void __launch_bounds__(kBlock) k_argmin_rg(const float* __restrict__ 
 }
 
 // line-search sums U.sx, Q.psisx, sx.psisx with sx = onehot(lab) - Q (:408-414)
+template <class SEG>
 __global__ void __launch_bounds__(kBlock) k_qp_dots(const float* __restrict__ U, const float* __restrict__ Q, const int* __restrict__ lab,
-                          const float* __restrict__ psx, size_t n, FastDiv divL, double* __restrict__ parts,
+                          const float* __restrict__ psx, SEG seg, FastDiv divL, double* __restrict__ parts,
                           const int* __restrict__ fl) {
-  if (fl[kF_DONE]) return;
+  const SegPos qs = seg.pos();
+  if (fl[qs.img * kF_COUNT + kF_DONE]) return;
   const int L = (int)divL.d;
   double v[3] = {0.0, 0.0, 0.0};
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+  const size_t e = (size_t)(qs.p0 + qs.n) * L;
+  for (size_t i = (size_t)qs.p0 * L + (size_t)qs.lb * blockDim.x + threadIdx.x; i < e; i += (size_t)qs.nb * blockDim.x) {
     const uint32_t pix = divL.div((uint32_t)i);
     const int l = (int)(i - (size_t)pix * L);
     const float q = Q[i], p = psx[i];
@@ -242,9 +303,12 @@ __global__ void __launch_bounds__(kBlock) k_qp_dots(const float* __restrict__ U,
 }
 
 // step size (:412-421); done = 2 when the reference breaks on denom==0||num==0
-__global__ void k_qp_step(double* sc, int* fl, int has_ho, double* st_num, double* st_denom, double* st_step,
-                          double* st_energy, int cap) {
-  if (fl[kF_DONE]) return;
+__global__ void k_qp_step(double* sc_all, int* fl_all, int has_ho, double* st_num, double* st_denom, double* st_step,
+                          double* st_energy, int cap, int nimg) {
+  for (int img = threadIdx.x; img < nimg; img += blockDim.x) {
+  double* sc = sc_all + (size_t)img * kS_COUNT;
+  int* fl = fl_all + (size_t)img * kF_COUNT;
+  if (fl[kF_DONE]) continue;
   sc[kS_EPREV] = sc[kS_ECUR];
   double num = sc[kS_DOTS + 0] + 2 * sc[kS_DOTS + 1];
   double denom = sc[kS_DOTS + 2];
@@ -263,54 +327,32 @@ __global__ void k_qp_step(double* sc, int* fl, int has_ho, double* st_num, doubl
   if (denom == 0 || num == 0) {
     if (it + 1 < cap + 1) st_energy[it + 1] = sc[kS_ECUR];  // no update: energy unchanged
     fl[kF_DONE] = 2;
-    return;
+    continue;
   }
   if (denom < 0) step = 1;
   sc[kS_STEP] = step;
   if (it < cap) st_step[it] = step;
-}
-
-// Q += step*sx; grad_y += 2*step*psisx + step*hb; partial Q.(grad_y+U) (:424-434)
-__global__ void __launch_bounds__(kBlock) k_qp_update(const float* __restrict__ U, float* __restrict__ Q, float* __restrict__ gy,
-                            const float* __restrict__ psx, const int* __restrict__ lab, const float* __restrict__ hb,
-                            int hb_npix, size_t n, FastDiv divL, const double* __restrict__ sc,
-                            double* __restrict__ parts, const int* __restrict__ fl) {
-  if (fl[kF_DONE]) return;
-  const int L = (int)divL.d;
-  const double step = sc[kS_STEP];
-  const float st = (float)step, st2 = (float)(2 * step);
-  double v[1] = {0.0};
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const uint32_t pix = divL.div((uint32_t)i);
-    const int l = (int)(i - (size_t)pix * L);
-    const float q0 = Q[i];
-    const float sx = (lab[pix] == l ? 1.f : 0.f) - q0;
-    const float q = q0 + st * sx;
-    const float h = (hb != nullptr && (int)pix < hb_npix) ? hb[i] : 0.f;
-    const float g = gy[i] + (st2 * psx[i] + st * h);
-    Q[i] = q;
-    gy[i] = g;
-    v[0] += (double)q * (double)(g + U[i]);
   }
-  block_sum<1>(v);
-  if (threadIdx.x == 0) parts[blockIdx.x] = v[0];
 }
 
 // Q += step*sx; grad_y += 2*step*psisx + step*hb; partial Q.(grad_y+U)
 // (:424-434), then the first minimum of the new grad_y row: the next
 // iteration's conditional gradient (src/qp.cpp:5-15)
+template <class SEG>
 __global__ void __launch_bounds__(kBlock) k_qp_update_rg(const float* __restrict__ U, float* __restrict__ Q, float* __restrict__ gy,
                                const float* __restrict__ psx, int* __restrict__ lab, const float* __restrict__ hb,
-                               int hb_npix, int N, int L, const double* __restrict__ sc, double* __restrict__ parts,
+                               int hb_npix, SEG sg, int L, const double* __restrict__ sc, double* __restrict__ parts,
                                const int* __restrict__ fl) {
-  if (fl[kF_DONE]) return;
+  const SegPos qs = sg.pos();
+  if (fl[qs.img * kF_COUNT + kF_DONE]) return;
   __shared__ float seg[kBlock];
-  const double step = sc[kS_STEP];
+  const double step = sc[(size_t)qs.img * kS_COUNT + kS_STEP];
   const float st = (float)step, st2 = (float)(2 * step);
   double e = 0.0;
-  const long ntiles = rg_tiles(N, L);
-  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const RG r = rg_at(L, N, tile);
+  const long ntiles = rg_tiles(qs.n, L);
+  for (long tile = qs.lb; tile < ntiles; tile += qs.nb) {
+    RG r = rg_at(L, qs.n, tile);
+    r.row += qs.p0;
     float* row = seg + (threadIdx.x & ~63) + r.g * L;
     if (r.valid) {
       const size_t idx = (size_t)r.row * L + r.k;
@@ -344,14 +386,19 @@ __global__ void __launch_bounds__(kBlock) k_qp_update_rg(const float* __restrict
 
 // new energy and the reference's stopping rule old - energy > DIFF_ENG=100
 // (:390, :434-435)
-__global__ void k_qp_check(double* sc, int* fl, int ref_stop, int has_ho, double* st_energy, int cap) {
-  if (fl[kF_DONE]) return;
-  double e = sc[kS_EPART];
-  if (has_ho) e += sc[kS_HO + kH_E];
-  sc[kS_ECUR] = e;
-  const int it = fl[kF_ITERS];
-  if (it < cap + 1) st_energy[it] = e;
-  if (ref_stop && !((sc[kS_EPREV] - e) > 100.0)) fl[kF_DONE] = 1;
+__global__ void k_qp_check(double* sc_all, int* fl_all, int ref_stop, int has_ho, double* st_energy, int cap,
+                           int nimg) {
+  for (int img = threadIdx.x; img < nimg; img += blockDim.x) {
+    double* sc = sc_all + (size_t)img * kS_COUNT;
+    int* fl = fl_all + (size_t)img * kF_COUNT;
+    if (fl[kF_DONE]) continue;
+    double e = sc[kS_EPART];
+    if (has_ho) e += sc[kS_HO + kH_E];
+    sc[kS_ECUR] = e;
+    const int it = fl[kF_ITERS];
+    if (it < cap + 1) st_energy[it] = e;
+    if (ref_stop && !((sc[kS_EPREV] - e) > 100.0)) fl[kF_DONE] = 1;
+  }
 }
 
 // currentMap (src/densecrf.cpp:929-938): first maximum of each column
@@ -382,10 +429,12 @@ __global__ void __launch_bounds__(kBlock) k_pairwise_true_accum(const float* __r
 }
 
 // Σ_i (unary(l_i, i) + pairwise_i) in FP64 (:873-885)
+template <class SEG>
 __global__ void __launch_bounds__(kBlock) k_assign_energy(const float* __restrict__ U, const int* __restrict__ lab,
-                                const float* __restrict__ r, int N, int L, double* __restrict__ parts) {
+                                const float* __restrict__ r, SEG seg, int L, double* __restrict__ parts) {
+  const SegPos q = seg.pos();
   double v[1] = {0.0};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+  for (long i = q.p0 + q.lb * blockDim.x + threadIdx.x; i < q.p0 + q.n; i += q.nb * blockDim.x) {
     const int l = lab[i];
     const float u = (l >= 0 && l < L) ? U[(size_t)i * L + l] : 0.f;
     v[0] += (double)(u + r[i]);

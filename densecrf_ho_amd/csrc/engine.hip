@@ -5,4 +5,5 @@
 #include "ho.hip"
 #include "lp.hip"
 #include "adjoint.hip"
+#include "batch.hip"
 #include "capi.hip"

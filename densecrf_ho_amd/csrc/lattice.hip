@@ -653,11 +653,14 @@ __global__ void __launch_bounds__(kBlock) k_blur2(int R2, int M, const int* __re
 
 // bins of every (pixel, label) of the rescaled field, packed lo | up << 4;
 // edge = 1 if some bin_up != bin_lo (then the upper pass is done in full)
+// A lattice that is the union of several images' lattices rescales each
+// image with its own {min, max}: pimg[pixel] picks the pair mm[2*img].
 __global__ void __launch_bounds__(kBlock) k_obin(const float* __restrict__ Q, const float* __restrict__ mm, size_t n,
-                       uint8_t* __restrict__ bins, int* __restrict__ edge) {
+                       uint8_t* __restrict__ bins, int* __restrict__ edge, const uint16_t* __restrict__ pimg,
+                       FastDiv divL) {
   bool e = false;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const float q = rescaled(Q, i, mm);
+    const float q = rescaled(Q, i, pimg ? mm + 2 * pimg[divL.div((uint32_t)i)] : mm);
     const int lo = bin_lo(q), up = bin_up(q);
     e = e || lo != up;
     bins[i] = (uint8_t)(lo | (up << 4));
@@ -1013,7 +1016,8 @@ void Lattice::ordered(const float* Q, const float* d_minmax, int L, const SliceA
   {
     KTimer tm("obin" + tag, s);
     k_zero_int<<<1, 1, 0, s>>>(edge);
-    k_obin<<<grid_for(NL, kBlock, 2048), kBlock, 0, s>>>(Q, d_minmax, NL, bins_.get(), edge);
+    k_obin<<<grid_for(NL, kBlock, 2048), kBlock, 0, s>>>(Q, d_minmax, NL, bins_.get(), edge,
+                                                         pimg_.n ? pimg_.get() : nullptr, FastDiv((uint32_t)L));
   }
   {
     KTimer tm("osplat" + tag, s);
@@ -1082,6 +1086,90 @@ void Lattice::ordered(const float* Q, const float* d_minmax, int L, const SliceA
   else
     slice(k_oslice<0>);
   check_launch("k_oslice");
+}
+
+namespace {
+// dst[i] = src[i] + add (add only to entries >= 0 when skip_neg)
+__global__ void __launch_bounds__(kBlock) k_copy_add(const int* __restrict__ src, int* __restrict__ dst, size_t n,
+                                                     int add, int skip_neg) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int v = src[t];
+  dst[t] = (skip_neg && v < 0) ? v : v + add;
+}
+__global__ void __launch_bounds__(kBlock) k_fill_u16(uint16_t* __restrict__ p, size_t n, uint16_t v) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+}  // namespace
+
+std::unique_ptr<Lattice> Lattice::concat(const std::vector<const Lattice*>& parts, hipStream_t s) {
+  DCRF_REQUIRE(!parts.empty() && parts.size() <= 65535, "lattice union: 1..65535 parts");
+  const int d = parts[0]->d_, d1 = d + 1;
+  long N = 0, M = 0, E = 0, nch = 0, nmul = 0, npart = 0;
+  for (const Lattice* p : parts) {
+    DCRF_REQUIRE(p->d_ == d, "lattice union: parts differ in feature dimension");
+    N += p->N_;
+    M += p->M_;
+    E += (long)p->N_ * d1;
+    nch += p->nchunks_;
+    nmul += p->nmulti_;
+    npart += p->npart_;
+  }
+  DCRF_REQUIRE(N * d1 < (1L << 31) && M < (1L << 30), "lattice union: too large for 32-bit indices");
+  auto u = std::make_unique<Lattice>(s);
+  u->N_ = (int)N;
+  u->Np_ = (int)N;
+  u->d_ = d;
+  u->M_ = (int)M;
+  u->nchunks_ = (int)nch;
+  u->nmulti_ = (int)nmul;
+  u->npart_ = (int)npart;
+  u->offset_.resize(E);
+  u->bary_.resize(E);
+  u->nbr_.resize((size_t)d1 * M * 2);
+  u->sp_pix_.resize(E);
+  u->sp_w_.resize(E);
+  u->seg_.resize(M + 1);
+  u->chstart_.resize(M + 1);
+  u->pstart_.resize(M + 1);
+  u->chpoint_.resize(std::max(nch, 1L));
+  u->multi_.resize(std::max(nmul, 1L));
+  u->pimg_.resize(N);
+  long pb = 0, mb = 0, eb = 0, cb = 0, ub = 0, qb = 0;
+  for (size_t b = 0; b < parts.size(); ++b) {
+    const Lattice& p = *parts[b];
+    const size_t e = (size_t)p.N_ * d1, m = p.M_;
+    auto add = [&](const int* src, int* dst, size_t n, long a, int skip) {
+      if (n) k_copy_add<<<grid_for(n), kBlock, 0, s>>>(src, dst, n, (int)a, skip);
+    };
+    add(p.offset_.get(), u->offset_.get() + eb, e, mb, 0);
+    DCRF_HIP(hipMemcpyAsync(u->bary_.get() + eb, p.bary_.get(), e * sizeof(float), hipMemcpyDeviceToDevice, s));
+    for (int j = 0; j < d1; ++j)
+      add(p.nbr_.get() + (size_t)j * m * 2, u->nbr_.get() + ((size_t)j * M + mb) * 2, m * 2, mb, 1);
+    add(p.sp_pix_.get(), u->sp_pix_.get() + eb, e, pb, 0);
+    DCRF_HIP(hipMemcpyAsync(u->sp_w_.get() + eb, p.sp_w_.get(), e * sizeof(float), hipMemcpyDeviceToDevice, s));
+    // per-point starts: M entries each, the union's final entry is written last
+    add(p.seg_.get(), u->seg_.get() + mb, m, eb, 0);
+    add(p.chstart_.get(), u->chstart_.get() + mb, m, cb, 0);
+    add(p.pstart_.get(), u->pstart_.get() + mb, m, qb, 0);
+    add(p.chpoint_.get(), u->chpoint_.get() + cb, p.nchunks_, mb, 0);
+    if (p.nmulti_ > 0) add(p.multi_.get(), u->multi_.get() + ub, p.nmulti_, mb, 0);
+    k_fill_u16<<<grid_for(p.N_), kBlock, 0, s>>>(u->pimg_.get() + pb, p.N_, (uint16_t)b);
+    pb += p.N_;
+    mb += m;
+    eb += e;
+    cb += p.nchunks_;
+    ub += p.nmulti_;
+    qb += p.npart_;
+  }
+  const int tail[3] = {(int)E, (int)nch, (int)npart};
+  DCRF_HIP(hipMemcpyAsync(u->seg_.get() + M, &tail[0], sizeof(int), hipMemcpyHostToDevice, s));
+  DCRF_HIP(hipMemcpyAsync(u->chstart_.get() + M, &tail[1], sizeof(int), hipMemcpyHostToDevice, s));
+  DCRF_HIP(hipMemcpyAsync(u->pstart_.get() + M, &tail[2], sizeof(int), hipMemcpyHostToDevice, s));
+  check_launch("lattice union");
+  DCRF_HIP(hipStreamSynchronize(s));
+  return u;
 }
 
 void Lattice::export_tables(int* offset, int* rank, float* bary, short* keys, int* nbr) const {

@@ -8,6 +8,9 @@
 // reference's HashTable (src/permutohedral.cpp:100-128).
 #pragma once
 
+#include <memory>
+#include <vector>
+
 #include "common.hpp"
 
 namespace dcrf {
@@ -62,6 +65,12 @@ class Lattice {
   // the fly from Q; then out (+)= scale * result.
   void ordered(const float* Q, const float* d_minmax, int L, const SliceArgs& out);
 
+  // The disjoint union of several lattices of the same dimension (one per
+  // image): pixels and points are renumbered image after image, so filtering
+  // the union equals filtering every part.  The ordered filter then takes one
+  // {min, max} pair per part.
+  static std::unique_ptr<Lattice> concat(const std::vector<const Lattice*>& parts, hipStream_t s);
+
   // copies of the lattice tables for parity checks (host pointers, may be null)
   void export_tables(int* offset, int* rank, float* bary, short* keys, int* nbr) const;
 
@@ -96,6 +105,7 @@ class Lattice {
   DBuf<float> tb_;
   DBuf<int> edge_;
   DBuf<uint8_t> bins_;
+  DBuf<uint16_t> pimg_;  // part of each pixel (union lattices only)
   bool have_tb_ = false;
 };
 

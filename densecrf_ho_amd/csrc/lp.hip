@@ -24,7 +24,8 @@ namespace {
 // LP scalar slots (doubles) and flags (ints) in LPState::sc / fl
 constexpr int kL_GAP = 0, kL_TT = 1, kL_DELTA = 2, kL_QPV = 3, kL_EINT = 4, kL_EHO = 5, kL_EPAIR = 6, kL_TMP = 8,
               kL_NSC = 16;
-constexpr int kLF_SWEEP_DONE = 0, kLF_SWEEPS = 1, kLF_ERR = 2, kLF_NFL = 4;
+// kLF_TOTAL: sweeps done so far in the current inner step
+constexpr int kLF_SWEEP_DONE = 0, kLF_SWEEPS = 1, kLF_ERR = 2, kLF_TOTAL = 3, kLF_NFL = 4;
 constexpr int kSweepBatch = 16;
 
 // sum of the first L entries of a register array in Eigen's order for an
@@ -91,15 +92,18 @@ struct LPConst {
 // alpha/u update of the previous step (:737-738, fused here), then
 // Y = lambda * (tmp - sum(tmp)/M) + cur_Q with tmp = alpha + u - U (:627-635).
 // Row groups; sum(tmp) is the sum of an aligned copy (qp_gamma_multiplyC).
-__global__ void __launch_bounds__(kBlock) k_lp_Y(int N, int L, LPConst c, float* __restrict__ alpha, float* __restrict__ u,
+template <class SEG>
+__global__ void __launch_bounds__(kBlock) k_lp_Y(SEG sg, int L, LPConst c, float* __restrict__ alpha, float* __restrict__ u,
                        const float* __restrict__ sa, const float* __restrict__ su, const double* __restrict__ sc,
                        int apply_update, const float* __restrict__ U, const float* __restrict__ curQ,
                        float* __restrict__ Y) {
   __shared__ float seg[kBlock];
-  const float dl = apply_update ? (float)sc[kL_DELTA] : 0.f;
-  const long ntiles = rg_tiles(N, L);
-  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const RG r = rg_at(L, N, tile);
+  const SegPos qs = sg.pos();
+  const float dl = apply_update ? (float)sc[(size_t)qs.img * kL_NSC + kL_DELTA] : 0.f;
+  const long ntiles = rg_tiles(qs.n, L);
+  for (long tile = qs.lb; tile < ntiles; tile += qs.nb) {
+    RG r = rg_at(L, qs.n, tile);
+    r.row += qs.p0;
     float* row = seg + (threadIdx.x & ~63) + r.g * L;
     const size_t idx = (size_t)r.row * L + r.k;
     float t = 0.f;
@@ -156,18 +160,27 @@ __device__ inline __attribute__((always_inline)) float qp_sweep(float (&g)[MAXL]
 //     gout, per-sweep block partial sums of qp_values to parts
 //   replay (replay == 1): if the loop stopped after s < nsw sweeps
 //     (fl[kLF_SWEEPS]), redo s sweeps from gin into gout; otherwise no-op
-template <int MAXL, int LT = 0>  // LT > 0: the label count is known at compile time
-__global__ void __launch_bounds__(kBlock) k_lp_sweeps(int N, int Lr, LPConst c, const float* __restrict__ gin, float* __restrict__ gout,
+//   An image whose loop ended in an earlier batch of this inner step only
+//   carries gamma over (gin -> gout) in pass 1.  Block b works on pixels
+//   [lb * kBlock, lb * kBlock + kBlock) of its image.
+template <int MAXL, int LT = 0, class SEG>  // LT > 0: the label count is known at compile time
+__global__ void __launch_bounds__(kBlock) k_lp_sweeps(SEG sg, int Lr, LPConst c, const float* __restrict__ gin, float* __restrict__ gout,
                             const float* __restrict__ Y, int nsw, int replay, double* __restrict__ parts,
-                            const int* __restrict__ fl) {
+                            const int* __restrict__ fl_all) {
   __shared__ float tile[kBlock * MAXL];
   const int L = LT > 0 ? LT : Lr;
+  const SegPos qs = sg.pos();
+  const int* fl = fl_all + (size_t)qs.img * kLF_NFL;
   if (replay && fl[kLF_SWEEPS] >= nsw) return;
-  const int n = replay ? fl[kLF_SWEEPS] : nsw;
-  const int p0 = blockIdx.x * kBlock;
-  const int np = min(kBlock, N - p0);
+  const long p0 = qs.p0 + qs.lb * kBlock;
+  const int np = (int)min((long)kBlock, qs.n - qs.lb * kBlock);
   const size_t base = (size_t)p0 * L;
   const int t = threadIdx.x;
+  if (!replay && fl[kLF_SWEEP_DONE]) {
+    for (int i = t; i < np * L; i += kBlock) gout[base + i] = gin[base + i];
+    return;
+  }
+  const int n = replay ? fl[kLF_SWEEPS] : nsw;
   const bool act = t < np;
   float g[MAXL], y[MAXL];
   for (int i = t; i < np * L; i += kBlock) tile[i] = gin[base + i];
@@ -205,15 +218,25 @@ __global__ void __launch_bounds__(kBlock) k_lp_sweeps(int N, int Lr, LPConst c, 
 // differs from the previous one by less than qp_tol; fl[kLF_SWEEPS] = sweeps
 // to replay, fl[kLF_SWEEP_DONE] = 1 if the loop ends in this batch.
 // sc[kL_QPV] carries the previous total across batches.
-__global__ void k_lp_sweep_decide(const double* __restrict__ parts, int nblocks, int nsw, float qp_tol, int done_before,
-                                  int max_total, double* __restrict__ sc, int* __restrict__ fl) {
+// One block per image: the image's blocks are [first[i], first[i+1]) of the
+// sweep launch (all nblocks when first is null); an image whose loop already
+// ended in this inner step is left as it is (its replay is a no-op).
+__global__ void k_lp_sweep_decide(const double* __restrict__ parts, int nblocks, const int* __restrict__ first, int nsw,
+                                  float qp_tol, int max_total, double* __restrict__ sc_all, int* __restrict__ fl_all) {
   __shared__ double tot[kSweepBatch];
   __shared__ double red[kSweepBatch][1024 / 64];
+  double* sc = sc_all + (size_t)blockIdx.x * kL_NSC;
+  int* fl = fl_all + (size_t)blockIdx.x * kLF_NFL;
+  if (fl[kLF_SWEEP_DONE]) {
+    if (threadIdx.x == 0) fl[kLF_SWEEPS] = nsw;
+    return;
+  }
+  const int b0 = first ? first[blockIdx.x] : 0, b1 = first ? first[blockIdx.x + 1] : nblocks;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   double acc[kSweepBatch];
 #pragma unroll
   for (int sw = 0; sw < kSweepBatch; ++sw) acc[sw] = 0.0;
-  for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+  for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x)
 #pragma unroll
     for (int sw = 0; sw < kSweepBatch; ++sw)
       if (sw < nsw) acc[sw] += parts[(size_t)sw * nblocks + b];
@@ -230,6 +253,7 @@ __global__ void k_lp_sweep_decide(const double* __restrict__ parts, int nblocks,
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
+  const int done_before = fl[kLF_TOTAL];
   float prev = done_before == 0 ? 3.402823466e38f : (float)sc[kL_QPV];
   int stop = nsw, finished = 0;
   for (int sw = 0; sw < nsw; ++sw) {
@@ -245,6 +269,15 @@ __global__ void k_lp_sweep_decide(const double* __restrict__ parts, int nblocks,
   sc[kL_QPV] = (double)prev;
   fl[kLF_SWEEPS] = stop;
   fl[kLF_SWEEP_DONE] = finished;
+  fl[kLF_TOTAL] = done_before + stop;
+}
+
+// start of an inner step's sweep loop, one thread per image
+__global__ void k_lp_sweep_reset(int* __restrict__ fl_all, int nimg) {
+  for (int i = threadIdx.x; i < nimg; i += blockDim.x) {
+    fl_all[(size_t)i * kLF_NFL + kLF_SWEEP_DONE] = 0;
+    fl_all[(size_t)i * kLF_NFL + kLF_TOTAL] = 0;
+  }
 }
 
 // NaN-propagating min / max (Eigen's minCoeff/maxCoeff see NaN as unordered;
@@ -255,15 +288,18 @@ __device__ inline float nmax(float a, float b) { return (a != a || b != b) ? NAN
 // beta = -colsum(alpha+u+gamma-U)/M, Q = lambda*(alpha+beta+u+gamma-U) + cur_Q
 // (:684-697), row groups; the column sum follows Eigen's order for the
 // column's alignment.  Block min/max of Q for rescale (:699).
-__global__ void __launch_bounds__(kBlock) k_lp_betaQ(int N, int L, float lambda, const float* __restrict__ alpha, const float* __restrict__ u,
+template <class SEG>
+__global__ void __launch_bounds__(kBlock) k_lp_betaQ(SEG sg, int L, float lambda, const float* __restrict__ alpha, const float* __restrict__ u,
                            const float* __restrict__ gamma, const float* __restrict__ U,
                            const float* __restrict__ curQ, float* __restrict__ Q, float* __restrict__ mmparts) {
   __shared__ float seg[kBlock];
   __shared__ float smn[kBlock], smx[kBlock];
   float mn = INFINITY, mx = -INFINITY;
-  const long ntiles = rg_tiles(N, L);
-  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const RG r = rg_at(L, N, tile);
+  const SegPos qs = sg.pos();
+  const long ntiles = rg_tiles(qs.n, L);
+  for (long tile = qs.lb; tile < ntiles; tile += qs.nb) {
+    RG r = rg_at(L, qs.n, tile);
+    r.row += qs.p0;
     float* row = seg + (threadIdx.x & ~63) + r.g * L;
     const size_t idx = (size_t)r.row * L + r.k;
     float a = 0.f, uu = 0.f, gm = 0.f, uv = 0.f;
@@ -303,11 +339,16 @@ __global__ void __launch_bounds__(kBlock) k_lp_betaQ(int N, int L, float lambda,
 
 // rescale (src/eigen_utils.cpp:120-140): mm = {min, max(Q - min)}; error when
 // max < 0 (only possible with NaN here).  One block.
-__global__ void k_lp_minmax_final(const float* __restrict__ parts, int nb, float* __restrict__ mm,
-                                  int* __restrict__ fl) {
+// (one block per image over its blocks [first[i], first[i+1]), or all nb
+// blocks when first is null; mm and fl of image i at mm + 2i, fl + i*kLF_NFL)
+__global__ void k_lp_minmax_final(const float* __restrict__ parts, int nb, const int* __restrict__ first,
+                                  float* __restrict__ mm_all, int* __restrict__ fl_all) {
   __shared__ float smn[1024], smx[1024];
+  float* mm = mm_all + 2 * (size_t)blockIdx.x;
+  int* fl = fl_all + (size_t)blockIdx.x * kLF_NFL;
+  const int b0 = first ? first[blockIdx.x] : 0, b1 = first ? first[blockIdx.x + 1] : nb;
   float mn = INFINITY, mx = -INFINITY;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+  for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
     mn = nmin(mn, parts[2 * b]);
     mx = nmax(mx, parts[2 * b + 1]);
   }
@@ -330,11 +371,14 @@ __global__ void k_lp_minmax_final(const float* __restrict__ parts, int nb, float
 }
 
 // tmp = alpha + u - (sa + su); gap = tmp.Q; tt = tmp.tmp (:725-729)
-__global__ void __launch_bounds__(kBlock) k_lp_gap(size_t n, const float* __restrict__ alpha, const float* __restrict__ u,
+template <class SEG>
+__global__ void __launch_bounds__(kBlock) k_lp_gap(SEG sg, int L, const float* __restrict__ alpha, const float* __restrict__ u,
                          const float* __restrict__ sa, const float* __restrict__ su, const float* __restrict__ Q,
                          double* __restrict__ parts) {
+  const SegPos qs = sg.pos();
   double v[2] = {0.0, 0.0};
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+  const size_t e = (size_t)(qs.p0 + qs.n) * L;
+  for (size_t i = (size_t)qs.p0 * L + (size_t)qs.lb * blockDim.x + threadIdx.x; i < e; i += (size_t)qs.nb * blockDim.x) {
     const float uu = u ? u[i] : 0.f, ss = su ? su[i] : 0.f;
     const float t = (alpha[i] + uu) - (sa[i] + ss);
     v[0] += (double)t * (double)Q[i];
@@ -347,16 +391,20 @@ __global__ void __launch_bounds__(kBlock) k_lp_gap(size_t n, const float* __rest
   }
 }
 // delta = clamp((float)(gap / (lambda * tt)), 0, 1) (:729-730); stats
-__global__ void k_lp_delta(double* __restrict__ sc, float lambda, double* __restrict__ st_gap,
-                           double* __restrict__ st_delta, int idx, int cap) {
-  const double gap = sc[kL_GAP], tt = sc[kL_TT];
-  float d = (float)(gap / ((double)lambda * tt));
-  d = (d < 0.0f) ? 0.0f : d;  // std::max(d, 0.0f)
-  d = (1.0f < d) ? 1.0f : d;  // std::min(d, 1.0f)
-  sc[kL_DELTA] = d;
-  if (idx < cap) {
-    st_gap[idx] = gap;
-    st_delta[idx] = d;
+// (one thread per image; the record st_gap / st_delta is kept for image 0)
+__global__ void k_lp_delta(double* __restrict__ sc_all, float lambda, double* __restrict__ st_gap,
+                           double* __restrict__ st_delta, int idx, int cap, int nimg) {
+  for (int img = threadIdx.x; img < nimg; img += blockDim.x) {
+    double* sc = sc_all + (size_t)img * kL_NSC;
+    const double gap = sc[kL_GAP], tt = sc[kL_TT];
+    float d = (float)(gap / ((double)lambda * tt));
+    d = (d < 0.0f) ? 0.0f : d;  // std::max(d, 0.0f)
+    d = (1.0f < d) ? 1.0f : d;  // std::min(d, 1.0f)
+    sc[kL_DELTA] = d;
+    if (img == 0 && idx < cap) {
+      st_gap[idx] = gap;
+      st_delta[idx] = d;
+    }
   }
 }
 
@@ -488,7 +536,7 @@ void CRF::int_energy_dev(const int* lab, double* dst, double* scratch, hipStream
     p.lat->filter(a, o, L_, false);
     k_pairwise_true_accum<<<grid_for(N_), kBlock, 0, s>>>(m.tmp.get(), lab, m.rpix.get(), N_, L_);
   }
-  k_assign_energy<<<kRedBlocks, kBlock, 0, s>>>(U_.get(), lab, m.rpix.get(), N_, L_, m.parts.get());
+  k_assign_energy<<<kRedBlocks, kBlock, 0, s>>>(U_.get(), lab, m.rpix.get(), OneImage{N_}, L_, m.parts.get());
   finalize_sums<1><<<1, kBlock, 0, s>>>(m.parts.get(), kRedBlocks, dst);
   if (R_ > 0) {
     ho_assign_dev(lab, scratch, m.iscratch.get(), s);
@@ -626,7 +674,7 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
       ++pit;
       {
         KTimer tm("lp_Y", s);
-        k_lp_Y<<<rg_grid(N, L, 8192), kBlock, 0, s>>>(N, L, c, st.alpha.get(), u, st.sa.get(), su, sc,
+        k_lp_Y<<<rg_grid(N, L, 8192), kBlock, 0, s>>>(OneImage{N}, L, c, st.alpha.get(), u, st.sa.get(), su, sc,
                                                      pit > 1 ? 1 : 0, U_.get(), st.curQ.get(), st.Y.get());
       }
       // qp-gamma sweeps in batches: pass 1 runs `nsw` sweeps gamma -> gamma2,
@@ -634,27 +682,30 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
       // that many sweeps only when the stop comes before the end of the batch.
       // The batch length follows the previous step's sweep count.
       int done_sweeps = 0;
+      k_lp_sweep_reset<<<1, 64, 0, s>>>(fl, 1);
       while (true) {
         const int guess = pit == 1 ? kSweepBatch : std::max(2, std::min(kSweepBatch, prev_sweeps));
         const int nsw = std::min(guess, P.qp_max_iter - done_sweeps);
         {
           KTimer tm("lp_sweeps", s);
           if (L == 21)
-            k_lp_sweeps<21, 21><<<nbp, kBlock, 0, s>>>(N, L, c, st.gamma.get(), st.gamma2.get(), st.Y.get(), nsw, 0,
-                                                       st.parts.get(), fl);
+            k_lp_sweeps<21, 21><<<nbp, kBlock, 0, s>>>(OneImage{N}, L, c, st.gamma.get(), st.gamma2.get(), st.Y.get(),
+                                                       nsw, 0, st.parts.get(), fl);
           else
-            DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbp, kBlock, 0, s>>>(N, L, c, st.gamma.get(), st.gamma2.get(),
-                                                                           st.Y.get(), nsw, 0, st.parts.get(), fl)));
+            DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbp, kBlock, 0, s>>>(OneImage{N}, L, c, st.gamma.get(),
+                                                                           st.gamma2.get(), st.Y.get(), nsw, 0,
+                                                                           st.parts.get(), fl)));
         }
-        k_lp_sweep_decide<<<1, 1024, 0, s>>>(st.parts.get(), nbp, nsw, P.qp_tol, done_sweeps, P.qp_max_iter, sc, fl);
+        k_lp_sweep_decide<<<1, 1024, 0, s>>>(st.parts.get(), nbp, nullptr, nsw, P.qp_tol, P.qp_max_iter, sc, fl);
         {
           KTimer tm("lp_sweeps_replay", s);
           if (L == 21)
-            k_lp_sweeps<21, 21><<<nbp, kBlock, 0, s>>>(N, L, c, st.gamma.get(), st.gamma2.get(), st.Y.get(), nsw, 1,
-                                                       st.parts.get(), fl);
+            k_lp_sweeps<21, 21><<<nbp, kBlock, 0, s>>>(OneImage{N}, L, c, st.gamma.get(), st.gamma2.get(), st.Y.get(),
+                                                       nsw, 1, st.parts.get(), fl);
           else
-            DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbp, kBlock, 0, s>>>(N, L, c, st.gamma.get(), st.gamma2.get(),
-                                                                           st.Y.get(), nsw, 1, st.parts.get(), fl)));
+            DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbp, kBlock, 0, s>>>(OneImage{N}, L, c, st.gamma.get(),
+                                                                           st.gamma2.get(), st.Y.get(), nsw, 1,
+                                                                           st.parts.get(), fl)));
         }
         std::swap(st.gamma, st.gamma2);
         int hf[2] = {0, 0};
@@ -668,9 +719,9 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
       {
         KTimer tm("lp_betaQ", s);
         const unsigned nbq = rg_grid(N, L, 4096);
-        k_lp_betaQ<<<nbq, kBlock, 0, s>>>(N, L, lambda, st.alpha.get(), u, st.gamma.get(), U_.get(), st.curQ.get(),
-                                          st.Q.get(), st.mmparts.get());
-        k_lp_minmax_final<<<1, 1024, 0, s>>>(st.mmparts.get(), nbq, st.mm.get(), fl);
+        k_lp_betaQ<<<nbq, kBlock, 0, s>>>(OneImage{N}, L, lambda, st.alpha.get(), u, st.gamma.get(), U_.get(),
+                                          st.curQ.get(), st.Q.get(), st.mmparts.get());
+        k_lp_minmax_final<<<1, 1024, 0, s>>>(st.mmparts.get(), nbq, nullptr, st.mm.get(), fl);
       }
       for (int k = 0; k < num_pairwise(); ++k) {
         Potential& p = pairwise(k);
@@ -683,9 +734,10 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
       if (ho) ucondgrad_dev(st.Q.get(), su, s);
       {
         KTimer tm("lp_gap", s);
-        k_lp_gap<<<kRedBlocks, kBlock, 0, s>>>(NL, st.alpha.get(), u, st.sa.get(), su, st.Q.get(), st.parts.get());
+        k_lp_gap<<<kRedBlocks, kBlock, 0, s>>>(OneImage{N}, L, st.alpha.get(), u, st.sa.get(), su, st.Q.get(),
+                                               st.parts.get());
         finalize_sums<2><<<1, kBlock, 0, s>>>(st.parts.get(), kRedBlocks, sc + kL_GAP);
-        k_lp_delta<<<1, 1, 0, s>>>(sc, lambda, st.st_gap.get(), st.st_delta.get(), inner_idx, cap);
+        k_lp_delta<<<1, 1, 0, s>>>(sc, lambda, st.st_gap.get(), st.st_delta.get(), inner_idx, cap, 1);
       }
       ++inner_idx;
     } while (pit < P.fw_max_iter);

@@ -55,6 +55,7 @@ extern "C" {
 typedef struct dcrf_lattice_s* dcrf_lattice_t;
 typedef struct dcrf_pairwise_s* dcrf_pairwise_t;
 typedef struct dcrf_crf_s* dcrf_crf_t;
+typedef struct dcrf_batch_s* dcrf_batch_t;
 
 const char* dcrf_last_error(void);
 /* library version string */
@@ -228,6 +229,42 @@ int dcrf_crf_meanfield_backward(dcrf_crf_t crf, const float* dQ, float* dU, doub
  * unrolled-inference gradient. */
 int dcrf_crf_cross_entropy(dcrf_crf_t crf, const float* Q, const int* labels, float* dQ, double* loss,
                            int on_device);
+
+/* ------------------------------------------------------------------------ */
+/* Batches: many images of uneven sizes solved together.  No reference       */
+/* counterpart (the reference makes one DenseCRF per image and runs them one */
+/* after another); every image follows the per-image DenseCRF algorithm      */
+/* (qp_inference_super_pixels_non_convex, lp_inference_prox_super_pixels     */
+/* with best_int; no superpixel layers).  Fields of a batch are the images'  */
+/* fields concatenated: image b owns pixels [offsets[b], offsets[b+1]).      */
+/* ------------------------------------------------------------------------ */
+int dcrf_batch_create(int L, int B, const int* npix, dcrf_batch_t* out);
+/* images of W[b] x H[b] pixels (needed for the 2-D kernels below) */
+int dcrf_batch_create_2d(int L, int B, const int* W, const int* H, dcrf_batch_t* out);
+int dcrf_batch_destroy(dcrf_batch_t b);
+/* offsets[0..B] (long long) of every image's first pixel in the batch fields */
+int dcrf_batch_pixel_offsets(dcrf_batch_t b, long long* offsets);
+/* replay the fixed parts of every solver iteration from HIP graphs (default on) */
+int dcrf_batch_set_graphs(dcrf_batch_t b, int on);
+/* setUnaryEnergy of image `img` (N_img x L) */
+int dcrf_batch_set_unary(dcrf_batch_t b, int img, const float* unary, int on_device);
+/* addPairwiseEnergy(features[img], PottsCompatibility(w), DIAG_KERNEL, ntype) for every image */
+int dcrf_batch_add_pairwise(dcrf_batch_t b, const float* const* features, int d, float potts_w, int ntype,
+                            int on_device);
+/* addPairwiseGaussian / addPairwiseBilateral for every image (images: host RGB per image) */
+int dcrf_batch_add_pairwise_gaussian(dcrf_batch_t b, float sx, float sy, float potts_w);
+int dcrf_batch_add_pairwise_bilateral(dcrf_batch_t b, float sx, float sy, float sr, float sg, float sb,
+                                      const unsigned char* const* images, float potts_w);
+/* unary_init of every image into the batch field Q */
+int dcrf_batch_unary_init(dcrf_batch_t b, float* Q, int on_device);
+/* QP of every image (init null: unary_init); iterations[b] (may be null) */
+int dcrf_batch_qp(dcrf_batch_t b, const float* init, float* out, int max_iter, int ref_stop, int on_device,
+                  int* iterations);
+/* LP of every image; sweeps[b * cap + j] = qp-gamma sweeps of inner step j (may be null) */
+int dcrf_batch_lp(dcrf_batch_t b, const float* init, float* out, const dcrf_lp_params* params, int on_device,
+                  int* sweeps, int cap);
+/* assignment_energy_true of every image's labelling (labels: batch pixel order) */
+int dcrf_batch_assignment_energy_true(dcrf_batch_t b, const int* labels, int on_device, double* energies);
 
 /* ------------------------------------------------------------------------ */
 /* Device memory and kernel timing.  No counterpart in the reference: they  */

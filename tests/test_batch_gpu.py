@@ -1,0 +1,122 @@
+"""Batched solving (many images of uneven sizes in one union lattice per
+pairwise term, per-image solver control on the device, HIP-graph replay)
+against the one-image GPU path and the FP64 oracle.
+
+Tolerances: QP fields within 1e-5 of the one-image path and energies within
+1e-4 (relative) of the FP64 oracle; LP labellings agree on >= 99.9% of the
+pixels with the one-image path and their energies within 1e-4; graph replay
+is bit-identical to eager execution.
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# uneven sizes, two with N % 4 != 0 (ghost lanes per image)
+SIZES = [(31, 23), (48, 40), (17, 13), (64, 36)]
+
+
+def _workloads(L):
+    from densecrf_ho_amd import synthetic as S
+    return [S.Workload(W, H, max(2, W * H // 300), 50 + i, L=L) for i, (W, H) in enumerate(SIZES)]
+
+
+def _batch(wls, L, graphs=True):
+    import densecrf_ho_amd as D
+    b = D.DenseCRFBatch(L, SIZES)
+    b.set_graphs(graphs)
+    for i, wl in enumerate(wls):
+        b.setUnaryEnergy(i, wl.U)
+    b.addPairwiseGaussian(3, 3, D.PottsCompatibility(3))
+    b.addPairwiseBilateral(50, 50, 15, 15, 15, [wl.img for wl in wls], D.PottsCompatibility(5))
+    return b
+
+
+def _single(wl, L):
+    import densecrf_ho_amd as D
+    c = D.DenseCRF2D(wl.W, wl.H, L)
+    c.setUnaryEnergy(wl.U)
+    c.addPairwiseGaussian(3, 3, D.PottsCompatibility(3))
+    c.addPairwiseBilateral(50, 50, 15, 15, 15, wl.img, D.PottsCompatibility(5))
+    return c
+
+
+@pytest.mark.parametrize("L", [5, 21])
+def test_batch_qp_matches_one_image_path_and_oracle(L):
+    wls = _workloads(L)
+    b = _batch(wls, L)
+    Q, it = b.qp(max_iter=10, ref_stop=False)
+    for i, wl in enumerate(wls):
+        print("image", i, wl.W, wl.H, flush=True)
+        c = _single(wl, L)
+        q1 = c.qp_inference_super_pixels_non_convex(c.unary_init(), max_iter=10, ref_stop=False)
+        assert np.abs(b.image(Q, i) - q1).max() < 1e-5
+        o = O.CRF(wl.N, L)
+        o.set_unary(wl.U)
+        for f, w in wl.kernels:
+            o.add_pairwise(f, w)
+        qo, _ = o.qp(O.unary_init(wl.U), max_iter=10, fp64=True)
+        assert np.abs(b.image(Q, i) - qo).max() < 1e-3
+        e = o.energy_true(b.image(Q, i).argmax(1).astype(np.int32), fp64=True)
+        eo = o.energy_true(qo.argmax(1).astype(np.int32), fp64=True)
+        assert abs(e - eo) <= 1e-4 * abs(eo)
+
+
+def test_batch_qp_reference_stopping_matches_one_image_path():
+    L = 5
+    wls = _workloads(L)
+    b = _batch(wls, L)
+    Q, it = b.qp(max_iter=0, ref_stop=True)
+    for i, wl in enumerate(wls):
+        c = _single(wl, L)
+        q1, st = c.qp_inference_super_pixels_non_convex(c.unary_init(), stats=True)
+        assert it[i] == st["iters"]
+        assert np.abs(b.image(Q, i) - q1).max() < 1e-5
+
+
+def test_batch_graph_replay_equals_eager():
+    """every replayed iteration (QP and LP pieces) equals eager execution;
+    two solves in a row reuse the captured graphs"""
+    L = 21
+    wls = _workloads(L)
+    be = _batch(wls, L, graphs=False)
+    bg = _batch(wls, L, graphs=True)
+    from densecrf_ho_amd import LP_inf_params
+    P = LP_inf_params(prox_max_iter=2, fw_max_iter=4)
+    for rep in range(2):
+        print("solve", rep, flush=True)
+        Qe, ite = be.qp(max_iter=6, ref_stop=False)
+        Qg, itg = bg.qp(max_iter=6, ref_stop=False)
+        np.testing.assert_array_equal(Qe, Qg)
+        np.testing.assert_array_equal(ite, itg)
+        Le, swe = be.lp(Qe, P)
+        Lg, swg = bg.lp(Qg, P)
+        np.testing.assert_array_equal(swe, swg)
+        np.testing.assert_array_equal(Le, Lg)
+
+
+def test_batch_lp_matches_one_image_path():
+    import densecrf_ho_amd as D
+    L = 21
+    wls = _workloads(L)
+    b = _batch(wls, L)
+    P = D.LP_inf_params(prox_max_iter=3, fw_max_iter=5)
+    Q, _ = b.qp(max_iter=10, ref_stop=False)
+    Ql, sw = b.lp(Q, P)
+    lab_b = Ql.argmax(1).astype(np.int32)
+    eb = b.assignment_energy_true(lab_b)
+    for i, wl in enumerate(wls):
+        print("image", i, flush=True)
+        c = _single(wl, L)
+        q0 = np.ascontiguousarray(b.image(Q, i))
+        q1, st = c.lp_inference_prox_super_pixels(q0, P, stats=True)
+        np.testing.assert_array_equal(sw[i][: len(st["qp_sweeps"])], st["qp_sweeps"])
+        l1 = q1.argmax(1).astype(np.int32)
+        lb = b.image(lab_b, i)
+        assert (lb == l1).mean() >= 0.999
+        e1 = c.assignment_energy_true(l1)
+        assert abs(eb[i] - c.assignment_energy_true(np.ascontiguousarray(lb))) <= 1e-9 * abs(eb[i]) + 1e-9
+        assert abs(eb[i] - e1) <= 1e-4 * abs(e1)
