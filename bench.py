@@ -106,6 +106,147 @@ def cpu_baseline():
                        f"at {W}x{H} (LP scaled by pixel count)")}
 
 
+def adjoint_bench(D, S, steps=3):
+    """Unrolled mean field (5 updates) and its adjoint on 8 images of 512x512,
+    21 labels: spatial Potts (w=3) and bilateral Matrix (m = -5 I) terms,
+    cross-entropy against seeded random labels.  Inputs resident in HBM."""
+    B, Wd, T = 8, 512, 5
+    K = round(Wd * Wd / 4000)
+    items = []
+    for i in range(B):
+        wl = S.Workload(Wd, Wd, K, 4000 + i, L=L)
+        c = D.DenseCRF2D(Wd, Wd, L)
+        c.setUnaryEnergy(D.DeviceArray.from_numpy(wl.U))
+        c.addPairwiseEnergy(D.DeviceArray.from_numpy(wl.kernels[0][0]), D.PottsCompatibility(3.0))
+        c.addPairwiseEnergy(D.DeviceArray.from_numpy(wl.kernels[1][0]), D.MatrixCompatibility(-5.0 * np.eye(L)))
+        lab = np.random.default_rng(4100 + i).integers(0, L, wl.N).astype(np.int32)
+        items.append((c, D.DeviceArray.from_numpy(lab), D.DeviceArray((wl.N, L)), D.DeviceArray((wl.N, L)),
+                      D.DeviceArray((wl.N, L))))
+
+    def fwd():
+        for c, lab, Q, dQ, dU in items:
+            c.meanfield(T, out=Q)
+        D.synchronize()
+
+    def fwd_bwd():
+        loss = 0.0
+        for c, lab, Q, dQ, dU in items:
+            c.meanfield(T, out=Q)
+            l, _ = c.cross_entropy(Q, lab, out=dQ)
+            c.meanfield_backward(dQ, out=dU)
+            loss += l
+        D.synchronize()
+        return loss
+
+    fwd()
+    fwd_bwd()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fwd()
+    tf = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fwd_bwd()
+    tb = (time.perf_counter() - t0) / steps
+    return {"workload": f"mean field {T} it + adjoint, {B} x {Wd}x{Wd}, {L} labels, Potts + Matrix terms, "
+                        "cross-entropy loss", "forward_ms": round(tf * 1e3, 3),
+            "forward_adjoint_ms": round(tb * 1e3, 3), "ratio": round(tb / tf, 3)}
+
+
+BATCH_IMAGES = 64
+
+
+def _batch_image(i):
+    """image i of the batched workload: seed 3000+i, H ~ U[240,1080],
+    W ~ U[320,1920], one Voronoi cell per 4000 pixels"""
+    from densecrf_ho_amd import synthetic as S
+    rng = np.random.default_rng(3000 + i)
+    H = int(rng.integers(240, 1081))
+    W = int(rng.integers(320, 1921))
+    wl = S.Workload(W, H, max(1, round(W * H / 4000)), 3000 + i, L=L)
+    return W, H, wl.img, wl.U
+
+
+def batch_workload(n):
+    """the batch's images, generated on host cores in worker processes"""
+    import multiprocessing as mp
+    workers = max(1, min(16, os.cpu_count() or 1))
+    with mp.get_context("spawn").Pool(workers) as pool:
+        return pool.map(_batch_image, range(n))
+
+
+def _progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def batch_bench(D, imgs):
+    """QP (10 it) + LP (5x10) of every image: the batched engine (one union
+    lattice per kernel, graph replay) against the one-image path on the same
+    images.  Inputs resident in HBM; lattice builds untimed."""
+    sizes = [(W, H) for W, H, _, _ in imgs]
+    npix = sum(W * H for W, H in sizes)
+    b = D.DenseCRFBatch(L, sizes)
+    for i, (_, _, _, U) in enumerate(imgs):
+        b.setUnaryEnergy(i, D.DeviceArray.from_numpy(U))
+    b.addPairwiseGaussian(3, 3, D.PottsCompatibility(3))
+    b.addPairwiseBilateral(50, 50, 15, 15, 15, [im for _, _, im, _ in imgs], D.PottsCompatibility(5))
+    Qqp = D.DeviceArray((b.N, L))
+    Qlp = D.DeviceArray((b.N, L))
+    P = D.LP_inf_params(prox_max_iter=LP_PROX, fw_max_iter=LP_FW)
+
+    def solve():
+        b.qp(None, max_iter=QP_ITERS, ref_stop=False, out=Qqp)
+        b.lp(Qqp, P, out=Qlp)
+        D.synchronize()
+
+    _progress(f"batch of {len(imgs)} images, {npix} pixels built")
+    solve()  # first pass runs eagerly, the second captures the graphs
+    solve()
+    _progress("batch warm")
+    t0 = time.perf_counter()
+    solve()
+    t_batch = time.perf_counter() - t0
+    # kernel times of one batched solve (eager: graphs are not replayed while profiling)
+    D.profile_enable(True)
+    solve()
+    prof = D.profile_report()
+    D.profile_enable(False)
+    lab = np.ascontiguousarray(Qlp.numpy().argmax(1).astype(np.int32))
+    e_batch = b.assignment_energy_true(lab)
+    b.close()
+    del Qqp, Qlp
+    _progress(f"batched solve {t_batch:.3f} s")
+    # the one-image path, image after image
+    t_single = 0.0
+    e_single = []
+    for (W, H, im, U) in imgs:
+        c = D.DenseCRF2D(W, H, L)
+        c.setUnaryEnergy(D.DeviceArray.from_numpy(U))
+        c.addPairwiseGaussian(3, 3, D.PottsCompatibility(3))
+        c.addPairwiseBilateral(50, 50, 15, 15, 15, im, D.PottsCompatibility(5))
+        q0 = D.DeviceArray.from_numpy(c.unary_init())
+        q1 = D.DeviceArray((W * H, L))
+        q2 = D.DeviceArray((W * H, L))
+        D.synchronize()
+        t0 = time.perf_counter()
+        c.qp_inference_super_pixels_non_convex(q0, max_iter=QP_ITERS, ref_stop=False, out=q1)
+        c.lp_inference_prox_super_pixels(q1, P, out=q2)
+        D.synchronize()
+        t_single += time.perf_counter() - t0
+        e_single.append(c.assignment_energy_true(np.ascontiguousarray(q2.numpy().argmax(1).astype(np.int32))))
+        c.close()
+    _progress(f"one-image path {t_single:.3f} s")
+    e_single = np.array(e_single)
+    rel = np.abs(e_batch - e_single) / np.abs(e_single)
+    return {"workload": f"{len(imgs)} images, H ~ U[240,1080], W ~ U[320,1920] (seeds 3000+i), {L} labels, "
+                        f"QP {QP_ITERS} it + LP {LP_PROX}x{LP_FW}", "pixels": int(npix),
+            "batched_s": round(t_batch, 4), "images_per_s": round(len(imgs) / t_batch, 3),
+            "one_image_path_s": round(t_single, 4), "one_image_path_images_per_s": round(len(imgs) / t_single, 3),
+            "speedup": round(t_single / t_batch, 3),
+            "energy_rel_diff_max": float(rel.max()),
+            "kernels": {n: {"launches": c_, "total_ms": round(m, 3)} for n, (c_, m) in sorted(prof.items())}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,11 +254,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--qp-only", action="store_true", help="time the QP alone (profiling runs)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
+    ap.add_argument("--batch-images", type=int, default=BATCH_IMAGES)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(images would shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    # host-side generation of the batch first: worker processes start before
+    # this process touches the GPU
+    imgs = None if (args.no_batch or args.qp_only) else batch_workload(args.batch_images)
+    if imgs is not None:
+        _progress(f"generated {len(imgs)} batch images")
 
     import densecrf_ho_amd as D
     from densecrf_ho_amd import synthetic as S
@@ -220,6 +369,12 @@ def main():
                      "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": int(ab)},
         "kernels": {n: {"launches": c, "total_ms": round(m, 3)} for n, (c, m) in sorted(prof.items())},
     }
+    if not args.qp_only:
+        out["adjoint"] = adjoint_bench(D, S)
+    if imgs is not None:
+        del crf
+        out["batch"] = batch_bench(D, imgs)
+        out["images_per_s"] = out["batch"]["images_per_s"]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

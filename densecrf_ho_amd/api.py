@@ -544,6 +544,11 @@ class DenseCRFBatch:
     def set_graphs(self, on: bool):
         _check(_lib.load().dcrf_batch_set_graphs(self._h, int(on)))
 
+    def graph_captures(self) -> int:
+        n = C.c_longlong()
+        _check(_lib.load().dcrf_batch_graph_captures(self._h, C.byref(n)))
+        return n.value
+
     def setUnaryEnergy(self, b: int, unary):
         a = _Arg(unary, np.float32)
         _check(_lib.load().dcrf_batch_set_unary(self._h, int(b), a.ptr, a.dev))

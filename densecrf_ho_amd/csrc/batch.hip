@@ -162,10 +162,15 @@ void GraphCache::clear() {
   exec.clear();
   seen.clear();
 }
-void GraphCache::run(long key, hipStream_t s, bool use, const std::function<void()>& body) {
+void GraphCache::run(long key, hipStream_t s, bool use, const std::vector<const void*>& buffers,
+                     const std::function<void()>& body) {
   if (!use || prof_on()) {
     body();
     return;
+  }
+  if (buffers != captured_with) {  // some buffer moved: no graph may be replayed
+    clear();
+    captured_with = buffers;
   }
   auto it = exec.find(key);
   if (it == exec.end()) {
@@ -182,6 +187,7 @@ void GraphCache::run(long key, hipStream_t s, bool use, const std::function<void
     const hipError_t rc = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     DCRF_HIP(rc);
+    ++captures;
     it = exec.emplace(key, e).first;
   }
   DCRF_HIP(hipGraphLaunch(it->second, s));
@@ -251,6 +257,23 @@ void Batch::add_pairwise(const std::vector<const float*>& feats, int d, float w,
 void Batch::unary_init(float* Q) {
   k_softmax_neg<<<grid_for(N_), kBlock, 0, s_>>>(U_.get(), Q, (int)N_, L_);
   check_launch("k_softmax_neg");
+}
+
+std::vector<const void*> Batch::work_buffers() const {
+  std::vector<const void*> v;
+  for (auto& p : pw_) {
+    p->lat->work_buffers(v);
+    v.push_back(p->ctmp.get());
+  }
+  for (const void* p : {(const void*)Q_.get(), (const void*)gy_.get(), (const void*)psx_.get(),
+                        (const void*)tmp_.get(), (const void*)rpix_.get(), (const void*)lab_.get(),
+                        (const void*)parts_.get(), (const void*)sc_.get(), (const void*)fl_.get(),
+                        (const void*)alpha_.get(), (const void*)sa_.get(), (const void*)gamma_.get(),
+                        (const void*)gamma2_.get(), (const void*)Y_.get(), (const void*)lQ_.get(),
+                        (const void*)curQ_.get(), (const void*)best_.get(), (const void*)mmparts_.get(),
+                        (const void*)mm_.get(), (const void*)lsc_.get(), (const void*)lfl_.get()})
+    v.push_back(p);
+  return v;
 }
 
 ImageBlocks Batch::red_map() const { return red_.map(pix_.get()); }
@@ -341,7 +364,7 @@ std::vector<int> Batch::qp(const float* init, float* out, int max_iter, bool ref
   std::vector<int> hf((size_t)B_ * kF_COUNT);
   while (issued < cap_iters) {
     const int n = std::min(group, cap_iters - issued);
-    for (int i = 0; i < n; ++i) graphs_.run(kGraphQP, s, use_graph_, iteration);
+    for (int i = 0; i < n; ++i) run_graph(kGraphQP, iteration);
     issued += n;
     DCRF_HIP(hipMemcpyAsync(hf.data(), fl, hf.size() * sizeof(int), hipMemcpyDeviceToHost, s));
     DCRF_HIP(hipStreamSynchronize(s));
@@ -469,7 +492,7 @@ std::vector<std::vector<int>> Batch::lp(const float* init, float* out, const LPP
     k_fill<<<grid_for(NL), kBlock, 0, s>>>(alpha_.get(), NL, 0.f);
     for (int pit = 1; pit <= P.fw_max_iter; ++pit) {
       const int upd = pit > 1 ? 1 : 0;
-      graphs_.run(kGraphY + upd, s, use_graph_, [&]() {
+      run_graph(kGraphY + upd, [&]() {
         KTimer tm("lp_Y", s);
         k_lp_Y<<<nbr, kBlock, 0, s>>>(rm, L, c, alpha_.get(), nullptr, sa_.get(), nullptr, sc, upd, U_.get(),
                                       curQ_.get(), Y_.get());
@@ -483,7 +506,7 @@ std::vector<std::vector<int>> Batch::lp(const float* init, float* out, const LPP
         const int parity = gcur == gamma_.get() ? 0 : 1;
         float* gi = gcur;
         float* go = gnext;
-        graphs_.run(kGraphSweep + 2 * nsw + parity, s, use_graph_, [&]() {
+        run_graph(kGraphSweep + 2 * nsw + parity, [&]() {
           {
             KTimer tm("lp_sweeps", s);
             if (L == 21)
@@ -517,7 +540,7 @@ std::vector<std::vector<int>> Batch::lp(const float* init, float* out, const LPP
       }
       const int parity = gcur == gamma_.get() ? 0 : 1;
       float* gm = gcur;
-      graphs_.run(kGraphPost + parity, s, use_graph_, [&]() {
+      run_graph(kGraphPost + parity, [&]() {
         {
           KTimer tm("lp_betaQ", s);
           k_lp_betaQ<<<nbr, kBlock, 0, s>>>(rm, L, lambda, alpha_.get(), nullptr, gm, U_.get(), curQ_.get(),

@@ -27,12 +27,21 @@ struct BlockPlan {
 // allocated and every table built), is captured the second time, and from
 // then on replays.  Sequences that read a double-buffered array are keyed by
 // which buffer is current.
+//
+// A graph holds the device addresses it was captured with.  `buffers` is the
+// list of addresses of every buffer a captured sequence may touch; when it
+// differs from the list at capture time (a buffer was reallocated, e.g. a
+// lattice's value rows grew for the ordered filter) every graph is dropped
+// and captured again, so no replay can reach freed memory.
 struct GraphCache {
   std::map<long, hipGraphExec_t> exec;
   std::set<long> seen;
+  std::vector<const void*> captured_with;
+  long captures = 0;
   ~GraphCache();
   void clear();
-  void run(long key, hipStream_t s, bool use, const std::function<void()>& body);
+  void run(long key, hipStream_t s, bool use, const std::vector<const void*>& buffers,
+           const std::function<void()>& body);
 };
 
 class Batch {
@@ -48,6 +57,7 @@ class Batch {
     use_graph_ = on;
     graphs_.clear();
   }
+  long graph_captures() const { return graphs_.captures; }
 
   void set_unary(int b, const float* dU);                                          // device, N_b x L
   void add_pairwise(const std::vector<const float*>& feats, int d, float w, int ntype);  // device, per image
@@ -65,6 +75,11 @@ class Batch {
   static constexpr int kSweepBatchB = 16;
   static constexpr long kGraphQP = 1, kGraphY = 2, kGraphPost = 10, kGraphSweep = 100;
   void ensure_state();
+  // addresses of every buffer a captured solver sequence may touch
+  std::vector<const void*> work_buffers() const;
+  void run_graph(long key, const std::function<void()>& body) {
+    graphs_.run(key, s_, use_graph_, work_buffers(), body);
+  }
   ImageBlocks red_map() const;
   ImageBlocks sweep_map() const;
   void int_energy_dev(const int* lab, double* dst, int stride);

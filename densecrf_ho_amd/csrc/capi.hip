@@ -656,6 +656,12 @@ int dcrf_batch_set_graphs(dcrf_batch_t b, int on) {
     b->b->set_graphs(on != 0);
   });
 }
+int dcrf_batch_graph_captures(dcrf_batch_t b, long long* n) {
+  return guarded([&] {
+    DCRF_REQUIRE(b && n, "dcrf_batch_graph_captures: bad arguments");
+    *n = b->b->graph_captures();
+  });
+}
 int dcrf_batch_set_unary(dcrf_batch_t b, int img, const float* unary, int on_device) {
   return guarded([&] {
     DCRF_REQUIRE(b && unary && img >= 0 && img < b->b->B(), "dcrf_batch_set_unary: bad arguments");

@@ -71,6 +71,15 @@ class Lattice {
   // {min, max} pair per part.
   static std::unique_ptr<Lattice> concat(const std::vector<const Lattice*>& parts, hipStream_t s);
 
+  // addresses of the work buffers a filter call writes (value rows, chunk
+  // partials, ordered-filter bins and totals); they change when a call with
+  // wider rows reallocates them, which invalidates captured launch sequences
+  void work_buffers(std::vector<const void*>& out) const {
+    for (const void* p : {(const void*)vA_.get(), (const void*)vB_.get(), (const void*)part_.get(),
+                          (const void*)bins_.get(), (const void*)tb_.get(), (const void*)edge_.get()})
+      out.push_back(p);
+  }
+
   // copies of the lattice tables for parity checks (host pointers, may be null)
   void export_tables(int* offset, int* rank, float* bary, short* keys, int* nbr) const;
 

@@ -246,6 +246,9 @@ int dcrf_batch_destroy(dcrf_batch_t b);
 int dcrf_batch_pixel_offsets(dcrf_batch_t b, long long* offsets);
 /* replay the fixed parts of every solver iteration from HIP graphs (default on) */
 int dcrf_batch_set_graphs(dcrf_batch_t b, int on);
+/* number of HIP graphs captured so far (a graph is captured again whenever a
+ * buffer it refers to was reallocated) */
+int dcrf_batch_graph_captures(dcrf_batch_t b, long long* n);
 /* setUnaryEnergy of image `img` (N_img x L) */
 int dcrf_batch_set_unary(dcrf_batch_t b, int img, const float* unary, int on_device);
 /* addPairwiseEnergy(features[img], PottsCompatibility(w), DIAG_KERNEL, ntype) for every image */

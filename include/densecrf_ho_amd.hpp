@@ -78,20 +78,48 @@ inline void check(int rc) {
 enum NormalizationType { NO_NORMALIZATION, NORMALIZE_BEFORE, NORMALIZE_AFTER, NORMALIZE_SYMMETRIC };
 enum KernelType { CONST_KERNEL, DIAG_KERNEL, FULL_KERNEL };
 
-// LabelCompatibility (include/labelcompatibility.h:36-49); the engine runs
-// the Potts form mu(a,b) = -w [a==b]
+// LabelCompatibility (include/labelcompatibility.h:36-92): Potts
+// (out = -w Q), Diagonal (out = diag(v) Q) and Matrix (out = W Q with
+// W = (m + m^T)/2)
 class LabelCompatibility {
  public:
   virtual ~LabelCompatibility() = default;
-  virtual float potts_weight() const = 0;
+  virtual int type() const = 0;                      // DCRF_COMPAT_*
+  virtual std::vector<float> c_params() const = 0;  // parameters as the C ABI takes them
+  virtual float potts_weight() const { return 1.f; }
 };
 class PottsCompatibility : public LabelCompatibility {
  public:
   explicit PottsCompatibility(float weight = 1.0f) : w_(weight) {}
+  int type() const override { return DCRF_COMPAT_POTTS; }
+  std::vector<float> c_params() const override { return {w_}; }
   float potts_weight() const override { return w_; }
 
  protected:
   float w_;
+};
+class DiagonalCompatibility : public LabelCompatibility {
+ public:
+  explicit DiagonalCompatibility(const std::vector<float>& v) : v_(v) {}
+#ifdef DCRF_USE_EIGEN
+  explicit DiagonalCompatibility(const Eigen::VectorXf& v) : v_(v.data(), v.data() + v.size()) {}
+#endif
+  int type() const override { return DCRF_COMPAT_DIAGONAL; }
+  std::vector<float> c_params() const override { return v_; }
+
+ protected:
+  std::vector<float> v_;
+};
+class MatrixCompatibility : public LabelCompatibility {
+ public:
+  explicit MatrixCompatibility(const MatrixXf& m) : m_(m.data(), m.data() + m.rows() * m.cols()) {
+    if (m.rows() != m.cols()) throw std::runtime_error("MatrixCompatibility: the matrix must be square");
+  }
+  int type() const override { return DCRF_COMPAT_MATRIX; }
+  std::vector<float> c_params() const override { return m_; }  // column-major
+
+ protected:
+  std::vector<float> m_;
 };
 
 // LP_inf_params (include/densecrf.h:43-69), defaults of src/densecrf.cpp:287-302
@@ -246,8 +274,49 @@ class DenseCRF {
                                  KernelType kernel_type = DIAG_KERNEL,
                                  NormalizationType normalization_type = NORMALIZE_SYMMETRIC) {
     std::unique_ptr<LabelCompatibility> f(function);
-    check(dcrf_crf_add_pairwise(h_, features.data(), (int)features.rows(), f->potts_weight(),
-                                (int)normalization_type, 0));
+    if (f->type() == DCRF_COMPAT_POTTS) {
+      check(dcrf_crf_add_pairwise(h_, features.data(), (int)features.rows(), f->potts_weight(),
+                                  (int)normalization_type, 0));
+    } else {
+      const std::vector<float> p = f->c_params();
+      check(dcrf_crf_add_pairwise_compat(h_, features.data(), (int)features.rows(), f->type(), p.data(),
+                                         (int)normalization_type, 0));
+    }
+    ++npairwise_;
+  }
+  // PairwisePotential::parameters / setParameters / gradient(b, Q) of term k
+  // (src/pairwise.cpp:382-395)
+  std::vector<float> pairwise_parameters(int k) const {
+    int n = 0;
+    check(dcrf_crf_pairwise_nparams(h_, k, &n));
+    std::vector<float> v(n);
+    check(dcrf_crf_pairwise_parameters(h_, k, v.data()));
+    return v;
+  }
+  void set_pairwise_parameters(int k, const std::vector<float>& v) {
+    if (v.size() != pairwise_parameters(k).size()) throw std::runtime_error("setParameters: wrong size");
+    check(dcrf_crf_pairwise_set_parameters(h_, k, v.data()));
+  }
+  std::vector<float> pairwise_gradient(int k, const MatrixXf& b, const MatrixXf& Q) const {
+    std::vector<float> g(pairwise_parameters(k).size());
+    check(dcrf_crf_pairwise_gradient(h_, k, b.data(), Q.data(), g.data(), 0));
+    return g;
+  }
+  // unrolled mean field from softmax(-U) and its adjoint: dL/dQ_T -> dL/dU
+  // and the parameter gradient of every term (back to back, FP64)
+  MatrixXf meanfield(int iterations) {
+    MatrixXf Q(M_, N_);
+    check(dcrf_crf_meanfield(h_, iterations, Q.data(), 0));
+    return Q;
+  }
+  MatrixXf meanfield_backward(const MatrixXf& dQ, std::vector<double>* dparams = nullptr) {
+    MatrixXf dU(M_, N_);
+    size_t n = 0;
+    for (int k = 0; k < npairwise_; ++k) n += pairwise_parameters(k).size();
+    std::vector<double> g(n ? n : 1);
+    check(dcrf_crf_meanfield_backward(h_, dQ.data(), dU.data(), g.data(), 0));
+    if (dparams) dparams->assign(g.begin(), g.begin() + n);
+    return dU;
   }
   void setUnaryEnergy(const MatrixXf& unary) { check(dcrf_crf_set_unary(h_, unary.data(), 0)); }
   // superpixel layer from a region map (the state addSuperPixel derives from
@@ -320,6 +389,7 @@ class DenseCRF {
     return lab;
   }
   int N_, M_;
+  int npairwise_ = 0;
   dcrf_crf_t h_ = nullptr;
 };
 
@@ -331,8 +401,9 @@ class DenseCRF2D : public DenseCRF {
                            KernelType kernel_type = DIAG_KERNEL,
                            NormalizationType normalization_type = NORMALIZE_SYMMETRIC) {
     std::unique_ptr<LabelCompatibility> f(function ? function : new PottsCompatibility(1.f));
-    if (normalization_type == NORMALIZE_SYMMETRIC) {
+    if (normalization_type == NORMALIZE_SYMMETRIC && f->type() == DCRF_COMPAT_POTTS) {
       check(dcrf_crf_add_pairwise_gaussian(h_, W_, H_, sx, sy, f->potts_weight()));
+      ++npairwise_;
       return;
     }
     MatrixXf feature(2, N_);
@@ -347,8 +418,9 @@ class DenseCRF2D : public DenseCRF {
                             LabelCompatibility* function = nullptr, KernelType kernel_type = DIAG_KERNEL,
                             NormalizationType normalization_type = NORMALIZE_SYMMETRIC) {
     std::unique_ptr<LabelCompatibility> f(function ? function : new PottsCompatibility(1.f));
-    if (normalization_type == NORMALIZE_SYMMETRIC) {
+    if (normalization_type == NORMALIZE_SYMMETRIC && f->type() == DCRF_COMPAT_POTTS) {
       check(dcrf_crf_add_pairwise_bilateral(h_, W_, H_, sx, sy, sr, sg, sb, im, f->potts_weight()));
+      ++npairwise_;
       return;
     }
     MatrixXf feature(5, N_);
@@ -365,6 +437,48 @@ class DenseCRF2D : public DenseCRF {
 
  protected:
   int W_, H_;
+};
+
+// Many images of uneven sizes solved together (no reference counterpart;
+// each image follows the per-image algorithm).  Fields are the images'
+// fields concatenated: image b owns columns [offsets()[b], offsets()[b+1]).
+class DenseCRFBatch {
+ public:
+  DenseCRFBatch(int M, const std::vector<int>& W, const std::vector<int>& H) : M_(M), B_((int)W.size()) {
+    if (W.size() != H.size()) throw std::runtime_error("DenseCRFBatch: W and H differ in length");
+    check(dcrf_batch_create_2d(M, B_, W.data(), H.data(), &h_));
+    off_.resize(B_ + 1);
+    check(dcrf_batch_pixel_offsets(h_, off_.data()));
+  }
+  ~DenseCRFBatch() {
+    if (h_) dcrf_batch_destroy(h_);
+  }
+  DenseCRFBatch(const DenseCRFBatch&) = delete;
+  DenseCRFBatch& operator=(const DenseCRFBatch&) = delete;
+  const std::vector<long long>& offsets() const { return off_; }
+  long long N() const { return off_.back(); }
+  void setUnaryEnergy(int b, const MatrixXf& unary) { check(dcrf_batch_set_unary(h_, b, unary.data(), 0)); }
+  void addPairwiseGaussian(float sx, float sy, float w) { check(dcrf_batch_add_pairwise_gaussian(h_, sx, sy, w)); }
+  void addPairwiseBilateral(float sx, float sy, float sr, float sg, float sb,
+                            const std::vector<const unsigned char*>& images, float w) {
+    check(dcrf_batch_add_pairwise_bilateral(h_, sx, sy, sr, sg, sb, images.data(), w));
+  }
+  MatrixXf qp(int max_iter) {
+    MatrixXf Q(M_, (long)N());
+    check(dcrf_batch_qp(h_, nullptr, Q.data(), max_iter, max_iter > 0 ? 0 : 1, 0, nullptr));
+    return Q;
+  }
+  MatrixXf lp(const MatrixXf& init, const LP_inf_params& params) {
+    MatrixXf Q(M_, (long)N());
+    dcrf_lp_params p = params.to_c();
+    check(dcrf_batch_lp(h_, init.data(), Q.data(), &p, 0, nullptr, 0));
+    return Q;
+  }
+
+ private:
+  int M_, B_;
+  std::vector<long long> off_;
+  dcrf_batch_t h_ = nullptr;
 };
 
 }  // namespace dcrf_ho

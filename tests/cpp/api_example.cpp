@@ -4,6 +4,8 @@
 // from the unaries, run the QP and then the LP, print the energies.
 //
 //   api_example W H L unary.f32 image.u8 out_qp.f32 out_lp.f32
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -68,5 +70,36 @@ int main(int argc, char** argv) {
   } catch (const std::runtime_error& e) {
     std::printf("exception %s\n", e.what());
   }
+
+  // unrolled mean field with a Matrix compatibility term and its adjoint
+  DenseCRF2D mf(W, H, L);
+  mf.setUnaryEnergy(unary);
+  mf.addPairwiseGaussian(3, 3, new PottsCompatibility(3));
+  MatrixXf m(L, L);
+  for (int j = 0; j < L; ++j)
+    for (int i = 0; i < L; ++i) m(i, j) = i == j ? -5.f : 0.1f * (float)((i + 2 * j) % 5);
+  mf.addPairwiseBilateral(50, 50, 15, 15, 15, im.data(), new MatrixCompatibility(m));
+  MatrixXf Qmf = mf.meanfield(3);
+  MatrixXf dQ(L, N);
+  for (int i = 0; i < N; ++i)
+    for (int l = 0; l < L; ++l) dQ(l, i) = l == i % L ? -1.f / Qmf(l, i) : 0.f;
+  std::vector<double> dparams;
+  MatrixXf dU = mf.meanfield_backward(dQ, &dparams);
+  std::printf("meanfield_params %d\n", (int)dparams.size());
+  std::printf("meanfield_dU00 %.9g\n", dU(0, 0));
+
+  // a batch of two images (the same image twice) against the one-image QP
+  DenseCRFBatch batch(L, {W, W}, {H, H});
+  batch.setUnaryEnergy(0, unary);
+  batch.setUnaryEnergy(1, unary);
+  batch.addPairwiseGaussian(3, 3, 3);
+  batch.addPairwiseBilateral(50, 50, 15, 15, 15, {im.data(), im.data()}, 5);
+  MatrixXf Qb = batch.qp(5);
+  MatrixXf Q5 = crf.qp_inference_super_pixels_non_convex(Q, 5);
+  double diff = 0;
+  for (int b = 0; b < 2; ++b)
+    for (int i = 0; i < N; ++i)
+      for (int l = 0; l < L; ++l) diff = std::max(diff, (double)std::abs(Qb(l, b * N + i) - Q5(l, i)));
+  std::printf("batch_qp_diff %.9g\n", diff);
   return 0;
 }

@@ -98,6 +98,23 @@ def test_batch_graph_replay_equals_eager():
         np.testing.assert_array_equal(Le, Lg)
 
 
+def test_batch_graphs_recaptured_after_buffers_grow():
+    """the LP's ordered filter widens the lattices' value rows (the buffers
+    move); a QP graph captured before must not be replayed afterwards"""
+    import densecrf_ho_amd as D
+    L = 5
+    wls = _workloads(L)
+    b = _batch(wls, L, graphs=True)
+    Q1, _ = b.qp(max_iter=4, ref_stop=False)
+    c1 = b.graph_captures()
+    assert c1 >= 1
+    b.lp(Q1, D.LP_inf_params(prox_max_iter=1, fw_max_iter=2))
+    c2 = b.graph_captures()
+    Q2, _ = b.qp(max_iter=4, ref_stop=False)
+    assert b.graph_captures() > c2  # captured again with the new buffers
+    np.testing.assert_array_equal(Q1, Q2)
+
+
 def test_batch_lp_matches_one_image_path():
     import densecrf_ho_amd as D
     L = 21

@@ -42,3 +42,8 @@ def test_cpp_example_matches_oracle(tmp_path):
     Qlp = np.fromfile(tmp_path / "lp.f32", np.float32).reshape(wl.N, L)
     Ql, _ = orc.lp(Qo, O.LPParams(prox_max_iter=2))
     assert (Qlp.argmax(1) == Ql.argmax(1)).mean() >= 0.999
+    # adjoint through the C++ layer: Potts (1) + Matrix (L(L+1)/2) parameters
+    assert int(vals["meanfield_params"]) == 1 + L * (L + 1) // 2
+    assert np.isfinite(float(vals["meanfield_dU00"]))
+    # a batch of the same image twice equals the one-image QP
+    assert float(vals["batch_qp_diff"]) < 1e-5
