@@ -434,11 +434,26 @@ __global__ void __launch_bounds__(kBlock) k_slice(int V, RowMap rm, int N, int d
   }
 }
 
-// k_combine that does nothing unless *need (the upper pass without edges)
+// Bin-edge flags of the ordered filter, one per part of the lattice (one
+// image, or each image of a union) plus flag[parts] = "any part".  A part
+// whose flag is 0 takes the upper pass from T - LO[c+1] and skips its own.
+struct EdgeMap {
+  const int* flag;
+  const uint16_t* pimg;   // part of each pixel (null: one part)
+  const uint16_t* ppimg;  // part of each lattice point (null: one part)
+  int parts;
+  __device__ inline bool any() const { return flag[parts] != 0; }
+  __device__ inline bool pixel(long pix) const { return flag[pimg ? pimg[pix] : 0] != 0; }
+  __device__ inline bool point(long p) const { return flag[ppimg ? ppimg[p] : 0] != 0; }
+};
+
+// k_combine of the lower pass, or of the upper pass (upper != 0) for the
+// points of parts with bin edges only
 __global__ void __launch_bounds__(kBlock) k_combine_if(int V, RowMap rm, const int* __restrict__ multi, int nmulti,
                              const int* __restrict__ chstart, const int* __restrict__ pstart,
-                             const float* __restrict__ part, float* __restrict__ values, const int* __restrict__ need) {
-  if (need && *need == 0) return;
+                             const float* __restrict__ part, float* __restrict__ values, EdgeMap edge,
+                             int upper) {
+  if (upper && !edge.any()) return;
   const long nvb = (long)((nmulti + rm.gpw - 1) / rm.gpw * 64 + blockDim.x - 1) / blockDim.x;
   for (long vb = xcd_block(); vb < nvb; vb += gridDim.x) {
     long m;
@@ -446,6 +461,7 @@ __global__ void __launch_bounds__(kBlock) k_combine_if(int V, RowMap rm, const i
     rm.at_block(V, vb, m, k0, ks);
     if (m < 0 || m >= nmulti) continue;
     const int p = multi[m];
+    if (upper && !edge.point(p)) continue;
     const int c0 = pstart[p], c1 = c0 + chstart[p + 1] - chstart[p];
     for (int k = k0; k < V; k += ks) {
       float acc = part[(size_t)c0 * V + k];
@@ -563,16 +579,16 @@ __global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ b
                          const int* __restrict__ sp_pix, const float* __restrict__ sp_w, const int* __restrict__ seg,
                          const int* __restrict__ chstart, const int* __restrict__ chpoint,
                          const int* __restrict__ pstart, int nchunks, float* __restrict__ values,
-                         float* __restrict__ part, int* __restrict__ edge) {
+                         float* __restrict__ part, EdgeMap edge) {
   __shared__ int spx[kBlock];
   __shared__ float swt[kBlock];
   __shared__ int maxlen;
-  if (upper && *edge == 0) return;
+  if (upper && !edge.any()) return;
   for (long vb = xcd_block(); vb < nvblocks; vb += gridDim.x) {
   long c;
   int k0, ks;
   rm.at_block(L, vb, c, k0, ks);
-  const bool act = c >= 0 && c < nchunks;
+  const bool act = c >= 0 && c < nchunks && (!upper || edge.point(chpoint[c]));
   const int gb = (threadIdx.x & ~63) + ((threadIdx.x & 63) / L) * L;
   int b = 0, e = 0, p = 0, c0 = 0, nc = 1;
   if (act) {
@@ -628,14 +644,16 @@ __global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ b
 }
 
 // one blur axis over rows of R2 float2 (ordered state): one wave per lattice
-// point, lanes stride the row; need == 0 makes it a no-op
+// point, lanes stride the row; the upper pass (upper != 0) only blurs the
+// points of parts with bin edges (a point's neighbours are in its own part)
 template <int TAG>
 __global__ void __launch_bounds__(kBlock) k_blur2(int R2, int M, const int* __restrict__ nbr, const float2* __restrict__ src,
-                        float2* __restrict__ dst, const int* __restrict__ need) {
-  if (need && *need == 0) return;
+                        float2* __restrict__ dst, EdgeMap edge, int upper) {
+  if (upper && !edge.any()) return;
   const int lane = threadIdx.x & 63;
   const long nw = (long)gridDim.x * (blockDim.x >> 6);
   for (long i = ((long)xcd_block() * blockDim.x + threadIdx.x) >> 6; i < M; i += nw) {
+    if (upper && !edge.point(i)) continue;
     const int n1 = nbr[2 * i] + 1, n2 = nbr[2 * i + 1] + 1;
     const float2* so = src + (size_t)(i + 1) * R2;
     const float2* s1 = src + (size_t)n1 * R2;
@@ -657,15 +675,19 @@ __global__ void __launch_bounds__(kBlock) k_blur2(int R2, int M, const int* __re
 // image with its own {min, max}: pimg[pixel] picks the pair mm[2*img].
 __global__ void __launch_bounds__(kBlock) k_obin(const float* __restrict__ Q, const float* __restrict__ mm, size_t n,
                        uint8_t* __restrict__ bins, int* __restrict__ edge, const uint16_t* __restrict__ pimg,
-                       FastDiv divL) {
+                       FastDiv divL, int parts) {
   bool e = false;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const float q = rescaled(Q, i, pimg ? mm + 2 * pimg[divL.div((uint32_t)i)] : mm);
+    const int img = pimg ? pimg[divL.div((uint32_t)i)] : 0;
+    const float q = rescaled(Q, i, mm + 2 * img);
     const int lo = bin_lo(q), up = bin_up(q);
-    e = e || lo != up;
+    if (lo != up) {
+      e = true;
+      if (edge[img] == 0) atomicOr(edge + img, 1);
+    }
     bins[i] = (uint8_t)(lo | (up << 4));
   }
-  if (__any(e) && (threadIdx.x & 63) == 0) atomicOr(edge, 1);
+  if (__any(e) && (threadIdx.x & 63) == 0) atomicOr(edge + parts, 1);
 }
 
 // lower slice then upper slice (:638-650, :686-696), then out (+)= scale*x.
@@ -674,7 +696,7 @@ template <int TAG>
 __global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ bins, int L, RowMap rm, int N, int d1,
                          const int* __restrict__ offset, const float* __restrict__ bary,
                          const float* __restrict__ lo_v, const float* __restrict__ up_v, const float* __restrict__ Tb,
-                         const int* __restrict__ edge, float alpha, SliceArgs o) {
+                         EdgeMap edge, float alpha, SliceArgs o) {
   __shared__ int soff[kBlock];
   __shared__ float sbw[kBlock];
   long pix;
@@ -691,7 +713,7 @@ __global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ b
   if (!act) return;
   auto voff = [&](int j) { return staged ? soff[gb + j] : offset[(size_t)pix * d1 + j] + 1; };
   auto vw = [&](int j) { return staged ? sbw[gb + j] : bary[(size_t)pix * d1 + j]; };
-  const bool full = *edge != 0;
+  const bool full = edge.pixel(pix);
   const int bb = bins[(size_t)pix * L + k0];
   const int lo = bb & 15, up = bb >> 4;
   // one pass over the vertices; the upper terms wait in registers so the sum
@@ -721,6 +743,9 @@ __global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ b
 }
 
 __global__ void k_zero_int(int* p) { *p = 0; }
+__global__ void k_zero_ints(int* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
 
 __global__ void __launch_bounds__(kBlock) k_zero(float* p, size_t n) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -987,7 +1012,7 @@ void Lattice::ensure_total_weight() {
   }
   tb_.resize(M_ + 1);
   DCRF_HIP(hipMemcpyAsync(tb_.get(), src, (M_ + 1) * sizeof(float), hipMemcpyDeviceToDevice, s));
-  edge_.resize(1);
+  edge_.resize(nparts_ + 1);
   DCRF_HIP(hipStreamSynchronize(s));
   check_launch("total weight");
   have_tb_ = true;
@@ -1010,14 +1035,16 @@ void Lattice::ordered(const float* Q, const float* d_minmax, int L, const SliceA
   k_zero<<<grid_for(row), kBlock, 0, s>>>(vA_.get() + half, row);
   k_zero<<<grid_for(row), kBlock, 0, s>>>(vB_.get() + half, row);
   int* edge = edge_.get();
+  const EdgeMap em{edge, pimg_.n ? pimg_.get() : nullptr, ppimg_.n ? ppimg_.get() : nullptr, nparts_};
   const std::string tag = prof_on() ? "[d=" + std::to_string(d_) + ",L=" + std::to_string(L) + "]" : std::string();
   const size_t NL = (size_t)N_ * L;
   bins_.resize(NL);
   {
     KTimer tm("obin" + tag, s);
-    k_zero_int<<<1, 1, 0, s>>>(edge);
+    k_zero_ints<<<1, kBlock, 0, s>>>(edge, nparts_ + 1);
     k_obin<<<grid_for(NL, kBlock, 2048), kBlock, 0, s>>>(Q, d_minmax, NL, bins_.get(), edge,
-                                                         pimg_.n ? pimg_.get() : nullptr, FastDiv((uint32_t)L));
+                                                         pimg_.n ? pimg_.get() : nullptr, FastDiv((uint32_t)L),
+                                                         nparts_);
   }
   {
     KTimer tm("osplat" + tag, s);
@@ -1028,7 +1055,7 @@ void Lattice::ordered(const float* Q, const float* d_minmax, int L, const SliceA
       auto launch = [&](auto kern) {
         kern<<<grid, kBlock, 0, s>>>(bins_.get(), L, rm, nvb, upper, sp_pix_.get(), sp_w_.get(), seg_.get(),
                                      chstart_.get(), chpoint_.get(), pstart_.get(), nchunks_, dst_half, part_.get(),
-                                     edge);
+                                     em);
       };
       if (d_ == 2)
         upper ? launch(k_osplat<102>) : launch(k_osplat<2>);
@@ -1040,35 +1067,38 @@ void Lattice::ordered(const float* Q, const float* d_minmax, int L, const SliceA
         const RowMap rw = rowmap(row);
         const unsigned g = upper ? std::min(rows_grid(nmulti_, row), 1024u) : rows_grid(nmulti_, row);
         k_combine_if<<<g, kBlock, 0, s>>>(row, rw, multi_.get(), nmulti_, chstart_.get(), pstart_.get(), part_.get(),
-                                          dst_half, upper ? edge : nullptr);
+                                          dst_half, em, upper);
       }
     }
     check_launch("k_osplat");
   }
   const int R2 = row / 2;
-  auto blur = [&](auto kern, const int* nb, const float* a, float* b, const int* need) {
-    const unsigned g = need ? std::min(grid_for((size_t)M_ * 64), 2048u) : grid_for((size_t)M_ * 64);
-    kern<<<g, kBlock, 0, s>>>(R2, M_, nb, reinterpret_cast<const float2*>(a), reinterpret_cast<float2*>(b), need);
+  auto blur = [&](auto kern, const int* nb, const float* a, float* b, int upper) {
+    // the upper pass is usually skipped whole: a capped grid keeps its no-op cheap
+    const unsigned g = upper && nparts_ == 1 ? std::min(grid_for((size_t)M_ * 64), 2048u)
+                                              : grid_for((size_t)M_ * 64);
+    kern<<<g, kBlock, 0, s>>>(R2, M_, nb, reinterpret_cast<const float2*>(a), reinterpret_cast<float2*>(b), em,
+                              upper);
   };
   for (int j = 0; j <= d_; ++j) {  // forward order only (:621, :669)
     const int* nb = nbr_.get() + (size_t)j * M_ * 2;
     {
       KTimer tm("oblur" + tag, s);
       if (d_ == 2)
-        blur(k_blur2<2>, nb, srcL, dstL, nullptr);
+        blur(k_blur2<2>, nb, srcL, dstL, 0);
       else if (d_ == 5)
-        blur(k_blur2<5>, nb, srcL, dstL, nullptr);
+        blur(k_blur2<5>, nb, srcL, dstL, 0);
       else
-        blur(k_blur2<0>, nb, srcL, dstL, nullptr);
+        blur(k_blur2<0>, nb, srcL, dstL, 0);
     }
     {
       KTimer tm("oblur_upper" + tag, s);
       if (d_ == 2)
-        blur(k_blur2<102>, nb, srcU, dstU, edge);
+        blur(k_blur2<102>, nb, srcU, dstU, 1);
       else if (d_ == 5)
-        blur(k_blur2<105>, nb, srcU, dstU, edge);
+        blur(k_blur2<105>, nb, srcU, dstU, 1);
       else
-        blur(k_blur2<100>, nb, srcU, dstU, edge);
+        blur(k_blur2<100>, nb, srcU, dstU, 1);
     }
     check_launch("k_blur2");
     std::swap(srcL, dstL);
@@ -1077,7 +1107,7 @@ void Lattice::ordered(const float* Q, const float* d_minmax, int L, const SliceA
   KTimer tm("oslice" + tag, s);
   auto slice = [&](auto kern) {
     kern<<<rows_grid(N_, L), kBlock, 0, s>>>(bins_.get(), L, rm, N_, d_ + 1, offset_.get(), bary_.get(), srcL, srcU,
-                                             tb_.get(), edge, lattice_alpha(d_), out);
+                                             tb_.get(), em, lattice_alpha(d_), out);
   };
   if (d_ == 2)
     slice(k_oslice<2>);
@@ -1136,6 +1166,8 @@ std::unique_ptr<Lattice> Lattice::concat(const std::vector<const Lattice*>& part
   u->chpoint_.resize(std::max(nch, 1L));
   u->multi_.resize(std::max(nmul, 1L));
   u->pimg_.resize(N);
+  u->ppimg_.resize(M + 1);
+  u->nparts_ = (int)parts.size();
   long pb = 0, mb = 0, eb = 0, cb = 0, ub = 0, qb = 0;
   for (size_t b = 0; b < parts.size(); ++b) {
     const Lattice& p = *parts[b];
@@ -1156,6 +1188,7 @@ std::unique_ptr<Lattice> Lattice::concat(const std::vector<const Lattice*>& part
     add(p.chpoint_.get(), u->chpoint_.get() + cb, p.nchunks_, mb, 0);
     if (p.nmulti_ > 0) add(p.multi_.get(), u->multi_.get() + ub, p.nmulti_, mb, 0);
     k_fill_u16<<<grid_for(p.N_), kBlock, 0, s>>>(u->pimg_.get() + pb, p.N_, (uint16_t)b);
+    if (m) k_fill_u16<<<grid_for(m), kBlock, 0, s>>>(u->ppimg_.get() + mb, m, (uint16_t)b);
     pb += p.N_;
     mb += m;
     eb += e;

@@ -114,7 +114,9 @@ class Lattice {
   DBuf<float> tb_;
   DBuf<int> edge_;
   DBuf<uint8_t> bins_;
-  DBuf<uint16_t> pimg_;  // part of each pixel (union lattices only)
+  DBuf<uint16_t> pimg_;   // part of each pixel (union lattices only)
+  DBuf<uint16_t> ppimg_;  // part of each lattice point (union lattices only)
+  int nparts_ = 1;
   bool have_tb_ = false;
 };
 
