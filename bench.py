@@ -148,7 +148,11 @@ def adjoint_bench(D, S, steps=3):
     for _ in range(steps):
         fwd_bwd()
     tb = (time.perf_counter() - t0) / steps
-    return {"workload": f"mean field {T} it + adjoint, {B} x {Wd}x{Wd}, {L} labels, Potts + Matrix terms, "
+    D.profile_enable(True)
+    fwd_bwd()
+    prof = D.profile_report()
+    D.profile_enable(False)
+    return {"kernels": {n: {"launches": c_, "total_ms": round(m, 3)} for n, (c_, m) in sorted(prof.items())},"workload": f"mean field {T} it + adjoint, {B} x {Wd}x{Wd}, {L} labels, Potts + Matrix terms, "
                         "cross-entropy loss", "forward_ms": round(tf * 1e3, 3),
             "forward_adjoint_ms": round(tb * 1e3, 3), "ratio": round(tb / tf, 3)}
 
@@ -213,12 +217,27 @@ def batch_bench(D, imgs):
     D.profile_enable(False)
     lab = np.ascontiguousarray(Qlp.numpy().argmax(1).astype(np.int32))
     e_batch = b.assignment_energy_true(lab)
+    qp_batch = Qqp.numpy()
+    lp_batch = Qlp.numpy()
+    # the same with the ordered filter's lattice rows in bfloat16
+    b.set_value_type(True)
+    solve()
+    solve()
+    t0 = time.perf_counter()
+    solve()
+    t_bf16 = time.perf_counter() - t0
+    lab16 = np.ascontiguousarray(Qlp.numpy().argmax(1).astype(np.int32))
+    e_bf16 = b.assignment_energy_true(lab16)
+    rel16 = np.abs(e_bf16 - e_batch) / np.abs(e_batch)
+    _progress(f"batched solve with bfloat16 rows {t_bf16:.3f} s")
     b.close()
     del Qqp, Qlp
     _progress(f"batched solve {t_batch:.3f} s")
     # the one-image path, image after image
     t_single = 0.0
     e_single = []
+    qp_diff = []
+    lab_agree = []
     for (W, H, im, U) in imgs:
         c = D.DenseCRF2D(W, H, L)
         c.setUnaryEnergy(D.DeviceArray.from_numpy(U))
@@ -233,7 +252,11 @@ def batch_bench(D, imgs):
         c.lp_inference_prox_super_pixels(q1, P, out=q2)
         D.synchronize()
         t_single += time.perf_counter() - t0
-        e_single.append(c.assignment_energy_true(np.ascontiguousarray(q2.numpy().argmax(1).astype(np.int32))))
+        l2 = np.ascontiguousarray(q2.numpy().argmax(1).astype(np.int32))
+        e_single.append(c.assignment_energy_true(l2))
+        o0, o1 = b.offsets[len(e_single) - 1], b.offsets[len(e_single)]
+        qp_diff.append(float(np.abs(q1.numpy() - qp_batch[o0:o1]).max()))
+        lab_agree.append(float((l2 == lp_batch[o0:o1].argmax(1)).mean()))
         c.close()
     _progress(f"one-image path {t_single:.3f} s")
     e_single = np.array(e_single)
@@ -243,7 +266,14 @@ def batch_bench(D, imgs):
             "batched_s": round(t_batch, 4), "images_per_s": round(len(imgs) / t_batch, 3),
             "one_image_path_s": round(t_single, 4), "one_image_path_images_per_s": round(len(imgs) / t_single, 3),
             "speedup": round(t_single / t_batch, 3),
-            "energy_rel_diff_max": float(rel.max()),
+            "bf16_rows": {"batched_s": round(t_bf16, 4), "images_per_s": round(len(imgs) / t_bf16, 3),
+                          "speedup_vs_one_image_path": round(t_single / t_bf16, 3),
+                          "lp_energy_rel_diff_vs_float_rows": {"max": float(rel16.max()),
+                                                               "median": float(np.median(rel16))},
+                          "label_agreement_vs_float_rows": float((lab16 == lab).mean())},
+            "qp_max_abs_diff_vs_one_image": float(max(qp_diff)),
+            "lp_energy_rel_diff_vs_one_image": {"max": float(rel.max()), "median": float(np.median(rel))},
+            "lp_label_agreement_vs_one_image": {"min": float(min(lab_agree)), "median": float(np.median(lab_agree))},
             "kernels": {n: {"launches": c_, "total_ms": round(m, 3)} for n, (c_, m) in sorted(prof.items())}}
 
 

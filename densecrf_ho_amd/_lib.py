@@ -81,6 +81,8 @@ SIGNATURES = {
     "dcrf_batch_pixel_offsets": (_i, [_vp, _vp]),
     "dcrf_batch_set_graphs": (_i, [_vp, _i]),
     "dcrf_batch_graph_captures": (_i, [_vp, _vp]),
+    "dcrf_batch_set_value_type": (_i, [_vp, _i]),
+    "dcrf_crf_set_value_type": (_i, [_vp, _i]),
     "dcrf_batch_set_unary": (_i, [_vp, _i, _vp, _i]),
     "dcrf_batch_add_pairwise": (_i, [_vp, _vp, _i, _f, _i, _i]),
     "dcrf_batch_add_pairwise_gaussian": (_i, [_vp, _f, _f, _f]),

@@ -298,6 +298,10 @@ class DenseCRF:
                                                             int(normalization_type), a.dev))
         self.npairwise += 1
 
+    def set_value_type(self, bf16: bool):
+        """ordered-filter lattice rows of the terms added so far in bfloat16 (True) or float"""
+        _check(_lib.load().dcrf_crf_set_value_type(self._h, int(bool(bf16))))
+
     def pairwise_parameters(self, k):
         """PairwisePotential::parameters() of potential k."""
         n = C.c_int()
@@ -543,6 +547,10 @@ class DenseCRFBatch:
 
     def set_graphs(self, on: bool):
         _check(_lib.load().dcrf_batch_set_graphs(self._h, int(on)))
+
+    def set_value_type(self, bf16: bool):
+        """ordered-filter lattice rows stored in bfloat16 (True) or float"""
+        _check(_lib.load().dcrf_batch_set_value_type(self._h, int(bool(bf16))))
 
     def graph_captures(self) -> int:
         n = C.c_longlong()

@@ -23,7 +23,7 @@ namespace dcrf {
 namespace {
 
 constexpr int kMaxPw = 8;          // potentials fused into one mean-field update launch
-constexpr int kOuterBlocks = 512;  // blocks of the L x L gradient reduction
+constexpr int kOuterBlocks = 2048;  // blocks of the L x L gradient reduction
 
 // out (+)= C x per pixel; C Diagonal (cm = v[L]) or Matrix (cm = W[L*L]
 // column-major).  The sum over j runs in index order like the oracle's.
@@ -61,17 +61,20 @@ __global__ void __launch_bounds__(kBlock) k_compat_rg(int N, int L, int ctype, c
 }
 
 // Per-block partials of G = sum_i b_i m_i^T (G[a*L + c] = sum_i b(a,i) m(c,i)).
-// One wave per block, ⌊64/L⌋ pixels per step; each lane owns the L entries
-// of its (pixel group, label a) row of the LDS accumulator, so no atomics
-// are needed.  The block then sums its rows in a fixed order in FP64.
+// One wave per block, ⌊64/L⌋ pixels per step; lane (pixel group, label a)
+// keeps its row G[a][0..L) in registers (MAXL >= L) and adds b(a,i) m(c,i)
+// for the pixel's m row staged in LDS.  At the end the wave's rows meet in
+// LDS and the block writes their sum (FP64, fixed order).
+template <int MAXL>
 __global__ void __launch_bounds__(64) k_outer_w(int N, int L, const float* __restrict__ b,
                                                 const float* __restrict__ m, double* __restrict__ parts) {
   __shared__ float red[64 * kMaxLabels];  // gpw * L * L <= 64 * L
   __shared__ float seg[64];
   const int gpw = 64 / L, LL = L * L, lane = threadIdx.x;
   const int g = lane / L, k = lane - g * L;
-  for (int i = lane; i < gpw * LL; i += 64) red[i] = 0.f;
-  __syncthreads();
+  float acc[MAXL];
+#pragma unroll
+  for (int c = 0; c < MAXL; ++c) acc[c] = 0.f;
   const long nt = ((long)N + gpw - 1) / gpw;
   for (long tile = blockIdx.x; tile < nt; tile += gridDim.x) {
     const long row = tile * gpw + g;
@@ -84,12 +87,20 @@ __global__ void __launch_bounds__(64) k_outer_w(int N, int L, const float* __res
     }
     __syncthreads();
     if (valid) {
-      float* acc = red + (size_t)(g * L + k) * L;
       const float* mr = seg + g * L;
-      for (int c = 0; c < L; ++c) acc[c] = acc[c] + bv * mr[c];
+#pragma unroll
+      for (int c = 0; c < MAXL; ++c)
+        if (c < L) acc[c] = acc[c] + bv * mr[c];
     }
     __syncthreads();
   }
+  if (g < gpw) {
+    float* dst = red + (size_t)(g * L + k) * L;
+#pragma unroll
+    for (int c = 0; c < MAXL; ++c)
+      if (c < L) dst[c] = acc[c];
+  }
+  __syncthreads();
   for (int e = lane; e < LL; e += 64) {
     double s = 0.0;
     for (int q = 0; q < gpw; ++q) s += (double)red[(size_t)q * LL + e];
@@ -336,7 +347,16 @@ static void compat_grad_accum_dev(int ctype, int N, int L, const float* b, const
                                   double* parts, hipStream_t s) {
   const long tiles = ((long)N + 64 / L - 1) / (64 / L);
   const unsigned nb = (unsigned)std::max(1L, std::min<long>(tiles, kOuterBlocks));
-  k_outer_w<<<nb, 64, 0, s>>>(N, L, b, m, parts);
+  if (L == 21)
+    k_outer_w<21><<<nb, 64, 0, s>>>(N, L, b, m, parts);
+  else if (L <= 8)
+    k_outer_w<8><<<nb, 64, 0, s>>>(N, L, b, m, parts);
+  else if (L <= 16)
+    k_outer_w<16><<<nb, 64, 0, s>>>(N, L, b, m, parts);
+  else if (L <= 32)
+    k_outer_w<32><<<nb, 64, 0, s>>>(N, L, b, m, parts);
+  else
+    k_outer_w<64><<<nb, 64, 0, s>>>(N, L, b, m, parts);
   check_launch("k_outer_w");
   double* G = parts + (size_t)kOuterBlocks * L * L;
   k_outer_sum<<<L * L, kBlock, 0, s>>>(parts, (int)nb, L * L, G);

@@ -206,14 +206,13 @@ Batch::Batch(int L, const std::vector<int>& npix) : L_(L), B_((int)npix.size()) 
   DCRF_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
   pix_.resize(B_ + 1);
   DCRF_HIP(hipMemcpyAsync(pix_.get(), pix_h_.data(), (B_ + 1) * sizeof(long), hipMemcpyHostToDevice, s_));
-  // reduction / row-group plan: blocks in proportion to size, at least 2048 in total
-  std::vector<int> red(B_), sw(B_);
-  for (int b = 0; b < B_; ++b) {
-    const long n = npix[b];
-    const long tiles = rg_tiles(n, L);
-    red[b] = (int)std::max(1L, std::min(tiles, (long)std::ceil(4096.0 * n / (double)N_)));
-    sw[b] = (int)((n + kBlock - 1) / kBlock);
-  }
+  // reduction / row-group plan: every image gets the block count the
+  // one-image path uses for its reductions (kRedBlocks), so each image's
+  // partial sums, and with them every line-search step, gap and energy, are
+  // those of the one-image path bit for bit; sweep plan: one block per 256
+  // pixels, as there
+  std::vector<int> red(B_, kRedBlocks), sw(B_);
+  for (int b = 0; b < B_; ++b) sw[b] = (int)((npix[b] + kBlock - 1) / kBlock);
   red_.build(red, s_);
   sweep_.build(sw, s_);
   U_.resize((size_t)N_ * L);
@@ -248,6 +247,7 @@ void Batch::add_pairwise(const std::vector<const float*>& feats, int d, float w,
   }
   auto u = Lattice::concat(raw, s_);
   parts.clear();
+  u->set_bf16_rows(bf16_);
   auto p = std::make_unique<Potential>();
   potential_init_lattice(*p, std::move(u), w, ntype, s_);
   pw_.push_back(std::move(p));
@@ -474,7 +474,7 @@ std::vector<std::vector<int>> Batch::lp(const float* init, float* out, const LPP
   };
 
   // ---- initialisation (:534-550) ------------------------------------------------
-  k_lp_project<<<rg_grid(N, L, 8192), kBlock, 0, s>>>(N, L, init, lQ_.get(), lab, 0);
+  k_lp_project<<<nbr, kBlock, 0, s>>>(rm, L, init, lQ_.get(), lab, 0);
   DCRF_HIP(hipMemcpyAsync(best_.get(), lQ_.get(), NL * sizeof(float), hipMemcpyDeviceToDevice, s));
   int_energy_dev(lab, sc + kL_EINT, kL_NSC);
   lp_energy_dev(lQ_.get(), sc + kL_TMP + 1, sc + kL_TMP + 2, kL_NSC);
@@ -566,7 +566,7 @@ std::vector<std::vector<int>> Batch::lp(const float* init, float* out, const LPP
     check_err();
     {
       KTimer tm("lp_project", s);
-      k_lp_project<<<rg_grid(N, L, 8192), kBlock, 0, s>>>(N, L, lQ_.get(), lQ_.get(), lab, 1);
+      k_lp_project<<<nbr, kBlock, 0, s>>>(rm, L, lQ_.get(), lQ_.get(), lab, 1);
     }
     int_energy_dev(lab, sc + kL_EINT, kL_NSC);
     kb_lp_outer_decide<<<1, 256, 0, s>>>(sc, ost_.get(), ofl, P.prox_energy_tol, B);

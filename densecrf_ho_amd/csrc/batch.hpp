@@ -58,6 +58,12 @@ class Batch {
     graphs_.clear();
   }
   long graph_captures() const { return graphs_.captures; }
+  // ordered-filter lattice rows in bfloat16 (true) or float
+  void set_bf16_rows(bool on) {
+    bf16_ = on;
+    for (auto& p : pw_) p->lat->set_bf16_rows(on);
+    graphs_.clear();
+  }
 
   void set_unary(int b, const float* dU);                                          // device, N_b x L
   void add_pairwise(const std::vector<const float*>& feats, int d, float w, int ntype);  // device, per image
@@ -94,6 +100,7 @@ class Batch {
   DBuf<float> U_;
   std::vector<std::unique_ptr<Potential>> pw_;
   bool use_graph_ = true;
+  bool bf16_ = false;
   GraphCache graphs_;
   // QP state
   DBuf<float> Q_, gy_, psx_, tmp_, rpix_;

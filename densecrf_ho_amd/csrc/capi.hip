@@ -656,6 +656,19 @@ int dcrf_batch_set_graphs(dcrf_batch_t b, int on) {
     b->b->set_graphs(on != 0);
   });
 }
+int dcrf_batch_set_value_type(dcrf_batch_t b, int type) {
+  return guarded([&] {
+    DCRF_REQUIRE(b && (type == DCRF_VALUES_F32 || type == DCRF_VALUES_BF16), "dcrf_batch_set_value_type: bad arguments");
+    b->b->set_bf16_rows(type == DCRF_VALUES_BF16);
+  });
+}
+int dcrf_crf_set_value_type(dcrf_crf_t crf, int type) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && (type == DCRF_VALUES_F32 || type == DCRF_VALUES_BF16), "dcrf_crf_set_value_type: bad arguments");
+    CRF& c = *crf->crf;
+    for (int k = 0; k < c.num_pairwise(); ++k) c.pairwise(k).lat->set_bf16_rows(type == DCRF_VALUES_BF16);
+  });
+}
 int dcrf_batch_graph_captures(dcrf_batch_t b, long long* n) {
   return guarded([&] {
     DCRF_REQUIRE(b && n, "dcrf_batch_graph_captures: bad arguments");

@@ -434,6 +434,27 @@ __global__ void __launch_bounds__(kBlock) k_slice(int V, RowMap rm, int N, int d
   }
 }
 
+// Storage of the ordered filter's lattice rows: float, or bfloat16 bits
+// (round to nearest even); all arithmetic is float.
+__device__ inline uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float ld_v(const float* p, size_t i) { return p[i]; }
+__device__ inline float ld_v(const uint16_t* p, size_t i) { return __uint_as_float((uint32_t)p[i] << 16); }
+__device__ inline void st_v(float* p, size_t i, float v) { p[i] = v; }
+__device__ inline void st_v(uint16_t* p, size_t i, float v) { p[i] = f2bf(v); }
+__device__ inline float2 ld_pair(const float* row, long k) { return reinterpret_cast<const float2*>(row)[k]; }
+__device__ inline float2 ld_pair(const uint16_t* row, long k) {
+  const uint32_t v = reinterpret_cast<const uint32_t*>(row)[k];
+  return make_float2(__uint_as_float(v << 16), __uint_as_float(v & 0xffff0000u));
+}
+__device__ inline void st_pair(float* row, long k, float2 v) { reinterpret_cast<float2*>(row)[k] = v; }
+__device__ inline void st_pair(uint16_t* row, long k, float2 v) {
+  reinterpret_cast<uint32_t*>(row)[k] = (uint32_t)f2bf(v.x) | ((uint32_t)f2bf(v.y) << 16);
+}
+
 // Bin-edge flags of the ordered filter, one per part of the lattice (one
 // image, or each image of a union) plus flag[parts] = "any part".  A part
 // whose flag is 0 takes the upper pass from T - LO[c+1] and skips its own.
@@ -449,9 +470,10 @@ struct EdgeMap {
 
 // k_combine of the lower pass, or of the upper pass (upper != 0) for the
 // points of parts with bin edges only
+template <class T>
 __global__ void __launch_bounds__(kBlock) k_combine_if(int V, RowMap rm, const int* __restrict__ multi, int nmulti,
                              const int* __restrict__ chstart, const int* __restrict__ pstart,
-                             const float* __restrict__ part, float* __restrict__ values, EdgeMap edge,
+                             const float* __restrict__ part, T* __restrict__ values, EdgeMap edge,
                              int upper) {
   if (upper && !edge.any()) return;
   const long nvb = (long)((nmulti + rm.gpw - 1) / rm.gpw * 64 + blockDim.x - 1) / blockDim.x;
@@ -466,7 +488,7 @@ __global__ void __launch_bounds__(kBlock) k_combine_if(int V, RowMap rm, const i
     for (int k = k0; k < V; k += ks) {
       float acc = part[(size_t)c0 * V + k];
       for (int c = c0 + 1; c < c1; ++c) acc = acc + part[(size_t)c * V + k];
-      values[(size_t)(p + 1) * V + k] = acc;
+      st_v(values, (size_t)(p + 1) * V + k, acc);
     }
   }
 }
@@ -573,12 +595,12 @@ __device__ inline float rescaled(const float* Q, size_t idx, const float* mm) {
 // bin_up == bin_lo for every (pixel, label); the splat raises `edge` when
 // some q sits on a bin edge (bin_up == bin_lo - 1), and then the upper pass
 // is splatted and blurred in full into UP[point][label][10].
-template <int TAG>  // TAG only names the instantiation (lattice dim, +100 for the upper pass)
+template <int TAG, class T>  // TAG only names the instantiation (lattice dim, +100 for the upper pass)
 __global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ bins, int L, RowMap rm, long nvblocks,
                                                    int upper,
                          const int* __restrict__ sp_pix, const float* __restrict__ sp_w, const int* __restrict__ seg,
                          const int* __restrict__ chstart, const int* __restrict__ chpoint,
-                         const int* __restrict__ pstart, int nchunks, float* __restrict__ values,
+                         const int* __restrict__ pstart, int nchunks, T* __restrict__ values,
                          float* __restrict__ part, EdgeMap edge) {
   __shared__ int spx[kBlock];
   __shared__ float swt[kBlock];
@@ -633,11 +655,15 @@ __global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ b
     __syncthreads();
   }
   if (act) {
-    float* dst = nc == 1 ? values + ((size_t)(p + 1) * L + k0) * 10
-                         : part + ((size_t)(pstart[p] + (c - c0)) * L + k0) * 10;
-    float2* d2 = reinterpret_cast<float2*>(dst);
+    if (nc == 1) {  // the point's row
+      T* dst = values + ((size_t)(p + 1) * L + k0) * 10;
 #pragma unroll
-    for (int q = 0; q < 5; ++q) d2[q] = make_float2(acc[2 * q], acc[2 * q + 1]);
+      for (int q = 0; q < 5; ++q) st_pair(dst, q, make_float2(acc[2 * q], acc[2 * q + 1]));
+    } else {  // a chunk partial (float), summed by k_combine_if
+      float* dst = part + ((size_t)(pstart[p] + (c - c0)) * L + k0) * 10;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) st_pair(dst, q, make_float2(acc[2 * q], acc[2 * q + 1]));
+    }
   }
   __syncthreads();  // maxlen / staging buffers are reused by the next virtual block
   }
@@ -646,25 +672,25 @@ __global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ b
 // one blur axis over rows of R2 float2 (ordered state): one wave per lattice
 // point, lanes stride the row; the upper pass (upper != 0) only blurs the
 // points of parts with bin edges (a point's neighbours are in its own part)
-template <int TAG>
-__global__ void __launch_bounds__(kBlock) k_blur2(int R2, int M, const int* __restrict__ nbr, const float2* __restrict__ src,
-                        float2* __restrict__ dst, EdgeMap edge, int upper) {
+template <int TAG, class T>
+__global__ void __launch_bounds__(kBlock) k_blur2(int R2, int M, const int* __restrict__ nbr, const T* __restrict__ src,
+                        T* __restrict__ dst, EdgeMap edge, int upper) {
   if (upper && !edge.any()) return;
   const int lane = threadIdx.x & 63;
   const long nw = (long)gridDim.x * (blockDim.x >> 6);
   for (long i = ((long)xcd_block() * blockDim.x + threadIdx.x) >> 6; i < M; i += nw) {
     if (upper && !edge.point(i)) continue;
     const int n1 = nbr[2 * i] + 1, n2 = nbr[2 * i + 1] + 1;
-    const float2* so = src + (size_t)(i + 1) * R2;
-    const float2* s1 = src + (size_t)n1 * R2;
-    const float2* s2 = src + (size_t)n2 * R2;
-    float2* d = dst + (size_t)(i + 1) * R2;
+    const T* so = src + (size_t)(i + 1) * 2 * R2;
+    const T* s1 = src + (size_t)n1 * 2 * R2;
+    const T* s2 = src + (size_t)n2 * 2 * R2;
+    T* d = dst + (size_t)(i + 1) * 2 * R2;
     for (int k = lane; k < R2; k += 64) {
-      const float2 o = so[k], a = s1[k], b = s2[k];
+      const float2 o = ld_pair(so, k), a = ld_pair(s1, k), b = ld_pair(s2, k);
       float2 r;
       r.x = o.x + 0.5f * (a.x + b.x);
       r.y = o.y + 0.5f * (a.y + b.y);
-      d[k] = r;
+      st_pair(d, k, r);
     }
   }
 }
@@ -692,10 +718,10 @@ __global__ void __launch_bounds__(kBlock) k_obin(const float* __restrict__ Q, co
 
 // lower slice then upper slice (:638-650, :686-696), then out (+)= scale*x.
 // Without edges the upper value at bin c is T - LO[c+1].
-template <int TAG>
+template <int TAG, class T>
 __global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ bins, int L, RowMap rm, int N, int d1,
                          const int* __restrict__ offset, const float* __restrict__ bary,
-                         const float* __restrict__ lo_v, const float* __restrict__ up_v, const float* __restrict__ Tb,
+                         const T* __restrict__ lo_v, const T* __restrict__ up_v, const float* __restrict__ Tb,
                          EdgeMap edge, float alpha, SliceArgs o) {
   __shared__ int soff[kBlock];
   __shared__ float sbw[kBlock];
@@ -726,13 +752,13 @@ __global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ b
     if (j < d1) {
       const int of = voff(j);
       const float w = vw(j);
-      const float* row = lo_v + ((size_t)of * L + k0) * 10;
-      acc = acc + row[lo] * (-w * alpha);
+      const T* row = lo_v + ((size_t)of * L + k0) * 10;
+      acc = acc + ld_v(row, lo) * (-w * alpha);
       float u;
       if (full)
-        u = up_v[((size_t)of * L + k0) * 10 + up];
+        u = ld_v(up_v, ((size_t)of * L + k0) * 10 + up);
       else
-        u = Tb[of] - (up < 9 ? row[up + 1] : 0.f);
+        u = Tb[of] - (up < 9 ? ld_v(row, up + 1) : 0.f);
       ut[j] = u * (w * alpha);
     }
   }
@@ -1019,21 +1045,33 @@ void Lattice::ensure_total_weight() {
 }
 
 void Lattice::ordered(const float* Q, const float* d_minmax, int L, const SliceArgs& out) {
+  if (bf16_)
+    ordered_t<uint16_t>(Q, d_minmax, L, out);
+  else
+    ordered_t<float>(Q, d_minmax, L, out);
+}
+
+// T: storage of the lattice rows (float, or bfloat16 bits)
+template <class T>
+void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const SliceArgs& out) {
   DCRF_REQUIRE(L > 0, "ordered filter: label count must be positive");
   DCRF_REQUIRE(L <= 64, "ordered filter: at most 64 labels");
   ensure_total_weight();
   const int row = L * 10;
-  ensure_values(2 * row);  // LO and UP halves of each buffer
-  const size_t half = (size_t)(M_ + 1) * row;
+  ensure_values(2 * row);  // LO and UP halves of each buffer (float-sized; bf16 rows use half)
+  const size_t half = (size_t)(M_ + 1) * row;  // elements of T
   hipStream_t s = stream_;
   const RowMap rm = rowmap(L);
-  float* srcL = vA_.get();
-  float* dstL = vB_.get();
-  float* srcU = vA_.get() + half;
-  float* dstU = vB_.get() + half;
-  // row 0 of each half is the zero row
-  k_zero<<<grid_for(row), kBlock, 0, s>>>(vA_.get() + half, row);
-  k_zero<<<grid_for(row), kBlock, 0, s>>>(vB_.get() + half, row);
+  T* srcL = reinterpret_cast<T*>(vA_.get());
+  T* dstL = reinterpret_cast<T*>(vB_.get());
+  T* srcU = srcL + half;
+  T* dstU = dstL + half;
+  // row 0 of each half is the zero row (row * sizeof(T) bytes, a whole number of floats)
+  const size_t zf = (size_t)row * sizeof(T) / sizeof(float);
+  k_zero<<<grid_for(zf), kBlock, 0, s>>>(reinterpret_cast<float*>(srcL), zf);
+  k_zero<<<grid_for(zf), kBlock, 0, s>>>(reinterpret_cast<float*>(dstL), zf);
+  k_zero<<<grid_for(zf), kBlock, 0, s>>>(reinterpret_cast<float*>(srcU), zf);
+  k_zero<<<grid_for(zf), kBlock, 0, s>>>(reinterpret_cast<float*>(dstU), zf);
   int* edge = edge_.get();
   const EdgeMap em{edge, pimg_.n ? pimg_.get() : nullptr, ppimg_.n ? ppimg_.get() : nullptr, nparts_};
   const std::string tag = prof_on() ? "[d=" + std::to_string(d_) + ",L=" + std::to_string(L) + "]" : std::string();
@@ -1049,7 +1087,7 @@ void Lattice::ordered(const float* Q, const float* d_minmax, int L, const SliceA
   {
     KTimer tm("osplat" + tag, s);
     for (int upper = 0; upper < 2; ++upper) {
-      float* dst_half = upper ? srcU : srcL;
+      T* dst_half = upper ? srcU : srcL;
       const long nvb = rows_grid(nchunks_, L);
       const unsigned grid = upper ? (unsigned)std::min<long>(nvb, 2048) : (unsigned)nvb;
       auto launch = [&](auto kern) {
@@ -1058,47 +1096,46 @@ void Lattice::ordered(const float* Q, const float* d_minmax, int L, const SliceA
                                      em);
       };
       if (d_ == 2)
-        upper ? launch(k_osplat<102>) : launch(k_osplat<2>);
+        upper ? launch(k_osplat<102, T>) : launch(k_osplat<2, T>);
       else if (d_ == 5)
-        upper ? launch(k_osplat<105>) : launch(k_osplat<5>);
+        upper ? launch(k_osplat<105, T>) : launch(k_osplat<5, T>);
       else
-        upper ? launch(k_osplat<100>) : launch(k_osplat<0>);
+        upper ? launch(k_osplat<100, T>) : launch(k_osplat<0, T>);
       if (nmulti_ > 0) {
         const RowMap rw = rowmap(row);
         const unsigned g = upper ? std::min(rows_grid(nmulti_, row), 1024u) : rows_grid(nmulti_, row);
-        k_combine_if<<<g, kBlock, 0, s>>>(row, rw, multi_.get(), nmulti_, chstart_.get(), pstart_.get(), part_.get(),
-                                          dst_half, em, upper);
+        k_combine_if<T><<<g, kBlock, 0, s>>>(row, rw, multi_.get(), nmulti_, chstart_.get(), pstart_.get(),
+                                             part_.get(), dst_half, em, upper);
       }
     }
     check_launch("k_osplat");
   }
   const int R2 = row / 2;
-  auto blur = [&](auto kern, const int* nb, const float* a, float* b, int upper) {
+  auto blur = [&](auto kern, const int* nb, const T* a, T* b, int upper) {
     // the upper pass is usually skipped whole: a capped grid keeps its no-op cheap
     const unsigned g = upper && nparts_ == 1 ? std::min(grid_for((size_t)M_ * 64), 2048u)
                                               : grid_for((size_t)M_ * 64);
-    kern<<<g, kBlock, 0, s>>>(R2, M_, nb, reinterpret_cast<const float2*>(a), reinterpret_cast<float2*>(b), em,
-                              upper);
+    kern<<<g, kBlock, 0, s>>>(R2, M_, nb, a, b, em, upper);
   };
   for (int j = 0; j <= d_; ++j) {  // forward order only (:621, :669)
     const int* nb = nbr_.get() + (size_t)j * M_ * 2;
     {
       KTimer tm("oblur" + tag, s);
       if (d_ == 2)
-        blur(k_blur2<2>, nb, srcL, dstL, 0);
+        blur(k_blur2<2, T>, nb, srcL, dstL, 0);
       else if (d_ == 5)
-        blur(k_blur2<5>, nb, srcL, dstL, 0);
+        blur(k_blur2<5, T>, nb, srcL, dstL, 0);
       else
-        blur(k_blur2<0>, nb, srcL, dstL, 0);
+        blur(k_blur2<0, T>, nb, srcL, dstL, 0);
     }
     {
       KTimer tm("oblur_upper" + tag, s);
       if (d_ == 2)
-        blur(k_blur2<102>, nb, srcU, dstU, 1);
+        blur(k_blur2<102, T>, nb, srcU, dstU, 1);
       else if (d_ == 5)
-        blur(k_blur2<105>, nb, srcU, dstU, 1);
+        blur(k_blur2<105, T>, nb, srcU, dstU, 1);
       else
-        blur(k_blur2<100>, nb, srcU, dstU, 1);
+        blur(k_blur2<100, T>, nb, srcU, dstU, 1);
     }
     check_launch("k_blur2");
     std::swap(srcL, dstL);
@@ -1110,11 +1147,11 @@ void Lattice::ordered(const float* Q, const float* d_minmax, int L, const SliceA
                                              tb_.get(), em, lattice_alpha(d_), out);
   };
   if (d_ == 2)
-    slice(k_oslice<2>);
+    slice(k_oslice<2, T>);
   else if (d_ == 5)
-    slice(k_oslice<5>);
+    slice(k_oslice<5, T>);
   else
-    slice(k_oslice<0>);
+    slice(k_oslice<0, T>);
   check_launch("k_oslice");
 }
 

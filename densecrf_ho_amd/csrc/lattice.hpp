@@ -71,6 +71,11 @@ class Lattice {
   // {min, max} pair per part.
   static std::unique_ptr<Lattice> concat(const std::vector<const Lattice*>& parts, hipStream_t s);
 
+  // storage of the ordered filter's lattice rows: float (default) or
+  // bfloat16 (arithmetic stays float; half the row bytes)
+  void set_bf16_rows(bool on) { bf16_ = on; }
+  bool bf16_rows() const { return bf16_; }
+
   // addresses of the work buffers a filter call writes (value rows, chunk
   // partials, ordered-filter bins and totals); they change when a call with
   // wider rows reallocates them, which invalidates captured launch sequences
@@ -91,6 +96,9 @@ class Lattice {
 
  private:
   void ensure_values(size_t floats);
+  template <class T>
+  void ordered_t(const float* Q, const float* d_minmax, int L, const SliceArgs& out);
+  bool bf16_ = false;
   void ensure_total_weight();
   hipStream_t stream_;
   int N_ = 0, d_ = 0, Np_ = 0, M_ = 0;

@@ -312,7 +312,8 @@ __global__ void __launch_bounds__(kBlock) k_lp_betaQ(SEG sg, int L, float lambda
     }
     __syncthreads();
     if (r.valid) {
-      float b = -eigen_sum(L, col_astart(r.row, L), [&](int i) { return row[i]; });
+      // alignment of the column within its own image's matrix
+      float b = -eigen_sum(L, col_astart(r.row - qs.p0, L), [&](int i) { return row[i]; });
       b = b / (float)L;
       const float q = lambda * ((((a + b) + uu) + gm) - uv) + curQ[idx];
       Q[idx] = q;
@@ -412,12 +413,16 @@ __global__ void k_lp_delta(double* __restrict__ sc_all, float lambda, double* __
 // (:929-938); row groups.  The descending sort of sortCols becomes a rank per
 // value (ties broken by index), the threshold loop runs on the sorted row in
 // LDS, and the column sums follow Eigen's order for the column's alignment.
-__global__ void __launch_bounds__(kBlock) k_lp_project(int N, int L, const float* __restrict__ Qin, float* __restrict__ Qout,
+template <class SEG>
+__global__ void __launch_bounds__(kBlock) k_lp_project(SEG sg, int L, const float* __restrict__ Qin, float* __restrict__ Qout,
                              int* __restrict__ lab, int do_feasible) {
   __shared__ float sq[kBlock], ss[kBlock];
-  const long ntiles = rg_tiles(N, L);
-  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const RG r = rg_at(L, N, tile);
+  const SegPos qs = sg.pos();
+  const long ntiles = rg_tiles(qs.n, L);
+  for (long tile = qs.lb; tile < ntiles; tile += qs.nb) {
+    RG r = rg_at(L, qs.n, tile);
+    const long lcol = r.row;  // column within its own image (sets the Eigen sum alignment)
+    r.row += qs.p0;
     const int wb = (threadIdx.x & ~63) + r.g * L;
     float* rowq = sq + wb;
     float* rows = ss + wb;
@@ -429,7 +434,7 @@ __global__ void __launch_bounds__(kBlock) k_lp_project(int N, int L, const float
     if (do_feasible) {
       float cs = 0.f;
       if (r.valid) {
-        cs = eigen_sum(L, col_astart(r.row, L), [&](int i) { return rowq[i]; });
+        cs = eigen_sum(L, col_astart(lcol, L), [&](int i) { return rowq[i]; });
         int rank = 0;
         for (int j = 0; j < L; ++j) {
           const float qj = rowq[j];
@@ -459,7 +464,7 @@ __global__ void __launch_bounds__(kBlock) k_lp_project(int N, int L, const float
       __syncthreads();
     }
     if (r.valid) {
-      const double sum = (double)eigen_sum(L, col_astart(r.row, L), [&](int i) { return rowq[i]; });
+      const double sum = (double)eigen_sum(L, col_astart(lcol, L), [&](int i) { return rowq[i]; });
       const float out = (sum == 0 || sum != sum) ? (float)(1.0 / L) : v / (float)sum;
       Qout[idx] = out;
       rows[r.k] = out;
@@ -646,7 +651,7 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
   };
 
   // ---- initialisation (:534-550) ------------------------------------------------
-  k_lp_project<<<rg_grid(N, L, 8192), kBlock, 0, s>>>(N, L, init, st.Q.get(), st.lab.get(), 0);
+  k_lp_project<<<rg_grid(N, L, 8192), kBlock, 0, s>>>(OneImage{N}, L, init, st.Q.get(), st.lab.get(), 0);
   DCRF_HIP(hipMemcpyAsync(st.best.get(), st.Q.get(), NL * sizeof(float), hipMemcpyDeviceToDevice, s));
   int_energy_dev(st.lab.get(), sc + kL_EINT, sc + kL_TMP, s);
   double int_energy = read_d(sc + kL_EINT);
@@ -745,7 +750,7 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
     // project, renormalize, labelling (:746-752)
     {
       KTimer tm("lp_project", s);
-      k_lp_project<<<rg_grid(N, L, 8192), kBlock, 0, s>>>(N, L, st.Q.get(), st.Q.get(), st.lab.get(), 1);
+      k_lp_project<<<rg_grid(N, L, 8192), kBlock, 0, s>>>(OneImage{N}, L, st.Q.get(), st.Q.get(), st.lab.get(), 1);
     }
     const double prev_int = int_energy;
     int_energy_dev(st.lab.get(), sc + kL_EINT, sc + kL_TMP, s);

@@ -246,6 +246,14 @@ int dcrf_batch_destroy(dcrf_batch_t b);
 int dcrf_batch_pixel_offsets(dcrf_batch_t b, long long* offsets);
 /* replay the fixed parts of every solver iteration from HIP graphs (default on) */
 int dcrf_batch_set_graphs(dcrf_batch_t b, int on);
+/* storage of the ordered filter's lattice rows (arithmetic is float either
+ * way): DCRF_VALUES_F32 (default) or DCRF_VALUES_BF16 (half the bytes;
+ * results move by the rounding of the stored rows).  No reference
+ * counterpart.  The CRF form applies to the terms added so far. */
+#define DCRF_VALUES_F32 0
+#define DCRF_VALUES_BF16 1
+int dcrf_batch_set_value_type(dcrf_batch_t b, int type);
+int dcrf_crf_set_value_type(dcrf_crf_t crf, int type);
 /* number of HIP graphs captured so far (a graph is captured again whenever a
  * buffer it refers to was reallocated) */
 int dcrf_batch_graph_captures(dcrf_batch_t b, long long* n);
