@@ -398,6 +398,7 @@ def main():
                      "traffic_source": traffic["source"] if traffic else None,
                      "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": int(ab)},
         "kernels": {n: {"launches": c, "total_ms": round(m, 3)} for n, (c, m) in sorted(prof.items())},
+        "autotune": {k: {"grid": g, "ms": round(ms, 4)} for k, (g, ms, _) in sorted(D.autotune_report().items())},
     }
     if not args.qp_only:
         out["adjoint"] = adjoint_bench(D, S)

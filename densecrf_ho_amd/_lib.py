@@ -97,6 +97,8 @@ SIGNATURES = {
     "dcrf_device_synchronize": (_i, []),
     "dcrf_profile_enable": (_i, [_i]),
     "dcrf_profile_report": (_i, [C.c_char_p, _i]),
+    "dcrf_autotune_enable": (_i, [_i]),
+    "dcrf_autotune_report": (_i, [C.c_char_p, _i]),
 }
 
 _lib = None

@@ -93,6 +93,28 @@ def profile_report() -> dict:
     return out
 
 
+def autotune_enable(on: bool = True):
+    _check(_lib.load().dcrf_autotune_enable(int(on)))
+
+
+def autotune_report() -> dict:
+    """{kernel@size_class: (grid, ms, {candidate grid: ms})} of the launch-shape autotuner."""
+    L = _lib.load()
+    n = L.dcrf_autotune_report(None, 0)
+    buf = C.create_string_buffer(max(n, 1))
+    L.dcrf_autotune_report(buf, n)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        head, trail = line.split("|", 1)
+        key, grid, ms = head.split()
+        cands = {}
+        for t in trail.split():
+            g, m = t.split(":")
+            cands[int(g)] = float(m)
+        out[key] = (int(grid), float(ms), cands)
+    return out
+
+
 class _Arg:
     """Pointer + on_device flag for an input or output array."""
 

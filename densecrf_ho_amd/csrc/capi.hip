@@ -5,6 +5,7 @@
 
 #include "../../include/densecrf_ho_amd.h"
 #include "batch.hpp"
+#include "tuner.hpp"
 #include "crf.hpp"
 #include "crf_kernels.hpp"
 
@@ -799,6 +800,20 @@ int dcrf_batch_assignment_energy_true(dcrf_batch_t b, const int* labels, int on_
   });
 }
 
+int dcrf_autotune_enable(int on) {
+  return guarded([&] { Tuner::get().set_enabled(on != 0); });
+}
+int dcrf_autotune_report(char* buf, int len) {
+  std::string r;
+  const int rc = guarded([&] { r = Tuner::get().report(); });
+  if (rc != kOk) return -rc;
+  if (buf && len > 0) {
+    const size_t n = std::min((size_t)len - 1, r.size());
+    std::memcpy(buf, r.data(), n);
+    buf[n] = 0;
+  }
+  return (int)r.size() + 1;
+}
 int dcrf_profile_report(char* buf, int len) {
   std::string r;
   const int rc = guarded([&] { r = Profiler::get().report(); });

@@ -8,6 +8,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "lattice.hpp"
+#include "tuner.hpp"
 
 namespace dcrf {
 namespace {
@@ -1089,18 +1090,26 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
     for (int upper = 0; upper < 2; ++upper) {
       T* dst_half = upper ? srcU : srcL;
       const long nvb = rows_grid(nchunks_, L);
-      const unsigned grid = upper ? (unsigned)std::min<long>(nvb, 2048) : (unsigned)nvb;
-      auto launch = [&](auto kern) {
+      auto launch = [&](auto kern, unsigned grid) {
         kern<<<grid, kBlock, 0, s>>>(bins_.get(), L, rm, nvb, upper, sp_pix_.get(), sp_w_.get(), seg_.get(),
                                      chstart_.get(), chpoint_.get(), pstart_.get(), nchunks_, dst_half, part_.get(),
                                      em);
       };
+      auto run = [&](auto kl, auto ku) {
+        if (upper) {
+          launch(ku, (unsigned)std::min<long>(nvb, 2048));
+        } else {  // the lower pass writes every row: tuned grid of its grid-stride loop
+          const unsigned g = Tuner::get().pick(d_ == 5 ? "osplat5" : "osplat", nvb, grid_candidates((unsigned)nvb), s,
+                                               [&](unsigned gg) { launch(kl, gg); });
+          launch(kl, g);
+        }
+      };
       if (d_ == 2)
-        upper ? launch(k_osplat<102, T>) : launch(k_osplat<2, T>);
+        run(k_osplat<2, T>, k_osplat<102, T>);
       else if (d_ == 5)
-        upper ? launch(k_osplat<105, T>) : launch(k_osplat<5, T>);
+        run(k_osplat<5, T>, k_osplat<105, T>);
       else
-        upper ? launch(k_osplat<100, T>) : launch(k_osplat<0, T>);
+        run(k_osplat<0, T>, k_osplat<100, T>);
       if (nmulti_ > 0) {
         const RowMap rw = rowmap(row);
         const unsigned g = upper ? std::min(rows_grid(nmulti_, row), 1024u) : rows_grid(nmulti_, row);
@@ -1112,9 +1121,14 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   }
   const int R2 = row / 2;
   auto blur = [&](auto kern, const int* nb, const T* a, T* b, int upper) {
-    // the upper pass is usually skipped whole: a capped grid keeps its no-op cheap
-    const unsigned g = upper && nparts_ == 1 ? std::min(grid_for((size_t)M_ * 64), 2048u)
-                                              : grid_for((size_t)M_ * 64);
+    const unsigned full = grid_for((size_t)M_ * 64);
+    unsigned g = full;
+    if (upper && nparts_ == 1)  // the upper pass is usually skipped whole: a capped grid keeps its no-op cheap
+      g = std::min(full, 2048u);
+    else if (!upper)  // src -> dst, the same bits for any grid: tuned
+      g = Tuner::get().pick(d_ == 5 ? "oblur5" : "oblur", M_, grid_candidates(full), s, [&](unsigned gg) {
+        kern<<<gg, kBlock, 0, s>>>(R2, M_, nb, a, b, em, upper);
+      });
     kern<<<g, kBlock, 0, s>>>(R2, M_, nb, a, b, em, upper);
   };
   for (int j = 0; j <= d_; ++j) {  // forward order only (:621, :669)

@@ -17,6 +17,7 @@
 
 #include "crf.hpp"
 #include "crf_kernels.hpp"
+#include "tuner.hpp"
 
 namespace dcrf {
 namespace {
@@ -613,7 +614,7 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
   const int nbp = (int)grid_for(N);
   st.parts.resize(std::max<size_t>((size_t)kSweepBatch * nbp, (size_t)2 * kRedBlocks));
   st.sc.resize(kL_NSC);
-  st.mmparts.resize(2 * (size_t)std::max<long>(nbp, 4096));
+  st.mmparts.resize(2 * (size_t)std::max<long>(nbp, 16384));  // up to the largest betaQ grid tried
   st.mm.resize(2);
   st.fl.resize(kLF_NFL);
   const int cap = std::max(1, std::min(stats_cap, 1 << 16));
@@ -679,8 +680,16 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
       ++pit;
       {
         KTimer tm("lp_Y", s);
-        k_lp_Y<<<rg_grid(N, L, 8192), kBlock, 0, s>>>(OneImage{N}, L, c, st.alpha.get(), u, st.sa.get(), su, sc,
-                                                     pit > 1 ? 1 : 0, U_.get(), st.curQ.get(), st.Y.get());
+        auto ylaunch = [&](unsigned g, int upd) {
+          k_lp_Y<<<g, kBlock, 0, s>>>(OneImage{N}, L, c, st.alpha.get(), u, st.sa.get(), su, sc, upd, U_.get(),
+                                      st.curQ.get(), st.Y.get());
+        };
+        // tuned on the first step (which only writes Y); later steps also
+        // update alpha in place and reuse the choice
+        const unsigned gy = pit == 1 ? Tuner::get().pick("lp_Y", N, grid_candidates(rg_grid(N, L)), s,
+                                                          [&](unsigned g) { ylaunch(g, 0); })
+                                     : Tuner::get().cached("lp_Y", N, rg_grid(N, L, 8192));
+        ylaunch(gy, pit > 1 ? 1 : 0);
       }
       // qp-gamma sweeps in batches: pass 1 runs `nsw` sweeps gamma -> gamma2,
       // the decision finds where the reference's loop stops, the replay redoes
@@ -723,9 +732,12 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
       res.qp_sweeps.push_back(done_sweeps);
       {
         KTimer tm("lp_betaQ", s);
-        const unsigned nbq = rg_grid(N, L, 4096);
-        k_lp_betaQ<<<nbq, kBlock, 0, s>>>(OneImage{N}, L, lambda, st.alpha.get(), u, st.gamma.get(), U_.get(),
+        auto bq = [&](unsigned g) {
+          k_lp_betaQ<<<g, kBlock, 0, s>>>(OneImage{N}, L, lambda, st.alpha.get(), u, st.gamma.get(), U_.get(),
                                           st.curQ.get(), st.Q.get(), st.mmparts.get());
+        };
+        const unsigned nbq = Tuner::get().pick("lp_betaQ", N, grid_candidates(rg_grid(N, L, 16384)), s, bq);
+        bq(nbq);
         k_lp_minmax_final<<<1, 1024, 0, s>>>(st.mmparts.get(), nbq, nullptr, st.mm.get(), fl);
       }
       for (int k = 0; k < num_pairwise(); ++k) {

@@ -291,6 +291,13 @@ int dcrf_device_synchronize(void);
  * report writes lines "name count total_ms" into buf (NUL-terminated) and
  * returns the length it needs (call with len 0 to size the buffer). */
 int dcrf_profile_enable(int on);
+/* Launch-shape autotuner (no reference counterpart): grid-stride kernels whose
+ * results do not depend on the grid are timed over a few candidate grids at
+ * their first launch for a problem size and keep the fastest.  enable(0)
+ * turns it off (also DCRF_AUTOTUNE=0); report writes "kernel@size_class grid
+ * ms | candidate:ms ..." lines like dcrf_profile_report. */
+int dcrf_autotune_enable(int on);
+int dcrf_autotune_report(char* buf, int len);
 int dcrf_profile_report(char* buf, int len);
 
 #ifdef __cplusplus
