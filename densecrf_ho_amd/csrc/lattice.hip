@@ -470,12 +470,13 @@ struct EdgeMap {
 };
 
 // k_combine of the lower pass, or of the upper pass (upper != 0) for the
-// points of parts with bin edges only
+// points of parts with bin edges only: partial rows of V values, point rows of
+// stride RS
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_combine_if(int V, RowMap rm, const int* __restrict__ multi, int nmulti,
-                             const int* __restrict__ chstart, const int* __restrict__ pstart,
-                             const float* __restrict__ part, T* __restrict__ values, EdgeMap edge,
-                             int upper) {
+__global__ void __launch_bounds__(kBlock) k_combine_if(int V, int RS, RowMap rm, const int* __restrict__ multi,
+                                                       int nmulti, const int* __restrict__ chstart,
+                                                       const int* __restrict__ pstart, const float* __restrict__ part,
+                                                       T* __restrict__ values, EdgeMap edge, int upper) {
   if (upper && !edge.any()) return;
   const long nvb = (long)((nmulti + rm.gpw - 1) / rm.gpw * 64 + blockDim.x - 1) / blockDim.x;
   for (long vb = xcd_block(); vb < nvb; vb += gridDim.x) {
@@ -489,7 +490,7 @@ __global__ void __launch_bounds__(kBlock) k_combine_if(int V, RowMap rm, const i
     for (int k = k0; k < V; k += ks) {
       float acc = part[(size_t)c0 * V + k];
       for (int c = c0 + 1; c < c1; ++c) acc = acc + part[(size_t)c * V + k];
-      st_v(values, (size_t)(p + 1) * V + k, acc);
+      st_v(values, (size_t)(p + 1) * RS + k, acc);
     }
   }
 }
@@ -590,109 +591,120 @@ __device__ inline float rescaled(const float* Q, size_t idx, const float* mm) {
   return q;
 }
 
-// Ordered filter state: LO[point][label][10] holds the lower pass (+w to bins
-// 0..bin_lo, :548-569).  The upper pass (+w to bins bin_up..9) equals
-// T - LO[bin+1] (T = the point's total weight, lo[10] = 0) whenever
-// bin_up == bin_lo for every (pixel, label); the splat raises `edge` when
-// some q sits on a bin edge (bin_up == bin_lo - 1), and then the upper pass
-// is splatted and blurred in full into UP[point][label][10].
-template <int TAG, class T>  // TAG only names the instantiation (lattice dim, +100 for the upper pass)
-__global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ bins, int L, RowMap rm, long nvblocks,
-                                                   int upper,
-                         const int* __restrict__ sp_pix, const float* __restrict__ sp_w, const int* __restrict__ seg,
-                         const int* __restrict__ chstart, const int* __restrict__ chpoint,
-                         const int* __restrict__ pstart, int nchunks, T* __restrict__ values,
-                         float* __restrict__ part, EdgeMap edge) {
-  __shared__ int spx[kBlock];
-  __shared__ float swt[kBlock];
-  __shared__ int maxlen;
-  if (upper && !edge.any()) return;
+// Ordered filter state, bin-major rows of orow_stride(L) values per lattice
+// point.  The lower pass adds w to bins 0..bin_lo (:548-569), so its bin 0
+// holds every entry's weight: it equals the point's total weight Tb, which is
+// splatted and blurred with the same float operations in the same order, and
+// is not stored.  LO rows hold bins 1..9, bin b of label k at (b-1)*L + k.
+// The upper pass (+w to bins bin_up..9) equals Tb - LO[bin+1] whenever
+// bin_up == bin_lo for every (pixel, label); when some q of a part sits on a
+// bin edge (bin_up == bin_lo - 1) the upper pass of that part is splatted and
+// blurred in full into UP rows holding bins 0..8 (bin 9 is Tb), bin b at b*L + k.
+// Either way the slice finds the upper value of bin c at c*L + k.
+constexpr int kOBins = 9;      // stored bins per label
+constexpr int kOBatch = 8;     // splat entries whose loads are issued together
+
+// (point, label) per lane; the chunk's entries are read in batches of kOBatch
+// (all of a batch's loads in flight together) and added in entry order
+template <int TAG, class T>  // TAG names the instantiation: lattice dim, +100 for the upper pass
+__global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ bins, int L, int RS, RowMap rm,
+                                                   long nvblocks, const int* __restrict__ sp_pix,
+                                                   const float* __restrict__ sp_w, const int* __restrict__ seg,
+                                                   const int* __restrict__ chstart, const int* __restrict__ chpoint,
+                                                   const int* __restrict__ pstart, int nchunks, T* __restrict__ values,
+                                                   float* __restrict__ part, EdgeMap edge) {
+  constexpr bool kUpper = TAG >= 100;
+  if (kUpper && !edge.any()) return;
   for (long vb = xcd_block(); vb < nvblocks; vb += gridDim.x) {
-  long c;
-  int k0, ks;
-  rm.at_block(L, vb, c, k0, ks);
-  const bool act = c >= 0 && c < nchunks && (!upper || edge.point(chpoint[c]));
-  const int gb = (threadIdx.x & ~63) + ((threadIdx.x & 63) / L) * L;
-  int b = 0, e = 0, p = 0, c0 = 0, nc = 1;
-  if (act) {
-    p = chpoint[c];
-    c0 = chstart[p];
-    nc = chstart[p + 1] - c0;
-    b = seg[p] + (int)(c - c0) * kChunk;
-    e = min(seg[p + 1], b + kChunk);
-  }
-  float acc[10];
+    long c;
+    int k0, ks;
+    rm.at_block(L, vb, c, k0, ks);
+    if (c < 0 || c >= nchunks) continue;
+    const int p = chpoint[c];
+    if (kUpper && !edge.point(p)) continue;
+    const int c0 = chstart[p], nc = chstart[p + 1] - c0;
+    const int b = seg[p] + (int)(c - c0) * kChunk;
+    const int e = min(seg[p + 1], b + kChunk);
+    float acc[kOBins];
 #pragma unroll
-  for (int q = 0; q < 10; ++q) acc[q] = 0.f;
-  if (threadIdx.x == 0) maxlen = 0;
-  __syncthreads();
-  if (act && k0 == 0) atomicMax(&maxlen, e - b);
-  __syncthreads();
-  const int rounds = (maxlen + L - 1) / L;  // block-uniform: every thread reaches each barrier
-  for (int r = 0; r < rounds; ++r) {
-    const int t0 = b + r * L;
-    if (act) {
-      const int t = t0 + k0;
-      spx[gb + k0] = t < e ? sp_pix[t] : 0;
-      swt[gb + k0] = t < e ? sp_w[t] : 0.f;
-    }
-    __syncthreads();
-    if (act) {
-      const int n = min(L, e - t0);
-#pragma unroll 4
-      for (int j = 0; j < n; ++j) {
-        const float w = swt[gb + j];
-        const int bb = bins[(size_t)spx[gb + j] * L + k0];
-        const int lo = bb & 15, up = bb >> 4;
-        if (upper) {
+    for (int q = 0; q < kOBins; ++q) acc[q] = 0.f;
+    for (int t0 = b; t0 < e; t0 += kOBatch) {
+      int px[kOBatch];
+      float w[kOBatch];
 #pragma unroll
-          for (int r2 = 0; r2 < 10; ++r2) acc[r2] = acc[r2] + (r2 >= up ? w : 0.f);
-        } else {
+      for (int i = 0; i < kOBatch; ++i) {  // entries past the end repeat the last one with weight 0
+        const int t = min(t0 + i, e - 1);
+        px[i] = sp_pix[t];
+        w[i] = sp_w[t];
+        if (t0 + i >= e) w[i] = 0.f;
+      }
+      int bv[kOBatch];
 #pragma unroll
-          for (int r2 = 0; r2 < 10; ++r2) acc[r2] = acc[r2] + (r2 <= lo ? w : 0.f);
+      for (int i = 0; i < kOBatch; ++i) bv[i] = bins[(size_t)px[i] * L + k0];
+#pragma unroll
+      for (int i = 0; i < kOBatch; ++i) {
+        const int lo = bv[i] & 15, up = bv[i] >> 4;
+#pragma unroll
+        for (int r = 0; r < kOBins; ++r) {
+          if (kUpper)
+            acc[r] = acc[r] + (r >= up ? w[i] : 0.f);  // bins 0..8
+          else
+            acc[r] = acc[r] + (r + 1 <= lo ? w[i] : 0.f);  // bins 1..9
         }
       }
     }
-    __syncthreads();
-  }
-  if (act) {
     if (nc == 1) {  // the point's row
-      T* dst = values + ((size_t)(p + 1) * L + k0) * 10;
+      T* dst = values + (size_t)(p + 1) * RS;
 #pragma unroll
-      for (int q = 0; q < 5; ++q) st_pair(dst, q, make_float2(acc[2 * q], acc[2 * q + 1]));
+      for (int r = 0; r < kOBins; ++r) st_v(dst, (size_t)r * L + k0, acc[r]);
+      if (k0 == 0)
+        for (int x = kOBins * L; x < RS; ++x) st_v(dst, x, 0.f);  // row padding
     } else {  // a chunk partial (float), summed by k_combine_if
-      float* dst = part + ((size_t)(pstart[p] + (c - c0)) * L + k0) * 10;
+      float* dst = part + (size_t)(pstart[p] + (c - c0)) * (kOBins * L);
 #pragma unroll
-      for (int q = 0; q < 5; ++q) st_pair(dst, q, make_float2(acc[2 * q], acc[2 * q + 1]));
+      for (int r = 0; r < kOBins; ++r) dst[(size_t)r * L + k0] = acc[r];
     }
-  }
-  __syncthreads();  // maxlen / staging buffers are reused by the next virtual block
   }
 }
 
-// one blur axis over rows of R2 float2 (ordered state): one wave per lattice
-// point, lanes stride the row; the upper pass (upper != 0) only blurs the
-// points of parts with bin edges (a point's neighbours are in its own part)
+// four consecutive row values: float4, or four bfloat16 in a uint2
+__device__ inline float4 ld_quad(const float* p, size_t q) { return reinterpret_cast<const float4*>(p)[q]; }
+__device__ inline float4 ld_quad(const uint16_t* p, size_t q) {
+  const uint2 v = reinterpret_cast<const uint2*>(p)[q];
+  return make_float4(__uint_as_float(v.x << 16), __uint_as_float(v.x & 0xffff0000u), __uint_as_float(v.y << 16),
+                     __uint_as_float(v.y & 0xffff0000u));
+}
+__device__ inline void st_quad(float* p, size_t q, float4 v) { reinterpret_cast<float4*>(p)[q] = v; }
+__device__ inline void st_quad(uint16_t* p, size_t q, float4 v) {
+  uint2 o;
+  o.x = (uint32_t)f2bf(v.x) | ((uint32_t)f2bf(v.y) << 16);
+  o.y = (uint32_t)f2bf(v.z) | ((uint32_t)f2bf(v.w) << 16);
+  reinterpret_cast<uint2*>(p)[q] = o;
+}
+
+// one blur axis over the ordered state: one lane per (point, quad of the
+// row); the upper pass (upper != 0) only blurs the points of parts with bin
+// edges (a point's neighbours are in its own part)
 template <int TAG, class T>
-__global__ void __launch_bounds__(kBlock) k_blur2(int R2, int M, const int* __restrict__ nbr, const T* __restrict__ src,
-                        T* __restrict__ dst, EdgeMap edge, int upper) {
+__global__ void __launch_bounds__(kBlock) k_blur2(int R4, FastDiv divR4, uint32_t total, const int* __restrict__ nbr,
+                                                  const T* __restrict__ src, T* __restrict__ dst, EdgeMap edge,
+                                                  int upper) {
   if (upper && !edge.any()) return;
-  const int lane = threadIdx.x & 63;
-  const long nw = (long)gridDim.x * (blockDim.x >> 6);
-  for (long i = ((long)xcd_block() * blockDim.x + threadIdx.x) >> 6; i < M; i += nw) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t t = xcd_block() * blockDim.x + threadIdx.x; t < total; t += stride) {
+    const uint32_t i = divR4.div(t);
+    const uint32_t q = t - i * (uint32_t)R4;
     if (upper && !edge.point(i)) continue;
     const int n1 = nbr[2 * i] + 1, n2 = nbr[2 * i + 1] + 1;
-    const T* so = src + (size_t)(i + 1) * 2 * R2;
-    const T* s1 = src + (size_t)n1 * 2 * R2;
-    const T* s2 = src + (size_t)n2 * 2 * R2;
-    T* d = dst + (size_t)(i + 1) * 2 * R2;
-    for (int k = lane; k < R2; k += 64) {
-      const float2 o = ld_pair(so, k), a = ld_pair(s1, k), b = ld_pair(s2, k);
-      float2 r;
-      r.x = o.x + 0.5f * (a.x + b.x);
-      r.y = o.y + 0.5f * (a.y + b.y);
-      st_pair(d, k, r);
-    }
+    const float4 o = ld_quad(src, (size_t)(i + 1) * R4 + q);
+    const float4 a = ld_quad(src, (size_t)n1 * R4 + q);
+    const float4 b = ld_quad(src, (size_t)n2 * R4 + q);
+    float4 r;
+    r.x = o.x + 0.5f * (a.x + b.x);
+    r.y = o.y + 0.5f * (a.y + b.y);
+    r.z = o.z + 0.5f * (a.z + b.z);
+    r.w = o.w + 0.5f * (a.w + b.w);
+    st_quad(dst, (size_t)(i + 1) * R4 + q, r);
   }
 }
 
@@ -718,56 +730,62 @@ __global__ void __launch_bounds__(kBlock) k_obin(const float* __restrict__ Q, co
 }
 
 // lower slice then upper slice (:638-650, :686-696), then out (+)= scale*x.
-// Without edges the upper value at bin c is T - LO[c+1].
+// Every vertex's loads are issued before the sums, which keep the
+// reference's order (all lower terms, then all upper terms).  TAG 2 and 5
+// fix the vertex count at compile time; TAG 0 reads it from d1.
 template <int TAG, class T>
-__global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ bins, int L, RowMap rm, int N, int d1,
-                         const int* __restrict__ offset, const float* __restrict__ bary,
-                         const T* __restrict__ lo_v, const T* __restrict__ up_v, const float* __restrict__ Tb,
-                         EdgeMap edge, float alpha, SliceArgs o) {
-  __shared__ int soff[kBlock];
-  __shared__ float sbw[kBlock];
+__global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ bins, int L, int RS, RowMap rm, int N,
+                                                   int d1, const int* __restrict__ offset,
+                                                   const float* __restrict__ bary, const T* __restrict__ lo_v,
+                                                   const T* __restrict__ up_v, const float* __restrict__ Tb,
+                                                   EdgeMap edge, float alpha, SliceArgs o) {
+  constexpr int kD1 = TAG == 2 ? 3 : TAG == 5 ? 6 : 0;
+  constexpr int NV = kD1 ? kD1 : kMaxD + 1;
+  const int nv = kD1 ? kD1 : d1;
   long pix;
   int k0, ks;
   rm.at(L, pix, k0, ks);
-  const bool act = pix >= 0 && pix < N;
-  const int gb = (threadIdx.x & ~63) + ((threadIdx.x & 63) / L) * L;
-  const bool staged = L >= d1;  // otherwise the group has too few lanes/slots
-  if (staged && act && k0 < d1) {
-    soff[gb + k0] = offset[(size_t)pix * d1 + k0] + 1;
-    sbw[gb + k0] = bary[(size_t)pix * d1 + k0];
-  }
-  __syncthreads();
-  if (!act) return;
-  auto voff = [&](int j) { return staged ? soff[gb + j] : offset[(size_t)pix * d1 + j] + 1; };
-  auto vw = [&](int j) { return staged ? sbw[gb + j] : bary[(size_t)pix * d1 + j]; };
+  if (pix < 0 || pix >= N) return;
   const bool full = edge.pixel(pix);
+  const T* urow = full ? up_v : lo_v;
   const int bb = bins[(size_t)pix * L + k0];
   const int lo = bb & 15, up = bb >> 4;
-  // one pass over the vertices; the upper terms wait in registers so the sum
-  // keeps the reference's order (all lower terms, then all upper terms)
-  constexpr int kMaxD1 = kMaxD + 1;
-  float ut[kMaxD1];
-  float acc = 0.f;
+  const int ilo = (max(lo, 1) - 1) * L + k0;  // LO[lo] (bin 0 is Tb)
+  const int iup = min(up, 8) * L + k0;        // LO[up + 1], or UP[up] (bin 9 is Tb)
+  int of[NV];
+  float w[NV];
 #pragma unroll
-  for (int j = 0; j < kMaxD1; ++j) {
-    if (j < d1) {
-      const int of = voff(j);
-      const float w = vw(j);
-      const T* row = lo_v + ((size_t)of * L + k0) * 10;
-      acc = acc + ld_v(row, lo) * (-w * alpha);
-      float u;
-      if (full)
-        u = ld_v(up_v, ((size_t)of * L + k0) * 10 + up);
-      else
-        u = Tb[of] - (up < 9 ? ld_v(row, up + 1) : 0.f);
-      ut[j] = u * (w * alpha);
+  for (int j = 0; j < NV; ++j) {
+    if (j < nv) {
+      of[j] = offset[(size_t)pix * nv + j] + 1;
+      w[j] = bary[(size_t)pix * nv + j];
     }
   }
+  float lv[NV], uv[NV];
 #pragma unroll
-  for (int j = 0; j < kMaxD1; ++j)
-    if (j < d1) acc = acc + ut[j];
+  for (int j = 0; j < NV; ++j) {
+    if (j < nv) {
+      const size_t base = (size_t)of[j] * RS;
+      const float t = Tb[of[j]];
+      const float a = ld_v(lo_v, base + ilo);
+      const float u = ld_v(urow, base + iup);
+      lv[j] = lo == 0 ? t : a;
+      uv[j] = full ? (up == 9 ? t : u) : t - (up < 9 ? u : 0.f);
+    }
+  }
+  float acc = 0.f;
+#pragma unroll
+  for (int j = 0; j < NV; ++j)
+    if (j < nv) acc = acc + lv[j] * (-w[j] * alpha);
+#pragma unroll
+  for (int j = 0; j < NV; ++j)
+    if (j < nv) acc = acc + uv[j] * (w[j] * alpha);
   slice_store(o, (int)pix, k0, L, acc);
 }
+
+// values per lattice point row of the ordered state (bins 1..9 or 0..8 of L
+// labels), padded to a multiple of 4 for the blur's quads
+inline int orow_stride(int L) { return (kOBins * L + 3) & ~3; }
 
 __global__ void k_zero_int(int* p) { *p = 0; }
 __global__ void k_zero_ints(int* p, int n) {
@@ -1058,8 +1076,8 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   DCRF_REQUIRE(L > 0, "ordered filter: label count must be positive");
   DCRF_REQUIRE(L <= 64, "ordered filter: at most 64 labels");
   ensure_total_weight();
-  const int row = L * 10;
-  ensure_values(2 * row);  // LO and UP halves of each buffer (float-sized; bf16 rows use half)
+  const int row = orow_stride(L);
+  ensure_values(2 * (size_t)row);  // LO and UP halves of each buffer (float-sized; bf16 rows use half)
   const size_t half = (size_t)(M_ + 1) * row;  // elements of T
   hipStream_t s = stream_;
   const RowMap rm = rowmap(L);
@@ -1091,7 +1109,7 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
       T* dst_half = upper ? srcU : srcL;
       const long nvb = rows_grid(nchunks_, L);
       auto launch = [&](auto kern, unsigned grid) {
-        kern<<<grid, kBlock, 0, s>>>(bins_.get(), L, rm, nvb, upper, sp_pix_.get(), sp_w_.get(), seg_.get(),
+        kern<<<grid, kBlock, 0, s>>>(bins_.get(), L, row, rm, nvb, sp_pix_.get(), sp_w_.get(), seg_.get(),
                                      chstart_.get(), chpoint_.get(), pstart_.get(), nchunks_, dst_half, part_.get(),
                                      em);
       };
@@ -1111,25 +1129,29 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
       else
         run(k_osplat<0, T>, k_osplat<100, T>);
       if (nmulti_ > 0) {
-        const RowMap rw = rowmap(row);
-        const unsigned g = upper ? std::min(rows_grid(nmulti_, row), 1024u) : rows_grid(nmulti_, row);
-        k_combine_if<T><<<g, kBlock, 0, s>>>(row, rw, multi_.get(), nmulti_, chstart_.get(), pstart_.get(),
+        const int pv = kOBins * L;  // values of a partial row
+        const RowMap rw = rowmap(pv);
+        const unsigned g = upper ? std::min(rows_grid(nmulti_, pv), 1024u) : rows_grid(nmulti_, pv);
+        k_combine_if<T><<<g, kBlock, 0, s>>>(pv, row, rw, multi_.get(), nmulti_, chstart_.get(), pstart_.get(),
                                              part_.get(), dst_half, em, upper);
       }
     }
     check_launch("k_osplat");
   }
-  const int R2 = row / 2;
+  const int R4 = row / 4;
+  const size_t nquads = (size_t)M_ * R4;
+  DCRF_REQUIRE(nquads < (1u << 31), "ordered filter: lattice too large for one blur launch");
+  const FastDiv divR4((uint32_t)R4);
   auto blur = [&](auto kern, const int* nb, const T* a, T* b, int upper) {
-    const unsigned full = grid_for((size_t)M_ * 64);
+    const unsigned full = grid_for(nquads);
     unsigned g = full;
     if (upper && nparts_ == 1)  // the upper pass is usually skipped whole: a capped grid keeps its no-op cheap
       g = std::min(full, 2048u);
     else if (!upper)  // src -> dst, the same bits for any grid: tuned
       g = Tuner::get().pick(d_ == 5 ? "oblur5" : "oblur", M_, grid_candidates(full), s, [&](unsigned gg) {
-        kern<<<gg, kBlock, 0, s>>>(R2, M_, nb, a, b, em, upper);
+        kern<<<gg, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, nb, a, b, em, upper);
       });
-    kern<<<g, kBlock, 0, s>>>(R2, M_, nb, a, b, em, upper);
+    kern<<<g, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, nb, a, b, em, upper);
   };
   for (int j = 0; j <= d_; ++j) {  // forward order only (:621, :669)
     const int* nb = nbr_.get() + (size_t)j * M_ * 2;
@@ -1157,8 +1179,8 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   }
   KTimer tm("oslice" + tag, s);
   auto slice = [&](auto kern) {
-    kern<<<rows_grid(N_, L), kBlock, 0, s>>>(bins_.get(), L, rm, N_, d_ + 1, offset_.get(), bary_.get(), srcL, srcU,
-                                             tb_.get(), em, lattice_alpha(d_), out);
+    kern<<<rows_grid(N_, L), kBlock, 0, s>>>(bins_.get(), L, row, rm, N_, d_ + 1, offset_.get(), bary_.get(), srcL,
+                                             srcU, tb_.get(), em, lattice_alpha(d_), out);
   };
   if (d_ == 2)
     slice(k_oslice<2, T>);
