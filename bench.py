@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -49,15 +50,18 @@ def algorithmic_bytes(name: str, N: int, M: dict, V: int = L, S: int = 4) -> flo
     if kind == "slice":
         return S * m * V + 8 * e + 2 * S * N * V + 4 * N
     # ordered filter: bins are one byte per (pixel, label); the lattice state
-    # holds 10 lower bins per (point, label); the upper pass is T - lower
+    # holds 9 stored bins per (point, label) (bin 0 is the point's total
+    # weight); the upper pass is T - lower
     if kind == "obin":
         return S * N * V + N * V
+    if kind == "ogather":
+        return 4 * e + 2 * e * V
     if kind == "osplat":
-        return N * V + 8 * e + S * m * 10 * V
+        return e * V + 4 * e + S * m * 9 * V
     if kind == "oblur":
-        return 2 * S * m * 10 * V + 8 * m
+        return 2 * S * m * 9 * V + 8 * m
     if kind == "oslice":
-        return N * V + 8 * e + S * m * 10 * V + S * m + 2 * S * N * V
+        return N * V + 8 * e + S * m * 9 * V + S * m + 2 * S * N * V
     return float("nan")
 
 
@@ -359,7 +363,7 @@ def main():
             traffic = json.load(fh)
     # dominant lattice kernel by total time
     lat_k = {n: v for n, v in prof.items() if "[" in n}
-    dom = max(lat_k, key=lambda n: lat_k[n][1])
+    dom = max((n for n in lat_k if math.isfinite(algorithmic_bytes(n, wl.N, Ms))), key=lambda n: lat_k[n][1])
     cnt, tot = lat_k[dom]
     avg_ms = tot / cnt
     ab = algorithmic_bytes(dom, wl.N, Ms)

@@ -602,67 +602,155 @@ __device__ inline float rescaled(const float* Q, size_t idx, const float* mm) {
 // blurred in full into UP rows holding bins 0..8 (bin 9 is Tb), bin b at b*L + k.
 // Either way the slice finds the upper value of bin c at c*L + k.
 constexpr int kOBins = 9;      // stored bins per label
-constexpr int kOBatch = 8;     // splat entries whose loads are issued together
+constexpr int kOPad = 16;      // chunk entries padded to a multiple of this for the ordered splat
 
-// (point, label) per lane; the chunk's entries are read in batches of kOBatch
-// (all of a batch's loads in flight together) and added in entry order
-template <int TAG, class T>  // TAG names the instantiation: lattice dim, +100 for the upper pass
-__global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ bins, int L, int RS, RowMap rm,
-                                                   long nvblocks, const int* __restrict__ sp_pix,
-                                                   const float* __restrict__ sp_w, const int* __restrict__ seg,
+// Static per lattice, for the ordered splat: every chunk's entries padded to
+// a multiple of kOPad (cwoff / cplen per chunk), their pixels (pixT; padding
+// repeats the chunk's last pixel) and weights (wT; padding has weight 0), and
+// per quad of padded entries its chunk's {cwoff, cplen} (qinfo).
+__global__ void __launch_bounds__(kBlock) k_oplen(const int* __restrict__ seg, const int* __restrict__ chstart,
+                                                  const int* __restrict__ chpoint, int nchunks, int* __restrict__ plen) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > nchunks) return;
+  if (c == nchunks) {  // the scan's total
+    plen[c] = 0;
+    return;
+  }
+  const int p = chpoint[c];
+  const int b = seg[p] + (c - chstart[p]) * kChunk;
+  const int e = min(seg[p + 1], b + kChunk);
+  plen[c] = (e - b + kOPad - 1) / kOPad * kOPad;
+}
+__global__ void __launch_bounds__(kBlock) k_ofill(const int* __restrict__ seg, const int* __restrict__ chstart,
+                                                  const int* __restrict__ chpoint, int nchunks,
+                                                  const int* __restrict__ cwoff, const int* __restrict__ cplen,
+                                                  const int* __restrict__ sp_pix, const float* __restrict__ sp_w,
+                                                  int* __restrict__ pixT, float* __restrict__ wT,
+                                                  int2* __restrict__ qinfo) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nchunks) return;
+  const int p = chpoint[c];
+  const int b = seg[p] + (c - chstart[p]) * kChunk;
+  const int e = min(seg[p + 1], b + kChunk);
+  const int wo = cwoff[c], pl = cplen[c];
+  for (int j = 0; j < pl; ++j) {
+    const int t = min(b + j, e - 1);
+    pixT[wo + j] = sp_pix[t];
+    wT[wo + j] = b + j < e ? sp_w[t] : 0.f;
+    if ((j & 3) == 0) qinfo[(wo + j) >> 2] = make_int2(wo, pl);
+  }
+}
+
+// Static per lattice, for the ordered slice: each (pixel, vertex) as
+// {point row (int bits), barycentric weight, the point's total weight, 0}
+__global__ void __launch_bounds__(kBlock) k_ovtx(const int* __restrict__ offset, const float* __restrict__ bary,
+                                                 const float* __restrict__ Tb, long E, float4* __restrict__ vtx) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= E) return;
+  const int of = offset[t] + 1;
+  vtx[t] = make_float4(__int_as_float(of), bary[t], Tb[of], 0.f);
+}
+
+// Per call: the bins of every padded entry, per chunk label-major
+// (ebT[cwoff*L + k*cplen + j] = bins of entry j's pixel, label k), so that a
+// splat lane reads kOPad entries of its label with one 16-byte load.  One
+// thread per quad of entries reads the quad's four bins rows whole (rows of
+// LP bytes, LP a multiple of 8) and writes each label's four bytes as one word.
+template <int LT>  // label count, or 0: read from L (at most 64)
+__global__ void __launch_bounds__(kBlock) k_otranspose(const uint8_t* __restrict__ bins, int L, int LP,
+                                                       const int* __restrict__ pixT, const int2* __restrict__ qinfo,
+                                                       long nquads, uint8_t* __restrict__ ebT) {
+  constexpr int KMAX = LT ? LT : 64;
+  constexpr int HMAX = (KMAX + 7) / 8;  // uint2 per bins row
+  const int Lr = LT ? LT : L;
+  const int LPr = LT ? (LT + 7) / 8 * 8 : LP;
+  const int H = LPr / 8;
+  for (long qi = (long)blockIdx.x * blockDim.x + threadIdx.x; qi < nquads; qi += (long)gridDim.x * blockDim.x) {
+    const int2 inf = qinfo[qi];
+    const int4 px = reinterpret_cast<const int4*>(pixT)[qi];
+    const int pxs[4] = {px.x, px.y, px.z, px.w};
+    uint2 R[4][HMAX];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const uint2* rp = reinterpret_cast<const uint2*>(bins + (size_t)pxs[r] * LPr);
+#pragma unroll
+      for (int h = 0; h < HMAX; ++h) R[r][h] = h < H ? rp[h] : make_uint2(0u, 0u);
+    }
+    uint8_t* dst = ebT + (size_t)inf.x * Lr + (4 * qi - inf.x);
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      if (k < Lr) {
+        const int h = k >> 3, sh = (k & 3) * 8;
+        uint32_t v = 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const uint32_t wd = (k & 4) ? R[r][h].y : R[r][h].x;
+          v |= ((wd >> sh) & 0xffu) << (8 * r);
+        }
+        *reinterpret_cast<uint32_t*>(dst + (size_t)k * inf.y) = v;
+      }
+    }
+  }
+}
+
+// (point, label) per lane: the chunk's padded entries in groups of kOPad (one
+// 16-byte load of bins, four of weights), added in entry order; padding adds 0
+template <int TAG, class T, int LT>  // TAG: lattice dim, +100 for the upper pass; LT: label count or 0
+__global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ ebT, const float* __restrict__ wT,
+                                                   const int* __restrict__ cwoff, const int* __restrict__ cplen,
+                                                   int L, int RS, RowMap rm, long nvblocks,
                                                    const int* __restrict__ chstart, const int* __restrict__ chpoint,
                                                    const int* __restrict__ pstart, int nchunks, T* __restrict__ values,
                                                    float* __restrict__ part, EdgeMap edge) {
   constexpr bool kUpper = TAG >= 100;
+  const int Lr = LT ? LT : L;
   if (kUpper && !edge.any()) return;
   for (long vb = xcd_block(); vb < nvblocks; vb += gridDim.x) {
     long c;
     int k0, ks;
-    rm.at_block(L, vb, c, k0, ks);
+    rm.at_block(Lr, vb, c, k0, ks);
     if (c < 0 || c >= nchunks) continue;
     const int p = chpoint[c];
     if (kUpper && !edge.point(p)) continue;
     const int c0 = chstart[p], nc = chstart[p + 1] - c0;
-    const int b = seg[p] + (int)(c - c0) * kChunk;
-    const int e = min(seg[p + 1], b + kChunk);
+    const int wo = cwoff[c], pl = cplen[c];
+    const uint8_t* eb = ebT + (size_t)wo * Lr + (size_t)k0 * pl;
+    const float* wp = wT + wo;
     float acc[kOBins];
 #pragma unroll
     for (int q = 0; q < kOBins; ++q) acc[q] = 0.f;
-    for (int t0 = b; t0 < e; t0 += kOBatch) {
-      int px[kOBatch];
-      float w[kOBatch];
+    for (int j0 = 0; j0 < pl; j0 += kOPad) {
+      const uint4 bq = *reinterpret_cast<const uint4*>(eb + j0);
+      const float4 w0 = reinterpret_cast<const float4*>(wp + j0)[0];
+      const float4 w1 = reinterpret_cast<const float4*>(wp + j0)[1];
+      const float4 w2 = reinterpret_cast<const float4*>(wp + j0)[2];
+      const float4 w3 = reinterpret_cast<const float4*>(wp + j0)[3];
+      const uint32_t wd[4] = {bq.x, bq.y, bq.z, bq.w};
+      const float wv[kOPad] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w,
+                               w2.x, w2.y, w2.z, w2.w, w3.x, w3.y, w3.z, w3.w};
 #pragma unroll
-      for (int i = 0; i < kOBatch; ++i) {  // entries past the end repeat the last one with weight 0
-        const int t = min(t0 + i, e - 1);
-        px[i] = sp_pix[t];
-        w[i] = sp_w[t];
-        if (t0 + i >= e) w[i] = 0.f;
-      }
-      int bv[kOBatch];
-#pragma unroll
-      for (int i = 0; i < kOBatch; ++i) bv[i] = bins[(size_t)px[i] * L + k0];
-#pragma unroll
-      for (int i = 0; i < kOBatch; ++i) {
-        const int lo = bv[i] & 15, up = bv[i] >> 4;
+      for (int i = 0; i < kOPad; ++i) {
+        const int bv = (wd[i >> 2] >> ((i & 3) * 8)) & 0xff;
+        const int lo = bv & 15, up = bv >> 4;
 #pragma unroll
         for (int r = 0; r < kOBins; ++r) {
           if (kUpper)
-            acc[r] = acc[r] + (r >= up ? w[i] : 0.f);  // bins 0..8
+            acc[r] = acc[r] + (r >= up ? wv[i] : 0.f);  // bins 0..8
           else
-            acc[r] = acc[r] + (r + 1 <= lo ? w[i] : 0.f);  // bins 1..9
+            acc[r] = acc[r] + (r + 1 <= lo ? wv[i] : 0.f);  // bins 1..9
         }
       }
     }
     if (nc == 1) {  // the point's row
       T* dst = values + (size_t)(p + 1) * RS;
 #pragma unroll
-      for (int r = 0; r < kOBins; ++r) st_v(dst, (size_t)r * L + k0, acc[r]);
+      for (int r = 0; r < kOBins; ++r) st_v(dst, (size_t)r * Lr + k0, acc[r]);
       if (k0 == 0)
-        for (int x = kOBins * L; x < RS; ++x) st_v(dst, x, 0.f);  // row padding
+        for (int x = kOBins * Lr; x < RS; ++x) st_v(dst, x, 0.f);  // row padding
     } else {  // a chunk partial (float), summed by k_combine_if
-      float* dst = part + (size_t)(pstart[p] + (c - c0)) * (kOBins * L);
+      float* dst = part + (size_t)(pstart[p] + (c - c0)) * (kOBins * Lr);
 #pragma unroll
-      for (int r = 0; r < kOBins; ++r) dst[(size_t)r * L + k0] = acc[r];
+      for (int r = 0; r < kOBins; ++r) dst[(size_t)r * Lr + k0] = acc[r];
     }
   }
 }
@@ -708,23 +796,25 @@ __global__ void __launch_bounds__(kBlock) k_blur2(int R4, FastDiv divR4, uint32_
   }
 }
 
-// bins of every (pixel, label) of the rescaled field, packed lo | up << 4;
-// edge = 1 if some bin_up != bin_lo (then the upper pass is done in full)
+// bins of every (pixel, label) of the rescaled field, packed lo | up << 4,
+// in rows of LP bytes per pixel; edge = 1 if some bin_up != bin_lo (then the
+// upper pass is done in full)
 // A lattice that is the union of several images' lattices rescales each
 // image with its own {min, max}: pimg[pixel] picks the pair mm[2*img].
 __global__ void __launch_bounds__(kBlock) k_obin(const float* __restrict__ Q, const float* __restrict__ mm, size_t n,
-                       uint8_t* __restrict__ bins, int* __restrict__ edge, const uint16_t* __restrict__ pimg,
+                       uint8_t* __restrict__ bins, int LP, int* __restrict__ edge, const uint16_t* __restrict__ pimg,
                        FastDiv divL, int parts) {
   bool e = false;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const int img = pimg ? pimg[divL.div((uint32_t)i)] : 0;
+    const uint32_t pix = divL.div((uint32_t)i);
+    const int img = pimg ? pimg[pix] : 0;
     const float q = rescaled(Q, i, mm + 2 * img);
     const int lo = bin_lo(q), up = bin_up(q);
     if (lo != up) {
       e = true;
       if (edge[img] == 0) atomicOr(edge + img, 1);
     }
-    bins[i] = (uint8_t)(lo | (up << 4));
+    bins[(size_t)pix * LP + (i - (size_t)pix * divL.d)] = (uint8_t)(lo | (up << 4));
   }
   if (__any(e) && (threadIdx.x & 63) == 0) atomicOr(edge + parts, 1);
 }
@@ -734,10 +824,9 @@ __global__ void __launch_bounds__(kBlock) k_obin(const float* __restrict__ Q, co
 // reference's order (all lower terms, then all upper terms).  TAG 2 and 5
 // fix the vertex count at compile time; TAG 0 reads it from d1.
 template <int TAG, class T>
-__global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ bins, int L, int RS, RowMap rm, int N,
-                                                   int d1, const int* __restrict__ offset,
-                                                   const float* __restrict__ bary, const T* __restrict__ lo_v,
-                                                   const T* __restrict__ up_v, const float* __restrict__ Tb,
+__global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ bins, int L, int LP, int RS, RowMap rm,
+                                                   int N, int d1, const float4* __restrict__ vtx,
+                                                   const T* __restrict__ lo_v, const T* __restrict__ up_v,
                                                    EdgeMap edge, float alpha, SliceArgs o) {
   constexpr int kD1 = TAG == 2 ? 3 : TAG == 5 ? 6 : 0;
   constexpr int NV = kD1 ? kD1 : kMaxD + 1;
@@ -748,25 +837,20 @@ __global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ b
   if (pix < 0 || pix >= N) return;
   const bool full = edge.pixel(pix);
   const T* urow = full ? up_v : lo_v;
-  const int bb = bins[(size_t)pix * L + k0];
+  const int bb = bins[(size_t)pix * LP + k0];
   const int lo = bb & 15, up = bb >> 4;
   const int ilo = (max(lo, 1) - 1) * L + k0;  // LO[lo] (bin 0 is Tb)
   const int iup = min(up, 8) * L + k0;        // LO[up + 1], or UP[up] (bin 9 is Tb)
-  int of[NV];
-  float w[NV];
+  float4 vt[NV];
 #pragma unroll
-  for (int j = 0; j < NV; ++j) {
-    if (j < nv) {
-      of[j] = offset[(size_t)pix * nv + j] + 1;
-      w[j] = bary[(size_t)pix * nv + j];
-    }
-  }
+  for (int j = 0; j < NV; ++j)
+    if (j < nv) vt[j] = vtx[(size_t)pix * nv + j];
   float lv[NV], uv[NV];
 #pragma unroll
   for (int j = 0; j < NV; ++j) {
     if (j < nv) {
-      const size_t base = (size_t)of[j] * RS;
-      const float t = Tb[of[j]];
+      const size_t base = (size_t)__float_as_int(vt[j].x) * RS;
+      const float t = vt[j].z;
       const float a = ld_v(lo_v, base + ilo);
       const float u = ld_v(urow, base + iup);
       lv[j] = lo == 0 ? t : a;
@@ -776,16 +860,18 @@ __global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ b
   float acc = 0.f;
 #pragma unroll
   for (int j = 0; j < NV; ++j)
-    if (j < nv) acc = acc + lv[j] * (-w[j] * alpha);
+    if (j < nv) acc = acc + lv[j] * (-vt[j].y * alpha);
 #pragma unroll
   for (int j = 0; j < NV; ++j)
-    if (j < nv) acc = acc + uv[j] * (w[j] * alpha);
+    if (j < nv) acc = acc + uv[j] * (vt[j].y * alpha);
   slice_store(o, (int)pix, k0, L, acc);
 }
 
 // values per lattice point row of the ordered state (bins 1..9 or 0..8 of L
 // labels), padded to a multiple of 4 for the blur's quads
 inline int orow_stride(int L) { return (kOBins * L + 3) & ~3; }
+// bytes per pixel row of the bins (a multiple of 8, read as uint2)
+inline int obin_stride(int L) { return (L + 7) & ~7; }
 
 __global__ void k_zero_int(int* p) { *p = 0; }
 __global__ void k_zero_ints(int* p, int n) {
@@ -1063,6 +1149,35 @@ void Lattice::ensure_total_weight() {
   have_tb_ = true;
 }
 
+// static tables of the ordered filter (padded chunk layout for the splat,
+// vertex records for the slice), built at the lattice's first ordered call
+void Lattice::ensure_ordered_tables() {
+  if (have_otab_) return;
+  hipStream_t s = stream_;
+  cplen_.resize(nchunks_ + 1);
+  cwoff_.resize(nchunks_ + 1);
+  k_oplen<<<grid_for(nchunks_ + 1), kBlock, 0, s>>>(seg_.get(), chstart_.get(), chpoint_.get(), nchunks_,
+                                                     cplen_.get());
+  check_launch("k_oplen");
+  scan_excl(cplen_.get(), cwoff_.get(), nchunks_ + 1, s);
+  int tot = 0;
+  DCRF_HIP(hipMemcpy(&tot, cwoff_.get() + nchunks_, sizeof(int), hipMemcpyDeviceToHost));
+  otot_ = tot;
+  pixT_.resize(std::max(tot, 4));
+  wT_.resize(std::max(tot, 4));
+  qinfo_.resize(std::max(tot / 4, 1));
+  k_ofill<<<grid_for(nchunks_), kBlock, 0, s>>>(seg_.get(), chstart_.get(), chpoint_.get(), nchunks_, cwoff_.get(),
+                                                 cplen_.get(), sp_pix_.get(), sp_w_.get(), pixT_.get(), wT_.get(),
+                                                 qinfo_.get());
+  check_launch("k_ofill");
+  const long E = (long)N_ * (d_ + 1);
+  vtx_.resize(std::max<long>(E, 1));
+  k_ovtx<<<grid_for(E), kBlock, 0, s>>>(offset_.get(), bary_.get(), tb_.get(), E, vtx_.get());
+  check_launch("k_ovtx");
+  DCRF_HIP(hipStreamSynchronize(s));
+  have_otab_ = true;
+}
+
 void Lattice::ordered(const float* Q, const float* d_minmax, int L, const SliceArgs& out) {
   if (bf16_)
     ordered_t<uint16_t>(Q, d_minmax, L, out);
@@ -1095,13 +1210,25 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   const EdgeMap em{edge, pimg_.n ? pimg_.get() : nullptr, ppimg_.n ? ppimg_.get() : nullptr, nparts_};
   const std::string tag = prof_on() ? "[d=" + std::to_string(d_) + ",L=" + std::to_string(L) + "]" : std::string();
   const size_t NL = (size_t)N_ * L;
-  bins_.resize(NL);
+  const int LP = obin_stride(L);
+  bins_.resize((size_t)N_ * LP);
   {
     KTimer tm("obin" + tag, s);
     k_zero_ints<<<1, kBlock, 0, s>>>(edge, nparts_ + 1);
-    k_obin<<<grid_for(NL, kBlock, 2048), kBlock, 0, s>>>(Q, d_minmax, NL, bins_.get(), edge,
+    k_obin<<<grid_for(NL, kBlock, 2048), kBlock, 0, s>>>(Q, d_minmax, NL, bins_.get(), LP, edge,
                                                          pimg_.n ? pimg_.get() : nullptr, FastDiv((uint32_t)L),
                                                          nparts_);
+  }
+  ensure_ordered_tables();
+  {
+    KTimer tm("otranspose" + tag, s);
+    ebT_.resize((size_t)otot_ * L);
+    const long nq = otot_ / 4;
+    if (L == 21)
+      k_otranspose<21><<<grid_for(nq), kBlock, 0, s>>>(bins_.get(), L, LP, pixT_.get(), qinfo_.get(), nq, ebT_.get());
+    else
+      k_otranspose<0><<<grid_for(nq), kBlock, 0, s>>>(bins_.get(), L, LP, pixT_.get(), qinfo_.get(), nq, ebT_.get());
+    check_launch("k_otranspose");
   }
   {
     KTimer tm("osplat" + tag, s);
@@ -1109,7 +1236,7 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
       T* dst_half = upper ? srcU : srcL;
       const long nvb = rows_grid(nchunks_, L);
       auto launch = [&](auto kern, unsigned grid) {
-        kern<<<grid, kBlock, 0, s>>>(bins_.get(), L, row, rm, nvb, sp_pix_.get(), sp_w_.get(), seg_.get(),
+        kern<<<grid, kBlock, 0, s>>>(ebT_.get(), wT_.get(), cwoff_.get(), cplen_.get(), L, row, rm, nvb,
                                      chstart_.get(), chpoint_.get(), pstart_.get(), nchunks_, dst_half, part_.get(),
                                      em);
       };
@@ -1122,12 +1249,16 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
           launch(kl, g);
         }
       };
-      if (d_ == 2)
-        run(k_osplat<2, T>, k_osplat<102, T>);
+      if (d_ == 5 && L == 21)
+        run(k_osplat<5, T, 21>, k_osplat<105, T, 21>);
+      else if (d_ == 2 && L == 21)
+        run(k_osplat<2, T, 21>, k_osplat<102, T, 21>);
       else if (d_ == 5)
-        run(k_osplat<5, T>, k_osplat<105, T>);
+        run(k_osplat<5, T, 0>, k_osplat<105, T, 0>);
+      else if (d_ == 2)
+        run(k_osplat<2, T, 0>, k_osplat<102, T, 0>);
       else
-        run(k_osplat<0, T>, k_osplat<100, T>);
+        run(k_osplat<0, T, 0>, k_osplat<100, T, 0>);
       if (nmulti_ > 0) {
         const int pv = kOBins * L;  // values of a partial row
         const RowMap rw = rowmap(pv);
@@ -1179,8 +1310,8 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   }
   KTimer tm("oslice" + tag, s);
   auto slice = [&](auto kern) {
-    kern<<<rows_grid(N_, L), kBlock, 0, s>>>(bins_.get(), L, row, rm, N_, d_ + 1, offset_.get(), bary_.get(), srcL,
-                                             srcU, tb_.get(), em, lattice_alpha(d_), out);
+    kern<<<rows_grid(N_, L), kBlock, 0, s>>>(bins_.get(), L, LP, row, rm, N_, d_ + 1, vtx_.get(), srcL, srcU, em,
+                                             lattice_alpha(d_), out);
   };
   if (d_ == 2)
     slice(k_oslice<2, T>);
