@@ -404,6 +404,19 @@ __global__ void __launch_bounds__(kBlock) k_blur4(int V4, int M, const int* __re
   }
 }
 
+// slice_store with its two reads (norm, previous output) issued earlier by the caller
+struct SlicePre {
+  float norm, prev;
+};
+__device__ inline SlicePre slice_pre(const SliceArgs& o, int pix, int k, int V) {
+  return SlicePre{o.norm ? o.norm[pix] : 1.f, o.accumulate ? o.out[(size_t)pix * V + k] : 0.f};
+}
+__device__ inline void slice_store_pre(const SliceArgs& o, int pix, int k, int V, float x, const SlicePre& pre) {
+  if (o.norm) x = x * pre.norm;
+  if (o.apply_scale) x = o.scale * x;
+  if (o.post_mult != 1.f) x = o.post_mult * x;
+  o.out[(size_t)pix * V + k] = o.accumulate ? pre.prev + x : x;
+}
 __device__ inline void slice_store(const SliceArgs& o, int pix, int k, int V, float x) {
   if (o.norm) x = x * o.norm[pix];
   if (o.apply_scale) x = o.scale * x;
@@ -835,6 +848,7 @@ __global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ b
   int k0, ks;
   rm.at(L, pix, k0, ks);
   if (pix < 0 || pix >= N) return;
+  const SlicePre pre = slice_pre(o, (int)pix, k0, L);
   const bool full = edge.pixel(pix);
   const T* urow = full ? up_v : lo_v;
   const int bb = bins[(size_t)pix * LP + k0];
@@ -864,7 +878,7 @@ __global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ b
 #pragma unroll
   for (int j = 0; j < NV; ++j)
     if (j < nv) acc = acc + uv[j] * (vt[j].y * alpha);
-  slice_store(o, (int)pix, k0, L, acc);
+  slice_store_pre(o, (int)pix, k0, L, acc, pre);
 }
 
 // values per lattice point row of the ordered state (bins 1..9 or 0..8 of L

@@ -32,30 +32,37 @@ constexpr int kSweepBatch = 16;
 // sum of the first L entries of a register array in Eigen's order for an
 // aligned vector (first aligned element 0): blocks of 8 into two 4-wide
 // accumulators, one more block of 4 if present, (p0+p2)+(p1+p3), then the tail
+template <int MAXL, class F>
+__device__ inline float eigen_sum_fn(int L, F v);
 template <int MAXL>
-__device__ inline float eigen_sum_reg(const float (&v)[MAXL], int L) {
+__device__ inline float eigen_sum_reg(const float (&a)[MAXL], int L) {
+  return eigen_sum_fn<MAXL>(L, [&](int i) { return a[i]; });
+}
+// the same sum over v(0..L-1), the values produced on the fly
+template <int MAXL, class F>
+__device__ inline float eigen_sum_fn(int L, F v) {
   const int as2 = (L / 8) * 8, as1 = (L / 4) * 4;
   if (as1 == 0) {
-    float r = v[0];
+    float r = v(0);
 #pragma unroll
     for (int i = 1; i < MAXL; ++i)
-      if (i < L) r = r + v[i];
+      if (i < L) r = r + v(i);
     return r;
   }
-  float p0 = v[0], p1 = v[1], p2 = v[2], p3 = v[3];
+  float p0 = v(0), p1 = v(1), p2 = v(2), p3 = v(3);
   if (as1 > 4) {
-    float q0 = v[4], q1 = v[5], q2 = v[6], q3 = v[7];
+    float q0 = v(4), q1 = v(5), q2 = v(6), q3 = v(7);
 #pragma unroll
     for (int b = 8; b + 7 < MAXL; b += 8)
       if (b < as2) {
-        p0 = p0 + v[b];
-        p1 = p1 + v[b + 1];
-        p2 = p2 + v[b + 2];
-        p3 = p3 + v[b + 3];
-        q0 = q0 + v[b + 4];
-        q1 = q1 + v[b + 5];
-        q2 = q2 + v[b + 6];
-        q3 = q3 + v[b + 7];
+        p0 = p0 + v(b);
+        p1 = p1 + v(b + 1);
+        p2 = p2 + v(b + 2);
+        p3 = p3 + v(b + 3);
+        q0 = q0 + v(b + 4);
+        q1 = q1 + v(b + 5);
+        q2 = q2 + v(b + 6);
+        q3 = q3 + v(b + 7);
       }
     p0 = p0 + q0;
     p1 = p1 + q1;
@@ -65,17 +72,17 @@ __device__ inline float eigen_sum_reg(const float (&v)[MAXL], int L) {
 #pragma unroll
       for (int b = 8; b + 3 < MAXL; b += 8)
         if (b == as2) {
-          p0 = p0 + v[b];
-          p1 = p1 + v[b + 1];
-          p2 = p2 + v[b + 2];
-          p3 = p3 + v[b + 3];
+          p0 = p0 + v(b);
+          p1 = p1 + v(b + 1);
+          p2 = p2 + v(b + 2);
+          p3 = p3 + v(b + 3);
         }
     }
   }
   float r = (p0 + p2) + (p1 + p3);
 #pragma unroll
   for (int i = 4; i < MAXL; ++i)
-    if (i >= as1 && i < L) r = r + v[i];
+    if (i >= as1 && i < L) r = r + v(i);
   return r;
 }
 template <int MAXL>
@@ -133,25 +140,27 @@ __global__ void __launch_bounds__(kBlock) k_lp_Y(SEG sg, int L, LPConst c, float
 
 // one qp-gamma sweep of a pixel (qp_gamma_step :495-514, qp_gamma_multiplyC
 // :517-523, qp_values :667); returns the pixel's qp value
+// The row of Y is held as its positive and negative parts nh = max(y, 0),
+// ph = max(-y, 0) (the terms of the update); y = nh - ph where the qp value
+// needs it (exact: one of the two is zero).
 template <int MAXL>
-__device__ inline __attribute__((always_inline)) float qp_sweep(float (&g)[MAXL], const float (&y)[MAXL], int L,
-                                                                const LPConst& c) {
+__device__ inline __attribute__((always_inline)) float qp_sweep(float (&g)[MAXL], const float (&ph)[MAXL],
+                                                                const float (&nh)[MAXL], int L, const LPConst& c) {
   const float sum = eigen_sum_reg<MAXL>(g, L);
 #pragma unroll
   for (int l = 0; l < MAXL; ++l)
     if (l < L) {
-      const float ph = fmaxf(-y[l], 0.f), nh = fmaxf(y[l], 0.f);
       const float t = (sum - g[l]) * c.c_2l_over_M;
-      const float stp = (t + ph) + c.qp_delta;
+      const float stp = (t + ph[l]) + c.qp_delta;
       float t2 = (c.c_1_over_M * sum + c.c_1m2_over_M * g[l]) * c.lambda;
-      t2 = (t2 + nh) + c.qp_delta;
+      t2 = (t2 + nh[l]) + c.qp_delta;
       g[l] = g[l] * (stp / t2);
     }
   const float s2 = eigen_sum_reg<MAXL>(g, L);
-  float vt[MAXL];
-#pragma unroll
-  for (int l = 0; l < MAXL; ++l) vt[l] = l < L ? (c.c_m1_over_M * s2 + g[l]) * c.lambda : 0.f;
-  return (float)(0.5 * (double)eigen_dot_reg<MAXL>(g, vt, L) + (double)eigen_dot_reg<MAXL>(g, y, L));
+  const float cs2 = c.c_m1_over_M * s2;
+  const float d1 = eigen_sum_fn<MAXL>(L, [&](int l) { return g[l] * ((cs2 + g[l]) * c.lambda); });
+  const float d2 = eigen_sum_fn<MAXL>(L, [&](int l) { return g[l] * (nh[l] - ph[l]); });
+  return (float)(0.5 * (double)d1 + (double)d2);
 }
 
 // qp-gamma sweeps (:650-675), one thread per pixel holding its row in
@@ -165,7 +174,7 @@ __device__ inline __attribute__((always_inline)) float qp_sweep(float (&g)[MAXL]
 //   carries gamma over (gin -> gout) in pass 1.  Block b works on pixels
 //   [lb * kBlock, lb * kBlock + kBlock) of its image.
 template <int MAXL, int LT = 0, class SEG>  // LT > 0: the label count is known at compile time
-__global__ void __launch_bounds__(kBlock) k_lp_sweeps(SEG sg, int Lr, LPConst c, const float* __restrict__ gin, float* __restrict__ gout,
+__global__ void __launch_bounds__(kBlock, 6) k_lp_sweeps(SEG sg, int Lr, LPConst c, const float* __restrict__ gin, float* __restrict__ gout,
                             const float* __restrict__ Y, int nsw, int replay, double* __restrict__ parts,
                             const int* __restrict__ fl_all) {
   __shared__ float tile[kBlock * MAXL];
@@ -183,7 +192,8 @@ __global__ void __launch_bounds__(kBlock) k_lp_sweeps(SEG sg, int Lr, LPConst c,
   }
   const int n = replay ? fl[kLF_SWEEPS] : nsw;
   const bool act = t < np;
-  float g[MAXL], y[MAXL];
+  __shared__ double red[kSweepBatch][kBlock / 64];  // per sweep and wave: wave sum of the qp values
+  float g[MAXL], ph[MAXL], nh[MAXL];
   for (int i = t; i < np * L; i += kBlock) tile[i] = gin[base + i];
   __syncthreads();
 #pragma unroll
@@ -192,22 +202,25 @@ __global__ void __launch_bounds__(kBlock) k_lp_sweeps(SEG sg, int Lr, LPConst c,
   for (int i = t; i < np * L; i += kBlock) tile[i] = Y[base + i];
   __syncthreads();
 #pragma unroll
-  for (int l = 0; l < MAXL; ++l) y[l] = (act && l < L) ? tile[t * L + l] : 0.f;
-  double acc[kSweepBatch];
-#pragma unroll
-  for (int sw = 0; sw < kSweepBatch; ++sw) acc[sw] = 0.0;
-  for (int sw = 0; sw < n; ++sw) {
-    const float v = act ? qp_sweep<MAXL>(g, y, L, c) : 0.f;
-#pragma unroll
-    for (int q = 0; q < kSweepBatch; ++q)
-      if (q == sw) acc[q] = (double)v;
+  for (int l = 0; l < MAXL; ++l) {
+    const float y = (act && l < L) ? tile[t * L + l] : 0.f;
+    ph[l] = fmaxf(-y, 0.f);
+    nh[l] = fmaxf(y, 0.f);
   }
-  if (!replay) {
-    block_sum<kSweepBatch>(acc);
-    if (t == 0)
-      for (int sw = 0; sw < n; ++sw) parts[(size_t)sw * gridDim.x + blockIdx.x] = acc[sw];
+  for (int sw = 0; sw < n; ++sw) {
+    const float v = act ? qp_sweep<MAXL>(g, ph, nh, L, c) : 0.f;
+    if (!replay) {
+      const double ws = wave_sum((double)v);
+      if ((t & 63) == 0) red[sw][t >> 6] = ws;
+    }
   }
   __syncthreads();
+  if (!replay && t == 0)  // block sum per sweep: the waves' sums in wave order
+    for (int sw = 0; sw < n; ++sw) {
+      double bs = 0;
+      for (int w = 0; w < kBlock / 64; ++w) bs += red[sw][w];
+      parts[(size_t)sw * gridDim.x + blockIdx.x] = bs;
+    }
 #pragma unroll
   for (int l = 0; l < MAXL; ++l)
     if (act && l < L) tile[t * L + l] = g[l];
