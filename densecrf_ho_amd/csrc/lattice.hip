@@ -15,6 +15,7 @@ namespace {
 
 constexpr int kMaxD = 12;
 constexpr int kChunk = 64;  // splat entries summed sequentially per chunk
+constexpr int kOBatch = 8;  // ordered splat entries whose loads are issued together
 
 // ---- packed keys: 4 int16 coordinates per 64-bit word ---------------------
 template <int D>
@@ -248,7 +249,12 @@ __global__ void __launch_bounds__(kBlock) k_iota(int* __restrict__ v, int n) {
 __global__ void __launch_bounds__(kBlock) k_fill_sp(const int* __restrict__ ent, const float* __restrict__ bary, int E, int d1,
                           int* __restrict__ sp_pix, float* __restrict__ sp_w) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= E) return;
+  if (t >= E + kOBatch) return;
+  if (t >= E) {  // padding entries
+    sp_pix[t] = 0;
+    sp_w[t] = 0.f;
+    return;
+  }
   const int e = ent[t];
   sp_pix[t] = e / d1;
   sp_w[t] = bary[e];
@@ -615,44 +621,6 @@ __device__ inline float rescaled(const float* Q, size_t idx, const float* mm) {
 // blurred in full into UP rows holding bins 0..8 (bin 9 is Tb), bin b at b*L + k.
 // Either way the slice finds the upper value of bin c at c*L + k.
 constexpr int kOBins = 9;      // stored bins per label
-constexpr int kOPad = 16;      // chunk entries padded to a multiple of this for the ordered splat
-
-// Static per lattice, for the ordered splat: every chunk's entries padded to
-// a multiple of kOPad (cwoff / cplen per chunk), their pixels (pixT; padding
-// repeats the chunk's last pixel) and weights (wT; padding has weight 0), and
-// per quad of padded entries its chunk's {cwoff, cplen} (qinfo).
-__global__ void __launch_bounds__(kBlock) k_oplen(const int* __restrict__ seg, const int* __restrict__ chstart,
-                                                  const int* __restrict__ chpoint, int nchunks, int* __restrict__ plen) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c > nchunks) return;
-  if (c == nchunks) {  // the scan's total
-    plen[c] = 0;
-    return;
-  }
-  const int p = chpoint[c];
-  const int b = seg[p] + (c - chstart[p]) * kChunk;
-  const int e = min(seg[p + 1], b + kChunk);
-  plen[c] = (e - b + kOPad - 1) / kOPad * kOPad;
-}
-__global__ void __launch_bounds__(kBlock) k_ofill(const int* __restrict__ seg, const int* __restrict__ chstart,
-                                                  const int* __restrict__ chpoint, int nchunks,
-                                                  const int* __restrict__ cwoff, const int* __restrict__ cplen,
-                                                  const int* __restrict__ sp_pix, const float* __restrict__ sp_w,
-                                                  int* __restrict__ pixT, float* __restrict__ wT,
-                                                  int2* __restrict__ qinfo) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nchunks) return;
-  const int p = chpoint[c];
-  const int b = seg[p] + (c - chstart[p]) * kChunk;
-  const int e = min(seg[p + 1], b + kChunk);
-  const int wo = cwoff[c], pl = cplen[c];
-  for (int j = 0; j < pl; ++j) {
-    const int t = min(b + j, e - 1);
-    pixT[wo + j] = sp_pix[t];
-    wT[wo + j] = b + j < e ? sp_w[t] : 0.f;
-    if ((j & 3) == 0) qinfo[(wo + j) >> 2] = make_int2(wo, pl);
-  }
-}
 
 // Static per lattice, for the ordered slice: each (pixel, vertex) as
 // {point row (int bits), barycentric weight, the point's total weight, 0}
@@ -664,106 +632,79 @@ __global__ void __launch_bounds__(kBlock) k_ovtx(const int* __restrict__ offset,
   vtx[t] = make_float4(__int_as_float(of), bary[t], Tb[of], 0.f);
 }
 
-// Per call: the bins of every padded entry, per chunk label-major
-// (ebT[cwoff*L + k*cplen + j] = bins of entry j's pixel, label k), so that a
-// splat lane reads kOPad entries of its label with one 16-byte load.  One
-// thread per quad of entries reads the quad's four bins rows whole (rows of
-// LP bytes, LP a multiple of 8) and writes each label's four bytes as one word.
-template <int LT>  // label count, or 0: read from L (at most 64)
-__global__ void __launch_bounds__(kBlock) k_otranspose(const uint8_t* __restrict__ bins, int L, int LP,
-                                                       const int* __restrict__ pixT, const int2* __restrict__ qinfo,
-                                                       long nquads, uint8_t* __restrict__ ebT) {
-  constexpr int KMAX = LT ? LT : 64;
-  constexpr int HMAX = (KMAX + 7) / 8;  // uint2 per bins row
-  const int Lr = LT ? LT : L;
-  const int LPr = LT ? (LT + 7) / 8 * 8 : LP;
-  const int H = LPr / 8;
-  for (long qi = (long)blockIdx.x * blockDim.x + threadIdx.x; qi < nquads; qi += (long)gridDim.x * blockDim.x) {
-    const int2 inf = qinfo[qi];
-    const int4 px = reinterpret_cast<const int4*>(pixT)[qi];
-    const int pxs[4] = {px.x, px.y, px.z, px.w};
-    uint2 R[4][HMAX];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const uint2* rp = reinterpret_cast<const uint2*>(bins + (size_t)pxs[r] * LPr);
-#pragma unroll
-      for (int h = 0; h < HMAX; ++h) R[r][h] = h < H ? rp[h] : make_uint2(0u, 0u);
-    }
-    uint8_t* dst = ebT + (size_t)inf.x * Lr + (4 * qi - inf.x);
-#pragma unroll
-    for (int k = 0; k < KMAX; ++k) {
-      if (k < Lr) {
-        const int h = k >> 3, sh = (k & 3) * 8;
-        uint32_t v = 0;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const uint32_t wd = (k & 4) ? R[r][h].y : R[r][h].x;
-          v |= ((wd >> sh) & 0xffu) << (8 * r);
-        }
-        *reinterpret_cast<uint32_t*>(dst + (size_t)k * inf.y) = v;
-      }
-    }
-  }
-}
-
-// (point, label) per lane: the chunk's padded entries in groups of kOPad (one
-// 16-byte load of bins, four of weights), added in entry order; padding adds 0
-template <int TAG, class T, int LT>  // TAG: lattice dim, +100 for the upper pass; LT: label count or 0
-__global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ ebT, const float* __restrict__ wT,
-                                                   const int* __restrict__ cwoff, const int* __restrict__ cplen,
-                                                   int L, int RS, RowMap rm, long nvblocks,
+// One lane per (point chunk, quad of labels): each entry's bins of the four
+// labels are one 4-byte word of its pixel's bins row (rows of LP bytes, LP a
+// multiple of 8; labels past L are row padding, computed and dropped).  The
+// chunk's entries are read in batches of kOBatch (all of a batch's loads in
+// flight together; the entry lists hold kOBatch entries of padding, pixel 0
+// with weight 0, and entries past the chunk's end get weight 0) and added in
+// entry order.
+template <int TAG, class T>  // TAG: lattice dim, +100 for the upper pass
+__global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ bins, int L, int LP, int RS,
+                                                   long nlanes, const int* __restrict__ sp_pix,
+                                                   const float* __restrict__ sp_w, const int* __restrict__ seg,
                                                    const int* __restrict__ chstart, const int* __restrict__ chpoint,
                                                    const int* __restrict__ pstart, int nchunks, T* __restrict__ values,
                                                    float* __restrict__ part, EdgeMap edge) {
   constexpr bool kUpper = TAG >= 100;
-  const int Lr = LT ? LT : L;
   if (kUpper && !edge.any()) return;
-  for (long vb = xcd_block(); vb < nvblocks; vb += gridDim.x) {
-    long c;
-    int k0, ks;
-    rm.at_block(Lr, vb, c, k0, ks);
-    if (c < 0 || c >= nchunks) continue;
+  const int nq = (L + 3) >> 2;
+  for (long g = (long)xcd_block() * blockDim.x + threadIdx.x; g < nlanes; g += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(g / nq), q = (int)(g - (long)c * nq);
     const int p = chpoint[c];
     if (kUpper && !edge.point(p)) continue;
     const int c0 = chstart[p], nc = chstart[p + 1] - c0;
-    const int wo = cwoff[c], pl = cplen[c];
-    const uint8_t* eb = ebT + (size_t)wo * Lr + (size_t)k0 * pl;
-    const float* wp = wT + wo;
-    float acc[kOBins];
+    const int b = seg[p] + (c - c0) * kChunk;
+    const int e = min(seg[p + 1], b + kChunk);
+    float acc[4][kOBins];
 #pragma unroll
-    for (int q = 0; q < kOBins; ++q) acc[q] = 0.f;
-    for (int j0 = 0; j0 < pl; j0 += kOPad) {
-      const uint4 bq = *reinterpret_cast<const uint4*>(eb + j0);
-      const float4 w0 = reinterpret_cast<const float4*>(wp + j0)[0];
-      const float4 w1 = reinterpret_cast<const float4*>(wp + j0)[1];
-      const float4 w2 = reinterpret_cast<const float4*>(wp + j0)[2];
-      const float4 w3 = reinterpret_cast<const float4*>(wp + j0)[3];
-      const uint32_t wd[4] = {bq.x, bq.y, bq.z, bq.w};
-      const float wv[kOPad] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w,
-                               w2.x, w2.y, w2.z, w2.w, w3.x, w3.y, w3.z, w3.w};
+    for (int kk = 0; kk < 4; ++kk)
 #pragma unroll
-      for (int i = 0; i < kOPad; ++i) {
-        const int bv = (wd[i >> 2] >> ((i & 3) * 8)) & 0xff;
-        const int lo = bv & 15, up = bv >> 4;
+      for (int r = 0; r < kOBins; ++r) acc[kk][r] = 0.f;
+    for (int t0 = b; t0 < e; t0 += kOBatch) {
+      int px[kOBatch];
+      float w[kOBatch];
 #pragma unroll
-        for (int r = 0; r < kOBins; ++r) {
-          if (kUpper)
-            acc[r] = acc[r] + (r >= up ? wv[i] : 0.f);  // bins 0..8
-          else
-            acc[r] = acc[r] + (r + 1 <= lo ? wv[i] : 0.f);  // bins 1..9
+      for (int i = 0; i < kOBatch; ++i) {
+        px[i] = sp_pix[t0 + i];
+        w[i] = sp_w[t0 + i];
+      }
+      uint32_t bw[kOBatch];
+#pragma unroll
+      for (int i = 0; i < kOBatch; ++i) bw[i] = *reinterpret_cast<const uint32_t*>(bins + (size_t)px[i] * LP + 4 * q);
+#pragma unroll
+      for (int i = 0; i < kOBatch; ++i) {
+        const float wi = t0 + i < e ? w[i] : 0.f;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+          const int bv = (bw[i] >> (8 * kk)) & 0xff;
+          const int lo = bv & 15, up = bv >> 4;
+#pragma unroll
+          for (int r = 0; r < kOBins; ++r) {
+            if (kUpper)
+              acc[kk][r] = acc[kk][r] + (r >= up ? wi : 0.f);  // bins 0..8
+            else
+              acc[kk][r] = acc[kk][r] + (r + 1 <= lo ? wi : 0.f);  // bins 1..9
+          }
         }
       }
     }
     if (nc == 1) {  // the point's row
       T* dst = values + (size_t)(p + 1) * RS;
 #pragma unroll
-      for (int r = 0; r < kOBins; ++r) st_v(dst, (size_t)r * Lr + k0, acc[r]);
-      if (k0 == 0)
-        for (int x = kOBins * Lr; x < RS; ++x) st_v(dst, x, 0.f);  // row padding
-    } else {  // a chunk partial (float), summed by k_combine_if
-      float* dst = part + (size_t)(pstart[p] + (c - c0)) * (kOBins * Lr);
+      for (int kk = 0; kk < 4; ++kk)
+        if (4 * q + kk < L)
 #pragma unroll
-      for (int r = 0; r < kOBins; ++r) dst[(size_t)r * Lr + k0] = acc[r];
+          for (int r = 0; r < kOBins; ++r) st_v(dst, (size_t)r * L + 4 * q + kk, acc[kk][r]);
+      if (q == 0)
+        for (int x = kOBins * L; x < RS; ++x) st_v(dst, x, 0.f);  // row padding
+    } else {  // a chunk partial (float), summed by k_combine_if
+      float* dst = part + (size_t)(pstart[p] + (c - c0)) * (kOBins * L);
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk)
+        if (4 * q + kk < L)
+#pragma unroll
+          for (int r = 0; r < kOBins; ++r) dst[(size_t)r * L + 4 * q + kk] = acc[kk][r];
     }
   }
 }
@@ -784,28 +725,32 @@ __device__ inline void st_quad(uint16_t* p, size_t q, float4 v) {
 }
 
 // one blur axis over the ordered state: one lane per (point, quad of the
-// row); the upper pass (upper != 0) only blurs the points of parts with bin
-// edges (a point's neighbours are in its own part)
+// row), over the lower rows and, when some part has bin edges, the upper rows
+// (half_q quads further) of the points of those parts (a point's neighbours
+// are in its own part)
 template <int TAG, class T>
-__global__ void __launch_bounds__(kBlock) k_blur2(int R4, FastDiv divR4, uint32_t total, const int* __restrict__ nbr,
-                                                  const T* __restrict__ src, T* __restrict__ dst, EdgeMap edge,
-                                                  int upper) {
-  if (upper && !edge.any()) return;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t t = xcd_block() * blockDim.x + threadIdx.x; t < total; t += stride) {
-    const uint32_t i = divR4.div(t);
-    const uint32_t q = t - i * (uint32_t)R4;
+__global__ void __launch_bounds__(kBlock) k_blur2(int R4, FastDiv divR4, uint32_t nq, size_t half_q,
+                                                  const int* __restrict__ nbr, const T* __restrict__ src,
+                                                  T* __restrict__ dst, EdgeMap edge) {
+  const uint64_t total = edge.any() ? 2ull * nq : nq;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t = (uint64_t)xcd_block() * blockDim.x + threadIdx.x; t < total; t += stride) {
+    const bool upper = t >= nq;
+    const uint32_t tt = (uint32_t)(upper ? t - nq : t);
+    const uint32_t i = divR4.div(tt);
+    const uint32_t q = tt - i * (uint32_t)R4;
     if (upper && !edge.point(i)) continue;
+    const size_t h = upper ? half_q : 0;
     const int n1 = nbr[2 * i] + 1, n2 = nbr[2 * i + 1] + 1;
-    const float4 o = ld_quad(src, (size_t)(i + 1) * R4 + q);
-    const float4 a = ld_quad(src, (size_t)n1 * R4 + q);
-    const float4 b = ld_quad(src, (size_t)n2 * R4 + q);
+    const float4 o = ld_quad(src, h + (size_t)(i + 1) * R4 + q);
+    const float4 a = ld_quad(src, h + (size_t)n1 * R4 + q);
+    const float4 b = ld_quad(src, h + (size_t)n2 * R4 + q);
     float4 r;
     r.x = o.x + 0.5f * (a.x + b.x);
     r.y = o.y + 0.5f * (a.y + b.y);
     r.z = o.z + 0.5f * (a.z + b.z);
     r.w = o.w + 0.5f * (a.w + b.w);
-    st_quad(dst, (size_t)(i + 1) * R4 + q, r);
+    st_quad(dst, h + (size_t)(i + 1) * R4 + q, r);
   }
 }
 
@@ -1023,9 +968,10 @@ void Lattice::build(const float* d_feat, int N, int d) {
                                                 ent_sorted.get(), E, 0, bits, s));
     DCRF_HIP(hipStreamSynchronize(s));
   }
-  sp_pix_.resize(E);
-  sp_w_.resize(E);
-  k_fill_sp<<<grid_for(E), kBlock, 0, s>>>(ent_sorted.get(), bary_.get(), E, d1, sp_pix_.get(), sp_w_.get());
+  sp_pix_.resize(E + kOBatch);  // padding read by the ordered splat's last batch: pixel 0, weight 0
+  sp_w_.resize(E + kOBatch);
+  k_fill_sp<<<grid_for(E + kOBatch), kBlock, 0, s>>>(ent_sorted.get(), bary_.get(), E, d1, sp_pix_.get(),
+                                                     sp_w_.get());
   check_launch("k_fill_sp");
 
   DBuf<int> nch(M + 1), mflag(M + 1), mpos(M + 1), pn(M + 1);
@@ -1163,27 +1109,11 @@ void Lattice::ensure_total_weight() {
   have_tb_ = true;
 }
 
-// static tables of the ordered filter (padded chunk layout for the splat,
-// vertex records for the slice), built at the lattice's first ordered call
+// static vertex records of the ordered slice, built at the lattice's first
+// ordered call (after the total weight)
 void Lattice::ensure_ordered_tables() {
   if (have_otab_) return;
   hipStream_t s = stream_;
-  cplen_.resize(nchunks_ + 1);
-  cwoff_.resize(nchunks_ + 1);
-  k_oplen<<<grid_for(nchunks_ + 1), kBlock, 0, s>>>(seg_.get(), chstart_.get(), chpoint_.get(), nchunks_,
-                                                     cplen_.get());
-  check_launch("k_oplen");
-  scan_excl(cplen_.get(), cwoff_.get(), nchunks_ + 1, s);
-  int tot = 0;
-  DCRF_HIP(hipMemcpy(&tot, cwoff_.get() + nchunks_, sizeof(int), hipMemcpyDeviceToHost));
-  otot_ = tot;
-  pixT_.resize(std::max(tot, 4));
-  wT_.resize(std::max(tot, 4));
-  qinfo_.resize(std::max(tot / 4, 1));
-  k_ofill<<<grid_for(nchunks_), kBlock, 0, s>>>(seg_.get(), chstart_.get(), chpoint_.get(), nchunks_, cwoff_.get(),
-                                                 cplen_.get(), sp_pix_.get(), sp_w_.get(), pixT_.get(), wT_.get(),
-                                                 qinfo_.get());
-  check_launch("k_ofill");
   const long E = (long)N_ * (d_ + 1);
   vtx_.resize(std::max<long>(E, 1));
   k_ovtx<<<grid_for(E), kBlock, 0, s>>>(offset_.get(), bary_.get(), tb_.get(), E, vtx_.get());
@@ -1235,44 +1165,31 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   }
   ensure_ordered_tables();
   {
-    KTimer tm("otranspose" + tag, s);
-    ebT_.resize((size_t)otot_ * L);
-    const long nq = otot_ / 4;
-    if (L == 21)
-      k_otranspose<21><<<grid_for(nq), kBlock, 0, s>>>(bins_.get(), L, LP, pixT_.get(), qinfo_.get(), nq, ebT_.get());
-    else
-      k_otranspose<0><<<grid_for(nq), kBlock, 0, s>>>(bins_.get(), L, LP, pixT_.get(), qinfo_.get(), nq, ebT_.get());
-    check_launch("k_otranspose");
-  }
-  {
     KTimer tm("osplat" + tag, s);
     for (int upper = 0; upper < 2; ++upper) {
       T* dst_half = upper ? srcU : srcL;
-      const long nvb = rows_grid(nchunks_, L);
+      const long nlanes = (long)nchunks_ * ((L + 3) / 4);
+      const unsigned nvb = grid_for((size_t)nlanes);
       auto launch = [&](auto kern, unsigned grid) {
-        kern<<<grid, kBlock, 0, s>>>(ebT_.get(), wT_.get(), cwoff_.get(), cplen_.get(), L, row, rm, nvb,
+        kern<<<grid, kBlock, 0, s>>>(bins_.get(), L, LP, row, nlanes, sp_pix_.get(), sp_w_.get(), seg_.get(),
                                      chstart_.get(), chpoint_.get(), pstart_.get(), nchunks_, dst_half, part_.get(),
                                      em);
       };
       auto run = [&](auto kl, auto ku) {
         if (upper) {
-          launch(ku, (unsigned)std::min<long>(nvb, 2048));
+          launch(ku, std::min(nvb, 2048u));
         } else {  // the lower pass writes every row: tuned grid of its grid-stride loop
-          const unsigned g = Tuner::get().pick(d_ == 5 ? "osplat5" : "osplat", nvb, grid_candidates((unsigned)nvb), s,
+          const unsigned g = Tuner::get().pick(d_ == 5 ? "osplat5" : "osplat", nvb, grid_candidates(nvb), s,
                                                [&](unsigned gg) { launch(kl, gg); });
           launch(kl, g);
         }
       };
-      if (d_ == 5 && L == 21)
-        run(k_osplat<5, T, 21>, k_osplat<105, T, 21>);
-      else if (d_ == 2 && L == 21)
-        run(k_osplat<2, T, 21>, k_osplat<102, T, 21>);
-      else if (d_ == 5)
-        run(k_osplat<5, T, 0>, k_osplat<105, T, 0>);
+      if (d_ == 5)
+        run(k_osplat<5, T>, k_osplat<105, T>);
       else if (d_ == 2)
-        run(k_osplat<2, T, 0>, k_osplat<102, T, 0>);
+        run(k_osplat<2, T>, k_osplat<102, T>);
       else
-        run(k_osplat<0, T, 0>, k_osplat<100, T, 0>);
+        run(k_osplat<0, T>, k_osplat<100, T>);
       if (nmulti_ > 0) {
         const int pv = kOBins * L;  // values of a partial row
         const RowMap rw = rowmap(pv);
@@ -1287,36 +1204,25 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   const size_t nquads = (size_t)M_ * R4;
   DCRF_REQUIRE(nquads < (1u << 31), "ordered filter: lattice too large for one blur launch");
   const FastDiv divR4((uint32_t)R4);
-  auto blur = [&](auto kern, const int* nb, const T* a, T* b, int upper) {
+  const size_t half_q = half / 4;  // quads from a lower row to its upper row
+  auto blur = [&](auto kern, const int* nb, const T* a, T* b) {
     const unsigned full = grid_for(nquads);
-    unsigned g = full;
-    if (upper && nparts_ == 1)  // the upper pass is usually skipped whole: a capped grid keeps its no-op cheap
-      g = std::min(full, 2048u);
-    else if (!upper)  // src -> dst, the same bits for any grid: tuned
-      g = Tuner::get().pick(d_ == 5 ? "oblur5" : "oblur", M_, grid_candidates(full), s, [&](unsigned gg) {
-        kern<<<gg, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, nb, a, b, em, upper);
-      });
-    kern<<<g, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, nb, a, b, em, upper);
+    // src -> dst, the same bits for any grid: tuned on the lower rows
+    const unsigned g = Tuner::get().pick(d_ == 5 ? "oblur5" : "oblur", M_, grid_candidates(full), s, [&](unsigned gg) {
+      kern<<<gg, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, half_q, nb, a, b, em);
+    });
+    kern<<<g, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, half_q, nb, a, b, em);
   };
   for (int j = 0; j <= d_; ++j) {  // forward order only (:621, :669)
     const int* nb = nbr_.get() + (size_t)j * M_ * 2;
     {
       KTimer tm("oblur" + tag, s);
       if (d_ == 2)
-        blur(k_blur2<2, T>, nb, srcL, dstL, 0);
+        blur(k_blur2<2, T>, nb, srcL, dstL);
       else if (d_ == 5)
-        blur(k_blur2<5, T>, nb, srcL, dstL, 0);
+        blur(k_blur2<5, T>, nb, srcL, dstL);
       else
-        blur(k_blur2<0, T>, nb, srcL, dstL, 0);
-    }
-    {
-      KTimer tm("oblur_upper" + tag, s);
-      if (d_ == 2)
-        blur(k_blur2<102, T>, nb, srcU, dstU, 1);
-      else if (d_ == 5)
-        blur(k_blur2<105, T>, nb, srcU, dstU, 1);
-      else
-        blur(k_blur2<100, T>, nb, srcU, dstU, 1);
+        blur(k_blur2<0, T>, nb, srcL, dstL);
     }
     check_launch("k_blur2");
     std::swap(srcL, dstL);
@@ -1376,8 +1282,10 @@ std::unique_ptr<Lattice> Lattice::concat(const std::vector<const Lattice*>& part
   u->offset_.resize(E);
   u->bary_.resize(E);
   u->nbr_.resize((size_t)d1 * M * 2);
-  u->sp_pix_.resize(E);
-  u->sp_w_.resize(E);
+  u->sp_pix_.resize(E + kOBatch);  // padding read by the ordered splat's last batch: pixel 0, weight 0
+  u->sp_w_.resize(E + kOBatch);
+  DCRF_HIP(hipMemsetAsync(u->sp_pix_.get() + E, 0, kOBatch * sizeof(int), s));
+  DCRF_HIP(hipMemsetAsync(u->sp_w_.get() + E, 0, kOBatch * sizeof(float), s));
   u->seg_.resize(M + 1);
   u->chstart_.resize(M + 1);
   u->pstart_.resize(M + 1);

@@ -81,8 +81,8 @@ class Lattice {
   // wider rows reallocates them, which invalidates captured launch sequences
   void work_buffers(std::vector<const void*>& out) const {
     for (const void* p : {(const void*)vA_.get(), (const void*)vB_.get(), (const void*)part_.get(),
-                          (const void*)bins_.get(), (const void*)ebT_.get(), (const void*)tb_.get(),
-                          (const void*)edge_.get(), (const void*)vtx_.get(), (const void*)wT_.get()})
+                          (const void*)bins_.get(), (const void*)tb_.get(), (const void*)edge_.get(),
+                          (const void*)vtx_.get()})
       out.push_back(p);
   }
 
@@ -123,18 +123,11 @@ class Lattice {
   DBuf<float> tb_;
   DBuf<int> edge_;
   DBuf<uint8_t> bins_;
-  // ordered filter, static per lattice (built at its first call): chunk
-  // entries padded for 16-byte loads (per chunk offset / padded length, the
-  // pixels and weights, per quad of entries its chunk's {offset, length}) and
-  // each (pixel, vertex) as {point row, weight, total weight}
+  // ordered slice: each (pixel, vertex) as {point row, weight, total weight},
+  // built at the lattice's first ordered call
   void ensure_ordered_tables();
   bool have_otab_ = false;
-  long otot_ = 0;
-  DBuf<int> cwoff_, cplen_, pixT_;
-  DBuf<float> wT_;
-  DBuf<int2> qinfo_;
   DBuf<float4> vtx_;
-  DBuf<uint8_t> ebT_;  // per call: bins of the padded entries, label-major per chunk
   DBuf<uint16_t> pimg_;   // part of each pixel (union lattices only)
   DBuf<uint16_t> ppimg_;  // part of each lattice point (union lattices only)
   int nparts_ = 1;
