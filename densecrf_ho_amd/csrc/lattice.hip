@@ -478,12 +478,25 @@ __device__ inline void st_pair(uint16_t* row, long k, float2 v) {
 // Bin-edge flags of the ordered filter, one per part of the lattice (one
 // image, or each image of a union) plus flag[parts] = "any part".  A part
 // whose flag is 0 takes the upper pass from T - LO[c+1] and skips its own.
+// Bin-edge bookkeeping of one ordered-filter call, in an int array after the
+// per-part flags: flag[parts + kE_*].  A (pixel, label) value on a bin edge
+// has bin_up = bin_lo - 1; its (label, bin_lo) pair is its "combo".  With at
+// most kMaxCombo combos and kMaxEdgeList edge values the upper pass is
+// Tb - LO[c+1] + D[c+1], D the filtered weight of the edge values of combo
+// (label, c+1) ("correction" mode, mode() in 1..kMaxCombo); with more, the
+// upper pass of the parts with edges is splatted and blurred in full
+// ("full" mode, mode() > kMaxCombo).
+constexpr int kMaxCombo = 8, kMaxEdgeList = 8192;
+constexpr int kE_ANY = 0, kE_MODE = 1, kE_COMBO = 2, kE_NLIST = kE_COMBO + kMaxCombo, kE_BITS = kE_NLIST + 1,
+              kE_COUNT = kE_BITS + 64 * 16;  // bitmap of combos, code label*16 + bin
 struct EdgeMap {
   const int* flag;
   const uint16_t* pimg;   // part of each pixel (null: one part)
   const uint16_t* ppimg;  // part of each lattice point (null: one part)
   int parts;
-  __device__ inline bool any() const { return flag[parts] != 0; }
+  __device__ inline bool any() const { return flag[parts + kE_ANY] != 0; }
+  __device__ inline int mode() const { return flag[parts + kE_MODE]; }
+  __device__ inline bool full() const { return any() && mode() > kMaxCombo; }
   __device__ inline bool pixel(long pix) const { return flag[pimg ? pimg[pix] : 0] != 0; }
   __device__ inline bool point(long p) const { return flag[ppimg ? ppimg[p] : 0] != 0; }
 };
@@ -496,7 +509,7 @@ __global__ void __launch_bounds__(kBlock) k_combine_if(int V, int RS, RowMap rm,
                                                        int nmulti, const int* __restrict__ chstart,
                                                        const int* __restrict__ pstart, const float* __restrict__ part,
                                                        T* __restrict__ values, EdgeMap edge, int upper) {
-  if (upper && !edge.any()) return;
+  if (upper && !edge.full()) return;
   const long nvb = (long)((nmulti + rm.gpw - 1) / rm.gpw * 64 + blockDim.x - 1) / blockDim.x;
   for (long vb = xcd_block(); vb < nvb; vb += gridDim.x) {
     long m;
@@ -647,7 +660,7 @@ __global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ b
                                                    const int* __restrict__ pstart, int nchunks, T* __restrict__ values,
                                                    float* __restrict__ part, EdgeMap edge) {
   constexpr bool kUpper = TAG >= 100;
-  if (kUpper && !edge.any()) return;
+  if (kUpper && !edge.full()) return;
   const int nq = (L + 3) >> 2;
   for (long g = (long)xcd_block() * blockDim.x + threadIdx.x; g < nlanes; g += (long)gridDim.x * blockDim.x) {
     const int c = (int)(g / nq), q = (int)(g - (long)c * nq);
@@ -725,56 +738,127 @@ __device__ inline void st_quad(uint16_t* p, size_t q, float4 v) {
 }
 
 // one blur axis over the ordered state: one lane per (point, quad of the
-// row), over the lower rows and, when some part has bin edges, the upper rows
-// (half_q quads further) of the points of those parts (a point's neighbours
-// are in its own part)
+// row), over the lower rows; in correction mode also the correction rows
+// (kMaxCombo floats, two quads per point, in their own buffers); in full mode
+// the upper rows (half_q quads further) of the points of parts with bin edges
+// (a point's neighbours are in its own part)
+template <class T>
+__device__ inline void blur_quad(const int* __restrict__ nbr, const T* __restrict__ src, T* __restrict__ dst,
+                                 size_t base, uint32_t i, uint32_t R4, uint32_t q) {
+  const int n1 = nbr[2 * i] + 1, n2 = nbr[2 * i + 1] + 1;
+  const float4 o = ld_quad(src, base + (size_t)(i + 1) * R4 + q);
+  const float4 a = ld_quad(src, base + (size_t)n1 * R4 + q);
+  const float4 b = ld_quad(src, base + (size_t)n2 * R4 + q);
+  float4 r;
+  r.x = o.x + 0.5f * (a.x + b.x);
+  r.y = o.y + 0.5f * (a.y + b.y);
+  r.z = o.z + 0.5f * (a.z + b.z);
+  r.w = o.w + 0.5f * (a.w + b.w);
+  st_quad(dst, base + (size_t)(i + 1) * R4 + q, r);
+}
 template <int TAG, class T>
-__global__ void __launch_bounds__(kBlock) k_blur2(int R4, FastDiv divR4, uint32_t nq, size_t half_q,
+__global__ void __launch_bounds__(kBlock) k_blur2(int R4, FastDiv divR4, uint32_t nq, size_t half_q, uint32_t M,
                                                   const int* __restrict__ nbr, const T* __restrict__ src,
-                                                  T* __restrict__ dst, EdgeMap edge) {
-  const uint64_t total = edge.any() ? 2ull * nq : nq;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t t = (uint64_t)xcd_block() * blockDim.x + threadIdx.x; t < total; t += stride) {
-    const bool upper = t >= nq;
-    const uint32_t tt = (uint32_t)(upper ? t - nq : t);
-    const uint32_t i = divR4.div(tt);
-    const uint32_t q = tt - i * (uint32_t)R4;
-    if (upper && !edge.point(i)) continue;
-    const size_t h = upper ? half_q : 0;
-    const int n1 = nbr[2 * i] + 1, n2 = nbr[2 * i + 1] + 1;
-    const float4 o = ld_quad(src, h + (size_t)(i + 1) * R4 + q);
-    const float4 a = ld_quad(src, h + (size_t)n1 * R4 + q);
-    const float4 b = ld_quad(src, h + (size_t)n2 * R4 + q);
-    float4 r;
-    r.x = o.x + 0.5f * (a.x + b.x);
-    r.y = o.y + 0.5f * (a.y + b.y);
-    r.z = o.z + 0.5f * (a.z + b.z);
-    r.w = o.w + 0.5f * (a.w + b.w);
-    st_quad(dst, h + (size_t)(i + 1) * R4 + q, r);
+                                                  T* __restrict__ dst, const float* __restrict__ dsrc,
+                                                  float* __restrict__ ddst, EdgeMap edge) {
+  const uint32_t stride = gridDim.x * blockDim.x, t0 = xcd_block() * blockDim.x + threadIdx.x;
+  for (uint32_t t = t0; t < nq; t += stride) {  // lower rows
+    const uint32_t i = divR4.div(t);
+    blur_quad(nbr, src, dst, 0, i, (uint32_t)R4, t - i * (uint32_t)R4);
+  }
+  const int mode = edge.any() ? edge.mode() : 0;
+  if (mode >= 1 && mode <= kMaxCombo) {  // correction rows: two quads per point
+    for (uint32_t t = t0; t < 2 * M; t += stride) blur_quad(nbr, dsrc, ddst, 0, t >> 1, 2u, t & 1u);
+  } else if (mode > kMaxCombo) {  // upper rows of the points of parts with bin edges
+    for (uint32_t t = t0; t < nq; t += stride) {
+      const uint32_t i = divR4.div(t);
+      if (edge.point(i)) blur_quad(nbr, src, dst, half_q, i, (uint32_t)R4, t - i * (uint32_t)R4);
+    }
   }
 }
 
 // bins of every (pixel, label) of the rescaled field, packed lo | up << 4,
-// in rows of LP bytes per pixel; edge = 1 if some bin_up != bin_lo (then the
-// upper pass is done in full)
+// in rows of LP bytes per pixel; values on a bin edge (bin_up != bin_lo) set
+// their part's flag and their combo bit and are listed (the first
+// kMaxEdgeList of them)
 // A lattice that is the union of several images' lattices rescales each
 // image with its own {min, max}: pimg[pixel] picks the pair mm[2*img].
 __global__ void __launch_bounds__(kBlock) k_obin(const float* __restrict__ Q, const float* __restrict__ mm, size_t n,
                        uint8_t* __restrict__ bins, int LP, int* __restrict__ edge, const uint16_t* __restrict__ pimg,
-                       FastDiv divL, int parts) {
+                       FastDiv divL, int parts, int2* __restrict__ elist) {
   bool e = false;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     const uint32_t pix = divL.div((uint32_t)i);
     const int img = pimg ? pimg[pix] : 0;
     const float q = rescaled(Q, i, mm + 2 * img);
     const int lo = bin_lo(q), up = bin_up(q);
-    if (lo != up) {
+    const uint32_t k = (uint32_t)(i - (size_t)pix * divL.d);
+    if (lo != up) {  // on a bin edge: flag its part and its combo, and list it
       e = true;
+      int* E = edge + parts;
       if (edge[img] == 0) atomicOr(edge + img, 1);
+      const int code = (int)k * 16 + lo;
+      if (E[kE_BITS + code] == 0) atomicOr(E + kE_BITS + code, 1);
+      const int at = atomicAdd(E + kE_NLIST, 1);
+      if (at < kMaxEdgeList) elist[at] = make_int2((int)pix, code);
     }
-    bins[(size_t)pix * LP + (i - (size_t)pix * divL.d)] = (uint8_t)(lo | (up << 4));
+    bins[(size_t)pix * LP + k] = (uint8_t)(lo | (up << 4));
   }
-  if (__any(e) && (threadIdx.x & 63) == 0) atomicOr(edge + parts, 1);
+  if (__any(e) && (threadIdx.x & 63) == 0) atomicOr(edge + parts + kE_ANY, 1);
+}
+
+// One block after k_obin: the combos in code order (label, then bin), the
+// mode, and in correction mode each listed edge value's vertex weights added
+// to its combo's entry of dfix (64-bit fixed point in 2^-40 units: integer
+// atomics give the same sums in any order)
+constexpr double kDFix = 1099511627776.0;  // 2^40
+__global__ void __launch_bounds__(1024) k_ocombo(int* __restrict__ edge, int parts, int d1,
+                                                 const int2* __restrict__ elist, const int* __restrict__ offset,
+                                                 const float* __restrict__ bary,
+                                                 unsigned long long* __restrict__ dfix) {
+  __shared__ int wsum[16];
+  int* E = edge + parts;
+  if (!E[kE_ANY]) return;  // mode stays 0
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const bool bit = E[kE_BITS + t] != 0;
+  const unsigned long long bal = __ballot(bit);
+  if (lane == 0) wsum[wv] = __popcll(bal);
+  __syncthreads();
+  int before = 0, total = 0;
+  for (int w = 0; w < 16; ++w) {
+    if (w < wv) before += wsum[w];
+    total += wsum[w];
+  }
+  const int pos = before + __popcll(bal & ((1ull << lane) - 1ull));
+  if (bit && pos < kMaxCombo) E[kE_COMBO + pos] = t;
+  const int nl = E[kE_NLIST];
+  const bool corr = total <= kMaxCombo && nl <= kMaxEdgeList;
+  if (t == 0) E[kE_MODE] = corr ? total : kMaxCombo + 1;
+  if (!corr) return;
+  __syncthreads();  // combo codes written
+  for (int i = t; i < nl; i += blockDim.x) {
+    const int2 ev = elist[i];
+    int ci = 0;
+    for (int c = 0; c < total; ++c)
+      if (E[kE_COMBO + c] == ev.y) ci = c;
+    for (int j = 0; j < d1; ++j) {
+      const int of = offset[(size_t)ev.x * d1 + j] + 1;
+      const unsigned long long v = (unsigned long long)llrint((double)bary[(size_t)ev.x * d1 + j] * kDFix);
+      atomicAdd(dfix + (size_t)of * kMaxCombo + ci, v);
+    }
+  }
+}
+
+// correction mode: dv = dfix as float, and dfix cleared for the next call
+__global__ void __launch_bounds__(kBlock) k_odconv(const int* __restrict__ edge, int parts, size_t n,
+                                                   unsigned long long* __restrict__ dfix, float* __restrict__ dv) {
+  const int mode = edge[parts + kE_ANY] ? edge[parts + kE_MODE] : 0;
+  if (mode < 1 || mode > kMaxCombo) return;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const unsigned long long v = dfix[i];
+    dv[i] = (float)((double)v * (1.0 / kDFix));
+    if (v) dfix[i] = 0;
+  }
 }
 
 // lower slice then upper slice (:638-650, :686-696), then out (+)= scale*x.
@@ -785,7 +869,8 @@ template <int TAG, class T>
 __global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ bins, int L, int LP, int RS, RowMap rm,
                                                    int N, int d1, const float4* __restrict__ vtx,
                                                    const T* __restrict__ lo_v, const T* __restrict__ up_v,
-                                                   EdgeMap edge, float alpha, SliceArgs o) {
+                                                   const float* __restrict__ dv, EdgeMap edge, float alpha,
+                                                   SliceArgs o) {
   constexpr int kD1 = TAG == 2 ? 3 : TAG == 5 ? 6 : 0;
   constexpr int NV = kD1 ? kD1 : kMaxD + 1;
   const int nv = kD1 ? kD1 : d1;
@@ -794,26 +879,42 @@ __global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ b
   rm.at(L, pix, k0, ks);
   if (pix < 0 || pix >= N) return;
   const SlicePre pre = slice_pre(o, (int)pix, k0, L);
-  const bool full = edge.pixel(pix);
-  const T* urow = full ? up_v : lo_v;
   const int bb = bins[(size_t)pix * LP + k0];
-  const int lo = bb & 15, up = bb >> 4;
-  const int ilo = (max(lo, 1) - 1) * L + k0;  // LO[lo] (bin 0 is Tb)
-  const int iup = min(up, 8) * L + k0;        // LO[up + 1], or UP[up] (bin 9 is Tb)
   float4 vt[NV];
 #pragma unroll
   for (int j = 0; j < NV; ++j)
     if (j < nv) vt[j] = vtx[(size_t)pix * nv + j];
-  float lv[NV], uv[NV];
+  const int mode = edge.any() ? edge.mode() : 0;
+  const int lo = bb & 15, up = bb >> 4;
+  const int ilo = (max(lo, 1) - 1) * L + k0;  // LO[lo] (bin 0 is Tb)
+  const int iup = min(up, 8) * L + k0;        // LO[up + 1], or UP[up] (bin 9 is Tb)
+  // the lower rows' values first: they depend on the bins and vertices only
+  float a[NV], u[NV];
 #pragma unroll
   for (int j = 0; j < NV; ++j) {
     if (j < nv) {
       const size_t base = (size_t)__float_as_int(vt[j].x) * RS;
+      a[j] = ld_v(lo_v, base + ilo);
+      u[j] = ld_v(lo_v, base + iup);
+    }
+  }
+  const bool full = mode > kMaxCombo && edge.pixel(pix);
+  if (full)  // the upper rows were splatted and blurred in full: UP[up]
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+      if (j < nv) u[j] = ld_v(up_v, (size_t)__float_as_int(vt[j].x) * RS + iup);
+  int ci = -1;  // correction row of combo (k0, up + 1), if any
+  if (mode >= 1 && mode <= kMaxCombo && up < 9)
+    for (int c = 0; c < mode; ++c)
+      if (edge.flag[edge.parts + kE_COMBO + c] == k0 * 16 + up + 1) ci = c;
+  float lv[NV], uv[NV];
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    if (j < nv) {
       const float t = vt[j].z;
-      const float a = ld_v(lo_v, base + ilo);
-      const float u = ld_v(urow, base + iup);
-      lv[j] = lo == 0 ? t : a;
-      uv[j] = full ? (up == 9 ? t : u) : t - (up < 9 ? u : 0.f);
+      lv[j] = lo == 0 ? t : a[j];
+      uv[j] = full ? (up == 9 ? t : u[j]) : t - (up < 9 ? u[j] : 0.f);
+      if (ci >= 0) uv[j] = uv[j] + dv[(size_t)__float_as_int(vt[j].x) * kMaxCombo + ci];
     }
   }
   float acc = 0.f;
@@ -1103,7 +1204,7 @@ void Lattice::ensure_total_weight() {
   }
   tb_.resize(M_ + 1);
   DCRF_HIP(hipMemcpyAsync(tb_.get(), src, (M_ + 1) * sizeof(float), hipMemcpyDeviceToDevice, s));
-  edge_.resize(nparts_ + 1);
+  edge_.resize(nparts_ + kE_COUNT);
   DCRF_HIP(hipStreamSynchronize(s));
   check_launch("total weight");
   have_tb_ = true;
@@ -1118,6 +1219,16 @@ void Lattice::ensure_ordered_tables() {
   vtx_.resize(std::max<long>(E, 1));
   k_ovtx<<<grid_for(E), kBlock, 0, s>>>(offset_.get(), bary_.get(), tb_.get(), E, vtx_.get());
   check_launch("k_ovtx");
+  // bin-edge correction: the edge list, fixed-point sums (kept at zero between
+  // calls) and two float row buffers for the blur (row 0 stays zero)
+  elist_.resize(kMaxEdgeList);
+  const size_t nd = (size_t)(M_ + 1) * kMaxCombo;
+  dfix_.resize(nd);
+  dvA_.resize(nd);
+  dvB_.resize(nd);
+  DCRF_HIP(hipMemsetAsync(dfix_.get(), 0, nd * sizeof(unsigned long long), s));
+  DCRF_HIP(hipMemsetAsync(dvA_.get(), 0, nd * sizeof(float), s));
+  DCRF_HIP(hipMemsetAsync(dvB_.get(), 0, nd * sizeof(float), s));
   DCRF_HIP(hipStreamSynchronize(s));
   have_otab_ = true;
 }
@@ -1135,6 +1246,7 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   DCRF_REQUIRE(L > 0, "ordered filter: label count must be positive");
   DCRF_REQUIRE(L <= 64, "ordered filter: at most 64 labels");
   ensure_total_weight();
+  ensure_ordered_tables();  // before k_obin, which lists edge values into elist_
   const int row = orow_stride(L);
   ensure_values(2 * (size_t)row);  // LO and UP halves of each buffer (float-sized; bf16 rows use half)
   const size_t half = (size_t)(M_ + 1) * row;  // elements of T
@@ -1158,12 +1270,20 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   bins_.resize((size_t)N_ * LP);
   {
     KTimer tm("obin" + tag, s);
-    k_zero_ints<<<1, kBlock, 0, s>>>(edge, nparts_ + 1);
+    k_zero_ints<<<1, kBlock, 0, s>>>(edge, nparts_ + kE_COUNT);
     k_obin<<<grid_for(NL, kBlock, 2048), kBlock, 0, s>>>(Q, d_minmax, NL, bins_.get(), LP, edge,
                                                          pimg_.n ? pimg_.get() : nullptr, FastDiv((uint32_t)L),
-                                                         nparts_);
+                                                         nparts_, elist_.get());
   }
-  ensure_ordered_tables();
+  {  // bin edges: combos and mode; in correction mode the correction rows
+    KTimer tm("oedge" + tag, s);
+    k_ocombo<<<1, 1024, 0, s>>>(edge, nparts_, d_ + 1, elist_.get(), offset_.get(), bary_.get(), dfix_.get());
+    const size_t nd = (size_t)(M_ + 1) * kMaxCombo;
+    k_odconv<<<grid_for(nd, kBlock, 2048), kBlock, 0, s>>>(edge, nparts_, nd, dfix_.get(), dvA_.get());
+    check_launch("k_ocombo");
+  }
+  float* dsrc = dvA_.get();
+  float* ddst = dvB_.get();
   {
     KTimer tm("osplat" + tag, s);
     for (int upper = 0; upper < 2; ++upper) {
@@ -1209,9 +1329,9 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
     const unsigned full = grid_for(nquads);
     // src -> dst, the same bits for any grid: tuned on the lower rows
     const unsigned g = Tuner::get().pick(d_ == 5 ? "oblur5" : "oblur", M_, grid_candidates(full), s, [&](unsigned gg) {
-      kern<<<gg, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, half_q, nb, a, b, em);
+      kern<<<gg, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, half_q, (uint32_t)M_, nb, a, b, dsrc, ddst, em);
     });
-    kern<<<g, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, half_q, nb, a, b, em);
+    kern<<<g, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, half_q, (uint32_t)M_, nb, a, b, dsrc, ddst, em);
   };
   for (int j = 0; j <= d_; ++j) {  // forward order only (:621, :669)
     const int* nb = nbr_.get() + (size_t)j * M_ * 2;
@@ -1227,10 +1347,11 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
     check_launch("k_blur2");
     std::swap(srcL, dstL);
     std::swap(srcU, dstU);
+    std::swap(dsrc, ddst);
   }
   KTimer tm("oslice" + tag, s);
   auto slice = [&](auto kern) {
-    kern<<<rows_grid(N_, L), kBlock, 0, s>>>(bins_.get(), L, LP, row, rm, N_, d_ + 1, vtx_.get(), srcL, srcU, em,
+    kern<<<rows_grid(N_, L), kBlock, 0, s>>>(bins_.get(), L, LP, row, rm, N_, d_ + 1, vtx_.get(), srcL, srcU, dsrc, em,
                                              lattice_alpha(d_), out);
   };
   if (d_ == 2)

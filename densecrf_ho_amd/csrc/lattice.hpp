@@ -82,7 +82,8 @@ class Lattice {
   void work_buffers(std::vector<const void*>& out) const {
     for (const void* p : {(const void*)vA_.get(), (const void*)vB_.get(), (const void*)part_.get(),
                           (const void*)bins_.get(), (const void*)tb_.get(), (const void*)edge_.get(),
-                          (const void*)vtx_.get()})
+                          (const void*)vtx_.get(), (const void*)elist_.get(), (const void*)dfix_.get(),
+                          (const void*)dvA_.get(), (const void*)dvB_.get()})
       out.push_back(p);
   }
 
@@ -128,6 +129,11 @@ class Lattice {
   void ensure_ordered_tables();
   bool have_otab_ = false;
   DBuf<float4> vtx_;
+  // bin-edge correction of the ordered filter's upper pass: listed edge
+  // values, fixed-point sums, correction rows (ping-pong through the blur)
+  DBuf<int2> elist_;
+  DBuf<unsigned long long> dfix_;
+  DBuf<float> dvA_, dvB_;
   DBuf<uint16_t> pimg_;   // part of each pixel (union lattices only)
   DBuf<uint16_t> ppimg_;  // part of each lattice point (union lattices only)
   int nparts_ = 1;

@@ -68,6 +68,28 @@ def test_image_lattice_vs_oracle(W, H, K, d):
     assert rel_err(o2, ref.ordered(q, fp64=True)) < 1e-4
 
 
+@pytest.mark.parametrize("ncombo", [1, 3, 8, 9])
+def test_ordered_filter_bin_edge_modes(ncombo):
+    """q exactly on a bin edge (0.5, 0.7f and 0.9f give bin_up = bin_lo - 1)
+    for `ncombo` (label, bin) pairs, 40 pixels each: up to 8 pairs take the
+    upper pass as Tb - lower + a filtered correction, 9 pairs the full upper
+    pass; both within 1e-5 of the float oracle and 1e-4 of the FP64 oracle"""
+    from densecrf_ho_amd import synthetic as S
+    print("combos", ncombo, flush=True)
+    wl = S.Workload(96, 64, 12, 4)
+    feat = wl.kernels[1][0]
+    ref = O.Lattice(feat)
+    lat = gpu_lattice(feat)
+    rng = np.random.default_rng(11)
+    q = rng.random((wl.N, 21)).astype(np.float32)
+    combos = [(k, v) for k in range(3) for v in (0.5, 0.7, 0.9)][:ncombo]
+    for k, v in combos:
+        q[rng.choice(wl.N, 40, replace=False), k] = np.float32(v)
+    out = lat.compute_upper_minus_lower_ord(q)
+    assert rel_err(out, ref.ordered(q)) < 1e-5
+    assert rel_err(out, ref.ordered(q, fp64=True)) < 1e-4
+
+
 def test_ordered_filter_rejects_out_of_range(golden):
     import densecrf_ho_amd as D
     g = golden("lattice_b5_31x23.npz")
