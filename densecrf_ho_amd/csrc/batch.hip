@@ -265,6 +265,7 @@ std::vector<const void*> Batch::work_buffers() const {
     p->lat->work_buffers(v);
     v.push_back(p->ctmp.get());
   }
+  obins_.buffers(v);
   for (const void* p : {(const void*)Q_.get(), (const void*)gy_.get(), (const void*)psx_.get(),
                         (const void*)tmp_.get(), (const void*)rpix_.get(), (const void*)lab_.get(),
                         (const void*)parts_.get(), (const void*)sc_.get(), (const void*)fl_.get(),
@@ -413,11 +414,12 @@ void Batch::lp_energy_dev(const float* Q, double* dst, double* scratch, int stri
   finalize_seg<1><<<B_, kBlock, 0, s>>>(parts_.get(), first, 0, dst, stride);
   kb_minmax_parts<<<nb, kBlock, 0, s>>>(rm, L_, Q, mmparts_.get());
   k_lp_minmax_final<<<B_, 1024, 0, s>>>(mmparts_.get(), nb, first, mm_.get(), lfl_.get());
-  for (auto& pp : pw_) {
-    Potential& p = *pp;
+  for (size_t k = 0; k < pw_.size(); ++k) {
+    Potential& p = *pw_[k];
     SliceArgs o;
     o.out = tmp_.get();
-    compat_slice(p, o, L_, s, [&](const SliceArgs& oo) { p.lat->ordered(Q, mm_.get(), L_, oo); });
+    compat_slice(p, o, L_, s,
+                 [&](const SliceArgs& oo) { p.lat->ordered(Q, mm_.get(), L_, oo, &obins_, k == 0); });
     k_dot<<<nb, kBlock, 0, s>>>(Q, tmp_.get(), L_, rm, parts_.get());
     finalize_seg<1><<<B_, kBlock, 0, s>>>(parts_.get(), first, 0, scratch, stride);
     kb_sub<<<1, 256, 0, s>>>(dst, scratch, stride, B_);
@@ -552,7 +554,9 @@ std::vector<std::vector<int>> Batch::lp(const float* init, float* out, const LPP
           SliceArgs o;
           o.out = sa_.get();
           o.accumulate = k > 0;
-          compat_slice(p, o, L, s, [&](const SliceArgs& oo) { p.lat->ordered(lQ_.get(), mm_.get(), L, oo); });
+          compat_slice(p, o, L, s, [&](const SliceArgs& oo) {
+            p.lat->ordered(lQ_.get(), mm_.get(), L, oo, &obins_, k == 0);
+          });
         }
         if (num_pairwise() == 0) k_fill<<<grid_for(NL), kBlock, 0, s>>>(sa_.get(), NL, 0.f);
         {

@@ -104,6 +104,7 @@ class Batch {
   GraphCache graphs_;
   // QP state
   DBuf<float> Q_, gy_, psx_, tmp_, rpix_;
+  OrderedBins obins_;  // bins of the field the ordered filters of all pairwise terms share
   DBuf<int> lab_;
   DBuf<double> parts_, sc_;
   DBuf<int> fl_;

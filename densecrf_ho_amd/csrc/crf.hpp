@@ -164,6 +164,7 @@ class CRF {
   int R_ = 0;
   hipStream_t s_ = nullptr;
   DBuf<float> U_;
+  OrderedBins obins_;  // bins of the field the ordered filters of all potentials share
   std::vector<std::unique_ptr<Potential>> pw_;
   std::unique_ptr<Impl> im_;
 };

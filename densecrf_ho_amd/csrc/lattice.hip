@@ -1233,16 +1233,18 @@ void Lattice::ensure_ordered_tables() {
   have_otab_ = true;
 }
 
-void Lattice::ordered(const float* Q, const float* d_minmax, int L, const SliceArgs& out) {
+void Lattice::ordered(const float* Q, const float* d_minmax, int L, const SliceArgs& out, OrderedBins* shared,
+                      bool fresh) {
   if (bf16_)
-    ordered_t<uint16_t>(Q, d_minmax, L, out);
+    ordered_t<uint16_t>(Q, d_minmax, L, out, shared, fresh);
   else
-    ordered_t<float>(Q, d_minmax, L, out);
+    ordered_t<float>(Q, d_minmax, L, out, shared, fresh);
 }
 
 // T: storage of the lattice rows (float, or bfloat16 bits)
 template <class T>
-void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const SliceArgs& out) {
+void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const SliceArgs& out, OrderedBins* shared,
+                        bool fresh) {
   DCRF_REQUIRE(L > 0, "ordered filter: label count must be positive");
   DCRF_REQUIRE(L <= 64, "ordered filter: at most 64 labels");
   ensure_total_weight();
@@ -1262,22 +1264,31 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   k_zero<<<grid_for(zf), kBlock, 0, s>>>(reinterpret_cast<float*>(dstL), zf);
   k_zero<<<grid_for(zf), kBlock, 0, s>>>(reinterpret_cast<float*>(srcU), zf);
   k_zero<<<grid_for(zf), kBlock, 0, s>>>(reinterpret_cast<float*>(dstU), zf);
-  int* edge = edge_.get();
-  const EdgeMap em{edge, pimg_.n ? pimg_.get() : nullptr, ppimg_.n ? ppimg_.get() : nullptr, nparts_};
   const std::string tag = prof_on() ? "[d=" + std::to_string(d_) + ",L=" + std::to_string(L) + "]" : std::string();
   const size_t NL = (size_t)N_ * L;
   const int LP = obin_stride(L);
-  bins_.resize((size_t)N_ * LP);
-  {
+  if (shared) {
+    shared->bins.resize((size_t)N_ * LP);
+    shared->edge.resize(nparts_ + kE_COUNT);
+    shared->elist.resize(kMaxEdgeList);
+  } else {
+    bins_.resize((size_t)N_ * LP);
+  }
+  uint8_t* bins = shared ? shared->bins.get() : bins_.get();
+  int* edge = shared ? shared->edge.get() : edge_.get();
+  int2* elist = shared ? shared->elist.get() : elist_.get();
+  const EdgeMap em{edge, pimg_.n ? pimg_.get() : nullptr, ppimg_.n ? ppimg_.get() : nullptr, nparts_};
+  if (!shared || fresh) {
     KTimer tm("obin" + tag, s);
     k_zero_ints<<<1, kBlock, 0, s>>>(edge, nparts_ + kE_COUNT);
-    k_obin<<<grid_for(NL, kBlock, 2048), kBlock, 0, s>>>(Q, d_minmax, NL, bins_.get(), LP, edge,
+    k_obin<<<grid_for(NL, kBlock, 2048), kBlock, 0, s>>>(Q, d_minmax, NL, bins, LP, edge,
                                                          pimg_.n ? pimg_.get() : nullptr, FastDiv((uint32_t)L),
-                                                         nparts_, elist_.get());
+                                                         nparts_, elist);
   }
-  {  // bin edges: combos and mode; in correction mode the correction rows
+  {  // bin edges: combos and mode (the same for every lattice of a group);
+     // in correction mode this lattice's correction rows
     KTimer tm("oedge" + tag, s);
-    k_ocombo<<<1, 1024, 0, s>>>(edge, nparts_, d_ + 1, elist_.get(), offset_.get(), bary_.get(), dfix_.get());
+    k_ocombo<<<1, 1024, 0, s>>>(edge, nparts_, d_ + 1, elist, offset_.get(), bary_.get(), dfix_.get());
     const size_t nd = (size_t)(M_ + 1) * kMaxCombo;
     k_odconv<<<grid_for(nd, kBlock, 2048), kBlock, 0, s>>>(edge, nparts_, nd, dfix_.get(), dvA_.get());
     check_launch("k_ocombo");
@@ -1291,7 +1302,7 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
       const long nlanes = (long)nchunks_ * ((L + 3) / 4);
       const unsigned nvb = grid_for((size_t)nlanes);
       auto launch = [&](auto kern, unsigned grid) {
-        kern<<<grid, kBlock, 0, s>>>(bins_.get(), L, LP, row, nlanes, sp_pix_.get(), sp_w_.get(), seg_.get(),
+        kern<<<grid, kBlock, 0, s>>>(bins, L, LP, row, nlanes, sp_pix_.get(), sp_w_.get(), seg_.get(),
                                      chstart_.get(), chpoint_.get(), pstart_.get(), nchunks_, dst_half, part_.get(),
                                      em);
       };
@@ -1351,7 +1362,7 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   }
   KTimer tm("oslice" + tag, s);
   auto slice = [&](auto kern) {
-    kern<<<rows_grid(N_, L), kBlock, 0, s>>>(bins_.get(), L, LP, row, rm, N_, d_ + 1, vtx_.get(), srcL, srcU, dsrc, em,
+    kern<<<rows_grid(N_, L), kBlock, 0, s>>>(bins, L, LP, row, rm, N_, d_ + 1, vtx_.get(), srcL, srcU, dsrc, em,
                                              lattice_alpha(d_), out);
   };
   if (d_ == 2)

@@ -33,6 +33,18 @@ struct SplatArgs {
 };
 
 // What the slice does with the filtered value x of (pixel, k).
+// Bins of one field shared by the ordered filters of several lattices over
+// the same pixels (the pairwise terms of one CRF, or of one batch): the bins,
+// the per-part edge flags and bookkeeping, and the listed edge values
+struct OrderedBins {
+  DBuf<uint8_t> bins;
+  DBuf<int> edge;
+  DBuf<int2> elist;
+  void buffers(std::vector<const void*>& out) const {
+    for (const void* p : {(const void*)bins.get(), (const void*)edge.get(), (const void*)elist.get()}) out.push_back(p);
+  }
+};
+
 struct SliceArgs {
   float* out = nullptr;
   const float* norm = nullptr;  // multiply by norm[pix] (post-normalisation) if set
@@ -63,7 +75,11 @@ class Lattice {
   // seqCompute_upper_minus_lower_ord (src/permutohedral.cpp:590-701) on the
   // rescaled field q = (Q - qmin) / qmax (qmax == 0 -> no division), read on
   // the fly from Q; then out (+)= scale * result.
-  void ordered(const float* Q, const float* d_minmax, int L, const SliceArgs& out);
+  // With `shared`, the bins live there: `fresh` computes them from Q (the
+  // first lattice of a group), otherwise the ones of the same Q and {min,
+  // max} computed by an earlier call of the group are used.
+  void ordered(const float* Q, const float* d_minmax, int L, const SliceArgs& out, OrderedBins* shared = nullptr,
+               bool fresh = true);
 
   // The disjoint union of several lattices of the same dimension (one per
   // image): pixels and points are renumbered image after image, so filtering
@@ -99,7 +115,8 @@ class Lattice {
  private:
   void ensure_values(size_t floats);
   template <class T>
-  void ordered_t(const float* Q, const float* d_minmax, int L, const SliceArgs& out);
+  void ordered_t(const float* Q, const float* d_minmax, int L, const SliceArgs& out, OrderedBins* shared,
+                 bool fresh);
   bool bf16_ = false;
   void ensure_total_weight();
   hipStream_t stream_;

@@ -574,7 +574,8 @@ void CRF::lp_energy_dev(const float* Q, double* dst, double* scratch, hipStream_
     Potential& p = pairwise(k);
     SliceArgs o;
     o.out = m.tmp.get();
-    compat_slice(p, o, L_, s, [&](const SliceArgs& oo) { p.lat->ordered(Q, m.mm.get(), L_, oo); });
+    compat_slice(p, o, L_, s,
+                 [&](const SliceArgs& oo) { p.lat->ordered(Q, m.mm.get(), L_, oo, &obins_, k == 0); });
     launch_dot(s, Q, m.tmp.get(), NL, m.parts.get(), scratch);
     k_sub_scalar<<<1, 1, 0, s>>>(dst, scratch);
   }
@@ -758,7 +759,9 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
         SliceArgs o;
         o.out = st.sa.get();
         o.accumulate = k > 0;
-        compat_slice(p, o, L, s, [&](const SliceArgs& oo) { p.lat->ordered(st.Q.get(), st.mm.get(), L, oo); });
+        compat_slice(p, o, L, s, [&](const SliceArgs& oo) {
+          p.lat->ordered(st.Q.get(), st.mm.get(), L, oo, &obins_, k == 0);
+        });
       }
       if (num_pairwise() == 0) k_fill<<<grid_for(NL), kBlock, 0, s>>>(st.sa.get(), NL, 0.f);
       if (ho) ucondgrad_dev(st.Q.get(), su, s);
