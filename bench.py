@@ -65,6 +65,68 @@ def algorithmic_bytes(name: str, N: int, M: dict, V: int = L, S: int = 4) -> flo
     return float("nan")
 
 
+def oracle_parity():
+    """GPU results against the FP64 oracle at sizes the oracle finishes in
+    seconds: the plain QP (10 iterations) at 320x213 with 40 cells, the LP
+    5x10 at 160x120 from each side's own QP solution, and a 3-image batch
+    (fp32 and bf16 ordered-filter rows) against each image's FP64 LP.  Reports
+    the relative difference of the integer energy of the final labelling
+    (both evaluated by the FP64 oracle) and the label agreement."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import densecrf_ho_amd as D
+    from densecrf_ho_amd import synthetic as S
+
+    def pair(wl):
+        o = O.CRF(wl.N, L)
+        o.set_unary(wl.U)
+        g = D.DenseCRF2D(wl.W, wl.H, L)
+        g.setUnaryEnergy(wl.U)
+        for f, w in wl.kernels:
+            o.add_pairwise(f, w)
+            g.addPairwiseEnergy(f, D.PottsCompatibility(w))
+        return o, g
+
+    def cmp(o, qg, qo):
+        lg, lo = qg.argmax(1).astype(np.int32), qo.argmax(1).astype(np.int32)
+        eg, eo = o.energy_true(lg, fp64=True), o.energy_true(lo, fp64=True)
+        return {"energy_rel_diff": abs(eg - eo) / abs(eo), "label_agreement": float((lg == lo).mean())}
+
+    out = {}
+    wl = S.Workload(320, 213, 40, 0, L=L)
+    o, g = pair(wl)
+    qo, _ = o.qp(O.unary_init(wl.U), max_iter=QP_ITERS, fp64=True)
+    qg = g.qp_inference_super_pixels_non_convex(g.unary_init(), max_iter=QP_ITERS, ref_stop=False)
+    out["qp_320x213"] = cmp(o, qg, qo)
+    P = D.LP_inf_params(prox_max_iter=LP_PROX, fw_max_iter=LP_FW)
+    Po = O.LPParams(prox_max_iter=LP_PROX, fw_max_iter=LP_FW)
+    wl = S.Workload(160, 120, 5, 1, L=L)
+    o, g = pair(wl)
+    qo, _ = o.lp(o.qp(O.unary_init(wl.U), max_iter=QP_ITERS, fp64=True)[0], Po, fp64=True)
+    qg = g.lp_inference_prox_super_pixels(g.qp_inference_super_pixels_non_convex(g.unary_init(), max_iter=QP_ITERS,
+                                                                                ref_stop=False), P)
+    out["lp_160x120"] = cmp(o, qg, qo)
+    sizes = [(160, 120), (128, 96), (96, 72)]
+    wls = [S.Workload(w, h, max(1, round(w * h / 4000)), 3000 + i, L=L) for i, (w, h) in enumerate(sizes)]
+    refs = []
+    for w in wls:
+        o, _ = pair(w)
+        refs.append((o, o.lp(o.qp(O.unary_init(w.U), max_iter=QP_ITERS, fp64=True)[0], Po, fp64=True)[0]))
+    for name, bf16 in (("fp32", False), ("bf16", True)):
+        b = D.DenseCRFBatch(L, sizes)
+        for i, w in enumerate(wls):
+            b.setUnaryEnergy(i, w.U)
+        b.addPairwiseGaussian(3, 3, D.PottsCompatibility(3))
+        b.addPairwiseBilateral(50, 50, 15, 15, 15, [w.img for w in wls], D.PottsCompatibility(5))
+        b.set_value_type(bf16)
+        Qb, _ = b.qp(max_iter=QP_ITERS, ref_stop=False)
+        Ql, _ = b.lp(Qb, P)
+        r = [cmp(o, b.image(Ql, i), qo) for i, (o, qo) in enumerate(refs)]
+        out[f"batch_3img_{name}"] = {"energy_rel_diff_max": max(x["energy_rel_diff"] for x in r),
+                                     "label_agreement_min": min(x["label_agreement"] for x in r)}
+    return out
+
+
 def cpu_baseline():
     """Oracle on the host: QP with 1 and 2 iterations at 1080p and the LP with
     1x1 and 1x2 steps on a 480x270 crop of the same generator (area 1/16),
@@ -426,6 +488,8 @@ def main():
         out["images_per_s"] = out["batch"]["images_per_s"]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
+        _progress("oracle parity")
+        out["oracle_parity"] = oracle_parity()
     print(json.dumps(out), flush=True)
 
 

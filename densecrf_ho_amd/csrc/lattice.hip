@@ -478,27 +478,35 @@ __device__ inline void st_pair(uint16_t* row, long k, float2 v) {
 // Bin-edge flags of the ordered filter, one per part of the lattice (one
 // image, or each image of a union) plus flag[parts] = "any part".  A part
 // whose flag is 0 takes the upper pass from T - LO[c+1] and skips its own.
-// Bin-edge bookkeeping of one ordered-filter call, in an int array after the
-// per-part flags: flag[parts + kE_*].  A (pixel, label) value on a bin edge
-// has bin_up = bin_lo - 1; its (label, bin_lo) pair is its "combo".  With at
-// most kMaxCombo combos and kMaxEdgeList edge values the upper pass is
-// Tb - LO[c+1] + D[c+1], D the filtered weight of the edge values of combo
-// (label, c+1) ("correction" mode, mode() in 1..kMaxCombo); with more, the
-// upper pass of the parts with edges is splatted and blurred in full
-// ("full" mode, mode() > kMaxCombo).
-constexpr int kMaxCombo = 8, kMaxEdgeList = 8192;
-constexpr int kE_ANY = 0, kE_MODE = 1, kE_COMBO = 2, kE_NLIST = kE_COMBO + kMaxCombo, kE_BITS = kE_NLIST + 1,
-              kE_COUNT = kE_BITS + 64 * 16;  // bitmap of combos, code label*16 + bin
+// Bin-edge bookkeeping of one ordered-filter call, per part (image) of the
+// lattice.  A (pixel, label) value on a bin edge has bin_up = bin_lo - 1; its
+// (label, bin_lo) pair is its "combo".  A part with at most kMaxCombo combos
+// and kPartEdges edge values takes the upper pass as Tb - LO[c+1] + D[c+1],
+// D the filtered weight of its edge values of combo (label, c+1)
+// ("correction" mode 1..kMaxCombo, the number of combos); a part with more
+// has its upper pass splatted and blurred in full (mode kMaxCombo + 1).  Every
+// decision is per part, so a part of a union lattice is filtered exactly as
+// the image alone.  Per part p, at flag[p * kP_STRIDE + kP_*]: edge flag,
+// mode, combo codes (label * 16 + bin, in code order), listed count, and a
+// bitmap of the codes (64 labels x 16 bins); then two global flags (some part
+// in correction mode, some part in full mode).
+constexpr int kMaxCombo = 8, kPartEdges = 512;
+constexpr int kP_EDGE = 0, kP_MODE = 1, kP_COMBO = 2, kP_NLIST = kP_COMBO + kMaxCombo, kP_BITS = kP_NLIST + 1,
+              kP_STRIDE = kP_BITS + 32;
+constexpr int kG_CORR = 0, kG_FULL = 1, kG_COUNT = 2;
+inline size_t edge_ints(int parts) { return (size_t)parts * kP_STRIDE + kG_COUNT; }
 struct EdgeMap {
   const int* flag;
   const uint16_t* pimg;   // part of each pixel (null: one part)
   const uint16_t* ppimg;  // part of each lattice point (null: one part)
   int parts;
-  __device__ inline bool any() const { return flag[parts + kE_ANY] != 0; }
-  __device__ inline int mode() const { return flag[parts + kE_MODE]; }
-  __device__ inline bool full() const { return any() && mode() > kMaxCombo; }
-  __device__ inline bool pixel(long pix) const { return flag[pimg ? pimg[pix] : 0] != 0; }
-  __device__ inline bool point(long p) const { return flag[ppimg ? ppimg[p] : 0] != 0; }
+  __device__ inline int ppart(long pix) const { return pimg ? pimg[pix] : 0; }
+  __device__ inline int qpart(long p) const { return ppimg ? ppimg[p] : 0; }
+  __device__ inline int mode(int part) const { return flag[part * kP_STRIDE + kP_MODE]; }
+  __device__ inline int combo(int part, int c) const { return flag[part * kP_STRIDE + kP_COMBO + c]; }
+  __device__ inline bool any_corr() const { return flag[parts * kP_STRIDE + kG_CORR] != 0; }
+  __device__ inline bool any_full() const { return flag[parts * kP_STRIDE + kG_FULL] != 0; }
+  __device__ inline bool point_full(long p) const { return mode(qpart(p)) > kMaxCombo; }
 };
 
 // k_combine of the lower pass, or of the upper pass (upper != 0) for the
@@ -509,7 +517,7 @@ __global__ void __launch_bounds__(kBlock) k_combine_if(int V, int RS, RowMap rm,
                                                        int nmulti, const int* __restrict__ chstart,
                                                        const int* __restrict__ pstart, const float* __restrict__ part,
                                                        T* __restrict__ values, EdgeMap edge, int upper) {
-  if (upper && !edge.full()) return;
+  if (upper && !edge.any_full()) return;
   const long nvb = (long)((nmulti + rm.gpw - 1) / rm.gpw * 64 + blockDim.x - 1) / blockDim.x;
   for (long vb = xcd_block(); vb < nvb; vb += gridDim.x) {
     long m;
@@ -517,7 +525,7 @@ __global__ void __launch_bounds__(kBlock) k_combine_if(int V, int RS, RowMap rm,
     rm.at_block(V, vb, m, k0, ks);
     if (m < 0 || m >= nmulti) continue;
     const int p = multi[m];
-    if (upper && !edge.point(p)) continue;
+    if (upper && !edge.point_full(p)) continue;
     const int c0 = pstart[p], c1 = c0 + chstart[p + 1] - chstart[p];
     for (int k = k0; k < V; k += ks) {
       float acc = part[(size_t)c0 * V + k];
@@ -660,12 +668,12 @@ __global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ b
                                                    const int* __restrict__ pstart, int nchunks, T* __restrict__ values,
                                                    float* __restrict__ part, EdgeMap edge) {
   constexpr bool kUpper = TAG >= 100;
-  if (kUpper && !edge.full()) return;
+  if (kUpper && !edge.any_full()) return;
   const int nq = (L + 3) >> 2;
   for (long g = (long)xcd_block() * blockDim.x + threadIdx.x; g < nlanes; g += (long)gridDim.x * blockDim.x) {
     const int c = (int)(g / nq), q = (int)(g - (long)c * nq);
     const int p = chpoint[c];
-    if (kUpper && !edge.point(p)) continue;
+    if (kUpper && !edge.point_full(p)) continue;
     const int c0 = chstart[p], nc = chstart[p + 1] - c0;
     const int b = seg[p] + (c - c0) * kChunk;
     const int e = min(seg[p + 1], b + kChunk);
@@ -738,10 +746,10 @@ __device__ inline void st_quad(uint16_t* p, size_t q, float4 v) {
 }
 
 // one blur axis over the ordered state: one lane per (point, quad of the
-// row), over the lower rows; in correction mode also the correction rows
-// (kMaxCombo floats, two quads per point, in their own buffers); in full mode
-// the upper rows (half_q quads further) of the points of parts with bin edges
-// (a point's neighbours are in its own part)
+// row), over the lower rows; when some part is in correction mode also the
+// correction rows (kMaxCombo floats, two quads per point, in their own
+// buffers); when some part is in full mode the upper rows (half_q quads
+// further) of that part's points (a point's neighbours are in its own part)
 template <class T>
 __device__ inline void blur_quad(const int* __restrict__ nbr, const T* __restrict__ src, T* __restrict__ dst,
                                  size_t base, uint32_t i, uint32_t R4, uint32_t q) {
@@ -766,27 +774,24 @@ __global__ void __launch_bounds__(kBlock) k_blur2(int R4, FastDiv divR4, uint32_
     const uint32_t i = divR4.div(t);
     blur_quad(nbr, src, dst, 0, i, (uint32_t)R4, t - i * (uint32_t)R4);
   }
-  const int mode = edge.any() ? edge.mode() : 0;
-  if (mode >= 1 && mode <= kMaxCombo) {  // correction rows: two quads per point
+  if (edge.any_corr())  // correction rows: two quads per point
     for (uint32_t t = t0; t < 2 * M; t += stride) blur_quad(nbr, dsrc, ddst, 0, t >> 1, 2u, t & 1u);
-  } else if (mode > kMaxCombo) {  // upper rows of the points of parts with bin edges
+  if (edge.any_full())  // upper rows of the points of parts in full mode
     for (uint32_t t = t0; t < nq; t += stride) {
       const uint32_t i = divR4.div(t);
-      if (edge.point(i)) blur_quad(nbr, src, dst, half_q, i, (uint32_t)R4, t - i * (uint32_t)R4);
+      if (edge.point_full(i)) blur_quad(nbr, src, dst, half_q, i, (uint32_t)R4, t - i * (uint32_t)R4);
     }
-  }
 }
 
 // bins of every (pixel, label) of the rescaled field, packed lo | up << 4,
 // in rows of LP bytes per pixel; values on a bin edge (bin_up != bin_lo) set
-// their part's flag and their combo bit and are listed (the first
-// kMaxEdgeList of them)
+// their part's flag and combo bit and are listed (the first kPartEdges of
+// each part)
 // A lattice that is the union of several images' lattices rescales each
 // image with its own {min, max}: pimg[pixel] picks the pair mm[2*img].
 __global__ void __launch_bounds__(kBlock) k_obin(const float* __restrict__ Q, const float* __restrict__ mm, size_t n,
                        uint8_t* __restrict__ bins, int LP, int* __restrict__ edge, const uint16_t* __restrict__ pimg,
                        FastDiv divL, int parts, int2* __restrict__ elist) {
-  bool e = false;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     const uint32_t pix = divL.div((uint32_t)i);
     const int img = pimg ? pimg[pix] : 0;
@@ -794,33 +799,33 @@ __global__ void __launch_bounds__(kBlock) k_obin(const float* __restrict__ Q, co
     const int lo = bin_lo(q), up = bin_up(q);
     const uint32_t k = (uint32_t)(i - (size_t)pix * divL.d);
     if (lo != up) {  // on a bin edge: flag its part and its combo, and list it
-      e = true;
-      int* E = edge + parts;
-      if (edge[img] == 0) atomicOr(edge + img, 1);
+      int* P = edge + (size_t)img * kP_STRIDE;
+      if (P[kP_EDGE] == 0) atomicOr(P + kP_EDGE, 1);
       const int code = (int)k * 16 + lo;
-      if (E[kE_BITS + code] == 0) atomicOr(E + kE_BITS + code, 1);
-      const int at = atomicAdd(E + kE_NLIST, 1);
-      if (at < kMaxEdgeList) elist[at] = make_int2((int)pix, code);
+      const int bw = (int)(1u << (code & 31));
+      if ((P[kP_BITS + (code >> 5)] & bw) == 0) atomicOr(P + kP_BITS + (code >> 5), bw);
+      const int at = atomicAdd(P + kP_NLIST, 1);
+      if (at < kPartEdges) elist[(size_t)img * kPartEdges + at] = make_int2((int)pix, code);
     }
     bins[(size_t)pix * LP + k] = (uint8_t)(lo | (up << 4));
   }
-  if (__any(e) && (threadIdx.x & 63) == 0) atomicOr(edge + parts + kE_ANY, 1);
 }
 
-// One block after k_obin: the combos in code order (label, then bin), the
-// mode, and in correction mode each listed edge value's vertex weights added
-// to its combo's entry of dfix (64-bit fixed point in 2^-40 units: integer
-// atomics give the same sums in any order)
+// One block per part after k_obin: the part's combos in code order (label,
+// then bin) and its mode; in correction mode each listed edge value's vertex
+// weights are added to its combo's entry of dfix (64-bit fixed point in 2^-40
+// units: integer atomics give the same sums in any order)
 constexpr double kDFix = 1099511627776.0;  // 2^40
 __global__ void __launch_bounds__(1024) k_ocombo(int* __restrict__ edge, int parts, int d1,
                                                  const int2* __restrict__ elist, const int* __restrict__ offset,
                                                  const float* __restrict__ bary,
                                                  unsigned long long* __restrict__ dfix) {
   __shared__ int wsum[16];
-  int* E = edge + parts;
-  if (!E[kE_ANY]) return;  // mode stays 0
+  const int part = blockIdx.x;
+  int* P = edge + (size_t)part * kP_STRIDE;
+  if (!P[kP_EDGE]) return;  // mode stays 0
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const bool bit = E[kE_BITS + t] != 0;
+  const bool bit = ((uint32_t)P[kP_BITS + (t >> 5)] >> (t & 31)) & 1u;
   const unsigned long long bal = __ballot(bit);
   if (lane == 0) wsum[wv] = __popcll(bal);
   __syncthreads();
@@ -830,17 +835,20 @@ __global__ void __launch_bounds__(1024) k_ocombo(int* __restrict__ edge, int par
     total += wsum[w];
   }
   const int pos = before + __popcll(bal & ((1ull << lane) - 1ull));
-  if (bit && pos < kMaxCombo) E[kE_COMBO + pos] = t;
-  const int nl = E[kE_NLIST];
-  const bool corr = total <= kMaxCombo && nl <= kMaxEdgeList;
-  if (t == 0) E[kE_MODE] = corr ? total : kMaxCombo + 1;
+  if (bit && pos < kMaxCombo) P[kP_COMBO + pos] = t;
+  const int nl = P[kP_NLIST];
+  const bool corr = total <= kMaxCombo && nl <= kPartEdges;
+  if (t == 0) {
+    P[kP_MODE] = corr ? total : kMaxCombo + 1;
+    atomicOr(edge + (size_t)parts * kP_STRIDE + (corr ? kG_CORR : kG_FULL), 1);
+  }
   if (!corr) return;
   __syncthreads();  // combo codes written
   for (int i = t; i < nl; i += blockDim.x) {
-    const int2 ev = elist[i];
+    const int2 ev = elist[(size_t)part * kPartEdges + i];
     int ci = 0;
     for (int c = 0; c < total; ++c)
-      if (E[kE_COMBO + c] == ev.y) ci = c;
+      if (P[kP_COMBO + c] == ev.y) ci = c;
     for (int j = 0; j < d1; ++j) {
       const int of = offset[(size_t)ev.x * d1 + j] + 1;
       const unsigned long long v = (unsigned long long)llrint((double)bary[(size_t)ev.x * d1 + j] * kDFix);
@@ -852,8 +860,7 @@ __global__ void __launch_bounds__(1024) k_ocombo(int* __restrict__ edge, int par
 // correction mode: dv = dfix as float, and dfix cleared for the next call
 __global__ void __launch_bounds__(kBlock) k_odconv(const int* __restrict__ edge, int parts, size_t n,
                                                    unsigned long long* __restrict__ dfix, float* __restrict__ dv) {
-  const int mode = edge[parts + kE_ANY] ? edge[parts + kE_MODE] : 0;
-  if (mode < 1 || mode > kMaxCombo) return;
+  if (!edge[(size_t)parts * kP_STRIDE + kG_CORR]) return;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     const unsigned long long v = dfix[i];
     dv[i] = (float)((double)v * (1.0 / kDFix));
@@ -884,7 +891,8 @@ __global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ b
 #pragma unroll
   for (int j = 0; j < NV; ++j)
     if (j < nv) vt[j] = vtx[(size_t)pix * nv + j];
-  const int mode = edge.any() ? edge.mode() : 0;
+  const int part = edge.ppart(pix);
+  const int mode = edge.mode(part);
   const int lo = bb & 15, up = bb >> 4;
   const int ilo = (max(lo, 1) - 1) * L + k0;  // LO[lo] (bin 0 is Tb)
   const int iup = min(up, 8) * L + k0;        // LO[up + 1], or UP[up] (bin 9 is Tb)
@@ -898,7 +906,7 @@ __global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ b
       u[j] = ld_v(lo_v, base + iup);
     }
   }
-  const bool full = mode > kMaxCombo && edge.pixel(pix);
+  const bool full = mode > kMaxCombo;
   if (full)  // the upper rows were splatted and blurred in full: UP[up]
 #pragma unroll
     for (int j = 0; j < NV; ++j)
@@ -906,7 +914,7 @@ __global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ b
   int ci = -1;  // correction row of combo (k0, up + 1), if any
   if (mode >= 1 && mode <= kMaxCombo && up < 9)
     for (int c = 0; c < mode; ++c)
-      if (edge.flag[edge.parts + kE_COMBO + c] == k0 * 16 + up + 1) ci = c;
+      if (edge.combo(part, c) == k0 * 16 + up + 1) ci = c;
   float lv[NV], uv[NV];
 #pragma unroll
   for (int j = 0; j < NV; ++j) {
@@ -1204,7 +1212,7 @@ void Lattice::ensure_total_weight() {
   }
   tb_.resize(M_ + 1);
   DCRF_HIP(hipMemcpyAsync(tb_.get(), src, (M_ + 1) * sizeof(float), hipMemcpyDeviceToDevice, s));
-  edge_.resize(nparts_ + kE_COUNT);
+  edge_.resize(edge_ints(nparts_));
   DCRF_HIP(hipStreamSynchronize(s));
   check_launch("total weight");
   have_tb_ = true;
@@ -1221,7 +1229,7 @@ void Lattice::ensure_ordered_tables() {
   check_launch("k_ovtx");
   // bin-edge correction: the edge list, fixed-point sums (kept at zero between
   // calls) and two float row buffers for the blur (row 0 stays zero)
-  elist_.resize(kMaxEdgeList);
+  elist_.resize((size_t)nparts_ * kPartEdges);
   const size_t nd = (size_t)(M_ + 1) * kMaxCombo;
   dfix_.resize(nd);
   dvA_.resize(nd);
@@ -1269,8 +1277,8 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   const int LP = obin_stride(L);
   if (shared) {
     shared->bins.resize((size_t)N_ * LP);
-    shared->edge.resize(nparts_ + kE_COUNT);
-    shared->elist.resize(kMaxEdgeList);
+    shared->edge.resize(edge_ints(nparts_));
+    shared->elist.resize((size_t)nparts_ * kPartEdges);
   } else {
     bins_.resize((size_t)N_ * LP);
   }
@@ -1280,7 +1288,7 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   const EdgeMap em{edge, pimg_.n ? pimg_.get() : nullptr, ppimg_.n ? ppimg_.get() : nullptr, nparts_};
   if (!shared || fresh) {
     KTimer tm("obin" + tag, s);
-    k_zero_ints<<<1, kBlock, 0, s>>>(edge, nparts_ + kE_COUNT);
+    k_zero_ints<<<1, kBlock, 0, s>>>(edge, (int)edge_ints(nparts_));
     k_obin<<<grid_for(NL, kBlock, 2048), kBlock, 0, s>>>(Q, d_minmax, NL, bins, LP, edge,
                                                          pimg_.n ? pimg_.get() : nullptr, FastDiv((uint32_t)L),
                                                          nparts_, elist);
@@ -1288,7 +1296,7 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   {  // bin edges: combos and mode (the same for every lattice of a group);
      // in correction mode this lattice's correction rows
     KTimer tm("oedge" + tag, s);
-    k_ocombo<<<1, 1024, 0, s>>>(edge, nparts_, d_ + 1, elist, offset_.get(), bary_.get(), dfix_.get());
+    k_ocombo<<<nparts_, 1024, 0, s>>>(edge, nparts_, d_ + 1, elist, offset_.get(), bary_.get(), dfix_.get());
     const size_t nd = (size_t)(M_ + 1) * kMaxCombo;
     k_odconv<<<grid_for(nd, kBlock, 2048), kBlock, 0, s>>>(edge, nparts_, nd, dfix_.get(), dvA_.get());
     check_launch("k_ocombo");

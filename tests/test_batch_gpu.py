@@ -144,6 +144,36 @@ def test_batch_lp_matches_one_image_path(prox, fw):
         assert abs(eb[i] - e1) <= 1e-4 * abs(e1)
 
 
+def test_batch_lp_with_bin_edges_matches_one_image_path():
+    """each image's initial Q puts 0.5 / 0.5 on its own label pairs (0.5 lies
+    on a bin edge): 6 (label, bin) pairs per image, 10 over the batch.  Every
+    image must take the same edge mode as alone (correction of the upper
+    pass), so the batched LP equals the one-image LP bit for bit."""
+    import densecrf_ho_amd as D
+    L = 21
+    wls = _workloads(L)
+    b = _batch(wls, L)
+    rng = np.random.default_rng(9)
+    parts = []
+    for i, wl in enumerate(wls):
+        q = np.zeros((wl.N, L), np.float32)
+        q[np.arange(wl.N), rng.integers(0, L, wl.N)] = 1.0
+        for j in range(3):
+            pix = rng.choice(wl.N, max(4, wl.N // 20), replace=False)
+            q[pix] = 0.0
+            q[pix, 2 * i + 2 * j] = 0.5
+            q[pix, 2 * i + 2 * j + 1] = 0.5
+        parts.append(q)
+    Q0 = np.ascontiguousarray(np.concatenate(parts))
+    P = D.LP_inf_params(prox_max_iter=2, fw_max_iter=3)
+    Ql, _ = b.lp(Q0, P)
+    for i, wl in enumerate(wls):
+        print("image", i, flush=True)
+        c = _single(wl, L)
+        q1 = c.lp_inference_prox_super_pixels(parts[i], P)
+        np.testing.assert_array_equal(b.image(Ql, i), q1)
+
+
 def test_bf16_rows_ordered_filter_error():
     """bfloat16 storage of the ordered filter's lattice rows: the output moves
     by the rows' rounding (relative error below 1e-2 of the FP64 oracle's
