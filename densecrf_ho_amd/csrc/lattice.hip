@@ -516,7 +516,8 @@ template <class T>
 __global__ void __launch_bounds__(kBlock) k_combine_if(int V, int RS, RowMap rm, const int* __restrict__ multi,
                                                        int nmulti, const int* __restrict__ chstart,
                                                        const int* __restrict__ pstart, const float* __restrict__ part,
-                                                       T* __restrict__ values, EdgeMap edge, int upper) {
+                                                       const int* __restrict__ orow, T* __restrict__ values,
+                                                       EdgeMap edge, int upper) {
   if (upper && !edge.any_full()) return;
   const long nvb = (long)((nmulti + rm.gpw - 1) / rm.gpw * 64 + blockDim.x - 1) / blockDim.x;
   for (long vb = xcd_block(); vb < nvb; vb += gridDim.x) {
@@ -530,7 +531,7 @@ __global__ void __launch_bounds__(kBlock) k_combine_if(int V, int RS, RowMap rm,
     for (int k = k0; k < V; k += ks) {
       float acc = part[(size_t)c0 * V + k];
       for (int c = c0 + 1; c < c1; ++c) acc = acc + part[(size_t)c * V + k];
-      st_v(values, (size_t)(p + 1) * RS + k, acc);
+      st_v(values, (size_t)orow[p] * RS + k, acc);
     }
   }
 }
@@ -644,13 +645,54 @@ __device__ inline float rescaled(const float* Q, size_t idx, const float* mm) {
 constexpr int kOBins = 9;      // stored bins per label
 
 // Static per lattice, for the ordered slice: each (pixel, vertex) as
-// {point row (int bits), barycentric weight, the point's total weight, 0}
+// {row of the point in the ordered state (int bits), barycentric weight, the
+// point's total weight, 0}
 __global__ void __launch_bounds__(kBlock) k_ovtx(const int* __restrict__ offset, const float* __restrict__ bary,
-                                                 const float* __restrict__ Tb, long E, float4* __restrict__ vtx) {
+                                                 const float* __restrict__ Tb, const int* __restrict__ orow, long E,
+                                                 float4* __restrict__ vtx) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= E) return;
-  const int of = offset[t] + 1;
-  vtx[t] = make_float4(__int_as_float(of), bary[t], Tb[of], 0.f);
+  const int p = offset[t];
+  vtx[t] = make_float4(__int_as_float(orow[p]), bary[t], Tb[p + 1], 0.f);
+}
+
+// Morton code of each point's key (bit b of coordinate j at bit b*d + j,
+// min(64/d, 30) bits per coordinate, offset to be non-negative; wider keys
+// wrap, which only loosens the locality)
+__global__ void __launch_bounds__(kBlock) k_morton(const int16_t* __restrict__ keys, int M, int d,
+                                                   unsigned long long* __restrict__ code, int* __restrict__ ids) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= M) return;
+  const int bits = min(64 / d, 30);
+  unsigned long long c = 0;
+  for (int j = 0; j < d; ++j) {
+    const uint32_t v = (uint32_t)((int)keys[(size_t)p * d + j] + (1 << (bits - 1)));
+    for (int b = 0; b < bits; ++b) c |= (unsigned long long)((v >> b) & 1u) << (b * d + j);
+  }
+  code[p] = c;
+  ids[p] = p;
+}
+// rows from the sorted order: orow[order[r]] = r + 1; the part of each row
+__global__ void __launch_bounds__(kBlock) k_orow(const int* __restrict__ order, int M,
+                                                 const uint16_t* __restrict__ ppimg, int* __restrict__ orow,
+                                                 uint16_t* __restrict__ oppart) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= M) return;
+  orow[order[r]] = r + 1;
+  if (ppimg) oppart[r] = ppimg[order[r]];
+}
+// blur neighbours of row r along every axis, as rows (row - 1, -1 if absent)
+__global__ void __launch_bounds__(kBlock) k_onbr(const int* __restrict__ nbr, const int* __restrict__ order,
+                                                 const int* __restrict__ orow, int M, int d1,
+                                                 int* __restrict__ onbr) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)M * d1) return;
+  const int j = (int)(t / M), r = (int)(t - (long)j * M);
+  const int i = order[r];
+  for (int h = 0; h < 2; ++h) {
+    const int n = nbr[((size_t)j * M + i) * 2 + h];
+    onbr[((size_t)j * M + r) * 2 + h] = n >= 0 ? orow[n] - 1 : -1;
+  }
 }
 
 // One lane per (point chunk, quad of labels): each entry's bins of the four
@@ -665,7 +707,8 @@ __global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ b
                                                    long nlanes, const int* __restrict__ sp_pix,
                                                    const float* __restrict__ sp_w, const int* __restrict__ seg,
                                                    const int* __restrict__ chstart, const int* __restrict__ chpoint,
-                                                   const int* __restrict__ pstart, int nchunks, T* __restrict__ values,
+                                                   const int* __restrict__ pstart, int nchunks,
+                                                   const int* __restrict__ orow, T* __restrict__ values,
                                                    float* __restrict__ part, EdgeMap edge) {
   constexpr bool kUpper = TAG >= 100;
   if (kUpper && !edge.any_full()) return;
@@ -711,7 +754,7 @@ __global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ b
       }
     }
     if (nc == 1) {  // the point's row
-      T* dst = values + (size_t)(p + 1) * RS;
+      T* dst = values + (size_t)orow[p] * RS;
 #pragma unroll
       for (int kk = 0; kk < 4; ++kk)
         if (4 * q + kk < L)
@@ -818,7 +861,7 @@ __global__ void __launch_bounds__(kBlock) k_obin(const float* __restrict__ Q, co
 constexpr double kDFix = 1099511627776.0;  // 2^40
 __global__ void __launch_bounds__(1024) k_ocombo(int* __restrict__ edge, int parts, int d1,
                                                  const int2* __restrict__ elist, const int* __restrict__ offset,
-                                                 const float* __restrict__ bary,
+                                                 const float* __restrict__ bary, const int* __restrict__ orow,
                                                  unsigned long long* __restrict__ dfix) {
   __shared__ int wsum[16];
   const int part = blockIdx.x;
@@ -850,7 +893,7 @@ __global__ void __launch_bounds__(1024) k_ocombo(int* __restrict__ edge, int par
     for (int c = 0; c < total; ++c)
       if (P[kP_COMBO + c] == ev.y) ci = c;
     for (int j = 0; j < d1; ++j) {
-      const int of = offset[(size_t)ev.x * d1 + j] + 1;
+      const int of = orow[offset[(size_t)ev.x * d1 + j]];
       const unsigned long long v = (unsigned long long)llrint((double)bary[(size_t)ev.x * d1 + j] * kDFix);
       atomicAdd(dfix + (size_t)of * kMaxCombo + ci, v);
     }
@@ -896,16 +939,23 @@ __global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ b
   const int lo = bb & 15, up = bb >> 4;
   const int ilo = (max(lo, 1) - 1) * L + k0;  // LO[lo] (bin 0 is Tb)
   const int iup = min(up, 8) * L + k0;        // LO[up + 1], or UP[up] (bin 9 is Tb)
-  // the lower rows' values first: they depend on the bins and vertices only
+  // the lower rows' values first: they depend on the bins and vertices only.
+  // A lane loads only what it uses (bin 0 is Tb, bin 10 is 0): most labels
+  // have lo = 0 and read just their bin-1 slot, so a load touches fewer lines.
   float a[NV], u[NV];
 #pragma unroll
   for (int j = 0; j < NV; ++j) {
-    if (j < nv) {
-      const size_t base = (size_t)__float_as_int(vt[j].x) * RS;
-      a[j] = ld_v(lo_v, base + ilo);
-      u[j] = ld_v(lo_v, base + iup);
-    }
+    a[j] = 0.f;
+    u[j] = 0.f;
   }
+  if (lo > 0)
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+      if (j < nv) a[j] = ld_v(lo_v, (size_t)__float_as_int(vt[j].x) * RS + ilo);
+  if (up < 9)
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+      if (j < nv) u[j] = ld_v(lo_v, (size_t)__float_as_int(vt[j].x) * RS + iup);
   const bool full = mode > kMaxCombo;
   if (full)  // the upper rows were splatted and blurred in full: UP[up]
 #pragma unroll
@@ -1223,9 +1273,37 @@ void Lattice::ensure_total_weight() {
 void Lattice::ensure_ordered_tables() {
   if (have_otab_) return;
   hipStream_t s = stream_;
+  // row order: each part's points by the Morton code of their keys
+  {
+    const int M = M_;
+    DBuf<unsigned long long> code(std::max(M, 1)), code_s(std::max(M, 1));
+    DBuf<int> ids(std::max(M, 1)), order(std::max(M, 1));
+    k_morton<<<grid_for(M), kBlock, 0, s>>>(keys_.get(), M, d_, code.get(), ids.get());
+    std::vector<int> off = part_pt_off_.empty() ? std::vector<int>{0, M} : part_pt_off_;
+    DBuf<int> doff(off.size());
+    DCRF_HIP(hipMemcpyAsync(doff.get(), off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    const int nseg = (int)off.size() - 1;
+    size_t tmp = 0;
+    DCRF_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, tmp, code.get(), code_s.get(), ids.get(),
+                                                         order.get(), M, nseg, doff.get(), doff.get() + 1, 0, 64,
+                                                         s));
+    DBuf<char> t(tmp + 1);
+    DCRF_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(t.get(), tmp, code.get(), code_s.get(), ids.get(),
+                                                         order.get(), M, nseg, doff.get(), doff.get() + 1, 0, 64,
+                                                         s));
+    orow_.resize(std::max(M, 1));
+    if (ppimg_.n) oppart_.resize(std::max(M, 1));
+    k_orow<<<grid_for(M), kBlock, 0, s>>>(order.get(), M, ppimg_.n ? ppimg_.get() : nullptr, orow_.get(),
+                                          oppart_.n ? oppart_.get() : nullptr);
+    onbr_.resize((size_t)(d_ + 1) * M * 2 + 1);
+    k_onbr<<<grid_for((size_t)M * (d_ + 1)), kBlock, 0, s>>>(nbr_.get(), order.get(), orow_.get(), M, d_ + 1,
+                                                             onbr_.get());
+    check_launch("ordered row order");
+    DCRF_HIP(hipStreamSynchronize(s));  // temporaries are released on return
+  }
   const long E = (long)N_ * (d_ + 1);
   vtx_.resize(std::max<long>(E, 1));
-  k_ovtx<<<grid_for(E), kBlock, 0, s>>>(offset_.get(), bary_.get(), tb_.get(), E, vtx_.get());
+  k_ovtx<<<grid_for(E), kBlock, 0, s>>>(offset_.get(), bary_.get(), tb_.get(), orow_.get(), E, vtx_.get());
   check_launch("k_ovtx");
   // bin-edge correction: the edge list, fixed-point sums (kept at zero between
   // calls) and two float row buffers for the blur (row 0 stays zero)
@@ -1286,6 +1364,8 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   int* edge = shared ? shared->edge.get() : edge_.get();
   int2* elist = shared ? shared->elist.get() : elist_.get();
   const EdgeMap em{edge, pimg_.n ? pimg_.get() : nullptr, ppimg_.n ? ppimg_.get() : nullptr, nparts_};
+  // the blur works on rows of the ordered state: row neighbours, the part of each row
+  const EdgeMap emr{edge, em.pimg, oppart_.n ? oppart_.get() : nullptr, nparts_};
   if (!shared || fresh) {
     KTimer tm("obin" + tag, s);
     k_zero_ints<<<1, kBlock, 0, s>>>(edge, (int)edge_ints(nparts_));
@@ -1296,7 +1376,8 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   {  // bin edges: combos and mode (the same for every lattice of a group);
      // in correction mode this lattice's correction rows
     KTimer tm("oedge" + tag, s);
-    k_ocombo<<<nparts_, 1024, 0, s>>>(edge, nparts_, d_ + 1, elist, offset_.get(), bary_.get(), dfix_.get());
+    k_ocombo<<<nparts_, 1024, 0, s>>>(edge, nparts_, d_ + 1, elist, offset_.get(), bary_.get(), orow_.get(),
+                                      dfix_.get());
     const size_t nd = (size_t)(M_ + 1) * kMaxCombo;
     k_odconv<<<grid_for(nd, kBlock, 2048), kBlock, 0, s>>>(edge, nparts_, nd, dfix_.get(), dvA_.get());
     check_launch("k_ocombo");
@@ -1311,8 +1392,8 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
       const unsigned nvb = grid_for((size_t)nlanes);
       auto launch = [&](auto kern, unsigned grid) {
         kern<<<grid, kBlock, 0, s>>>(bins, L, LP, row, nlanes, sp_pix_.get(), sp_w_.get(), seg_.get(),
-                                     chstart_.get(), chpoint_.get(), pstart_.get(), nchunks_, dst_half, part_.get(),
-                                     em);
+                                     chstart_.get(), chpoint_.get(), pstart_.get(), nchunks_, orow_.get(), dst_half,
+                                     part_.get(), em);
       };
       auto run = [&](auto kl, auto ku) {
         if (upper) {
@@ -1334,7 +1415,7 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
         const RowMap rw = rowmap(pv);
         const unsigned g = upper ? std::min(rows_grid(nmulti_, pv), 1024u) : rows_grid(nmulti_, pv);
         k_combine_if<T><<<g, kBlock, 0, s>>>(pv, row, rw, multi_.get(), nmulti_, chstart_.get(), pstart_.get(),
-                                             part_.get(), dst_half, em, upper);
+                                             part_.get(), orow_.get(), dst_half, em, upper);
       }
     }
     check_launch("k_osplat");
@@ -1348,12 +1429,12 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
     const unsigned full = grid_for(nquads);
     // src -> dst, the same bits for any grid: tuned on the lower rows
     const unsigned g = Tuner::get().pick(d_ == 5 ? "oblur5" : "oblur", M_, grid_candidates(full), s, [&](unsigned gg) {
-      kern<<<gg, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, half_q, (uint32_t)M_, nb, a, b, dsrc, ddst, em);
+      kern<<<gg, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, half_q, (uint32_t)M_, nb, a, b, dsrc, ddst, emr);
     });
-    kern<<<g, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, half_q, (uint32_t)M_, nb, a, b, dsrc, ddst, em);
+    kern<<<g, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, half_q, (uint32_t)M_, nb, a, b, dsrc, ddst, emr);
   };
   for (int j = 0; j <= d_; ++j) {  // forward order only (:621, :669)
-    const int* nb = nbr_.get() + (size_t)j * M_ * 2;
+    const int* nb = onbr_.get() + (size_t)j * M_ * 2;
     {
       KTimer tm("oblur" + tag, s);
       if (d_ == 2)
@@ -1422,6 +1503,8 @@ std::unique_ptr<Lattice> Lattice::concat(const std::vector<const Lattice*>& part
   u->offset_.resize(E);
   u->bary_.resize(E);
   u->nbr_.resize((size_t)d1 * M * 2);
+  u->keys_.resize((size_t)M * d);
+  u->part_pt_off_.assign(1, 0);
   u->sp_pix_.resize(E + kOBatch);  // padding read by the ordered splat's last batch: pixel 0, weight 0
   u->sp_w_.resize(E + kOBatch);
   DCRF_HIP(hipMemsetAsync(u->sp_pix_.get() + E, 0, kOBatch * sizeof(int), s));
@@ -1455,6 +1538,10 @@ std::unique_ptr<Lattice> Lattice::concat(const std::vector<const Lattice*>& part
     if (p.nmulti_ > 0) add(p.multi_.get(), u->multi_.get() + ub, p.nmulti_, mb, 0);
     k_fill_u16<<<grid_for(p.N_), kBlock, 0, s>>>(u->pimg_.get() + pb, p.N_, (uint16_t)b);
     if (m) k_fill_u16<<<grid_for(m), kBlock, 0, s>>>(u->ppimg_.get() + mb, m, (uint16_t)b);
+    if (m)
+      DCRF_HIP(hipMemcpyAsync(u->keys_.get() + (size_t)mb * d, p.keys_.get(), (size_t)m * d * sizeof(int16_t),
+                              hipMemcpyDeviceToDevice, s));
+    u->part_pt_off_.push_back((int)(mb + m));
     pb += p.N_;
     mb += m;
     eb += e;

@@ -99,7 +99,8 @@ class Lattice {
     for (const void* p : {(const void*)vA_.get(), (const void*)vB_.get(), (const void*)part_.get(),
                           (const void*)bins_.get(), (const void*)tb_.get(), (const void*)edge_.get(),
                           (const void*)vtx_.get(), (const void*)elist_.get(), (const void*)dfix_.get(),
-                          (const void*)dvA_.get(), (const void*)dvB_.get()})
+                          (const void*)dvA_.get(), (const void*)dvB_.get(), (const void*)orow_.get(),
+                          (const void*)onbr_.get(), (const void*)oppart_.get()})
       out.push_back(p);
   }
 
@@ -146,6 +147,14 @@ class Lattice {
   void ensure_ordered_tables();
   bool have_otab_ = false;
   DBuf<float4> vtx_;
+  // Row order of the ordered state: each part's points sorted by the Morton
+  // code of their keys, so lattice neighbours sit close in memory (and in one
+  // XCD's L2).  orow_: point -> row (1-based, row 0 is the zero row); onbr_:
+  // the blur neighbours in rows ((d+1) x M pairs, row - 1, -1 if absent);
+  // oppart_: part of each row (union lattices).
+  DBuf<int> orow_, onbr_;
+  DBuf<uint16_t> oppart_;
+  std::vector<int> part_pt_off_;  // first point of each part and M (union lattices)
   // bin-edge correction of the ordered filter's upper pass: listed edge
   // values, fixed-point sums, correction rows (ping-pong through the blur)
   DBuf<int2> elist_;
