@@ -931,9 +931,18 @@ __global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ b
   const SlicePre pre = slice_pre(o, (int)pix, k0, L);
   const int bb = bins[(size_t)pix * LP + k0];
   float4 vt[NV];
+  if (L >= nv) {  // lane j of the pixel's label group loads vertex j's record; the group shares them by shuffles
+    float4 mine = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (k0 < nv) mine = vtx[(size_t)pix * nv + k0];
+    const int g0 = (int)(threadIdx.x & 63) - k0;
 #pragma unroll
-  for (int j = 0; j < NV; ++j)
-    if (j < nv) vt[j] = vtx[(size_t)pix * nv + j];
+    for (int j = 0; j < NV; ++j)
+      if (j < nv) vt[j] = make_float4(__shfl(mine.x, g0 + j), __shfl(mine.y, g0 + j), __shfl(mine.z, g0 + j), 0.f);
+  } else {
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+      if (j < nv) vt[j] = vtx[(size_t)pix * nv + j];
+  }
   const int part = edge.ppart(pix);
   const int mode = edge.mode(part);
   const int lo = bb & 15, up = bb >> 4;
