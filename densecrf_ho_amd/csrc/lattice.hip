@@ -832,25 +832,34 @@ __global__ void __launch_bounds__(kBlock) k_blur2(int R4, FastDiv divR4, uint32_
 // each part)
 // A lattice that is the union of several images' lattices rescales each
 // image with its own {min, max}: pimg[pixel] picks the pair mm[2*img].
-__global__ void __launch_bounds__(kBlock) k_obin(const float* __restrict__ Q, const float* __restrict__ mm, size_t n,
-                       uint8_t* __restrict__ bins, int LP, int* __restrict__ edge, const uint16_t* __restrict__ pimg,
-                       FastDiv divL, int parts, int2* __restrict__ elist) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const uint32_t pix = divL.div((uint32_t)i);
+__global__ void __launch_bounds__(kBlock) k_obin(const float* __restrict__ Q, const float* __restrict__ mm, int N,
+                                                 int L, FastDiv divNQ, uint8_t* __restrict__ bins, int LP,
+                                                 int* __restrict__ edge, const uint16_t* __restrict__ pimg,
+                                                 int2* __restrict__ elist) {
+  // one lane per (pixel, quad of labels): four bins, one 4-byte store (padding labels get 0)
+  const uint32_t nq = divNQ.d, n = (uint32_t)N * nq;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
+    const uint32_t pix = divNQ.div(g), q = g - pix * nq;
     const int img = pimg ? pimg[pix] : 0;
-    const float q = rescaled(Q, i, mm + 2 * img);
-    const int lo = bin_lo(q), up = bin_up(q);
-    const uint32_t k = (uint32_t)(i - (size_t)pix * divL.d);
-    if (lo != up) {  // on a bin edge: flag its part and its combo, and list it
-      int* P = edge + (size_t)img * kP_STRIDE;
-      if (P[kP_EDGE] == 0) atomicOr(P + kP_EDGE, 1);
-      const int code = (int)k * 16 + lo;
-      const int bw = (int)(1u << (code & 31));
-      if ((P[kP_BITS + (code >> 5)] & bw) == 0) atomicOr(P + kP_BITS + (code >> 5), bw);
-      const int at = atomicAdd(P + kP_NLIST, 1);
-      if (at < kPartEdges) elist[(size_t)img * kPartEdges + at] = make_int2((int)pix, code);
+    uint32_t word = 0;
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const int k = 4 * (int)q + kk;
+      if (k >= L) break;
+      const float qv = rescaled(Q, (size_t)pix * L + k, mm + 2 * img);
+      const int lo = bin_lo(qv), up = bin_up(qv);
+      if (lo != up) {  // on a bin edge: flag its part and its combo, and list it
+        int* P = edge + (size_t)img * kP_STRIDE;
+        if (P[kP_EDGE] == 0) atomicOr(P + kP_EDGE, 1);
+        const int code = k * 16 + lo;
+        const int bwd = (int)(1u << (code & 31));
+        if ((P[kP_BITS + (code >> 5)] & bwd) == 0) atomicOr(P + kP_BITS + (code >> 5), bwd);
+        const int at = atomicAdd(P + kP_NLIST, 1);
+        if (at < kPartEdges) elist[(size_t)img * kPartEdges + at] = make_int2((int)pix, code);
+      }
+      word |= (uint32_t)(lo | (up << 4)) << (8 * kk);
     }
-    bins[(size_t)pix * LP + k] = (uint8_t)(lo | (up << 4));
+    *reinterpret_cast<uint32_t*>(bins + (size_t)pix * LP + 4 * q) = word;
   }
 }
 
@@ -1360,7 +1369,6 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   k_zero<<<grid_for(zf), kBlock, 0, s>>>(reinterpret_cast<float*>(srcU), zf);
   k_zero<<<grid_for(zf), kBlock, 0, s>>>(reinterpret_cast<float*>(dstU), zf);
   const std::string tag = prof_on() ? "[d=" + std::to_string(d_) + ",L=" + std::to_string(L) + "]" : std::string();
-  const size_t NL = (size_t)N_ * L;
   const int LP = obin_stride(L);
   if (shared) {
     shared->bins.resize((size_t)N_ * LP);
@@ -1378,9 +1386,10 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   if (!shared || fresh) {
     KTimer tm("obin" + tag, s);
     k_zero_ints<<<1, kBlock, 0, s>>>(edge, (int)edge_ints(nparts_));
-    k_obin<<<grid_for(NL, kBlock, 2048), kBlock, 0, s>>>(Q, d_minmax, NL, bins, LP, edge,
-                                                         pimg_.n ? pimg_.get() : nullptr, FastDiv((uint32_t)L),
-                                                         nparts_, elist);
+    const uint32_t nq = (uint32_t)(L + 3) / 4;
+    k_obin<<<grid_for((size_t)N_ * nq, kBlock, 16384), kBlock, 0, s>>>(Q, d_minmax, N_, L, FastDiv(nq), bins, LP,
+                                                                        edge, pimg_.n ? pimg_.get() : nullptr,
+                                                                        elist);
   }
   {  // bin edges: combos and mode (the same for every lattice of a group);
      // in correction mode this lattice's correction rows
