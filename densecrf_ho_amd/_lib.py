@@ -96,6 +96,7 @@ SIGNATURES = {
     "dcrf_memcpy": (_i, [_vp, _vp, C.c_size_t, _i]),
     "dcrf_device_synchronize": (_i, []),
     "dcrf_profile_enable": (_i, [_i]),
+    "dcrf_profile_filter": (_i, [C.c_char_p]),
     "dcrf_profile_report": (_i, [C.c_char_p, _i]),
     "dcrf_autotune_enable": (_i, [_i]),
     "dcrf_autotune_report": (_i, [C.c_char_p, _i]),

@@ -80,6 +80,11 @@ def profile_enable(on: bool = True):
     _check(_lib.load().dcrf_profile_enable(int(on)))
 
 
+def profile_filter(prefix: str = ""):
+    """time only the launches whose name starts with `prefix` ("" for all)"""
+    _check(_lib.load().dcrf_profile_filter(prefix.encode() if prefix else None))
+
+
 def profile_report() -> dict:
     """{kernel name: (launch count, total ms)} since profile_enable()."""
     L = _lib.load()

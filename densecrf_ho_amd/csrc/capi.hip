@@ -617,6 +617,9 @@ int dcrf_profile_enable(int on) {
     Profiler::get().on.store(on != 0);
   });
 }
+int dcrf_profile_filter(const char* prefix) {
+  return guarded([&] { Profiler::get().prefix = prefix ? prefix : ""; });
+}
 // ---- batches ------------------------------------------------------------------
 int dcrf_batch_create(int L, int B, const int* npix, dcrf_batch_t* out) {
   return guarded([&] {

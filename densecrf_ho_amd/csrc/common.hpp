@@ -10,6 +10,7 @@
 #include <map>
 #include <mutex>
 #include <stdexcept>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -62,6 +63,7 @@ struct Profiler {
     hipEvent_t a, b;
   };
   std::atomic<bool> on{false};
+  std::string prefix;  // when not empty, only launches whose timer name starts with it are timed
   std::mutex mu;
   std::vector<Rec> recs;
   std::map<std::string, std::pair<long, double>> done;  // name -> (count, ms)
@@ -114,6 +116,8 @@ struct KTimer {
   std::string name;
   KTimer(const char* n, hipStream_t st) : s(st) {
     if (!Profiler::get().on.load(std::memory_order_relaxed)) return;
+    const std::string& pf = Profiler::get().prefix;
+    if (!pf.empty() && std::strncmp(n, pf.c_str(), pf.size()) != 0) return;
     name = n;
     if (hipEventCreate(&a) != hipSuccess || hipEventRecord(a, s) != hipSuccess) a = nullptr;
   }

@@ -291,6 +291,10 @@ int dcrf_device_synchronize(void);
  * report writes lines "name count total_ms" into buf (NUL-terminated) and
  * returns the length it needs (call with len 0 to size the buffer). */
 int dcrf_profile_enable(int on);
+/* Time only the launches whose name starts with `prefix` (NULL or "" for
+ * all), so that timing one kernel costs the others nothing.  No reference
+ * counterpart. */
+int dcrf_profile_filter(const char* prefix);
 /* Launch-shape autotuner (no reference counterpart): grid-stride kernels whose
  * results do not depend on the grid are timed over a few candidate grids at
  * their first launch for a problem size and keep the fastest.  enable(0)
