@@ -108,3 +108,36 @@ def test_config0_lp_vs_reference(golden):
     print("C0 LP: agreement", agree, "energy", e, "reference", ref["e_true_lp"], flush=True)
     assert agree >= 0.999
     assert abs(e - ref["e_true_lp"]) <= 1e-4 * abs(ref["e_true_lp"])
+
+
+def test_qp_lp_with_superpixels_vs_fp64_oracle():
+    """qp_sp then lp_sp at 128x96 with three Voronoi superpixel layers (12, 24
+    and 48 regions, Pn weight 0.5, LP multiplier 0.2) against the FP64 oracle:
+    labels >= 99.9 %, higher-order integer energies within 1e-4"""
+    import densecrf_ho_amd as D
+    from densecrf_ho_amd import synthetic as S
+    W, H, L = 128, 96, 21
+    wl = S.Workload(W, H, 12, 7, L=L)
+    layers = S.superpixel_layers(W, H, cells=(12, 24, 48), seeds=(31, 32, 33))
+    c = D.DenseCRF2D(W, H, L)
+    c.setUnaryEnergy(wl.U)
+    o = O.CRF(wl.N, L)
+    o.set_unary(wl.U)
+    for f, w in wl.kernels:
+        c.addPairwiseEnergy(f, D.PottsCompatibility(w))
+        o.add_pairwise(f, w)
+    for region, _ in layers:
+        c.addSuperPixelLayer(region, 0.5, 0.2)
+        o.add_layer(region, 0.5, 0.2)
+    q0 = O.unary_init(wl.U)
+    qg = c.qp_inference_super_pixels_non_convex(q0, max_iter=10, ref_stop=False)
+    qo, _ = o.qp(q0, max_iter=10, fp64=True)
+    print("qp label agreement", (qg.argmax(1) == qo.argmax(1)).mean(), flush=True)
+    assert (qg.argmax(1) == qo.argmax(1)).mean() >= 0.999
+    P = D.LP_inf_params(prox_max_iter=3, fw_max_iter=5)
+    lg = c.lp_inference_prox_super_pixels(qg, P).argmax(1).astype(np.int32)
+    lo = o.lp(qo, O.LPParams(prox_max_iter=3, fw_max_iter=5), fp64=True)[0].argmax(1).astype(np.int32)
+    eg, eo = o.energy_ho(lg, fp64=True), o.energy_ho(lo, fp64=True)
+    print("lp label agreement", (lg == lo).mean(), "energies", eg, eo, flush=True)
+    assert (lg == lo).mean() >= 0.999
+    assert abs(eg - eo) <= 1e-4 * abs(eo)
