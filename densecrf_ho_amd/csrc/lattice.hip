@@ -463,8 +463,33 @@ __device__ inline uint16_t f2bf(float f) {
 }
 __device__ inline float ld_v(const float* p, size_t i) { return p[i]; }
 __device__ inline float ld_v(const uint16_t* p, size_t i) { return __uint_as_float((uint32_t)p[i] << 16); }
+// two adjacent row values from a 4-byte (float) or 2-byte (bfloat16) aligned
+// position: one 8-byte load of floats, two 2-byte loads of bfloat16
+struct __attribute__((packed, aligned(4))) FloatPair {
+  float x, y;
+};
+__device__ inline float2 ld_pair(const float* p, size_t i) {
+  const FloatPair v = *reinterpret_cast<const FloatPair*>(p + i);
+  return make_float2(v.x, v.y);
+}
+__device__ inline float2 ld_pair(const uint16_t* p, size_t i) { return make_float2(ld_v(p, i), ld_v(p, i + 1)); }
 __device__ inline void st_v(float* p, size_t i, float v) { p[i] = v; }
 __device__ inline void st_v(uint16_t* p, size_t i, float v) { p[i] = f2bf(v); }
+// four consecutive row values: float4, or four bfloat16 in a uint2
+__device__ inline float4 ld_quad(const float* p, size_t q) { return reinterpret_cast<const float4*>(p)[q]; }
+__device__ inline float4 ld_quad(const uint16_t* p, size_t q) {
+  const uint2 v = reinterpret_cast<const uint2*>(p)[q];
+  return make_float4(__uint_as_float(v.x << 16), __uint_as_float(v.x & 0xffff0000u), __uint_as_float(v.y << 16),
+                     __uint_as_float(v.y & 0xffff0000u));
+}
+__device__ inline void st_quad(float* p, size_t q, float4 v) { reinterpret_cast<float4*>(p)[q] = v; }
+__device__ inline void st_quad(uint16_t* p, size_t q, float4 v) {
+  uint2 o;
+  o.x = (uint32_t)f2bf(v.x) | ((uint32_t)f2bf(v.y) << 16);
+  o.y = (uint32_t)f2bf(v.z) | ((uint32_t)f2bf(v.w) << 16);
+  reinterpret_cast<uint2*>(p)[q] = o;
+}
+
 __device__ inline float2 ld_pair(const float* row, long k) { return reinterpret_cast<const float2*>(row)[k]; }
 __device__ inline float2 ld_pair(const uint16_t* row, long k) {
   const uint32_t v = reinterpret_cast<const uint32_t*>(row)[k];
@@ -510,8 +535,8 @@ struct EdgeMap {
 };
 
 // k_combine of the lower pass, or of the upper pass (upper != 0) for the
-// points of parts with bin edges only: partial rows of V values, point rows of
-// stride RS
+// points of parts with bin edges only: V values of partial rows and point rows
+// of stride RS
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_combine_if(int V, int RS, RowMap rm, const int* __restrict__ multi,
                                                        int nmulti, const int* __restrict__ chstart,
@@ -529,8 +554,8 @@ __global__ void __launch_bounds__(kBlock) k_combine_if(int V, int RS, RowMap rm,
     if (upper && !edge.point_full(p)) continue;
     const int c0 = pstart[p], c1 = c0 + chstart[p + 1] - chstart[p];
     for (int k = k0; k < V; k += ks) {
-      float acc = part[(size_t)c0 * V + k];
-      for (int c = c0 + 1; c < c1; ++c) acc = acc + part[(size_t)c * V + k];
+      float acc = part[(size_t)c0 * RS + k];
+      for (int c = c0 + 1; c < c1; ++c) acc = acc + part[(size_t)c * RS + k];
       st_v(values, (size_t)orow[p] * RS + k, acc);
     }
   }
@@ -632,17 +657,25 @@ __device__ inline float rescaled(const float* Q, size_t idx, const float* mm) {
   return q;
 }
 
-// Ordered filter state, bin-major rows of orow_stride(L) values per lattice
+// Ordered filter state, label-major rows of orow_stride(L) values per lattice
 // point.  The lower pass adds w to bins 0..bin_lo (:548-569), so its bin 0
 // holds every entry's weight: it equals the point's total weight Tb, which is
 // splatted and blurred with the same float operations in the same order, and
-// is not stored.  LO rows hold bins 1..9, bin b of label k at (b-1)*L + k.
+// is not stored.  LO rows hold bins 1..9 of every label, label-major (bin b of
+// label k at k*9 + b - 1) or bin-major (at (b-1)*L + k), chosen per lattice.
 // The upper pass (+w to bins bin_up..9) equals Tb - LO[bin+1] whenever
 // bin_up == bin_lo for every (pixel, label); when some q of a part sits on a
 // bin edge (bin_up == bin_lo - 1) the upper pass of that part is splatted and
-// blurred in full into UP rows holding bins 0..8 (bin 9 is Tb), bin b at b*L + k.
-// Either way the slice finds the upper value of bin c at c*L + k.
+// blurred in full into UP rows holding bins 0..8 (bin 9 is Tb), bin b at k*9 + b
+// (label-major) or b*L + k (bin-major).
 constexpr int kOBins = 9;      // stored bins per label
+// Row layout per lattice kernel specialisation: label-major for the d = 2
+// kernels (three vertices per pixel: the slice's one 8-byte load per vertex
+// pays), bin-major otherwise (with six vertices the label-major slice touches
+// every line of a row per vertex and runs slower than two masked 4-byte loads
+// on bin-major rows).  The splat and the slice of one lattice use the same
+// specialisation, so they agree.
+constexpr bool label_major(int tag) { return tag == 2; }
 
 // Static per lattice, for the ordered slice: each (pixel, vertex) as
 // {row of the point in the ordered state (int bits), barycentric weight, the
@@ -718,6 +751,7 @@ __global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ b
                                                    const int* __restrict__ orow, T* __restrict__ values,
                                                    float* __restrict__ part, EdgeMap edge) {
   constexpr bool kUpper = TAG >= 100;
+  constexpr bool lmaj = label_major(TAG % 100);
   if (kUpper && !edge.any_full()) return;
   const int nq = (L + 3) >> 2;
   for (long g = (long)xcd_block() * blockDim.x + threadIdx.x; g < nlanes; g += (long)gridDim.x * blockDim.x) {
@@ -760,39 +794,44 @@ __global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ b
         }
       }
     }
+    // label-major rows: bin slot r of label k at k * kOBins + r, so the
+    // lane's four labels are 36 consecutive values (nine aligned quads);
+    // bin-major rows: at r * L + k
+    auto slot = [&](int f) { return acc[f / kOBins][f % kOBins]; };
+    const bool quads = lmaj && 4 * q + 4 <= L;
+    auto at = [&](int kk, int r) { return lmaj ? (size_t)(4 * q + kk) * kOBins + r : (size_t)r * L + 4 * q + kk; };
     if (nc == 1) {  // the point's row
-      T* dst = values + (size_t)orow[p] * RS;
+      const size_t r0 = (size_t)orow[p] * RS;
+      if (quads) {
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk)
-        if (4 * q + kk < L)
+        for (int i = 0; i < kOBins; ++i)
+          st_quad(values, r0 / 4 + (size_t)kOBins * q + i,
+                  make_float4(slot(4 * i), slot(4 * i + 1), slot(4 * i + 2), slot(4 * i + 3)));
+      } else {
 #pragma unroll
-          for (int r = 0; r < kOBins; ++r) st_v(dst, (size_t)r * L + 4 * q + kk, acc[kk][r]);
+        for (int kk = 0; kk < 4; ++kk)
+          if (4 * q + kk < L)
+#pragma unroll
+            for (int r = 0; r < kOBins; ++r) st_v(values, r0 + at(kk, r), acc[kk][r]);
+      }
       if (q == 0)
-        for (int x = kOBins * L; x < RS; ++x) st_v(dst, x, 0.f);  // row padding
-    } else {  // a chunk partial (float), summed by k_combine_if
-      float* dst = part + (size_t)(pstart[p] + (c - c0)) * (kOBins * L);
+        for (int x = kOBins * L; x < RS; ++x) st_v(values, r0 + x, 0.f);  // row padding
+    } else {  // a chunk partial (float, rows of stride RS), summed by k_combine_if
+      float* dst = part + (size_t)(pstart[p] + (c - c0)) * RS;
+      if (quads) {
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk)
-        if (4 * q + kk < L)
+        for (int i = 0; i < kOBins; ++i)
+          st_quad(dst, (size_t)kOBins * q + i,
+                  make_float4(slot(4 * i), slot(4 * i + 1), slot(4 * i + 2), slot(4 * i + 3)));
+      } else {
 #pragma unroll
-          for (int r = 0; r < kOBins; ++r) dst[(size_t)r * L + 4 * q + kk] = acc[kk][r];
+        for (int kk = 0; kk < 4; ++kk)
+          if (4 * q + kk < L)
+#pragma unroll
+            for (int r = 0; r < kOBins; ++r) dst[at(kk, r)] = acc[kk][r];
+      }
     }
   }
-}
-
-// four consecutive row values: float4, or four bfloat16 in a uint2
-__device__ inline float4 ld_quad(const float* p, size_t q) { return reinterpret_cast<const float4*>(p)[q]; }
-__device__ inline float4 ld_quad(const uint16_t* p, size_t q) {
-  const uint2 v = reinterpret_cast<const uint2*>(p)[q];
-  return make_float4(__uint_as_float(v.x << 16), __uint_as_float(v.x & 0xffff0000u), __uint_as_float(v.y << 16),
-                     __uint_as_float(v.y & 0xffff0000u));
-}
-__device__ inline void st_quad(float* p, size_t q, float4 v) { reinterpret_cast<float4*>(p)[q] = v; }
-__device__ inline void st_quad(uint16_t* p, size_t q, float4 v) {
-  uint2 o;
-  o.x = (uint32_t)f2bf(v.x) | ((uint32_t)f2bf(v.y) << 16);
-  o.y = (uint32_t)f2bf(v.z) | ((uint32_t)f2bf(v.w) << 16);
-  reinterpret_cast<uint2*>(p)[q] = o;
 }
 
 // one blur axis over the ordered state: one lane per (point, quad of the
@@ -937,6 +976,7 @@ __global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ b
                                                    const T* __restrict__ lo_v, const T* __restrict__ up_v,
                                                    const float* __restrict__ dv, EdgeMap edge, float alpha,
                                                    SliceArgs o) {
+  constexpr bool lmaj = label_major(TAG);
   constexpr int kD1 = TAG == 2 ? 3 : TAG == 5 ? 6 : 0;
   constexpr int NV = kD1 ? kD1 : kMaxD + 1;
   const int nv = kD1 ? kD1 : d1;
@@ -962,30 +1002,52 @@ __global__ void __launch_bounds__(kBlock) k_oslice(const uint8_t* __restrict__ b
   const int part = edge.ppart(pix);
   const int mode = edge.mode(part);
   const int lo = bb & 15, up = bb >> 4;
-  const int ilo = (max(lo, 1) - 1) * L + k0;  // LO[lo] (bin 0 is Tb)
-  const int iup = min(up, 8) * L + k0;        // LO[up + 1], or UP[up] (bin 9 is Tb)
-  // the lower rows' values first: they depend on the bins and vertices only.
-  // A lane loads only what it uses (bin 0 is Tb, bin 10 is 0): most labels
-  // have lo = 0 and read just their bin-1 slot, so a load touches fewer lines.
   float a[NV], u[NV];
-#pragma unroll
-  for (int j = 0; j < NV; ++j) {
-    a[j] = 0.f;
-    u[j] = 0.f;
-  }
-  if (lo > 0)
-#pragma unroll
-    for (int j = 0; j < NV; ++j)
-      if (j < nv) a[j] = ld_v(lo_v, (size_t)__float_as_int(vt[j].x) * RS + ilo);
-  if (up < 9)
-#pragma unroll
-    for (int j = 0; j < NV; ++j)
-      if (j < nv) u[j] = ld_v(lo_v, (size_t)__float_as_int(vt[j].x) * RS + iup);
   const bool full = mode > kMaxCombo;
-  if (full)  // the upper rows were splatted and blurred in full: UP[up]
+  if constexpr (lmaj) {
+    // Label-major rows: LO[b] (b = 1..9) of label k at k * kOBins + b - 1.
+    // The slice needs LO[lo] (bin 0 is Tb) and LO[up + 1] (bin 10 is 0), and
+    // up is lo or lo - 1, so one pair of adjacent slots LO[i0 + 1], LO[i0 + 2]
+    // holds both: one 8-byte load per vertex.
+    const int i0 = min(max(lo, 1), 8) - 1;
+    const size_t ip = (size_t)k0 * kOBins + i0;
+    const bool a_hi = lo - 1 - i0 == 1;  // LO[lo] is the pair's second slot
+    const bool u_hi = up - i0 == 1;      // LO[up + 1] is the pair's second slot
 #pragma unroll
     for (int j = 0; j < NV; ++j)
-      if (j < nv) u[j] = ld_v(up_v, (size_t)__float_as_int(vt[j].x) * RS + iup);
+      if (j < nv) {
+        const float2 pr = ld_pair(lo_v, (size_t)__float_as_int(vt[j].x) * RS + ip);
+        a[j] = a_hi ? pr.y : pr.x;
+        u[j] = u_hi ? pr.y : pr.x;
+      }
+    if (full && up < 9)  // the upper rows were splatted and blurred in full: UP[up] (bins 0..8)
+#pragma unroll
+      for (int j = 0; j < NV; ++j)
+        if (j < nv) u[j] = ld_v(up_v, (size_t)__float_as_int(vt[j].x) * RS + (size_t)k0 * kOBins + up);
+  } else {
+    // Bin-major rows: LO[b] of label k at (b - 1) * L + k (UP[b] at b * L + k).
+    // A lane loads only what it uses: most labels have lo = 0 and read just
+    // their bin-1 slot, one run of L values per row.
+    const int ilo = (max(lo, 1) - 1) * L + k0;  // LO[lo]
+    const int iup = min(up, 8) * L + k0;        // LO[up + 1], or UP[up]
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      a[j] = 0.f;
+      u[j] = 0.f;
+    }
+    if (lo > 0)
+#pragma unroll
+      for (int j = 0; j < NV; ++j)
+        if (j < nv) a[j] = ld_v(lo_v, (size_t)__float_as_int(vt[j].x) * RS + ilo);
+    if (up < 9)
+#pragma unroll
+      for (int j = 0; j < NV; ++j)
+        if (j < nv) u[j] = ld_v(lo_v, (size_t)__float_as_int(vt[j].x) * RS + iup);
+    if (full)  // the upper rows were splatted and blurred in full: UP[up]
+#pragma unroll
+      for (int j = 0; j < NV; ++j)
+        if (j < nv) u[j] = ld_v(up_v, (size_t)__float_as_int(vt[j].x) * RS + iup);
+  }
   int ci = -1;  // correction row of combo (k0, up + 1), if any
   if (mode >= 1 && mode <= kMaxCombo && up < 9)
     for (int c = 0; c < mode; ++c)
