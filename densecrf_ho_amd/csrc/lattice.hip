@@ -739,9 +739,9 @@ __global__ void __launch_bounds__(kBlock) k_onbr(const int* __restrict__ nbr, co
 // labels are one 4-byte word of its pixel's bins row (rows of LP bytes, LP a
 // multiple of 8; labels past L are row padding, computed and dropped).  The
 // chunk's entries are read in batches of kOBatch (all of a batch's loads in
-// flight together; the entry lists hold kOBatch entries of padding, pixel 0
-// with weight 0, and entries past the chunk's end get weight 0) and added in
-// entry order.
+// flight together, the next batch's entry loads in flight during this batch's
+// sums; the entry lists hold kOBatch entries of padding, pixel 0 with weight
+// 0, and entries past the chunk's end get weight 0) and added in entry order.
 template <int TAG, class T>  // TAG: lattice dim, +100 for the upper pass
 __global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ bins, int L, int LP, int RS,
                                                    long nlanes, const int* __restrict__ sp_pix,
@@ -766,20 +766,30 @@ __global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ b
     for (int kk = 0; kk < 4; ++kk)
 #pragma unroll
       for (int r = 0; r < kOBins; ++r) acc[kk][r] = 0.f;
+    int px[kOBatch];
+    float w[kOBatch];
+#pragma unroll
+    for (int i = 0; i < kOBatch; ++i) {
+      px[i] = sp_pix[b + i];
+      w[i] = sp_w[b + i];
+    }
     for (int t0 = b; t0 < e; t0 += kOBatch) {
-      int px[kOBatch];
-      float w[kOBatch];
-#pragma unroll
-      for (int i = 0; i < kOBatch; ++i) {
-        px[i] = sp_pix[t0 + i];
-        w[i] = sp_w[t0 + i];
-      }
       uint32_t bw[kOBatch];
-#pragma unroll
-      for (int i = 0; i < kOBatch; ++i) bw[i] = *reinterpret_cast<const uint32_t*>(bins + (size_t)px[i] * LP + 4 * q);
+      float wc[kOBatch];
 #pragma unroll
       for (int i = 0; i < kOBatch; ++i) {
-        const float wi = t0 + i < e ? w[i] : 0.f;
+        bw[i] = *reinterpret_cast<const uint32_t*>(bins + (size_t)px[i] * LP + 4 * q);
+        wc[i] = t0 + i < e ? w[i] : 0.f;
+      }
+      if (t0 + kOBatch < e)  // the next batch's entries load during this batch's sums
+#pragma unroll
+        for (int i = 0; i < kOBatch; ++i) {
+          px[i] = sp_pix[t0 + kOBatch + i];
+          w[i] = sp_w[t0 + kOBatch + i];
+        }
+#pragma unroll
+      for (int i = 0; i < kOBatch; ++i) {
+        const float wi = wc[i];
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
           const int bv = (bw[i] >> (8 * kk)) & 0xff;
