@@ -476,7 +476,10 @@ std::vector<std::vector<int>> Batch::lp(const float* init, float* out, const LPP
   };
 
   // ---- initialisation (:534-550) ------------------------------------------------
-  k_lp_project<<<nbr, kBlock, 0, s>>>(rm, L, init, lQ_.get(), lab, 0);
+  if (L == 21)
+    k_lp_project_px<21><<<nbr, kBlock, 0, s>>>(rm, init, lQ_.get(), lab, 0);
+  else
+    k_lp_project<<<nbr, kBlock, 0, s>>>(rm, L, init, lQ_.get(), lab, 0);
   DCRF_HIP(hipMemcpyAsync(best_.get(), lQ_.get(), NL * sizeof(float), hipMemcpyDeviceToDevice, s));
   int_energy_dev(lab, sc + kL_EINT, kL_NSC);
   lp_energy_dev(lQ_.get(), sc + kL_TMP + 1, sc + kL_TMP + 2, kL_NSC);
@@ -570,7 +573,10 @@ std::vector<std::vector<int>> Batch::lp(const float* init, float* out, const LPP
     check_err();
     {
       KTimer tm("lp_project", s);
-      k_lp_project<<<nbr, kBlock, 0, s>>>(rm, L, lQ_.get(), lQ_.get(), lab, 1);
+      if (L == 21)
+        k_lp_project_px<21><<<nbr, kBlock, 0, s>>>(rm, lQ_.get(), lQ_.get(), lab, 1);
+      else
+        k_lp_project<<<nbr, kBlock, 0, s>>>(rm, L, lQ_.get(), lQ_.get(), lab, 1);
     }
     int_energy_dev(lab, sc + kL_EINT, kL_NSC);
     kb_lp_outer_decide<<<1, 256, 0, s>>>(sc, ost_.get(), ofl, P.prox_energy_tol, B);

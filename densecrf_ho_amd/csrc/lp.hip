@@ -498,6 +498,167 @@ __global__ void __launch_bounds__(kBlock) k_lp_project(SEG sg, int L, const floa
   }
 }
 
+// ---- the same per pixel, one thread per pixel with its row in registers ----
+// (LT labels known at compile time).  The row moves through LDS so the
+// block's loads and stores coalesce.
+
+// Eigen's sum (the eigen_sum order) of a register row whose first 16-byte
+// aligned element is A
+template <int LT, int A>
+__device__ inline float eigen_sum_a(const float (&x)[LT]) {
+  constexpr int a = A > LT ? LT : A;
+  constexpr int as2 = ((LT - a) / 8) * 8, as1 = ((LT - a) / 4) * 4;
+  constexpr int ae2 = a + as2, ae = a + as1;
+  if constexpr (as1 > 0) {
+    float p0 = x[a], p1 = x[a + 1], p2 = x[a + 2], p3 = x[a + 3];
+    if constexpr (as1 > 4) {
+      float q0 = x[a + 4], q1 = x[a + 5], q2 = x[a + 6], q3 = x[a + 7];
+#pragma unroll
+      for (int i = a + 8; i < ae2; i += 8) {
+        p0 = p0 + x[i];
+        p1 = p1 + x[i + 1];
+        p2 = p2 + x[i + 2];
+        p3 = p3 + x[i + 3];
+        q0 = q0 + x[i + 4];
+        q1 = q1 + x[i + 5];
+        q2 = q2 + x[i + 6];
+        q3 = q3 + x[i + 7];
+      }
+      p0 = p0 + q0;
+      p1 = p1 + q1;
+      p2 = p2 + q2;
+      p3 = p3 + q3;
+      if constexpr (ae > ae2) {
+        p0 = p0 + x[ae2];
+        p1 = p1 + x[ae2 + 1];
+        p2 = p2 + x[ae2 + 2];
+        p3 = p3 + x[ae2 + 3];
+      }
+    }
+    float res = (p0 + p2) + (p1 + p3);
+#pragma unroll
+    for (int i = 0; i < a; ++i) res = res + x[i];
+#pragma unroll
+    for (int i = ae; i < LT; ++i) res = res + x[i];
+    return res;
+  } else {
+    float res = x[0];
+#pragma unroll
+    for (int i = 1; i < LT; ++i) res = res + x[i];
+    return res;
+  }
+}
+template <int LT>
+__device__ inline float eigen_sum_col(const float (&x)[LT], int a) {
+  switch (a) {
+    case 0:
+      return eigen_sum_a<LT, 0>(x);
+    case 1:
+      return eigen_sum_a<LT, 1>(x);
+    case 2:
+      return eigen_sum_a<LT, 2>(x);
+    default:
+      return eigen_sum_a<LT, 3>(x);
+  }
+}
+// descending sort of the first LT of a row padded (in thought) to N = 2^m
+// entries of -inf: Batcher's odd-even merge network, comparators that touch
+// a padding entry dropped (a padding entry never moves below LT).  Only the
+// sorted values are used, so the order among equal values does not matter.
+template <int N, int LT>
+__device__ inline void sort_desc(float (&u)[LT]) {
+#pragma unroll
+  for (int p = 1; p < N; p <<= 1)
+#pragma unroll
+    for (int k = p; k >= 1; k >>= 1)
+#pragma unroll
+      for (int j = k % p; j + k < N; j += 2 * k)
+#pragma unroll
+        for (int i = 0; i < k; ++i) {
+          const int x = i + j, y = i + j + k;
+          if (y < LT && x / (2 * p) == y / (2 * p)) {
+            const float hi = fmaxf(u[x], u[y]), lo = fminf(u[x], u[y]);
+            u[x] = hi;
+            u[y] = lo;
+          }
+        }
+}
+template <int LT>
+constexpr int pow2_at_least() {
+  int n = 1;
+  while (n < LT) n <<= 1;
+  return n;
+}
+template <int LT, class SEG>
+__global__ void __launch_bounds__(kBlock) k_lp_project_px(SEG sg, const float* __restrict__ Qin,
+                                                          float* __restrict__ Qout, int* __restrict__ lab,
+                                                          int do_feasible) {
+  __shared__ float tile[kBlock * LT];
+  const SegPos qs = sg.pos();
+  const int t = threadIdx.x;
+  const long ntiles = (qs.n + kBlock - 1) / kBlock;
+  for (long tl = qs.lb; tl < ntiles; tl += qs.nb) {
+    const long c0 = tl * kBlock;  // the tile's first column within its image
+    const int np = (int)min((long)kBlock, qs.n - c0);
+    const size_t base = (size_t)(qs.p0 + c0) * LT;
+    __syncthreads();  // the previous tile's stores have read the LDS
+    for (int i = t; i < np * LT; i += kBlock) tile[i] = Qin[base + i];
+    __syncthreads();
+    const bool act = t < np;
+    float q[LT], v[LT];
+#pragma unroll
+    for (int l = 0; l < LT; ++l) q[l] = act ? tile[t * LT + l] : 0.f;
+    const int a = col_astart(c0 + t, LT);
+    if (do_feasible) {
+      const float cs = eigen_sum_col<LT>(q, a);
+#pragma unroll
+      for (int l = 0; l < LT; ++l) v[l] = q[l];
+      sort_desc<pow2_at_least<LT>(), LT>(v);
+      // largest k with sum / k < u_k, the sum falling from the top (:480-486)
+      double sum = (double)(float)(cs - 1.f);
+      int K = -1;
+      for (int k = LT; k > 0; --k) {
+        const double uk = (double)v[k - 1];
+        if (sum / k < uk) {
+          K = k;
+          break;
+        }
+        sum -= uk;
+      }
+      const double sk = sum / K;
+#pragma unroll
+      for (int l = 0; l < LT; ++l) {
+        double tt = (double)q[l] - sk;
+        tt = (tt < 0.0) ? 0.0 : tt;  // std::max(t, 0.0)
+        tt = (1.0 < tt) ? 1.0 : tt;  // std::min(t, 1.0)
+        v[l] = (float)tt;
+      }
+    } else {
+#pragma unroll
+      for (int l = 0; l < LT; ++l) v[l] = q[l];
+    }
+    const double sum = (double)eigen_sum_col<LT>(v, a);
+    const bool uni = sum == 0 || sum != sum;
+#pragma unroll
+    for (int l = 0; l < LT; ++l) v[l] = uni ? (float)(1.0 / LT) : v[l] / (float)sum;
+    int m = 0;
+    float best = v[0];
+#pragma unroll
+    for (int l = 1; l < LT; ++l)
+      if (v[l] > best) {
+        best = v[l];
+        m = l;
+      }
+    if (act) {
+      lab[qs.p0 + c0 + t] = m;
+#pragma unroll
+      for (int l = 0; l < LT; ++l) tile[t * LT + l] = v[l];  // the thread's own row
+    }
+    __syncthreads();
+    for (int i = t; i < np * LT; i += kBlock) Qout[base + i] = tile[i];
+  }
+}
+
 // max_rounding (:832-840)
 __global__ void __launch_bounds__(kBlock) k_onehot(const int* __restrict__ lab, int N, int L, float* __restrict__ Q) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -778,7 +939,11 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
     // project, renormalize, labelling (:746-752)
     {
       KTimer tm("lp_project", s);
-      k_lp_project<<<rg_grid(N, L, 8192), kBlock, 0, s>>>(OneImage{N}, L, st.Q.get(), st.Q.get(), st.lab.get(), 1);
+      if (L == 21)
+        k_lp_project_px<21><<<(unsigned)std::max<long>(1, std::min<long>((N + kBlock - 1) / kBlock, 16384)), kBlock, 0, s>>>(
+            OneImage{N}, st.Q.get(), st.Q.get(), st.lab.get(), 1);
+      else
+        k_lp_project<<<rg_grid(N, L, 8192), kBlock, 0, s>>>(OneImage{N}, L, st.Q.get(), st.Q.get(), st.lab.get(), 1);
     }
     const double prev_int = int_energy;
     int_energy_dev(st.lab.get(), sc + kL_EINT, sc + kL_TMP, s);
