@@ -616,6 +616,100 @@ __global__ void __launch_bounds__(kBlock) k_splat_staged(SplatArgs a, int V, Row
   }
 }
 
+// One lane per (chunk, quad of values): each entry's four input values are
+// one 16-byte load at 4-byte alignment (rows of V floats; the last quad of a
+// row, when V % 4 != 0, loads its values one by one).  Entries are read in
+// batches of kOBatch, the next batch's entry loads in flight during this
+// batch's sums, and added in pixel order as the per-value loop does.
+struct __attribute__((packed, aligned(4))) Float4u {
+  float x, y, z, w;
+};
+template <SplatIn KIND>
+__device__ inline float4 splat_in4(const SplatArgs& a, int pix, int k, int V, bool whole) {
+  float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (KIND != SplatIn::kNotOnehot) {
+    const float* r = a.in + (size_t)pix * V + k;
+    if (whole) {
+      const Float4u u = *reinterpret_cast<const Float4u*>(r);
+      x = make_float4(u.x, u.y, u.z, u.w);
+    } else {
+      x.x = r[0];
+      if (k + 1 < V) x.y = r[1];
+      if (k + 2 < V) x.z = r[2];
+    }
+  }
+  if (KIND == SplatIn::kPlain) return x;
+  if (KIND == SplatIn::kScaled) {
+    const float n = a.norm[pix];
+    return make_float4(x.x * n, x.y * n, x.z * n, x.w * n);
+  }
+  const int lb = a.lab[pix];
+  auto oh = [&](int kk) { return lb == k + kk ? 1.f : 0.f; };
+  if (KIND == SplatIn::kNotOnehot) return make_float4(1.f - oh(0), 1.f - oh(1), 1.f - oh(2), 1.f - oh(3));
+  float4 d = make_float4(oh(0) - x.x, oh(1) - x.y, oh(2) - x.z, oh(3) - x.w);
+  if (KIND == SplatIn::kSxScaled) {
+    const float n = a.norm[pix];
+    d = make_float4(d.x * n, d.y * n, d.z * n, d.w * n);
+  }
+  return d;
+}
+template <SplatIn KIND>
+__global__ void __launch_bounds__(kBlock) k_splat_quad(SplatArgs a, int V, long nlanes,
+                                                       const int* __restrict__ sp_pix, const float* __restrict__ sp_w,
+                                                       const int* __restrict__ seg, const int* __restrict__ chstart,
+                                                       const int* __restrict__ chpoint, const int* __restrict__ pstart,
+                                                       float* __restrict__ values, float* __restrict__ part) {
+  const int nq = (V + 3) >> 2;
+  for (long g = (long)xcd_block() * blockDim.x + threadIdx.x; g < nlanes; g += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(g / nq), q = (int)(g - (long)c * nq);
+    const int k = 4 * q;
+    const bool whole = k + 4 <= V;
+    const int p = chpoint[c];
+    const int c0 = chstart[p], nc = chstart[p + 1] - c0;
+    const int b = seg[p] + (c - c0) * kChunk;
+    const int e = min(seg[p + 1], b + kChunk);
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    int px[kOBatch];
+    float w[kOBatch];
+#pragma unroll
+    for (int i = 0; i < kOBatch; ++i) {
+      px[i] = sp_pix[b + i];
+      w[i] = sp_w[b + i];
+    }
+    for (int t0 = b; t0 < e; t0 += kOBatch) {
+      float4 x[kOBatch];
+      float wc[kOBatch];
+#pragma unroll
+      for (int i = 0; i < kOBatch; ++i) {
+        x[i] = splat_in4<KIND>(a, px[i], k, V, whole);
+        wc[i] = w[i];
+      }
+      if (t0 + kOBatch < e)
+#pragma unroll
+        for (int i = 0; i < kOBatch; ++i) {
+          px[i] = sp_pix[t0 + kOBatch + i];
+          w[i] = sp_w[t0 + kOBatch + i];
+        }
+#pragma unroll
+      for (int i = 0; i < kOBatch; ++i)
+        if (t0 + i < e) {
+          acc.x = acc.x + wc[i] * x[i].x;
+          acc.y = acc.y + wc[i] * x[i].y;
+          acc.z = acc.z + wc[i] * x[i].z;
+          acc.w = acc.w + wc[i] * x[i].w;
+        }
+    }
+    float* dst = (nc == 1 ? values + (size_t)(p + 1) * V : part + (size_t)(pstart[p] + (c - c0)) * V) + k;
+    if (whole) {
+      *reinterpret_cast<Float4u*>(dst) = Float4u{acc.x, acc.y, acc.z, acc.w};
+    } else {
+      dst[0] = acc.x;
+      if (k + 1 < V) dst[1] = acc.y;
+      if (k + 2 < V) dst[2] = acc.z;
+    }
+  }
+}
+
 // slice with the pixel's d+1 (offset, weight) pairs staged in LDS (V >= d+1)
 template <bool SEQ>
 __global__ void __launch_bounds__(kBlock) k_slice_staged(int V, RowMap rm, int N, int d1, const int* __restrict__ offset,
@@ -1283,9 +1377,9 @@ void Lattice::filter(const SplatArgs& in, const SliceArgs& out, int V, bool reve
 #define DCRF_SPLAT(K)                                                                                             \
   case K:                                                                                                         \
     if (stage)                                                                                                    \
-      k_splat_staged<K><<<rows_grid(nchunks_, V), kBlock, 0, s>>>(in, V, rm, sp_pix_.get(), sp_w_.get(),        \
-                                                                  seg_.get(), chstart_.get(), chpoint_.get(),    \
-                                                                  pstart_.get(), nchunks_, src, part_.get());    \
+      k_splat_quad<K><<<grid_for((size_t)nchunks_ * ((V + 3) / 4), kBlock, 16384), kBlock, 0, s>>>(             \
+          in, V, (long)nchunks_ * ((V + 3) / 4), sp_pix_.get(), sp_w_.get(), seg_.get(), chstart_.get(),         \
+          chpoint_.get(), pstart_.get(), src, part_.get());                                                      \
     else                                                                                                          \
       k_splat<K><<<rows_grid(nchunks_, V), kBlock, 0, s>>>(in, V, rm, sp_pix_.get(), sp_w_.get(), seg_.get(),   \
                                                            chstart_.get(), chpoint_.get(), pstart_.get(),        \
