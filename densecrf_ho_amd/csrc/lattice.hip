@@ -565,56 +565,8 @@ __global__ void __launch_bounds__(kBlock) k_fill_f(float* __restrict__ p, size_t
   if (t < n) p[t] = v;
 }
 
-// ---- staged variants (V >= kStageMinV) ------------------------------------------
-// The group's V lanes copy V consecutive splat entries (pixel, weight) into a
-// block-shared LDS slice per round; the entry loop then reads LDS and issues
-// the value gathers back to back.  The summation order (pixel order) is the
-// same as the unstaged loop.
+// ---- wide-row variants (V >= kStageMinV) ----------------------------------------
 constexpr int kStageMinV = 8;
-
-template <SplatIn KIND>
-__global__ void __launch_bounds__(kBlock) k_splat_staged(SplatArgs a, int V, RowMap rm, const int* __restrict__ sp_pix,
-                               const float* __restrict__ sp_w, const int* __restrict__ seg,
-                               const int* __restrict__ chstart, const int* __restrict__ chpoint,
-                               const int* __restrict__ pstart, int nchunks, float* __restrict__ values,
-                               float* __restrict__ part) {
-  __shared__ int spx[kBlock];
-  __shared__ float swt[kBlock];
-  long c;
-  int k0, ks;
-  rm.at(V, c, k0, ks);
-  const bool act = c >= 0 && c < nchunks;
-  const int gb = (threadIdx.x & ~63) + (k0 >= 0 ? ((threadIdx.x & 63) / V) * V : 0);
-  int b = 0, e = 0, p = 0, c0 = 0, nc = 1;
-  if (act) {
-    p = chpoint[c];
-    c0 = chstart[p];
-    nc = chstart[p + 1] - c0;
-    b = seg[p] + (int)(c - c0) * kChunk;
-    e = min(seg[p + 1], b + kChunk);
-  }
-  float acc = 0.f;
-  const int rounds = (kChunk + V - 1) / V;
-  for (int r = 0; r < rounds; ++r) {
-    const int t0 = b + r * V;
-    if (act) {
-      const int t = t0 + k0;
-      spx[gb + k0] = t < e ? sp_pix[t] : 0;
-      swt[gb + k0] = t < e ? sp_w[t] : 0.f;
-    }
-    __syncthreads();
-    if (act) {
-      const int n = min(V, e - t0);
-#pragma unroll 8
-      for (int j = 0; j < n; ++j) acc = acc + swt[gb + j] * splat_in<KIND>(a, spx[gb + j], k0, V);
-    }
-    __syncthreads();
-  }
-  if (act) {
-    float* dst = nc == 1 ? values + (size_t)(p + 1) * V : part + (size_t)(pstart[p] + (c - c0)) * V;
-    dst[k0] = acc;
-  }
-}
 
 // One lane per (chunk, quad of values): each entry's four input values are
 // one 16-byte load at 4-byte alignment (rows of V floats; the last quad of a
@@ -707,6 +659,87 @@ __global__ void __launch_bounds__(kBlock) k_splat_quad(SplatArgs a, int V, long 
       if (k + 1 < V) dst[1] = acc.y;
       if (k + 2 < V) dst[2] = acc.z;
     }
+  }
+}
+
+// One lane per (pixel, quad of values) (V >= 8, non-SEQ rounding): each
+// vertex row's four values are one 16-byte load at 4-byte alignment (the last
+// quad of a row, when V % 4 != 0, loads its values one by one).  When the
+// pixel has at most as many vertices as quads, lane j of the pixel's group
+// loads vertex j's (offset, weight) and the group shares them by shuffles.
+__device__ inline float4 ld_row4(const float* r, int k, int V, bool whole) {
+  if (whole) {
+    const Float4u u = *reinterpret_cast<const Float4u*>(r);
+    return make_float4(u.x, u.y, u.z, u.w);
+  }
+  float4 x = make_float4(r[0], 0.f, 0.f, 0.f);
+  if (k + 1 < V) x.y = r[1];
+  if (k + 2 < V) x.z = r[2];
+  return x;
+}
+__global__ void __launch_bounds__(kBlock) k_slice_quad(int V, long nlanes, int N, int d1,
+                                                       const int* __restrict__ offset, const float* __restrict__ bary,
+                                                       const float* __restrict__ values, float alpha, SliceArgs o) {
+  const int nq = (V + 3) >> 2;
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long gc = g < nlanes ? g : nlanes - 1;  // lanes past the end mirror the last one and store nothing
+  const int pix = (int)(gc / nq), q = (int)(gc - (long)pix * nq);
+  const int k = 4 * q;
+  const bool whole = k + 4 <= V;
+  // a pixel's lanes are consecutive; its group may straddle two waves only
+  // when nq does not divide 64, so the shared loads need the whole group in
+  // this wave: lanes whose group starts before this wave's lane 0 or ends
+  // after lane 63 load their own vertices
+  const int lane = (int)(threadIdx.x & 63);
+  const int gl0 = lane - q;
+  const bool shared = d1 <= nq && gl0 >= 0 && gl0 + nq <= 64;
+  int myo = 0;
+  float myw = 0.f;
+  if (q < d1) {
+    myo = offset[(size_t)pix * d1 + q] + 1;
+    myw = bary[(size_t)pix * d1 + q];
+  }
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int j = 0; j < d1; ++j) {
+    int off;
+    float w;
+    const int so = __shfl(myo, gl0 + j), sw = __float_as_int(__shfl(myw, gl0 + j));
+    if (shared) {
+      off = so;
+      w = __int_as_float(sw);
+    } else {
+      off = offset[(size_t)pix * d1 + j] + 1;
+      w = bary[(size_t)pix * d1 + j];
+    }
+    const float4 v = ld_row4(values + (size_t)off * V + k, k, V, whole);
+    const float wa = w * alpha;
+    acc.x = acc.x + wa * v.x;
+    acc.y = acc.y + wa * v.y;
+    acc.z = acc.z + wa * v.z;
+    acc.w = acc.w + wa * v.w;
+  }
+  if (g >= nlanes) return;
+  float x[4] = {acc.x, acc.y, acc.z, acc.w};
+  const float nm = o.norm ? o.norm[pix] : 1.f;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (o.norm) x[i] = x[i] * nm;
+    if (o.apply_scale) x[i] = o.scale * x[i];
+    if (o.post_mult != 1.f) x[i] = o.post_mult * x[i];
+  }
+  float* dst = o.out + (size_t)pix * V + k;
+  if (whole) {
+    if (o.accumulate) {
+      const Float4u u = *reinterpret_cast<const Float4u*>(dst);
+      x[0] = u.x + x[0];
+      x[1] = u.y + x[1];
+      x[2] = u.z + x[2];
+      x[3] = u.w + x[3];
+    }
+    *reinterpret_cast<Float4u*>(dst) = Float4u{x[0], x[1], x[2], x[3]};
+  } else {
+    for (int i = 0; i < 3; ++i)
+      if (k + i < V) dst[i] = o.accumulate ? dst[i] + x[i] : x[i];
   }
 }
 
@@ -1413,7 +1446,12 @@ void Lattice::filter(const SplatArgs& in, const SliceArgs& out, int V, bool reve
   }
   const float alpha = lattice_alpha(d_);
   KTimer tm("slice" + tag, s);
-  if (V >= d_ + 1 && V <= 64) {
+  if (!seq && V >= kStageMinV && V <= 64) {
+    const long nl = (long)N_ * ((V + 3) / 4);
+    if (nl > 0)
+      k_slice_quad<<<grid_for((size_t)nl), kBlock, 0, s>>>(V, nl, N_, d_ + 1, offset_.get(), bary_.get(), src, alpha,
+                                                           out);
+  } else if (V >= d_ + 1 && V <= 64) {
     if (seq)
       k_slice_staged<true><<<rows_grid(N_, V), kBlock, 0, s>>>(V, rm, N_, d_ + 1, offset_.get(), bary_.get(), src,
                                                                alpha, out);
