@@ -389,26 +389,6 @@ __global__ void __launch_bounds__(kBlock) k_blur(int V, RowMap rm, int M, const 
   }
 }
 
-// float4 form for wide rows (ordered filter state), same arithmetic
-__global__ void __launch_bounds__(kBlock) k_blur4(int V4, int M, const int* __restrict__ nbr, const float4* __restrict__ src,
-                        float4* __restrict__ dst) {
-  const long wave = ((long)xcd_block() * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (wave >= M) return;
-  const long i = wave;
-  const int n1 = nbr[2 * i] + 1, n2 = nbr[2 * i + 1] + 1;
-  for (int k = lane; k < V4; k += 64) {
-    const float4 o = src[(size_t)(i + 1) * V4 + k];
-    const float4 a = src[(size_t)n1 * V4 + k];
-    const float4 b = src[(size_t)n2 * V4 + k];
-    float4 r;
-    r.x = o.x + 0.5f * (a.x + b.x);
-    r.y = o.y + 0.5f * (a.y + b.y);
-    r.z = o.z + 0.5f * (a.z + b.z);
-    r.w = o.w + 0.5f * (a.w + b.w);
-    dst[(size_t)(i + 1) * V4 + k] = r;
-  }
-}
 
 // slice_store with its two reads (norm, previous output) issued earlier by the caller
 struct SlicePre {
@@ -740,6 +720,32 @@ __global__ void __launch_bounds__(kBlock) k_slice_quad(int V, long nlanes, int N
   } else {
     for (int i = 0; i < 3; ++i)
       if (k + i < V) dst[i] = o.accumulate ? dst[i] + x[i] : x[i];
+  }
+}
+
+// one blur axis over rows of V >= 8 floats, one lane per (row, quad of
+// values), the arithmetic of k_blur<false>
+__global__ void __launch_bounds__(kBlock) k_blur_quad(int V, FastDiv divNQ, uint32_t nl,
+                                                      const int* __restrict__ nbr, const float* __restrict__ src,
+                                                      float* __restrict__ dst) {
+  const uint32_t nq = divNQ.d, stride = gridDim.x * blockDim.x;
+  for (uint32_t t = xcd_block() * blockDim.x + threadIdx.x; t < nl; t += stride) {
+    const uint32_t i = divNQ.div(t);
+    const int k = 4 * (int)(t - i * nq);
+    const bool whole = k + 4 <= V;
+    const int n1 = nbr[2 * i] + 1, n2 = nbr[2 * i + 1] + 1;
+    const float4 o = ld_row4(src + (size_t)(i + 1) * V + k, k, V, whole);
+    const float4 a = ld_row4(src + (size_t)n1 * V + k, k, V, whole);
+    const float4 b = ld_row4(src + (size_t)n2 * V + k, k, V, whole);
+    const float r[4] = {o.x + 0.5f * (a.x + b.x), o.y + 0.5f * (a.y + b.y), o.z + 0.5f * (a.z + b.z),
+                        o.w + 0.5f * (a.w + b.w)};
+    float* d = dst + (size_t)(i + 1) * V + k;
+    if (whole) {
+      *reinterpret_cast<Float4u*>(d) = Float4u{r[0], r[1], r[2], r[3]};
+    } else {
+      for (int j = 0; j < 3; ++j)
+        if (k + j < V) d[j] = r[j];
+    }
   }
 }
 
@@ -1433,12 +1439,17 @@ void Lattice::filter(const SplatArgs& in, const SliceArgs& out, int V, bool reve
   }
   }
   const bool seq = V <= 2;
+  const long bl = (long)M_ * ((V + 3) / 4);  // quad-blur lanes
+  DCRF_REQUIRE(bl < (1L << 31), "filter: lattice too large for one blur launch");
   for (int jj = 0; jj <= d_; ++jj) {
     const int j = reverse ? d_ - jj : jj;
     const int* nb = nbr_.get() + (size_t)j * M_ * 2;
     KTimer tm("blur" + tag, s);
     if (seq)
       k_blur<true><<<rows_grid(M_, V), kBlock, 0, s>>>(V, rm, M_, nb, src, dst);
+    else if (V >= kStageMinV && bl > 0)
+      k_blur_quad<<<grid_for((size_t)bl, kBlock, 16384), kBlock, 0, s>>>(V, FastDiv((uint32_t)((V + 3) / 4)),
+                                                                         (uint32_t)bl, nb, src, dst);
     else
       k_blur<false><<<rows_grid(M_, V), kBlock, 0, s>>>(V, rm, M_, nb, src, dst);
     check_launch("k_blur");
