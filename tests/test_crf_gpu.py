@@ -109,3 +109,31 @@ def test_config0_reference_stopping_rule(golden):
     assert (lab == g["lab_qp"]).mean() >= 0.999
     e = c.assignment_energy_true(lab)
     assert abs(e - ref["e_true_qp"]) <= 1e-4 * abs(ref["e_true_qp"])
+
+
+@pytest.mark.parametrize("L", [10, 11])
+def test_potentials_wide_rows_vs_oracle(L):
+    """Potentials at label counts whose rows end in a partial quad (L % 4 = 2
+    or 3): normalised, transposed and no-norm filters, accumulating over two
+    terms in the QP, against the FP64 oracle."""
+    import densecrf_ho_amd as D
+    from densecrf_ho_amd import synthetic as S
+    wl = S.Workload(80, 60, 12, 9, L=L)
+    c = D.DenseCRF2D(80, 60, L)
+    c.setUnaryEnergy(wl.U)
+    orc = O.CRF(wl.N, L)
+    orc.set_unary(wl.U)
+    for f, w in wl.kernels:
+        c.addPairwiseEnergy(f, D.PottsCompatibility(w))
+        orc.add_pairwise(f, w)
+    x = np.random.default_rng(L).standard_normal((wl.N, L)).astype(np.float32)
+    for k in range(2):
+        print("L", L, "potential", k, flush=True)
+        assert rel_err(c.apply(k, x), orc.apply(k, x, fp64=True)) < 1e-4
+        assert rel_err(c.apply(k, x, transpose=True), orc.apply(k, x, fp64=True, transpose=True)) < 1e-4
+        assert rel_err(c.apply(k, x, no_norm=True), orc.apply(k, x, fp64=True, no_norm=True)) < 1e-4
+    Q0 = O.unary_init(wl.U)
+    Qd, std = orc.qp(Q0, max_iter=5, fp64=True)
+    Qg, stg = c.qp_inference_super_pixels_non_convex(Q0, max_iter=5, ref_stop=False, stats=True)
+    np.testing.assert_allclose(stg["energy"], std["energy"], rtol=1e-4)
+    assert (Qg.argmax(1) == Qd.argmax(1)).mean() >= 0.999

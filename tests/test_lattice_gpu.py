@@ -98,3 +98,23 @@ def test_ordered_filter_rejects_out_of_range(golden):
     q[5, 2] = -0.25
     with pytest.raises(D.EngineError, match="Value out of bounds"):
         lat.compute_upper_minus_lower_ord(q)
+
+
+@pytest.mark.parametrize("V", [8, 9, 10, 11, 12, 24, 64])
+@pytest.mark.parametrize("d", [2, 5])
+def test_wide_rows_vs_oracle(V, d):
+    """Value widths V >= 8 take the kernels with one lane per quad of values:
+    whole quads and a last quad of 1, 2 or 3 values (V % 4), forward and
+    reverse blur order, against the oracle's float and FP64 filters."""
+    from densecrf_ho_amd import synthetic as S
+    print("case V", V, "d", d, flush=True)
+    wl = S.Workload(96, 64, 16, 5)
+    feat = wl.kernels[0 if d == 2 else 1][0]
+    ref = O.Lattice(feat)
+    lat = gpu_lattice(feat)
+    rng = np.random.default_rng(V * 10 + d)
+    x = rng.standard_normal((wl.N, V)).astype(np.float32)
+    for rev in (False, True):
+        out = lat.compute(x, reverse=rev)
+        assert rel_err(out, ref.filter(x, reverse=rev)) < 1e-5
+        assert rel_err(out, ref.filter(x.astype(np.float64), reverse=rev, fp64=True)) < 1e-4
