@@ -463,6 +463,16 @@ __device__ inline float4 ld_quad(const uint16_t* p, size_t q) {
                      __uint_as_float(v.y & 0xffff0000u));
 }
 __device__ inline void st_quad(float* p, size_t q, float4 v) { reinterpret_cast<float4*>(p)[q] = v; }
+// streaming store (not kept in L2 for reuse): the blur's output rows
+typedef float F4v __attribute__((ext_vector_type(4)));
+typedef uint32_t U2v __attribute__((ext_vector_type(2)));
+__device__ inline void st_quad_nt(float* p, size_t q, float4 v) {
+  __builtin_nontemporal_store(F4v{v.x, v.y, v.z, v.w}, reinterpret_cast<F4v*>(p) + q);
+}
+__device__ inline void st_quad_nt(uint16_t* p, size_t q, float4 v) {
+  const U2v o{(uint32_t)f2bf(v.x) | ((uint32_t)f2bf(v.y) << 16), (uint32_t)f2bf(v.z) | ((uint32_t)f2bf(v.w) << 16)};
+  __builtin_nontemporal_store(o, reinterpret_cast<U2v*>(p) + q);
+}
 __device__ inline void st_quad(uint16_t* p, size_t q, float4 v) {
   uint2 o;
   o.x = (uint32_t)f2bf(v.x) | ((uint32_t)f2bf(v.y) << 16);
@@ -470,15 +480,6 @@ __device__ inline void st_quad(uint16_t* p, size_t q, float4 v) {
   reinterpret_cast<uint2*>(p)[q] = o;
 }
 
-__device__ inline float2 ld_pair(const float* row, long k) { return reinterpret_cast<const float2*>(row)[k]; }
-__device__ inline float2 ld_pair(const uint16_t* row, long k) {
-  const uint32_t v = reinterpret_cast<const uint32_t*>(row)[k];
-  return make_float2(__uint_as_float(v << 16), __uint_as_float(v & 0xffff0000u));
-}
-__device__ inline void st_pair(float* row, long k, float2 v) { reinterpret_cast<float2*>(row)[k] = v; }
-__device__ inline void st_pair(uint16_t* row, long k, float2 v) {
-  reinterpret_cast<uint32_t*>(row)[k] = (uint32_t)f2bf(v.x) | ((uint32_t)f2bf(v.y) << 16);
-}
 
 // Bin-edge flags of the ordered filter, one per part of the lattice (one
 // image, or each image of a union) plus flag[parts] = "any part".  A part
@@ -984,7 +985,7 @@ __global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ b
 // further) of that part's points (a point's neighbours are in its own part)
 template <class T>
 __device__ inline void blur_quad(const int* __restrict__ nbr, const T* __restrict__ src, T* __restrict__ dst,
-                                 size_t base, uint32_t i, uint32_t R4, uint32_t q) {
+                                 size_t base, uint32_t i, uint32_t R4, uint32_t q, bool nt) {
   const int n1 = nbr[2 * i] + 1, n2 = nbr[2 * i + 1] + 1;
   const float4 o = ld_quad(src, base + (size_t)(i + 1) * R4 + q);
   const float4 a = ld_quad(src, base + (size_t)n1 * R4 + q);
@@ -994,24 +995,27 @@ __device__ inline void blur_quad(const int* __restrict__ nbr, const T* __restric
   r.y = o.y + 0.5f * (a.y + b.y);
   r.z = o.z + 0.5f * (a.z + b.z);
   r.w = o.w + 0.5f * (a.w + b.w);
-  st_quad(dst, base + (size_t)(i + 1) * R4 + q, r);
+  if (nt)
+    st_quad_nt(dst, base + (size_t)(i + 1) * R4 + q, r);
+  else
+    st_quad(dst, base + (size_t)(i + 1) * R4 + q, r);
 }
 template <int TAG, class T>
 __global__ void __launch_bounds__(kBlock) k_blur2(int R4, FastDiv divR4, uint32_t nq, size_t half_q, uint32_t M,
                                                   const int* __restrict__ nbr, const T* __restrict__ src,
                                                   T* __restrict__ dst, const float* __restrict__ dsrc,
-                                                  float* __restrict__ ddst, EdgeMap edge) {
+                                                  float* __restrict__ ddst, EdgeMap edge, int nt) {
   const uint32_t stride = gridDim.x * blockDim.x, t0 = xcd_block() * blockDim.x + threadIdx.x;
   for (uint32_t t = t0; t < nq; t += stride) {  // lower rows
     const uint32_t i = divR4.div(t);
-    blur_quad(nbr, src, dst, 0, i, (uint32_t)R4, t - i * (uint32_t)R4);
+    blur_quad(nbr, src, dst, 0, i, (uint32_t)R4, t - i * (uint32_t)R4, nt);
   }
   if (edge.any_corr())  // correction rows: two quads per point
-    for (uint32_t t = t0; t < 2 * M; t += stride) blur_quad(nbr, dsrc, ddst, 0, t >> 1, 2u, t & 1u);
+    for (uint32_t t = t0; t < 2 * M; t += stride) blur_quad(nbr, dsrc, ddst, 0, t >> 1, 2u, t & 1u, false);
   if (edge.any_full())  // upper rows of the points of parts in full mode
     for (uint32_t t = t0; t < nq; t += stride) {
       const uint32_t i = divR4.div(t);
-      if (edge.point_full(i)) blur_quad(nbr, src, dst, half_q, i, (uint32_t)R4, t - i * (uint32_t)R4);
+      if (edge.point_full(i)) blur_quad(nbr, src, dst, half_q, i, (uint32_t)R4, t - i * (uint32_t)R4, nt);
     }
 }
 
@@ -1682,24 +1686,27 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   DCRF_REQUIRE(nquads < (1u << 31), "ordered filter: lattice too large for one blur launch");
   const FastDiv divR4((uint32_t)R4);
   const size_t half_q = half / 4;  // quads from a lower row to its upper row
-  auto blur = [&](auto kern, const int* nb, const T* a, T* b) {
+  // the last axis streams its rows out (non-temporal stores): the slice that
+  // reads them then finds no dirty lines of the blur to write back in L2
+  auto blur = [&](auto kern, const int* nb, const T* a, T* b, int nt) {
     const unsigned full = grid_for(nquads);
     // src -> dst, the same bits for any grid: tuned on the lower rows
     const unsigned g = Tuner::get().pick(d_ == 5 ? "oblur5" : "oblur", M_, grid_candidates(full), s, [&](unsigned gg) {
-      kern<<<gg, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, half_q, (uint32_t)M_, nb, a, b, dsrc, ddst, emr);
+      kern<<<gg, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, half_q, (uint32_t)M_, nb, a, b, dsrc, ddst, emr, nt);
     });
-    kern<<<g, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, half_q, (uint32_t)M_, nb, a, b, dsrc, ddst, emr);
+    kern<<<g, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, half_q, (uint32_t)M_, nb, a, b, dsrc, ddst, emr, nt);
   };
   for (int j = 0; j <= d_; ++j) {  // forward order only (:621, :669)
     const int* nb = onbr_.get() + (size_t)j * M_ * 2;
     {
       KTimer tm("oblur" + tag, s);
+      const int nt = j == d_;
       if (d_ == 2)
-        blur(k_blur2<2, T>, nb, srcL, dstL);
+        blur(k_blur2<2, T>, nb, srcL, dstL, nt);
       else if (d_ == 5)
-        blur(k_blur2<5, T>, nb, srcL, dstL);
+        blur(k_blur2<5, T>, nb, srcL, dstL, nt);
       else
-        blur(k_blur2<0, T>, nb, srcL, dstL);
+        blur(k_blur2<0, T>, nb, srcL, dstL, nt);
     }
     check_launch("k_blur2");
     std::swap(srcL, dstL);
