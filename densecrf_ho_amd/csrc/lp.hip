@@ -392,8 +392,25 @@ __global__ void __launch_bounds__(kBlock) k_lp_gap(SEG sg, int L, const float* _
                          double* __restrict__ parts) {
   const SegPos qs = sg.pos();
   double v[2] = {0.0, 0.0};
-  const size_t e = (size_t)(qs.p0 + qs.n) * L;
-  for (size_t i = (size_t)qs.p0 * L + (size_t)qs.lb * blockDim.x + threadIdx.x; i < e; i += (size_t)qs.nb * blockDim.x) {
+  const size_t e = (size_t)(qs.p0 + qs.n) * L, step = (size_t)qs.nb * blockDim.x;
+  size_t i = (size_t)qs.p0 * L + (size_t)qs.lb * blockDim.x + threadIdx.x;
+  // four elements' loads in flight together, then their terms in order
+  for (; i + 3 * step < e; i += 4 * step) {
+    float t[4], q[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const size_t ij = i + j * step;
+      const float uu = u ? u[ij] : 0.f, ss = su ? su[ij] : 0.f;
+      t[j] = (alpha[ij] + uu) - (sa[ij] + ss);
+      q[j] = Q[ij];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      v[0] += (double)t[j] * (double)q[j];
+      v[1] += (double)t[j] * (double)t[j];
+    }
+  }
+  for (; i < e; i += step) {
     const float uu = u ? u[i] : 0.f, ss = su ? su[i] : 0.f;
     const float t = (alpha[i] + uu) - (sa[i] + ss);
     v[0] += (double)t * (double)Q[i];
