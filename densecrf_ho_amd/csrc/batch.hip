@@ -286,6 +286,8 @@ void Batch::ensure_state() {
   lab_.resize(N_);
   const int nparts = std::max(red_.nblocks, sweep_.nblocks);
   parts_.resize((size_t)std::max(kSweepBatchB, 3) * nparts + 16);
+  static_assert(kSweepBatchB == kSweepBatch, "per-image sweep totals are kSweepBatch apart");
+  swtot_.resize((size_t)B_ * kSweepBatchB);
   sc_.resize((size_t)B_ * kS_COUNT);
   fl_.resize((size_t)B_ * kF_COUNT);
 }
@@ -520,8 +522,8 @@ std::vector<std::vector<int>> Batch::lp(const float* init, float* out, const LPP
               DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbs, kBlock, 0, s>>>(sm, L, c, gi, go, Y_.get(), nsw, 0,
                                                                              parts_.get(), fl)));
           }
-          k_lp_sweep_decide<<<B, 1024, 0, s>>>(parts_.get(), nbs, sweep_.first.get(), nsw, P.qp_tol, P.qp_max_iter,
-                                               sc, fl);
+          k_lp_sweep_sums<<<dim3(B, nsw), 1024, 0, s>>>(parts_.get(), nbs, sweep_.first.get(), swtot_.get(), fl);
+          k_lp_sweep_decide<<<1, 256, 0, s>>>(swtot_.get(), B, nsw, P.qp_tol, P.qp_max_iter, sc, fl);
           {
             KTimer tm("lp_sweeps_replay", s);
             if (L == 21)

@@ -106,7 +106,7 @@ class Batch {
   DBuf<float> Q_, gy_, psx_, tmp_, rpix_;
   OrderedBins obins_;  // bins of the field the ordered filters of all pairwise terms share
   DBuf<int> lab_;
-  DBuf<double> parts_, sc_;
+  DBuf<double> parts_, sc_, swtot_;  // swtot_: per-image totals of a sweep batch
   DBuf<int> fl_;
   // LP state
   DBuf<float> alpha_, sa_, gamma_, gamma2_, Y_, lQ_, curQ_, best_, mmparts_, mm_;

@@ -228,62 +228,65 @@ __global__ void __launch_bounds__(kBlock, 6) k_lp_sweeps(SEG sg, int Lr, LPConst
   for (int i = t; i < np * L; i += kBlock) gout[base + i] = tile[i];
 }
 
+// per-sweep totals of the batch's block partials: block (image, sweep) sums
+// the partials of the image's blocks [first[i], first[i+1]) of the sweep
+// launch (all nblocks when first is null) into tot[image * kSweepBatch +
+// sweep]: each thread's blocks in order, the wave sums, the waves in order.
+// Images whose loop already ended in this inner step are skipped.
+__global__ void __launch_bounds__(1024) k_lp_sweep_sums(const double* __restrict__ parts, int nblocks,
+                                                        const int* __restrict__ first, double* __restrict__ tot,
+                                                        const int* __restrict__ fl_all) {
+  __shared__ double red[1024 / 64];
+  const int img = blockIdx.x, sw = blockIdx.y;
+  if (fl_all[(size_t)img * kLF_NFL + kLF_SWEEP_DONE]) return;
+  const int b0 = first ? first[img] : 0, b1 = first ? first[img + 1] : nblocks;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double acc = 0.0;
+  for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) acc += parts[(size_t)sw * nblocks + b];
+  const double w = wave_sum(acc);
+  if (lane == 0) red[wid] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = 0;
+    for (int i = 0; i < (int)blockDim.x / 64; ++i) v += red[i];
+    tot[(size_t)img * kSweepBatch + sw] = v;
+  }
+}
+
 // stopping sweep of the batch (:670-675): the first sweep whose float total
 // differs from the previous one by less than qp_tol; fl[kLF_SWEEPS] = sweeps
 // to replay, fl[kLF_SWEEP_DONE] = 1 if the loop ends in this batch.
-// sc[kL_QPV] carries the previous total across batches.
-// One block per image: the image's blocks are [first[i], first[i+1]) of the
-// sweep launch (all nblocks when first is null); an image whose loop already
-// ended in this inner step is left as it is (its replay is a no-op).
-__global__ void k_lp_sweep_decide(const double* __restrict__ parts, int nblocks, const int* __restrict__ first, int nsw,
-                                  float qp_tol, int max_total, double* __restrict__ sc_all, int* __restrict__ fl_all) {
-  __shared__ double tot[kSweepBatch];
-  __shared__ double red[kSweepBatch][1024 / 64];
-  double* sc = sc_all + (size_t)blockIdx.x * kL_NSC;
-  int* fl = fl_all + (size_t)blockIdx.x * kLF_NFL;
-  if (fl[kLF_SWEEP_DONE]) {
-    if (threadIdx.x == 0) fl[kLF_SWEEPS] = nsw;
-    return;
-  }
-  const int b0 = first ? first[blockIdx.x] : 0, b1 = first ? first[blockIdx.x + 1] : nblocks;
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  double acc[kSweepBatch];
-#pragma unroll
-  for (int sw = 0; sw < kSweepBatch; ++sw) acc[sw] = 0.0;
-  for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x)
-#pragma unroll
-    for (int sw = 0; sw < kSweepBatch; ++sw)
-      if (sw < nsw) acc[sw] += parts[(size_t)sw * nblocks + b];
-#pragma unroll
-  for (int sw = 0; sw < kSweepBatch; ++sw) {
-    const double w = wave_sum(acc[sw]);
-    if (lane == 0) red[sw][wid] = w;
-  }
-  __syncthreads();
-  if (threadIdx.x < (unsigned)nsw) {
-    double v = 0;
-    for (int w = 0; w < (int)blockDim.x / 64; ++w) v += red[threadIdx.x][w];
-    tot[threadIdx.x] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  const int done_before = fl[kLF_TOTAL];
-  float prev = done_before == 0 ? 3.402823466e38f : (float)sc[kL_QPV];
-  int stop = nsw, finished = 0;
-  for (int sw = 0; sw < nsw; ++sw) {
-    const float cur = (float)tot[sw];
-    const int total = done_before + sw + 1;
-    if (fabsf(prev - cur) < qp_tol || total >= max_total) {
-      stop = sw + 1;
-      finished = 1;
-      break;
+// sc[kL_QPV] carries the previous total across batches.  One thread per
+// image; an image whose loop already ended in this inner step is left as it
+// is (its replay is a no-op).
+__global__ void k_lp_sweep_decide(const double* __restrict__ tot, int nimg, int nsw, float qp_tol, int max_total,
+                                  double* __restrict__ sc_all, int* __restrict__ fl_all) {
+  for (int img = threadIdx.x; img < nimg; img += blockDim.x) {
+    double* sc = sc_all + (size_t)img * kL_NSC;
+    int* fl = fl_all + (size_t)img * kLF_NFL;
+    if (fl[kLF_SWEEP_DONE]) {
+      fl[kLF_SWEEPS] = nsw;
+      continue;
     }
-    prev = cur;
+    const double* tt = tot + (size_t)img * kSweepBatch;
+    const int done_before = fl[kLF_TOTAL];
+    float prev = done_before == 0 ? 3.402823466e38f : (float)sc[kL_QPV];
+    int stop = nsw, finished = 0;
+    for (int sw = 0; sw < nsw; ++sw) {
+      const float cur = (float)tt[sw];
+      const int total = done_before + sw + 1;
+      if (fabsf(prev - cur) < qp_tol || total >= max_total) {
+        stop = sw + 1;
+        finished = 1;
+        break;
+      }
+      prev = cur;
+    }
+    sc[kL_QPV] = (double)prev;
+    fl[kLF_SWEEPS] = stop;
+    fl[kLF_SWEEP_DONE] = finished;
+    fl[kLF_TOTAL] = done_before + stop;
   }
-  sc[kL_QPV] = (double)prev;
-  fl[kLF_SWEEPS] = stop;
-  fl[kLF_SWEEP_DONE] = finished;
-  fl[kLF_TOTAL] = done_before + stop;
 }
 
 // start of an inner step's sweep loop, one thread per image
@@ -710,7 +713,7 @@ __global__ void __launch_bounds__(kBlock) k_onehot(const int* __restrict__ lab, 
 struct CRF::LPState {
   DBuf<float> alpha, u, sa, su, gamma, gamma2, Y, Q, curQ, best, tmp;
   DBuf<int> lab, iscratch;
-  DBuf<double> parts, sc;
+  DBuf<double> parts, sc, swtot;
   DBuf<float> mmparts, mm;
   DBuf<int> fl;
   DBuf<double> st_gap, st_delta;
@@ -805,6 +808,7 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
   st.iscratch.resize(4);
   const int nbp = (int)grid_for(N);
   st.parts.resize(std::max<size_t>((size_t)kSweepBatch * nbp, (size_t)2 * kRedBlocks));
+  st.swtot.resize(kSweepBatch);
   st.sc.resize(kL_NSC);
   st.mmparts.resize(2 * (size_t)std::max<long>(nbp, 16384));  // up to the largest betaQ grid tried
   st.mm.resize(2);
@@ -902,7 +906,8 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
                                                                            st.gamma2.get(), st.Y.get(), nsw, 0,
                                                                            st.parts.get(), fl)));
         }
-        k_lp_sweep_decide<<<1, 1024, 0, s>>>(st.parts.get(), nbp, nullptr, nsw, P.qp_tol, P.qp_max_iter, sc, fl);
+        k_lp_sweep_sums<<<dim3(1, nsw), 1024, 0, s>>>(st.parts.get(), nbp, nullptr, st.swtot.get(), fl);
+        k_lp_sweep_decide<<<1, 64, 0, s>>>(st.swtot.get(), 1, nsw, P.qp_tol, P.qp_max_iter, sc, fl);
         {
           KTimer tm("lp_sweeps_replay", s);
           if (L == 21)
