@@ -1230,10 +1230,16 @@ __global__ void k_zero_ints(int* p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
 
-__global__ void __launch_bounds__(kBlock) k_zero(float* p, size_t n) {
+// n floats from each of gridDim.y pointers set to zero (one launch for the
+// zero rows of several buffers)
+struct ZeroRows {
+  float* p[4];
+};
+__global__ void __launch_bounds__(kBlock) k_zero_rows(ZeroRows z, size_t n) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) p[t] = 0.f;
+  if (t < n) z.p[blockIdx.y][t] = 0.f;
 }
+
 
 float lattice_alpha(int d) { return 1.0f / (1 + powf(2.0f, (float)-d)); }
 
@@ -1389,7 +1395,7 @@ void Lattice::build(const float* d_feat, int N, int d) {
   DCRF_HIP(hipStreamSynchronize(s));
 }
 
-void Lattice::ensure_values(size_t floats_per_row) {
+void Lattice::ensure_values(size_t floats_per_row, bool zero_row0) {
   const size_t need = (size_t)(M_ + 1) * floats_per_row;
   if (need > vcap_) {
     vA_.resize(need);
@@ -1399,9 +1405,11 @@ void Lattice::ensure_values(size_t floats_per_row) {
   // row 0 stands for absent neighbours and must read as zero; rows 1..M are
   // fully rewritten by every splat.  A kernel (not a memset) keeps this
   // capturable in a HIP graph.
-  k_zero<<<grid_for(floats_per_row), kBlock, 0, stream_>>>(vA_.get(), floats_per_row);
-  k_zero<<<grid_for(floats_per_row), kBlock, 0, stream_>>>(vB_.get(), floats_per_row);
-  check_launch("k_zero");
+  if (zero_row0) {
+    k_zero_rows<<<dim3(grid_for(floats_per_row), 2), kBlock, 0, stream_>>>(
+        ZeroRows{{vA_.get(), vB_.get(), nullptr, nullptr}}, floats_per_row);
+    check_launch("k_zero_rows");
+  }
   if (nmulti_ > 0) part_.resize((size_t)npart_ * floats_per_row);
 }
 
@@ -1597,7 +1605,7 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   ensure_total_weight();
   ensure_ordered_tables();  // before k_obin, which lists edge values into elist_
   const int row = orow_stride(L);
-  ensure_values(2 * (size_t)row);  // LO and UP halves of each buffer (float-sized; bf16 rows use half)
+  ensure_values(2 * (size_t)row, false);  // LO and UP halves of each buffer (float-sized; bf16 rows use half)
   const size_t half = (size_t)(M_ + 1) * row;  // elements of T
   hipStream_t s = stream_;
   const RowMap rm = rowmap(L);
@@ -1607,10 +1615,10 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   T* dstU = dstL + half;
   // row 0 of each half is the zero row (row * sizeof(T) bytes, a whole number of floats)
   const size_t zf = (size_t)row * sizeof(T) / sizeof(float);
-  k_zero<<<grid_for(zf), kBlock, 0, s>>>(reinterpret_cast<float*>(srcL), zf);
-  k_zero<<<grid_for(zf), kBlock, 0, s>>>(reinterpret_cast<float*>(dstL), zf);
-  k_zero<<<grid_for(zf), kBlock, 0, s>>>(reinterpret_cast<float*>(srcU), zf);
-  k_zero<<<grid_for(zf), kBlock, 0, s>>>(reinterpret_cast<float*>(dstU), zf);
+  k_zero_rows<<<dim3(grid_for(zf), 4), kBlock, 0, s>>>(
+      ZeroRows{{reinterpret_cast<float*>(srcL), reinterpret_cast<float*>(dstL), reinterpret_cast<float*>(srcU),
+                reinterpret_cast<float*>(dstU)}},
+      zf);
   const std::string tag = prof_on() ? "[d=" + std::to_string(d_) + ",L=" + std::to_string(L) + "]" : std::string();
   const int LP = obin_stride(L);
   if (shared) {

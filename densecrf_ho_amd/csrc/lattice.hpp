@@ -114,7 +114,7 @@ class Lattice {
   size_t bytes_values(int V) const { return (size_t)(M_ + 1) * V * sizeof(float); }
 
  private:
-  void ensure_values(size_t floats);
+  void ensure_values(size_t floats, bool zero_row0 = true);
   template <class T>
   void ordered_t(const float* Q, const float* d_minmax, int L, const SliceArgs& out, OrderedBins* shared,
                  bool fresh);
