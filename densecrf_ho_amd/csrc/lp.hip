@@ -114,8 +114,9 @@ __global__ void __launch_bounds__(kBlock) k_lp_Y(SEG sg, int L, LPConst c, float
     r.row += qs.p0;
     float* row = seg + (threadIdx.x & ~63) + r.g * L;
     const size_t idx = (size_t)r.row * L + r.k;
-    float t = 0.f;
+    float t = 0.f, cq = 0.f;
     if (r.valid) {
+      cq = curQ[idx];  // loaded with the other inputs, used after the row sum
       float a = alpha[idx];
       float uu = u ? u[idx] : 0.f;
       if (apply_update) {
@@ -132,7 +133,7 @@ __global__ void __launch_bounds__(kBlock) k_lp_Y(SEG sg, int L, LPConst c, float
     __syncthreads();
     if (r.valid) {
       const float S = eigen_sum(L, 0, [&](int i) { return row[i]; });
-      Y[idx] = ((c.c_m1_over_M * S + t) * c.lambda) + curQ[idx];
+      Y[idx] = ((c.c_m1_over_M * S + t) * c.lambda) + cq;
     }
     __syncthreads();
   }
@@ -319,8 +320,9 @@ __global__ void __launch_bounds__(kBlock) k_lp_betaQ(SEG sg, int L, float lambda
     r.row += qs.p0;
     float* row = seg + (threadIdx.x & ~63) + r.g * L;
     const size_t idx = (size_t)r.row * L + r.k;
-    float a = 0.f, uu = 0.f, gm = 0.f, uv = 0.f;
+    float a = 0.f, uu = 0.f, gm = 0.f, uv = 0.f, cq = 0.f;
     if (r.valid) {
+      cq = curQ[idx];  // loaded with the other inputs, used after the row sum
       a = alpha[idx];
       uu = u ? u[idx] : 0.f;
       gm = gamma[idx];
@@ -332,7 +334,7 @@ __global__ void __launch_bounds__(kBlock) k_lp_betaQ(SEG sg, int L, float lambda
       // alignment of the column within its own image's matrix
       float b = -eigen_sum(L, col_astart(r.row - qs.p0, L), [&](int i) { return row[i]; });
       b = b / (float)L;
-      const float q = lambda * ((((a + b) + uu) + gm) - uv) + curQ[idx];
+      const float q = lambda * ((((a + b) + uu) + gm) - uv) + cq;
       Q[idx] = q;
       mn = nmin(mn, q);
       mx = nmax(mx, q);
