@@ -144,10 +144,13 @@ __global__ void __launch_bounds__(kBlock) k_lp_Y(SEG sg, int L, LPConst c, float
 // The row of Y is held as its positive and negative parts nh = max(y, 0),
 // ph = max(-y, 0) (the terms of the update); y = nh - ph where the qp value
 // needs it (exact: one of the two is zero).
+// sum: in, the Eigen-order sum of g; out, that of the updated g (the next
+// sweep's sum: the same operations on the same values)
 template <int MAXL>
 __device__ inline __attribute__((always_inline)) float qp_sweep(float (&g)[MAXL], const float (&ph)[MAXL],
-                                                                const float (&nh)[MAXL], int L, const LPConst& c) {
-  const float sum = eigen_sum_reg<MAXL>(g, L);
+                                                                const float (&nh)[MAXL], int L, const LPConst& c,
+                                                                float& sum_io) {
+  const float sum = sum_io;
 #pragma unroll
   for (int l = 0; l < MAXL; ++l)
     if (l < L) {
@@ -158,6 +161,7 @@ __device__ inline __attribute__((always_inline)) float qp_sweep(float (&g)[MAXL]
       g[l] = g[l] * (stp / t2);
     }
   const float s2 = eigen_sum_reg<MAXL>(g, L);
+  sum_io = s2;
   const float cs2 = c.c_m1_over_M * s2;
   const float d1 = eigen_sum_fn<MAXL>(L, [&](int l) { return g[l] * ((cs2 + g[l]) * c.lambda); });
   const float d2 = eigen_sum_fn<MAXL>(L, [&](int l) { return g[l] * (nh[l] - ph[l]); });
@@ -208,8 +212,9 @@ __global__ void __launch_bounds__(kBlock, 6) k_lp_sweeps(SEG sg, int Lr, LPConst
     ph[l] = fmaxf(-y, 0.f);
     nh[l] = fmaxf(y, 0.f);
   }
+  float gsum = eigen_sum_reg<MAXL>(g, L);
   for (int sw = 0; sw < n; ++sw) {
-    const float v = act ? qp_sweep<MAXL>(g, ph, nh, L, c) : 0.f;
+    const float v = act ? qp_sweep<MAXL>(g, ph, nh, L, c, gsum) : 0.f;
     if (!replay) {
       const double ws = wave_sum((double)v);
       if ((t & 63) == 0) red[sw][t >> 6] = ws;
