@@ -34,6 +34,8 @@ struct CRF::HOState {
   DBuf<float> hb;  // smax x L
   // LP scratch: per region min/max positions
   DBuf<int> mpos;  // 4 per region: min_pix, min_lab, max_pix, max_lab
+  DBuf<float> umm;  // 2 per (region, label): min and max over the members
+  DBuf<int> useg;   // 2 per (region, label): last member taking the min / max position (-1: none)
   DBuf<double> parts;
   bool dirty = true;
 };
@@ -175,28 +177,57 @@ __global__ void __launch_bounds__(kBlock) k_ho_zstep(float* __restrict__ z, cons
 }
 
 // ---- LP side -------------------------------------------------------------------
-// computeUCondGrad: scan (label-major, members in order) with int `min`/`max`.
-// The running int min after element k is trunc(prefix-min of Q); element k
-// takes the min position iff Q_k < trunc(prefix-min before k) (the first
-// element always does).  Likewise for the max.  One wave per region.
-__global__ void __launch_bounds__(kBlock) k_ucond_scan(const float* __restrict__ Q, int L, const int* __restrict__ mstart,
-                             const int* __restrict__ mpix, int r0, int r1, int* __restrict__ mpos) {
-  const int r = r0 + (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+// computeUCondGrad: a scan of each region (label-major, members in order) with
+// int `min`/`max`.  The running int min after element k is trunc(prefix-min
+// of Q); element k takes the min position iff Q_k < trunc(prefix-min before
+// k) (the first element always does).  Likewise for the max.  The scan is
+// split by label: (1) min and max of every (region, label); (2) one wave per
+// (region, label) scans its members with the earlier labels' min / max as
+// carry and keeps the last member that takes each position; (3) per region
+// the last label with a taker gives the positions.  NaN never takes a
+// position and never enters a prefix (fminf / fmaxf skip it), as in the
+// reference's comparisons.
+__global__ void __launch_bounds__(kBlock) k_ucond_minmax(const float* __restrict__ Q, int L, const int* __restrict__ mstart,
+                                                         const int* __restrict__ mpix, int R, float* __restrict__ umm) {
+  const long w = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  if (r >= r1) return;
+  if (w >= (long)R * L) return;
+  const int r = (int)(w / L), l = (int)(w - (long)r * L);
   const int b = mstart[r], s = mstart[r + 1] - b;
-  const long n = (long)s * L;
-  float carry_min = INFINITY, carry_max = -INFINITY;  // prefix over earlier chunks
-  long best_min = -1, best_max = -1;
-  for (long base = 0; base < n; base += 64) {
-    const long k = base + lane;
-    float q = 0.f;
-    const bool valid = k < n;
-    if (valid) {
-      const int lab = (int)(k / s), m = (int)(k - (long)lab * s);
-      q = Q[(size_t)mpix[b + m] * L + lab];
-    }
-    // exclusive prefix min/max within the chunk
+  float mn = INFINITY, mx = -INFINITY;
+  for (int m = lane; m < s; m += 64) {
+    const float q = Q[(size_t)mpix[b + m] * L + l];
+    mn = fminf(mn, q);
+    mx = fmaxf(mx, q);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mn = fminf(mn, __shfl_xor(mn, o, 64));
+    mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+  }
+  if (lane == 0) {
+    umm[2 * w] = mn;
+    umm[2 * w + 1] = mx;
+  }
+}
+__global__ void __launch_bounds__(kBlock) k_ucond_seg(const float* __restrict__ Q, int L, const int* __restrict__ mstart,
+                                                      const int* __restrict__ mpix, int R,
+                                                      const float* __restrict__ umm, int* __restrict__ useg) {
+  const long w = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (w >= (long)R * L) return;
+  const int r = (int)(w / L), l = (int)(w - (long)r * L);
+  const int b = mstart[r], s = mstart[r + 1] - b;
+  float carry_min = INFINITY, carry_max = -INFINITY;  // over the region's earlier labels
+  for (int j = 0; j < l; ++j) {
+    carry_min = fminf(carry_min, umm[2 * ((long)r * L + j)]);
+    carry_max = fmaxf(carry_max, umm[2 * ((long)r * L + j) + 1]);
+  }
+  int best_min = -1, best_max = -1;
+  for (int base = 0; base < s; base += 64) {
+    const int m = base + lane;
+    const bool valid = m < s;
+    const float q = valid ? Q[(size_t)mpix[b + m] * L + l] : 0.f;
+    // exclusive prefix min / max within the chunk, then with the carry
     float pmin = valid ? q : INFINITY, pmax = valid ? q : -INFINITY;
     for (int o = 1; o < 64; o <<= 1) {
       const float a = __shfl_up(pmin, o, 64), c = __shfl_up(pmax, o, 64);
@@ -213,27 +244,50 @@ __global__ void __launch_bounds__(kBlock) k_ucond_scan(const float* __restrict__
     emin = fminf(emin, carry_min);
     emax = fmaxf(emax, carry_max);
     if (valid) {
-      // int state before k: INT_MAX / INT_MIN initially, else trunc(prefix)
+      // int state before the element: INT_MAX / INT_MIN initially, else trunc(prefix)
       const bool take_min = (emin == INFINITY) ? (2147483648.0f > q) : ((float)(int)emin > q);
       const bool take_max = (emax == -INFINITY) ? (-2147483648.0f < q) : ((float)(int)emax < q);
-      if (take_min) best_min = k;
-      if (take_max) best_max = k;
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-      best_min = max(best_min, (long)__shfl_xor(best_min, o, 64));
-      best_max = max(best_max, (long)__shfl_xor(best_max, o, 64));
+      if (take_min) best_min = m;
+      if (take_max) best_max = m;
     }
     carry_min = fminf(carry_min, __shfl(pmin, 63, 64));
     carry_max = fmaxf(carry_max, __shfl(pmax, 63, 64));
   }
-  if (lane == 0) {
-    const long kmin = best_min < 0 ? 0 : best_min, kmax = best_max < 0 ? 0 : best_max;
-    const int lmin = (int)(kmin / s), lmax = (int)(kmax / s);
-    mpos[4 * r + 0] = mpix[b + (int)(kmin - (long)lmin * s)];
-    mpos[4 * r + 1] = lmin;
-    mpos[4 * r + 2] = mpix[b + (int)(kmax - (long)lmax * s)];
-    mpos[4 * r + 3] = lmax;
+  for (int o = 32; o > 0; o >>= 1) {
+    best_min = max(best_min, __shfl_xor(best_min, o, 64));
+    best_max = max(best_max, __shfl_xor(best_max, o, 64));
   }
+  if (lane == 0) {
+    useg[2 * w] = best_min;
+    useg[2 * w + 1] = best_max;
+  }
+}
+// per region: the last label with a taker (label-major order) and its member;
+// no taker at all leaves the first element (lab 0, first member)
+__global__ void __launch_bounds__(kBlock) k_ucond_pick(const int* __restrict__ useg, int L, const int* __restrict__ mstart,
+                                                       const int* __restrict__ mpix, int R, int* __restrict__ mpos) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  const int b = mstart[r];
+  int lmin = 0, mmin = 0, lmax = 0, mmax = 0;
+  bool fmin = false, fmax = false;
+  for (int l = L - 1; l >= 0 && !(fmin && fmax); --l) {
+    const int a = useg[2 * ((long)r * L + l)], c = useg[2 * ((long)r * L + l) + 1];
+    if (!fmin && a >= 0) {
+      lmin = l;
+      mmin = a;
+      fmin = true;
+    }
+    if (!fmax && c >= 0) {
+      lmax = l;
+      mmax = c;
+      fmax = true;
+    }
+  }
+  mpos[4 * r + 0] = mpix[b + mmin];
+  mpos[4 * r + 1] = lmin;
+  mpos[4 * r + 2] = mpix[b + mmax];
+  mpos[4 * r + 3] = lmax;
 }
 
 // Us(min) = c*e, then Us(max) = -c*e (the max write wins), regions of one layer
@@ -359,6 +413,8 @@ CRF::HOState* CRF::ho_state() {
     h.prefB.resize(GL);
     h.hb.resize((size_t)h.smax * L_);
     h.mpos.resize(4 * (size_t)R);
+    h.umm.resize(2 * (size_t)R * L_);
+    h.useg.resize(2 * (size_t)R * L_);
     h.parts.resize(std::max(R, kRedBlocks));
     DCRF_HIP(hipStreamSynchronize(s_));
     h.dirty = false;
@@ -460,8 +516,11 @@ void CRF::ucondgrad_dev(const float* Q, float* Us, hipStream_t s) {
   HOState& h = *ho_state();
   k_fill<<<grid_for((size_t)N_ * L_), kBlock, 0, s>>>(Us, (size_t)N_ * L_, 0.f);
   const int nl = (int)h.h_layer_start.size();
-  k_ucond_scan<<<grid_for((size_t)h.R * 64), kBlock, 0, s>>>(Q, L_, h.mstart.get(), h.mpix.get(), 0, h.R,
-                                                             h.mpos.get());
+  const size_t lanes = (size_t)h.R * L_ * 64;  // one wave per (region, label)
+  k_ucond_minmax<<<grid_for(lanes), kBlock, 0, s>>>(Q, L_, h.mstart.get(), h.mpix.get(), h.R, h.umm.get());
+  k_ucond_seg<<<grid_for(lanes), kBlock, 0, s>>>(Q, L_, h.mstart.get(), h.mpix.get(), h.R, h.umm.get(),
+                                                 h.useg.get());
+  k_ucond_pick<<<grid_for(h.R), kBlock, 0, s>>>(h.useg.get(), L_, h.mstart.get(), h.mpix.get(), h.R, h.mpos.get());
   for (int li = 0; li < nl; ++li) {  // later layers overwrite earlier ones, as in region order
     const int r0 = h.h_layer_start[li], r1 = li + 1 < nl ? h.h_layer_start[li + 1] : h.R;
     if (r1 > r0)

@@ -141,3 +141,30 @@ def test_qp_lp_with_superpixels_vs_fp64_oracle():
     print("lp label agreement", (lg == lo).mean(), "energies", eg, eo, flush=True)
     assert (lg == lo).mean() >= 0.999
     assert abs(eg - eo) <= 1e-4 * abs(eo)
+
+
+def test_ucondgrad_large_regions_vs_oracle():
+    """Regions of hundreds to thousands of members (many 64-member chunks per
+    label, carries across chunks and labels): positions equal the oracle's
+    for values spanning several integers and for values mostly in [0, 1]
+    with a few negatives."""
+    import densecrf_ho_amd as D
+    from densecrf_ho_amd import synthetic as S
+    W, H, L = 160, 120, 21
+    wl = S.Workload(W, H, 8, 3, L=L)
+    c = D.DenseCRF2D(W, H, L)
+    c.setUnaryEnergy(wl.U)
+    o = O.CRF(wl.N, L)
+    o.set_unary(wl.U)
+    for f, w in wl.kernels:
+        c.addPairwiseEnergy(f, D.PottsCompatibility(w))
+        o.add_pairwise(f, w)
+    for region, _ in S.superpixel_layers(W, H, cells=(4, 9, 30), seeds=(41, 42, 43)):
+        c.addSuperPixelLayer(region, 0.5, 0.2)
+        o.add_layer(region, 0.5, 0.2)
+    rng = np.random.default_rng(11)
+    Qa = (rng.standard_normal((wl.N, L)) * 2.5).astype(np.float32)
+    Qb = rng.random((wl.N, L)).astype(np.float32)
+    Qb[rng.random((wl.N, L)) < 0.001] *= -1.0
+    for Q in (Qa, Qb):
+        np.testing.assert_array_equal(c.computeUCondGrad(Q), o.ucondgrad(Q))
