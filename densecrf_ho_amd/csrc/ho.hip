@@ -61,18 +61,30 @@ __global__ void __launch_bounds__(kBlock) k_ho_segsum(const float* __restrict__ 
   B[(size_t)g * L + l] = s;
 }
 
-// inclusive prefix over groups (PREFIX) or suffix (!PREFIX), one thread per label
+// inclusive prefix over groups (PREFIX) or suffix (!PREFIX) in double, one
+// block per label: each thread adds a run of groups, an exclusive scan of the
+// runs' totals across the block, then each thread writes its run
 template <bool PREFIX>
-__global__ void k_ho_scan(const double* __restrict__ B, double* __restrict__ out, int G, int L,
-                          const int* __restrict__ fl) {
+__global__ void __launch_bounds__(kBlock) k_ho_scan(const double* __restrict__ B, double* __restrict__ out, int G,
+                                                    int L, const int* __restrict__ fl) {
   if (fl && fl[kF_DONE]) return;
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= L) return;
-  double s = 0;
-  if (PREFIX)
-    for (int g = 0; g < G; ++g) out[(size_t)g * L + l] = (s += B[(size_t)g * L + l]);
-  else
-    for (int g = G - 1; g >= 0; --g) out[(size_t)g * L + l] = (s += B[(size_t)g * L + l]);
+  __shared__ double wtot[kBlock / 64];
+  const int l = blockIdx.x, t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int per = (G + kBlock - 1) / kBlock, i0 = min(G, t * per), i1 = min(G, i0 + per);
+  auto at = [&](int i) { return (size_t)(PREFIX ? i : G - 1 - i) * L + l; };
+  double run = 0;
+  for (int i = i0; i < i1; ++i) run += B[at(i)];
+  // exclusive scan of the runs: within the wave, then over the waves
+  double inc = run;
+  for (int o = 1; o < 64; o <<= 1) {
+    const double a = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += a;
+  }
+  if (lane == 63) wtot[wid] = inc;
+  __syncthreads();
+  double base = inc - run;
+  for (int w = 0; w < wid; ++w) base += wtot[w];
+  for (int i = i0; i < i1; ++i) out[at(i)] = (base += B[at(i)]);
 }
 
 // per (group, label): sum over the group's regions of p(l, r) where
@@ -120,13 +132,14 @@ __global__ void __launch_bounds__(kBlock) k_ho_add_hb(float* __restrict__ gy, co
 // init (:373-375):      grad_z = (1 + NtoR(Q-1)) .* c
 // iteration (:432):     grad_z = c + NtoR(Q-1) .* c
 // energy (:387, :435):  sum(z .* c) + sum((1 - z) .* c .* NtoR(Q-1))
-__global__ void k_ho_gradz_energy(const float* __restrict__ z, const float* __restrict__ e,
+// Grid-stride over (region, label); one partial per block (see k_ho_finalize).
+__global__ void __launch_bounds__(kBlock) k_ho_gradz_energy(const float* __restrict__ z, const float* __restrict__ e,
                                   const int* __restrict__ reg_grp, const double* __restrict__ prefQ, int R, int L,
                                   int init, float* __restrict__ gz, double* __restrict__ nrQ,
-                                  double* __restrict__ sho, const int* __restrict__ fl) {
+                                  double* __restrict__ parts, const int* __restrict__ fl) {
   if (fl && fl[kF_DONE]) return;
   double v[1] = {0.0};
-  for (int t = threadIdx.x; t < R * L; t += blockDim.x) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < R * L; t += gridDim.x * blockDim.x) {
     const int r = t / L, l = t - r * L;
     const double nr = prefQ[(size_t)reg_grp[r] * L + l];
     nrQ[t] = nr;
@@ -135,17 +148,18 @@ __global__ void k_ho_gradz_energy(const float* __restrict__ z, const float* __re
     v[0] += (double)(z[t] * c) + (double)((1.f - z[t]) * c) * nr;
   }
   block_sum<1>(v);
-  if (threadIdx.x == 0) sho[kH_E] = v[0];
+  if (threadIdx.x == 0) parts[blockIdx.x] = v[0];
 }
 
 // direction terms (:396-414): cond_grad_z, sx_z and the four scalars
-__global__ void k_ho_direction(const float* __restrict__ z, const float* __restrict__ gz, const float* __restrict__ e,
+// Grid-stride over (region, label); four partials per block (see k_ho_finalize).
+__global__ void __launch_bounds__(kBlock) k_ho_direction(const float* __restrict__ z, const float* __restrict__ gz, const float* __restrict__ e,
                                const int* __restrict__ reg_grp, const double* __restrict__ prefS,
                                const double* __restrict__ nrQ, int R, int L, float* __restrict__ cgz,
-                               float* __restrict__ sxz, double* __restrict__ sho, const int* __restrict__ fl) {
+                               float* __restrict__ sxz, double* __restrict__ parts, const int* __restrict__ fl) {
   if (fl[kF_DONE]) return;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int t = threadIdx.x; t < R * L; t += blockDim.x) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < R * L; t += gridDim.x * blockDim.x) {
     const int r = t / L, l = t - r * L;
     const float cg = gz[t] >= 0 ? 0.f : 1.f;  // descent_direction_z (src/qp.cpp:17-25)
     const float sz = cg - z[t];
@@ -160,13 +174,29 @@ __global__ void k_ho_direction(const float* __restrict__ z, const float* __restr
     v[3] += (double)szc * nrS;                 // denominator term
   }
   block_sum<4>(v);
-  if (threadIdx.x == 0) {
-    sho[kH_A] = v[0];
-    sho[kH_B] = v[1];
-    sho[kH_SXZ] = v[2];
-    sho[kH_DEN] = v[3];
-  }
+  if (threadIdx.x == 0)
+    for (int i = 0; i < 4; ++i) parts[4 * blockIdx.x + i] = v[i];
 }
+
+// the block partials of k_ho_direction (NV = 4: a, b, sum sx_z, denominator
+// into sho[kH_A..kH_DEN]) or k_ho_gradz_energy (NV = 1: energy into
+// sho[kH_E]) summed in block order; nothing once the QP has stopped
+template <int NV>
+__global__ void __launch_bounds__(kBlock) k_ho_finalize(const double* __restrict__ parts, int nparts,
+                                                        double* __restrict__ out, const int* __restrict__ fl) {
+  if (fl && fl[kF_DONE]) return;
+  double v[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = 0;
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] += parts[(size_t)b * NV + i];
+  block_sum<NV>(v);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) out[i] = v[i];
+}
+static_assert(kH_B == kH_A + 1 && kH_SXZ == kH_A + 2 && kH_DEN == kH_A + 3, "direction scalars are consecutive");
 
 // z += step * sx_z (:425)
 __global__ void __launch_bounds__(kBlock) k_ho_zstep(float* __restrict__ z, const float* __restrict__ sxz, int n, const double* __restrict__ sc,
@@ -362,6 +392,12 @@ __global__ void __launch_bounds__(kBlock) k_first_diff(const int* __restrict__ l
   if (i < N && (short)lab[i] != (short)lab[0]) atomicMin(out, i);
 }
 
+// blocks of the (region, label) reductions: fixed by the sizes, so the sums
+// are the same run to run
+inline int ho_red_blocks(int R, int L) {
+  return (int)std::max<long>(1, std::min<long>(((long)R * L + kBlock - 1) / kBlock, kRedBlocks));
+}
+
 }  // namespace
 
 float* ho_hb(CRF::HOState* h) { return h->hb.get(); }
@@ -415,7 +451,7 @@ CRF::HOState* CRF::ho_state() {
     h.mpos.resize(4 * (size_t)R);
     h.umm.resize(2 * (size_t)R * L_);
     h.useg.resize(2 * (size_t)R * L_);
-    h.parts.resize(std::max(R, kRedBlocks));
+    h.parts.resize(std::max(R, 4 * kRedBlocks));
     DCRF_HIP(hipStreamSynchronize(s_));
     h.dirty = false;
   }
@@ -454,12 +490,14 @@ void CRF::ho_qp_init(hipStream_t s, const float* Q, float* gy, double* sho) {
   const int G = h.G, L = L_, R = h.R;
   k_fill<<<grid_for((size_t)R * L), kBlock, 0, s>>>(h.z.get(), (size_t)R * L, 1.f);
   k_ho_segsum<0><<<grid_for((size_t)G * L), kBlock, 0, s>>>(Q, nullptr, L, h.usz.get(), G, h.segB.get(), nullptr);
-  k_ho_scan<true><<<1, 64, 0, s>>>(h.segB.get(), h.prefB.get(), G, L, nullptr);
-  k_ho_gradz_energy<<<1, kBlock, 0, s>>>(h.z.get(), h.e.get(), h.reg_grp.get(), h.prefB.get(), R, L, 1, h.gz.get(),
-                                         h.nrQ.get(), sho, nullptr);
+  k_ho_scan<true><<<L, kBlock, 0, s>>>(h.segB.get(), h.prefB.get(), G, L, nullptr);
+  const int nrb = ho_red_blocks(R, L);
+  k_ho_gradz_energy<<<nrb, kBlock, 0, s>>>(h.z.get(), h.e.get(), h.reg_grp.get(), h.prefB.get(), R, L, 1, h.gz.get(),
+                                           h.nrQ.get(), h.parts.get(), nullptr);
+  k_ho_finalize<1><<<1, kBlock, 0, s>>>(h.parts.get(), nrb, sho + kH_E, nullptr);
   k_ho_groupsum_zm1<<<grid_for((size_t)G * L), kBlock, 0, s>>>(h.z.get(), h.e.get(), h.grp_start.get(),
                                                                h.grp_reg.get(), G, L, h.segB.get(), nullptr);
-  k_ho_scan<false><<<1, 64, 0, s>>>(h.segB.get(), h.prefB.get(), G, L, nullptr);
+  k_ho_scan<false><<<L, kBlock, 0, s>>>(h.segB.get(), h.prefB.get(), G, L, nullptr);
   k_ho_bcast<<<grid_for((size_t)h.smax * L), kBlock, 0, s>>>(h.prefB.get(), h.usz.get(), G, h.smax, L, h.hb.get(),
                                                              nullptr);
   k_ho_add_hb<<<grid_for((size_t)h.smax * L), kBlock, 0, s>>>(gy, h.hb.get(), (size_t)h.smax * L);
@@ -471,9 +509,11 @@ void CRF::ho_qp_direction(hipStream_t s, const float* Q, const int* lab, double*
   HOState& h = *ho_;
   const int G = h.G, L = L_;
   k_ho_segsum<1><<<grid_for((size_t)G * L), kBlock, 0, s>>>(Q, lab, L, h.usz.get(), G, h.segB.get(), fl);
-  k_ho_scan<true><<<1, 64, 0, s>>>(h.segB.get(), h.prefB.get(), G, L, fl);
-  k_ho_direction<<<1, kBlock, 0, s>>>(h.z.get(), h.gz.get(), h.e.get(), h.reg_grp.get(), h.prefB.get(), h.nrQ.get(),
-                                      h.R, L, h.cgz.get(), h.sxz.get(), sho, fl);
+  k_ho_scan<true><<<L, kBlock, 0, s>>>(h.segB.get(), h.prefB.get(), G, L, fl);
+  const int nrb = ho_red_blocks(h.R, L);
+  k_ho_direction<<<nrb, kBlock, 0, s>>>(h.z.get(), h.gz.get(), h.e.get(), h.reg_grp.get(), h.prefB.get(),
+                                        h.nrQ.get(), h.R, L, h.cgz.get(), h.sxz.get(), h.parts.get(), fl);
+  k_ho_finalize<4><<<1, kBlock, 0, s>>>(h.parts.get(), nrb, sho + kH_A, fl);
   check_launch("ho_qp_direction");
 }
 
@@ -484,7 +524,7 @@ void CRF::ho_qp_zstep(hipStream_t s, double* sc, int* fl) {
   k_ho_zstep<<<grid_for((size_t)h.R * L), kBlock, 0, s>>>(h.z.get(), h.sxz.get(), h.R * L, sc, fl);
   k_ho_groupsum_zm1<<<grid_for((size_t)G * L), kBlock, 0, s>>>(h.z.get(), h.e.get(), h.grp_start.get(),
                                                                h.grp_reg.get(), G, L, h.segB.get(), fl);
-  k_ho_scan<false><<<1, 64, 0, s>>>(h.segB.get(), h.prefB.get(), G, L, fl);
+  k_ho_scan<false><<<L, kBlock, 0, s>>>(h.segB.get(), h.prefB.get(), G, L, fl);
   k_ho_bcast<<<grid_for((size_t)h.smax * L), kBlock, 0, s>>>(h.prefB.get(), h.usz.get(), G, h.smax, L, h.hb.get(),
                                                              fl);
   check_launch("ho_qp_zstep");
@@ -495,9 +535,11 @@ void CRF::ho_qp_after(hipStream_t s, const float* Q, double* sho, int* fl) {
   HOState& h = *ho_;
   const int G = h.G, L = L_;
   k_ho_segsum<0><<<grid_for((size_t)G * L), kBlock, 0, s>>>(Q, nullptr, L, h.usz.get(), G, h.segB.get(), fl);
-  k_ho_scan<true><<<1, 64, 0, s>>>(h.segB.get(), h.prefB.get(), G, L, fl);
-  k_ho_gradz_energy<<<1, kBlock, 0, s>>>(h.z.get(), h.e.get(), h.reg_grp.get(), h.prefB.get(), h.R, L, 0,
-                                         h.gz.get(), h.nrQ.get(), sho, fl);
+  k_ho_scan<true><<<L, kBlock, 0, s>>>(h.segB.get(), h.prefB.get(), G, L, fl);
+  const int nrb = ho_red_blocks(h.R, L);
+  k_ho_gradz_energy<<<nrb, kBlock, 0, s>>>(h.z.get(), h.e.get(), h.reg_grp.get(), h.prefB.get(), h.R, L, 0,
+                                           h.gz.get(), h.nrQ.get(), h.parts.get(), fl);
+  k_ho_finalize<1><<<1, kBlock, 0, s>>>(h.parts.get(), nrb, sho + kH_E, fl);
   check_launch("ho_qp_after");
 }
 
