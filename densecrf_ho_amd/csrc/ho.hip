@@ -34,8 +34,6 @@ struct CRF::HOState {
   DBuf<float> hb;  // smax x L
   // LP scratch: per region min/max positions
   DBuf<int> mpos;  // 4 per region: min_pix, min_lab, max_pix, max_lab
-  DBuf<float> umm;  // 2 per (region, label): min and max over the members
-  DBuf<int> useg;   // 2 per (region, label): last member taking the min / max position (-1: none)
   DBuf<double> parts;
   bool dirty = true;
 };
@@ -210,114 +208,121 @@ __global__ void __launch_bounds__(kBlock) k_ho_zstep(float* __restrict__ z, cons
 // computeUCondGrad: a scan of each region (label-major, members in order) with
 // int `min`/`max`.  The running int min after element k is trunc(prefix-min
 // of Q); element k takes the min position iff Q_k < trunc(prefix-min before
-// k) (the first element always does).  Likewise for the max.  The scan is
-// split by label: (1) min and max of every (region, label); (2) one wave per
-// (region, label) scans its members with the earlier labels' min / max as
-// carry and keeps the last member that takes each position; (3) per region
-// the last label with a taker gives the positions.  NaN never takes a
-// position and never enters a prefix (fminf / fmaxf skip it), as in the
+// k) (the first element always does).  Likewise for the max.  NaN never takes
+// a position and never enters a prefix (fminf / fmaxf skip it), as in the
 // reference's comparisons.
-__global__ void __launch_bounds__(kBlock) k_ucond_minmax(const float* __restrict__ Q, int L, const int* __restrict__ mstart,
-                                                         const int* __restrict__ mpix, int R, float* __restrict__ umm) {
-  const long w = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (w >= (long)R * L) return;
-  const int r = (int)(w / L), l = (int)(w - (long)r * L);
+//
+// One block per region.  Thread t < nslot*L (nslot = blockDim / L) owns label
+// l = t % L and the j-th (j = t / L) of nslot contiguous ranges of the
+// members, so at each step the block reads nslot whole member rows.
+//   pass 1: min and max of every (range, label), then of every label;
+//   carry of (range, label): min / max over the earlier labels and over the
+//           earlier ranges of its own label (exclusive prefix in scan order);
+//   pass 2: each thread scans its range in order from its carry and keeps
+//           the last member it takes;
+//   the last label with a taker (and its last taker) gives the positions.
+constexpr int kUcondBlock = 1024;
+constexpr int kUcondBatch = 8;  // members whose loads are issued together
+__global__ void __launch_bounds__(kUcondBlock) k_ucond_region(const float* __restrict__ Q, int L, const int* __restrict__ mstart,
+                                                         const int* __restrict__ mpix, int* __restrict__ mpos) {
+  __shared__ float rmn[kUcondBlock], rmx[kUcondBlock];  // per (range, label)
+  __shared__ float lmn[kMaxLabels], lmx[kMaxLabels];    // per label
+  __shared__ int bmn[kUcondBlock], bmx[kUcondBlock];    // per thread last takers
+  const int r = blockIdx.x, t = threadIdx.x;
+  const int nslot = blockDim.x / L, act = nslot * L;
+  const int l = t % L, j = t / L;
   const int b = mstart[r], s = mstart[r + 1] - b;
+  const int per = (s + nslot - 1) / nslot;
+  const int m0 = t < act ? min(s, j * per) : s, m1 = min(s, m0 + per);
+  // a batch of members' values, loads issued together (members past m1: NaN, skipped by fminf / fmaxf)
+  auto batch = [&](int m, float (&q)[kUcondBatch]) {
+    int px[kUcondBatch];
+#pragma unroll
+    for (int i = 0; i < kUcondBatch; ++i) px[i] = m + i < m1 ? mpix[b + m + i] : -1;
+#pragma unroll
+    for (int i = 0; i < kUcondBatch; ++i) q[i] = px[i] >= 0 ? Q[(size_t)px[i] * L + l] : NAN;
+  };
+  // pass 1
   float mn = INFINITY, mx = -INFINITY;
-  for (int m = lane; m < s; m += 64) {
-    const float q = Q[(size_t)mpix[b + m] * L + l];
-    mn = fminf(mn, q);
-    mx = fmaxf(mx, q);
+  for (int m = m0; m < m1; m += kUcondBatch) {
+    float q[kUcondBatch];
+    batch(m, q);
+#pragma unroll
+    for (int i = 0; i < kUcondBatch; ++i) {
+      mn = fminf(mn, q[i]);
+      mx = fmaxf(mx, q[i]);
+    }
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    mn = fminf(mn, __shfl_xor(mn, o, 64));
-    mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+  rmn[t] = mn;
+  rmx[t] = mx;
+  __syncthreads();
+  if (t < L) {
+    float a = INFINITY, c = -INFINITY;
+    for (int k = 0; k < nslot; ++k) {
+      a = fminf(a, rmn[k * L + t]);
+      c = fmaxf(c, rmx[k * L + t]);
+    }
+    lmn[t] = a;
+    lmx[t] = c;
   }
-  if (lane == 0) {
-    umm[2 * w] = mn;
-    umm[2 * w + 1] = mx;
+  __syncthreads();
+  float emin = INFINITY, emax = -INFINITY;  // prefix before the range's first member
+  if (t < act) {
+    for (int k = 0; k < l; ++k) {
+      emin = fminf(emin, lmn[k]);
+      emax = fmaxf(emax, lmx[k]);
+    }
+    for (int k = 0; k < j; ++k) {
+      emin = fminf(emin, rmn[k * L + l]);
+      emax = fmaxf(emax, rmx[k * L + l]);
+    }
   }
-}
-__global__ void __launch_bounds__(kBlock) k_ucond_seg(const float* __restrict__ Q, int L, const int* __restrict__ mstart,
-                                                      const int* __restrict__ mpix, int R,
-                                                      const float* __restrict__ umm, int* __restrict__ useg) {
-  const long w = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (w >= (long)R * L) return;
-  const int r = (int)(w / L), l = (int)(w - (long)r * L);
-  const int b = mstart[r], s = mstart[r + 1] - b;
-  float carry_min = INFINITY, carry_max = -INFINITY;  // over the region's earlier labels
-  for (int j = 0; j < l; ++j) {
-    carry_min = fminf(carry_min, umm[2 * ((long)r * L + j)]);
-    carry_max = fmaxf(carry_max, umm[2 * ((long)r * L + j) + 1]);
-  }
+  // pass 2
   int best_min = -1, best_max = -1;
-  for (int base = 0; base < s; base += 64) {
-    const int m = base + lane;
-    const bool valid = m < s;
-    const float q = valid ? Q[(size_t)mpix[b + m] * L + l] : 0.f;
-    // exclusive prefix min / max within the chunk, then with the carry
-    float pmin = valid ? q : INFINITY, pmax = valid ? q : -INFINITY;
-    for (int o = 1; o < 64; o <<= 1) {
-      const float a = __shfl_up(pmin, o, 64), c = __shfl_up(pmax, o, 64);
-      if (lane >= o) {
-        pmin = fminf(pmin, a);
-        pmax = fmaxf(pmax, c);
+  for (int m = m0; m < m1; m += kUcondBatch) {
+    float q[kUcondBatch];
+    batch(m, q);
+#pragma unroll
+    for (int i = 0; i < kUcondBatch; ++i)
+      if (m + i < m1) {
+        // int state before the element: INT_MAX / INT_MIN initially, else trunc(prefix)
+        const bool take_min = (emin == INFINITY) ? (2147483648.0f > q[i]) : ((float)(int)emin > q[i]);
+        const bool take_max = (emax == -INFINITY) ? (-2147483648.0f < q[i]) : ((float)(int)emax < q[i]);
+        if (take_min) best_min = m + i;
+        if (take_max) best_max = m + i;
+        emin = fminf(emin, q[i]);
+        emax = fmaxf(emax, q[i]);
+      }
+  }
+  bmn[t] = best_min;
+  bmx[t] = best_max;
+  __syncthreads();
+  if (t == 0) {
+    int lmin = 0, mmin = 0, lmax = 0, mmax = 0;
+    bool fmin = false, fmax = false;
+    for (int ll = L - 1; ll >= 0 && !(fmin && fmax); --ll) {
+      int a = -1, c = -1;
+      for (int k = 0; k < nslot; ++k) {
+        a = max(a, bmn[k * L + ll]);
+        c = max(c, bmx[k * L + ll]);
+      }
+      if (!fmin && a >= 0) {
+        lmin = ll;
+        mmin = a;
+        fmin = true;
+      }
+      if (!fmax && c >= 0) {
+        lmax = ll;
+        mmax = c;
+        fmax = true;
       }
     }
-    float emin = __shfl_up(pmin, 1, 64), emax = __shfl_up(pmax, 1, 64);
-    if (lane == 0) {
-      emin = INFINITY;
-      emax = -INFINITY;
-    }
-    emin = fminf(emin, carry_min);
-    emax = fmaxf(emax, carry_max);
-    if (valid) {
-      // int state before the element: INT_MAX / INT_MIN initially, else trunc(prefix)
-      const bool take_min = (emin == INFINITY) ? (2147483648.0f > q) : ((float)(int)emin > q);
-      const bool take_max = (emax == -INFINITY) ? (-2147483648.0f < q) : ((float)(int)emax < q);
-      if (take_min) best_min = m;
-      if (take_max) best_max = m;
-    }
-    carry_min = fminf(carry_min, __shfl(pmin, 63, 64));
-    carry_max = fmaxf(carry_max, __shfl(pmax, 63, 64));
+    // no taker at all leaves the first element (lab 0, first member)
+    mpos[4 * r + 0] = mpix[b + mmin];
+    mpos[4 * r + 1] = lmin;
+    mpos[4 * r + 2] = mpix[b + mmax];
+    mpos[4 * r + 3] = lmax;
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    best_min = max(best_min, __shfl_xor(best_min, o, 64));
-    best_max = max(best_max, __shfl_xor(best_max, o, 64));
-  }
-  if (lane == 0) {
-    useg[2 * w] = best_min;
-    useg[2 * w + 1] = best_max;
-  }
-}
-// per region: the last label with a taker (label-major order) and its member;
-// no taker at all leaves the first element (lab 0, first member)
-__global__ void __launch_bounds__(kBlock) k_ucond_pick(const int* __restrict__ useg, int L, const int* __restrict__ mstart,
-                                                       const int* __restrict__ mpix, int R, int* __restrict__ mpos) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= R) return;
-  const int b = mstart[r];
-  int lmin = 0, mmin = 0, lmax = 0, mmax = 0;
-  bool fmin = false, fmax = false;
-  for (int l = L - 1; l >= 0 && !(fmin && fmax); --l) {
-    const int a = useg[2 * ((long)r * L + l)], c = useg[2 * ((long)r * L + l) + 1];
-    if (!fmin && a >= 0) {
-      lmin = l;
-      mmin = a;
-      fmin = true;
-    }
-    if (!fmax && c >= 0) {
-      lmax = l;
-      mmax = c;
-      fmax = true;
-    }
-  }
-  mpos[4 * r + 0] = mpix[b + mmin];
-  mpos[4 * r + 1] = lmin;
-  mpos[4 * r + 2] = mpix[b + mmax];
-  mpos[4 * r + 3] = lmax;
 }
 
 // Us(min) = c*e, then Us(max) = -c*e (the max write wins), regions of one layer
@@ -449,8 +454,6 @@ CRF::HOState* CRF::ho_state() {
     h.prefB.resize(GL);
     h.hb.resize((size_t)h.smax * L_);
     h.mpos.resize(4 * (size_t)R);
-    h.umm.resize(2 * (size_t)R * L_);
-    h.useg.resize(2 * (size_t)R * L_);
     h.parts.resize(std::max(R, 4 * kRedBlocks));
     DCRF_HIP(hipStreamSynchronize(s_));
     h.dirty = false;
@@ -558,11 +561,7 @@ void CRF::ucondgrad_dev(const float* Q, float* Us, hipStream_t s) {
   HOState& h = *ho_state();
   k_fill<<<grid_for((size_t)N_ * L_), kBlock, 0, s>>>(Us, (size_t)N_ * L_, 0.f);
   const int nl = (int)h.h_layer_start.size();
-  const size_t lanes = (size_t)h.R * L_ * 64;  // one wave per (region, label)
-  k_ucond_minmax<<<grid_for(lanes), kBlock, 0, s>>>(Q, L_, h.mstart.get(), h.mpix.get(), h.R, h.umm.get());
-  k_ucond_seg<<<grid_for(lanes), kBlock, 0, s>>>(Q, L_, h.mstart.get(), h.mpix.get(), h.R, h.umm.get(),
-                                                 h.useg.get());
-  k_ucond_pick<<<grid_for(h.R), kBlock, 0, s>>>(h.useg.get(), L_, h.mstart.get(), h.mpix.get(), h.R, h.mpos.get());
+  k_ucond_region<<<h.R, kUcondBlock, 0, s>>>(Q, L_, h.mstart.get(), h.mpix.get(), h.mpos.get());
   for (int li = 0; li < nl; ++li) {  // later layers overwrite earlier ones, as in region order
     const int r0 = h.h_layer_start[li], r1 = li + 1 < nl ? h.h_layer_start[li + 1] : h.R;
     if (r1 > r0)
