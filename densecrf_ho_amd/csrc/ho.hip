@@ -40,23 +40,53 @@ struct CRF::HOState {
 
 namespace {
 
-// one thread per (size group, label): sum of P(pix, l) over pix in
-// [u_{g-1}, u_g); MODE 0: P = Q - 1, MODE 1: P = onehot(lab) - Q
+// prefix over pixels, read at the size-group ends: out[g][l] = sum of
+// P(pix, l) over pix < usz[g] (the bug-compatible region sums of
+// multiplySuperPixels run over pixel indices 0..s_r-1); MODE 0: P = Q - 1,
+// MODE 1: P = onehot(lab) - Q; each term in float, the sums in double.  One
+// block per label: each thread adds a contiguous run of pixels, an exclusive
+// scan of the runs across the block, then each thread walks its run again
+// and writes the group ends that fall in it.
+constexpr int kPixPrefixBlock = 1024;
 template <int MODE>
-__global__ void __launch_bounds__(kBlock) k_ho_segsum(const float* __restrict__ Q, const int* __restrict__ lab, int L,
-                            const int* __restrict__ usz, int G, double* __restrict__ B, const int* __restrict__ fl) {
+__global__ void __launch_bounds__(kPixPrefixBlock) k_ho_pixprefix(const float* __restrict__ Q, const int* __restrict__ lab,
+                                                                  int L, const int* __restrict__ usz, int G,
+                                                                  double* __restrict__ out, const int* __restrict__ fl) {
   if (fl && fl[kF_DONE]) return;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= G * L) return;
-  const int g = t / L, l = t - g * L;
-  const int p0 = g ? usz[g - 1] : 0, p1 = usz[g];
-  double s = 0;
-  for (int p = p0; p < p1; ++p) {
+  __shared__ double wtot[kPixPrefixBlock / 64];
+  const int l = blockIdx.x, t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int smax = usz[G - 1];
+  const int per = (smax + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int p0 = min(smax, t * per), p1 = min(smax, p0 + per);
+  auto term = [&](int p) {
     const float q = Q[(size_t)p * L + l];
     const float v = MODE == 0 ? q - 1.f : (lab[p] == l ? 1.f : 0.f) - q;
-    s += (double)v;
+    return (double)v;
+  };
+  double run = 0;
+  for (int p = p0; p < p1; ++p) run += term(p);
+  double inc = run;
+  for (int o = 1; o < 64; o <<= 1) {
+    const double a = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += a;
   }
-  B[(size_t)g * L + l] = s;
+  if (lane == 63) wtot[wid] = inc;
+  __syncthreads();
+  double acc = inc - run;
+  for (int w = 0; w < wid; ++w) acc += wtot[w];
+  int lo = 0, hi = G;  // first group ending after p0
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (usz[mid] > p0)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  int g = lo;
+  for (int p = p0; p < p1 && g < G; ++p) {
+    acc += term(p);
+    if (usz[g] == p + 1) out[(size_t)g++ * L + l] = acc;
+  }
 }
 
 // inclusive prefix over groups (PREFIX) or suffix (!PREFIX) in double, one
@@ -391,10 +421,19 @@ __global__ void __launch_bounds__(kBlock) k_ho_assign(const int* __restrict__ la
   if (threadIdx.x == 0) parts[blockIdx.x] = v[0];
 }
 
-// first pixel index whose label differs from lab[0] (N if none)
+// first pixel index whose label differs from lab[0] (N if none): the block's
+// first such index by a wave and block minimum, then one atomic per block
 __global__ void __launch_bounds__(kBlock) k_first_diff(const int* __restrict__ lab, int N, int* __restrict__ out) {
+  __shared__ int wmin[kBlock / 64];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < N && (short)lab[i] != (short)lab[0]) atomicMin(out, i);
+  int v = (i < N && (short)lab[i] != (short)lab[0]) ? i : INT_MAX;
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / 64; ++w) v = min(v, wmin[w]);
+    if (v != INT_MAX) atomicMin(out, v);
+  }
 }
 
 // blocks of the (region, label) reductions: fixed by the sizes, so the sums
@@ -492,8 +531,7 @@ void CRF::ho_qp_init(hipStream_t s, const float* Q, float* gy, double* sho) {
   HOState& h = *ho_;
   const int G = h.G, L = L_, R = h.R;
   k_fill<<<grid_for((size_t)R * L), kBlock, 0, s>>>(h.z.get(), (size_t)R * L, 1.f);
-  k_ho_segsum<0><<<grid_for((size_t)G * L), kBlock, 0, s>>>(Q, nullptr, L, h.usz.get(), G, h.segB.get(), nullptr);
-  k_ho_scan<true><<<L, kBlock, 0, s>>>(h.segB.get(), h.prefB.get(), G, L, nullptr);
+  k_ho_pixprefix<0><<<L, kPixPrefixBlock, 0, s>>>(Q, nullptr, L, h.usz.get(), G, h.prefB.get(), nullptr);
   const int nrb = ho_red_blocks(R, L);
   k_ho_gradz_energy<<<nrb, kBlock, 0, s>>>(h.z.get(), h.e.get(), h.reg_grp.get(), h.prefB.get(), R, L, 1, h.gz.get(),
                                            h.nrQ.get(), h.parts.get(), nullptr);
@@ -511,8 +549,7 @@ void CRF::ho_qp_init(hipStream_t s, const float* Q, float* gy, double* sho) {
 void CRF::ho_qp_direction(hipStream_t s, const float* Q, const int* lab, double* sho, int* fl) {
   HOState& h = *ho_;
   const int G = h.G, L = L_;
-  k_ho_segsum<1><<<grid_for((size_t)G * L), kBlock, 0, s>>>(Q, lab, L, h.usz.get(), G, h.segB.get(), fl);
-  k_ho_scan<true><<<L, kBlock, 0, s>>>(h.segB.get(), h.prefB.get(), G, L, fl);
+  k_ho_pixprefix<1><<<L, kPixPrefixBlock, 0, s>>>(Q, lab, L, h.usz.get(), G, h.prefB.get(), fl);
   const int nrb = ho_red_blocks(h.R, L);
   k_ho_direction<<<nrb, kBlock, 0, s>>>(h.z.get(), h.gz.get(), h.e.get(), h.reg_grp.get(), h.prefB.get(),
                                         h.nrQ.get(), h.R, L, h.cgz.get(), h.sxz.get(), h.parts.get(), fl);
@@ -537,8 +574,7 @@ void CRF::ho_qp_zstep(hipStream_t s, double* sc, int* fl) {
 void CRF::ho_qp_after(hipStream_t s, const float* Q, double* sho, int* fl) {
   HOState& h = *ho_;
   const int G = h.G, L = L_;
-  k_ho_segsum<0><<<grid_for((size_t)G * L), kBlock, 0, s>>>(Q, nullptr, L, h.usz.get(), G, h.segB.get(), fl);
-  k_ho_scan<true><<<L, kBlock, 0, s>>>(h.segB.get(), h.prefB.get(), G, L, fl);
+  k_ho_pixprefix<0><<<L, kPixPrefixBlock, 0, s>>>(Q, nullptr, L, h.usz.get(), G, h.prefB.get(), fl);
   const int nrb = ho_red_blocks(h.R, L);
   k_ho_gradz_energy<<<nrb, kBlock, 0, s>>>(h.z.get(), h.e.get(), h.reg_grp.get(), h.prefB.get(), h.R, L, 0,
                                            h.gz.get(), h.nrQ.get(), h.parts.get(), fl);
