@@ -26,7 +26,7 @@ namespace {
 
 // outer-loop state of the batched LP, per image
 constexpr int kO_BEST = 0, kO_PREV = 1, kO_NSC = 2;                       // doubles
-constexpr int kOF_COUNT = 0, kOF_STOP = 1, kOF_COPY = 2, kOF_NFL = 4;     // ints
+constexpr int kOF_COUNT = 0, kOF_STOP = 1, kOF_COPY = 2, kOF_LC = 3, kOF_NFL = 4;  // ints
 // the LP's per-image scalar and flag slots are those of lp.hip
 constexpr int kB_LNSC = kL_NSC, kB_EINT = kL_EINT, kB_TMP = kL_TMP;
 
@@ -79,6 +79,7 @@ __global__ void kb_lp_init_decide(const double* __restrict__ sc, double* __restr
     ost[(size_t)i * kO_NSC + kO_PREV] = eint;
     ofl[(size_t)i * kOF_NFL + kOF_COUNT] = 0;
     ofl[(size_t)i * kOF_NFL + kOF_STOP] = 0;
+    ofl[(size_t)i * kOF_NFL + kOF_LC] = 0;
     ofl[(size_t)i * kOF_NFL + kOF_COPY] = elp > eint ? 1 : 0;
   }
 }
@@ -131,6 +132,17 @@ __global__ void kb_lp_outer_decide(const double* __restrict__ sc, double* __rest
       f[kOF_COPY] = 1;
       o[kO_BEST] = e;
     }
+  }
+}
+
+// the less-confident stop (:784-794) per image, after the best copy: stop an
+// image when more than pct percent of its pixels are less confident, then
+// clear its count for the next outer step
+__global__ void kb_lp_less_confident_decide(const long* __restrict__ pix, int* __restrict__ ofl, float pct, int nimg) {
+  for (int i = threadIdx.x; i < nimg; i += blockDim.x) {
+    int* f = ofl + (size_t)i * kOF_NFL;
+    if (!f[kOF_STOP] && double(f[kOF_LC]) / double(pix[i + 1] - pix[i]) * 100 > (double)pct) f[kOF_STOP] = 1;
+    f[kOF_LC] = 0;
   }
 }
 
@@ -583,6 +595,10 @@ std::vector<std::vector<int>> Batch::lp(const float* init, float* out, const LPP
     int_energy_dev(lab, sc + kL_EINT, kL_NSC);
     kb_lp_outer_decide<<<1, 256, 0, s>>>(sc, ost_.get(), ofl, P.prox_energy_tol, B);
     kb_copy_if<<<nbr, kBlock, 0, s>>>(rm, L, ofl, lQ_.get(), best_.get());
+    if (P.less_confident_percent > 0) {
+      k_less_confident<<<nbr, kBlock, 0, s>>>(rm, L, best_.get(), P.confidence_tol, ofl + kOF_LC, kOF_NFL);
+      kb_lp_less_confident_decide<<<1, 256, 0, s>>>(rm.pix, ofl, P.less_confident_percent, B);
+    }
     DCRF_HIP(hipMemcpyAsync(hofl.data(), ofl, hofl.size() * sizeof(int), hipMemcpyDeviceToHost, s));
     DCRF_HIP(hipStreamSynchronize(s));
     bool all = true;

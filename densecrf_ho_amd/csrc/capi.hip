@@ -500,7 +500,6 @@ int dcrf_crf_lp(dcrf_crf_t crf, const float* init, float* out, const dcrf_lp_par
     dcrf_lp_params d;
     dcrf_lp_params_default(&d);
     const dcrf_lp_params& P = params ? *params : d;
-    DCRF_REQUIRE(P.less_confident_percent <= 0.f, "dcrf_crf_lp: less_confident_percent > 0 is not supported");
     LPParams lp;
     lp.prox_reg_const = P.prox_reg_const;
     lp.dual_gap_tol = P.dual_gap_tol;
@@ -511,6 +510,8 @@ int dcrf_crf_lp(dcrf_crf_t crf, const float* init, float* out, const dcrf_lp_par
     lp.qp_tol = P.qp_tol;
     lp.qp_const = P.qp_const;
     lp.best_int = P.best_int != 0;
+    lp.less_confident_percent = P.less_confident_percent;
+    lp.confidence_tol = P.confidence_tol;
     const size_t n = (size_t)c.N() * c.L();
     Staged<float> a(init, n, on_device != 0, c.stream(), true);
     Staged<float> b(out, n, on_device != 0, c.stream(), false);
@@ -783,6 +784,8 @@ int dcrf_batch_lp(dcrf_batch_t b, const float* init, float* out, const dcrf_lp_p
     P.qp_tol = p.qp_tol;
     P.qp_const = p.qp_const;
     P.best_int = p.best_int != 0;
+    P.less_confident_percent = p.less_confident_percent;
+    P.confidence_tol = p.confidence_tol;
     const size_t n = (size_t)B.N() * B.L();
     Staged<float> a(init, n, on_device != 0, B.stream(), true);
     Staged<float> o(out, n, on_device != 0, B.stream(), false);

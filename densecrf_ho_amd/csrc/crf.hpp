@@ -79,6 +79,9 @@ struct LPParams {
   int prox_max_iter = 10, fw_max_iter = 10, qp_max_iter = 1000;
   float qp_tol = 1e3f, qp_const = 1e-16f;
   bool best_int = true;
+  // stop once more than this percentage of pixels has no label above
+  // confidence_tol in the best iterate (0: no check)
+  float less_confident_percent = 0.f, confidence_tol = 0.95f;
 };
 
 struct LPResult {

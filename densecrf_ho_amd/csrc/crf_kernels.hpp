@@ -160,6 +160,28 @@ __global__ void finalize_seg(const double* __restrict__ parts, const int* __rest
     for (int i = 0; i < NV; ++i) out[(size_t)blockIdx.x * stride + i] = v[i];
 }
 
+// less_confident_pixels (src/densecrf.cpp:457-462) as a count: per image, the
+// number of pixels whose largest label value is <= tol, added to count[img]
+// (integer sums, so the result does not depend on the grid)
+template <class SEG>
+__global__ void __launch_bounds__(kBlock) k_less_confident(SEG sg, int L, const float* __restrict__ Q, float tol,
+                                                           int* __restrict__ count, int stride) {
+  __shared__ int sc;
+  const SegPos q = sg.pos();
+  if (threadIdx.x == 0) sc = 0;
+  __syncthreads();
+  int c = 0;
+  for (long i = q.lb * blockDim.x + threadIdx.x; i < q.n; i += q.nb * blockDim.x) {
+    const float* col = Q + (size_t)(q.p0 + i) * L;
+    float mx = col[0];
+    for (int l = 1; l < L; ++l) mx = col[l] > mx ? col[l] : mx;
+    c += mx <= tol ? 1 : 0;
+  }
+  if (c) atomicAdd(&sc, c);
+  __syncthreads();
+  if (threadIdx.x == 0 && sc) atomicAdd(count + (size_t)q.img * stride, sc);
+}
+
 // expAndNormalize(-U) (src/densecrf.cpp:328-336), one thread per pixel
 __global__ void __launch_bounds__(kBlock) k_softmax_neg(const float* __restrict__ U, float* __restrict__ Q, int N, int L) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

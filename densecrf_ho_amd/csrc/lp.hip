@@ -719,7 +719,7 @@ __global__ void __launch_bounds__(kBlock) k_onehot(const int* __restrict__ lab, 
 
 struct CRF::LPState {
   DBuf<float> alpha, u, sa, su, gamma, gamma2, Y, Q, curQ, best, tmp;
-  DBuf<int> lab, iscratch;
+  DBuf<int> lab, iscratch, lc;
   DBuf<double> parts, sc, swtot;
   DBuf<float> mmparts, mm;
   DBuf<int> fl;
@@ -813,6 +813,7 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
   }
   st.lab.resize(N);
   st.iscratch.resize(4);
+  st.lc.resize(1);
   const int nbp = (int)grid_for(N);
   st.parts.resize(std::max<size_t>((size_t)kSweepBatch * nbp, (size_t)2 * kRedBlocks));
   st.swtot.resize(kSweepBatch);
@@ -1002,6 +1003,15 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
         DCRF_HIP(hipMemcpyAsync(st.best.get(), st.Q.get(), NL * sizeof(float), hipMemcpyDeviceToDevice, s));
         best_energy = energy;
       }
+    }
+    if (P.less_confident_percent > 0) {  // :784-794
+      k_fill_int<<<1, 64, 0, s>>>(st.lc.get(), 1, 0);
+      k_less_confident<<<grid_for(N, kBlock, 4096), kBlock, 0, s>>>(OneImage{N}, L, st.best.get(), P.confidence_tol,
+                                                          st.lc.get(), 1);
+      int n_lc = 0;
+      DCRF_HIP(hipMemcpyAsync(&n_lc, st.lc.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+      DCRF_HIP(hipStreamSynchronize(s));
+      if (double(n_lc) / double(N) * 100 > (double)P.less_confident_percent) break;
     }
   } while (it < P.prox_max_iter);
   res.outer = it;

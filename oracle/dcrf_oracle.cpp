@@ -963,6 +963,7 @@ struct CRF {
     int prox_max_iter = 10, fw_max_iter = 10, qp_max_iter = 1000;
     float qp_tol = 1e3f, qp_const = 1e-16f;
     bool best_int = true;
+    float less_confident_percent = 0.f, confidence_tol = 0.95f;
   };
   struct LPStats {
     std::vector<int> qp_sweeps;
@@ -1152,6 +1153,16 @@ struct CRF {
           best_Q = Q;
           best_energy = energy;
         }
+      }
+      if (P.less_confident_percent > 0) {  // :784-794 with less_confident_pixels (:457-462)
+        int n_lc = 0;
+        for (int i = 0; i < N; ++i) {
+          const T* col = &best_Q[(size_t)i * L];
+          T mx = col[0];
+          for (int l = 1; l < L; ++l) mx = std::max(mx, col[l]);
+          if ((float)mx <= P.confidence_tol) ++n_lc;
+        }
+        if (double(n_lc) / double(N) * 100 > (double)P.less_confident_percent) break;
       }
     } while (it < P.prox_max_iter);
     if (st) st->outer = it;
@@ -1410,8 +1421,8 @@ void orc_crf_zlabels(void* h, float* z) {
   std::copy(c->f->z_labels.begin(), c->f->z_labels.end(), z);
 }
 
-// LP.  params: lambda, dual_gap_tol, prox_energy_tol, qp_tol, qp_const as
-// floats; ints prox_max_iter, fw_max_iter, qp_max_iter, best_int.  stats
+// LP.  params: lambda, dual_gap_tol, prox_energy_tol, qp_tol, qp_const,
+// less_confident_percent, confidence_tol as floats; ints prox_max_iter, fw_max_iter, qp_max_iter, best_int.  stats
 // arrays are filled up to `cap` entries; returns outer iteration count or -1.
 int orc_crf_lp(void* h, int dbl, const float* init, float* out, const float* fp, const int* ip, int* qp_sweeps,
                double* gap, double* delta, double* int_energy, int cap, double* init_energies) {
@@ -1427,6 +1438,8 @@ int orc_crf_lp(void* h, int dbl, const float* init, float* out, const float* fp,
     P.fw_max_iter = ip[1];
     P.qp_max_iter = ip[2];
     P.best_int = ip[3] != 0;
+    P.less_confident_percent = fp[5];
+    P.confidence_tol = fp[6];
   };
   auto dump = [&](auto& st) {
     for (size_t i = 0; i < st.qp_sweeps.size() && (int)i < cap; ++i) {

@@ -151,7 +151,8 @@ class LPParams:
     """``LP_inf_params`` (include/densecrf.h:43-69), defaults of src/densecrf.cpp:287-302."""
 
     def __init__(self, prox_reg_const=1e-2, dual_gap_tol=1e3, prox_energy_tol=1e3, prox_max_iter=10,
-                 fw_max_iter=10, qp_max_iter=1000, qp_tol=1e3, qp_const=1e-16, best_int=True):
+                 fw_max_iter=10, qp_max_iter=1000, qp_tol=1e3, qp_const=1e-16, best_int=True,
+                 less_confident_percent=0.0, confidence_tol=0.95):
         self.prox_reg_const = prox_reg_const
         self.dual_gap_tol = dual_gap_tol
         self.prox_energy_tol = prox_energy_tol
@@ -161,10 +162,12 @@ class LPParams:
         self.qp_tol = qp_tol
         self.qp_const = qp_const
         self.best_int = best_int
+        self.less_confident_percent = less_confident_percent
+        self.confidence_tol = confidence_tol
 
     def packed(self):
         fp = np.array([self.prox_reg_const, self.dual_gap_tol, self.prox_energy_tol, self.qp_tol,
-                       self.qp_const], np.float32)
+                       self.qp_const, self.less_confident_percent, self.confidence_tol], np.float32)
         ip = np.array([self.prox_max_iter, self.fw_max_iter, self.qp_max_iter, int(self.best_int)], np.int32)
         return fp, ip
 

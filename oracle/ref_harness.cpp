@@ -202,6 +202,8 @@ int ref_crf_lp(void* h, const float* init, float* out, const float* fp, const in
   P.fw_max_iter = ip[1];
   P.qp_max_iter = ip[2];
   P.best_int = ip[3] != 0;
+  P.less_confident_percent = fp[5];
+  P.confidence_tol = fp[6];
   try {
     MatrixXf q = c->lp_inference_prox_super_pixels(as_mat(init, c->L(), c->N()), P);
     std::copy(q.data(), q.data() + q.size(), out);

@@ -5,7 +5,8 @@ into oracle/_ref/ (development container only).  This script drives it
 through ctypes on seeded synthetic inputs and stores inputs and outputs as
 compressed .npz files.  Run from the repository root:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py          # every fixture
+    python tests/golden/make_golden.py lp_lc    # only lp_lc_31x23_L5.npz
 """
 
 from __future__ import annotations
@@ -200,7 +201,32 @@ def compat_case(name, W, H, K, seed, L):
     print(f"compat_{name}: N={wl.N} L={L}")
 
 
+LC_SETTINGS = ((0.0, 0.95), (10.0, 0.6), (30.0, 0.6), (40.0, 0.6))
+
+
+def lp_less_confident_case():
+    """LP with the less-confident stop (src/densecrf.cpp:784-794): the same
+    31x23, 5-label problem as crf_31x23_L5, four (percent, confidence_tol)
+    settings that stop after 4 (no check), 1, 2 and 3 outer steps."""
+    W, H, L = 31, 23, 5
+    wl = S.Workload(W, H, 8, 1, L=L)
+    r = O.RefCRF(W, H, L)
+    r.set_unary(wl.U)
+    r.add_gaussian(3, 3, 3)
+    r.add_bilateral(50, 50, 15, 15, 15, wl.img, 5)
+    Qqp = r.qp(r.unary_init())
+    d = dict(W=np.int32(W), H=np.int32(H), L=np.int32(L), U=wl.U, feat0=r.features(0, 2), feat1=r.features(1, 5),
+             Qqp=Qqp, settings=np.array(LC_SETTINGS, np.float32), prox_max_iter=np.int32(4))
+    for i, (pct, tol) in enumerate(LC_SETTINGS):
+        d[f"Qlp{i}"] = r.lp(Qqp, O.LPParams(prox_max_iter=4, less_confident_percent=pct, confidence_tol=tol))
+    np.savez_compressed(os.path.join(OUT, "lp_lc_31x23_L5.npz"), **d)
+    print("lp_lc_31x23_L5")
+
+
 def main():
+    if sys.argv[1:] == ["lp_lc"]:
+        lp_less_confident_case()
+        return
     W, H = 32, 24
     wl = S.Workload(W, H, 10, 0)
     lattice_case("g2_32x24", wl.kernels[0][0])
@@ -216,6 +242,7 @@ def main():
     qp_case("convex_31x23_L5", 31, 23, 8, 1, 5, weights=(-3.0, -5.0))
     qp_case("ties_31x23_L5", 31, 23, 8, 1, 5, dup_label=(1, 3))
     compat_case("31x23_L5", 31, 23, 8, 1, 5)
+    lp_less_confident_case()
     c0_case()
 
 

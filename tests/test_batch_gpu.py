@@ -144,6 +144,26 @@ def test_batch_lp_matches_one_image_path(prox, fw):
         assert abs(eb[i] - e1) <= 1e-4 * abs(e1)
 
 
+def test_batch_lp_less_confident_stop_matches_one_image_path():
+    """the less-confident stop is decided per image on the device; each image
+    ends where it ends alone, bit for bit"""
+    import densecrf_ho_amd as D
+    L = 21
+    wls = _workloads(L)
+    b = _batch(wls, L)
+    P = D.LP_inf_params(prox_max_iter=4, fw_max_iter=4, less_confident_percent=30.0, confidence_tol=0.6)
+    Q, _ = b.qp(max_iter=10, ref_stop=False)
+    Ql, _ = b.lp(Q, P)
+    outer = []
+    for i, wl in enumerate(wls):
+        print("image", i, flush=True)
+        c = _single(wl, L)
+        q1, st = c.lp_inference_prox_super_pixels(np.ascontiguousarray(b.image(Q, i)), P, stats=True)
+        outer.append(st["outer"])
+        np.testing.assert_array_equal(b.image(Ql, i), q1)
+    print("outer steps per image", outer, flush=True)
+
+
 def test_batch_lp_with_bin_edges_matches_one_image_path():
     """each image's initial Q puts 0.5 / 0.5 on its own label pairs (0.5 lies
     on a bin edge): 6 (label, bin) pairs per image, 10 over the batch.  Every

@@ -59,6 +59,24 @@ def test_lp_vs_reference(golden, case):
     assert rel_err(Q, g["Qlp"]) < 1e-3
 
 
+def test_lp_less_confident_stop_vs_reference(golden):
+    """the less-confident stop (src/densecrf.cpp:784-794) ends the GPU LP at
+    the oracle's outer step for each setting, with the reference's result"""
+    import densecrf_ho_amd as D
+    g = golden("lp_lc_31x23_L5.npz")
+    c = gpu_crf(g, layers=False)
+    o = orc_crf(g)
+    for i, (pct, tol) in enumerate(g["settings"]):
+        print("setting", i, pct, tol, flush=True)
+        kw = dict(prox_max_iter=int(g["prox_max_iter"]), less_confident_percent=float(pct), confidence_tol=float(tol))
+        Q, st = c.lp_inference_prox_super_pixels(g["Qqp"], D.LP_inf_params(**kw), stats=True)
+        _, sto = o.lp(g["Qqp"], O.LPParams(**kw))
+        assert st["outer"] == sto["outer"]
+        lab, labr = Q.argmax(1), g[f"Qlp{i}"].argmax(1)
+        assert (lab == labr).mean() >= 0.999
+        assert rel_err(Q, g[f"Qlp{i}"]) < 1e-3
+
+
 def test_higher_order_pieces_vs_reference(golden):
     g = golden("crf_32x24_L21_ho.npz")
     c = gpu_crf(g)

@@ -318,6 +318,25 @@ def test_lp_matches_reference(golden, case):
     assert len(st["qp_sweeps"]) >= 1 and st["qp_sweeps"].min() >= 2  # first sweep never stops
 
 
+def test_lp_less_confident_stop_matches_reference(golden):
+    """less_confident_percent / confidence_tol (src/densecrf.cpp:457-462,
+    784-794): four settings stop after 4 (no check), 1, 2 and 3 outer steps;
+    each final Q equals the compiled reference's bit for bit, and the four
+    results differ, so each fixture pins where the stop came."""
+    g = golden("lp_lc_31x23_L5.npz")
+    c = _crf(g)
+    outs = []
+    for i, (pct, tol) in enumerate(g["settings"]):
+        P = O.LPParams(prox_max_iter=int(g["prox_max_iter"]), less_confident_percent=float(pct),
+                       confidence_tol=float(tol))
+        Q, st = c.lp(g["Qqp"], P)
+        np.testing.assert_array_equal(Q, g[f"Qlp{i}"])
+        outs.append(st["outer"])
+    assert outs == [4, 1, 2, 3]
+    for i in range(1, 4):
+        assert not np.array_equal(g[f"Qlp{i}"], g[f"Qlp{i - 1}"])
+
+
 def test_config0_end_to_end(golden):
     """320x213, 40 cells, 21 labels: QP (reference stopping) then a 2-step LP."""
     from densecrf_ho_amd import synthetic as S
