@@ -44,10 +44,12 @@ SIGNATURES = {
     "dcrf_lattice_export": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "dcrf_lattice_compute": (_i, [_vp, _vp, _vp, _i, _i, _i]),
     "dcrf_lattice_compute_upper_minus_lower_ord": (_i, [_vp, _vp, _vp, _i, _i]),
+    "dcrf_lattice_compute_upper_minus_lower_ord_restricted": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _i, _i, _i]),
     "dcrf_pairwise_create": (_i, [_vp, _i, _i, _f, _i, _i, C.POINTER(_vp)]),
     "dcrf_pairwise_destroy": (_i, [_vp]),
     "dcrf_pairwise_apply": (_i, [_vp, _vp, _vp, _i, _i, _i]),
     "dcrf_pairwise_apply_upper_minus_lower_ord": (_i, [_vp, _vp, _vp, _i, _i]),
+    "dcrf_pairwise_apply_upper_minus_lower_ord_restricted": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _i, _i, _i]),
     "dcrf_crf_create": (_i, [_i, _i, C.POINTER(_vp)]),
     "dcrf_crf_destroy": (_i, [_vp]),
     "dcrf_crf_set_unary": (_i, [_vp, _vp, _i]),
@@ -65,6 +67,7 @@ SIGNATURES = {
     "dcrf_crf_add_superpixel_layer": (_i, [_vp, _vp, _i, _vp, _vp]),
     "dcrf_crf_apply": (_i, [_vp, _i, _vp, _vp, _i, _i, _i]),
     "dcrf_crf_apply_upper_minus_lower_ord": (_i, [_vp, _i, _vp, _vp, _i]),
+    "dcrf_crf_apply_upper_minus_lower_ord_restricted": (_i, [_vp, _i, _vp, _vp, _vp, _i, _vp, _i, _i]),
     "dcrf_crf_unary_init": (_i, [_vp, _vp, _i]),
     "dcrf_crf_qp": (_i, [_vp, _vp, _vp, _i, _i, _i, C.POINTER(QPStats)]),
     "dcrf_lp_params_default": (None, [C.POINTER(LPParams)]),

@@ -150,6 +150,20 @@ class _Arg:
             self.dev = 0
 
 
+def _restricted_args(rQ, pI, Q, store, rout, nrows_out):
+    """arguments of the restricted ordered filter, all on one side (host or device)"""
+    a = _Arg(rQ, np.float32)
+    ix = _Arg(pI, np.int32)
+    e = _Arg(Q, np.float32) if (store and Q is not None) else None
+    if rout is None:
+        rout = _empty_like_field(rQ, nrows_out)
+    o = _Arg(rout, np.float32, out=True)
+    devs = {a.dev, ix.dev, o.dev} | ({e.dev} if e is not None else set())
+    if len(devs) != 1:
+        raise TypeError("rQ, pI, Q and rout must all be host arrays or all device arrays")
+    return a, ix, e, o, rout
+
+
 def _empty_like_field(x, shape, dtype=np.float32):
     if isinstance(x, DeviceArray):
         return DeviceArray(shape, dtype)
@@ -253,6 +267,20 @@ class Permutohedral:
         o = _Arg(out, np.float32, out=True)
         _check(_lib.load().dcrf_lattice_compute_upper_minus_lower_ord(self._h, a.ptr, o.ptr, Q.shape[1], a.dev))
         return out
+
+    def compute_upper_minus_lower_ord_restricted(self, rout, rQ, pI, Q=None, store: bool = True):
+        """Permutohedral::compute_upper_minus_lower_ord_restricted(rout, rQ, pI, Q, store):
+        the ordered filter at the pixels pI (strictly ascending) with values rQ
+        (len(pI), V); the other pixels take Q (N, V), read only with store.
+        rout (len(pI), V) is added to (None: start from zeros)."""
+        if rout is None:
+            if _is_torch(rQ) or isinstance(rQ, DeviceArray):
+                raise TypeError("pass rout for device inputs (it is added to)")
+            rout = np.zeros(tuple(rQ.shape), np.float32)
+        a, ix, e, o, rout = _restricted_args(rQ, pI, Q, store, rout, tuple(rQ.shape))
+        _check(_lib.load().dcrf_lattice_compute_upper_minus_lower_ord_restricted(
+            self._h, o.ptr, a.ptr, ix.ptr, len(pI), e.ptr if e else None, rQ.shape[1], int(store), a.dev))
+        return rout
 
     def tables(self):
         """offset_, rank_, barycentric_ (Npad, d+1), keys (M, d), neighbours (d+1, M, 2)."""
@@ -412,6 +440,14 @@ class DenseCRF:
         out = _empty_like_field(Q, (self.N, self.M)) if out is None else out
         o = _Arg(out, np.float32, out=True)
         _check(_lib.load().dcrf_crf_apply_upper_minus_lower_ord(self._h, k, a.ptr, o.ptr, a.dev))
+        return out
+
+    def apply_upper_minus_lower_ord_restricted(self, k, rQ, pI, Q=None, store: bool = True, out=None):
+        """PairwisePotential::apply_upper_minus_lower_ord_restricted of potential k:
+        (len(pI), M) = compatibility(restricted ordered filter)."""
+        a, ix, e, o, out = _restricted_args(rQ, pI, Q, store, out, (len(pI), self.M))
+        _check(_lib.load().dcrf_crf_apply_upper_minus_lower_ord_restricted(
+            self._h, k, o.ptr, a.ptr, ix.ptr, len(pI), e.ptr if e else None, int(store), a.dev))
         return out
 
     def unary_init(self, device=None):

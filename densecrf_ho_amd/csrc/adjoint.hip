@@ -342,6 +342,14 @@ void compat_slice(Potential& p, const SliceArgs& o, int V, hipStream_t s,
   check_launch("k_compat_rg");
 }
 
+void compat_apply_rows(Potential& p, const float* in, float* out, long nrows, int V, hipStream_t s) {
+  DCRF_REQUIRE(p.ctype != kPotts, "compat_apply_rows: Potts is a scale");
+  DCRF_REQUIRE(V == p.cL, "label compatibility: field has a different label count");
+  if (nrows <= 0) return;
+  k_compat_rg<<<rg_grid(nrows, V, 4096), kBlock, 0, s>>>((int)nrows, V, p.ctype, p.cdev.get(), in, out, 0);
+  check_launch("k_compat_rg");
+}
+
 // compatibility gradient of b against an already filtered field m, into acc
 static void compat_grad_accum_dev(int ctype, int N, int L, const float* b, const float* m, double* acc,
                                   double* parts, hipStream_t s) {

@@ -160,6 +160,19 @@ void check_unit_interval(const float* d, size_t n, hipStream_t s) {
   if (!(mx <= 1.f) || !(mn >= 0.f)) throw Error(kOutOfBounds, "Value out of bounds");
 }
 
+// pI of the restricted ordered filter: strictly ascending pixel indices in
+// [0, N) (the reference walks the pixels and pI together and needs that order)
+void check_pixel_list(const int* d_pI, int nI, int N, hipStream_t s) {
+  DCRF_REQUIRE(nI >= 0 && nI <= N, "restricted ordered filter: bad pixel count");
+  if (nI == 0) return;
+  std::vector<int> h(nI);
+  DCRF_HIP(hipMemcpyAsync(h.data(), d_pI, nI * sizeof(int), hipMemcpyDeviceToHost, s));
+  DCRF_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < nI; ++i)
+    DCRF_REQUIRE(h[i] >= 0 && h[i] < N && (i == 0 || h[i] > h[i - 1]),
+                 "restricted ordered filter: pixel indices must be strictly ascending and in [0, N)");
+}
+
 }  // namespace dcrf
 
 using namespace dcrf;
@@ -233,6 +246,28 @@ int dcrf_lattice_compute_upper_minus_lower_ord(dcrf_lattice_t lat, const float* 
   });
 }
 
+int dcrf_lattice_compute_upper_minus_lower_ord_restricted(dcrf_lattice_t lat, float* rout, const float* rQ,
+                                                           const int* pI, int nI, const float* Q, int V, int store,
+                                                           int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(lat && V > 0 && nI >= 0 && (nI == 0 || (rout && rQ && pI)) && (!store || Q),
+                 "dcrf_lattice_compute_upper_minus_lower_ord_restricted: bad arguments");
+    const int N = lat->lat->N();
+    hipStream_t s = lat->s;
+    const size_t nr = (size_t)nI * V;
+    Staged<int> ix(pI, nI, on_device != 0, s, true);
+    Staged<float> a(rQ, nr, on_device != 0, s, true);
+    Staged<float> o(rout, nr, on_device != 0, s, true);  // added to
+    Staged<float> e(store ? Q : nullptr, store ? (size_t)N * V : 0, on_device != 0, s, store != 0);
+    check_pixel_list(ix.d, nI, N, s);
+    if (nr) check_unit_interval(a.d, nr, s);
+    if (store) check_unit_interval(e.d, (size_t)N * V, s);
+    lat->lat->ordered_restricted(a.d, ix.d, nI, e.d, V, store != 0, o.d);
+    o.copy_out();
+    DCRF_HIP(hipStreamSynchronize(s));
+  });
+}
+
 // ---- pairwise potential ---------------------------------------------------------
 int dcrf_pairwise_create(const float* features, int N, int d, float potts_w, int ntype, int on_device,
                          dcrf_pairwise_t* out) {
@@ -272,6 +307,31 @@ int dcrf_pairwise_apply_upper_minus_lower_ord(dcrf_pairwise_t pw, const float* Q
     potential_apply_ord(pw->p, a.d, pw->mm0.get(), b.d, L);
     b.copy_out();
     DCRF_HIP(hipStreamSynchronize(pw->s));
+  });
+}
+
+static void apply_ord_restricted(Potential& p, hipStream_t s, float* rout, const float* rQ, const int* pI, int nI,
+                                 const float* Q, int L, int store, int on_device) {
+  const int N = p.lat->N();
+  const size_t nr = (size_t)nI * L;
+  Staged<int> ix(pI, nI, on_device != 0, s, true);
+  Staged<float> a(rQ, nr, on_device != 0, s, true);
+  Staged<float> o(rout, nr, on_device != 0, s, false);
+  Staged<float> e(store ? Q : nullptr, store ? (size_t)N * L : 0, on_device != 0, s, store != 0);
+  check_pixel_list(ix.d, nI, N, s);
+  if (nr) check_unit_interval(a.d, nr, s);
+  if (store) check_unit_interval(e.d, (size_t)N * L, s);
+  potential_apply_ord_restricted(p, o.d, a.d, ix.d, nI, e.d, L, store != 0);
+  o.copy_out();
+  DCRF_HIP(hipStreamSynchronize(s));
+}
+int dcrf_pairwise_apply_upper_minus_lower_ord_restricted(dcrf_pairwise_t pw, float* rout, const float* rQ,
+                                                         const int* pI, int nI, const float* Q, int L, int store,
+                                                         int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(pw && L > 0 && nI >= 0 && (nI == 0 || (rout && rQ && pI)) && (!store || Q),
+                 "dcrf_pairwise_apply_upper_minus_lower_ord_restricted: bad arguments");
+    apply_ord_restricted(pw->p, pw->s, rout, rQ, pI, nI, Q, L, store, on_device);
   });
 }
 
@@ -439,6 +499,16 @@ int dcrf_crf_apply_upper_minus_lower_ord(dcrf_crf_t crf, int k, const float* Q, 
     c.apply_ord(k, a.d, b.d, true);
     b.copy_out();
     DCRF_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
+int dcrf_crf_apply_upper_minus_lower_ord_restricted(dcrf_crf_t crf, int k, float* rout, const float* rQ,
+                                                    const int* pI, int nI, const float* Q, int store, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && nI >= 0 && (nI == 0 || (rout && rQ && pI)) && (!store || Q),
+                 "dcrf_crf_apply_upper_minus_lower_ord_restricted: bad arguments");
+    CRF& c = *crf->crf;
+    DCRF_REQUIRE(k >= 0 && k < c.num_pairwise(), "dcrf_crf_apply_upper_minus_lower_ord_restricted: no such potential");
+    apply_ord_restricted(c.pairwise(k), c.stream(), rout, rQ, pI, nI, Q, c.L(), store, on_device);
   });
 }
 int dcrf_crf_unary_init(dcrf_crf_t crf, float* Q, int on_device) {

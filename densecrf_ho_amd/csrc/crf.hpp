@@ -68,6 +68,14 @@ void potential_init_lattice(Potential& p, std::unique_ptr<Lattice> lat, float w,
 void potential_apply(Potential& p, const float* in, float* out, int V, bool transpose, bool no_norm);
 // PairwisePotential::apply_upper_minus_lower_ord (the caller checks [0,1])
 void potential_apply_ord(Potential& p, const float* q01, const float* d_zero_minmax, float* out, int V);
+// PairwisePotential::apply_upper_minus_lower_ord_restricted (src/pairwise.cpp:229-239):
+// rout (nI x V) = compatibility(restricted ordered filter); the caller checks
+// pI and the [0,1] ranges
+void potential_apply_ord_restricted(Potential& p, float* rout, const float* rq, const int* pI, int nI,
+                                    const float* ext, int V, bool store);
+// LabelCompatibility::apply of a Diagonal or Matrix compatibility on nrows
+// label-contiguous rows: out = C in (the Potts form is a scale by -w)
+void compat_apply_rows(Potential& p, const float* in, float* out, long nrows, int V, hipStream_t s);
 
 struct QPResult {
   int iterations = 0, updates = 0;

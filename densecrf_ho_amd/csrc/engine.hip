@@ -2,6 +2,7 @@
 // anonymous namespaces of the included sources).
 #include "lattice.hip"
 #include "crf.hip"
+#include "restricted.hip"
 #include "ho.hip"
 #include "lp.hip"
 #include "adjoint.hip"

@@ -15,6 +15,8 @@
 
 namespace dcrf {
 
+struct RestrictedState;
+
 // Input of a splat: value `k` of pixel `pix`.  Concrete forms are defined in
 // lattice.hip; the enum selects the one used by a launch.
 enum class SplatIn : int {
@@ -104,6 +106,14 @@ class Lattice {
       out.push_back(p);
   }
 
+  // seqCompute_upper_minus_lower_ord_restricted (src/permutohedral.cpp:702-880):
+  // the ordered filter at the pixels pI (ascending, nI of them, device
+  // pointer) with values rq (nI x V); the other pixels take their values from
+  // ext (N x V, read only when `store`).  `store` keeps the splat of the other
+  // pixels and the active points for later calls without it.  Adds into out
+  // (nI x V).  Not re-entrant: the state lives in the lattice.
+  void ordered_restricted(const float* rq, const int* pI, int nI, const float* ext, int V, bool store, float* out);
+
   // copies of the lattice tables for parity checks (host pointers, may be null)
   void export_tables(int* offset, int* rank, float* bary, short* keys, int* nbr) const;
 
@@ -164,6 +174,7 @@ class Lattice {
   DBuf<uint16_t> ppimg_;  // part of each lattice point (union lattices only)
   int nparts_ = 1;
   bool have_tb_ = false;
+  std::shared_ptr<RestrictedState> rs_;  // restricted ordered filter state
 };
 
 }  // namespace dcrf

@@ -84,6 +84,19 @@ int dcrf_lattice_compute(dcrf_lattice_t lat, const float* in, float* out, int V,
  * include/permutohedral.h:83, src/permutohedral.cpp:590-701 (10-bin ordered
  * filter, no normalisation, no compatibility).  Q is V x N with values in [0,1]. */
 int dcrf_lattice_compute_upper_minus_lower_ord(dcrf_lattice_t lat, const float* Q, float* out, int V, int on_device);
+/* Permutohedral::compute_upper_minus_lower_ord_restricted(MatrixXf& rout, const MatrixXf& rQ,
+ * const std::vector<int>& pI, const MatrixXf& Q, bool store), include/permutohedral.h:84-85,
+ * src/permutohedral.cpp:702-880, 1074-1078.  The ordered filter at the nI pixels pI
+ * (strictly ascending) with values rQ (V x nI); every other pixel takes its values from
+ * Q (V x N), which is read only when store != 0.  A call with store keeps the splat of
+ * the other pixels and the lattice points that pI touches in the handle; later calls
+ * without store reuse both (their pI pixels are added to the stored splat, and only the
+ * stored points are blurred).  rout (V x nI) is ADDED to, as the reference does.  rQ and
+ * Q must lie in [0,1] (DCRF_OUT_OF_BOUNDS otherwise); a call without store before any
+ * call with store, or with another V, fails.  Not re-entrant per handle. */
+int dcrf_lattice_compute_upper_minus_lower_ord_restricted(dcrf_lattice_t lat, float* rout, const float* rQ,
+                                                           const int* pI, int nI, const float* Q, int V, int store,
+                                                           int on_device);
 
 /* ------------------------------------------------------------------------ */
 /* PairwisePotential with Potts compatibility (include/pairwise.h:63-95)     */
@@ -98,6 +111,13 @@ int dcrf_pairwise_destroy(dcrf_pairwise_t pw);
 int dcrf_pairwise_apply(dcrf_pairwise_t pw, const float* in, float* out, int L, int transpose, int on_device);
 /* PairwisePotential::apply_upper_minus_lower_ord, src/pairwise.cpp:214-228 */
 int dcrf_pairwise_apply_upper_minus_lower_ord(dcrf_pairwise_t pw, const float* Q, float* out, int L, int on_device);
+/* PairwisePotential::apply_upper_minus_lower_ord_restricted(rout, rQ, pI, Q, store),
+ * include/pairwise.h:78-79, src/pairwise.cpp:229-239: rout (L x nI) = 0, the restricted
+ * ordered filter (as dcrf_lattice_compute_upper_minus_lower_ord_restricted), then the
+ * compatibility (Potts: -w). */
+int dcrf_pairwise_apply_upper_minus_lower_ord_restricted(dcrf_pairwise_t pw, float* rout, const float* rQ,
+                                                         const int* pI, int nI, const float* Q, int L, int store,
+                                                         int on_device);
 
 /* ------------------------------------------------------------------------ */
 /* DenseCRF (include/densecrf.h:73-194)                                      */
@@ -147,6 +167,11 @@ int dcrf_crf_apply(dcrf_crf_t crf, int k, const float* in, float* out, int trans
 /* PairwisePotential::apply_upper_minus_lower_ord(out, Q), src/pairwise.cpp:214-228:
  * fails with DCRF_OUT_OF_BOUNDS when Q leaves [0,1]. */
 int dcrf_crf_apply_upper_minus_lower_ord(dcrf_crf_t crf, int k, const float* Q, float* out, int on_device);
+/* PairwisePotential::apply_upper_minus_lower_ord_restricted of potential k,
+ * src/pairwise.cpp:229-239, with that potential's compatibility (Potts, Diagonal or
+ * Matrix); rout is L x nI, Q is L x N. */
+int dcrf_crf_apply_upper_minus_lower_ord_restricted(dcrf_crf_t crf, int k, float* rout, const float* rQ,
+                                                    const int* pI, int nI, const float* Q, int store, int on_device);
 /* DenseCRF::unary_init(), src/densecrf.cpp:321-325 */
 int dcrf_crf_unary_init(dcrf_crf_t crf, float* Q, int on_device);
 

@@ -13,6 +13,7 @@
 // include/labelcompatibility.h:36-63, include/densecrf.h:43-194.
 #pragma once
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -210,6 +211,17 @@ class Permutohedral {
     if (out.rows() != Q.rows() || out.cols() != Q.cols()) out.resize(Q.rows(), Q.cols());
     check(dcrf_lattice_compute_upper_minus_lower_ord(h_, Q.data(), out.data(), (int)Q.rows(), 0));
   }
+  // rout (V x |pI|) is added to, as the reference does
+  void compute_upper_minus_lower_ord_restricted(MatrixXf& rout, const MatrixXf& rQ, const std::vector<int>& pI,
+                                                const MatrixXf& Q, const bool store) {
+    if (rout.rows() != rQ.rows() || rout.cols() != rQ.cols()) {
+      rout.resize(rQ.rows(), rQ.cols());
+      std::fill(rout.data(), rout.data() + (size_t)rout.rows() * rout.cols(), 0.f);
+    }
+    check(dcrf_lattice_compute_upper_minus_lower_ord_restricted(h_, rout.data(), rQ.data(), pI.data(), (int)pI.size(),
+                                                                store ? Q.data() : nullptr, (int)rQ.rows(),
+                                                                store ? 1 : 0, 0));
+  }
   int latticeSize() const {
     int M = 0;
     check(dcrf_lattice_info(h_, &M, nullptr));
@@ -246,6 +258,13 @@ class PairwisePotential {
   void apply_upper_minus_lower_ord(MatrixXf& out, const MatrixXf& Q) {
     if (out.rows() != Q.rows() || out.cols() != Q.cols()) out.resize(Q.rows(), Q.cols());
     check(dcrf_pairwise_apply_upper_minus_lower_ord(h_, Q.data(), out.data(), (int)Q.rows(), 0));
+  }
+  void apply_upper_minus_lower_ord_restricted(MatrixXf& rout, const MatrixXf& rQ, const std::vector<int>& pI,
+                                              const MatrixXf& Q, const bool store) {
+    if (rout.rows() != rQ.rows() || rout.cols() != rQ.cols()) rout.resize(rQ.rows(), rQ.cols());
+    check(dcrf_pairwise_apply_upper_minus_lower_ord_restricted(h_, rout.data(), rQ.data(), pI.data(), (int)pI.size(),
+                                                               store ? Q.data() : nullptr, (int)rQ.rows(),
+                                                               store ? 1 : 0, 0));
   }
   MatrixXf features() const { return f_; }
   KernelType ktype() const { return ktype_; }

@@ -23,6 +23,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -34,6 +35,7 @@ namespace orc {
 //   1 no padding lanes   2 round half away from zero   3 upper bin without 1e-9
 //   4 float blur for V<=2   5 QP step without the factor 2   6 last argmin
 //   7 superpixel sums over member ids   8 ucondgrad without int truncation
+//   9 restricted ordered filter blurring every point
 static int g_mut = 0;
 
 // ---------------------------------------------------------------------------
@@ -465,6 +467,114 @@ struct Lattice {
         }
       }
     }
+  }
+};
+
+// seqCompute_upper_minus_lower_ord_restricted (src/permutohedral.cpp:702-880):
+// the ordered filter of the pixels pI (ascending) with every other pixel
+// splatted from ext (full N x V field); only lattice points touched by pI are
+// blurred.  State kept between calls as in the reference object: the splat of
+// the pixels outside pI (one per pass) and the active points, both written by
+// a call with store.  out (|pI| x V) is added to, as in the reference.
+struct Restricted {
+  std::vector<float> stored[2];
+  std::vector<int> active;
+  int V = 0;
+  bool have = false;
+  void run(const Lattice& lat, const float* rin, const int* pI, int nI, const float* ext, int V_, bool store,
+           float* out) {
+    const int d = lat.d, d1 = d + 1, M = lat.M, N = lat.N, R = 10;
+    if (!store && (!have || V_ != V)) throw std::runtime_error("restricted: no stored values for this value size");
+    V = V_;
+    const size_t rows = (size_t)(M + 2) * V * R;
+    const float alpha = 1.0f / (1 + std::pow(2.0f, (float)-d));
+    std::vector<float> a(rows), b(rows);
+    std::vector<int> inI(N, -1);
+    for (int ii = 0; ii < nI; ++ii) inI[pI[ii]] = ii;
+    if (store) {
+      std::vector<char> act(M, 0);
+      active.clear();
+      for (int ii = 0; ii < nI; ++ii)  // :752-758, in pI order
+        for (int j = 0; j <= d; ++j) {
+          int o = lat.offset[(size_t)pI[ii] * d1 + j];
+          if (!act[o]) {
+            act[o] = 1;
+            active.push_back(o);
+          }
+        }
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+      const bool upper = pass == 1;
+      std::fill(a.begin(), a.end(), 0.f);
+      std::fill(b.begin(), b.end(), 0.f);
+      if (store) {  // :725-737 / :806-818: the pixels outside pI, from ext
+        for (int i = 0; i < N; ++i) {
+          if (inI[i] >= 0) continue;
+          for (int j = 0; j <= d; ++j) {
+            int o = lat.offset[(size_t)i * d1 + j] + 1;
+            float w = lat.bary[(size_t)i * d1 + j];
+            for (int k = 0; k < V; ++k) {
+              float qq = ext[(size_t)i * V + k];
+              float* s = &a[((size_t)o * V + k) * R];
+              if (upper)
+                for (int c = Lattice::bin_up(qq); c < R; ++c) s[c] = s[c] + w;
+              else
+                for (int c = 0, e = Lattice::bin_lo(qq); c <= e; ++c) s[c] = s[c] + w;
+            }
+          }
+        }
+        stored[pass] = a;
+      } else {
+        a = stored[pass];  // :761-763 / :833-835
+      }
+      for (int ii = 0; ii < nI; ++ii) {  // the pixels of pI, from rin
+        const int i = pI[ii];
+        for (int j = 0; j <= d; ++j) {
+          int o = lat.offset[(size_t)i * d1 + j] + 1;
+          float w = lat.bary[(size_t)i * d1 + j];
+          for (int k = 0; k < V; ++k) {
+            float qq = rin[(size_t)ii * V + k];
+            float* s = &a[((size_t)o * V + k) * R];
+            if (upper)
+              for (int c = Lattice::bin_up(qq); c < R; ++c) s[c] = s[c] + w;
+            else
+              for (int c = 0, e = Lattice::bin_lo(qq); c <= e; ++c) s[c] = s[c] + w;
+          }
+        }
+      }
+      float* values = a.data();
+      float* nv = b.data();
+      std::vector<int> all;
+      if (g_mut == 9)
+        for (int i = 0; i < M; ++i) all.push_back(i);
+      for (int j = 0; j <= d; ++j) {  // :778-793: active points only; the others keep what the buffer holds
+        for (int i : (g_mut == 9 ? all : active)) {
+          int n1 = lat.nbr[((size_t)j * M + i) * 2] + 1, n2 = lat.nbr[((size_t)j * M + i) * 2 + 1] + 1;
+          for (int k = 0; k < V; ++k) {
+            const float* ov = values + ((size_t)(i + 1) * V + k) * R;
+            const float* v1 = values + ((size_t)n1 * V + k) * R;
+            const float* v2 = values + ((size_t)n2 * V + k) * R;
+            float* nw = nv + ((size_t)(i + 1) * V + k) * R;
+            for (int c = 0; c < R; ++c) nw[c] = ov[c] + 0.5f * (v1[c] + v2[c]);
+          }
+        }
+        std::swap(values, nv);
+      }
+      for (int ii = 0; ii < nI; ++ii) {  // :796-807 / :866-878
+        const int i = pI[ii];
+        for (int j = 0; j <= d; ++j) {
+          int o = lat.offset[(size_t)i * d1 + j] + 1;
+          float w = lat.bary[(size_t)i * d1 + j];
+          const float coef = upper ? w * alpha : -w * alpha;
+          for (int k = 0; k < V; ++k) {
+            float qq = rin[(size_t)ii * V + k];
+            int c = upper ? Lattice::bin_up(qq) : Lattice::bin_lo(qq);
+            out[(size_t)ii * V + k] = out[(size_t)ii * V + k] + values[((size_t)o * V + k) * R + c] * coef;
+          }
+        }
+      }
+    }
+    if (store) have = true;
   }
 };
 
@@ -1217,6 +1327,7 @@ struct OrcCRF {
 };
 
 static thread_local std::string g_err;
+static std::map<void*, orc::Restricted> g_restricted;
 
 extern "C" {
 
@@ -1228,7 +1339,10 @@ void* orc_lattice_new(const float* feat, int N, int d) {
   L->init(feat, N, d);
   return L;
 }
-void orc_lattice_free(void* h) { delete (Lattice*)h; }
+void orc_lattice_free(void* h) {
+  g_restricted.erase(h);
+  delete (Lattice*)h;
+}
 int orc_lattice_M(void* h) { return ((Lattice*)h)->M; }
 int orc_lattice_Npad(void* h) { return ((Lattice*)h)->Npad; }
 void orc_lattice_get(void* h, int* offset, int* rank, float* bary, short* keys, int* nbr) {
@@ -1247,6 +1361,17 @@ void orc_filter_d(void* h, const double* in, double* out, int V, int reverse) {
 }
 void orc_ordered_f(void* h, const float* q, float* out, int V) { ((Lattice*)h)->ordered<float>(q, out, V); }
 void orc_ordered_d(void* h, const float* q, double* out, int V) { ((Lattice*)h)->ordered<double>(q, out, V); }
+// restricted ordered filter; its stored state lives beside the lattice handle
+int orc_ordered_restricted(void* h, const float* rin, const int* pI, int nI, const float* ext, int V, int store,
+                           float* out) {
+  try {
+    g_restricted[h].run(*(Lattice*)h, rin, pI, nI, ext, V, store != 0, out);
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+  return 0;
+}
 
 void* orc_crf_new(int N, int L) {
   OrcCRF* c = new OrcCRF{N, L, {}, new CRF<float>(N, L), new CRF<double>(N, L)};

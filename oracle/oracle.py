@@ -60,6 +60,7 @@ def lib():
         _sig(L, "orc_filter_d", None, _vp, _f64p, _f64p, _i, _i)
         _sig(L, "orc_ordered_f", None, _vp, _f32p, _f32p, _i)
         _sig(L, "orc_ordered_d", None, _vp, _f32p, _f64p, _i)
+        _sig(L, "orc_ordered_restricted", _i, _vp, _f32p, _i32p, _i, _f32p, _i, _i, _f32p)
         _sig(L, "orc_crf_new", _vp, _i, _i)
         _sig(L, "orc_crf_free", None, _vp)
         _sig(L, "orc_crf_set_unary", None, _vp, _f32p)
@@ -144,6 +145,18 @@ class Lattice:
         V = q.shape[1]
         out = np.zeros(q.shape, np.float64 if fp64 else np.float32)
         (self._L.orc_ordered_d if fp64 else self._L.orc_ordered_f)(self.h, q, out, V)
+        return out
+
+    def ordered_restricted(self, rq, pI, q, store, out=None):
+        """seqCompute_upper_minus_lower_ord_restricted: rq (|pI| x V) for the
+        pixels pI, q (N x V) for the others; out is added to (zeros if None)"""
+        rq = _c(rq, np.float32)
+        pI = _c(pI, np.int32)
+        V = rq.shape[1]
+        q = _c(q, np.float32) if q is not None else np.zeros((self.N, V), np.float32)
+        out = np.zeros((len(pI), V), np.float32) if out is None else _c(out, np.float32).copy()
+        if self._L.orc_ordered_restricted(self.h, rq, pI, len(pI), q, V, int(store), out) != 0:
+            raise RuntimeError(self._L.orc_last_error().decode())
         return out
 
 
@@ -379,6 +392,7 @@ def reflib():
         _sig(R, "ref_lattice_get", None, _vp, _i32p, _i32p, _f32p, _i32p, _i)
         _sig(R, "ref_lattice_compute", None, _vp, _f32p, _f32p, _i, _i, _i)
         _sig(R, "ref_lattice_ordered", None, _vp, _f32p, _f32p, _i, _i)
+        _sig(R, "ref_lattice_ordered_restricted", _i, _vp, _f32p, _i32p, _i, _f32p, _i, _i, _i, _f32p)
         _sig(R, "ref_crf_new", _vp, _i, _i, _i)
         _sig(R, "ref_crf_free", None, _vp)
         _sig(R, "ref_crf_set_unary", None, _vp, _f32p)
@@ -443,6 +457,15 @@ class RefLattice:
         q = _c(q, np.float32)
         out = np.zeros_like(q)
         self._R.ref_lattice_ordered(self.h, q, out, q.shape[1], self.N)
+        return out
+
+    def ordered_restricted(self, rq, pI, q, store, out=None):
+        rq = _c(rq, np.float32)
+        pI = _c(pI, np.int32)
+        V = rq.shape[1]
+        q = _c(q, np.float32) if q is not None else np.zeros((self.N, V), np.float32)
+        out = np.zeros((len(pI), V), np.float32) if out is None else _c(out, np.float32).copy()
+        self._R.ref_lattice_ordered_restricted(self.h, rq, pI, len(pI), q, V, self.N, int(store), out)
         return out
 
 

@@ -107,6 +107,17 @@ void ref_lattice_ordered(void* h, const float* q, float* out, int V, int N) {
   std::copy(o.data(), o.data() + (size_t)V * N, out);
 }
 
+// Permutohedral::compute_upper_minus_lower_ord_restricted: out (V x nI) is
+// passed in and added to, as the reference does
+int ref_lattice_ordered_restricted(void* h, const float* rq, const int* pI, int nI, const float* q, int V, int N,
+                                   int store, float* out) {
+  MatrixXf r = as_mat(rq, V, nI), full = as_mat(q, V, N), o = as_mat(out, V, nI);
+  std::vector<int> idx(pI, pI + nI);
+  ((RefLattice*)h)->compute_upper_minus_lower_ord_restricted(o, r, idx, full, store != 0);
+  std::copy(o.data(), o.data() + (size_t)V * nI, out);
+  return 0;
+}
+
 // ---- DenseCRF --------------------------------------------------------------
 void* ref_crf_new(int W, int H, int L) { return new RefCRF(W, H, L); }
 void ref_crf_free(void* h) { delete (RefCRF*)h; }

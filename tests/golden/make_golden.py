@@ -7,6 +7,7 @@ compressed .npz files.  Run from the repository root:
 
     python tests/golden/make_golden.py          # every fixture
     python tests/golden/make_golden.py lp_lc    # only lp_lc_31x23_L5.npz
+    python tests/golden/make_golden.py restricted  # only ordered_restricted_31x23.npz
 """
 
 from __future__ import annotations
@@ -223,9 +224,40 @@ def lp_less_confident_case():
     print("lp_lc_31x23_L5")
 
 
+def restricted_case():
+    """Permutohedral::compute_upper_minus_lower_ord_restricted
+    (src/permutohedral.cpp:702-880) on the 31x23 lattices (d=2 and d=5),
+    V=5, values with exact bin edges: (1) store with the pixels pI, (2) no
+    store, same pI, new values, (3) no store, another pI (points outside the
+    stored active set), output added to a nonzero field."""
+    wl = S.Workload(31, 23, 8, 1)
+    N, V = wl.N, 5
+    d = {}
+    for k, (f, _) in enumerate(wl.kernels):
+        lat = O.RefLattice(f)
+        rng = np.random.default_rng(40 + k)
+        q = edge_q(N, V, 41 + k)
+        pI = np.sort(rng.choice(N, N // 5, replace=False)).astype(np.int32)
+        pI2 = np.sort(rng.choice(N, N // 4, replace=False)).astype(np.int32)
+        rq1 = edge_q(len(pI), V, 42 + k)
+        rq2 = edge_q(len(pI), V, 43 + k)
+        rq3 = edge_q(len(pI2), V, 44 + k)
+        o3 = rng.standard_normal((len(pI2), V)).astype(np.float32)
+        d.update({f"feat{k}": f, f"q{k}": q, f"pI{k}": pI, f"pI2_{k}": pI2, f"rq1_{k}": rq1, f"rq2_{k}": rq2,
+                  f"rq3_{k}": rq3, f"o3_{k}": o3})
+        d[f"out1_{k}"] = lat.ordered_restricted(rq1, pI, q, True)
+        d[f"out2_{k}"] = lat.ordered_restricted(rq2, pI, None, False)
+        d[f"out3_{k}"] = lat.ordered_restricted(rq3, pI2, None, False, out=o3)
+    np.savez_compressed(os.path.join(OUT, "ordered_restricted_31x23.npz"), **d)
+    print("ordered_restricted_31x23")
+
+
 def main():
     if sys.argv[1:] == ["lp_lc"]:
         lp_less_confident_case()
+        return
+    if sys.argv[1:] == ["restricted"]:
+        restricted_case()
         return
     W, H = 32, 24
     wl = S.Workload(W, H, 10, 0)
@@ -243,6 +275,7 @@ def main():
     qp_case("ties_31x23_L5", 31, 23, 8, 1, 5, dup_label=(1, 3))
     compat_case("31x23_L5", 31, 23, 8, 1, 5)
     lp_less_confident_case()
+    restricted_case()
     c0_case()
 
 

@@ -137,3 +137,39 @@ def test_potentials_wide_rows_vs_oracle(L):
     Qg, stg = c.qp_inference_super_pixels_non_convex(Q0, max_iter=5, ref_stop=False, stats=True)
     np.testing.assert_allclose(stg["energy"], std["energy"], rtol=1e-4)
     assert (Qg.argmax(1) == Qd.argmax(1)).mean() >= 0.999
+
+
+@pytest.mark.parametrize("compat", ["potts", "matrix"])
+def test_restricted_ordered_potential_vs_oracle(golden, compat):
+    """PairwisePotential::apply_upper_minus_lower_ord_restricted
+    (src/pairwise.cpp:229-239): rout = 0, the restricted filter of the
+    potential's lattice, then its compatibility (Potts -w, or the Matrix form)"""
+    import densecrf_ho_amd as D
+    g = golden("crf_31x23_L5.npz")
+    W, H, L = int(g["W"]), int(g["H"]), int(g["L"])
+    c = D.DenseCRF2D(W, H, L)
+    c.setUnaryEnergy(g["U"])
+    rng = np.random.default_rng(17)
+    m = rng.standard_normal((L, L)).astype(np.float32)
+    fn = D.PottsCompatibility(5.0) if compat == "potts" else D.MatrixCompatibility(m)
+    c.addPairwiseEnergy(g["feat1"], fn)
+    N = W * H
+    q = rng.random((N, L)).astype(np.float32)
+    pI = np.sort(rng.choice(N, N // 3, replace=False)).astype(np.int32)
+    rq = rng.random((len(pI), L)).astype(np.float32)
+    out = c.apply_upper_minus_lower_ord_restricted(0, rq, pI, q, True)
+    r = O.Lattice(g["feat1"]).ordered_restricted(rq, pI, q, True)
+    if compat == "potts":
+        ref = np.float32(-5.0) * r
+    else:
+        Wm = ((m + m.T) / 2).astype(np.float32)
+        ref = (r.astype(np.float64) @ Wm.astype(np.float64).T).astype(np.float32)
+    assert rel_err(out, ref) < 1e-5
+    rq2 = rng.random((len(pI), L)).astype(np.float32)
+    out2 = c.apply_upper_minus_lower_ord_restricted(0, rq2, pI, None, False)
+    lat = O.Lattice(g["feat1"])
+    lat.ordered_restricted(rq, pI, q, True)
+    r2 = lat.ordered_restricted(rq2, pI, None, False)
+    ref2 = np.float32(-5.0) * r2 if compat == "potts" else \
+        (r2.astype(np.float64) @ Wm.astype(np.float64).T).astype(np.float32)
+    assert rel_err(out2, ref2) < 1e-5

@@ -118,3 +118,57 @@ def test_wide_rows_vs_oracle(V, d):
         out = lat.compute(x, reverse=rev)
         assert rel_err(out, ref.filter(x, reverse=rev)) < 1e-5
         assert rel_err(out, ref.filter(x.astype(np.float64), reverse=rev, fp64=True)) < 1e-4
+
+
+def _restricted_gpu_calls(lat, g, k):
+    out1 = lat.compute_upper_minus_lower_ord_restricted(None, g[f"rq1_{k}"], g[f"pI{k}"], g[f"q{k}"], True)
+    out2 = lat.compute_upper_minus_lower_ord_restricted(None, g[f"rq2_{k}"], g[f"pI{k}"], None, False)
+    out3 = lat.compute_upper_minus_lower_ord_restricted(g[f"o3_{k}"].copy(), g[f"rq3_{k}"], g[f"pI2_{k}"], None,
+                                                        False)
+    return out1, out2, out3
+
+
+@pytest.mark.parametrize("k", [0, 1])
+def test_restricted_ordered_vs_reference(golden, k):
+    """compute_upper_minus_lower_ord_restricted (src/permutohedral.cpp:702-880):
+    a call with store, one reusing the stored splat and active points, one
+    with another pixel set added into a nonzero output; within 1e-5 of the
+    compiled reference (the kernels keep its float order)"""
+    g = golden("ordered_restricted_31x23.npz")
+    print("case d =", 2 if k == 0 else 5, flush=True)
+    lat = gpu_lattice(g[f"feat{k}"])
+    for i, o in enumerate(_restricted_gpu_calls(lat, g, k)):
+        ref = g[f"out{i + 1}_{k}"]
+        print("call", i + 1, "max abs diff", float(np.abs(o - ref).max()), flush=True)
+        assert rel_err(o, ref) < 1e-5
+
+
+def test_restricted_ordered_errors(golden):
+    import densecrf_ho_amd as D
+    g = golden("ordered_restricted_31x23.npz")
+    lat = gpu_lattice(g["feat1"])
+    pI, rq = g["pI1"], g["rq1_1"]
+    with pytest.raises(D.EngineError, match="store first"):
+        lat.compute_upper_minus_lower_ord_restricted(None, rq, pI, None, False)
+    with pytest.raises(D.EngineError, match="ascending"):
+        lat.compute_upper_minus_lower_ord_restricted(None, rq, pI[::-1].copy(), g["q1"], True)
+    bad = rq.copy()
+    bad[3, 1] = 1.5
+    with pytest.raises(D.EngineError, match="Value out of bounds"):
+        lat.compute_upper_minus_lower_ord_restricted(None, bad, pI, g["q1"], True)
+
+
+@pytest.mark.parametrize("d", [2, 5])
+def test_restricted_ordered_every_pixel_is_ordered_filter(d):
+    """640x480 (N % 4 == 0): with every pixel in pI every lattice point is
+    active and the restricted filter is the ordered filter"""
+    from densecrf_ho_amd import synthetic as S
+    wl = S.Workload(640, 480, 120, 1, L=21)
+    feat = wl.kernels[0 if d == 2 else 1][0]
+    lat = gpu_lattice(feat)
+    q = np.random.default_rng(11).random((wl.N, 21)).astype(np.float32)
+    pI = np.arange(wl.N, dtype=np.int32)
+    a = lat.compute_upper_minus_lower_ord_restricted(None, q, pI, q, True)
+    b = lat.compute_upper_minus_lower_ord(q)
+    print("max abs diff", float(np.abs(a - b).max()), flush=True)
+    assert rel_err(a, b) < 1e-5

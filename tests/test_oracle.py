@@ -318,6 +318,45 @@ def test_lp_matches_reference(golden, case):
     assert len(st["qp_sweeps"]) >= 1 and st["qp_sweeps"].min() >= 2  # first sweep never stops
 
 
+def _restricted_calls(lat, g, k):
+    out1 = lat.ordered_restricted(g[f"rq1_{k}"], g[f"pI{k}"], g[f"q{k}"], True)
+    out2 = lat.ordered_restricted(g[f"rq2_{k}"], g[f"pI{k}"], None, False)
+    out3 = lat.ordered_restricted(g[f"rq3_{k}"], g[f"pI2_{k}"], None, False, out=g[f"o3_{k}"])
+    return out1, out2, out3
+
+
+@pytest.mark.parametrize("k", [0, 1])
+def test_restricted_ordered_matches_reference(golden, k):
+    """compute_upper_minus_lower_ord_restricted: store, reuse of the stored
+    splat and active set, a second pixel set, accumulation into out; bit for
+    bit against the compiled reference"""
+    g = golden("ordered_restricted_31x23.npz")
+    lat = O.Lattice(g[f"feat{k}"])
+    for i, o in enumerate(_restricted_calls(lat, g, k)):
+        np.testing.assert_array_equal(o, g[f"out{i + 1}_{k}"])
+
+
+def test_restricted_ordered_planted_mistake_is_caught(golden):
+    g = golden("ordered_restricted_31x23.npz")
+    try:
+        O.set_mutation(9)  # blur every lattice point instead of the active ones
+        lat = O.Lattice(g["feat1"])
+        outs = _restricted_calls(lat, g, 1)
+    finally:
+        O.set_mutation(0)
+    assert any(not np.array_equal(o, g[f"out{i + 1}_1"]) for i, o in enumerate(outs))
+
+
+def test_restricted_ordered_with_every_pixel_is_the_ordered_filter(golden):
+    """with every pixel in pI and N % 4 == 0 every lattice point is active,
+    so the restricted filter is the ordered filter (same float order)"""
+    g = golden("lattice_b5_32x24.npz")
+    lat = O.Lattice(g["feat"])
+    q = np.random.default_rng(2).random((lat.N, 5)).astype(np.float32)
+    pI = np.arange(lat.N, dtype=np.int32)
+    np.testing.assert_array_equal(lat.ordered_restricted(q, pI, q, True), lat.ordered(q))
+
+
 def test_lp_less_confident_stop_matches_reference(golden):
     """less_confident_percent / confidence_tol (src/densecrf.cpp:457-462,
     784-794): four settings stop after 4 (no check), 1, 2 and 3 outer steps;
