@@ -49,6 +49,9 @@ SIGNATURES = {
     "dcrf_pairwise_destroy": (_i, [_vp]),
     "dcrf_pairwise_apply": (_i, [_vp, _vp, _vp, _i, _i, _i]),
     "dcrf_pairwise_apply_upper_minus_lower_ord": (_i, [_vp, _vp, _vp, _i, _i]),
+    "dcrf_pairwise_apply_bf": (_i, [_vp, _vp, _vp, _i, _i]),
+    "dcrf_pairwise_apply_upper_minus_lower_bf": (_i, [_vp, _vp, _vp, _i, _i]),
+    "dcrf_pairwise_apply_upper_minus_lower_bf_ord": (_i, [_vp, _vp, _vp, _vp, _i, _i]),
     "dcrf_pairwise_apply_upper_minus_lower_ord_restricted": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _i, _i, _i]),
     "dcrf_crf_create": (_i, [_i, _i, C.POINTER(_vp)]),
     "dcrf_crf_destroy": (_i, [_vp]),
@@ -67,6 +70,9 @@ SIGNATURES = {
     "dcrf_crf_add_superpixel_layer": (_i, [_vp, _vp, _i, _vp, _vp]),
     "dcrf_crf_apply": (_i, [_vp, _i, _vp, _vp, _i, _i, _i]),
     "dcrf_crf_apply_upper_minus_lower_ord": (_i, [_vp, _i, _vp, _vp, _i]),
+    "dcrf_crf_apply_bf": (_i, [_vp, _i, _vp, _vp, _i]),
+    "dcrf_crf_apply_upper_minus_lower_bf": (_i, [_vp, _i, _vp, _vp, _i]),
+    "dcrf_crf_apply_upper_minus_lower_bf_ord": (_i, [_vp, _i, _vp, _vp, _vp, _i]),
     "dcrf_crf_apply_upper_minus_lower_ord_restricted": (_i, [_vp, _i, _vp, _vp, _vp, _i, _vp, _i, _i]),
     "dcrf_crf_unary_init": (_i, [_vp, _vp, _i]),
     "dcrf_crf_qp": (_i, [_vp, _vp, _vp, _i, _i, _i, C.POINTER(QPStats)]),

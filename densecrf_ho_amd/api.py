@@ -442,6 +442,33 @@ class DenseCRF:
         _check(_lib.load().dcrf_crf_apply_upper_minus_lower_ord(self._h, k, a.ptr, o.ptr, a.dev))
         return out
 
+    def apply_bf(self, k, Q, out=None):
+        """PairwisePotential::apply_bf of potential k (dense Gaussian, O(N^2 L))."""
+        a = _Arg(Q, np.float32)
+        out = _empty_like_field(Q, (self.N, self.M)) if out is None else out
+        o = _Arg(out, np.float32, out=True)
+        _check(_lib.load().dcrf_crf_apply_bf(self._h, k, a.ptr, o.ptr, a.dev))
+        return out
+
+    def apply_upper_minus_lower_bf(self, k, ind, out=None):
+        """PairwisePotential::apply_upper_minus_lower_bf of potential k; ind (N, M):
+        ind[c, l] is the pixel at sorted position c of label l; the result stays
+        in that sorted order."""
+        ix = _Arg(ind, np.int32)
+        out = _empty_like_field(ind, (self.N, self.M)) if out is None else out
+        o = _Arg(out, np.float32, out=True)
+        _check(_lib.load().dcrf_crf_apply_upper_minus_lower_bf(self._h, k, ix.ptr, o.ptr, ix.dev))
+        return out
+
+    def apply_upper_minus_lower_bf_ord(self, k, ind, Q, out=None):
+        """PairwisePotential::apply_upper_minus_lower_bf_ord of potential k."""
+        ix = _Arg(ind, np.int32)
+        a = _Arg(Q, np.float32)
+        out = _empty_like_field(Q, (self.N, self.M)) if out is None else out
+        o = _Arg(out, np.float32, out=True)
+        _check(_lib.load().dcrf_crf_apply_upper_minus_lower_bf_ord(self._h, k, ix.ptr, a.ptr, o.ptr, a.dev))
+        return out
+
     def apply_upper_minus_lower_ord_restricted(self, k, rQ, pI, Q=None, store: bool = True, out=None):
         """PairwisePotential::apply_upper_minus_lower_ord_restricted of potential k:
         (len(pI), M) = compatibility(restricted ordered filter)."""

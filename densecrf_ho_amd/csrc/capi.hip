@@ -173,6 +173,14 @@ void check_pixel_list(const int* d_pI, int nI, int N, hipStream_t s) {
                  "restricted ordered filter: pixel indices must be strictly ascending and in [0, N)");
 }
 
+// ind of the brute-force sorted filters: every entry a pixel index in [0, N)
+void check_index_matrix(const int* d_ind, size_t n, int N, hipStream_t s) {
+  std::vector<int> h(n);
+  DCRF_HIP(hipMemcpyAsync(h.data(), d_ind, n * sizeof(int), hipMemcpyDeviceToHost, s));
+  DCRF_HIP(hipStreamSynchronize(s));
+  for (size_t i = 0; i < n; ++i) DCRF_REQUIRE(h[i] >= 0 && h[i] < N, "brute-force filter: ind entry out of range");
+}
+
 }  // namespace dcrf
 
 using namespace dcrf;
@@ -332,6 +340,41 @@ int dcrf_pairwise_apply_upper_minus_lower_ord_restricted(dcrf_pairwise_t pw, flo
     DCRF_REQUIRE(pw && L > 0 && nI >= 0 && (nI == 0 || (rout && rQ && pI)) && (!store || Q),
                  "dcrf_pairwise_apply_upper_minus_lower_ord_restricted: bad arguments");
     apply_ord_restricted(pw->p, pw->s, rout, rQ, pI, nI, Q, L, store, on_device);
+  });
+}
+
+static void apply_bf_common(Potential& p, hipStream_t s, int kind, const int* ind, const float* Q, float* out, int L,
+                            int on_device) {
+  const int N = p.lat->N();
+  const size_t n = (size_t)N * L;
+  Staged<int> ix(kind ? ind : nullptr, kind ? n : 0, on_device != 0, s, kind != 0);
+  Staged<float> q(kind != 1 ? Q : nullptr, kind != 1 ? n : 0, on_device != 0, s, kind != 1);
+  Staged<float> o(out, n, on_device != 0, s, false);
+  if (kind) check_index_matrix(ix.d, n, N, s);
+  if (kind == 0)
+    potential_apply_bf(p, q.d, o.d, L);
+  else
+    potential_apply_uml_bf(p, ix.d, q.d, o.d, L, kind == 2);
+  o.copy_out();
+  DCRF_HIP(hipStreamSynchronize(s));
+}
+int dcrf_pairwise_apply_bf(dcrf_pairwise_t pw, const float* Q, float* out, int L, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(pw && Q && out && L > 0, "dcrf_pairwise_apply_bf: bad arguments");
+    apply_bf_common(pw->p, pw->s, 0, nullptr, Q, out, L, on_device);
+  });
+}
+int dcrf_pairwise_apply_upper_minus_lower_bf(dcrf_pairwise_t pw, const int* ind, float* out, int L, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(pw && ind && out && L > 0, "dcrf_pairwise_apply_upper_minus_lower_bf: bad arguments");
+    apply_bf_common(pw->p, pw->s, 1, ind, nullptr, out, L, on_device);
+  });
+}
+int dcrf_pairwise_apply_upper_minus_lower_bf_ord(dcrf_pairwise_t pw, const int* ind, const float* Q, float* out,
+                                                 int L, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(pw && ind && Q && out && L > 0, "dcrf_pairwise_apply_upper_minus_lower_bf_ord: bad arguments");
+    apply_bf_common(pw->p, pw->s, 2, ind, Q, out, L, on_device);
   });
 }
 
@@ -509,6 +552,31 @@ int dcrf_crf_apply_upper_minus_lower_ord_restricted(dcrf_crf_t crf, int k, float
     CRF& c = *crf->crf;
     DCRF_REQUIRE(k >= 0 && k < c.num_pairwise(), "dcrf_crf_apply_upper_minus_lower_ord_restricted: no such potential");
     apply_ord_restricted(c.pairwise(k), c.stream(), rout, rQ, pI, nI, Q, c.L(), store, on_device);
+  });
+}
+int dcrf_crf_apply_bf(dcrf_crf_t crf, int k, const float* Q, float* out, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && Q && out, "dcrf_crf_apply_bf: bad arguments");
+    CRF& c = *crf->crf;
+    DCRF_REQUIRE(k >= 0 && k < c.num_pairwise(), "dcrf_crf_apply_bf: no such potential");
+    apply_bf_common(c.pairwise(k), c.stream(), 0, nullptr, Q, out, c.L(), on_device);
+  });
+}
+int dcrf_crf_apply_upper_minus_lower_bf(dcrf_crf_t crf, int k, const int* ind, float* out, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && ind && out, "dcrf_crf_apply_upper_minus_lower_bf: bad arguments");
+    CRF& c = *crf->crf;
+    DCRF_REQUIRE(k >= 0 && k < c.num_pairwise(), "dcrf_crf_apply_upper_minus_lower_bf: no such potential");
+    apply_bf_common(c.pairwise(k), c.stream(), 1, ind, nullptr, out, c.L(), on_device);
+  });
+}
+int dcrf_crf_apply_upper_minus_lower_bf_ord(dcrf_crf_t crf, int k, const int* ind, const float* Q, float* out,
+                                            int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && ind && Q && out, "dcrf_crf_apply_upper_minus_lower_bf_ord: bad arguments");
+    CRF& c = *crf->crf;
+    DCRF_REQUIRE(k >= 0 && k < c.num_pairwise(), "dcrf_crf_apply_upper_minus_lower_bf_ord: no such potential");
+    apply_bf_common(c.pairwise(k), c.stream(), 2, ind, Q, out, c.L(), on_device);
   });
 }
 int dcrf_crf_unary_init(dcrf_crf_t crf, float* Q, int on_device) {

@@ -87,6 +87,8 @@ void potential_init(Potential& p, const float* d_feat, int N, int d, float w, in
   auto lat = std::make_unique<Lattice>(s);
   lat->build(d_feat, N, d);
   potential_init_lattice(p, std::move(lat), w, ntype, s);
+  p.feat.resize((size_t)N * d);
+  DCRF_HIP(hipMemcpyAsync(p.feat.get(), d_feat, (size_t)N * d * sizeof(float), hipMemcpyDeviceToDevice, s));
 }
 
 void potential_init_lattice(Potential& p, std::unique_ptr<Lattice> lat, float w, int ntype, hipStream_t s) {

@@ -35,6 +35,7 @@ struct Potential {
   std::vector<float> cparam;
   DBuf<float> cdev;
   DBuf<float> ctmp;  // N x L filter output before the compatibility pass
+  DBuf<float> feat;  // N x d features (brute-force filters), when built from features
   // the NO_NORMALIZATION copy used by the true energies is always
   // PottsCompatibility(parameters()(0)), taken when the term is added
   // (src/densecrf.cpp:161-168)
@@ -73,6 +74,14 @@ void potential_apply_ord(Potential& p, const float* q01, const float* d_zero_min
 // pI and the [0,1] ranges
 void potential_apply_ord_restricted(Potential& p, float* rout, const float* rq, const int* pI, int nI,
                                     const float* ext, int V, bool store);
+// PairwisePotential::apply_bf (src/pairwise.cpp:368-383): dense Gaussian of
+// the features, then the compatibility; Q, out N x L
+void potential_apply_bf(Potential& p, const float* Q, float* out, int L);
+// apply_upper_minus_lower_bf (ord = false, src/pairwise.cpp:294-320) and
+// apply_upper_minus_lower_bf_ord (ord = true, :321-353); ind is N x L (the
+// pixel at sorted position c of label l is ind[c*L + l], a valid index: the
+// caller checks), Q is read only when ord; out N x L in sorted order
+void potential_apply_uml_bf(Potential& p, const int* ind, const float* Q, float* out, int L, bool ord);
 // LabelCompatibility::apply of a Diagonal or Matrix compatibility on nrows
 // label-contiguous rows: out = C in (the Potts form is a scale by -w)
 void compat_apply_rows(Potential& p, const float* in, float* out, long nrows, int V, hipStream_t s);

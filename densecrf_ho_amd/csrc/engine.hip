@@ -5,6 +5,7 @@
 #include "restricted.hip"
 #include "ho.hip"
 #include "lp.hip"
+#include "bf.hip"
 #include "adjoint.hip"
 #include "batch.hip"
 #include "capi.hip"

@@ -115,6 +115,20 @@ int dcrf_pairwise_apply_upper_minus_lower_ord(dcrf_pairwise_t pw, const float* Q
  * include/pairwise.h:78-79, src/pairwise.cpp:229-239: rout (L x nI) = 0, the restricted
  * ordered filter (as dcrf_lattice_compute_upper_minus_lower_ord_restricted), then the
  * compatibility (Potts: -w). */
+/* PairwisePotential::apply_bf(out, Q), include/pairwise.h:73, src/pairwise.cpp:368-383:
+ * the dense Gaussian exp(-|f_c - f_b|^2) over all pixel pairs (no lattice, no
+ * normalisation), then the compatibility.  O(N^2 L): for small inputs. */
+int dcrf_pairwise_apply_bf(dcrf_pairwise_t pw, const float* Q, float* out, int L, int on_device);
+/* PairwisePotential::apply_upper_minus_lower_bf(out, ind), include/pairwise.h:76,
+ * src/pairwise.cpp:294-320: per label l, pixels taken in the order ind (L x N, entry
+ * (l, c) at ind[c*L + l]); out(l, c) = sum_{b>c} K - sum_{b<c} K, left in that sorted
+ * order; then the compatibility. */
+int dcrf_pairwise_apply_upper_minus_lower_bf(dcrf_pairwise_t pw, const int* ind, float* out, int L, int on_device);
+/* PairwisePotential::apply_upper_minus_lower_bf_ord(out, ind, Q), include/pairwise.h:77,
+ * src/pairwise.cpp:321-353: as above with the split by value, out(l, c) =
+ * sum_{Q_b <= Q_c} K - sum_{Q_b >= Q_c} K. */
+int dcrf_pairwise_apply_upper_minus_lower_bf_ord(dcrf_pairwise_t pw, const int* ind, const float* Q, float* out,
+                                                 int L, int on_device);
 int dcrf_pairwise_apply_upper_minus_lower_ord_restricted(dcrf_pairwise_t pw, float* rout, const float* rQ,
                                                          const int* pI, int nI, const float* Q, int L, int store,
                                                          int on_device);
@@ -170,6 +184,13 @@ int dcrf_crf_apply_upper_minus_lower_ord(dcrf_crf_t crf, int k, const float* Q, 
 /* PairwisePotential::apply_upper_minus_lower_ord_restricted of potential k,
  * src/pairwise.cpp:229-239, with that potential's compatibility (Potts, Diagonal or
  * Matrix); rout is L x nI, Q is L x N. */
+/* The brute-force filters of potential k (apply_bf, apply_upper_minus_lower_bf,
+ * apply_upper_minus_lower_bf_ord, src/pairwise.cpp:294-383) with its compatibility;
+ * layouts as the dcrf_pairwise_ forms, L x N fields. */
+int dcrf_crf_apply_bf(dcrf_crf_t crf, int k, const float* Q, float* out, int on_device);
+int dcrf_crf_apply_upper_minus_lower_bf(dcrf_crf_t crf, int k, const int* ind, float* out, int on_device);
+int dcrf_crf_apply_upper_minus_lower_bf_ord(dcrf_crf_t crf, int k, const int* ind, const float* Q, float* out,
+                                            int on_device);
 int dcrf_crf_apply_upper_minus_lower_ord_restricted(dcrf_crf_t crf, int k, float* rout, const float* rQ,
                                                     const int* pI, int nI, const float* Q, int store, int on_device);
 /* DenseCRF::unary_init(), src/densecrf.cpp:321-325 */

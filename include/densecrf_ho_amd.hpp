@@ -30,31 +30,35 @@ namespace dcrf_ho {
 
 #ifdef DCRF_USE_EIGEN
 using MatrixXf = Eigen::MatrixXf;
+using MatrixXi = Eigen::MatrixXi;
 using VectorXs = Eigen::Matrix<short, Eigen::Dynamic, 1>;
 #else
-// column-major float matrix (the subset of Eigen::MatrixXf the API uses)
-class MatrixXf {
+// column-major matrix (the subset of Eigen::MatrixXf / MatrixXi the API uses)
+template <class T>
+class DenseMatrix {
  public:
-  MatrixXf() = default;
-  MatrixXf(long r, long c) : r_(r), c_(c), v_((size_t)r * c, 0.f) {}
+  DenseMatrix() = default;
+  DenseMatrix(long r, long c) : r_(r), c_(c), v_((size_t)r * c, T(0)) {}
   long rows() const { return r_; }
   long cols() const { return c_; }
   long size() const { return r_ * c_; }
-  float* data() { return v_.data(); }
-  const float* data() const { return v_.data(); }
+  T* data() { return v_.data(); }
+  const T* data() const { return v_.data(); }
   void resize(long r, long c) {
     r_ = r;
     c_ = c;
-    v_.assign((size_t)r * c, 0.f);
+    v_.assign((size_t)r * c, T(0));
   }
-  float& operator()(long i, long j) { return v_[(size_t)j * r_ + i]; }
-  float operator()(long i, long j) const { return v_[(size_t)j * r_ + i]; }
-  void fill(float x) { std::fill(v_.begin(), v_.end(), x); }
+  T& operator()(long i, long j) { return v_[(size_t)j * r_ + i]; }
+  T operator()(long i, long j) const { return v_[(size_t)j * r_ + i]; }
+  void fill(T x) { std::fill(v_.begin(), v_.end(), x); }
 
  private:
   long r_ = 0, c_ = 0;
-  std::vector<float> v_;
+  std::vector<T> v_;
 };
+using MatrixXf = DenseMatrix<float>;
+using MatrixXi = DenseMatrix<int>;
 class VectorXs {
  public:
   VectorXs() = default;
@@ -265,6 +269,18 @@ class PairwisePotential {
     check(dcrf_pairwise_apply_upper_minus_lower_ord_restricted(h_, rout.data(), rQ.data(), pI.data(), (int)pI.size(),
                                                                store ? Q.data() : nullptr, (int)rQ.rows(),
                                                                store ? 1 : 0, 0));
+  }
+  void apply_bf(MatrixXf& out, const MatrixXf& Q) const {
+    if (out.rows() != Q.rows() || out.cols() != Q.cols()) out.resize(Q.rows(), Q.cols());
+    check(dcrf_pairwise_apply_bf(h_, Q.data(), out.data(), (int)Q.rows(), 0));
+  }
+  void apply_upper_minus_lower_bf(MatrixXf& out, const MatrixXi& ind) const {
+    if (out.rows() != ind.rows() || out.cols() != ind.cols()) out.resize(ind.rows(), ind.cols());
+    check(dcrf_pairwise_apply_upper_minus_lower_bf(h_, ind.data(), out.data(), (int)ind.rows(), 0));
+  }
+  void apply_upper_minus_lower_bf_ord(MatrixXf& out, const MatrixXi& ind, const MatrixXf& Q) const {
+    if (out.rows() != Q.rows() || out.cols() != Q.cols()) out.resize(Q.rows(), Q.cols());
+    check(dcrf_pairwise_apply_upper_minus_lower_bf_ord(h_, ind.data(), Q.data(), out.data(), (int)Q.rows(), 0));
   }
   MatrixXf features() const { return f_; }
   KernelType ktype() const { return ktype_; }

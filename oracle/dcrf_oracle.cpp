@@ -578,6 +578,76 @@ struct Restricted {
   }
 };
 
+// Brute-force filters of PairwisePotential (before the compatibility):
+// apply_bf (src/pairwise.cpp:368-383) and apply_upper_minus_lower_bf /
+// _bf_ord (:294-353).  K(c, b) = exp(-|f_c - f_b|^2): the squared norm of the
+// float difference vector in Eigen 3.2.7's SSE2 reduction order for a fresh
+// (aligned) VectorXf (include/Eigen/src/Core/Redux.h:193-245, predux of
+// arch/SSE/PacketMath.h:423-427), exp in double (the C library's exp of the
+// promoted value), sums in float with the double term added before rounding.
+static float sq_norm_eigen(const float* t, int n) {
+  if (n < 4) {
+    float r = t[0];
+    for (int i = 1; i < n; ++i) r = r + t[i];
+    return r;
+  }
+  const int a1 = n / 4 * 4, a2 = n / 8 * 8;
+  float p0[4] = {t[0], t[1], t[2], t[3]};
+  if (a1 > 4) {
+    float p1[4] = {t[4], t[5], t[6], t[7]};
+    for (int i = 8; i < a2; i += 8)
+      for (int q = 0; q < 4; ++q) {
+        p0[q] = p0[q] + t[i + q];
+        p1[q] = p1[q] + t[i + 4 + q];
+      }
+    for (int q = 0; q < 4; ++q) p0[q] = p0[q] + p1[q];
+    if (a1 > a2)
+      for (int q = 0; q < 4; ++q) p0[q] = p0[q] + t[a2 + q];
+  }
+  float r = (p0[0] + p0[2]) + (p0[1] + p0[3]);
+  for (int i = a1; i < n; ++i) r = r + t[i];
+  return r;
+}
+static double bf_kernel(const float* a, const float* b, int d) {
+  float t[64];
+  for (int k = 0; k < d; ++k) {
+    const float x = a[k] - b[k];
+    t[k] = x * x;
+  }
+  return ::exp((double)-sq_norm_eigen(t, d));
+}
+static void bf_apply(const float* f, int N, int d, const float* Q, int L, float* out) {
+  std::fill(out, out + (size_t)N * L, 0.f);
+  for (int l = 0; l < L; ++l)
+    for (int c = 0; c < N; ++c)
+      for (int b = 0; b < N; ++b)
+        out[(size_t)c * L + l] = (float)((double)out[(size_t)c * L + l] +
+                                         bf_kernel(&f[(size_t)c * d], &f[(size_t)b * d], d) * Q[(size_t)b * L + l]);
+}
+// ord: split by value Q(l, ind(l, .)) instead of position; out stays sorted
+static void bf_uml(const float* f, int N, int d, const int* ind, const float* Q, int L, float* out) {
+  for (int l = 0; l < L; ++l)
+    for (int c = 0; c < N; ++c) {
+      const int pc = ind[(size_t)c * L + l];
+      float acc = 0.f;
+      for (int pass = 0; pass < 2; ++pass)
+        for (int b = 0; b < N; ++b) {
+          const int pb = ind[(size_t)b * L + l];
+          bool take;
+          if (Q) {
+            const float qb = Q[(size_t)pb * L + l], qc = Q[(size_t)pc * L + l];
+            take = pass == 0 ? !(qb > qc) : !(qb < qc);
+          } else {
+            take = pass == 0 ? b > c : b < c;
+          }
+          if (!take) continue;
+          const double K = bf_kernel(&f[(size_t)pc * d], &f[(size_t)pb * d], d);
+          acc = (float)(pass == 0 ? (double)acc + K : (double)acc - K);
+        }
+      out[(size_t)c * L + l] = acc;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Dense CRF with Potts pairwise terms and Pn superpixel layers.
 // ---------------------------------------------------------------------------
@@ -1361,6 +1431,10 @@ void orc_filter_d(void* h, const double* in, double* out, int V, int reverse) {
 }
 void orc_ordered_f(void* h, const float* q, float* out, int V) { ((Lattice*)h)->ordered<float>(q, out, V); }
 void orc_ordered_d(void* h, const float* q, double* out, int V) { ((Lattice*)h)->ordered<double>(q, out, V); }
+void orc_bf_apply(const float* f, int N, int d, const float* Q, int L, float* out) { orc::bf_apply(f, N, d, Q, L, out); }
+void orc_bf_uml(const float* f, int N, int d, const int* ind, const float* Q, int L, float* out) {
+  orc::bf_uml(f, N, d, ind, Q, L, out);
+}
 // restricted ordered filter; its stored state lives beside the lattice handle
 int orc_ordered_restricted(void* h, const float* rin, const int* pI, int nI, const float* ext, int V, int store,
                            float* out) {

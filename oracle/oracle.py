@@ -60,6 +60,8 @@ def lib():
         _sig(L, "orc_filter_d", None, _vp, _f64p, _f64p, _i, _i)
         _sig(L, "orc_ordered_f", None, _vp, _f32p, _f32p, _i)
         _sig(L, "orc_ordered_d", None, _vp, _f32p, _f64p, _i)
+        _sig(L, "orc_bf_apply", None, _f32p, _i, _i, _f32p, _i, _f32p)
+        _sig(L, "orc_bf_uml", None, _f32p, _i, _i, _i32p, _vp, _i, _f32p)
         _sig(L, "orc_ordered_restricted", _i, _vp, _f32p, _i32p, _i, _f32p, _i, _i, _f32p)
         _sig(L, "orc_crf_new", _vp, _i, _i)
         _sig(L, "orc_crf_free", None, _vp)
@@ -158,6 +160,27 @@ class Lattice:
         if self._L.orc_ordered_restricted(self.h, rq, pI, len(pI), q, V, int(store), out) != 0:
             raise RuntimeError(self._L.orc_last_error().decode())
         return out
+
+
+def bf_apply(feat, Q):
+    """PairwisePotential::apply_bf before the compatibility (src/pairwise.cpp:368-383)"""
+    feat = _c(feat, np.float32)
+    Q = _c(Q, np.float32)
+    out = np.zeros_like(Q)
+    lib().orc_bf_apply(feat, feat.shape[0], feat.shape[1], Q, Q.shape[1], out)
+    return out
+
+
+def bf_uml(feat, ind, Q=None):
+    """apply_upper_minus_lower_bf (Q None) / _bf_ord before the compatibility
+    (src/pairwise.cpp:294-353); ind (N, L); the result in sorted order"""
+    feat = _c(feat, np.float32)
+    ind = _c(ind, np.int32)
+    out = np.zeros(ind.shape, np.float32)
+    q = None if Q is None else _c(Q, np.float32)
+    lib().orc_bf_uml(feat, feat.shape[0], feat.shape[1], ind, None if q is None else q.ctypes.data_as(C.c_void_p),
+                     ind.shape[1], out)
+    return out
 
 
 class LPParams:
@@ -392,6 +415,7 @@ def reflib():
         _sig(R, "ref_lattice_get", None, _vp, _i32p, _i32p, _f32p, _i32p, _i)
         _sig(R, "ref_lattice_compute", None, _vp, _f32p, _f32p, _i, _i, _i)
         _sig(R, "ref_lattice_ordered", None, _vp, _f32p, _f32p, _i, _i)
+        _sig(R, "ref_crf_apply_bf", None, _vp, _i, _i, _vp, _vp, _f32p)
         _sig(R, "ref_lattice_ordered_restricted", _i, _vp, _f32p, _i32p, _i, _f32p, _i, _i, _i, _f32p)
         _sig(R, "ref_crf_new", _vp, _i, _i, _i)
         _sig(R, "ref_crf_free", None, _vp)
@@ -526,6 +550,15 @@ class RefCRF:
         x = _c(x, np.float32)
         out = np.zeros_like(x)
         self._R.ref_crf_apply(self.h, k, x, out, which)
+        return out
+
+    def apply_bf(self, k, which, ind=None, Q=None):
+        """0 apply_bf(Q), 1 apply_upper_minus_lower_bf(ind), 2 _bf_ord(ind, Q)"""
+        out = np.zeros((self.N, self.L), np.float32)
+        ind = None if ind is None else _c(ind, np.int32)
+        Q = None if Q is None else _c(Q, np.float32)
+        p = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)  # noqa: E731
+        self._R.ref_crf_apply_bf(self.h, k, which, p(ind), p(Q), out)
         return out
 
     def apply_ord(self, k, q):

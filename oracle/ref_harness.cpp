@@ -176,6 +176,20 @@ void ref_crf_apply(void* h, int k, const float* in, float* out, int which) {
   else c->nn(k)->apply(o, i);
   std::copy(o.data(), o.data() + o.size(), out);
 }
+// brute-force filters of potential k: 0 apply_bf(Q), 1 apply_upper_minus_lower_bf(ind),
+// 2 apply_upper_minus_lower_bf_ord(ind, Q); ind is L x N column-major
+void ref_crf_apply_bf(void* h, int k, int which, const int* ind, const float* q, float* out) {
+  RefCRF* c = (RefCRF*)h;
+  const int L = c->L(), N = c->N();
+  MatrixXf o(L, N), Q = q ? as_mat(q, L, N) : MatrixXf::Zero(L, N);
+  MatrixXi I(L, N);
+  if (ind) std::copy(ind, ind + (size_t)L * N, I.data());
+  o.fill(0);
+  if (which == 0) c->pw(k)->apply_bf(o, Q);
+  else if (which == 1) c->pw(k)->apply_upper_minus_lower_bf(o, I);
+  else c->pw(k)->apply_upper_minus_lower_bf_ord(o, I, Q);
+  std::copy(o.data(), o.data() + o.size(), out);
+}
 int ref_crf_apply_ord(void* h, int k, const float* q, float* out) {
   RefCRF* c = (RefCRF*)h;
   MatrixXf i = as_mat(q, c->L(), c->N()), o(c->L(), c->N());

@@ -8,6 +8,7 @@ compressed .npz files.  Run from the repository root:
     python tests/golden/make_golden.py          # every fixture
     python tests/golden/make_golden.py lp_lc    # only lp_lc_31x23_L5.npz
     python tests/golden/make_golden.py restricted  # only ordered_restricted_31x23.npz
+    python tests/golden/make_golden.py bf       # only bf_12x9_L4.npz
 """
 
 from __future__ import annotations
@@ -252,7 +253,38 @@ def restricted_case():
     print("ordered_restricted_31x23")
 
 
+def bf_case():
+    """brute-force filters of the reference's PairwisePotential
+    (src/pairwise.cpp:294-383) on a 12x9 image, 4 labels, both kernels:
+    apply_bf(Q), apply_upper_minus_lower_bf(ind) with random orders, and
+    apply_upper_minus_lower_bf_ord(ind, Q) with random orders and with the
+    ascending order of Q (which has ties)"""
+    W, H, L = 12, 9, 4
+    wl = S.Workload(W, H, 3, 1, L=L)
+    r = O.RefCRF(W, H, L)
+    r.set_unary(wl.U)
+    r.add_gaussian(3, 3, 3)
+    r.add_bilateral(50, 50, 15, 15, 15, wl.img, 5)
+    rng = np.random.default_rng(12)
+    Q = rng.random((wl.N, L)).astype(np.float32)
+    Q[5:9] = Q[1]  # equal values: ties in the value split
+    ind = np.stack([rng.permutation(wl.N) for _ in range(L)], 1).astype(np.int32)
+    ind_q = np.argsort(Q, axis=0, kind="stable").astype(np.int32)
+    d = dict(W=np.int32(W), H=np.int32(H), L=np.int32(L), U=wl.U, img=wl.img, Q=Q, ind=ind, ind_q=ind_q)
+    for k in range(2):
+        d[f"feat{k}"] = r.features(k, 2 if k == 0 else 5)
+        d[f"bf{k}"] = r.apply_bf(k, 0, Q=Q)
+        d[f"uml{k}"] = r.apply_bf(k, 1, ind=ind)
+        d[f"umlord{k}"] = r.apply_bf(k, 2, ind=ind, Q=Q)
+        d[f"umlord_q{k}"] = r.apply_bf(k, 2, ind=ind_q, Q=Q)
+    np.savez_compressed(os.path.join(OUT, "bf_12x9_L4.npz"), **d)
+    print("bf_12x9_L4")
+
+
 def main():
+    if sys.argv[1:] == ["bf"]:
+        bf_case()
+        return
     if sys.argv[1:] == ["lp_lc"]:
         lp_less_confident_case()
         return
@@ -276,6 +308,7 @@ def main():
     compat_case("31x23_L5", 31, 23, 8, 1, 5)
     lp_less_confident_case()
     restricted_case()
+    bf_case()
     c0_case()
 
 

@@ -173,3 +173,49 @@ def test_restricted_ordered_potential_vs_oracle(golden, compat):
     ref2 = np.float32(-5.0) * r2 if compat == "potts" else \
         (r2.astype(np.float64) @ Wm.astype(np.float64).T).astype(np.float32)
     assert rel_err(out2, ref2) < 1e-5
+
+
+def _bf_crf(g):
+    import densecrf_ho_amd as D
+    W, H, L = int(g["W"]), int(g["H"]), int(g["L"])
+    c = D.DenseCRF2D(W, H, L)
+    c.setUnaryEnergy(g["U"])
+    c.addPairwiseEnergy(g["feat0"], D.PottsCompatibility(3.0))
+    c.addPairwiseEnergy(g["feat1"], D.PottsCompatibility(5.0))
+    return c
+
+
+@pytest.mark.parametrize("k", [0, 1])
+def test_brute_force_filters_vs_reference(golden, k):
+    """apply_bf / apply_upper_minus_lower_bf / _bf_ord of potential k against
+    the compiled reference's outputs (within 1e-5 of the largest value)"""
+    g = golden("bf_12x9_L4.npz")
+    c = _bf_crf(g)
+    print("potential", k, flush=True)
+    assert rel_err(c.apply_bf(k, g["Q"]), g[f"bf{k}"]) < 1e-5
+    assert rel_err(c.apply_upper_minus_lower_bf(k, g["ind"]), g[f"uml{k}"]) < 1e-5
+    assert rel_err(c.apply_upper_minus_lower_bf_ord(k, g["ind"], g["Q"]), g[f"umlord{k}"]) < 1e-5
+    assert rel_err(c.apply_upper_minus_lower_bf_ord(k, g["ind_q"], g["Q"]), g[f"umlord_q{k}"]) < 1e-5
+
+
+def test_brute_force_filters_vs_oracle_larger():
+    """40x30 image, 21 labels, bilateral kernel: GPU brute-force filters
+    against the oracle (every sum over 1200 pixels)"""
+    import densecrf_ho_amd as D
+    from densecrf_ho_amd import synthetic as S
+    wl = S.Workload(40, 30, 6, 3, L=21)
+    c = D.DenseCRF2D(40, 30, 21)
+    c.setUnaryEnergy(wl.U)
+    f = wl.kernels[1][0]
+    c.addPairwiseEnergy(f, D.PottsCompatibility(5.0))
+    rng = np.random.default_rng(8)
+    Q = rng.random((wl.N, 21)).astype(np.float32)
+    ind = np.argsort(Q, axis=0).astype(np.int32)
+    w = np.float32(-5.0)
+    assert rel_err(c.apply_bf(0, Q), w * O.bf_apply(f, Q)) < 1e-5
+    assert rel_err(c.apply_upper_minus_lower_bf(0, ind), w * O.bf_uml(f, ind)) < 1e-5
+    assert rel_err(c.apply_upper_minus_lower_bf_ord(0, ind, Q), w * O.bf_uml(f, ind, Q)) < 1e-5
+    bad = ind.copy()
+    bad[3, 2] = wl.N
+    with pytest.raises(D.EngineError, match="out of range"):
+        c.apply_upper_minus_lower_bf(0, bad)

@@ -357,6 +357,38 @@ def test_restricted_ordered_with_every_pixel_is_the_ordered_filter(golden):
     np.testing.assert_array_equal(lat.ordered_restricted(q, pI, q, True), lat.ordered(q))
 
 
+@pytest.mark.parametrize("k", [0, 1])
+def test_brute_force_filters_match_reference(golden, k):
+    """apply_bf, apply_upper_minus_lower_bf and _bf_ord (src/pairwise.cpp:294-383)
+    before the Potts weight: bit for bit against the compiled reference"""
+    g = golden("bf_12x9_L4.npz")
+    w = np.float32(3.0 if k == 0 else 5.0)
+    f = g[f"feat{k}"]
+    np.testing.assert_array_equal(-w * O.bf_apply(f, g["Q"]), g[f"bf{k}"])
+    np.testing.assert_array_equal(-w * O.bf_uml(f, g["ind"]), g[f"uml{k}"])
+    np.testing.assert_array_equal(-w * O.bf_uml(f, g["ind"], g["Q"]), g[f"umlord{k}"])
+    np.testing.assert_array_equal(-w * O.bf_uml(f, g["ind_q"], g["Q"]), g[f"umlord_q{k}"])
+
+
+def test_brute_force_filters_closed_forms(golden):
+    """apply_bf is K Q with K = exp(-|f_c - f_b|^2) (FP64 dense product); in
+    the ascending order of distinct values the value split equals the
+    position split with the opposite sign (upper_bf_ord = -upper_bf); a
+    permutation of one label's order permutes its sorted output"""
+    g = golden("bf_12x9_L4.npz")
+    f = g["feat1"].astype(np.float64)
+    K = np.exp(-((f[:, None, :] - f[None, :, :]) ** 2).sum(-1))
+    np.testing.assert_allclose(O.bf_apply(g["feat1"], g["Q"]), K @ g["Q"].astype(np.float64), rtol=1e-5)
+    rng = np.random.default_rng(3)
+    Q = rng.random(g["Q"].shape).astype(np.float32)  # distinct values
+    ind = np.argsort(Q, axis=0).astype(np.int32)
+    a, b = O.bf_uml(g["feat1"], ind, Q), O.bf_uml(g["feat1"], ind)
+    assert np.abs(a + b).max() <= 1e-5 * np.abs(b).max()
+    Ks = np.stack([K[np.ix_(ind[:, l], ind[:, l])] for l in range(ind.shape[1])])
+    up = np.stack([np.triu(Ks[l], 1).sum(1) - np.tril(Ks[l], -1).sum(1) for l in range(ind.shape[1])], 1)
+    np.testing.assert_allclose(b, up, rtol=1e-5, atol=1e-5 * np.abs(up).max())
+
+
 def test_lp_less_confident_stop_matches_reference(golden):
     """less_confident_percent / confidence_tol (src/densecrf.cpp:457-462,
     784-794): four settings stop after 4 (no check), 1, 2 and 3 outer steps;
