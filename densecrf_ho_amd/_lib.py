@@ -100,6 +100,10 @@ SIGNATURES = {
     "dcrf_batch_qp": (_i, [_vp, _vp, _vp, _i, _i, _i, _vp]),
     "dcrf_batch_lp": (_i, [_vp, _vp, _vp, C.POINTER(LPParams), _i, _vp, _i]),
     "dcrf_batch_assignment_energy_true": (_i, [_vp, _vp, _i, _vp]),
+    "dcrf_io_load_unary": (_i, [C.c_char_p, _vp, C.c_size_t, _ip, _ip, _ip]),
+    "dcrf_io_load_image": (_i, [C.c_char_p, _vp, C.c_size_t, _ip, _ip]),
+    "dcrf_io_save_map": (_i, [_vp, _i, _i, _i, C.c_char_p]),
+    "dcrf_superpixel_terms": (_i, [_vp, _i, _vp, _i, _f, _f, _vp, _vp]),
     "dcrf_malloc": (_i, [C.POINTER(_vp), C.c_size_t]),
     "dcrf_free": (_i, [_vp]),
     "dcrf_memcpy": (_i, [_vp, _vp, C.c_size_t, _i]),

@@ -214,6 +214,48 @@ class MatrixCompatibility:
         return np.ascontiguousarray(self.m.T).reshape(-1)  # column-major, as Eigen stores it
 
 
+def load_unary(path: str):
+    """load_unary (src/file_storage.cpp:262-291): the .c_unary file as -log
+    probabilities; returns (U (N, L) float32, W, H)."""
+    L = _lib.load()
+    W, H, M = C.c_int(), C.c_int(), C.c_int()
+    _check(L.dcrf_io_load_unary(path.encode(), None, 0, C.byref(W), C.byref(H), C.byref(M)))
+    U = np.empty((W.value * H.value, M.value), np.float32)
+    _check(L.dcrf_io_load_unary(path.encode(), U.ctypes.data_as(C.c_void_p), U.size, C.byref(W), C.byref(H),
+                                C.byref(M)))
+    return U, W.value, H.value
+
+
+def load_image(path: str):
+    """load_image (src/file_storage.cpp:197-222) for uncompressed BMP: returns
+    (RGB uint8 (N*3,), W, H), pixel (x, y) at (x + y W) * 3."""
+    L = _lib.load()
+    W, H = C.c_int(), C.c_int()
+    _check(L.dcrf_io_load_image(path.encode(), None, 0, C.byref(W), C.byref(H)))
+    im = np.empty(W.value * H.value * 3, np.uint8)
+    _check(L.dcrf_io_load_image(path.encode(), im.ctypes.data_as(C.c_void_p), im.size, C.byref(W), C.byref(H)))
+    return im, W.value, H.value
+
+
+def save_map(Q, W: int, H: int, path: str):
+    """save_map(Q, size, path, "MSRC"): argmax labels in the MSRC colours (PNG, or PPM)."""
+    q = np.ascontiguousarray(Q, np.float32)
+    _check(_lib.load().dcrf_io_save_map(q.ctypes.data_as(C.c_void_p), W, H, q.shape[1], path.encode()))
+
+
+def superpixel_terms(img, region, constant: float = 50.0, normaliser: float = 1000.0):
+    """the per-region terms of DenseCRF2D::addSuperPixel for a given
+    segmentation: (exp_of_superpixels (R,), constants (R,))"""
+    im = np.ascontiguousarray(img, np.uint8).ravel()
+    reg = np.ascontiguousarray(region, np.int32).ravel()
+    R = int(reg.max()) + 1
+    e = np.zeros(R, np.float32)
+    c = np.zeros(R, np.float32)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    _check(_lib.load().dcrf_superpixel_terms(p(im), len(reg), p(reg), R, constant, normaliser, p(e), p(c)))
+    return e, c
+
+
 class Permutohedral:
     """Permutohedral lattice (include/permutohedral.h:50-88)."""
 

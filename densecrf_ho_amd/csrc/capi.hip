@@ -4,6 +4,7 @@
 #include <string>
 
 #include "../../include/densecrf_ho_amd.h"
+#include "../../include/densecrf_ho_amd_io.hpp"
 #include "batch.hpp"
 #include "tuner.hpp"
 #include "crf.hpp"
@@ -968,6 +969,51 @@ int dcrf_profile_report(char* buf, int len) {
     buf[n] = 0;
   }
   return (int)r.size() + 1;
+}
+
+// ---- file formats (host only) ------------------------------------------------------
+int dcrf_io_load_unary(const char* path, float* out, size_t cap, int* W, int* H, int* L) {
+  return guarded([&] {
+    DCRF_REQUIRE(path && W && H && L, "dcrf_io_load_unary: bad arguments");
+    dcrf_ho::img_size sz{-1, -1};
+    dcrf_ho::MatrixXf u = dcrf_ho::load_unary(path, sz);
+    *W = sz.width;
+    *H = sz.height;
+    *L = (int)u.rows();
+    if (out) {
+      DCRF_REQUIRE(cap >= (size_t)u.size(), "dcrf_io_load_unary: output too small");
+      std::memcpy(out, u.data(), (size_t)u.size() * sizeof(float));
+    }
+  });
+}
+int dcrf_io_load_image(const char* path, unsigned char* out, size_t cap, int* W, int* H) {
+  return guarded([&] {
+    DCRF_REQUIRE(path && W && H, "dcrf_io_load_image: bad arguments");
+    dcrf_ho::img_size sz{-1, -1};
+    std::unique_ptr<unsigned char[]> im(dcrf_ho::load_image(path, sz));
+    *W = sz.width;
+    *H = sz.height;
+    if (out) {
+      DCRF_REQUIRE(cap >= (size_t)sz.width * sz.height * 3, "dcrf_io_load_image: output too small");
+      std::memcpy(out, im.get(), (size_t)sz.width * sz.height * 3);
+    }
+  });
+}
+int dcrf_io_save_map(const float* Q, int W, int H, int L, const char* path) {
+  return guarded([&] {
+    DCRF_REQUIRE(Q && path && W > 0 && H > 0 && L > 0, "dcrf_io_save_map: bad arguments");
+    dcrf_ho::MatrixXf q(L, (long)W * H);
+    std::memcpy(q.data(), Q, (size_t)L * W * H * sizeof(float));
+    dcrf_ho::save_map(q, dcrf_ho::img_size{W, H}, path, "MSRC");
+  });
+}
+
+int dcrf_superpixel_terms(const unsigned char* img, int N, const int* region, int R, float constant, float normaliser,
+                          float* e, float* c) {
+  return guarded([&] {
+    DCRF_REQUIRE(img && region && e && c && N > 0 && R > 0, "dcrf_superpixel_terms: bad arguments");
+    dcrf_ho::superpixel_terms(img, N, region, R, constant, normaliser, e, c);
+  });
 }
 
 }  // extern "C"

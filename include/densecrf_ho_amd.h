@@ -350,6 +350,27 @@ int dcrf_autotune_enable(int on);
 int dcrf_autotune_report(char* buf, int len);
 int dcrf_profile_report(char* buf, int len);
 
+/* ---- file formats of the reference's example pipeline (host only, no GPU) ---------- */
+/* load_unary (src/file_storage.cpp:262-291 over ProbImage::decompress and boostToProb,
+ * external/ProbImage/probimage.cpp:64-78, 140-165): the .c_unary file as -log
+ * probabilities, L x N label-contiguous.  out may be NULL to query W, H, L; cap is the
+ * number of floats out holds. */
+int dcrf_io_load_unary(const char* path, float* out, size_t cap, int* W, int* H, int* L);
+/* load_image (src/file_storage.cpp:197-222): an uncompressed 24 / 32 bpp BMP as RGB,
+ * pixel (x, y) at (x + y W) * 3.  out may be NULL to query W and H. */
+int dcrf_io_load_image(const char* path, unsigned char* out, size_t cap, int* W, int* H);
+/* save_map(Q, size, path, "MSRC") (src/file_storage.cpp:332-380): the argmax labelling
+ * in the MSRC colours, PNG (or binary PPM for a ".ppm" path). */
+int dcrf_io_save_map(const float* Q, int W, int H, int L, const char* path);
+
+/* The per-region terms of DenseCRF2D::addSuperPixel (src/densecrf.cpp:193-251) for a
+ * segmentation given as region ids in [0, R) (the reference computes it by mean-shift):
+ * e[r] = constant * exp(-sd_r / normaliser), c[r] = constant, with the reference's grey
+ * value read from img[i], img[i+1], img[i+2].  Host only; feed the result to
+ * dcrf_crf_add_superpixel_layer. */
+int dcrf_superpixel_terms(const unsigned char* img, int N, const int* region, int R, float constant, float normaliser,
+                          float* e, float* c);
+
 #ifdef __cplusplus
 }
 #endif

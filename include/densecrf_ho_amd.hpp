@@ -14,6 +14,7 @@
 #pragma once
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -429,6 +430,36 @@ class DenseCRF {
 };
 
 // DenseCRF2D (include/densecrf.h:172-194)
+// The per-region terms of DenseCRF2D::addSuperPixel (src/densecrf.cpp:212-246)
+// for a segmentation given as region ids in [0, R): e[r] = constant *
+// exp(-sd_r / normaliser) with sd_r the summed squared deviation of the grey
+// value from the region mean over the region's size, c[r] = constant.  The
+// grey value of pixel i is read from img[i], img[i+1], img[i+2] (not
+// img[3i..3i+2]), as the reference reads it; float sums of double terms as in
+// the reference.
+inline void superpixel_terms(const unsigned char* img, int N, const int* region, int R, float constant,
+                             float normaliser, float* e, float* c) {
+  std::vector<float> count(R, 0.f), mean(R, 0.f), sd(R, 0.f);
+  for (int i = 0; i < N; i++) {
+    const int r = region[i];
+    if (r < 0 || r >= R) throw std::runtime_error("addSuperPixel: region id out of range");
+    mean[r] = (float)(mean[r] + 0.2989 * (double)img[i]);
+    mean[r] = (float)(mean[r] + 0.5870 * (double)img[i + 1]);
+    mean[r] = (float)(mean[r] + 0.1140 * (double)img[i + 2]);
+    count[r] += 1;
+  }
+  for (int r = 0; r < R; ++r) mean[r] = mean[r] / count[r];
+  for (int i = 0; i < N; i++) {
+    const int r = region[i];
+    const double g = 0.2989 * (double)img[i] + 0.5870 * (double)img[i + 1] + 0.1140 * (double)img[i + 2];
+    sd[r] = (float)(sd[r] + (mean[r] - g) * (mean[r] - g));
+  }
+  for (int r = 0; r < R; ++r) {
+    e[r] = constant * std::exp(-1 * (sd[r] / (normaliser * count[r])));
+    c[r] = constant;
+  }
+}
+
 class DenseCRF2D : public DenseCRF {
  public:
   DenseCRF2D(int W, int H, int M) : DenseCRF(W * H, M), W_(W), H_(H) {}
@@ -468,6 +499,20 @@ class DenseCRF2D : public DenseCRF {
         feature(4, j * W_ + i) = im[(i + j * W_) * 3 + 2] / sb;
       }
     addPairwiseEnergy(feature, f.release(), kernel_type, normalization_type);
+  }
+  // DenseCRF2D::addSuperPixel (src/densecrf.cpp:193-251) with the mean-shift
+  // segmentation given as its result, a region id per pixel in [0, R): the
+  // layer's exp_of_superpixels_ = constant * exp(-sd / normaliser) from the
+  // grey value of each region, constants_ = constant.  The grey value of pixel
+  // i is read from img[i], img[i+1], img[i+2] (not img[3i..3i+2]), as the
+  // reference reads it.  Accumulation types follow the reference: float
+  // vectors with double terms.
+  void addSuperPixel(const unsigned char* img, const std::vector<int>& region, int R, float constant,
+                     float normaliser) {
+    if ((long)region.size() != (long)W_ * H_) throw std::runtime_error("addSuperPixel: one region id per pixel");
+    std::vector<float> e(R), c(R);
+    superpixel_terms(img, W_ * H_, region.data(), R, constant, normaliser, e.data(), c.data());
+    addSuperPixelLayer(region, e, c);
   }
 
  protected:

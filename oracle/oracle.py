@@ -415,6 +415,9 @@ def reflib():
         _sig(R, "ref_lattice_get", None, _vp, _i32p, _i32p, _f32p, _i32p, _i)
         _sig(R, "ref_lattice_compute", None, _vp, _f32p, _f32p, _i, _i, _i)
         _sig(R, "ref_lattice_ordered", None, _vp, _f32p, _f32p, _i, _i)
+        _sig(R, "ref_probimage", None, C.c_char_p, _vp, _i32p)
+        _sig(R, "ref_crf_add_superpixel", _i, _vp, _vp, _i, _i, _i, _f, _f)
+        _sig(R, "ref_crf_layers", None, _vp, _i32p, _f32p, _f32p)
         _sig(R, "ref_crf_apply_bf", None, _vp, _i, _i, _vp, _vp, _f32p)
         _sig(R, "ref_lattice_ordered_restricted", _i, _vp, _f32p, _i32p, _i, _f32p, _i, _i, _i, _f32p)
         _sig(R, "ref_crf_new", _vp, _i, _i, _i)
@@ -444,6 +447,17 @@ def reflib():
         _sig(R, "ref_renormalize", None, _f32p, _i, _i)
         _ref = R
     return _ref
+
+
+def ref_probimage(path):
+    """ProbImage::decompress + boostToProb of the compiled reference: (N, D) probabilities, W, H"""
+    R = reflib()
+    dims = np.zeros(3, np.int32)
+    R.ref_probimage(path.encode(), None, dims)
+    W, H, D = (int(x) for x in dims)
+    p = np.zeros((W * H, D), np.float32)
+    R.ref_probimage(path.encode(), p.ctypes.data_as(C.c_void_p), dims)
+    return p, W, H
 
 
 class RefLattice:
@@ -551,6 +565,16 @@ class RefCRF:
         out = np.zeros_like(x)
         self._R.ref_crf_apply(self.h, k, x, out, which)
         return out
+
+    def add_superpixel(self, img, sr, rr, minreg, const, norm):
+        """DenseCRF2D::addSuperPixel (mean-shift); returns (region map, e_r, c_r) of all layers"""
+        img = _c(img, np.uint8)
+        R = self._R.ref_crf_add_superpixel(self.h, img.ctypes.data_as(C.c_void_p), sr, rr, minreg, const, norm)
+        reg = np.full(self.N, -1, np.int32)
+        e = np.zeros(R, np.float32)
+        c = np.zeros(R, np.float32)
+        self._R.ref_crf_layers(self.h, reg, e, c)
+        return reg, e, c
 
     def apply_bf(self, k, which, ind=None, Q=None):
         """0 apply_bf(Q), 1 apply_upper_minus_lower_bf(ind), 2 _bf_ord(ind, Q)"""

@@ -15,6 +15,7 @@
 #include "densecrf.h"
 #include "eigen_utils.hpp"
 #include "permutohedral.h"
+#include "probimage.h"
 
 // free functions of src/densecrf.cpp that the header does not declare
 void renormalize(MatrixXf& Q);
@@ -61,6 +62,14 @@ struct RefCRF : public DenseCRF2D {
     }
   }
   void ucond(MatrixXf& Us, const MatrixXf& Q) const { computeUCondGrad(Us, Q); }
+  // region id per pixel (layer order) and exp_of_superpixels_ / constants_
+  void layers(int* region, float* e, float* c) const {
+    for (int r = 0; r < R_; ++r) {
+      for (double m : super_pixel_container_[r]) region[(int)m] = r;
+      if (e) e[r] = exp_of_superpixels_(r);
+      if (c) c[r] = constants_[r];
+    }
+  }
   int R() const { return R_; }
   const MatrixXf& z() const { return z_labels_; }
 };
@@ -118,6 +127,21 @@ int ref_lattice_ordered_restricted(void* h, const float* rq, const int* pI, int 
   return 0;
 }
 
+// ProbImage::decompress then boostToProb (external/ProbImage/probimage.cpp):
+// dims = {width, height, depth}; probs (width*height*depth) may be null
+void ref_probimage(const char* path, float* probs, int* dims) {
+  ProbImage p;
+  p.decompress(path);
+  p.boostToProb();
+  dims[0] = p.width();
+  dims[1] = p.height();
+  dims[2] = p.depth();
+  if (probs)
+    for (int y = 0; y < p.height(); ++y)
+      for (int x = 0; x < p.width(); ++x)
+        for (int k = 0; k < p.depth(); ++k) probs[((size_t)y * p.width() + x) * p.depth() + k] = p(x, y, k);
+}
+
 // ---- DenseCRF --------------------------------------------------------------
 void* ref_crf_new(int W, int H, int L) { return new RefCRF(W, H, L); }
 void ref_crf_free(void* h) { delete (RefCRF*)h; }
@@ -167,6 +191,13 @@ void ref_crf_add_layer(void* h, const int* region, int R, const float* e, const 
   ((RefCRF*)h)->add_layer(region, R, e, c);
 }
 int ref_crf_R(void* h) { return ((RefCRF*)h)->R(); }
+// DenseCRF2D::addSuperPixel (mean-shift segmentation of img), then the layer state
+int ref_crf_add_superpixel(void* h, const unsigned char* img, int sr, int rr, int minreg, float cst, float nrm) {
+  std::vector<unsigned char> im(img, img + (size_t)((RefCRF*)h)->N() * 3);
+  ((RefCRF*)h)->addSuperPixel(im.data(), sr, rr, minreg, cst, nrm);
+  return ((RefCRF*)h)->R();
+}
+void ref_crf_layers(void* h, int* region, float* e, float* c) { ((RefCRF*)h)->layers(region, e, c); }
 // which: 0 apply, 1 applyTranspose, 2 no-normalisation copy's apply
 void ref_crf_apply(void* h, int k, const float* in, float* out, int which) {
   RefCRF* c = (RefCRF*)h;

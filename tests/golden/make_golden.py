@@ -9,6 +9,7 @@ compressed .npz files.  Run from the repository root:
     python tests/golden/make_golden.py lp_lc    # only lp_lc_31x23_L5.npz
     python tests/golden/make_golden.py restricted  # only ordered_restricted_31x23.npz
     python tests/golden/make_golden.py bf       # only bf_12x9_L4.npz
+    python tests/golden/make_golden.py data     # only data_img.json / data_img_labels.npz
 """
 
 from __future__ import annotations
@@ -281,7 +282,56 @@ def bf_case():
     print("bf_12x9_L4")
 
 
+def data_case():
+    """the reference's own example input (data/img.bmp, data/img.c_unary, kept
+    in tests/golden/data/) through its example_inference defaults
+    (src/example_inference.cpp:99-128): the unaries as the compiled
+    ProbImage decodes them (checksum), and the discretised energies and
+    labellings of the unary, qp and lp methods"""
+    import hashlib
+    import densecrf_ho_amd as D
+    src = os.path.join(OUT, "data")
+    p, W, H = O.ref_probimage(os.path.join(src, "img.c_unary"))
+    U = (-np.log(p.astype(np.float64))).astype(np.float32)  # load_unary's -log (src/file_storage.cpp:285)
+    im, Wi, Hi = D.load_image(os.path.join(src, "img.bmp"))
+    assert (Wi, Hi) == (W, H)
+    r = O.RefCRF(W, H, U.shape[1])
+    r.set_unary(U)
+    r.add_gaussian(2, 2, 3)
+    r.add_bilateral(30, 30, 8, 8, 8, im, 2)
+    Q0 = r.unary_init()
+    out = dict(W=W, H=H, L=int(U.shape[1]), unary_sha256=hashlib.sha256(U.tobytes()).hexdigest(),
+               unary_samples=[float(U[i, k]) for i, k in [(0, 0), (1234, 5), (68159, 20)]],
+               image_sha256=hashlib.sha256(im.tobytes()).hexdigest())
+    labs = {}
+    for method, Q in [("unary", Q0), ("qp", r.qp(Q0)), ("lp", r.lp(Q0, O.LPParams()))]:
+        lab = r.current_map(Q)
+        out[f"e_{method}"] = float(r.energy_true(lab))
+        labs[method] = lab.astype(np.uint8)
+    # qp_sp / lp_sp: the reference's mean-shift superpixels (addSuperPixel(img,
+    # 8, 4, 100, 50, 1000), src/example_inference.cpp:48,67) kept as a region
+    # map with the layer's exp_of_superpixels_; qp_sp reports the higher-order
+    # energy, lp_sp the pairwise one (src/example_inference.cpp:76-87)
+    reg, e, c = r.add_superpixel(im, 8, 4, 100, 50.0, 1000.0)
+    labs["regions"] = reg.astype(np.int32)
+    labs["exp_of_superpixels"] = e
+    labs["constants"] = c
+    lab = r.current_map(r.qp(Q0))
+    out["e_qp_sp"] = float(r.energy_ho(lab))
+    labs["qp_sp"] = lab.astype(np.uint8)
+    lab = r.current_map(r.lp(Q0, O.LPParams()))
+    out["e_lp_sp"] = float(r.energy_true(lab))
+    labs["lp_sp"] = lab.astype(np.uint8)
+    with open(os.path.join(OUT, "data_img.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    np.savez_compressed(os.path.join(OUT, "data_img_labels.npz"), **labs)
+    print("data_img", {k: v for k, v in out.items() if k.startswith("e_")})
+
+
 def main():
+    if sys.argv[1:] == ["data"]:
+        data_case()
+        return
     if sys.argv[1:] == ["bf"]:
         bf_case()
         return
@@ -309,6 +359,7 @@ def main():
     lp_less_confident_case()
     restricted_case()
     bf_case()
+    data_case()
     c0_case()
 
 

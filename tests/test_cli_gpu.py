@@ -1,0 +1,50 @@
+"""The reference's command line (example_inference) on the reference's own
+example input (data/img.bmp, data/img.c_unary, kept in tests/golden/data/)
+with its default parameters: each method's discretised energy within 1e-4
+(relative) of the compiled reference's, and the saved MSRC map's labels equal
+to the reference's on >= 99.9% of the pixels.  The superpixel methods take
+the reference's own mean-shift segmentation as a region map."""
+
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+from test_io import MSRC, _read_png
+
+pytestmark = pytest.mark.gpu
+
+DATA = os.path.join(GOLDEN, "data")
+
+
+@pytest.mark.parametrize("method", ["unary", "qp", "lp", "qp_sp", "lp_sp"])
+def test_example_inference_on_reference_data(tmp_path, method):
+    exe = os.path.join(ROOT, "densecrf_ho_amd", "example_inference")
+    assert os.path.exists(exe), "built by __graft_entry__.build()"
+    with open(os.path.join(GOLDEN, "data_img.json")) as f:
+        ref = json.load(f)
+    g = np.load(os.path.join(GOLDEN, "data_img_labels.npz"))
+    out = tmp_path / "seg.png"
+    cmd = [exe, method, "--image", os.path.join(DATA, "img.bmp"), "--unary", os.path.join(DATA, "img.c_unary"),
+           "--output", str(out)]
+    if method.endswith("_sp"):
+        g["regions"].astype("<i4").tofile(tmp_path / "regions.i32")
+        cmd += ["--regions", str(tmp_path / "regions.i32")]
+    print("running", method, flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    line = [x for x in r.stdout.splitlines() if x.startswith("# method:")][-1]
+    e = float(line.split("\t")[1])
+    er = ref[f"e_{method}"]
+    print("energy", e, "reference", er, flush=True)
+    assert abs(e - er) <= 1e-4 * abs(er)
+    img = _read_png(out)
+    W, H = ref["W"], ref["H"]
+    want = MSRC[g[method]].reshape(H, W, 3)
+    agree = float((img == want).all(-1).mean())
+    print("label agreement", agree, flush=True)
+    assert agree >= 0.999
