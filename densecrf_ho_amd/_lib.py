@@ -49,6 +49,13 @@ SIGNATURES = {
     "dcrf_pairwise_destroy": (_i, [_vp]),
     "dcrf_pairwise_apply": (_i, [_vp, _vp, _vp, _i, _i, _i]),
     "dcrf_pairwise_apply_upper_minus_lower_ord": (_i, [_vp, _vp, _vp, _i, _i]),
+    "dcrf_lattice_gradient": (_i, [_vp, _vp, _vp, _vp, _i, _i]),
+    "dcrf_pairwise_set_kernel_type": (_i, [_vp, _i]),
+    "dcrf_pairwise_kernel_nparams": (_i, [_vp]),
+    "dcrf_pairwise_kernel_parameters": (_i, [_vp, _vp]),
+    "dcrf_pairwise_set_kernel_parameters": (_i, [_vp, _vp, _i]),
+    "dcrf_pairwise_kernel_gradient": (_i, [_vp, _vp, _vp, _i, _vp, _i]),
+    "dcrf_crf_pairwise_kernel_gradient": (_i, [_vp, _i, _vp, _vp, _vp, _i]),
     "dcrf_pairwise_apply_bf": (_i, [_vp, _vp, _vp, _i, _i]),
     "dcrf_pairwise_apply_upper_minus_lower_bf": (_i, [_vp, _vp, _vp, _i, _i]),
     "dcrf_pairwise_apply_upper_minus_lower_bf_ord": (_i, [_vp, _vp, _vp, _vp, _i, _i]),

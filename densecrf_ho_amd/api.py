@@ -324,6 +324,15 @@ class Permutohedral:
             self._h, o.ptr, a.ptr, ix.ptr, len(pI), e.ptr if e else None, rQ.shape[1], int(store), a.dev))
         return rout
 
+    def gradient(self, a, b):
+        """Permutohedral::gradient(df, a, b, V): (N, d) = d/df a^T K b for a, b (N, V)."""
+        x = _Arg(a, np.float32)
+        y = _Arg(b, np.float32)
+        df = _empty_like_field(a, (self.N, self.d))
+        o = _Arg(df, np.float32, out=True)
+        _check(_lib.load().dcrf_lattice_gradient(self._h, o.ptr, x.ptr, y.ptr, a.shape[1], x.dev))
+        return df
+
     def tables(self):
         """offset_, rank_, barycentric_ (Npad, d+1), keys (M, d), neighbours (d+1, M, 2)."""
         d1 = self.d + 1
@@ -335,6 +344,70 @@ class Permutohedral:
         p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
         _check(_lib.load().dcrf_lattice_export(self._h, p(off), p(rk), p(bc), p(keys), p(nbr)))
         return dict(offset=off, rank=rk, bary=bc, keys=keys[: self.M], nbr=nbr[:, : self.M])
+
+
+CONST_KERNEL, DIAG_KERNEL, FULL_KERNEL = 0, 1, 2
+
+
+class PairwisePotential:
+    """PairwisePotential (include/pairwise.h:63-95) with a Potts compatibility,
+    standalone (its own lattice); kernel_type defaults to CONST_KERNEL as in
+    the reference's constructor."""
+
+    def __init__(self, features, function: "PottsCompatibility", kernel_type=CONST_KERNEL,
+                 normalization_type=3):
+        L = _lib.load()
+        a = _Arg(features, np.float32)
+        N, d = tuple(a.arr.shape)
+        h = C.c_void_p()
+        _check(L.dcrf_pairwise_create(a.ptr, N, d, float(function.w), int(normalization_type), a.dev, C.byref(h)))
+        self._h = h
+        self.N, self.d = N, d
+        _check(L.dcrf_pairwise_set_kernel_type(self._h, int(kernel_type)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None:
+            try:
+                _lib.load().dcrf_pairwise_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    def _field(self, Q, fn, *extra):
+        a = _Arg(Q, np.float32)
+        out = _empty_like_field(Q, tuple(Q.shape))
+        o = _Arg(out, np.float32, out=True)
+        _check(fn(self._h, a.ptr, o.ptr, Q.shape[1], *extra, a.dev))
+        return out
+
+    def apply(self, Q):
+        return self._field(Q, _lib.load().dcrf_pairwise_apply, 0)
+
+    def applyTranspose(self, Q):
+        return self._field(Q, _lib.load().dcrf_pairwise_apply, 1)
+
+    def apply_upper_minus_lower_ord(self, Q):
+        return self._field(Q, _lib.load().dcrf_pairwise_apply_upper_minus_lower_ord)
+
+    def kernelParameters(self):
+        n = _lib.load().dcrf_pairwise_kernel_nparams(self._h)
+        v = np.zeros(max(n, 1), np.float32)
+        _check(_lib.load().dcrf_pairwise_kernel_parameters(self._h, v.ctypes.data_as(C.c_void_p)))
+        return v[:n]
+
+    def setKernelParameters(self, v):
+        v = np.ascontiguousarray(v, np.float32)
+        _check(_lib.load().dcrf_pairwise_set_kernel_parameters(self._h, v.ctypes.data_as(C.c_void_p), len(v)))
+
+    def kernelGradient(self, b, Q):
+        """PairwisePotential::kernelGradient(b, Q): gradient of b^T K C(Q) w.r.t. the kernel parameters."""
+        n = _lib.load().dcrf_pairwise_kernel_nparams(self._h)
+        x = _Arg(b, np.float32)
+        q = _Arg(Q, np.float32)
+        out = np.zeros(max(n, 1), np.float32)
+        _check(_lib.load().dcrf_pairwise_kernel_gradient(self._h, x.ptr, q.ptr, Q.shape[1],
+                                                         out.ctypes.data_as(C.c_void_p), x.dev))
+        return out[:n]
 
 
 class LP_inf_params:
@@ -363,6 +436,7 @@ class DenseCRF:
         _check(L.dcrf_crf_create(self.N, self.M, C.byref(h)))
         self._h = h
         self.npairwise = 0
+        self._pw_dims = []  # feature dimension of each pairwise term
         self.R = 0
 
     def close(self):
@@ -394,6 +468,7 @@ class DenseCRF:
                                                             par.ctypes.data_as(C.c_void_p),
                                                             int(normalization_type), a.dev))
         self.npairwise += 1
+        self._pw_dims.append(d)
 
     def set_value_type(self, bf16: bool):
         """ordered-filter lattice rows of the terms added so far in bfloat16 (True) or float"""
@@ -483,6 +558,15 @@ class DenseCRF:
         o = _Arg(out, np.float32, out=True)
         _check(_lib.load().dcrf_crf_apply_upper_minus_lower_ord(self._h, k, a.ptr, o.ptr, a.dev))
         return out
+
+    def pairwise_kernel_gradient(self, k, b, Q):
+        """PairwisePotential::kernelGradient(b, Q) of potential k (DIAG kernel): (d,)"""
+        x = _Arg(b, np.float32)
+        q = _Arg(Q, np.float32)
+        out = np.zeros(16, np.float32)
+        _check(_lib.load().dcrf_crf_pairwise_kernel_gradient(self._h, k, x.ptr, q.ptr,
+                                                             out.ctypes.data_as(C.c_void_p), x.dev))
+        return out[: self._pw_dims[k]]
 
     def apply_bf(self, k, Q, out=None):
         """PairwisePotential::apply_bf of potential k (dense Gaussian, O(N^2 L))."""
@@ -628,6 +712,7 @@ class DenseCRF2D(DenseCRF):
         w = (function or PottsCompatibility(1.0)).w
         _check(_lib.load().dcrf_crf_add_pairwise_gaussian(self._h, self.W, self.H, float(sx), float(sy), w))
         self.npairwise += 1
+        self._pw_dims.append(2)
 
     def addPairwiseBilateral(self, sx, sy, sr, sg, sb, im, function: PottsCompatibility | None = None,
                              kernel_type=1, normalization_type=NORMALIZE_SYMMETRIC):
@@ -641,6 +726,7 @@ class DenseCRF2D(DenseCRF):
                                                            float(sg), float(sb),
                                                            im.ctypes.data_as(C.c_void_p), w))
         self.npairwise += 1
+        self._pw_dims.append(5)
 
 
 class DenseCRFBatch:

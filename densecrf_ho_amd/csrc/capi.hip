@@ -277,6 +277,20 @@ int dcrf_lattice_compute_upper_minus_lower_ord_restricted(dcrf_lattice_t lat, fl
   });
 }
 
+int dcrf_lattice_gradient(dcrf_lattice_t lat, float* df, const float* a, const float* b, int V, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(lat && df && a && b && V > 0, "dcrf_lattice_gradient: bad arguments");
+    const int N = lat->lat->N(), d = lat->lat->d();
+    hipStream_t s = lat->s;
+    Staged<float> x(a, (size_t)N * V, on_device != 0, s, true);
+    Staged<float> y(b, (size_t)N * V, on_device != 0, s, true);
+    Staged<float> o(df, (size_t)N * d, on_device != 0, s, false);
+    lat->lat->gradient(x.d, y.d, V, o.d);
+    o.copy_out();
+    DCRF_HIP(hipStreamSynchronize(s));
+  });
+}
+
 // ---- pairwise potential ---------------------------------------------------------
 int dcrf_pairwise_create(const float* features, int N, int d, float potts_w, int ntype, int on_device,
                          dcrf_pairwise_t* out) {
@@ -286,6 +300,7 @@ int dcrf_pairwise_create(const float* features, int N, int d, float potts_w, int
     DCRF_HIP(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
     Staged<float> f(features, (size_t)N * d, on_device != 0, h->s, true);
     potential_init(h->p, f.d, N, d, potts_w, ntype, h->s);
+    h->p.ktype = kConstKernel;  // PairwisePotential's default (include/pairwise.h:71)
     h->mm0.resize(2);
     DCRF_HIP(hipMemsetAsync(h->mm0.get(), 0, 2 * sizeof(float), h->s));
     DCRF_HIP(hipStreamSynchronize(h->s));
@@ -376,6 +391,44 @@ int dcrf_pairwise_apply_upper_minus_lower_bf_ord(dcrf_pairwise_t pw, const int* 
   return guarded([&] {
     DCRF_REQUIRE(pw && ind && Q && out && L > 0, "dcrf_pairwise_apply_upper_minus_lower_bf_ord: bad arguments");
     apply_bf_common(pw->p, pw->s, 2, ind, Q, out, L, on_device);
+  });
+}
+
+int dcrf_pairwise_set_kernel_type(dcrf_pairwise_t pw, int ktype) {
+  return guarded([&] {
+    DCRF_REQUIRE(pw && ktype >= 0 && ktype <= 2, "dcrf_pairwise_set_kernel_type: bad arguments");
+    DCRF_REQUIRE(pw->p.kparams.empty(), "dcrf_pairwise_set_kernel_type: set before any kernel parameters");
+    pw->p.ktype = ktype;
+  });
+}
+int dcrf_pairwise_kernel_nparams(dcrf_pairwise_t pw) { return pw ? potential_kernel_nparams(pw->p) : -1; }
+int dcrf_pairwise_kernel_parameters(dcrf_pairwise_t pw, float* out) {
+  return guarded([&] {
+    DCRF_REQUIRE(pw, "dcrf_pairwise_kernel_parameters: bad arguments");
+    const std::vector<float> v = potential_kernel_parameters(pw->p);
+    DCRF_REQUIRE(out || v.empty(), "dcrf_pairwise_kernel_parameters: null output");
+    std::copy(v.begin(), v.end(), out);
+  });
+}
+int dcrf_pairwise_set_kernel_parameters(dcrf_pairwise_t pw, const float* v, int n) {
+  return guarded([&] {
+    DCRF_REQUIRE(pw && (v || n == 0), "dcrf_pairwise_set_kernel_parameters: bad arguments");
+    potential_set_kernel_parameters(pw->p, v, n, pw->s);
+  });
+}
+static void kernel_gradient_common(Potential& p, hipStream_t s, const float* b, const float* Q, int L, float* out,
+                                   int on_device) {
+  const size_t n = (size_t)p.lat->N() * L;
+  Staged<float> x(b, n, on_device != 0, s, true);
+  Staged<float> q(Q, n, on_device != 0, s, true);
+  potential_pairwise_kernel_gradient(p, x.d, q.d, L, out);
+}
+int dcrf_pairwise_kernel_gradient(dcrf_pairwise_t pw, const float* b, const float* Q, int L, float* out,
+                                  int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(pw && b && Q && L > 0 && (out || potential_kernel_nparams(pw->p) == 0),
+                 "dcrf_pairwise_kernel_gradient: bad arguments");
+    kernel_gradient_common(pw->p, pw->s, b, Q, L, out, on_device);
   });
 }
 
@@ -578,6 +631,15 @@ int dcrf_crf_apply_upper_minus_lower_bf_ord(dcrf_crf_t crf, int k, const int* in
     CRF& c = *crf->crf;
     DCRF_REQUIRE(k >= 0 && k < c.num_pairwise(), "dcrf_crf_apply_upper_minus_lower_bf_ord: no such potential");
     apply_bf_common(c.pairwise(k), c.stream(), 2, ind, Q, out, c.L(), on_device);
+  });
+}
+int dcrf_crf_pairwise_kernel_gradient(dcrf_crf_t crf, int k, const float* b, const float* Q, float* out,
+                                      int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && b && Q && out, "dcrf_crf_pairwise_kernel_gradient: bad arguments");
+    CRF& c = *crf->crf;
+    DCRF_REQUIRE(k >= 0 && k < c.num_pairwise(), "dcrf_crf_pairwise_kernel_gradient: no such potential");
+    kernel_gradient_common(c.pairwise(k), c.stream(), b, Q, c.L(), out, on_device);
   });
 }
 int dcrf_crf_unary_init(dcrf_crf_t crf, float* Q, int on_device) {

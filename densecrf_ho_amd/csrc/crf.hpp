@@ -21,6 +21,8 @@ enum NormType : int { kNoNorm = 0, kNormBefore = 1, kNormAfter = 2, kNormSymmetr
 
 // LabelCompatibility kinds (include/labelcompatibility.h:51-92)
 enum CompatType : int { kPotts = 0, kDiagonal = 1, kMatrix = 2 };
+// KernelType (include/pairwise.h:40-44)
+enum KernelKind : int { kConstKernel = 0, kDiagKernel = 1, kFullKernel = 2 };
 
 struct Potential {
   std::unique_ptr<Lattice> lat;
@@ -35,7 +37,11 @@ struct Potential {
   std::vector<float> cparam;
   DBuf<float> cdev;
   DBuf<float> ctmp;  // N x L filter output before the compatibility pass
-  DBuf<float> feat;  // N x d features (brute-force filters), when built from features
+  DBuf<float> feat;  // N x d features as given (brute-force filters, kernel gradient)
+  // DenseKernel's kernel type and parameters (ones(d) for DIAG, identity for
+  // FULL until set); the lattice is built on the features they transform
+  int ktype = kDiagKernel;
+  std::vector<float> kparams;
   // the NO_NORMALIZATION copy used by the true energies is always
   // PottsCompatibility(parameters()(0)), taken when the term is added
   // (src/densecrf.cpp:161-168)
@@ -82,6 +88,20 @@ void potential_apply_bf(Potential& p, const float* Q, float* out, int L);
 // pixel at sorted position c of label l is ind[c*L + l], a valid index: the
 // caller checks), Q is read only when ord; out N x L in sorted order
 void potential_apply_uml_bf(Potential& p, const int* ind, const float* Q, float* out, int L, bool ord);
+// DenseKernel::featureGradient (src/pairwise.cpp:95-122): fg (N x d, device)
+// = d/df a^T K_norm b for a, b N x V (device)
+void potential_feature_gradient(Potential& p, const float* a, const float* b, int V, float* fg);
+// kernel parameter count of p (0 / d / d*d) and DenseKernel::gradient
+// (:170-181) into host_out; PairwisePotential::kernelGradient passes
+// b and compatibility(Q)
+int potential_kernel_nparams(const Potential& p);
+void potential_kernel_gradient(Potential& p, const float* a, const float* b, int V, float* host_out);
+// DenseKernel::setParameters (:149-161): rebuild the lattice (and norm) on
+// diag(v) f (DIAG) or V f (FULL, v column-major d x d)
+void potential_set_kernel_parameters(Potential& p, const float* v, int n, hipStream_t s);
+std::vector<float> potential_kernel_parameters(const Potential& p);
+// PairwisePotential::kernelGradient(b, Q) (src/pairwise.cpp:402-407); host out
+void potential_pairwise_kernel_gradient(Potential& p, const float* b, const float* Q, int L, float* host_out);
 // LabelCompatibility::apply of a Diagonal or Matrix compatibility on nrows
 // label-contiguous rows: out = C in (the Potts form is a scale by -w)
 void compat_apply_rows(Potential& p, const float* in, float* out, long nrows, int V, hipStream_t s);

@@ -6,6 +6,7 @@
 #include "ho.hip"
 #include "lp.hip"
 #include "bf.hip"
+#include "kgrad.hip"
 #include "adjoint.hip"
 #include "batch.hip"
 #include "capi.hip"

@@ -1418,9 +1418,38 @@ void Lattice::filter(const SplatArgs& in, const SliceArgs& out, int V, bool reve
   ensure_values(V);
   hipStream_t s = stream_;
   const RowMap rm = rowmap(V);
+  const std::string tag = prof_on() ? "[d=" + std::to_string(d_) + ",V=" + std::to_string(V) + "]" : std::string();
+  const bool seq = V <= 2;
+  const float* src = splat_blur(in, V, reverse, seq, tag);
+  const float alpha = lattice_alpha(d_);
+  KTimer tm("slice" + tag, s);
+  if (!seq && V >= kStageMinV && V <= 64) {
+    const long nl = (long)N_ * ((V + 3) / 4);
+    if (nl > 0)
+      k_slice_quad<<<grid_for((size_t)nl), kBlock, 0, s>>>(V, nl, N_, d_ + 1, offset_.get(), bary_.get(), src, alpha,
+                                                           out);
+  } else if (V >= d_ + 1 && V <= 64) {
+    if (seq)
+      k_slice_staged<true><<<rows_grid(N_, V), kBlock, 0, s>>>(V, rm, N_, d_ + 1, offset_.get(), bary_.get(), src,
+                                                               alpha, out);
+    else
+      k_slice_staged<false><<<rows_grid(N_, V), kBlock, 0, s>>>(V, rm, N_, d_ + 1, offset_.get(), bary_.get(), src,
+                                                                alpha, out);
+  } else if (seq)
+    k_slice<true><<<rows_grid(N_, V), kBlock, 0, s>>>(V, rm, N_, d_ + 1, offset_.get(), bary_.get(), src, alpha, out);
+  else
+    k_slice<false><<<rows_grid(N_, V), kBlock, 0, s>>>(V, rm, N_, d_ + 1, offset_.get(), bary_.get(), src, alpha, out);
+  check_launch("k_slice");
+}
+
+// splat of `in` into the value rows and the d+1 blur axes (reversed order
+// when `reverse`; `seq`: the blur sums in double and rounds once, as
+// seqCompute does); returns the rows holding the result
+const float* Lattice::splat_blur(const SplatArgs& in, int V, bool reverse, bool seq, const std::string& tag) {
+  hipStream_t s = stream_;
+  const RowMap rm = rowmap(V);
   float* src = vA_.get();
   float* dst = vB_.get();
-  const std::string tag = prof_on() ? "[d=" + std::to_string(d_) + ",V=" + std::to_string(V) + "]" : std::string();
   {
   KTimer tm("splat" + tag, s);
   const bool stage = V >= kStageMinV && V <= 64;
@@ -1450,7 +1479,6 @@ void Lattice::filter(const SplatArgs& in, const SliceArgs& out, int V, bool reve
     check_launch("k_combine");
   }
   }
-  const bool seq = V <= 2;
   const long bl = (long)M_ * ((V + 3) / 4);  // quad-blur lanes
   DCRF_REQUIRE(bl < (1L << 31), "filter: lattice too large for one blur launch");
   for (int jj = 0; jj <= d_; ++jj) {
@@ -1467,25 +1495,7 @@ void Lattice::filter(const SplatArgs& in, const SliceArgs& out, int V, bool reve
     check_launch("k_blur");
     std::swap(src, dst);
   }
-  const float alpha = lattice_alpha(d_);
-  KTimer tm("slice" + tag, s);
-  if (!seq && V >= kStageMinV && V <= 64) {
-    const long nl = (long)N_ * ((V + 3) / 4);
-    if (nl > 0)
-      k_slice_quad<<<grid_for((size_t)nl), kBlock, 0, s>>>(V, nl, N_, d_ + 1, offset_.get(), bary_.get(), src, alpha,
-                                                           out);
-  } else if (V >= d_ + 1 && V <= 64) {
-    if (seq)
-      k_slice_staged<true><<<rows_grid(N_, V), kBlock, 0, s>>>(V, rm, N_, d_ + 1, offset_.get(), bary_.get(), src,
-                                                               alpha, out);
-    else
-      k_slice_staged<false><<<rows_grid(N_, V), kBlock, 0, s>>>(V, rm, N_, d_ + 1, offset_.get(), bary_.get(), src,
-                                                                alpha, out);
-  } else if (seq)
-    k_slice<true><<<rows_grid(N_, V), kBlock, 0, s>>>(V, rm, N_, d_ + 1, offset_.get(), bary_.get(), src, alpha, out);
-  else
-    k_slice<false><<<rows_grid(N_, V), kBlock, 0, s>>>(V, rm, N_, d_ + 1, offset_.get(), bary_.get(), src, alpha, out);
-  check_launch("k_slice");
+  return src;
 }
 
 // total splatted weight per point, blurred with the ordered filter's float

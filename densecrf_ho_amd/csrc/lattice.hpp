@@ -9,6 +9,7 @@
 #pragma once
 
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -114,6 +115,10 @@ class Lattice {
   // (nI x V).  Not re-entrant: the state lives in the lattice.
   void ordered_restricted(const float* rq, const int* pI, int nI, const float* ext, int V, bool store, float* out);
 
+  // Permutohedral::gradient (src/permutohedral.cpp:1095-1178): df (N x d,
+  // device) = d/df a^T K b for a, b N x V.
+  void gradient(const float* a, const float* b, int V, float* df);
+
   // copies of the lattice tables for parity checks (host pointers, may be null)
   void export_tables(int* offset, int* rank, float* bary, short* keys, int* nbr) const;
 
@@ -125,6 +130,7 @@ class Lattice {
 
  private:
   void ensure_values(size_t floats, bool zero_row0 = true);
+  const float* splat_blur(const SplatArgs& in, int V, bool reverse, bool seq, const std::string& tag);
   template <class T>
   void ordered_t(const float* Q, const float* d_minmax, int L, const SliceArgs& out, OrderedBins* shared,
                  bool fresh);

@@ -84,6 +84,10 @@ int dcrf_lattice_compute(dcrf_lattice_t lat, const float* in, float* out, int V,
  * include/permutohedral.h:83, src/permutohedral.cpp:590-701 (10-bin ordered
  * filter, no normalisation, no compatibility).  Q is V x N with values in [0,1]. */
 int dcrf_lattice_compute_upper_minus_lower_ord(dcrf_lattice_t lat, const float* Q, float* out, int V, int on_device);
+/* Permutohedral::gradient(float* df, const float* a, const float* b, int value_size),
+ * include/permutohedral.h:87, src/permutohedral.cpp:1095-1178: df (d x N) = d/df of
+ * a^T K b for a, b (V x N), K the unnormalised lattice filter. */
+int dcrf_lattice_gradient(dcrf_lattice_t lat, float* df, const float* a, const float* b, int V, int on_device);
 /* Permutohedral::compute_upper_minus_lower_ord_restricted(MatrixXf& rout, const MatrixXf& rQ,
  * const std::vector<int>& pI, const MatrixXf& Q, bool store), include/permutohedral.h:84-85,
  * src/permutohedral.cpp:702-880, 1074-1078.  The ordered filter at the nI pixels pI
@@ -115,6 +119,23 @@ int dcrf_pairwise_apply_upper_minus_lower_ord(dcrf_pairwise_t pw, const float* Q
  * include/pairwise.h:78-79, src/pairwise.cpp:229-239: rout (L x nI) = 0, the restricted
  * ordered filter (as dcrf_lattice_compute_upper_minus_lower_ord_restricted), then the
  * compatibility (Potts: -w). */
+/* KernelType of the potential (0 CONST_KERNEL, the default here as in
+ * PairwisePotential's constructor, include/pairwise.h:40-44,71; 1 DIAG_KERNEL;
+ * 2 FULL_KERNEL); set before any kernel parameters. */
+int dcrf_pairwise_set_kernel_type(dcrf_pairwise_t pw, int ktype);
+/* PairwisePotential::kernelParameters / setKernelParameters, src/pairwise.cpp:396-401
+ * (DenseKernel::parameters / setParameters, :136-161): 0, d or d*d values (DIAG: the
+ * feature scales, ones at first; FULL: a d x d column-major matrix, the identity at
+ * first).  Setting rebuilds the lattice and the normalisation on diag(v) f or V f. */
+int dcrf_pairwise_kernel_nparams(dcrf_pairwise_t pw);
+int dcrf_pairwise_kernel_parameters(dcrf_pairwise_t pw, float* out);
+int dcrf_pairwise_set_kernel_parameters(dcrf_pairwise_t pw, const float* v, int n);
+/* PairwisePotential::kernelGradient(b, Q), src/pairwise.cpp:402-407: the gradient of
+ * b^T K compatibility(Q) w.r.t. the kernel parameters (DenseKernel::gradient and
+ * featureGradient, :90-122, :170-181); b and Q are L x N, out (host) holds
+ * dcrf_pairwise_kernel_nparams values. */
+int dcrf_pairwise_kernel_gradient(dcrf_pairwise_t pw, const float* b, const float* Q, int L, float* out,
+                                  int on_device);
 /* PairwisePotential::apply_bf(out, Q), include/pairwise.h:73, src/pairwise.cpp:368-383:
  * the dense Gaussian exp(-|f_c - f_b|^2) over all pixel pairs (no lattice, no
  * normalisation), then the compatibility.  O(N^2 L): for small inputs. */
@@ -184,6 +205,10 @@ int dcrf_crf_apply_upper_minus_lower_ord(dcrf_crf_t crf, int k, const float* Q, 
 /* PairwisePotential::apply_upper_minus_lower_ord_restricted of potential k,
  * src/pairwise.cpp:229-239, with that potential's compatibility (Potts, Diagonal or
  * Matrix); rout is L x nI, Q is L x N. */
+/* PairwisePotential::kernelGradient(b, Q) of potential k (a DIAG_KERNEL, the default of
+ * addPairwiseEnergy): d values into out (host). */
+int dcrf_crf_pairwise_kernel_gradient(dcrf_crf_t crf, int k, const float* b, const float* Q, float* out,
+                                      int on_device);
 /* The brute-force filters of potential k (apply_bf, apply_upper_minus_lower_bf,
  * apply_upper_minus_lower_bf_ord, src/pairwise.cpp:294-383) with its compatibility;
  * layouts as the dcrf_pairwise_ forms, L x N fields. */

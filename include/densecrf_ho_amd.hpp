@@ -216,6 +216,10 @@ class Permutohedral {
     if (out.rows() != Q.rows() || out.cols() != Q.cols()) out.resize(Q.rows(), Q.cols());
     check(dcrf_lattice_compute_upper_minus_lower_ord(h_, Q.data(), out.data(), (int)Q.rows(), 0));
   }
+  // df (d x N, column-major like the reference's features) of a^T K b
+  void gradient(float* df, const float* a, const float* b, int value_size) const {
+    check(dcrf_lattice_gradient(h_, df, a, b, value_size, 0));
+  }
   // rout (V x |pI|) is added to, as the reference does
   void compute_upper_minus_lower_ord_restricted(MatrixXf& rout, const MatrixXf& rQ, const std::vector<int>& pI,
                                                 const MatrixXf& Q, const bool store) {
@@ -247,6 +251,7 @@ class PairwisePotential {
       : compat_(compatibility), f_(features), ktype_(ktype), ntype_(ntype) {
     check(dcrf_pairwise_create(features.data(), (int)features.cols(), (int)features.rows(),
                                compatibility->potts_weight(), (int)ntype, 0, &h_));
+    check(dcrf_pairwise_set_kernel_type(h_, (int)ktype));
   }
   ~PairwisePotential() {
     if (h_) dcrf_pairwise_destroy(h_);
@@ -270,6 +275,20 @@ class PairwisePotential {
     check(dcrf_pairwise_apply_upper_minus_lower_ord_restricted(h_, rout.data(), rQ.data(), pI.data(), (int)pI.size(),
                                                                store ? Q.data() : nullptr, (int)rQ.rows(),
                                                                store ? 1 : 0, 0));
+  }
+  // kernelParameters / setKernelParameters / kernelGradient (src/pairwise.cpp:396-407)
+  std::vector<float> kernelParameters() const {
+    std::vector<float> v((size_t)std::max(dcrf_pairwise_kernel_nparams(h_), 0));
+    check(dcrf_pairwise_kernel_parameters(h_, v.data()));
+    return v;
+  }
+  void setKernelParameters(const std::vector<float>& v) {
+    check(dcrf_pairwise_set_kernel_parameters(h_, v.data(), (int)v.size()));
+  }
+  std::vector<float> kernelGradient(const MatrixXf& b, const MatrixXf& Q) const {
+    std::vector<float> g((size_t)std::max(dcrf_pairwise_kernel_nparams(h_), 0));
+    check(dcrf_pairwise_kernel_gradient(h_, b.data(), Q.data(), (int)Q.rows(), g.data(), 0));
+    return g;
   }
   void apply_bf(MatrixXf& out, const MatrixXf& Q) const {
     if (out.rows() != Q.rows() || out.cols() != Q.cols()) out.resize(Q.rows(), Q.cols());

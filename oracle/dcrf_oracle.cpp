@@ -404,6 +404,66 @@ struct Lattice {
   // Bin rules of the ordered filter (src/permutohedral.cpp:548-589):
   // lower bin  = min(floor(float(q*10)), 9)
   // upper bin  = max(floor((double(q) - 1e-9) * 10), 0)
+  // Permutohedral::gradient (src/permutohedral.cpp:1095-1178): df (N x d) of
+  // a^T K b; splat in float, blur sums in double and rounds once, slice
+  // through the rank-ordered vertices in float
+  void gradient(const float* a, const float* b, int V, float* df) const {
+    const int d1 = d + 1;
+    std::vector<float> values((size_t)(M + 2) * V), nv((size_t)(M + 2) * V);
+    std::fill(df, df + (size_t)N * d, 0.f);
+    std::vector<float> scale(d);
+    const float inv_std = (float)(std::sqrt(2.0 / 3.0) * d1);
+    for (int i = 0; i < d; ++i) scale[i] = (float)(1.0 / std::sqrt(double((i + 2) * (i + 1))) * inv_std);
+    const float alpha = 1.0f / (1 + std::pow(2.0f, (float)-d)) / d1;
+    for (int dir = 0; dir < 2; ++dir) {
+      std::fill(values.begin(), values.end(), 0.f);
+      std::fill(nv.begin(), nv.end(), 0.f);
+      const float* x = dir ? b : a;
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j <= d; ++j) {
+          const int o = offset[(size_t)i * d1 + j] + 1;
+          const float w = bary[(size_t)i * d1 + j];
+          for (int k = 0; k < V; ++k) values[(size_t)o * V + k] += w * x[(size_t)i * V + k];
+        }
+      float* cur = values.data();
+      float* nxt = nv.data();
+      for (int jj = 0; jj <= d; ++jj) {
+        const int j = dir ? d - jj : jj;
+        for (int i = 0; i < M; ++i) {
+          const int n1 = nbr[((size_t)j * M + i) * 2] + 1, n2 = nbr[((size_t)j * M + i) * 2 + 1] + 1;
+          for (int k = 0; k < V; ++k)
+            nxt[(size_t)(i + 1) * V + k] =
+                (float)(cur[(size_t)(i + 1) * V + k] + 0.5 * (cur[(size_t)n1 * V + k] + cur[(size_t)n2 * V + k]));
+        }
+        std::swap(cur, nxt);
+      }
+      const float* y = dir ? a : b;
+      std::vector<float> ra((size_t)d1 * V), sm(V);
+      for (int i = 0; i < N; ++i) {
+        std::fill(ra.begin(), ra.end(), 0.f);
+        for (int j = 0; j <= d; ++j) {
+          const int r0 = d - rank[(size_t)i * d1 + j];
+          const int r1 = r0 + 1 > d ? 0 : r0 + 1;
+          const int o0 = offset[(size_t)i * d1 + r0] + 1, o1 = offset[(size_t)i * d1 + r1] + 1;
+          for (int k = 0; k < V; ++k) {
+            ra[(size_t)j * V + k] += alpha * cur[(size_t)o0 * V + k];
+            ra[(size_t)j * V + k] -= alpha * cur[(size_t)o1 * V + k];
+          }
+        }
+        for (int k = 0; k < V; ++k) sm[k] = ra[k];
+        for (int j = 1; j <= d; ++j) {
+          float grad = 0;
+          for (int k = 0; k < V; ++k) {
+            const float v = scale[j - 1] * (sm[k] - j * ra[(size_t)j * V + k]);
+            grad += y[(size_t)i * V + k] * v;
+            sm[k] += ra[(size_t)j * V + k];
+          }
+          df[(size_t)i * d + j - 1] += grad;
+        }
+      }
+    }
+  }
+
   static int bin_lo(float q) { return std::min((int)std::floor(q * 10), 9); }
   static int bin_up(float q) { return std::max((int)std::floor(((double)q - (g_mut == 3 ? 0.0 : 1e-9)) * 10), 0); }
 
@@ -1431,6 +1491,9 @@ void orc_filter_d(void* h, const double* in, double* out, int V, int reverse) {
 }
 void orc_ordered_f(void* h, const float* q, float* out, int V) { ((Lattice*)h)->ordered<float>(q, out, V); }
 void orc_ordered_d(void* h, const float* q, double* out, int V) { ((Lattice*)h)->ordered<double>(q, out, V); }
+void orc_lattice_gradient(void* h, const float* a, const float* b, int V, float* df) {
+  ((Lattice*)h)->gradient(a, b, V, df);
+}
 void orc_bf_apply(const float* f, int N, int d, const float* Q, int L, float* out) { orc::bf_apply(f, N, d, Q, L, out); }
 void orc_bf_uml(const float* f, int N, int d, const int* ind, const float* Q, int L, float* out) {
   orc::bf_uml(f, N, d, ind, Q, L, out);

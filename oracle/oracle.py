@@ -61,6 +61,7 @@ def lib():
         _sig(L, "orc_ordered_f", None, _vp, _f32p, _f32p, _i)
         _sig(L, "orc_ordered_d", None, _vp, _f32p, _f64p, _i)
         _sig(L, "orc_bf_apply", None, _f32p, _i, _i, _f32p, _i, _f32p)
+        _sig(L, "orc_lattice_gradient", None, _vp, _f32p, _f32p, _i, _f32p)
         _sig(L, "orc_bf_uml", None, _f32p, _i, _i, _i32p, _vp, _i, _f32p)
         _sig(L, "orc_ordered_restricted", _i, _vp, _f32p, _i32p, _i, _f32p, _i, _i, _f32p)
         _sig(L, "orc_crf_new", _vp, _i, _i)
@@ -148,6 +149,14 @@ class Lattice:
         out = np.zeros(q.shape, np.float64 if fp64 else np.float32)
         (self._L.orc_ordered_d if fp64 else self._L.orc_ordered_f)(self.h, q, out, V)
         return out
+
+    def gradient(self, a, b):
+        """Permutohedral::gradient: (N, d) = d/df a^T K b, a and b (N, V)"""
+        a = _c(a, np.float32)
+        b = _c(b, np.float32)
+        df = np.zeros((self.N, self.d), np.float32)
+        self._L.orc_lattice_gradient(self.h, a, b, a.shape[1], df)
+        return df
 
     def ordered_restricted(self, rq, pI, q, store, out=None):
         """seqCompute_upper_minus_lower_ord_restricted: rq (|pI| x V) for the
@@ -416,6 +425,8 @@ def reflib():
         _sig(R, "ref_lattice_compute", None, _vp, _f32p, _f32p, _i, _i, _i)
         _sig(R, "ref_lattice_ordered", None, _vp, _f32p, _f32p, _i, _i)
         _sig(R, "ref_probimage", None, C.c_char_p, _vp, _i32p)
+        _sig(R, "ref_lattice_gradient", None, _vp, _f32p, _f32p, _i, _i, _f32p)
+        _sig(R, "ref_pairwise_kernel_gradient", _i, _f32p, _i, _i, _f, _i, _i, _vp, _i, _f32p, _f32p, _i, _f32p)
         _sig(R, "ref_crf_add_superpixel", _i, _vp, _vp, _i, _i, _i, _f, _f)
         _sig(R, "ref_crf_layers", None, _vp, _i32p, _f32p, _f32p)
         _sig(R, "ref_crf_apply_bf", None, _vp, _i, _i, _vp, _vp, _f32p)
@@ -460,6 +471,20 @@ def ref_probimage(path):
     return p, W, H
 
 
+def ref_pairwise_kernel_gradient(feat, w, ktype, ntype, b, Q, kp=None):
+    """standalone reference PairwisePotential(feat, Potts(w), ktype, ntype)
+    [.setKernelParameters(kp)].kernelGradient(b, Q)"""
+    feat = _c(feat, np.float32)
+    b = _c(b, np.float32)
+    Q = _c(Q, np.float32)
+    kpa = None if kp is None else _c(kp, np.float32)
+    out = np.zeros(feat.shape[1] ** 2 + 1, np.float32)
+    n = reflib().ref_pairwise_kernel_gradient(feat, feat.shape[1], feat.shape[0], w, ktype, ntype,
+                                             None if kpa is None else kpa.ctypes.data_as(C.c_void_p),
+                                             0 if kpa is None else len(kpa), b, Q, b.shape[1], out)
+    return out[:n]
+
+
 class RefLattice:
     def __init__(self, feat):
         feat = _c(feat, np.float32)
@@ -496,6 +521,14 @@ class RefLattice:
         out = np.zeros_like(q)
         self._R.ref_lattice_ordered(self.h, q, out, q.shape[1], self.N)
         return out
+
+    def gradient(self, a, b):
+        """Permutohedral::gradient: (N, d) of a^T K b"""
+        a = _c(a, np.float32)
+        b = _c(b, np.float32)
+        df = np.zeros((self.N, self.d), np.float32)
+        self._R.ref_lattice_gradient(self.h, a, b, a.shape[1], self.N, df)
+        return df
 
     def ordered_restricted(self, rq, pI, q, store, out=None):
         rq = _c(rq, np.float32)

@@ -142,6 +142,25 @@ void ref_probimage(const char* path, float* probs, int* dims) {
         for (int k = 0; k < p.depth(); ++k) probs[((size_t)y * p.width() + x) * p.depth() + k] = p(x, y, k);
 }
 
+// Permutohedral::gradient: df (d x N) of a^T K b, a and b V x N
+void ref_lattice_gradient(void* h, const float* a, const float* b, int V, int N, float* df) {
+  ((RefLattice*)h)->gradient(df, a, b, V);
+}
+// a standalone PairwisePotential(features, Potts(w), ktype, ntype), optional
+// setKernelParameters(kp), then kernelGradient(b, Q) into out; returns its size
+int ref_pairwise_kernel_gradient(const float* feat, int d, int N, float w, int ktype, int ntype, const float* kp,
+                                 int nkp, const float* b, const float* Q, int L, float* out) {
+  PairwisePotential p(as_mat(feat, d, N), new PottsCompatibility(w), (KernelType)ktype, (NormalizationType)ntype);
+  if (kp) {
+    VectorXf v(nkp);
+    std::copy(kp, kp + nkp, v.data());
+    p.setKernelParameters(v);
+  }
+  VectorXf g = p.kernelGradient(as_mat(b, L, N), as_mat(Q, L, N));
+  std::copy(g.data(), g.data() + g.size(), out);
+  return (int)g.size();
+}
+
 // ---- DenseCRF --------------------------------------------------------------
 void* ref_crf_new(int W, int H, int L) { return new RefCRF(W, H, L); }
 void ref_crf_free(void* h) { delete (RefCRF*)h; }

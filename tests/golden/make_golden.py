@@ -10,6 +10,7 @@ compressed .npz files.  Run from the repository root:
     python tests/golden/make_golden.py restricted  # only ordered_restricted_31x23.npz
     python tests/golden/make_golden.py bf       # only bf_12x9_L4.npz
     python tests/golden/make_golden.py data     # only data_img.json / data_img_labels.npz
+    python tests/golden/make_golden.py kgrad    # only kgrad_31x23.npz
 """
 
 from __future__ import annotations
@@ -322,13 +323,71 @@ def data_case():
     lab = r.current_map(r.lp(Q0, O.LPParams()))
     out["e_lp_sp"] = float(r.energy_true(lab))
     labs["lp_sp"] = lab.astype(np.uint8)
+    # The LP methods amplify rounding: run the reference again from initial
+    # fields one ulp away from its own unary_init (the oracle's softmax, which
+    # differs from Eigen's vectorised exp by an ulp, and two random 1-ulp
+    # perturbations) and keep the spread of its final energies and of the
+    # label agreement with the unperturbed run.
+    rng = np.random.default_rng(77)
+    inits = [O.unary_init(U)]
+    for _ in range(2):
+        q = Q0.copy()
+        m = rng.random(q.shape) < 0.02
+        q[m] = np.nextafter(q[m], np.where(rng.random(m.sum()) < 0.5, 0, 1).astype(np.float32))
+        inits.append(q)
+    for method, layers in [("lp", False), ("lp_sp", True)]:
+        rr = O.RefCRF(W, H, U.shape[1])
+        rr.set_unary(U)
+        rr.add_gaussian(2, 2, 3)
+        rr.add_bilateral(30, 30, 8, 8, 8, im, 2)
+        if layers:
+            rr.add_layer(reg, e, c)
+        es, ag = [out[f"e_{method}"]], []
+        for q in inits:
+            lab = rr.current_map(rr.lp(q, O.LPParams()))
+            es.append(float(rr.energy_true(lab)))
+            ag.append(float((lab == labs[method]).mean()))
+        out[f"e_{method}_spread"] = [min(es), max(es)]
+        out[f"agree_{method}_min"] = min(ag)
     with open(os.path.join(OUT, "data_img.json"), "w") as f:
         json.dump(out, f, indent=1)
     np.savez_compressed(os.path.join(OUT, "data_img_labels.npz"), **labs)
     print("data_img", {k: v for k, v in out.items() if k.startswith("e_")})
 
 
+def kgrad_case():
+    """kernel (feature) gradients: Permutohedral::gradient on the 31x23
+    lattices (d=2 and d=5, V=5), and a standalone PairwisePotential's
+    kernelGradient(b, Q) for DIAG and FULL kernels under the four
+    normalisation types, and after setKernelParameters
+    (src/permutohedral.cpp:1095-1178, src/pairwise.cpp:90-181, :396-407)"""
+    wl = S.Workload(31, 23, 8, 1)
+    rng = np.random.default_rng(31)
+    d = {}
+    for k, (f, _) in enumerate(wl.kernels):
+        a = rng.standard_normal((wl.N, 5)).astype(np.float32)
+        b = rng.standard_normal((wl.N, 5)).astype(np.float32)
+        q = rng.random((wl.N, 5)).astype(np.float32)
+        d.update({f"feat{k}": f, f"a{k}": a, f"b{k}": b, f"q{k}": q})
+        d[f"df{k}"] = O.RefLattice(f).gradient(a, b)
+        for kt in (1, 2):
+            for nt in (0, 1, 2, 3):
+                d[f"kg{k}_{kt}_{nt}"] = O.ref_pairwise_kernel_gradient(f, 3.0, kt, nt, b, q)
+        dd = f.shape[1]
+        kp_diag = (0.5 + rng.random(dd)).astype(np.float32)
+        kp_full = (np.eye(dd) + 0.1 * rng.standard_normal((dd, dd))).astype(np.float32).ravel(order="F")
+        d[f"kp_diag{k}"] = kp_diag
+        d[f"kp_full{k}"] = kp_full
+        d[f"kgp{k}_1"] = O.ref_pairwise_kernel_gradient(f, 3.0, 1, 3, b, q, kp=kp_diag)
+        d[f"kgp{k}_2"] = O.ref_pairwise_kernel_gradient(f, 3.0, 2, 3, b, q, kp=kp_full)
+    np.savez_compressed(os.path.join(OUT, "kgrad_31x23.npz"), **d)
+    print("kgrad_31x23")
+
+
 def main():
+    if sys.argv[1:] == ["kgrad"]:
+        kgrad_case()
+        return
     if sys.argv[1:] == ["data"]:
         data_case()
         return
@@ -359,6 +418,7 @@ def main():
     lp_less_confident_case()
     restricted_case()
     bf_case()
+    kgrad_case()
     data_case()
     c0_case()
 

@@ -172,3 +172,41 @@ def test_restricted_ordered_every_pixel_is_ordered_filter(d):
     b = lat.compute_upper_minus_lower_ord(q)
     print("max abs diff", float(np.abs(a - b).max()), flush=True)
     assert rel_err(a, b) < 1e-5
+
+
+@pytest.mark.parametrize("k", [0, 1])
+def test_lattice_gradient_vs_reference(golden, k):
+    """Permutohedral::gradient: d/df a^T K b against the compiled reference
+    (within 1e-5 of the largest entry)"""
+    g = golden("kgrad_31x23.npz")
+    lat = gpu_lattice(g[f"feat{k}"])
+    df = lat.gradient(g[f"a{k}"], g[f"b{k}"])
+    print("max abs diff", float(np.abs(df - g[f"df{k}"]).max()), flush=True)
+    assert rel_err(df, g[f"df{k}"]) < 1e-5
+
+
+@pytest.mark.parametrize("k", [0, 1])
+def test_pairwise_kernel_gradient_vs_reference(golden, k):
+    """PairwisePotential::kernelGradient for DIAG and FULL kernels under the
+    four normalisations, and after setKernelParameters, against the compiled
+    reference (1e-3 of the largest entry: the parameter sums run over every
+    pixel, in FP64 here and in float in the reference)"""
+    import densecrf_ho_amd as D
+    g = golden("kgrad_31x23.npz")
+    f, b, q = g[f"feat{k}"], g[f"b{k}"], g[f"q{k}"]
+    for kt in (1, 2):
+        for nt in (0, 1, 2, 3):
+            print("kernel type", kt, "normalisation", nt, flush=True)
+            p = D.PairwisePotential(f, D.PottsCompatibility(3.0), kt, nt)
+            got = p.kernelGradient(b, q)
+            assert rel_err(got, g[f"kg{k}_{kt}_{nt}"]) < 1e-3
+    p = D.PairwisePotential(f, D.PottsCompatibility(3.0), D.DIAG_KERNEL, 3)
+    np.testing.assert_array_equal(p.kernelParameters(), np.ones(f.shape[1], np.float32))
+    p.setKernelParameters(g[f"kp_diag{k}"])
+    np.testing.assert_array_equal(p.kernelParameters(), g[f"kp_diag{k}"])
+    assert rel_err(p.kernelGradient(b, q), g[f"kgp{k}_1"]) < 1e-3
+    p = D.PairwisePotential(f, D.PottsCompatibility(3.0), D.FULL_KERNEL, 3)
+    p.setKernelParameters(g[f"kp_full{k}"])
+    assert rel_err(p.kernelGradient(b, q), g[f"kgp{k}_2"]) < 1e-3
+    c = D.PairwisePotential(f, D.PottsCompatibility(3.0))  # CONST_KERNEL: no parameters
+    assert c.kernelParameters().size == 0 and c.kernelGradient(b, q).size == 0

@@ -389,6 +389,27 @@ def test_brute_force_filters_closed_forms(golden):
     np.testing.assert_allclose(b, up, rtol=1e-5, atol=1e-5 * np.abs(up).max())
 
 
+@pytest.mark.parametrize("k", [0, 1])
+def test_lattice_gradient_matches_reference(golden, k):
+    """Permutohedral::gradient (src/permutohedral.cpp:1095-1178) bit for bit"""
+    g = golden("kgrad_31x23.npz")
+    lat = O.Lattice(g[f"feat{k}"])
+    np.testing.assert_array_equal(lat.gradient(g[f"a{k}"], g[f"b{k}"]), g[f"df{k}"])
+
+
+def test_lattice_gradient_is_bilinear_and_swaps_with_transpose(golden):
+    """d/df a^T K b is linear in a and in b, and grad(a, b) of K equals
+    grad(b, a) of the transposed filter (the two passes swap roles)"""
+    g = golden("kgrad_31x23.npz")
+    lat = O.Lattice(g["feat1"])
+    a, b = g["a1"], g["b1"]
+    a2 = np.random.default_rng(4).standard_normal(a.shape).astype(np.float32)
+    ga = lat.gradient(a, b) + lat.gradient(a2, b)
+    np.testing.assert_allclose(lat.gradient(a + a2, b), ga, rtol=1e-4, atol=1e-4 * np.abs(ga).max())
+    gb = lat.gradient(a, 2 * b)
+    np.testing.assert_allclose(gb, 2 * lat.gradient(a, b), rtol=1e-5, atol=1e-5 * np.abs(gb).max())
+
+
 def test_lp_less_confident_stop_matches_reference(golden):
     """less_confident_percent / confidence_tol (src/densecrf.cpp:457-462,
     784-794): four settings stop after 4 (no check), 1, 2 and 3 outer steps;
