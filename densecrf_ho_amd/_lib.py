@@ -56,6 +56,9 @@ SIGNATURES = {
     "dcrf_pairwise_set_kernel_parameters": (_i, [_vp, _vp, _i]),
     "dcrf_pairwise_kernel_gradient": (_i, [_vp, _vp, _vp, _i, _vp, _i]),
     "dcrf_crf_pairwise_kernel_gradient": (_i, [_vp, _i, _vp, _vp, _vp, _i]),
+    "dcrf_lattice_compute_upper_minus_lower_dc": (_i, [_vp, _vp, _i, _i, _i, _i, _i]),
+    "dcrf_pairwise_apply_upper_minus_lower_dc": (_i, [_vp, _vp, _vp, _i, _i]),
+    "dcrf_crf_apply_upper_minus_lower_dc": (_i, [_vp, _i, _vp, _vp, _i]),
     "dcrf_pairwise_apply_bf": (_i, [_vp, _vp, _vp, _i, _i]),
     "dcrf_pairwise_apply_upper_minus_lower_bf": (_i, [_vp, _vp, _vp, _i, _i]),
     "dcrf_pairwise_apply_upper_minus_lower_bf_ord": (_i, [_vp, _vp, _vp, _vp, _i, _i]),

@@ -324,6 +324,13 @@ class Permutohedral:
             self._h, o.ptr, a.ptr, ix.ptr, len(pI), e.ptr if e else None, rQ.shape[1], int(store), a.dev))
         return rout
 
+    def compute_upper_minus_lower_dc(self, out, low, middle_low, middle_high, high):
+        """Permutohedral::compute_upper_minus_lower_dc: one value per pixel, out (N,) added to."""
+        o = _Arg(out, np.float32, out=True)
+        _check(_lib.load().dcrf_lattice_compute_upper_minus_lower_dc(self._h, o.ptr, low, middle_low, middle_high,
+                                                                       high, o.dev))
+        return out
+
     def gradient(self, a, b):
         """Permutohedral::gradient(df, a, b, V): (N, d) = d/df a^T K b for a, b (N, V)."""
         x = _Arg(a, np.float32)
@@ -584,6 +591,15 @@ class DenseCRF:
         out = _empty_like_field(ind, (self.N, self.M)) if out is None else out
         o = _Arg(out, np.float32, out=True)
         _check(_lib.load().dcrf_crf_apply_upper_minus_lower_bf(self._h, k, ix.ptr, o.ptr, ix.dev))
+        return out
+
+    def apply_upper_minus_lower_dc(self, k, ind, out=None):
+        """PairwisePotential::apply_upper_minus_lower_dc of potential k (divide and
+        conquer over the sorted positions ind (N, M)); the result in sorted order."""
+        ix = _Arg(ind, np.int32)
+        out = _empty_like_field(ind, (self.N, self.M)) if out is None else out
+        o = _Arg(out, np.float32, out=True)
+        _check(_lib.load().dcrf_crf_apply_upper_minus_lower_dc(self._h, k, ix.ptr, o.ptr, ix.dev))
         return out
 
     def apply_upper_minus_lower_bf_ord(self, k, ind, Q, out=None):

@@ -277,6 +277,20 @@ int dcrf_lattice_compute_upper_minus_lower_ord_restricted(dcrf_lattice_t lat, fl
   });
 }
 
+int dcrf_lattice_compute_upper_minus_lower_dc(dcrf_lattice_t lat, float* out, int low, int middle_low,
+                                              int middle_high, int high, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(lat && out, "dcrf_lattice_compute_upper_minus_lower_dc: bad arguments");
+    const int N = lat->lat->N();
+    DCRF_REQUIRE(0 <= low && low <= middle_low && middle_low <= middle_high && middle_high <= high && high <= N,
+                 "dcrf_lattice_compute_upper_minus_lower_dc: ranges must satisfy 0 <= low <= middle_low <= "
+                 "middle_high <= high <= N");
+    Staged<float> o(out, N, on_device != 0, lat->s, true);  // added to
+    lat->lat->compute_upper_minus_lower_dc(o.d, low, middle_low, middle_high, high);
+    o.copy_out();
+    DCRF_HIP(hipStreamSynchronize(lat->s));
+  });
+}
 int dcrf_lattice_gradient(dcrf_lattice_t lat, float* df, const float* a, const float* b, int V, int on_device) {
   return guarded([&] {
     DCRF_REQUIRE(lat && df && a && b && V > 0, "dcrf_lattice_gradient: bad arguments");
@@ -359,16 +373,20 @@ int dcrf_pairwise_apply_upper_minus_lower_ord_restricted(dcrf_pairwise_t pw, flo
   });
 }
 
+// kind: 0 apply_bf(Q), 1 upper_minus_lower_bf(ind), 2 _bf_ord(ind, Q), 3 _dc(ind)
 static void apply_bf_common(Potential& p, hipStream_t s, int kind, const int* ind, const float* Q, float* out, int L,
                             int on_device) {
   const int N = p.lat->N();
   const size_t n = (size_t)N * L;
+  const bool uses_q = kind == 0 || kind == 2;
   Staged<int> ix(kind ? ind : nullptr, kind ? n : 0, on_device != 0, s, kind != 0);
-  Staged<float> q(kind != 1 ? Q : nullptr, kind != 1 ? n : 0, on_device != 0, s, kind != 1);
+  Staged<float> q(uses_q ? Q : nullptr, uses_q ? n : 0, on_device != 0, s, uses_q);
   Staged<float> o(out, n, on_device != 0, s, false);
   if (kind) check_index_matrix(ix.d, n, N, s);
   if (kind == 0)
     potential_apply_bf(p, q.d, o.d, L);
+  else if (kind == 3)
+    potential_apply_uml_dc(p, ix.d, o.d, L);
   else
     potential_apply_uml_bf(p, ix.d, q.d, o.d, L, kind == 2);
   o.copy_out();
@@ -384,6 +402,12 @@ int dcrf_pairwise_apply_upper_minus_lower_bf(dcrf_pairwise_t pw, const int* ind,
   return guarded([&] {
     DCRF_REQUIRE(pw && ind && out && L > 0, "dcrf_pairwise_apply_upper_minus_lower_bf: bad arguments");
     apply_bf_common(pw->p, pw->s, 1, ind, nullptr, out, L, on_device);
+  });
+}
+int dcrf_pairwise_apply_upper_minus_lower_dc(dcrf_pairwise_t pw, const int* ind, float* out, int L, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(pw && ind && out && L > 0, "dcrf_pairwise_apply_upper_minus_lower_dc: bad arguments");
+    apply_bf_common(pw->p, pw->s, 3, ind, nullptr, out, L, on_device);
   });
 }
 int dcrf_pairwise_apply_upper_minus_lower_bf_ord(dcrf_pairwise_t pw, const int* ind, const float* Q, float* out,
@@ -622,6 +646,14 @@ int dcrf_crf_apply_upper_minus_lower_bf(dcrf_crf_t crf, int k, const int* ind, f
     CRF& c = *crf->crf;
     DCRF_REQUIRE(k >= 0 && k < c.num_pairwise(), "dcrf_crf_apply_upper_minus_lower_bf: no such potential");
     apply_bf_common(c.pairwise(k), c.stream(), 1, ind, nullptr, out, c.L(), on_device);
+  });
+}
+int dcrf_crf_apply_upper_minus_lower_dc(dcrf_crf_t crf, int k, const int* ind, float* out, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && ind && out, "dcrf_crf_apply_upper_minus_lower_dc: bad arguments");
+    CRF& c = *crf->crf;
+    DCRF_REQUIRE(k >= 0 && k < c.num_pairwise(), "dcrf_crf_apply_upper_minus_lower_dc: no such potential");
+    apply_bf_common(c.pairwise(k), c.stream(), 3, ind, nullptr, out, c.L(), on_device);
   });
 }
 int dcrf_crf_apply_upper_minus_lower_bf_ord(dcrf_crf_t crf, int k, const int* ind, const float* Q, float* out,

@@ -102,6 +102,9 @@ void potential_set_kernel_parameters(Potential& p, const float* v, int n, hipStr
 std::vector<float> potential_kernel_parameters(const Potential& p);
 // PairwisePotential::kernelGradient(b, Q) (src/pairwise.cpp:402-407); host out
 void potential_pairwise_kernel_gradient(Potential& p, const float* b, const float* Q, int L, float* host_out);
+// PairwisePotential::apply_upper_minus_lower_dc (src/pairwise.cpp:241-293);
+// ind N x L (checked by the caller), out N x L in sorted order
+void potential_apply_uml_dc(Potential& p, const int* ind, float* out, int L);
 // LabelCompatibility::apply of a Diagonal or Matrix compatibility on nrows
 // label-contiguous rows: out = C in (the Potts form is a scale by -w)
 void compat_apply_rows(Potential& p, const float* in, float* out, long nrows, int V, hipStream_t s);

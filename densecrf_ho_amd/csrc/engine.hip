@@ -6,6 +6,7 @@
 #include "ho.hip"
 #include "lp.hip"
 #include "bf.hip"
+#include "dc.hip"
 #include "kgrad.hip"
 #include "adjoint.hip"
 #include "batch.hip"

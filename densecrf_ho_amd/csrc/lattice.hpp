@@ -17,6 +17,7 @@
 namespace dcrf {
 
 struct RestrictedState;
+struct DcScratch;
 
 // Input of a splat: value `k` of pixel `pix`.  Concrete forms are defined in
 // lattice.hip; the enum selects the one used by a launch.
@@ -114,6 +115,13 @@ class Lattice {
   // pixels and the active points for later calls without it.  Adds into out
   // (nI x V).  Not re-entrant: the state lives in the lattice.
   void ordered_restricted(const float* rq, const int* pI, int nI, const float* ext, int V, bool store, float* out);
+
+  // Permutohedral::compute_upper_minus_lower_dc (src/permutohedral.cpp:881-997)
+  // on one value per pixel: out (N, device) += the upper pass of [mh, high)
+  // into [low, ml), -= the lower pass of [low, ml) into [mh, high).
+  void compute_upper_minus_lower_dc(float* out, int low, int ml, int mh, int high);
+  // the same step with caller-held scratch (zero outside a step), enqueued only
+  void dc_step(float* out, int low, int ml, int mh, int high, DcScratch& sc);
 
   // Permutohedral::gradient (src/permutohedral.cpp:1095-1178): df (N x d,
   // device) = d/df a^T K b for a, b N x V.

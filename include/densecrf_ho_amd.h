@@ -87,6 +87,13 @@ int dcrf_lattice_compute_upper_minus_lower_ord(dcrf_lattice_t lat, const float* 
 /* Permutohedral::gradient(float* df, const float* a, const float* b, int value_size),
  * include/permutohedral.h:87, src/permutohedral.cpp:1095-1178: df (d x N) = d/df of
  * a^T K b for a, b (V x N), K the unnormalised lattice filter. */
+/* Permutohedral::compute_upper_minus_lower_dc(MatrixXf& out, int low, int middle_low,
+ * int middle_high, int high), include/permutohedral.h:82, src/permutohedral.cpp:881-997
+ * and 1066-1070: one value per pixel; out (N) gets += the pass of [middle_high, high)
+ * sliced into [low, middle_low) and -= the pass of [low, middle_low) sliced into
+ * [middle_high, high), blurred over those pixels' lattice points only. */
+int dcrf_lattice_compute_upper_minus_lower_dc(dcrf_lattice_t lat, float* out, int low, int middle_low,
+                                              int middle_high, int high, int on_device);
 int dcrf_lattice_gradient(dcrf_lattice_t lat, float* df, const float* a, const float* b, int V, int on_device);
 /* Permutohedral::compute_upper_minus_lower_ord_restricted(MatrixXf& rout, const MatrixXf& rQ,
  * const std::vector<int>& pI, const MatrixXf& Q, bool store), include/permutohedral.h:84-85,
@@ -148,6 +155,12 @@ int dcrf_pairwise_apply_upper_minus_lower_bf(dcrf_pairwise_t pw, const int* ind,
 /* PairwisePotential::apply_upper_minus_lower_bf_ord(out, ind, Q), include/pairwise.h:77,
  * src/pairwise.cpp:321-353: as above with the split by value, out(l, c) =
  * sum_{Q_b <= Q_c} K - sum_{Q_b >= Q_c} K. */
+/* PairwisePotential::apply_upper_minus_lower_dc(out, ind), include/pairwise.h:75,
+ * src/pairwise.cpp:241-293: per label a lattice on the features in the order ind and
+ * the divide-and-conquer over the sorted positions (blocks of <= 30 by brute force with
+ * alpha 1/0.6, larger ones halved with the lattice step between the halves); out (L x N)
+ * in the sorted order, then the compatibility. */
+int dcrf_pairwise_apply_upper_minus_lower_dc(dcrf_pairwise_t pw, const int* ind, float* out, int L, int on_device);
 int dcrf_pairwise_apply_upper_minus_lower_bf_ord(dcrf_pairwise_t pw, const int* ind, const float* Q, float* out,
                                                  int L, int on_device);
 int dcrf_pairwise_apply_upper_minus_lower_ord_restricted(dcrf_pairwise_t pw, float* rout, const float* rQ,
@@ -214,6 +227,7 @@ int dcrf_crf_pairwise_kernel_gradient(dcrf_crf_t crf, int k, const float* b, con
  * layouts as the dcrf_pairwise_ forms, L x N fields. */
 int dcrf_crf_apply_bf(dcrf_crf_t crf, int k, const float* Q, float* out, int on_device);
 int dcrf_crf_apply_upper_minus_lower_bf(dcrf_crf_t crf, int k, const int* ind, float* out, int on_device);
+int dcrf_crf_apply_upper_minus_lower_dc(dcrf_crf_t crf, int k, const int* ind, float* out, int on_device);
 int dcrf_crf_apply_upper_minus_lower_bf_ord(dcrf_crf_t crf, int k, const int* ind, const float* Q, float* out,
                                             int on_device);
 int dcrf_crf_apply_upper_minus_lower_ord_restricted(dcrf_crf_t crf, int k, float* rout, const float* rQ,

@@ -216,6 +216,9 @@ class Permutohedral {
     if (out.rows() != Q.rows() || out.cols() != Q.cols()) out.resize(Q.rows(), Q.cols());
     check(dcrf_lattice_compute_upper_minus_lower_ord(h_, Q.data(), out.data(), (int)Q.rows(), 0));
   }
+  void compute_upper_minus_lower_dc(MatrixXf& out, int low, int middle_low, int middle_high, int high) const {
+    check(dcrf_lattice_compute_upper_minus_lower_dc(h_, out.data(), low, middle_low, middle_high, high, 0));
+  }
   // df (d x N, column-major like the reference's features) of a^T K b
   void gradient(float* df, const float* a, const float* b, int value_size) const {
     check(dcrf_lattice_gradient(h_, df, a, b, value_size, 0));
@@ -297,6 +300,10 @@ class PairwisePotential {
   void apply_upper_minus_lower_bf(MatrixXf& out, const MatrixXi& ind) const {
     if (out.rows() != ind.rows() || out.cols() != ind.cols()) out.resize(ind.rows(), ind.cols());
     check(dcrf_pairwise_apply_upper_minus_lower_bf(h_, ind.data(), out.data(), (int)ind.rows(), 0));
+  }
+  void apply_upper_minus_lower_dc(MatrixXf& out, const MatrixXi& ind) const {
+    if (out.rows() != ind.rows() || out.cols() != ind.cols()) out.resize(ind.rows(), ind.cols());
+    check(dcrf_pairwise_apply_upper_minus_lower_dc(h_, ind.data(), out.data(), (int)ind.rows(), 0));
   }
   void apply_upper_minus_lower_bf_ord(MatrixXf& out, const MatrixXi& ind, const MatrixXf& Q) const {
     if (out.rows() != Q.rows() || out.cols() != Q.cols()) out.resize(Q.rows(), Q.cols());

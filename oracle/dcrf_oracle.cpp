@@ -708,6 +708,110 @@ static void bf_uml(const float* f, int N, int d, const int* ind, const float* Q,
     }
 }
 
+// PairwisePotential::apply_upper_minus_lower_dc (src/pairwise.cpp:241-293)
+// before the compatibility: per label, the features in the order ind(l, .),
+// a lattice on them, and the divide-and-conquer of
+// apply_upper_minus_lower_sorted_slice: blocks of at most 30 pixels by brute
+// force (alpha 1/0.6 in double), larger ones split in two halves (sharing the
+// middle pixel when odd) plus Permutohedral::compute_upper_minus_lower_dc
+// (src/permutohedral.cpp:881-997) between them.  out (N x L) stays in the
+// sorted order, as in the reference.
+struct DcLattice {
+  const Lattice& lat;
+  std::vector<float> values, nv;
+  std::vector<char> act;
+  explicit DcLattice(const Lattice& l) : lat(l), values(l.M + 2), nv(l.M + 2), act(l.M) {}
+  // compute_upper_minus_lower_dc(out, low, middle_low, middle_high, high)
+  void dc(float* out, int low, int ml, int mh, int high) {
+    const int d = lat.d, d1 = d + 1;
+    std::fill(values.begin(), values.end(), 0.f);
+    std::fill(nv.begin(), nv.end(), 0.f);
+    std::fill(act.begin(), act.end(), 0);
+    std::vector<int> list;
+    for (int i = mh; i < high; ++i)
+      for (int j = 0; j <= d; ++j) {
+        const int o = lat.offset[(size_t)i * d1 + j];
+        values[o + 1] += lat.bary[(size_t)i * d1 + j];
+        if (!act[o]) {
+          act[o] = 1;
+          list.push_back(o);
+        }
+      }
+    for (int i = low; i < ml; ++i)
+      for (int j = 0; j <= d; ++j) {
+        const int o = lat.offset[(size_t)i * d1 + j];
+        if (!act[o]) {
+          act[o] = 1;
+          list.push_back(o);
+        }
+      }
+    const float alpha = 1.0f / (1 + std::pow(2.0f, (float)-d));
+    auto blur = [&]() -> float* {
+      float* cur = values.data();
+      float* nxt = nv.data();
+      for (int j = 0; j <= d; ++j) {
+        for (int i : list) {
+          const int n1 = lat.nbr[((size_t)j * lat.M + i) * 2], n2 = lat.nbr[((size_t)j * lat.M + i) * 2 + 1];
+          nxt[i + 1] = (float)(cur[i + 1] + 0.5 * (cur[n1 + 1] + cur[n2 + 1]));
+        }
+        std::swap(cur, nxt);
+      }
+      return cur;
+    };
+    float* v = blur();
+    for (int i = low; i < ml; ++i)
+      for (int j = 0; j <= d; ++j) {
+        const int o = lat.offset[(size_t)i * d1 + j] + 1;
+        out[i] += lat.bary[(size_t)i * d1 + j] * v[o] * alpha;
+      }
+    std::fill(values.begin(), values.end(), 0.f);
+    std::fill(nv.begin(), nv.end(), 0.f);
+    for (int i = low; i < ml; ++i)
+      for (int j = 0; j <= d; ++j) values[lat.offset[(size_t)i * d1 + j] + 1] += lat.bary[(size_t)i * d1 + j];
+    v = blur();
+    for (int i = mh; i < high; ++i)
+      for (int j = 0; j <= d; ++j) {
+        const int o = lat.offset[(size_t)i * d1 + j] + 1;
+        out[i] -= lat.bary[(size_t)i * d1 + j] * v[o] * alpha;
+      }
+  }
+};
+static void dc_sorted_slice(DcLattice& L, const float* fs, int d, float* out, int mn, int mx) {
+  const int size = mx - mn;
+  if (size <= 30) {
+    const double alpha = 1.0 / 0.6;
+    for (int c = mn; c < mx; ++c)
+      for (int b = mn; b < c; ++b) out[c] = (float)(out[c] - bf_kernel(&fs[(size_t)c * d], &fs[(size_t)b * d], d) * alpha);
+    for (int c = mn; c < mx; ++c)
+      for (int b = c + 1; b < mx; ++b)
+        out[c] = (float)(out[c] + bf_kernel(&fs[(size_t)c * d], &fs[(size_t)b * d], d) * alpha);
+    return;
+  }
+  int ml, mh;
+  if (size % 2 == 0) {
+    ml = mh = mn + size / 2;
+  } else {
+    ml = (int)std::floor(mn + size / 2.0);
+    mh = ml + 1;
+  }
+  dc_sorted_slice(L, fs, d, out, mn, mh);
+  dc_sorted_slice(L, fs, d, out, ml, mx);
+  L.dc(out, mn, ml, mh, mx);
+}
+static void uml_dc(const float* f, int N, int d, const int* ind, int Lb, float* out) {
+  std::vector<float> fs((size_t)N * d), col(N);
+  for (int l = 0; l < Lb; ++l) {
+    for (int j = 0; j < N; ++j)
+      for (int k = 0; k < d; ++k) fs[(size_t)j * d + k] = f[(size_t)ind[(size_t)j * Lb + l] * d + k];
+    Lattice lat;
+    lat.init(fs.data(), N, d);
+    DcLattice dl(lat);
+    std::fill(col.begin(), col.end(), 0.f);
+    dc_sorted_slice(dl, fs.data(), d, col.data(), 0, N);
+    for (int j = 0; j < N; ++j) out[(size_t)j * Lb + l] = col[j];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Dense CRF with Potts pairwise terms and Pn superpixel layers.
 // ---------------------------------------------------------------------------
@@ -1494,6 +1598,11 @@ void orc_ordered_d(void* h, const float* q, double* out, int V) { ((Lattice*)h)-
 void orc_lattice_gradient(void* h, const float* a, const float* b, int V, float* df) {
   ((Lattice*)h)->gradient(a, b, V, df);
 }
+void orc_lattice_dc(void* h, float* out, int low, int ml, int mh, int high) {
+  orc::DcLattice dl(*(Lattice*)h);
+  dl.dc(out, low, ml, mh, high);
+}
+void orc_uml_dc(const float* f, int N, int d, const int* ind, int L, float* out) { orc::uml_dc(f, N, d, ind, L, out); }
 void orc_bf_apply(const float* f, int N, int d, const float* Q, int L, float* out) { orc::bf_apply(f, N, d, Q, L, out); }
 void orc_bf_uml(const float* f, int N, int d, const int* ind, const float* Q, int L, float* out) {
   orc::bf_uml(f, N, d, ind, Q, L, out);

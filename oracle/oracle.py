@@ -62,6 +62,8 @@ def lib():
         _sig(L, "orc_ordered_d", None, _vp, _f32p, _f64p, _i)
         _sig(L, "orc_bf_apply", None, _f32p, _i, _i, _f32p, _i, _f32p)
         _sig(L, "orc_lattice_gradient", None, _vp, _f32p, _f32p, _i, _f32p)
+        _sig(L, "orc_uml_dc", None, _f32p, _i, _i, _i32p, _i, _f32p)
+        _sig(L, "orc_lattice_dc", None, _vp, _f32p, _i, _i, _i, _i)
         _sig(L, "orc_bf_uml", None, _f32p, _i, _i, _i32p, _vp, _i, _f32p)
         _sig(L, "orc_ordered_restricted", _i, _vp, _f32p, _i32p, _i, _f32p, _i, _i, _f32p)
         _sig(L, "orc_crf_new", _vp, _i, _i)
@@ -150,6 +152,12 @@ class Lattice:
         (self._L.orc_ordered_d if fp64 else self._L.orc_ordered_f)(self.h, q, out, V)
         return out
 
+    def dc(self, out, low, ml, mh, high):
+        """compute_upper_minus_lower_dc on one value per pixel; out (N,) added to"""
+        out = _c(out, np.float32).copy()
+        self._L.orc_lattice_dc(self.h, out, low, ml, mh, high)
+        return out
+
     def gradient(self, a, b):
         """Permutohedral::gradient: (N, d) = d/df a^T K b, a and b (N, V)"""
         a = _c(a, np.float32)
@@ -189,6 +197,16 @@ def bf_uml(feat, ind, Q=None):
     q = None if Q is None else _c(Q, np.float32)
     lib().orc_bf_uml(feat, feat.shape[0], feat.shape[1], ind, None if q is None else q.ctypes.data_as(C.c_void_p),
                      ind.shape[1], out)
+    return out
+
+
+def uml_dc(feat, ind):
+    """apply_upper_minus_lower_dc before the compatibility (src/pairwise.cpp:241-293);
+    ind (N, L); the result in sorted order"""
+    feat = _c(feat, np.float32)
+    ind = _c(ind, np.int32)
+    out = np.zeros(ind.shape, np.float32)
+    lib().orc_uml_dc(feat, feat.shape[0], feat.shape[1], ind, ind.shape[1], out)
     return out
 
 
@@ -426,6 +444,7 @@ def reflib():
         _sig(R, "ref_lattice_ordered", None, _vp, _f32p, _f32p, _i, _i)
         _sig(R, "ref_probimage", None, C.c_char_p, _vp, _i32p)
         _sig(R, "ref_lattice_gradient", None, _vp, _f32p, _f32p, _i, _i, _f32p)
+        _sig(R, "ref_lattice_dc", None, _vp, _f32p, _i, _i, _i, _i, _i)
         _sig(R, "ref_pairwise_kernel_gradient", _i, _f32p, _i, _i, _f, _i, _i, _vp, _i, _f32p, _f32p, _i, _f32p)
         _sig(R, "ref_crf_add_superpixel", _i, _vp, _vp, _i, _i, _i, _f, _f)
         _sig(R, "ref_crf_layers", None, _vp, _i32p, _f32p, _f32p)
@@ -522,6 +541,11 @@ class RefLattice:
         self._R.ref_lattice_ordered(self.h, q, out, q.shape[1], self.N)
         return out
 
+    def dc(self, out, low, ml, mh, high):
+        out = _c(out, np.float32).copy()
+        self._R.ref_lattice_dc(self.h, out, self.N, low, ml, mh, high)
+        return out
+
     def gradient(self, a, b):
         """Permutohedral::gradient: (N, d) of a^T K b"""
         a = _c(a, np.float32)
@@ -610,7 +634,7 @@ class RefCRF:
         return reg, e, c
 
     def apply_bf(self, k, which, ind=None, Q=None):
-        """0 apply_bf(Q), 1 apply_upper_minus_lower_bf(ind), 2 _bf_ord(ind, Q)"""
+        """0 apply_bf(Q), 1 apply_upper_minus_lower_bf(ind), 2 _bf_ord(ind, Q), 3 _dc(ind)"""
         out = np.zeros((self.N, self.L), np.float32)
         ind = None if ind is None else _c(ind, np.int32)
         Q = None if Q is None else _c(Q, np.float32)

@@ -142,6 +142,12 @@ void ref_probimage(const char* path, float* probs, int* dims) {
         for (int k = 0; k < p.depth(); ++k) probs[((size_t)y * p.width() + x) * p.depth() + k] = p(x, y, k);
 }
 
+// Permutohedral::compute_upper_minus_lower_dc on one value per pixel, out (N) added to
+void ref_lattice_dc(void* h, float* out, int N, int low, int ml, int mh, int high) {
+  MatrixXf o = as_mat(out, 1, N);
+  ((RefLattice*)h)->compute_upper_minus_lower_dc(o, low, ml, mh, high);
+  std::copy(o.data(), o.data() + N, out);
+}
 // Permutohedral::gradient: df (d x N) of a^T K b, a and b V x N
 void ref_lattice_gradient(void* h, const float* a, const float* b, int V, int N, float* df) {
   ((RefLattice*)h)->gradient(df, a, b, V);
@@ -227,7 +233,8 @@ void ref_crf_apply(void* h, int k, const float* in, float* out, int which) {
   std::copy(o.data(), o.data() + o.size(), out);
 }
 // brute-force filters of potential k: 0 apply_bf(Q), 1 apply_upper_minus_lower_bf(ind),
-// 2 apply_upper_minus_lower_bf_ord(ind, Q); ind is L x N column-major
+// 2 apply_upper_minus_lower_bf_ord(ind, Q), 3 apply_upper_minus_lower_dc(ind); ind is
+// L x N column-major
 void ref_crf_apply_bf(void* h, int k, int which, const int* ind, const float* q, float* out) {
   RefCRF* c = (RefCRF*)h;
   const int L = c->L(), N = c->N();
@@ -237,7 +244,8 @@ void ref_crf_apply_bf(void* h, int k, int which, const int* ind, const float* q,
   o.fill(0);
   if (which == 0) c->pw(k)->apply_bf(o, Q);
   else if (which == 1) c->pw(k)->apply_upper_minus_lower_bf(o, I);
-  else c->pw(k)->apply_upper_minus_lower_bf_ord(o, I, Q);
+  else if (which == 2) c->pw(k)->apply_upper_minus_lower_bf_ord(o, I, Q);
+  else c->pw(k)->apply_upper_minus_lower_dc(o, I);
   std::copy(o.data(), o.data() + o.size(), out);
 }
 int ref_crf_apply_ord(void* h, int k, const float* q, float* out) {

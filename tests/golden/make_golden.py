@@ -11,6 +11,7 @@ compressed .npz files.  Run from the repository root:
     python tests/golden/make_golden.py bf       # only bf_12x9_L4.npz
     python tests/golden/make_golden.py data     # only data_img.json / data_img_labels.npz
     python tests/golden/make_golden.py kgrad    # only kgrad_31x23.npz
+    python tests/golden/make_golden.py dc       # only dc_L4.npz
 """
 
 from __future__ import annotations
@@ -384,7 +385,44 @@ def kgrad_case():
     print("kgrad_31x23")
 
 
+def dc_case():
+    """the divide-and-conquer filter: PairwisePotential::apply_upper_minus_lower_dc
+    (src/pairwise.cpp:241-293) on 12x9 and 31x23 images, 4 labels, random
+    orders, both kernels; Permutohedral::compute_upper_minus_lower_dc
+    (src/permutohedral.cpp:881-997) for even and odd splits added to a
+    nonzero row"""
+    d = {}
+    for W, H in [(12, 9), (31, 23)]:
+        L = 4
+        wl = S.Workload(W, H, 3, 1, L=L)
+        r = O.RefCRF(W, H, L)
+        r.set_unary(wl.U)
+        r.add_gaussian(3, 3, 3)
+        r.add_bilateral(50, 50, 15, 15, 15, wl.img, 5)
+        rng = np.random.default_rng(W)
+        ind = np.stack([rng.permutation(wl.N) for _ in range(L)], 1).astype(np.int32)
+        tag = f"{W}x{H}"
+        d[f"U_{tag}"] = wl.U
+        d[f"img_{tag}"] = wl.img
+        d[f"ind_{tag}"] = ind
+        for k in range(2):
+            f = r.features(k, 2 if k == 0 else 5)
+            d[f"feat{k}_{tag}"] = f
+            d[f"dc{k}_{tag}"] = r.apply_bf(k, 3, ind=ind)
+            lat = O.RefLattice(f)
+            out0 = rng.standard_normal(wl.N).astype(np.float32)
+            d[f"row0_{k}_{tag}"] = out0
+            N = wl.N
+            d[f"lat_even{k}_{tag}"] = lat.dc(out0, 0, N // 2, N // 2, N)
+            d[f"lat_odd{k}_{tag}"] = lat.dc(out0, 3, (N - 3) // 2 + 3, (N - 3) // 2 + 4, N - 2)
+    np.savez_compressed(os.path.join(OUT, "dc_L4.npz"), **d)
+    print("dc_L4")
+
+
 def main():
+    if sys.argv[1:] == ["dc"]:
+        dc_case()
+        return
     if sys.argv[1:] == ["kgrad"]:
         kgrad_case()
         return
@@ -419,6 +457,7 @@ def main():
     restricted_case()
     bf_case()
     kgrad_case()
+    dc_case()
     data_case()
     c0_case()
 

@@ -5,11 +5,13 @@ compiled reference's, and the saved MSRC map's labels equal to the
 reference's on >= 99.9% of the pixels.  The superpixel methods take the
 reference's own mean-shift segmentation as a region map.
 
-Energy bars: 1e-4 relative for unary, qp and qp_sp.  The LP methods run ten
-proximal steps from the unaries on a real image, and their final energy
-follows float rounding: on this input the FP64 oracle's LP ends 3.2e-3 from
-the float reference and the float oracle 1.6e-4 from it, so the LP bar is
-1e-3 relative (labels still >= 99.9%)."""
+Energy bars: 1e-4 relative for unary, qp and qp_sp, with >= 99.9% of the
+labels equal.  The LP methods run ten proximal steps from the unaries on a
+real image, and their final energy follows float rounding: the reference's
+own LP, started from fields one ulp from its unary_init, ends at different
+energies (data_img.json records the spread of those runs and their lowest
+label agreement with the unperturbed run).  The LP methods are held to that
+spread, widened by 1e-3, and to that agreement less 0.5%."""
 
 import json
 import os
@@ -47,10 +49,16 @@ def test_example_inference_on_reference_data(tmp_path, method):
     e = float(line.split("\t")[1])
     er = ref[f"e_{method}"]
     print("energy", e, "reference", er, flush=True)
-    assert abs(e - er) <= (1e-3 if method.startswith("lp") else 1e-4) * abs(er)
+    lp = method.startswith("lp")
+    if lp:
+        lo, hi = ref[f"e_{method}_spread"]
+        print("reference spread", lo, hi, flush=True)
+        assert lo * (1 - 1e-3) <= e <= hi * (1 + 1e-3)
+    else:
+        assert abs(e - er) <= 1e-4 * abs(er)
     img = _read_png(out)
     W, H = ref["W"], ref["H"]
     want = MSRC[g[method]].reshape(H, W, 3)
     agree = float((img == want).all(-1).mean())
     print("label agreement", agree, flush=True)
-    assert agree >= 0.999
+    assert agree >= (ref[f"agree_{method}_min"] - 0.005 if lp else 0.999)

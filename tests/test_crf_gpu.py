@@ -219,3 +219,43 @@ def test_brute_force_filters_vs_oracle_larger():
     bad[3, 2] = wl.N
     with pytest.raises(D.EngineError, match="out of range"):
         c.apply_upper_minus_lower_bf(0, bad)
+
+
+@pytest.mark.parametrize("tag", ["12x9", "31x23"])
+def test_divide_and_conquer_filter_vs_reference(golden, tag):
+    """apply_upper_minus_lower_dc of both potentials and the lattice step
+    compute_upper_minus_lower_dc (even and odd split) against the compiled
+    reference (within 1e-5 of the largest value)"""
+    import densecrf_ho_amd as D
+    g = golden("dc_L4.npz")
+    W, H = (int(x) for x in tag.split("x"))
+    c = D.DenseCRF2D(W, H, 4)
+    c.setUnaryEnergy(g[f"U_{tag}"])
+    c.addPairwiseEnergy(g[f"feat0_{tag}"], D.PottsCompatibility(3.0))
+    c.addPairwiseEnergy(g[f"feat1_{tag}"], D.PottsCompatibility(5.0))
+    for k in range(2):
+        print("potential", k, flush=True)
+        out = c.apply_upper_minus_lower_dc(k, g[f"ind_{tag}"])
+        assert rel_err(out, g[f"dc{k}_{tag}"]) < 1e-5
+        lat = D.Permutohedral().init(g[f"feat{k}_{tag}"])
+        N = lat.N
+        row = g[f"row0_{k}_{tag}"]
+        a = lat.compute_upper_minus_lower_dc(row.copy(), 0, N // 2, N // 2, N)
+        assert rel_err(a, g[f"lat_even{k}_{tag}"]) < 1e-5
+        b = lat.compute_upper_minus_lower_dc(row.copy(), 3, (N - 3) // 2 + 3, (N - 3) // 2 + 4, N - 2)
+        assert rel_err(b, g[f"lat_odd{k}_{tag}"]) < 1e-5
+
+
+def test_divide_and_conquer_filter_vs_oracle_larger():
+    """80x60 image (4800 pixels: nodes of up to 2400 pixels in one
+    workgroup), 21 labels, orders sorted by random values"""
+    import densecrf_ho_amd as D
+    from densecrf_ho_amd import synthetic as S
+    wl = S.Workload(80, 60, 10, 5, L=21)
+    c = D.DenseCRF2D(80, 60, 21)
+    c.setUnaryEnergy(wl.U)
+    f = wl.kernels[1][0]
+    c.addPairwiseEnergy(f, D.PottsCompatibility(5.0))
+    ind = np.argsort(np.random.default_rng(6).random((wl.N, 21)), axis=0).astype(np.int32)
+    out = c.apply_upper_minus_lower_dc(0, ind)
+    assert rel_err(out, np.float32(-5.0) * O.uml_dc(f, ind)) < 1e-5

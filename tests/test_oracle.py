@@ -28,6 +28,26 @@ How each oracle function is pinned:
       textbook simplex projection; renormalize; ucondgrad (planted 8 fails).
   energies (assignment true / higher order, LP energy, QP value)
       fixtures (exact doubles).
+  LP less-confident stop (less_confident_percent, confidence_tol)
+      fixtures of four settings that stop after 4, 1, 2 and 3 outer steps
+      (bit-exact final Q, pairwise different).
+  restricted ordered filter (store / reuse / another pixel set, accumulation)
+      fixtures at d=2 and d=5 with values at bin edges (bit-exact); planted
+      mistake 9 (blur every point) fails; with every pixel in pI it equals
+      the ordered filter exactly.
+  brute-force filters (apply_bf, upper_minus_lower_bf, _bf_ord)
+      fixtures (bit-exact, including ties in the value split), the dense
+      FP64 product K Q, the closed form of the position split, and
+      bf_ord = -bf for the ascending order of distinct values.
+  lattice feature gradient (Permutohedral::gradient)
+      fixtures at d=2 and d=5 (bit-exact), bilinearity.
+  divide-and-conquer filter (apply_upper_minus_lower_dc, compute_..._dc)
+      fixtures at 12x9 and 31x23, both kernels, even and odd splits
+      (bit-exact).
+
+Not restated in the oracle (the GPU is checked against reference fixtures
+directly): DenseKernel::featureGradient / gradient and kernel parameters
+(kgrad_31x23.npz), the file formats and addSuperPixel terms (test_io.py).
 
 The compiled reference (oracle/_ref) is used in addition where it was built.
 """
@@ -408,6 +428,23 @@ def test_lattice_gradient_is_bilinear_and_swaps_with_transpose(golden):
     np.testing.assert_allclose(lat.gradient(a + a2, b), ga, rtol=1e-4, atol=1e-4 * np.abs(ga).max())
     gb = lat.gradient(a, 2 * b)
     np.testing.assert_allclose(gb, 2 * lat.gradient(a, b), rtol=1e-5, atol=1e-5 * np.abs(gb).max())
+
+
+@pytest.mark.parametrize("tag", ["12x9", "31x23"])
+@pytest.mark.parametrize("k", [0, 1])
+def test_divide_and_conquer_filter_matches_reference(golden, tag, k):
+    """apply_upper_minus_lower_dc (per-label sorted lattices, brute-force
+    leaves, lattice steps) and compute_upper_minus_lower_dc for an even and
+    an odd split, bit for bit"""
+    g = golden("dc_L4.npz")
+    w = np.float32(3.0 if k == 0 else 5.0)
+    f = g[f"feat{k}_{tag}"]
+    np.testing.assert_array_equal(-w * O.uml_dc(f, g[f"ind_{tag}"]), g[f"dc{k}_{tag}"])
+    lat = O.Lattice(f)
+    N = lat.N
+    row = g[f"row0_{k}_{tag}"]
+    np.testing.assert_array_equal(lat.dc(row, 0, N // 2, N // 2, N), g[f"lat_even{k}_{tag}"])
+    np.testing.assert_array_equal(lat.dc(row, 3, (N - 3) // 2 + 3, (N - 3) // 2 + 4, N - 2), g[f"lat_odd{k}_{tag}"])
 
 
 def test_lp_less_confident_stop_matches_reference(golden):
