@@ -396,6 +396,33 @@ class PairwisePotential:
     def apply_upper_minus_lower_ord(self, Q):
         return self._field(Q, _lib.load().dcrf_pairwise_apply_upper_minus_lower_ord)
 
+    def apply_upper_minus_lower_ord_restricted(self, rQ, pI, Q=None, store: bool = True):
+        """(len(pI), L) = -w * restricted ordered filter (src/pairwise.cpp:229-239)."""
+        a, ix, e, o, out = _restricted_args(rQ, pI, Q, store, None, tuple(rQ.shape))
+        _check(_lib.load().dcrf_pairwise_apply_upper_minus_lower_ord_restricted(
+            self._h, o.ptr, a.ptr, ix.ptr, len(pI), e.ptr if e else None, rQ.shape[1], int(store), a.dev))
+        return out
+
+    def apply_bf(self, Q):
+        return self._field(Q, _lib.load().dcrf_pairwise_apply_bf)
+
+    def _sorted(self, fn, ind, *q):
+        ix = _Arg(ind, np.int32)
+        out = _empty_like_field(ind, tuple(ind.shape))
+        o = _Arg(out, np.float32, out=True)
+        qa = [_Arg(x, np.float32).ptr for x in q]
+        _check(fn(self._h, ix.ptr, *qa, o.ptr, ind.shape[1], ix.dev))
+        return out
+
+    def apply_upper_minus_lower_bf(self, ind):
+        return self._sorted(_lib.load().dcrf_pairwise_apply_upper_minus_lower_bf, ind)
+
+    def apply_upper_minus_lower_bf_ord(self, ind, Q):
+        return self._sorted(_lib.load().dcrf_pairwise_apply_upper_minus_lower_bf_ord, ind, Q)
+
+    def apply_upper_minus_lower_dc(self, ind):
+        return self._sorted(_lib.load().dcrf_pairwise_apply_upper_minus_lower_dc, ind)
+
     def kernelParameters(self):
         n = _lib.load().dcrf_pairwise_kernel_nparams(self._h)
         v = np.zeros(max(n, 1), np.float32)
@@ -718,6 +745,13 @@ class DenseCRF2D(DenseCRF):
     def __init__(self, W: int, H: int, M: int):
         super().__init__(W * H, M)
         self.W, self.H = int(W), int(H)
+
+    def addSuperPixel(self, img, region, constant: float = 50.0, normaliser: float = 1000.0):
+        """DenseCRF2D::addSuperPixel (src/densecrf.cpp:193-251) with the mean-shift
+        segmentation given as a region map (one id per pixel): the layer's terms
+        come from the image as in the reference."""
+        e, c = superpixel_terms(img, region, constant, normaliser)
+        self.addSuperPixelLayer(region, e, c)
 
     def addPairwiseGaussian(self, sx, sy, function: PottsCompatibility | None = None, kernel_type=1,
                             normalization_type=NORMALIZE_SYMMETRIC):

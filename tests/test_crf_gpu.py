@@ -259,3 +259,47 @@ def test_divide_and_conquer_filter_vs_oracle_larger():
     ind = np.argsort(np.random.default_rng(6).random((wl.N, 21)), axis=0).astype(np.int32)
     out = c.apply_upper_minus_lower_dc(0, ind)
     assert rel_err(out, np.float32(-5.0) * O.uml_dc(f, ind)) < 1e-5
+
+
+def test_standalone_pairwise_potential_filters(golden):
+    """the standalone PairwisePotential (its own lattice and handle): the
+    brute-force, divide-and-conquer and restricted filters against the
+    reference fixtures and the oracle"""
+    import densecrf_ho_amd as D
+    g = golden("bf_12x9_L4.npz")
+    p = D.PairwisePotential(g["feat1"], D.PottsCompatibility(5.0))
+    assert rel_err(p.apply_bf(g["Q"]), g["bf1"]) < 1e-5
+    assert rel_err(p.apply_upper_minus_lower_bf(g["ind"]), g["uml1"]) < 1e-5
+    assert rel_err(p.apply_upper_minus_lower_bf_ord(g["ind"], g["Q"]), g["umlord1"]) < 1e-5
+    dc = p.apply_upper_minus_lower_dc(g["ind"])
+    assert rel_err(dc, np.float32(-5.0) * O.uml_dc(g["feat1"], g["ind"])) < 1e-5
+    rng = np.random.default_rng(21)
+    N = g["Q"].shape[0]
+    pI = np.sort(rng.choice(N, N // 2, replace=False)).astype(np.int32)
+    rq = rng.random((len(pI), 4)).astype(np.float32)
+    got = p.apply_upper_minus_lower_ord_restricted(rq, pI, g["Q"], True)
+    ref = np.float32(-5.0) * O.Lattice(g["feat1"]).ordered_restricted(rq, pI, g["Q"], True)
+    assert rel_err(got, ref) < 1e-5
+
+
+def test_add_superpixel_from_region_map(golden):
+    """DenseCRF2D.addSuperPixel with the reference's mean-shift regions of its
+    own data/img.bmp: the QP with that layer ends at the reference's
+    higher-order energy (qp_sp of example_inference)"""
+    import densecrf_ho_amd as D
+    with open(os.path.join(GOLDEN, "data_img.json")) as f:
+        ref = json.load(f)
+    g = np.load(os.path.join(GOLDEN, "data_img_labels.npz"))
+    U, W, H = D.load_unary(os.path.join(GOLDEN, "data", "img.c_unary"))
+    im, _, _ = D.load_image(os.path.join(GOLDEN, "data", "img.bmp"))
+    c = D.DenseCRF2D(W, H, U.shape[1])
+    c.setUnaryEnergy(U)
+    c.addPairwiseGaussian(2, 2, D.PottsCompatibility(3))
+    c.addPairwiseBilateral(30, 30, 8, 8, 8, im, D.PottsCompatibility(2))
+    c.addSuperPixel(im, g["regions"], 50.0, 1000.0)
+    Q = c.qp_inference_super_pixels_non_convex(c.unary_init())
+    lab = Q.argmax(1).astype(np.int32)
+    e = c.assignment_energy_higher_order(lab)
+    print("qp_sp energy", e, "reference", ref["e_qp_sp"], flush=True)
+    assert abs(e - ref["e_qp_sp"]) <= 1e-4 * abs(ref["e_qp_sp"])
+    assert (lab == g["qp_sp"]).mean() >= 0.999
