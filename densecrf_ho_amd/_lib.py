@@ -59,6 +59,8 @@ SIGNATURES = {
     "dcrf_lattice_compute_upper_minus_lower_dc": (_i, [_vp, _vp, _i, _i, _i, _i, _i]),
     "dcrf_pairwise_apply_upper_minus_lower_dc": (_i, [_vp, _vp, _vp, _i, _i]),
     "dcrf_crf_apply_upper_minus_lower_dc": (_i, [_vp, _i, _vp, _vp, _i]),
+    "dcrf_crf_unary_energy": (_i, [_vp, _vp, _vp, _i]),
+    "dcrf_crf_pairwise_energy_true": (_i, [_vp, _vp, _i, _vp, _i]),
     "dcrf_pairwise_apply_bf": (_i, [_vp, _vp, _vp, _i, _i]),
     "dcrf_pairwise_apply_upper_minus_lower_bf": (_i, [_vp, _vp, _vp, _i, _i]),
     "dcrf_pairwise_apply_upper_minus_lower_bf_ord": (_i, [_vp, _vp, _vp, _vp, _i, _i]),

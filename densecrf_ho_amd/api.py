@@ -714,6 +714,22 @@ class DenseCRF:
         _check(fn(self._h, a.ptr, a.dev, C.byref(e)))
         return e.value
 
+    def unaryEnergy(self, labels):
+        """DenseCRF::unaryEnergy(l): per-pixel U(l_i, i) (float32, (N,))"""
+        lab = np.ascontiguousarray(labels, np.int32)
+        out = np.zeros(self.N, np.float32)
+        _check(_lib.load().dcrf_crf_unary_energy(self._h, lab.ctypes.data_as(C.c_void_p),
+                                                 out.ctypes.data_as(C.c_void_p), 0))
+        return out
+
+    def pairwise_energy_true(self, labels, term: int = -1):
+        """DenseCRF::pairwise_energy_true(l, term): per-pixel pairwise energy (float32, (N,))"""
+        lab = np.ascontiguousarray(labels, np.int32)
+        out = np.zeros(self.N, np.float32)
+        _check(_lib.load().dcrf_crf_pairwise_energy_true(self._h, lab.ctypes.data_as(C.c_void_p), term,
+                                                         out.ctypes.data_as(C.c_void_p), 0))
+        return out
+
     def assignment_energy_true(self, labels):
         return self._energy(_lib.load().dcrf_crf_assignment_energy_true, labels, np.int32)
 

@@ -785,6 +785,28 @@ int dcrf_crf_assignment_energy_true(dcrf_crf_t crf, const int* labels, int on_de
     *energy = c.assignment_energy_true(a.d);
   });
 }
+int dcrf_crf_unary_energy(dcrf_crf_t crf, const int* labels, float* out, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && labels && out, "dcrf_crf_unary_energy: bad arguments");
+    CRF& c = *crf->crf;
+    Staged<int> a(labels, (size_t)c.N(), on_device != 0, c.stream(), true);
+    Staged<float> o(out, (size_t)c.N(), on_device != 0, c.stream(), false);
+    c.unary_energy(a.d, o.d);
+    o.copy_out();
+    DCRF_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
+int dcrf_crf_pairwise_energy_true(dcrf_crf_t crf, const int* labels, int term, float* out, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && labels && out, "dcrf_crf_pairwise_energy_true: bad arguments");
+    CRF& c = *crf->crf;
+    Staged<int> a(labels, (size_t)c.N(), on_device != 0, c.stream(), true);
+    Staged<float> o(out, (size_t)c.N(), on_device != 0, c.stream(), false);
+    c.pairwise_energy_true(a.d, term, o.d);
+    o.copy_out();
+    DCRF_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
 int dcrf_crf_assignment_energy_higher_order(dcrf_crf_t crf, const int* labels, int on_device, double* energy) {
   return guarded([&] {
     DCRF_REQUIRE(crf && labels && energy, "dcrf_crf_assignment_energy_higher_order: bad arguments");

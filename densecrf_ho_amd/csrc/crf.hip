@@ -368,6 +368,32 @@ void CRF::current_map(const float* Q, int* lab) {
   check_launch("k_argmax");
 }
 
+void CRF::unary_energy(const int* lab, float* out) {
+  k_unary_energy<<<grid_for(N_), kBlock, 0, s_>>>(U_.get(), lab, N_, L_, out);
+  check_launch("k_unary_energy");
+}
+
+// pairwise_energy_true (src/densecrf.cpp:902-927): Q = 1 - onehot(l) through
+// the no-norm potential, r_i = -Q(l_i, i); term -1 adds the terms in order
+void CRF::pairwise_energy_true(const int* lab, int term, float* out) {
+  Impl& m = *im_;
+  DCRF_REQUIRE(term >= -1 && term < num_pairwise(), "pairwise_energy_true: no such term");
+  k_fill<<<grid_for(N_), kBlock, 0, s_>>>(out, (size_t)N_, 0.f);
+  for (int k = term < 0 ? 0 : term; k < (term < 0 ? num_pairwise() : term + 1); ++k) {
+    Potential& p = pairwise(k);
+    SplatArgs a;
+    a.kind = SplatIn::kNotOnehot;
+    a.lab = lab;
+    SliceArgs o;
+    o.out = m.tmp.get();
+    o.scale = -p.nn_w;
+    o.apply_scale = true;
+    p.lat->filter(a, o, L_, false);
+    k_pairwise_true_accum<<<grid_for(N_), kBlock, 0, s_>>>(m.tmp.get(), lab, out, N_, L_);
+    check_launch("k_pairwise_true_accum");
+  }
+}
+
 // per-pixel unary + true pairwise energy (unaryEnergy + pairwise_energy_true,
 // src/densecrf.cpp:887-927); returns the FP64 total
 double CRF::assignment_energy_true(const int* lab) {

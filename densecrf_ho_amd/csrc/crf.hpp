@@ -159,6 +159,10 @@ class CRF {
   // ---- energies ---------------------------------------------------------------
   void current_map(const float* Q, int* lab);                 // currentMap (device)
   double assignment_energy_true(const int* lab);              // device labels
+  // unaryEnergy / pairwise_energy_true (src/densecrf.cpp:887-927): per-pixel
+  // float vectors into out (device); term -1 sums every pairwise term
+  void unary_energy(const int* lab, float* out);
+  void pairwise_energy_true(const int* lab, int term, float* out);
   double assignment_energy_ho(const int* lab);
   double lp_energy(const float* Q);                           // compute_energy_LP_higher_order
   double qp_value(const float* Q);                            // compute_LR_QP_value(Q, 0)

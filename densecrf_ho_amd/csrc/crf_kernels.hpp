@@ -450,6 +450,15 @@ __global__ void __launch_bounds__(kBlock) k_pairwise_true_accum(const float* __r
   r[i] = r[i] + v;
 }
 
+// unaryEnergy (:887-899): U(l_i, i), 0 for a label out of range
+__global__ void __launch_bounds__(kBlock) k_unary_energy(const float* __restrict__ U, const int* __restrict__ lab,
+                                                         int N, int L, float* __restrict__ r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int l = lab[i];
+  r[i] = (l >= 0 && l < L) ? U[(size_t)i * L + l] : 0.f;
+}
+
 // Σ_i (unary(l_i, i) + pairwise_i) in FP64 (:873-885)
 template <class SEG>
 __global__ void __launch_bounds__(kBlock) k_assign_energy(const float* __restrict__ U, const int* __restrict__ lab,

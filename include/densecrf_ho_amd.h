@@ -284,6 +284,11 @@ int dcrf_crf_lp(dcrf_crf_t crf, const float* init, float* out, const dcrf_lp_par
 int dcrf_crf_current_map(dcrf_crf_t crf, const float* Q, int* labels, int on_device);
 /* DenseCRF::assignment_energy_true(l), src/densecrf.cpp:873-885 */
 int dcrf_crf_assignment_energy_true(dcrf_crf_t crf, const int* labels, int on_device, double* energy);
+/* DenseCRF::unaryEnergy(l) / pairwise_energy_true(l, term), include/densecrf.h:150-153,
+ * src/densecrf.cpp:887-927 (the reference's public debugging functions): per-pixel
+ * float energies into out (N); term -1 sums every pairwise term (no-norm Potts copies). */
+int dcrf_crf_unary_energy(dcrf_crf_t crf, const int* labels, float* out, int on_device);
+int dcrf_crf_pairwise_energy_true(dcrf_crf_t crf, const int* labels, int term, float* out, int on_device);
 /* DenseCRF::assignment_energy_higher_order(l), src/densecrf.cpp:844-870 */
 int dcrf_crf_assignment_energy_higher_order(dcrf_crf_t crf, const int* labels, int on_device, double* energy);
 /* DenseCRF::compute_energy_LP_higher_order(Q), src/densecrf.cpp:982-1037 */

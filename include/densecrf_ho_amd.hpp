@@ -424,6 +424,19 @@ class DenseCRF {
     check(dcrf_crf_assignment_energy_true(h_, lab.data(), 0, &e));
     return e;
   }
+  // public debugging functions of the reference (include/densecrf.h:148-153)
+  std::vector<float> unaryEnergy(const VectorXs& l) const {
+    std::vector<int> lab(l.data(), l.data() + l.size());
+    std::vector<float> r((size_t)N_);
+    check(dcrf_crf_unary_energy(h_, lab.data(), r.data(), 0));
+    return r;
+  }
+  std::vector<float> pairwise_energy_true(const VectorXs& l, int term = -1) const {
+    std::vector<int> lab(l.data(), l.data() + l.size());
+    std::vector<float> r((size_t)N_);
+    check(dcrf_crf_pairwise_energy_true(h_, lab.data(), term, r.data(), 0));
+    return r;
+  }
   double assignment_energy_higher_order(const VectorXs& l) const {
     std::vector<int> lab = labels(l);
     double e = 0;

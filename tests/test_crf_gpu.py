@@ -303,3 +303,24 @@ def test_add_superpixel_from_region_map(golden):
     print("qp_sp energy", e, "reference", ref["e_qp_sp"], flush=True)
     assert abs(e - ref["e_qp_sp"]) <= 1e-4 * abs(ref["e_qp_sp"])
     assert (lab == g["qp_sp"]).mean() >= 0.999
+
+
+def test_per_pixel_energies_add_up(golden):
+    """unaryEnergy + pairwise_energy_true (the reference's debugging
+    functions) summed in FP64 give assignment_energy_true; term -1 is the sum
+    of the single terms in order; pinned to the reference's total energy"""
+    import densecrf_ho_amd as D
+    g = golden("crf_31x23_L5.npz")
+    W, H, L = int(g["W"]), int(g["H"]), int(g["L"])
+    c = D.DenseCRF2D(W, H, L)
+    c.setUnaryEnergy(g["U"])
+    c.addPairwiseEnergy(g["feat0"], D.PottsCompatibility(3.0))
+    c.addPairwiseEnergy(g["feat1"], D.PottsCompatibility(5.0))
+    lab = g["lab_qp"]
+    u = c.unaryEnergy(lab)
+    p = c.pairwise_energy_true(lab)
+    np.testing.assert_array_equal(p, c.pairwise_energy_true(lab, 0) + c.pairwise_energy_true(lab, 1))
+    np.testing.assert_array_equal(u, g["U"][np.arange(W * H), lab])
+    e = float(np.sum((u + p).astype(np.float64)))
+    assert abs(e - c.assignment_energy_true(lab)) <= 1e-9 * abs(e)
+    assert abs(e - float(g["e_true"])) <= 1e-5 * abs(float(g["e_true"]))
