@@ -1000,53 +1000,15 @@ __device__ inline void blur_quad(const int* __restrict__ nbr, const T* __restric
   else
     st_quad(dst, base + (size_t)(i + 1) * R4 + q, r);
 }
-// the lower rows with two quads per lane (rows of an even number of quads):
-// quads q and q + R4/2 of point i, one 8-byte load of the neighbour pair, six
-// loads in flight before the arithmetic; each load instruction of a wave
-// still covers consecutive quads of a row
-template <class T>
-__device__ inline void blur_quad_pair(const int* __restrict__ nbr, const T* __restrict__ src, T* __restrict__ dst,
-                                      uint32_t i, uint32_t R4, uint32_t q, bool nt) {
-  const int2 nn = reinterpret_cast<const int2*>(nbr)[i];
-  const size_t h = R4 / 2;
-  const size_t r0 = (size_t)(i + 1) * R4 + q, r1 = (size_t)(nn.x + 1) * R4 + q, r2 = (size_t)(nn.y + 1) * R4 + q;
-  const float4 o0 = ld_quad(src, r0), o1 = ld_quad(src, r0 + h);
-  const float4 a0 = ld_quad(src, r1), a1 = ld_quad(src, r1 + h);
-  const float4 b0 = ld_quad(src, r2), b1 = ld_quad(src, r2 + h);
-  float4 x, y;
-  x.x = o0.x + 0.5f * (a0.x + b0.x);
-  x.y = o0.y + 0.5f * (a0.y + b0.y);
-  x.z = o0.z + 0.5f * (a0.z + b0.z);
-  x.w = o0.w + 0.5f * (a0.w + b0.w);
-  y.x = o1.x + 0.5f * (a1.x + b1.x);
-  y.y = o1.y + 0.5f * (a1.y + b1.y);
-  y.z = o1.z + 0.5f * (a1.z + b1.z);
-  y.w = o1.w + 0.5f * (a1.w + b1.w);
-  if (nt) {
-    st_quad_nt(dst, r0, x);
-    st_quad_nt(dst, r0 + h, y);
-  } else {
-    st_quad(dst, r0, x);
-    st_quad(dst, r0 + h, y);
-  }
-}
-template <int TAG, class T, int QPL = 1>
+template <int TAG, class T>
 __global__ void __launch_bounds__(kBlock) k_blur2(int R4, FastDiv divR4, uint32_t nq, size_t half_q, uint32_t M,
                                                   const int* __restrict__ nbr, const T* __restrict__ src,
                                                   T* __restrict__ dst, const float* __restrict__ dsrc,
-                                                  float* __restrict__ ddst, EdgeMap edge, int nt, FastDiv divH) {
+                                                  float* __restrict__ ddst, EdgeMap edge, int nt) {
   const uint32_t stride = gridDim.x * blockDim.x, t0 = xcd_block() * blockDim.x + threadIdx.x;
-  if (QPL == 2) {
-    const uint32_t h = (uint32_t)R4 / 2, nl = nq / 2;
-    for (uint32_t t = t0; t < nl; t += stride) {  // lower rows, two quads per lane
-      const uint32_t i = divH.div(t);
-      blur_quad_pair(nbr, src, dst, i, (uint32_t)R4, t - i * h, nt);
-    }
-  } else {
-    for (uint32_t t = t0; t < nq; t += stride) {  // lower rows
-      const uint32_t i = divR4.div(t);
-      blur_quad(nbr, src, dst, 0, i, (uint32_t)R4, t - i * (uint32_t)R4, nt);
-    }
+  for (uint32_t t = t0; t < nq; t += stride) {  // lower rows
+    const uint32_t i = divR4.div(t);
+    blur_quad(nbr, src, dst, 0, i, (uint32_t)R4, t - i * (uint32_t)R4, nt);
   }
   if (edge.any_corr())  // correction rows: two quads per point
     for (uint32_t t = t0; t < 2 * M; t += stride) blur_quad(nbr, dsrc, ddst, 0, t >> 1, 2u, t & 1u, false);
@@ -1744,16 +1706,13 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
   const size_t half_q = half / 4;  // quads from a lower row to its upper row
   // the last axis streams its rows out (non-temporal stores): the slice that
   // reads them then finds no dirty lines of the blur to write back in L2
-  const bool pairs = R4 % 2 == 0;  // two quads per lane over the lower rows
-  const FastDiv divH((uint32_t)std::max(1, R4 / 2));
   auto blur = [&](auto kern, const int* nb, const T* a, T* b, int nt) {
-    const unsigned full = grid_for(pairs ? nquads / 2 : nquads);
+    const unsigned full = grid_for(nquads);
     // src -> dst, the same bits for any grid: tuned on the lower rows
     const unsigned g = Tuner::get().pick(d_ == 5 ? "oblur5" : "oblur", M_, grid_candidates(full), s, [&](unsigned gg) {
-      kern<<<gg, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, half_q, (uint32_t)M_, nb, a, b, dsrc, ddst, emr, nt,
-                                 divH);
+      kern<<<gg, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, half_q, (uint32_t)M_, nb, a, b, dsrc, ddst, emr, nt);
     });
-    kern<<<g, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, half_q, (uint32_t)M_, nb, a, b, dsrc, ddst, emr, nt, divH);
+    kern<<<g, kBlock, 0, s>>>(R4, divR4, (uint32_t)nquads, half_q, (uint32_t)M_, nb, a, b, dsrc, ddst, emr, nt);
   };
   for (int j = 0; j <= d_; ++j) {  // forward order only (:621, :669)
     const int* nb = onbr_.get() + (size_t)j * M_ * 2;
@@ -1761,11 +1720,11 @@ void Lattice::ordered_t(const float* Q, const float* d_minmax, int L, const Slic
       KTimer tm("oblur" + tag, s);
       const int nt = j == d_;
       if (d_ == 2)
-        pairs ? blur(k_blur2<2, T, 2>, nb, srcL, dstL, nt) : blur(k_blur2<2, T>, nb, srcL, dstL, nt);
+        blur(k_blur2<2, T>, nb, srcL, dstL, nt);
       else if (d_ == 5)
-        pairs ? blur(k_blur2<5, T, 2>, nb, srcL, dstL, nt) : blur(k_blur2<5, T>, nb, srcL, dstL, nt);
+        blur(k_blur2<5, T>, nb, srcL, dstL, nt);
       else
-        pairs ? blur(k_blur2<0, T, 2>, nb, srcL, dstL, nt) : blur(k_blur2<0, T>, nb, srcL, dstL, nt);
+        blur(k_blur2<0, T>, nb, srcL, dstL, nt);
     }
     check_launch("k_blur2");
     std::swap(srcL, dstL);
