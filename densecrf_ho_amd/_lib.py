@@ -60,6 +60,7 @@ SIGNATURES = {
     "dcrf_pairwise_apply_upper_minus_lower_dc": (_i, [_vp, _vp, _vp, _i, _i]),
     "dcrf_crf_apply_upper_minus_lower_dc": (_i, [_vp, _i, _vp, _vp, _i]),
     "dcrf_crf_unary_energy": (_i, [_vp, _vp, _vp, _i]),
+    "dcrf_crf_energy_higher_order": (_i, [_vp, _vp, _i, C.POINTER(C.c_double)]),
     "dcrf_crf_pairwise_energy_true": (_i, [_vp, _vp, _i, _vp, _i]),
     "dcrf_pairwise_apply_bf": (_i, [_vp, _vp, _vp, _i, _i]),
     "dcrf_pairwise_apply_upper_minus_lower_bf": (_i, [_vp, _vp, _vp, _i, _i]),

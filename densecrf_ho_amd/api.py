@@ -739,6 +739,10 @@ class DenseCRF:
     def compute_energy_LP_higher_order(self, Q):
         return self._energy(_lib.load().dcrf_crf_lp_energy, Q, np.float32)
 
+    def compute_energy_higher_order(self, Q):
+        """DenseCRF::compute_energy_higher_order(Q): QP energy with the last QP's z_labels_"""
+        return self._energy(_lib.load().dcrf_crf_energy_higher_order, Q, np.float32)
+
     def computeUCondGrad(self, Q, out=None):
         a = _Arg(Q, np.float32)
         out = _empty_like_field(Q, (self.N, self.M)) if out is None else out

@@ -815,6 +815,14 @@ int dcrf_crf_assignment_energy_higher_order(dcrf_crf_t crf, const int* labels, i
     *energy = c.assignment_energy_ho(a.d);
   });
 }
+int dcrf_crf_energy_higher_order(dcrf_crf_t crf, const float* Q, int on_device, double* energy) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && Q && energy, "dcrf_crf_energy_higher_order: bad arguments");
+    CRF& c = *crf->crf;
+    Staged<float> a(Q, (size_t)c.N() * c.L(), on_device != 0, c.stream(), true);
+    *energy = c.energy_higher_order(a.d);
+  });
+}
 int dcrf_crf_lp_energy(dcrf_crf_t crf, const float* Q, int on_device, double* energy) {
   return guarded([&] {
     DCRF_REQUIRE(crf && Q && energy, "dcrf_crf_lp_energy: bad arguments");

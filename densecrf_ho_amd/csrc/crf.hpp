@@ -166,6 +166,10 @@ class CRF {
   double assignment_energy_ho(const int* lab);
   double lp_energy(const float* Q);                           // compute_energy_LP_higher_order
   double qp_value(const float* Q);                            // compute_LR_QP_value(Q, 0)
+  // compute_energy_higher_order (src/densecrf.cpp:960-980): qp_value(Q) plus
+  // the region term with the last QP's z_labels_
+  double energy_higher_order(const float* Q) { return qp_value(Q) + ho_energy_term(Q); }
+  double ho_energy_term(const float* Q);
   void ucondgrad(const float* Q, float* Us);                  // computeUCondGrad
 
   // z_labels_ after the last QP (L x R, region-major), host pointer

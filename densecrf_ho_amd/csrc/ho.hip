@@ -35,7 +35,9 @@ struct CRF::HOState {
   // LP scratch: per region min/max positions
   DBuf<int> mpos;  // 4 per region: min_pix, min_lab, max_pix, max_lab
   DBuf<double> parts;
+  DBuf<double> eterm;  // compute_energy_higher_order's region term
   bool dirty = true;
+  bool have_z = false;  // z holds the last QP's z_labels_
 };
 
 namespace {
@@ -524,6 +526,7 @@ void CRF::add_layer(const int* h_region, int R, const float* h_e, const float* h
   h.nmem = (int)h.h_mpix.size();
   R_ = h.R;
   h.dirty = true;
+  h.have_z = false;
 }
 
 // QP start: z = 1, grad_z, cond_grad_z, grad_y += RtoN((z-1).*c), energy term
@@ -531,6 +534,7 @@ void CRF::ho_qp_init(hipStream_t s, const float* Q, float* gy, double* sho) {
   HOState& h = *ho_;
   const int G = h.G, L = L_, R = h.R;
   k_fill<<<grid_for((size_t)R * L), kBlock, 0, s>>>(h.z.get(), (size_t)R * L, 1.f);
+  h.have_z = true;
   k_ho_pixprefix<0><<<L, kPixPrefixBlock, 0, s>>>(Q, nullptr, L, h.usz.get(), G, h.prefB.get(), nullptr);
   const int nrb = ho_red_blocks(R, L);
   k_ho_gradz_energy<<<nrb, kBlock, 0, s>>>(h.z.get(), h.e.get(), h.reg_grp.get(), h.prefB.get(), R, L, 1, h.gz.get(),
@@ -583,6 +587,28 @@ void CRF::ho_qp_after(hipStream_t s, const float* Q, double* sho, int* fl) {
 }
 
 void CRF::ho_qp_finish(hipStream_t) {}
+
+// the region term of compute_energy_higher_order (src/densecrf.cpp:974-975):
+// sum(z .* c) + multiplySuperPixels((1 - z) .* c, Q - 1) with the z_labels_
+// of the last QP; 0 without superpixel layers
+double CRF::ho_energy_term(const float* Q) {
+  HOState* hp = ho_state();
+  if (!hp) return 0.0;
+  HOState& h = *hp;
+  DCRF_REQUIRE(h.have_z, "compute_energy_higher_order: z_labels_ are set by a QP run; run the QP first");
+  hipStream_t s = s_;
+  h.eterm.resize(1);
+  k_ho_pixprefix<0><<<L_, kPixPrefixBlock, 0, s>>>(Q, nullptr, L_, h.usz.get(), h.G, h.prefB.get(), nullptr);
+  const int nrb = ho_red_blocks(h.R, L_);
+  k_ho_gradz_energy<<<nrb, kBlock, 0, s>>>(h.z.get(), h.e.get(), h.reg_grp.get(), h.prefB.get(), h.R, L_, 0,
+                                           h.gz.get(), h.nrQ.get(), h.parts.get(), nullptr);
+  k_ho_finalize<1><<<1, kBlock, 0, s>>>(h.parts.get(), nrb, h.eterm.get(), nullptr);
+  check_launch("ho_energy_term");
+  double e = 0;
+  DCRF_HIP(hipMemcpyAsync(&e, h.eterm.get(), sizeof(double), hipMemcpyDeviceToHost, s));
+  DCRF_HIP(hipStreamSynchronize(s));
+  return e;
+}
 
 // z_labels_ after the last QP (L x R, region-major)
 void CRF::z_labels(float* host) {

@@ -291,6 +291,11 @@ int dcrf_crf_unary_energy(dcrf_crf_t crf, const int* labels, float* out, int on_
 int dcrf_crf_pairwise_energy_true(dcrf_crf_t crf, const int* labels, int term, float* out, int on_device);
 /* DenseCRF::assignment_energy_higher_order(l), src/densecrf.cpp:844-870 */
 int dcrf_crf_assignment_energy_higher_order(dcrf_crf_t crf, const int* labels, int on_device, double* energy);
+/* DenseCRF::compute_energy_higher_order(Q), include/densecrf.h:162, src/densecrf.cpp:960-980:
+ * U.Q + sum_k Q.(K_k Q) (FP64 dots) + sum(z .* c) + multiplySuperPixels((1 - z) .* c, Q - 1)
+ * with the z_labels_ of the last QP (the region term needs a QP run first when superpixel
+ * layers exist; DCRF_INVALID_ARGUMENT otherwise). */
+int dcrf_crf_energy_higher_order(dcrf_crf_t crf, const float* Q, int on_device, double* energy);
 /* DenseCRF::compute_energy_LP_higher_order(Q), src/densecrf.cpp:982-1037 */
 int dcrf_crf_lp_energy(dcrf_crf_t crf, const float* Q, int on_device, double* energy);
 /* DenseCRF::computeUCondGrad(Us, Q), src/densecrf.cpp:803-830 (protected in the

@@ -437,6 +437,11 @@ class DenseCRF {
     check(dcrf_crf_pairwise_energy_true(h_, lab.data(), term, r.data(), 0));
     return r;
   }
+  double compute_energy_higher_order(const MatrixXf& Q) const {
+    double e = 0;
+    check(dcrf_crf_energy_higher_order(h_, Q.data(), 0, &e));
+    return e;
+  }
   double assignment_energy_higher_order(const VectorXs& l) const {
     std::vector<int> lab = labels(l);
     double e = 0;

@@ -471,6 +471,7 @@ def reflib():
         _sig(R, "ref_crf_energy_ho", _d, _vp, _i32p)
         _sig(R, "ref_crf_lp_energy", _d, _vp, _f32p)
         _sig(R, "ref_crf_qp_value", _d, _vp, _f32p)
+        _sig(R, "ref_crf_energy_higher_order", _d, _vp, _f32p)
         _sig(R, "ref_crf_ucondgrad", None, _vp, _f32p, _f32p)
         _sig(R, "ref_crf_current_map", None, _vp, _f32p, _i32p)
         _sig(R, "ref_feasible", None, _f32p, _f32p, _i, _i)
@@ -632,6 +633,10 @@ class RefCRF:
         c = np.zeros(R, np.float32)
         self._R.ref_crf_layers(self.h, reg, e, c)
         return reg, e, c
+
+    def energy_higher_order(self, Q):
+        """compute_energy_higher_order(Q) (uses the z_labels_ of the last qp())"""
+        return self._R.ref_crf_energy_higher_order(self.h, _c(Q, np.float32))
 
     def apply_bf(self, k, which, ind=None, Q=None):
         """0 apply_bf(Q), 1 apply_upper_minus_lower_bf(ind), 2 _bf_ord(ind, Q), 3 _dc(ind)"""

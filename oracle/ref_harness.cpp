@@ -308,6 +308,10 @@ double ref_crf_lp_energy(void* h, const float* Q) {
   RefCRF* c = (RefCRF*)h;
   return c->compute_energy_LP_higher_order(as_mat(Q, c->L(), c->N()));
 }
+double ref_crf_energy_higher_order(void* h, const float* Q) {
+  RefCRF* c = (RefCRF*)h;
+  return c->compute_energy_higher_order(as_mat(Q, c->L(), c->N()));
+}
 double ref_crf_qp_value(void* h, const float* Q) {
   RefCRF* c = (RefCRF*)h;
   return c->compute_LR_QP_value(as_mat(Q, c->L(), c->N()), MatrixXf::Zero(c->L(), c->N()));

@@ -12,6 +12,7 @@ compressed .npz files.  Run from the repository root:
     python tests/golden/make_golden.py data     # only data_img.json / data_img_labels.npz
     python tests/golden/make_golden.py kgrad    # only kgrad_31x23.npz
     python tests/golden/make_golden.py dc       # only dc_L4.npz
+    python tests/golden/make_golden.py eho      # only eho_32x24_L21.npz
 """
 
 from __future__ import annotations
@@ -419,7 +420,28 @@ def dc_case():
     print("dc_L4")
 
 
+def eho_case():
+    """DenseCRF::compute_energy_higher_order (src/densecrf.cpp:960-980) after the
+    QP of crf_32x24_L21_ho (its z_labels_): at the QP's result and at another Q"""
+    W, H, L = 32, 24, 21
+    wl = S.Workload(W, H, 10, 2, L=L)
+    r = O.RefCRF(W, H, L)
+    r.set_unary(wl.U)
+    r.add_gaussian(3, 3, 3)
+    r.add_bilateral(50, 50, 15, 15, 15, wl.img, 5)
+    for i, (reg, _) in enumerate(S.superpixel_layers(W, H, (5, 9, 17), [21 + i for i in range(3)])):
+        r.add_layer(reg, 0.5, 0.2)
+    Qqp = r.qp(r.unary_init())
+    Qo = np.random.default_rng(9).random((W * H, L)).astype(np.float32)
+    np.savez_compressed(os.path.join(OUT, "eho_32x24_L21.npz"), Qo=Qo, e_qp=np.float64(r.energy_higher_order(Qqp)),
+                        e_other=np.float64(r.energy_higher_order(Qo)))
+    print("eho_32x24_L21")
+
+
 def main():
+    if sys.argv[1:] == ["eho"]:
+        eho_case()
+        return
     if sys.argv[1:] == ["dc"]:
         dc_case()
         return
@@ -458,6 +480,7 @@ def main():
     bf_case()
     kgrad_case()
     dc_case()
+    eho_case()
     data_case()
     c0_case()
 

@@ -186,3 +186,21 @@ def test_ucondgrad_large_regions_vs_oracle():
     Qb[rng.random((wl.N, L)) < 0.001] *= -1.0
     for Q in (Qa, Qb):
         np.testing.assert_array_equal(c.computeUCondGrad(Q), o.ucondgrad(Q))
+
+
+def test_compute_energy_higher_order_vs_reference(golden):
+    """compute_energy_higher_order after the QP (the z_labels_ it leaves), at
+    the reference QP's result and at another field, within 1e-4 of the
+    compiled reference; without a QP run the region term is refused"""
+    import densecrf_ho_amd as D
+    g = golden("crf_32x24_L21_ho.npz")
+    ge = golden("eho_32x24_L21.npz")
+    c = gpu_crf(g)
+    with pytest.raises(D.EngineError, match="run the QP first"):
+        c.compute_energy_higher_order(g["Qqp"])
+    c.qp_inference_super_pixels_non_convex(g["Q0"])
+    e = c.compute_energy_higher_order(g["Qqp"])
+    print("energy", e, "reference", float(ge["e_qp"]), flush=True)
+    assert abs(e - float(ge["e_qp"])) <= 1e-4 * abs(float(ge["e_qp"]))
+    e2 = c.compute_energy_higher_order(ge["Qo"])
+    assert abs(e2 - float(ge["e_other"])) <= 1e-4 * abs(float(ge["e_other"]))
