@@ -101,5 +101,43 @@ int main(int argc, char** argv) {
     for (int i = 0; i < N; ++i)
       for (int l = 0; l < L; ++l) diff = std::max(diff, (double)std::abs(Qb(l, b * N + i) - Q5(l, i)));
   std::printf("batch_qp_diff %.9g\n", diff);
+
+  // the other filters of PairwisePotential and Permutohedral on the spatial
+  // features: the restricted filter with every pixel equals the ordered
+  // filter when N % 4 == 0; the brute-force and divide-and-conquer filters
+  // agree on a sorted order; the kernel gradient of a DIAG kernel has d values
+  MatrixXf q01(L, N);
+  for (int i = 0; i < N; ++i)
+    for (int l = 0; l < L; ++l) q01(l, i) = (float)((i * 7 + l * 3) % 11) / 10.f;
+  std::vector<int> all(N);
+  for (int i = 0; i < N; ++i) all[i] = i;
+  MatrixXf ord, rest(L, N);
+  pw.apply_upper_minus_lower_ord(ord, q01);
+  pw.apply_upper_minus_lower_ord_restricted(rest, q01, all, q01, true);
+  double rdiff = 0, omax = 0;
+  for (int i = 0; i < N; ++i)
+    for (int l = 0; l < L; ++l) {
+      rdiff = std::max(rdiff, (double)std::abs(ord(l, i) - rest(l, i)));
+      omax = std::max(omax, (double)std::abs(ord(l, i)));
+    }
+  std::printf("restricted_vs_ordered %.9g\n", rdiff / omax);
+  MatrixXi ind(L, N);
+  for (int l = 0; l < L; ++l)
+    for (int i = 0; i < N; ++i) ind(l, i) = (i * (2 * l + 1)) % N;  // pixel orders (any indices in [0, N))
+  MatrixXf bf, dc;
+  pw.apply_upper_minus_lower_bf(bf, ind);
+  pw.apply_upper_minus_lower_dc(dc, ind);
+  std::printf("bf_mid %.9g\n", (double)bf(0, N / 2));
+  std::printf("dc_mid %.9g\n", (double)dc(0, N / 2));
+  PairwisePotential kp(f, new PottsCompatibility(3), DIAG_KERNEL);
+  std::vector<float> kg = kp.kernelGradient(q01, Q);
+  std::printf("kernel_gradient_size %d\n", (int)kg.size());
+  std::vector<float> df((size_t)N * 2);
+  Permutohedral lat;
+  lat.init(f);
+  lat.gradient(df.data(), q01.data(), Q.data(), L);
+  double dfs = 0;
+  for (float v : df) dfs += std::abs(v);
+  std::printf("lattice_gradient_abs %.9g\n", dfs);
   return 0;
 }

@@ -47,3 +47,8 @@ def test_cpp_example_matches_oracle(tmp_path):
     assert np.isfinite(float(vals["meanfield_dU00"]))
     # a batch of the same image twice equals the one-image QP
     assert float(vals["batch_qp_diff"]) < 1e-5
+    # the other filters and gradients through the C++ layer
+    assert float(vals["restricted_vs_ordered"]) < 1e-5
+    assert np.isfinite(float(vals["bf_mid"])) and np.isfinite(float(vals["dc_mid"]))
+    assert int(vals["kernel_gradient_size"]) == 2
+    assert np.isfinite(float(vals["lattice_gradient_abs"])) and float(vals["lattice_gradient_abs"]) > 0
