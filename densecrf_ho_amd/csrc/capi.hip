@@ -83,10 +83,10 @@ struct Staged {
     own.resize(count);
     d = own.get();
     host = const_cast<T*>(p);
-    if (copy_in) DCRF_HIP(hipMemcpyAsync(d, p, count * sizeof(T), hipMemcpyHostToDevice, s));
+    if (copy_in && count > 0) DCRF_HIP(hipMemcpyAsync(d, p, count * sizeof(T), hipMemcpyHostToDevice, s));
   }
   void copy_out() {
-    if (host) DCRF_HIP(hipMemcpyAsync(host, d, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    if (host && n > 0) DCRF_HIP(hipMemcpyAsync(host, d, n * sizeof(T), hipMemcpyDeviceToHost, s));
   }
 };
 

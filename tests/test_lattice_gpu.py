@@ -156,6 +156,13 @@ def test_restricted_ordered_errors(golden):
     bad[3, 1] = 1.5
     with pytest.raises(D.EngineError, match="Value out of bounds"):
         lat.compute_upper_minus_lower_ord_restricted(None, bad, pI, g["q1"], True)
+    # no pixel in pI: nothing to slice, an empty result (the reference reads pI[0] here)
+    empty = lat.compute_upper_minus_lower_ord_restricted(None, np.zeros((0, 5), np.float32),
+                                                         np.zeros(0, np.int32), g["q1"], True)
+    assert empty.shape == (0, 5)
+    # the stored state then serves a call without store
+    out = lat.compute_upper_minus_lower_ord_restricted(None, rq, pI, None, False)
+    assert out.shape == rq.shape and np.isfinite(out).all()
 
 
 @pytest.mark.parametrize("d", [2, 5])
