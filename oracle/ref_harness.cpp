@@ -2,8 +2,8 @@
 // fixture script (tests/golden/make_golden.py) and the oracle tests can drive
 // it with ctypes.  TEST INFRASTRUCTURE ONLY: it is built from the reference
 // sources where they lie (see oracle/Makefile) into oracle/_ref/, and only in
-// the development container where those sources exist.  The reference never
-// travels to the GPU box; only the fixtures written from it do.
+// the development container where those sources exist.  The reference is
+// never needed at run time; only the fixtures written from it are.
 //
 // Protected members of the reference classes are reached by deriving from
 // them; no reference code is restated here.
