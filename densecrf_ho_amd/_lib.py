@@ -117,6 +117,7 @@ SIGNATURES = {
     "dcrf_io_load_image": (_i, [C.c_char_p, _vp, C.c_size_t, _ip, _ip]),
     "dcrf_io_save_map": (_i, [_vp, _i, _i, _i, C.c_char_p]),
     "dcrf_superpixel_terms": (_i, [_vp, _i, _vp, _i, _f, _f, _vp, _vp]),
+    "dcrf_less_confident_pixels": (_i, [_vp, _i, _i, _f, _vp, _ip, _i]),
     "dcrf_malloc": (_i, [C.POINTER(_vp), C.c_size_t]),
     "dcrf_free": (_i, [_vp]),
     "dcrf_memcpy": (_i, [_vp, _vp, C.c_size_t, _i]),

@@ -243,6 +243,17 @@ def save_map(Q, W: int, H: int, path: str):
     _check(_lib.load().dcrf_io_save_map(q.ctypes.data_as(C.c_void_p), W, H, q.shape[1], path.encode()))
 
 
+def less_confident_pixels(Q, tol: float = 0.99):
+    """less_confident_pixels (src/densecrf.cpp:457-462): ascending indices of the
+    pixels whose largest label value in Q (N, L) is <= tol"""
+    q = np.ascontiguousarray(Q, np.float32)
+    idx = np.zeros(q.shape[0], np.int32)
+    n = C.c_int()
+    _check(_lib.load().dcrf_less_confident_pixels(q.ctypes.data_as(C.c_void_p), q.shape[0], q.shape[1], tol,
+                                                  idx.ctypes.data_as(C.c_void_p), C.byref(n), 0))
+    return idx[: n.value]
+
+
 def superpixel_terms(img, region, constant: float = 50.0, normaliser: float = 1000.0):
     """the per-region terms of DenseCRF2D::addSuperPixel for a given
     segmentation: (exp_of_superpixels (R,), constants (R,))"""

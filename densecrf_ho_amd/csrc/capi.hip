@@ -1140,4 +1140,18 @@ int dcrf_superpixel_terms(const unsigned char* img, int N, const int* region, in
   });
 }
 
+int dcrf_less_confident_pixels(const float* Q, int N, int L, float tol, int* indices, int* count, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(Q && indices && count && N > 0 && L > 0, "dcrf_less_confident_pixels: bad arguments");
+    hipStream_t s = nullptr;
+    Staged<float> q(Q, (size_t)N * L, on_device != 0, s, true);
+    Staged<int> o(indices, (size_t)N, on_device != 0, s, false);
+    const int n = less_confident_pixels_dev(q.d, N, L, tol, o.d, s);
+    o.n = (size_t)n;  // only the first n entries are written
+    o.copy_out();
+    DCRF_HIP(hipStreamSynchronize(s));
+    *count = n;
+  });
+}
+
 }  // extern "C"

@@ -148,7 +148,37 @@ __global__ void __launch_bounds__(kBlock) k_rscale(float* __restrict__ x, size_t
   if (t < n) x[t] = a * x[t];
 }
 
+// less_confident_pixels (src/densecrf.cpp:457-462): flag of every pixel whose
+// largest label value is <= tol
+__global__ void __launch_bounds__(kBlock) k_lc_flag(const float* __restrict__ Q, int N, int L, float tol,
+                                                    int* __restrict__ flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > N) return;
+  if (i == N) {
+    flag[N] = 0;
+    return;
+  }
+  const float* col = Q + (size_t)i * L;
+  float mx = col[0];
+  for (int l = 1; l < L; ++l) mx = col[l] > mx ? col[l] : mx;
+  flag[i] = mx <= tol ? 1 : 0;
+}
+
 }  // namespace
+
+// the pixel indices (ascending) of less_confident_pixels into idx (device,
+// N entries); returns their count
+int less_confident_pixels_dev(const float* Q, int N, int L, float tol, int* idx, hipStream_t s) {
+  DBuf<int> flag(N + 1), pos(N + 1);
+  k_lc_flag<<<grid_for((size_t)N + 1), kBlock, 0, s>>>(Q, N, L, tol, flag.get());
+  scan_excl(flag.get(), pos.get(), N + 1, s);
+  k_rcompact<<<grid_for(N), kBlock, 0, s>>>(flag.get(), pos.get(), N, idx);
+  check_launch("less_confident_pixels");
+  int n = 0;
+  DCRF_HIP(hipMemcpyAsync(&n, pos.get() + N, sizeof(int), hipMemcpyDeviceToHost, s));
+  DCRF_HIP(hipStreamSynchronize(s));
+  return n;
+}
 
 void Lattice::ordered_restricted(const float* rq, const int* pI, int nI, const float* ext, int V, bool store,
                                  float* out) {

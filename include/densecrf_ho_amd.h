@@ -420,6 +420,12 @@ int dcrf_io_save_map(const float* Q, int W, int H, int L, const char* path);
 int dcrf_superpixel_terms(const unsigned char* img, int N, const int* region, int R, float constant, float normaliser,
                           float* e, float* c);
 
+/* less_confident_pixels(indices, Q, tol), include/densecrf.h:41, src/densecrf.cpp:457-462:
+ * the pixels (ascending) whose largest value of Q (L x N) is <= tol, e.g. the pixel set
+ * of the restricted ordered filter.  indices must hold N entries; *count receives how
+ * many were written. */
+int dcrf_less_confident_pixels(const float* Q, int N, int L, float tol, int* indices, int* count, int on_device);
+
 #ifdef __cplusplus
 }
 #endif

@@ -504,6 +504,14 @@ inline void superpixel_terms(const unsigned char* img, int N, const int* region,
   }
 }
 
+// less_confident_pixels (include/densecrf.h:41, src/densecrf.cpp:457-462)
+inline void less_confident_pixels(std::vector<int>& indices, const MatrixXf& Q, float tol = 0.99f) {
+  indices.assign((size_t)Q.cols(), 0);
+  int n = 0;
+  check(dcrf_less_confident_pixels(Q.data(), (int)Q.cols(), (int)Q.rows(), tol, indices.data(), &n, 0));
+  indices.resize((size_t)n);
+}
+
 class DenseCRF2D : public DenseCRF {
  public:
   DenseCRF2D(int W, int H, int M) : DenseCRF(W * H, M), W_(W), H_(H) {}

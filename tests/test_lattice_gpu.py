@@ -238,3 +238,15 @@ def test_gradient_and_restricted_at_320x213_vs_oracle(d):
     rq = rng.random((len(pI), 21)).astype(np.float32)
     out = lat.compute_upper_minus_lower_ord_restricted(None, rq, pI, q, True)
     assert rel_err(out, ref.ordered_restricted(rq, pI, q, True)) < 1e-5
+
+
+def test_less_confident_pixels():
+    """less_confident_pixels (src/densecrf.cpp:457-462): ascending indices of
+    the pixels whose largest value is <= tol (ties at tol included)"""
+    import densecrf_ho_amd as D
+    rng = np.random.default_rng(13)
+    Q = rng.random((50000, 21)).astype(np.float32) ** 8
+    Q[7] = np.float32(0.5)  # a pixel exactly at the tolerance
+    for tol in (0.5, 0.9, 0.99):
+        got = D.less_confident_pixels(Q, tol)
+        np.testing.assert_array_equal(got, np.nonzero(Q.max(1) <= np.float32(tol))[0])
