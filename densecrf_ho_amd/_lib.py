@@ -118,6 +118,7 @@ SIGNATURES = {
     "dcrf_io_save_map": (_i, [_vp, _i, _i, _i, C.c_char_p]),
     "dcrf_superpixel_terms": (_i, [_vp, _i, _vp, _i, _f, _f, _vp, _vp]),
     "dcrf_less_confident_pixels": (_i, [_vp, _i, _i, _f, _vp, _ip, _i]),
+    "dcrf_less_confident_pixels_noalloc": (_i, [_vp, _i, _i, _f, _vp, _ip]),
     "dcrf_malloc": (_i, [C.POINTER(_vp), C.c_size_t]),
     "dcrf_free": (_i, [_vp]),
     "dcrf_memcpy": (_i, [_vp, _vp, C.c_size_t, _i]),

@@ -1142,7 +1142,9 @@ int dcrf_superpixel_terms(const unsigned char* img, int N, const int* region, in
 
 int dcrf_less_confident_pixels(const float* Q, int N, int L, float tol, int* indices, int* count, int on_device) {
   return guarded([&] {
-    DCRF_REQUIRE(Q && indices && count && N > 0 && L > 0, "dcrf_less_confident_pixels: bad arguments");
+    DCRF_REQUIRE(count && N >= 0 && L > 0 && (N == 0 || (Q && indices)), "dcrf_less_confident_pixels: bad arguments");
+    *count = 0;
+    if (N == 0) return;  // an empty field has no pixel (src/densecrf.cpp:457-462 returns an empty list)
     hipStream_t s = nullptr;
     Staged<float> q(Q, (size_t)N * L, on_device != 0, s, true);
     Staged<int> o(indices, (size_t)N, on_device != 0, s, false);
@@ -1151,6 +1153,31 @@ int dcrf_less_confident_pixels(const float* Q, int N, int L, float tol, int* ind
     o.copy_out();
     DCRF_HIP(hipStreamSynchronize(s));
     *count = n;
+  });
+}
+
+// a device (or managed) allocation, not host memory: the no-allocation entry
+// point cannot stage host buffers
+static void require_device_pointer(const void* p, const char* what) {
+  hipPointerAttribute_t a;
+  const hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) (void)hipGetLastError();  // leave no error for the next launch check
+  if (e != hipSuccess || (a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged))
+    throw Error(kInvalidArgument, std::string("dcrf_less_confident_pixels_noalloc: ") + what +
+                                      " must be a device pointer");
+}
+
+int dcrf_less_confident_pixels_noalloc(const float* Q, int N, int L, float tol, int* indices, int* count) {
+  return guarded([&] {
+    DCRF_REQUIRE(count && N >= 0 && L > 0 && (N == 0 || (Q && indices)),
+                 "dcrf_less_confident_pixels_noalloc: bad arguments");
+    *count = 0;
+    DCRF_REQUIRE((long long)N <= kLcMaxPixels,
+                 "dcrf_less_confident_pixels_noalloc: N exceeds the 16,777,216 pixels its fixed scratch holds");
+    if (N == 0) return;
+    require_device_pointer(Q, "Q");
+    require_device_pointer(indices, "indices");
+    *count = less_confident_pixels_noalloc_dev(Q, N, L, tol, indices, nullptr);
   });
 }
 

@@ -10,4 +10,5 @@
 #include "kgrad.hip"
 #include "adjoint.hip"
 #include "batch.hip"
+#include "noalloc.hip"
 #include "capi.hip"

@@ -425,6 +425,17 @@ int dcrf_superpixel_terms(const unsigned char* img, int N, const int* region, in
  * of the restricted ordered filter.  indices must hold N entries; *count receives how
  * many were written. */
 int dcrf_less_confident_pixels(const float* Q, int N, int L, float tol, int* indices, int* count, int on_device);
+/* No counterpart in the reference.  dcrf_less_confident_pixels for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc): it calls no hipMalloc / hipFree and no allocating library
+ * routine.  Q (L x N) and indices (N entries) must be device pointers (host pointers are
+ * refused); *count is a host int.  Its only working memory is a fixed device array of
+ * 128 KiB per GPU, held until the process ends, so N is at most 16,777,216 (larger N is
+ * refused with DCRF_INVALID_ARGUMENT).  Every call on one GPU shares that array: calls
+ * that overlap on one GPU race.  It runs on the null stream and waits for it; Q must be
+ * complete before the call (the engine's own entry points return with their results
+ * complete). */
+int dcrf_less_confident_pixels_noalloc(const float* Q, int N, int L, float tol, int* indices, int* count);
 
 #ifdef __cplusplus
 }
