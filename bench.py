@@ -226,23 +226,10 @@ def adjoint_bench(D, S, steps=3):
 BATCH_IMAGES = 64
 
 
-def _batch_image(i):
-    """image i of the batched workload: seed 3000+i, H ~ U[240,1080],
-    W ~ U[320,1920], one Voronoi cell per 4000 pixels"""
-    from densecrf_ho_amd import synthetic as S
-    rng = np.random.default_rng(3000 + i)
-    H = int(rng.integers(240, 1081))
-    W = int(rng.integers(320, 1921))
-    wl = S.Workload(W, H, max(1, round(W * H / 4000)), 3000 + i, L=L)
-    return W, H, wl.img, wl.U
-
-
 def batch_workload(n):
-    """the batch's images, generated on host cores in worker processes"""
-    import multiprocessing as mp
-    workers = max(1, min(16, os.cpu_count() or 1))
-    with mp.get_context("spawn").Pool(workers) as pool:
-        return pool.map(_batch_image, range(n))
+    """the batch's images (seeds 3000+i), generated on host cores in worker processes"""
+    from densecrf_ho_amd import synthetic as S
+    return S.batch_images(n)
 
 
 def _progress(msg):

@@ -120,3 +120,27 @@ class Workload:
 def superpixel_layers(W: int, H: int, cells=(300, 600, 1200), seeds=(21, 22, 23)):
     """Voronoi region maps standing in for mean-shift superpixel layers."""
     return [(voronoi_regions(W, H, k, s), k) for k, s in zip(cells, seeds)]
+
+
+def batch_image(i: int, L: int = 21):
+    """Image i of the batched workload of 64 uneven images: seed 3000+i,
+    H ~ U[240, 1080], W ~ U[320, 1920], one Voronoi cell per 4000 pixels.
+    Returns (W, H, RGB image, unaries)."""
+    rng = np.random.default_rng(3000 + i)
+    H = int(rng.integers(240, 1081))
+    W = int(rng.integers(320, 1921))
+    wl = Workload(W, H, max(1, round(W * H / 4000)), 3000 + i, L=L)
+    return W, H, wl.img, wl.U
+
+
+def batch_images(n: int, workers: int = 16):
+    """The first n images of the batched workload, generated in worker
+    processes (at most `workers`, and no more than the host's CPUs)."""
+    import multiprocessing as mp
+    import os
+    w = max(1, min(workers, os.cpu_count() or 1, n))
+    if w == 1:
+        return [batch_image(i) for i in range(n)]
+    with mp.get_context("spawn").Pool(w) as pool:
+        return pool.map(batch_image, range(n))
+

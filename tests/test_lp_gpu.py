@@ -52,6 +52,12 @@ def test_lp_vs_reference(golden, case):
     Qo, sto = o.lp(g["Qqp"], O.LPParams(prox_max_iter=int(g["lp_outer_cap"])))
     print("gpu sweeps", st["qp_sweeps"][:20], "oracle", sto["qp_sweeps"][:20], flush=True)
     print("gpu int energies", st["int_energy"], "oracle", sto["int_energy"], flush=True)
+    # the float oracle is the reference's LP bit for bit on these fixtures
+    # (test_oracle.py::test_lp_matches_reference), so its counts are the reference's
+    np.testing.assert_array_equal(Qo, g["Qlp"])
+    np.testing.assert_array_equal(st["qp_sweeps"], sto["qp_sweeps"])
+    assert st["outer"] == sto["outer"]
+    np.testing.assert_allclose(st["int_energy"], sto["int_energy"], rtol=1e-4)
     lab, labr = Q.argmax(1), g["Qlp"].argmax(1)
     assert (lab == labr).mean() >= 0.999
     e, er = o.energy_ho(lab, fp64=True), o.energy_ho(labr, fp64=True)
