@@ -56,6 +56,8 @@ SIGNATURES = {
     "dcrf_pairwise_set_kernel_parameters": (_i, [_vp, _vp, _i]),
     "dcrf_pairwise_kernel_gradient": (_i, [_vp, _vp, _vp, _i, _vp, _i]),
     "dcrf_crf_pairwise_kernel_gradient": (_i, [_vp, _i, _vp, _vp, _vp, _i]),
+    "dcrf_crf_pairwise_set_kernel_type": (_i, [_vp, _i, _i]),
+    "dcrf_crf_pairwise_kernel_nparams": (_i, [_vp, _i, _ip]),
     "dcrf_lattice_compute_upper_minus_lower_dc": (_i, [_vp, _vp, _i, _i, _i, _i, _i]),
     "dcrf_pairwise_apply_upper_minus_lower_dc": (_i, [_vp, _vp, _vp, _i, _i]),
     "dcrf_crf_apply_upper_minus_lower_dc": (_i, [_vp, _i, _vp, _vp, _i]),
@@ -69,6 +71,11 @@ SIGNATURES = {
     "dcrf_crf_create": (_i, [_i, _i, C.POINTER(_vp)]),
     "dcrf_crf_destroy": (_i, [_vp]),
     "dcrf_crf_set_unary": (_i, [_vp, _vp, _i]),
+    "dcrf_crf_set_unary_logistic": (_i, [_vp, _vp, _vp, _i, _i]),
+    "dcrf_logistic_unary": (_i, [_vp, _vp, _i, _i, _i, _vp, _i]),
+    "dcrf_logistic_unary_gradient": (_i, [_vp, _vp, _i, _i, _i, _vp, _i]),
+    "dcrf_crf_add_pairwise_potential": (_i, [_vp, _vp]),
+    "dcrf_pairwise_set_compatibility": (_i, [_vp, _i, _vp, _i]),
     "dcrf_crf_add_pairwise": (_i, [_vp, _vp, _i, _f, _i, _i]),
     "dcrf_crf_add_pairwise_compat": (_i, [_vp, _vp, _i, _i, _vp, _i, _i]),
     "dcrf_crf_pairwise_nparams": (_i, [_vp, _i, _ip]),
@@ -77,6 +84,7 @@ SIGNATURES = {
     "dcrf_crf_pairwise_gradient": (_i, [_vp, _i, _vp, _vp, _vp, _i]),
     "dcrf_crf_meanfield": (_i, [_vp, _i, _vp, _i]),
     "dcrf_crf_meanfield_backward": (_i, [_vp, _vp, _vp, _vp, _i]),
+    "dcrf_crf_meanfield_id": (_i, [_vp, C.POINTER(C.c_longlong)]),
     "dcrf_crf_cross_entropy": (_i, [_vp, _vp, _vp, _vp, C.POINTER(_d), _i]),
     "dcrf_crf_add_pairwise_gaussian": (_i, [_vp, _i, _i, _f, _f, _f]),
     "dcrf_crf_add_pairwise_bilateral": (_i, [_vp, _i, _i, _f, _f, _f, _f, _f, _vp, _f]),

@@ -368,9 +368,9 @@ CONST_KERNEL, DIAG_KERNEL, FULL_KERNEL = 0, 1, 2
 
 
 class PairwisePotential:
-    """PairwisePotential (include/pairwise.h:63-95) with a Potts compatibility,
-    standalone (its own lattice); kernel_type defaults to CONST_KERNEL as in
-    the reference's constructor."""
+    """PairwisePotential (include/pairwise.h:63-95) with a Potts, Diagonal or
+    Matrix compatibility, standalone (its own lattice); kernel_type defaults to
+    CONST_KERNEL as in the reference's constructor."""
 
     def __init__(self, features, function: "PottsCompatibility", kernel_type=CONST_KERNEL,
                  normalization_type=3):
@@ -378,18 +378,26 @@ class PairwisePotential:
         a = _Arg(features, np.float32)
         N, d = tuple(a.arr.shape)
         h = C.c_void_p()
-        _check(L.dcrf_pairwise_create(a.ptr, N, d, float(function.w), int(normalization_type), a.dev, C.byref(h)))
+        w = float(function.w) if function.kind == 0 else 1.0
+        _check(L.dcrf_pairwise_create(a.ptr, N, d, w, int(normalization_type), a.dev, C.byref(h)))
         self._h = h
         self.N, self.d = N, d
+        if function.kind != 0:
+            par = np.ascontiguousarray(function.params(), np.float32)
+            nl = par.size if function.kind == 1 else int(round(np.sqrt(par.size)))
+            _check(L.dcrf_pairwise_set_compatibility(self._h, function.kind, par.ctypes.data_as(C.c_void_p), nl))
         _check(L.dcrf_pairwise_set_kernel_type(self._h, int(kernel_type)))
 
-    def __del__(self):
+    def close(self):
         if getattr(self, "_h", None) is not None:
             try:
                 _lib.load().dcrf_pairwise_destroy(self._h)
             except Exception:
                 pass
             self._h = None
+
+    def __del__(self):
+        self.close()
 
     def _field(self, Q, fn, *extra):
         a = _Arg(Q, np.float32)
@@ -471,6 +479,118 @@ class LP_inf_params:
         return p
 
 
+class UnaryEnergy:
+    """UnaryEnergy (include/unary.h:31-39).  get() is the unary as an (N, L)
+    array (the memory of the reference's L x N matrix); parameters, setParameters
+    and gradient are empty unless a subclass overrides them.  A DenseCRF puts
+    the unary on the device with _upload(), again before each computation
+    unless `versioned` says that `version` tells every change (then only when it
+    moved): a subclass whose get() changes on its own is read afresh each time,
+    as the reference calls get() in each solve."""
+
+    versioned = False
+    version = 0
+
+    def get(self):
+        raise NotImplementedError
+
+    def parameters(self):
+        return np.zeros(0, np.float32)
+
+    def setParameters(self, v):
+        pass
+
+    def gradient(self, b):
+        return np.zeros(0, np.float32)
+
+    def _upload(self, crf):
+        a = _Arg(self.get(), np.float32)
+        _check(_lib.load().dcrf_crf_set_unary(crf._h, a.ptr, a.dev))
+
+
+class ConstUnaryEnergy(UnaryEnergy):
+    """ConstUnaryEnergy (include/unary.h:40-46): a fixed (N, L) unary (host
+    array, DeviceArray or torch tensor)."""
+
+    versioned = True
+
+    def __init__(self, unary):
+        self.unary = unary
+
+    def get(self):
+        return self.unary
+
+    def _upload(self, crf):
+        a = _Arg(self.unary, np.float32)
+        _check(_lib.load().dcrf_crf_set_unary(crf._h, a.ptr, a.dev))
+
+
+class LogisticUnaryEnergy(UnaryEnergy):
+    """LogisticUnaryEnergy(L, f) (include/unary.h:47-57, src/unary.cpp:53-68):
+    get() = L f on the GPU's matrix cores.  As every array of this API, the
+    arguments are the transposes of the reference's matrices with the same
+    memory: Lw is (F, L) (the reference's L x F), f is (N, F) (its F x N);
+    get() returns (N, L).  parameters() is Lw flattened (the reference's L
+    resized to L*F x 1) and gradient(b) is b f^T flattened likewise."""
+
+    versioned = True
+
+    def __init__(self, Lw, f):
+        self.Lw = np.ascontiguousarray(Lw, np.float32)
+        self.f = f
+        F = self.Lw.shape[0]
+        if tuple(self.f.shape)[1] != F:
+            raise ValueError("LogisticUnaryEnergy: f must be (N, F) with F = Lw.shape[0]")
+        self.version = 0
+
+    def _dims(self):
+        F, L = self.Lw.shape
+        return L, F, int(tuple(self.f.shape)[0])
+
+    def get(self, out=None):
+        L, F, N = self._dims()
+        x = _Arg(self.f, np.float32)
+        out = _empty_like_field(self.f, (N, L)) if out is None else out
+        o = _Arg(out, np.float32, out=True)
+        w = self.Lw
+        if x.dev:  # device features: the parameters go to the device as well
+            w = DeviceArray.from_numpy(self.Lw)
+        _check(_lib.load().dcrf_logistic_unary(w.ctypes.data_as(C.c_void_p) if not x.dev else w.ptr, x.ptr, L, F, N,
+                                               o.ptr, x.dev))
+        return out
+
+    def parameters(self):
+        return self.Lw.reshape(-1).copy()
+
+    def setParameters(self, v):
+        v = np.ascontiguousarray(v, np.float32).reshape(-1)
+        if v.size != self.Lw.size:
+            raise ValueError("LogisticUnaryEnergy.setParameters: wrong size")
+        self.Lw = v.reshape(self.Lw.shape).copy()
+        self.version += 1
+
+    def gradient(self, b):
+        L, F, N = self._dims()
+        x = _Arg(b, np.float32)
+        y = _Arg(self.f, np.float32)
+        if x.dev != y.dev:
+            raise TypeError("b and f must both be host arrays or both device buffers")
+        if x.dev:
+            g = DeviceArray((F * L,))
+            _check(_lib.load().dcrf_logistic_unary_gradient(x.ptr, y.ptr, L, F, N, g.ptr, 1))
+            return g.numpy()
+        g = np.zeros(F * L, np.float32)
+        _check(_lib.load().dcrf_logistic_unary_gradient(x.ptr, y.ptr, L, F, N, g.ctypes.data_as(C.c_void_p), 0))
+        return g
+
+    def _upload(self, crf):
+        L, F, N = self._dims()
+        x = _Arg(self.f, np.float32)
+        w = DeviceArray.from_numpy(self.Lw) if x.dev else self.Lw
+        _check(_lib.load().dcrf_crf_set_unary_logistic(crf._h, w.ptr if x.dev else w.ctypes.data_as(C.c_void_p),
+                                                       x.ptr, F, x.dev))
+
+
 class DenseCRF:
     """DenseCRF (include/densecrf.h:73-170) on the GPU engine."""
 
@@ -483,6 +603,8 @@ class DenseCRF:
         self.npairwise = 0
         self._pw_dims = []  # feature dimension of each pairwise term
         self.R = 0
+        self._unary = None
+        self._uploaded = (None, 0)  # (unary object, version) on the device
 
     def close(self):
         if getattr(self, "_h", None) is not None:
@@ -496,12 +618,42 @@ class DenseCRF:
             pass
 
     # ---- model -------------------------------------------------------------------
-    def setUnaryEnergy(self, unary):
-        a = _Arg(unary, np.float32)
-        _check(_lib.load().dcrf_crf_set_unary(self._h, a.ptr, a.dev))
+    def setUnaryEnergy(self, unary, f=None):
+        """setUnaryEnergy (src/densecrf.cpp:255-266): an (N, L) array (a
+        ConstUnaryEnergy), a UnaryEnergy object, or (Lw, f) for a
+        LogisticUnaryEnergy(Lw, f)"""
+        if f is not None:
+            unary = LogisticUnaryEnergy(unary, f)
+        elif not isinstance(unary, UnaryEnergy):
+            unary = ConstUnaryEnergy(unary)
+        self._unary = unary
+        self._uploaded = (None, 0)
+        self._sync_unary()
+
+    def getUnaryEnergy(self):
+        return self._unary
+
+    def _sync_unary(self):
+        u = self._unary
+        if u is None:
+            return
+        if self._uploaded[0] is u and u.versioned and self._uploaded[1] == u.version:
+            return
+        u._upload(self)
+        self._uploaded = (u, u.version)
 
     def addPairwiseEnergy(self, features, function=None, kernel_type=1, normalization_type=NORMALIZE_SYMMETRIC):
-        """addPairwiseEnergy(features (N, d), LabelCompatibility, kernel type, normalisation)."""
+        """addPairwiseEnergy(features (N, d), LabelCompatibility, kernel type, normalisation),
+        or addPairwiseEnergy(PairwisePotential) (src/densecrf.cpp:161-168): the
+        potential's term is added and the object is consumed, as the reference
+        takes ownership of it."""
+        if isinstance(features, PairwisePotential):
+            pw = features
+            _check(_lib.load().dcrf_crf_add_pairwise_potential(self._h, pw._h))
+            self.npairwise += 1
+            self._pw_dims.append(pw.d)
+            pw.close()
+            return
         function = function or PottsCompatibility(1.0)
         a = _Arg(features, np.float32)
         d = a.arr.shape[1]
@@ -512,6 +664,7 @@ class DenseCRF:
             _check(_lib.load().dcrf_crf_add_pairwise_compat(self._h, a.ptr, d, function.kind,
                                                             par.ctypes.data_as(C.c_void_p),
                                                             int(normalization_type), a.dev))
+        _check(_lib.load().dcrf_crf_pairwise_set_kernel_type(self._h, self.npairwise, int(kernel_type)))
         self.npairwise += 1
         self._pw_dims.append(d)
 
@@ -547,14 +700,22 @@ class DenseCRF:
     def meanfield(self, iterations: int, out=None, device=None):
         """Q after `iterations` mean-field updates from softmax(-U); keeps the
         per-iteration checkpoints for meanfield_backward."""
+        self._sync_unary()
         if out is None:
             out = np.empty((self.N, self.M), np.float32) if device is None else _torch_empty((self.N, self.M), device)
         o = _Arg(out, np.float32, out=True)
         _check(_lib.load().dcrf_crf_meanfield(self._h, int(iterations), o.ptr, o.dev))
         return out
 
+    def meanfield_id(self) -> int:
+        """number of the last meanfield forward on this CRF (0: none yet)"""
+        n = C.c_longlong()
+        _check(_lib.load().dcrf_crf_meanfield_id(self._h, C.byref(n)))
+        return n.value
+
     def meanfield_backward(self, dQ, out=None):
-        """dL/dQ_T -> (dL/dU, [dL/dparameters of each potential] (FP64))."""
+        """dL/dQ_T -> (dL/dU, [dL/dparameters of each potential] (FP64)) of the
+        last meanfield; raises if the unary or a pairwise term changed since."""
         a = _Arg(dQ, np.float32)
         out = _empty_like_field(dQ, (self.N, self.M)) if out is None else out
         o = _Arg(out, np.float32, out=True)
@@ -605,13 +766,16 @@ class DenseCRF:
         return out
 
     def pairwise_kernel_gradient(self, k, b, Q):
-        """PairwisePotential::kernelGradient(b, Q) of potential k (DIAG kernel): (d,)"""
+        """PairwisePotential::kernelGradient(b, Q) of potential k: (0,), (d,) or
+        (d*d,) values for a CONST, DIAG or FULL kernel"""
         x = _Arg(b, np.float32)
         q = _Arg(Q, np.float32)
-        out = np.zeros(16, np.float32)
+        n = C.c_int()
+        _check(_lib.load().dcrf_crf_pairwise_kernel_nparams(self._h, int(k), C.byref(n)))
+        out = np.zeros(max(n.value, 1), np.float32)
         _check(_lib.load().dcrf_crf_pairwise_kernel_gradient(self._h, k, x.ptr, q.ptr,
                                                              out.ctypes.data_as(C.c_void_p), x.dev))
-        return out[: self._pw_dims[k]]
+        return out[: n.value]
 
     def apply_bf(self, k, Q, out=None):
         """PairwisePotential::apply_bf of potential k (dense Gaussian, O(N^2 L))."""
@@ -658,6 +822,7 @@ class DenseCRF:
         return out
 
     def unary_init(self, device=None):
+        self._sync_unary()
         out = np.empty((self.N, self.M), np.float32) if device is None else _torch_empty((self.N, self.M), device)
         o = _Arg(out, np.float32, out=True)
         _check(_lib.load().dcrf_crf_unary_init(self._h, o.ptr, o.dev))
@@ -670,6 +835,7 @@ class DenseCRF:
         with ref_stop=False runs exactly max_iter iterations."""
         if ref_stop is None:
             ref_stop = max_iter <= 0
+        self._sync_unary()
         a = _Arg(init, np.float32)
         out = _empty_like_field(init, (self.N, self.M)) if out is None else out
         o = _Arg(out, np.float32, out=True)
@@ -691,6 +857,7 @@ class DenseCRF:
     def lp_inference_prox_super_pixels(self, init, params: LP_inf_params | None = None, stats: bool = False,
                                        out=None):
         params = params or LP_inf_params()
+        self._sync_unary()
         a = _Arg(init, np.float32)
         out = _empty_like_field(init, (self.N, self.M)) if out is None else out
         o = _Arg(out, np.float32, out=True)
@@ -720,6 +887,7 @@ class DenseCRF:
         return out
 
     def _energy(self, fn, x, dtype):
+        self._sync_unary()
         a = _Arg(x, dtype)
         e = C.c_double()
         _check(fn(self._h, a.ptr, a.dev, C.byref(e)))
@@ -727,6 +895,7 @@ class DenseCRF:
 
     def unaryEnergy(self, labels):
         """DenseCRF::unaryEnergy(l): per-pixel U(l_i, i) (float32, (N,))"""
+        self._sync_unary()
         lab = np.ascontiguousarray(labels, np.int32)
         out = np.zeros(self.N, np.float32)
         _check(_lib.load().dcrf_crf_unary_energy(self._h, lab.ctypes.data_as(C.c_void_p),
@@ -792,6 +961,7 @@ class DenseCRF2D(DenseCRF):
                                           normalization_type)
         w = (function or PottsCompatibility(1.0)).w
         _check(_lib.load().dcrf_crf_add_pairwise_gaussian(self._h, self.W, self.H, float(sx), float(sy), w))
+        _check(_lib.load().dcrf_crf_pairwise_set_kernel_type(self._h, self.npairwise, int(kernel_type)))
         self.npairwise += 1
         self._pw_dims.append(2)
 
@@ -806,6 +976,7 @@ class DenseCRF2D(DenseCRF):
         _check(_lib.load().dcrf_crf_add_pairwise_bilateral(self._h, self.W, self.H, float(sx), float(sy), float(sr),
                                                            float(sg), float(sb),
                                                            im.ctypes.data_as(C.c_void_p), w))
+        _check(_lib.load().dcrf_crf_pairwise_set_kernel_type(self._h, self.npairwise, int(kernel_type)))
         self.npairwise += 1
         self._pw_dims.append(5)
 

@@ -400,11 +400,14 @@ void potential_gradient_dev(Potential& p, int L, const float* b, const float* Q,
 // ---------------------------------------------------------------------------
 struct CRF::MFState {
   int T = -1;
+  unsigned long long model_gen = 0;  // model generation the checkpoints belong to
+  long long id = 0;                  // number of this forward run
   DBuf<float> Qs, Ms, g1, g2, b, dU;
   DBuf<double> parts, acc;
 };
 
 int CRF::meanfield_iterations() const { return mf_ ? mf_->T : -1; }
+long long CRF::meanfield_id() const { return mf_ ? mf_->id : 0; }
 
 // DenseKernel::apply / applyTranspose of potential p (no compatibility), out (+)= K in
 static void kernel_filter(Potential& p, const float* in, float* out, int V, bool transpose) {
@@ -437,6 +440,8 @@ void CRF::meanfield(int T, float* Qout) {
   st.Qs.resize(NL * (T + 1));
   st.Ms.resize(std::max<size_t>(NL * T * K, 1));
   st.T = T;
+  st.model_gen = model_gen_;
+  st.id = ++mf_count_;
   k_softmax_neg<<<grid_for(N_), kBlock, 0, s>>>(U_.get(), st.Qs.get(), N_, L_);
   check_launch("k_softmax_neg");
   const unsigned grid = rg_grid(N_, L_, 8192);
@@ -461,6 +466,8 @@ void CRF::meanfield(int T, float* Qout) {
 void CRF::meanfield_backward(const float* dQ, float* dU, double* dparams) {
   DCRF_REQUIRE(mf_ && mf_->T >= 0, "meanfield_backward: run meanfield first");
   MFState& st = *mf_;
+  DCRF_REQUIRE(st.model_gen == model_gen_,
+               "meanfield_backward: the unary or a pairwise term changed after the last meanfield");
   const int T = st.T, K = num_pairwise();
   const size_t NL = (size_t)N_ * L_;
   hipStream_t s = s_;

@@ -163,8 +163,11 @@ void check_unit_interval(const float* d, size_t n, hipStream_t s) {
 
 // pI of the restricted ordered filter: strictly ascending pixel indices in
 // [0, N) (the reference walks the pixels and pI together and needs that order)
-void check_pixel_list(const int* d_pI, int nI, int N, hipStream_t s) {
+void check_pixel_list(const int* d_pI, int nI, int N, bool store, hipStream_t s) {
   DCRF_REQUIRE(nI >= 0 && nI <= N, "restricted ordered filter: bad pixel count");
+  // the reference reads pI[0] when it stores (src/permutohedral.cpp:724-730): an
+  // empty list has no defined result there, so it is refused
+  DCRF_REQUIRE(nI > 0 || !store, "restricted ordered filter: a call with store needs at least one pixel");
   if (nI == 0) return;
   std::vector<int> h(nI);
   DCRF_HIP(hipMemcpyAsync(h.data(), d_pI, nI * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -268,7 +271,7 @@ int dcrf_lattice_compute_upper_minus_lower_ord_restricted(dcrf_lattice_t lat, fl
     Staged<float> a(rQ, nr, on_device != 0, s, true);
     Staged<float> o(rout, nr, on_device != 0, s, true);  // added to
     Staged<float> e(store ? Q : nullptr, store ? (size_t)N * V : 0, on_device != 0, s, store != 0);
-    check_pixel_list(ix.d, nI, N, s);
+    check_pixel_list(ix.d, nI, N, store != 0, s);
     if (nr) check_unit_interval(a.d, nr, s);
     if (store) check_unit_interval(e.d, (size_t)N * V, s);
     lat->lat->ordered_restricted(a.d, ix.d, nI, e.d, V, store != 0, o.d);
@@ -356,7 +359,7 @@ static void apply_ord_restricted(Potential& p, hipStream_t s, float* rout, const
   Staged<float> a(rQ, nr, on_device != 0, s, true);
   Staged<float> o(rout, nr, on_device != 0, s, false);
   Staged<float> e(store ? Q : nullptr, store ? (size_t)N * L : 0, on_device != 0, s, store != 0);
-  check_pixel_list(ix.d, nI, N, s);
+  check_pixel_list(ix.d, nI, N, store != 0, s);
   if (nr) check_unit_interval(a.d, nr, s);
   if (store) check_unit_interval(e.d, (size_t)N * L, s);
   potential_apply_ord_restricted(p, o.d, a.d, ix.d, nI, e.d, L, store != 0);
@@ -418,6 +421,14 @@ int dcrf_pairwise_apply_upper_minus_lower_bf_ord(dcrf_pairwise_t pw, const int* 
   });
 }
 
+int dcrf_pairwise_set_compatibility(dcrf_pairwise_t pw, int compat_type, const float* params, int L) {
+  return guarded([&] {
+    DCRF_REQUIRE(pw && params && compat_type >= DCRF_COMPAT_POTTS && compat_type <= DCRF_COMPAT_MATRIX &&
+                     (compat_type == DCRF_COMPAT_POTTS || L > 0),
+                 "dcrf_pairwise_set_compatibility: bad arguments");
+    potential_set_compat(pw->p, compat_type, params, L, pw->s);
+  });
+}
 int dcrf_pairwise_set_kernel_type(dcrf_pairwise_t pw, int ktype) {
   return guarded([&] {
     DCRF_REQUIRE(pw && ktype >= 0 && ktype <= 2, "dcrf_pairwise_set_kernel_type: bad arguments");
@@ -515,6 +526,7 @@ int dcrf_crf_pairwise_set_parameters(dcrf_crf_t crf, int k, const float* params)
     DCRF_REQUIRE(crf && params && k >= 0 && k < crf->crf->num_pairwise(),
                  "dcrf_crf_pairwise_set_parameters: bad arguments");
     potential_set_parameters(crf->crf->pairwise(k), crf->crf->L(), params, crf->crf->stream());
+    crf->crf->touch_model();
   });
 }
 int dcrf_crf_pairwise_gradient(dcrf_crf_t crf, int k, const float* b, const float* Q, float* grad, int on_device) {
@@ -549,6 +561,12 @@ int dcrf_crf_meanfield_backward(dcrf_crf_t crf, const float* dQ, float* dU, doub
     c.meanfield_backward(g.d, u.d, dparams);
     u.copy_out();
     DCRF_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
+int dcrf_crf_meanfield_id(dcrf_crf_t crf, long long* id) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && id, "dcrf_crf_meanfield_id: bad arguments");
+    *id = crf->crf->meanfield_id();
   });
 }
 int dcrf_crf_cross_entropy(dcrf_crf_t crf, const float* Q, const int* labels, float* dQ, double* loss,
@@ -672,6 +690,45 @@ int dcrf_crf_pairwise_kernel_gradient(dcrf_crf_t crf, int k, const float* b, con
     CRF& c = *crf->crf;
     DCRF_REQUIRE(k >= 0 && k < c.num_pairwise(), "dcrf_crf_pairwise_kernel_gradient: no such potential");
     kernel_gradient_common(c.pairwise(k), c.stream(), b, Q, c.L(), out, on_device);
+  });
+}
+int dcrf_crf_add_pairwise_potential(dcrf_crf_t crf, dcrf_pairwise_t pw) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && pw && pw->p.lat, "dcrf_crf_add_pairwise_potential: bad arguments");
+    CRF& c = *crf->crf;
+    Potential& src = pw->p;
+    DCRF_REQUIRE(src.lat->N() == c.N(), "dcrf_crf_add_pairwise_potential: the potential has another pixel count");
+    DCRF_REQUIRE(src.ctype == kPotts || src.cL == c.L(),
+                 "dcrf_crf_add_pairwise_potential: the compatibility has another label count");
+    DCRF_REQUIRE(src.feat.get() != nullptr, "dcrf_crf_add_pairwise_potential: the potential keeps no features");
+    DCRF_HIP(hipStreamSynchronize(pw->s));
+    // the same potential over the CRF's stream: features, normalisation,
+    // compatibility, kernel type and parameters; the no-norm copy takes
+    // parameters()(0) now, as DenseCRF::addPairwiseEnergy does (src/densecrf.cpp:161-168)
+    c.add_pairwise(src.feat.get(), src.d, src.w, src.ntype);
+    Potential& dst = c.pairwise(c.num_pairwise() - 1);
+    if (src.ctype != kPotts) potential_set_compat(dst, src.ctype, src.cparam.data(), src.cL, c.stream());
+    dst.ktype = src.ktype;
+    if (!src.kparams.empty())
+      potential_set_kernel_parameters(dst, src.kparams.data(), (int)src.kparams.size(), c.stream());
+    DCRF_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
+int dcrf_crf_pairwise_set_kernel_type(dcrf_crf_t crf, int k, int ktype) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && ktype >= 0 && ktype <= 2, "dcrf_crf_pairwise_set_kernel_type: bad arguments");
+    CRF& c = *crf->crf;
+    DCRF_REQUIRE(k >= 0 && k < c.num_pairwise(), "dcrf_crf_pairwise_set_kernel_type: no such potential");
+    DCRF_REQUIRE(c.pairwise(k).kparams.empty(), "dcrf_crf_pairwise_set_kernel_type: set before any kernel parameters");
+    c.pairwise(k).ktype = ktype;
+  });
+}
+int dcrf_crf_pairwise_kernel_nparams(dcrf_crf_t crf, int k, int* n) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && n, "dcrf_crf_pairwise_kernel_nparams: bad arguments");
+    CRF& c = *crf->crf;
+    DCRF_REQUIRE(k >= 0 && k < c.num_pairwise(), "dcrf_crf_pairwise_kernel_nparams: no such potential");
+    *n = potential_kernel_nparams(c.pairwise(k));
   });
 }
 int dcrf_crf_unary_init(dcrf_crf_t crf, float* Q, int on_device) {
@@ -935,6 +992,7 @@ int dcrf_crf_set_value_type(dcrf_crf_t crf, int type) {
     DCRF_REQUIRE(crf && (type == DCRF_VALUES_F32 || type == DCRF_VALUES_BF16), "dcrf_crf_set_value_type: bad arguments");
     CRF& c = *crf->crf;
     for (int k = 0; k < c.num_pairwise(); ++k) c.pairwise(k).lat->set_bf16_rows(type == DCRF_VALUES_BF16);
+    c.touch_model();
   });
 }
 int dcrf_batch_graph_captures(dcrf_batch_t b, long long* n) {
@@ -1153,6 +1211,46 @@ int dcrf_less_confident_pixels(const float* Q, int N, int L, float tol, int* ind
     o.copy_out();
     DCRF_HIP(hipStreamSynchronize(s));
     *count = n;
+  });
+}
+
+// ---- unary energies (include/unary.h:31-57) -----------------------------------------
+int dcrf_logistic_unary(const float* Lw, const float* f, int L, int F, int N, float* U, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(Lw && f && U && L > 0 && F > 0 && N > 0, "dcrf_logistic_unary: bad arguments");
+    hipStream_t s = nullptr;
+    Staged<float> w(Lw, (size_t)L * F, on_device != 0, s, true);
+    Staged<float> x(f, (size_t)F * N, on_device != 0, s, true);
+    Staged<float> u(U, (size_t)L * N, on_device != 0, s, false);
+    logistic_unary_dev(w.d, x.d, L, F, N, u.d, s);
+    u.copy_out();
+    DCRF_HIP(hipStreamSynchronize(s));
+  });
+}
+int dcrf_logistic_unary_gradient(const float* b, const float* f, int L, int F, int N, float* g, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(b && f && g && L > 0 && F > 0 && N > 0, "dcrf_logistic_unary_gradient: bad arguments");
+    hipStream_t s = nullptr;
+    Staged<float> x(b, (size_t)L * N, on_device != 0, s, true);
+    Staged<float> y(f, (size_t)F * N, on_device != 0, s, true);
+    Staged<float> o(g, (size_t)L * F, on_device != 0, s, false);
+    DBuf<float> part(logistic_grad_parts(L, F, N));
+    logistic_unary_gradient_dev(x.d, y.d, L, F, N, o.d, part.get(), s);
+    o.copy_out();
+    DCRF_HIP(hipStreamSynchronize(s));
+  });
+}
+int dcrf_crf_set_unary_logistic(dcrf_crf_t crf, const float* Lw, const float* f, int F, int on_device) {
+  return guarded([&] {
+    DCRF_REQUIRE(crf && Lw && f && F > 0, "dcrf_crf_set_unary_logistic: bad arguments");
+    CRF& c = *crf->crf;
+    hipStream_t s = c.stream();
+    Staged<float> w(Lw, (size_t)c.L() * F, on_device != 0, s, true);
+    Staged<float> x(f, (size_t)F * c.N(), on_device != 0, s, true);
+    DBuf<float> u((size_t)c.L() * c.N());
+    logistic_unary_dev(w.d, x.d, c.L(), F, c.N(), u.get(), s);
+    c.set_unary(u.get());
+    DCRF_HIP(hipStreamSynchronize(s));
   });
 }
 

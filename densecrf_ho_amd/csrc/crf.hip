@@ -77,6 +77,7 @@ CRF::~CRF() {
 }
 
 void CRF::set_unary(const float* dU) {
+  touch_model();
   DCRF_HIP(hipMemcpyAsync(U_.get(), dU, U_.bytes(), hipMemcpyDeviceToDevice, s_));
 }
 
@@ -119,6 +120,7 @@ void potential_init_lattice(Potential& p, std::unique_ptr<Lattice> lat, float w,
 }
 
 void CRF::add_pairwise(const float* d_feat, int d, float w, int ntype) {
+  touch_model();
   auto p = std::make_unique<Potential>();
   potential_init(*p, d_feat, N_, d, w, ntype, s_);
   pw_.push_back(std::move(p));

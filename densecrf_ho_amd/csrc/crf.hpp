@@ -183,6 +183,14 @@ class CRF {
   // back to back, into the host array dparams (FP64)
   void meanfield_backward(const float* dQ, float* dU, double* dparams);
   int meanfield_iterations() const;
+  // The model's generation: it changes whenever the unary, a pairwise term or
+  // its parameters change.  meanfield records it, and meanfield_backward
+  // refuses to run when it has changed since (the checkpoints would belong to
+  // another model).  meanfield_id() numbers the forward runs (0: none yet), so
+  // a caller holding an earlier forward can tell that it was overwritten.
+  void touch_model() { ++model_gen_; }
+  unsigned long long model_generation() const { return model_gen_; }
+  long long meanfield_id() const;
   // -sum_i log Q(y_i, i) over labelled pixels (y_i in [0,L)) and its gradient
   double cross_entropy(const float* Q, const int* lab, float* dQ);
   // PairwisePotential::gradient for potential k, host result (parameters() layout)
@@ -213,6 +221,8 @@ class CRF {
 
   int N_, L_;
   int R_ = 0;
+  unsigned long long model_gen_ = 0;
+  long long mf_count_ = 0;
   hipStream_t s_ = nullptr;
   DBuf<float> U_;
   OrderedBins obins_;  // bins of the field the ordered filters of all potentials share

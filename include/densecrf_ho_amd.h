@@ -130,6 +130,11 @@ int dcrf_pairwise_apply_upper_minus_lower_ord(dcrf_pairwise_t pw, const float* Q
  * PairwisePotential's constructor, include/pairwise.h:40-44,71; 1 DIAG_KERNEL;
  * 2 FULL_KERNEL); set before any kernel parameters. */
 int dcrf_pairwise_set_kernel_type(dcrf_pairwise_t pw, int ktype);
+/* The LabelCompatibility of a standalone potential (PairwisePotential(features, function,
+ * ...), src/pairwise.cpp:186-202; include/labelcompatibility.h:51-92): Potts (params[0]),
+ * Diagonal (L values) or Matrix (L*L column-major, stored as (m + m^T)/2).  Its filters
+ * then take fields of L labels (Potts: any). */
+int dcrf_pairwise_set_compatibility(dcrf_pairwise_t pw, int compat_type, const float* params, int L);
 /* PairwisePotential::kernelParameters / setKernelParameters, src/pairwise.cpp:396-401
  * (DenseKernel::parameters / setParameters, :136-161): 0, d or d*d values (DIAG: the
  * feature scales, ones at first; FULL: a d x d column-major matrix, the identity at
@@ -176,6 +181,18 @@ int dcrf_crf_create(int N, int L, dcrf_crf_t* out);
 int dcrf_crf_destroy(dcrf_crf_t crf);
 /* DenseCRF::setUnaryEnergy(const MatrixXf& unary), src/densecrf.cpp:261-263 */
 int dcrf_crf_set_unary(dcrf_crf_t crf, const float* unary, int on_device);
+/* DenseCRF::setUnaryEnergy(const MatrixXf& L, const MatrixXf& f), src/densecrf.cpp:264-266:
+ * the unary becomes LogisticUnaryEnergy(Lw, f).get() = Lw f, computed on the device
+ * (dcrf_logistic_unary); Lw is L x F, f is F x N. */
+int dcrf_crf_set_unary_logistic(dcrf_crf_t crf, const float* Lw, const float* f, int F, int on_device);
+/* LogisticUnaryEnergy::get(), src/unary.cpp:56-58 (include/unary.h:49-57): U (L x N) =
+ * Lw (L x F) f (F x N), column-major (U[p*L + l], Lw[k*L + l], f[p*F + k]); fp32 matrix
+ * cores (v_mfma_f32_16x16x4_f32), fp32 accumulation.  1 <= L <= 64. */
+int dcrf_logistic_unary(const float* Lw, const float* f, int L, int F, int N, float* U, int on_device);
+/* LogisticUnaryEnergy::gradient(b), src/unary.cpp:64-68: g = b f^T (L x F, column-major,
+ * i.e. the reference's result resized to L*F x 1), b L x N.  fp32 matrix-core partial
+ * sums over blocks of 4096 pixels, added in block order in FP64. */
+int dcrf_logistic_unary_gradient(const float* b, const float* f, int L, int F, int N, float* g, int on_device);
 /* DenseCRF::addPairwiseEnergy(features, new PottsCompatibility(w), DIAG_KERNEL, ntype),
  * src/densecrf.cpp:157-169 (also keeps the NO_NORMALIZATION copy used by
  * the energies; here both share one lattice) */
@@ -218,8 +235,23 @@ int dcrf_crf_apply_upper_minus_lower_ord(dcrf_crf_t crf, int k, const float* Q, 
 /* PairwisePotential::apply_upper_minus_lower_ord_restricted of potential k,
  * src/pairwise.cpp:229-239, with that potential's compatibility (Potts, Diagonal or
  * Matrix); rout is L x nI, Q is L x N. */
-/* PairwisePotential::kernelGradient(b, Q) of potential k (a DIAG_KERNEL, the default of
- * addPairwiseEnergy): d values into out (host). */
+/* The KernelType of potential k (0 CONST_KERNEL, 1 DIAG_KERNEL, 2 FULL_KERNEL): the
+ * kernel_type argument of DenseCRF::addPairwiseEnergy (src/densecrf.cpp:157-160, which
+ * builds PairwisePotential(features, function, kernel_type, ntype), include/pairwise.h:
+ * 40-44,71).  DIAG_KERNEL after adding, as the reference's default; set before any
+ * kernel parameters.  With the initial (unit) parameters every type filters alike; the
+ * type fixes the kernel parameters and their gradient: 0, d or d*d values
+ * (dcrf_crf_pairwise_kernel_nparams). */
+int dcrf_crf_pairwise_set_kernel_type(dcrf_crf_t crf, int k, int ktype);
+/* DenseCRF::addPairwiseEnergy(PairwisePotential* potential), src/densecrf.cpp:161-168: adds
+ * the potential of pw (its features, normalisation, compatibility, kernel type and kernel
+ * parameters) as the CRF's next term, with the NO_NORMALIZATION Potts copy of its
+ * parameters()(0) taken now.  pw itself is not changed; the reference takes ownership of
+ * the object, so the C++ layer destroys pw afterwards. */
+int dcrf_crf_add_pairwise_potential(dcrf_crf_t crf, dcrf_pairwise_t pw);
+int dcrf_crf_pairwise_kernel_nparams(dcrf_crf_t crf, int k, int* n);
+/* PairwisePotential::kernelGradient(b, Q) of potential k: dcrf_crf_pairwise_kernel_nparams
+ * values into out (host). */
 int dcrf_crf_pairwise_kernel_gradient(dcrf_crf_t crf, int k, const float* b, const float* Q, float* out,
                                       int on_device);
 /* The brute-force filters of potential k (apply_bf, apply_upper_minus_lower_bf,
@@ -319,6 +351,12 @@ int dcrf_crf_meanfield(dcrf_crf_t crf, int iterations, float* Q, int on_device);
  * gradient of every potential's parameters (parameters() layout, potentials
  * back to back, host FP64 array; may be null). */
 int dcrf_crf_meanfield_backward(dcrf_crf_t crf, const float* dQ, float* dU, double* dparams, int on_device);
+/* meanfield_backward fails (DCRF_INVALID_ARGUMENT) when the unary, a pairwise term or its
+ * parameters changed after the last dcrf_crf_meanfield: its checkpoints belong to that
+ * forward.  dcrf_crf_meanfield_id numbers the forward runs (1, 2, ...; 0 before the first),
+ * so a caller that kept an earlier forward's number can tell that it was overwritten.  No
+ * reference counterpart. */
+int dcrf_crf_meanfield_id(dcrf_crf_t crf, long long* id);
 /* loss = -sum_i log Q(labels[i], i) over pixels with labels[i] in [0,L) (FP64)
  * and dQ = dloss/dQ.  No reference counterpart: the upstream loss of the
  * unrolled-inference gradient. */

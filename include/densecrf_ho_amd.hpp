@@ -31,6 +31,7 @@ namespace dcrf_ho {
 
 #ifdef DCRF_USE_EIGEN
 using MatrixXf = Eigen::MatrixXf;
+using VectorXf = Eigen::VectorXf;
 using MatrixXi = Eigen::MatrixXi;
 using VectorXs = Eigen::Matrix<short, Eigen::Dynamic, 1>;
 #else
@@ -60,6 +61,16 @@ class DenseMatrix {
 };
 using MatrixXf = DenseMatrix<float>;
 using MatrixXi = DenseMatrix<int>;
+// column vector (the subset of Eigen::VectorXf the API uses)
+class VectorXf : public DenseMatrix<float> {
+ public:
+  VectorXf() = default;
+  explicit VectorXf(long n) : DenseMatrix<float>(n, 1) {}
+  float& operator()(long i) { return data()[i]; }
+  float operator()(long i) const { return data()[i]; }
+  float& operator[](long i) { return data()[i]; }
+  float operator[](long i) const { return data()[i]; }
+};
 class VectorXs {
  public:
   VectorXs() = default;
@@ -126,6 +137,83 @@ class MatrixCompatibility : public LabelCompatibility {
 
  protected:
   std::vector<float> m_;
+};
+
+inline VectorXf vector_of(const float* p, long n) {
+  VectorXf v(n);
+  std::copy(p, p + n, v.data());
+  return v;
+}
+
+// UnaryEnergy (include/unary.h:31-39): get() is the L x N unary; parameters,
+// setParameters and gradient are empty unless a subclass overrides them.  The
+// engine reads the unary through upload(): the default copies get() to the
+// device; the classes below put theirs there directly.  A DenseCRF uploads its
+// unary again before each computation unless versioned() says that version()
+// tells every change (then only when it moved), so a subclass whose get()
+// changes on its own is read afresh each time, as the reference calls get().
+class UnaryEnergy {
+ public:
+  virtual ~UnaryEnergy() = default;
+  virtual MatrixXf get() const = 0;
+  virtual VectorXf parameters() const { return VectorXf(); }
+  virtual void setParameters(const VectorXf&) {}
+  virtual VectorXf gradient(const MatrixXf&) const { return VectorXf(); }
+  virtual void upload(dcrf_crf_t h) const {
+    const MatrixXf u = get();
+    check(dcrf_crf_set_unary(h, u.data(), 0));
+  }
+  virtual bool versioned() const { return false; }
+  unsigned long long version() const { return version_; }
+
+ protected:
+  void touch() { ++version_; }
+  unsigned long long version_ = 0;
+};
+// ConstUnaryEnergy (include/unary.h:40-46, src/unary.cpp:46-51)
+class ConstUnaryEnergy : public UnaryEnergy {
+ public:
+  explicit ConstUnaryEnergy(const MatrixXf& unary) : unary_(unary) {}
+  MatrixXf get() const override { return unary_; }
+  void upload(dcrf_crf_t h) const override { check(dcrf_crf_set_unary(h, unary_.data(), 0)); }
+  bool versioned() const override { return true; }
+
+ protected:
+  MatrixXf unary_;
+};
+// LogisticUnaryEnergy (include/unary.h:47-57, src/unary.cpp:53-68): get() = L f on
+// the GPU's matrix cores; parameters() is L resized to L*F x 1 (column-major);
+// gradient(b) = b f^T resized likewise
+class LogisticUnaryEnergy : public UnaryEnergy {
+ public:
+  LogisticUnaryEnergy(const MatrixXf& L, const MatrixXf& feature) : L_(L), f_(feature) {
+    if (L.cols() != feature.rows()) throw std::runtime_error("LogisticUnaryEnergy: L.cols() != f.rows()");
+  }
+  MatrixXf get() const override {
+    MatrixXf u(L_.rows(), f_.cols());
+    check(dcrf_logistic_unary(L_.data(), f_.data(), (int)L_.rows(), (int)L_.cols(), (int)f_.cols(), u.data(), 0));
+    return u;
+  }
+  VectorXf parameters() const override { return vector_of(L_.data(), (long)L_.rows() * L_.cols()); }
+  void setParameters(const VectorXf& v) override {
+    if ((long)v.rows() != (long)L_.rows() * L_.cols())
+      throw std::runtime_error("LogisticUnaryEnergy::setParameters: wrong size");
+    std::copy(v.data(), v.data() + v.rows(), L_.data());
+    touch();
+  }
+  VectorXf gradient(const MatrixXf& b) const override {
+    VectorXf g((long)L_.rows() * L_.cols());
+    check(dcrf_logistic_unary_gradient(b.data(), f_.data(), (int)L_.rows(), (int)L_.cols(), (int)f_.cols(), g.data(),
+                                       0));
+    return g;
+  }
+  void upload(dcrf_crf_t h) const override {
+    check(dcrf_crf_set_unary_logistic(h, L_.data(), f_.data(), (int)L_.cols(), 0));
+  }
+  bool versioned() const override { return true; }
+
+ protected:
+  MatrixXf L_, f_;
 };
 
 // LP_inf_params (include/densecrf.h:43-69), defaults of src/densecrf.cpp:287-302
@@ -254,8 +342,15 @@ class PairwisePotential {
       : compat_(compatibility), f_(features), ktype_(ktype), ntype_(ntype) {
     check(dcrf_pairwise_create(features.data(), (int)features.cols(), (int)features.rows(),
                                compatibility->potts_weight(), (int)ntype, 0, &h_));
+    if (compatibility->type() != DCRF_COMPAT_POTTS) {
+      const std::vector<float> p = compatibility->c_params();
+      const int L = compatibility->type() == DCRF_COMPAT_DIAGONAL ? (int)p.size()
+                                                                   : (int)std::lround(std::sqrt((double)p.size()));
+      check(dcrf_pairwise_set_compatibility(h_, compatibility->type(), p.data(), L));
+    }
     check(dcrf_pairwise_set_kernel_type(h_, (int)ktype));
   }
+  dcrf_pairwise_t handle() const { return h_; }
   ~PairwisePotential() {
     if (h_) dcrf_pairwise_destroy(h_);
   }
@@ -344,6 +439,14 @@ class DenseCRF {
       check(dcrf_crf_add_pairwise_compat(h_, features.data(), (int)features.rows(), f->type(), p.data(),
                                          (int)normalization_type, 0));
     }
+    check(dcrf_crf_pairwise_set_kernel_type(h_, npairwise_, (int)kernel_type));
+    ++npairwise_;
+  }
+  // DenseCRF::addPairwiseEnergy(PairwisePotential*) (src/densecrf.cpp:161-168):
+  // ownership of `potential` is transferred, as in the reference
+  virtual void addPairwiseEnergy(PairwisePotential* potential) {
+    std::unique_ptr<PairwisePotential> p(potential);
+    check(dcrf_crf_add_pairwise_potential(h_, p->handle()));
     ++npairwise_;
   }
   // PairwisePotential::parameters / setParameters / gradient(b, Q) of term k
@@ -367,6 +470,7 @@ class DenseCRF {
   // unrolled mean field from softmax(-U) and its adjoint: dL/dQ_T -> dL/dU
   // and the parameter gradient of every term (back to back, FP64)
   MatrixXf meanfield(int iterations) {
+    sync_unary();
     MatrixXf Q(M_, N_);
     check(dcrf_crf_meanfield(h_, iterations, Q.data(), 0));
     return Q;
@@ -380,7 +484,15 @@ class DenseCRF {
     if (dparams) dparams->assign(g.begin(), g.begin() + n);
     return dU;
   }
-  void setUnaryEnergy(const MatrixXf& unary) { check(dcrf_crf_set_unary(h_, unary.data(), 0)); }
+  // setUnaryEnergy / getUnaryEnergy (src/densecrf.cpp:255-270): the CRF owns the unary
+  void setUnaryEnergy(UnaryEnergy* unary) {
+    unary_.reset(unary);
+    uploaded_ = nullptr;
+    sync_unary();
+  }
+  void setUnaryEnergy(const MatrixXf& unary) { setUnaryEnergy(new ConstUnaryEnergy(unary)); }
+  void setUnaryEnergy(const MatrixXf& L, const MatrixXf& f) { setUnaryEnergy(new LogisticUnaryEnergy(L, f)); }
+  UnaryEnergy* getUnaryEnergy() { return unary_.get(); }
   // superpixel layer from a region map (the state addSuperPixel derives from
   // mean-shift): region[i] in [0,R), exp_of_superpixels_ e and constants_ c
   void addSuperPixelLayer(const std::vector<int>& region, const std::vector<float>& e,
@@ -389,17 +501,20 @@ class DenseCRF {
   }
   MatrixXf unary_init() const {
     MatrixXf Q(M_, N_);
+    sync_unary();
     check(dcrf_crf_unary_init(h_, Q.data(), 0));
     return Q;
   }
   // reference stopping rule (src/densecrf.cpp:390); max_iter > 0 runs a fixed count
   const MatrixXf qp_inference_super_pixels_non_convex(const MatrixXf& init, int max_iter = 0) {
     MatrixXf Q(M_, N_);
+    sync_unary();
     check(dcrf_crf_qp(h_, init.data(), Q.data(), max_iter, max_iter > 0 ? 0 : 1, 0, nullptr));
     return Q;
   }
   const MatrixXf lp_inference_prox_super_pixels(const MatrixXf& init, const LP_inf_params& params) const {
     MatrixXf Q(M_, N_);
+    sync_unary();
     const dcrf_lp_params p = params.to_c();
     check(dcrf_crf_lp(h_, init.data(), Q.data(), &p, 0, nullptr));
     return Q;
@@ -419,6 +534,7 @@ class DenseCRF {
     return r;
   }
   double assignment_energy_true(const VectorXs& l) const {
+    sync_unary();
     std::vector<int> lab = labels(l);
     double e = 0;
     check(dcrf_crf_assignment_energy_true(h_, lab.data(), 0, &e));
@@ -426,6 +542,7 @@ class DenseCRF {
   }
   // public debugging functions of the reference (include/densecrf.h:148-153)
   std::vector<float> unaryEnergy(const VectorXs& l) const {
+    sync_unary();
     std::vector<int> lab(l.data(), l.data() + l.size());
     std::vector<float> r((size_t)N_);
     check(dcrf_crf_unary_energy(h_, lab.data(), r.data(), 0));
@@ -438,23 +555,27 @@ class DenseCRF {
     return r;
   }
   double compute_energy_higher_order(const MatrixXf& Q) const {
+    sync_unary();
     double e = 0;
     check(dcrf_crf_energy_higher_order(h_, Q.data(), 0, &e));
     return e;
   }
   double assignment_energy_higher_order(const VectorXs& l) const {
+    sync_unary();
     std::vector<int> lab = labels(l);
     double e = 0;
     check(dcrf_crf_assignment_energy_higher_order(h_, lab.data(), 0, &e));
     return e;
   }
   double compute_energy_LP_higher_order(const MatrixXf& Q) const {
+    sync_unary();
     double e = 0;
     check(dcrf_crf_lp_energy(h_, Q.data(), 0, &e));
     return e;
   }
   // compute_LR_QP_value(Q, diag_dom): the engine evaluates diag_dom == 0
   double compute_LR_QP_value(const MatrixXf& Q, const MatrixXf& diag_dom) const {
+    sync_unary();
     for (long i = 0; i < diag_dom.size(); ++i)
       if (diag_dom.data()[i] != 0.f) throw std::runtime_error("compute_LR_QP_value: diag_dom must be zero");
     double e = 0;
@@ -468,9 +589,20 @@ class DenseCRF {
     for (int i = 0; i < N_; ++i) lab[(size_t)i] = l[i];
     return lab;
   }
+  // the device copy of the unary follows unary_ (see UnaryEnergy)
+  void sync_unary() const {
+    if (!unary_) return;
+    if (uploaded_ == unary_.get() && unary_->versioned() && uploaded_version_ == unary_->version()) return;
+    unary_->upload(h_);
+    uploaded_ = unary_.get();
+    uploaded_version_ = unary_->version();
+  }
   int N_, M_;
   int npairwise_ = 0;
   dcrf_crf_t h_ = nullptr;
+  std::unique_ptr<UnaryEnergy> unary_;
+  mutable const UnaryEnergy* uploaded_ = nullptr;
+  mutable unsigned long long uploaded_version_ = 0;
 };
 
 // DenseCRF2D (include/densecrf.h:172-194)
@@ -521,6 +653,7 @@ class DenseCRF2D : public DenseCRF {
     std::unique_ptr<LabelCompatibility> f(function ? function : new PottsCompatibility(1.f));
     if (normalization_type == NORMALIZE_SYMMETRIC && f->type() == DCRF_COMPAT_POTTS) {
       check(dcrf_crf_add_pairwise_gaussian(h_, W_, H_, sx, sy, f->potts_weight()));
+      check(dcrf_crf_pairwise_set_kernel_type(h_, npairwise_, (int)kernel_type));
       ++npairwise_;
       return;
     }
@@ -538,6 +671,7 @@ class DenseCRF2D : public DenseCRF {
     std::unique_ptr<LabelCompatibility> f(function ? function : new PottsCompatibility(1.f));
     if (normalization_type == NORMALIZE_SYMMETRIC && f->type() == DCRF_COMPAT_POTTS) {
       check(dcrf_crf_add_pairwise_bilateral(h_, W_, H_, sx, sy, sr, sg, sb, im, f->potts_weight()));
+      check(dcrf_crf_pairwise_set_kernel_type(h_, npairwise_, (int)kernel_type));
       ++npairwise_;
       return;
     }

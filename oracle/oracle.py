@@ -450,6 +450,7 @@ def reflib():
         _sig(R, "ref_crf_layers", None, _vp, _i32p, _f32p, _f32p)
         _sig(R, "ref_crf_apply_bf", None, _vp, _i, _i, _vp, _vp, _f32p)
         _sig(R, "ref_lattice_ordered_restricted", _i, _vp, _f32p, _i32p, _i, _f32p, _i, _i, _i, _f32p)
+        _sig(R, "ref_logistic_unary", None, _f32p, _i, _i, _f32p, _i, _vp, _f32p, _f32p)
         _sig(R, "ref_crf_new", _vp, _i, _i, _i)
         _sig(R, "ref_crf_free", None, _vp)
         _sig(R, "ref_crf_set_unary", None, _vp, _f32p)
@@ -489,6 +490,20 @@ def ref_probimage(path):
     p = np.zeros((W * H, D), np.float32)
     R.ref_probimage(path.encode(), p.ctypes.data_as(C.c_void_p), dims)
     return p, W, H
+
+
+def ref_logistic_unary(Lw, f, b=None):
+    """the compiled reference's LogisticUnaryEnergy(L, f): (get() as (N, L),
+    gradient(b) as L*F values or None); Lw is (F, L) and f (N, F), the memory
+    of the reference's L x F and F x N matrices"""
+    Lw, f = _c(Lw, np.float32), _c(f, np.float32)
+    F, L = Lw.shape
+    N = f.shape[0]
+    U = np.zeros((N, L), np.float32)
+    g = np.zeros(F * L, np.float32)
+    bb = None if b is None else _c(b, np.float32)
+    reflib().ref_logistic_unary(Lw, L, F, f, N, None if bb is None else bb.ctypes.data_as(C.c_void_p), U, g)
+    return U, (None if b is None else g)
 
 
 def ref_pairwise_kernel_gradient(feat, w, ktype, ntype, b, Q, kp=None):

@@ -167,6 +167,18 @@ int ref_pairwise_kernel_gradient(const float* feat, int d, int N, float w, int k
   return (int)g.size();
 }
 
+// LogisticUnaryEnergy(L, f) (src/unary.cpp:53-68): get() into U (L x N) and, when
+// b is given, gradient(b) (L*F values) into g
+void ref_logistic_unary(const float* Lw, int L, int F, const float* f, int N, const float* b, float* U, float* g) {
+  LogisticUnaryEnergy u(as_mat(Lw, L, F), as_mat(f, F, N));
+  MatrixXf r = u.get();
+  std::copy(r.data(), r.data() + r.size(), U);
+  if (b) {
+    VectorXf gg = u.gradient(as_mat(b, L, N));
+    std::copy(gg.data(), gg.data() + gg.size(), g);
+  }
+}
+
 // ---- DenseCRF --------------------------------------------------------------
 void* ref_crf_new(int W, int H, int L) { return new RefCRF(W, H, L); }
 void ref_crf_free(void* h) { delete (RefCRF*)h; }

@@ -40,7 +40,7 @@ int main(int argc, char** argv) {
   std::copy(u.begin(), u.end(), unary.data());
 
   DenseCRF2D crf(W, H, L);
-  crf.setUnaryEnergy(unary);
+  crf.setUnaryEnergy(new ConstUnaryEnergy(unary));
   crf.addPairwiseGaussian(3, 3, new PottsCompatibility(3));
   crf.addPairwiseBilateral(50, 50, 15, 15, 15, im.data(), new PottsCompatibility(5));
   MatrixXf Q = crf.unary_init();
@@ -139,5 +139,58 @@ int main(int argc, char** argv) {
   double dfs = 0;
   for (float v : df) dfs += std::abs(v);
   std::printf("lattice_gradient_abs %.9g\n", dfs);
+
+  // LogisticUnaryEnergy (src/unary.cpp:53-68): the unary L f of F = 6 per-pixel
+  // features; setUnaryEnergy(L, f) runs the QP on the same unary as a
+  // ConstUnaryEnergy holding get(), and setParameters reaches the next solve
+  const int F = 6;
+  MatrixXf feat(F, N), Lw(L, F);
+  for (int i = 0; i < N; ++i) {
+    feat(0, i) = 1.f;
+    feat(1, i) = (i % W) / (float)W;
+    feat(2, i) = (i / W) / (float)H;
+    for (int c = 0; c < 3; ++c) feat(3 + c, i) = im[(size_t)i * 3 + c] / 255.f;
+  }
+  for (int k = 0; k < F; ++k)
+    for (int l = 0; l < L; ++l) Lw(l, k) = 0.5f * (float)(((l * 5 + k * 3) % 7) - 3);
+  LogisticUnaryEnergy* lu = new LogisticUnaryEnergy(Lw, feat);
+  const MatrixXf Ul = lu->get();
+  double umax = 0;
+  for (long i = 0; i < Ul.size(); ++i) umax = std::max(umax, (double)std::abs(Ul.data()[i]));
+  std::printf("logistic_get_max %.9g\n", umax);
+  std::printf("logistic_params %d\n", (int)lu->parameters().rows());
+  std::printf("logistic_gradient_size %d\n", (int)lu->gradient(q01).rows());
+  DenseCRF2D lc(W, H, L), cc(W, H, L);
+  lc.setUnaryEnergy(lu);
+  cc.setUnaryEnergy(new ConstUnaryEnergy(Ul));
+  for (DenseCRF2D* c : {&lc, &cc}) {
+    c->addPairwiseGaussian(3, 3, new PottsCompatibility(3));
+    c->addPairwiseBilateral(50, 50, 15, 15, 15, im.data(), new PottsCompatibility(5));
+  }
+  const MatrixXf ql = lc.qp_inference_super_pixels_non_convex(lc.unary_init(), 5);
+  const MatrixXf qc = cc.qp_inference_super_pixels_non_convex(cc.unary_init(), 5);
+  double ldiff = 0;
+  for (long i = 0; i < ql.size(); ++i) ldiff = std::max(ldiff, (double)std::abs(ql.data()[i] - qc.data()[i]));
+  std::printf("logistic_vs_const_qp_diff %.9g\n", ldiff);
+  VectorXf p2 = lc.getUnaryEnergy()->parameters();
+  for (long i = 0; i < p2.rows(); ++i) p2(i) *= 2.f;
+  lc.getUnaryEnergy()->setParameters(p2);
+  const MatrixXf u2 = lc.unary_init(), u1 = cc.unary_init();
+  double udiff = 0;
+  for (long i = 0; i < u2.size(); ++i) udiff = std::max(udiff, (double)std::abs(u2.data()[i] - u1.data()[i]));
+  std::printf("logistic_set_parameters_changes_unary %d\n", udiff > 1e-3 ? 1 : 0);
+
+  // addPairwiseEnergy(PairwisePotential*) (src/densecrf.cpp:161-168) adds the same
+  // term as addPairwiseEnergy(features, function): equal filtered fields
+  DenseCRF a(N, L), b(N, L);
+  a.setUnaryEnergy(unary);
+  b.setUnaryEnergy(unary);
+  a.addPairwiseEnergy(f, new MatrixCompatibility(m));
+  b.addPairwiseEnergy(new PairwisePotential(f, new MatrixCompatibility(m), DIAG_KERNEL, NORMALIZE_SYMMETRIC));
+  const MatrixXf ma = a.meanfield(2), mb = b.meanfield(2);
+  double pdiff = 0;
+  for (long i = 0; i < ma.size(); ++i) pdiff = std::max(pdiff, (double)std::abs(ma.data()[i] - mb.data()[i]));
+  std::printf("potential_object_vs_features_diff %.9g\n", pdiff);
+  std::printf("potential_object_params %d\n", (int)b.pairwise_parameters(0).size());
   return 0;
 }

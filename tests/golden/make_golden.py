@@ -13,6 +13,7 @@ compressed .npz files.  Run from the repository root:
     python tests/golden/make_golden.py kgrad    # only kgrad_31x23.npz
     python tests/golden/make_golden.py dc       # only dc_L4.npz
     python tests/golden/make_golden.py eho      # only eho_32x24_L21.npz
+    python tests/golden/make_golden.py logistic # only logistic_unary.npz
 """
 
 from __future__ import annotations
@@ -438,7 +439,31 @@ def eho_case():
     print("eho_32x24_L21")
 
 
+def logistic_case():
+    """LogisticUnaryEnergy(L, f).get() and gradient(b) of the compiled reference
+    (src/unary.cpp:53-68) on three shapes: 21 labels x 37 features x 713 pixels
+    (feature and pixel counts off the kernels' tiles) and 5 x 3 x 1000 with
+    normal values, and 21 x 70 x 4200 (several feature chunks, feature blocks
+    and pixel blocks) with multiples of 1/8 in [-2, 2], whose products and sums
+    are exact in float in any order"""
+    rng = np.random.default_rng(61)
+    out = {}
+    for i, (L, F, N) in enumerate([(21, 37, 713), (5, 3, 1000), (21, 70, 4200)]):
+        if i < 2:
+            draw = lambda shape: rng.standard_normal(shape).astype(np.float32)  # noqa: E731
+        else:
+            draw = lambda shape: (rng.integers(-16, 17, shape) / 8).astype(np.float32)  # noqa: E731
+        Lw, f, b = draw((F, L)), draw((N, F)), draw((N, L))
+        U, g = O.ref_logistic_unary(Lw, f, b)
+        out.update({f"Lw{i}": Lw, f"f{i}": f, f"b{i}": b, f"U{i}": U, f"g{i}": g})
+    np.savez_compressed(os.path.join(OUT, "logistic_unary.npz"), **out)
+    print("logistic_unary")
+
+
 def main():
+    if sys.argv[1:] == ["logistic"]:
+        logistic_case()
+        return
     if sys.argv[1:] == ["eho"]:
         eho_case()
         return

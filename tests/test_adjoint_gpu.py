@@ -196,3 +196,35 @@ def test_autograd_wrapper_matches_adjoint(golden):
     np.testing.assert_allclose(U.grad.numpy(), dU, rtol=1e-5, atol=1e-6 * np.abs(dU).max())
     for p, gk in zip(params, gs):
         np.testing.assert_allclose(p.grad.numpy(), gk, rtol=1e-5, atol=1e-6 * np.abs(gk).max())
+
+
+def test_backward_after_another_forward_or_model_change_raises(golden):
+    """the engine keeps one forward's checkpoints: two forwards then one
+    backward through the autograd wrapper raises, and so does the C ABI's
+    backward after setUnaryEnergy or set_pairwise_parameters; a fresh forward
+    makes backward work again"""
+    import densecrf_ho_amd as D
+    D.load()
+    torch = pytest.importorskip("torch")
+    from densecrf_ho_amd.autograd import meanfield
+    g = golden("compat_31x23_L5.npz")
+    N, L = g["x"].shape
+    c, _ = _mf_pair(g)
+    params = [torch.tensor(c.pairwise_parameters(k), requires_grad=True) for k in range(3)]
+    U = torch.tensor(g["U"], requires_grad=True)
+    Q1 = meanfield(U, params, c, 2)
+    Q2 = meanfield(U, params, c, 3)
+    with pytest.raises(RuntimeError, match="another meanfield ran"):
+        (Q1.sum() + Q2.sum()).backward()
+    dQ = np.ones((N, L), np.float32)
+    c.meanfield(2)
+    c.setUnaryEnergy(g["U"])
+    with pytest.raises(D.EngineError, match="changed after the last meanfield"):
+        c.meanfield_backward(dQ)
+    c.meanfield(2)
+    c.set_pairwise_parameters(0, c.pairwise_parameters(0))
+    with pytest.raises(D.EngineError, match="changed after the last meanfield"):
+        c.meanfield_backward(dQ)
+    c.meanfield(2)
+    dU, _ = c.meanfield_backward(dQ)
+    assert np.isfinite(dU).all()

@@ -52,3 +52,11 @@ def test_cpp_example_matches_oracle(tmp_path):
     assert np.isfinite(float(vals["bf_mid"])) and np.isfinite(float(vals["dc_mid"]))
     assert int(vals["kernel_gradient_size"]) == 2
     assert np.isfinite(float(vals["lattice_gradient_abs"])) and float(vals["lattice_gradient_abs"]) > 0
+    # UnaryEnergy objects: LogisticUnaryEnergy through setUnaryEnergy(UnaryEnergy*)
+    assert float(vals["logistic_get_max"]) > 0
+    assert int(vals["logistic_params"]) == L * 6 and int(vals["logistic_gradient_size"]) == L * 6
+    assert float(vals["logistic_vs_const_qp_diff"]) == 0.0
+    assert int(vals["logistic_set_parameters_changes_unary"]) == 1
+    # addPairwiseEnergy(PairwisePotential*) with a Matrix compatibility
+    assert float(vals["potential_object_vs_features_diff"]) == 0.0
+    assert int(vals["potential_object_params"]) == L * (L + 1) // 2

@@ -156,13 +156,18 @@ def test_restricted_ordered_errors(golden):
     bad[3, 1] = 1.5
     with pytest.raises(D.EngineError, match="Value out of bounds"):
         lat.compute_upper_minus_lower_ord_restricted(None, bad, pI, g["q1"], True)
-    # no pixel in pI: nothing to slice, an empty result (the reference reads pI[0] here)
+    # no pixel in pI with store: the reference reads pI[0] there (undefined), so it is refused
+    with pytest.raises(D.EngineError, match="at least one pixel"):
+        lat.compute_upper_minus_lower_ord_restricted(None, np.zeros((0, 5), np.float32), np.zeros(0, np.int32),
+                                                     g["q1"], True)
+    # after a call with store, an empty list without store slices nothing
+    first = lat.compute_upper_minus_lower_ord_restricted(None, rq, pI, g["q1"], True)
     empty = lat.compute_upper_minus_lower_ord_restricted(None, np.zeros((0, 5), np.float32),
-                                                         np.zeros(0, np.int32), g["q1"], True)
+                                                         np.zeros(0, np.int32), None, False)
     assert empty.shape == (0, 5)
-    # the stored state then serves a call without store
-    out = lat.compute_upper_minus_lower_ord_restricted(None, rq, pI, None, False)
-    assert out.shape == rq.shape and np.isfinite(out).all()
+    # and the stored state still serves the next call as it did the first
+    again = lat.compute_upper_minus_lower_ord_restricted(None, rq, pI, None, False)
+    np.testing.assert_array_equal(again, first)
 
 
 @pytest.mark.parametrize("d", [2, 5])
