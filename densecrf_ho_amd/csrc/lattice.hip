@@ -16,6 +16,12 @@ namespace {
 constexpr int kMaxD = 12;
 constexpr int kChunk = 64;  // splat entries summed sequentially per chunk
 constexpr int kOBatch = 8;  // ordered splat entries whose loads are issued together
+constexpr int kOHist = 10;  // ordered splat histogram slots per label (bins 0..9)
+constexpr int kOSBatch = 8;  // ordered splat entries whose loads are issued together (two 16-byte loads)
+// entries of padding (pixel 0, weight 0) after the sorted entry lists, read by
+// the last batch of the splats (at least kOBatch and kOSBatch)
+constexpr int kSpPad = 16;
+static_assert(kSpPad >= kOBatch && kSpPad >= kOSBatch, "splat batches read past the last entry");
 
 // ---- packed keys: 4 int16 coordinates per 64-bit word ---------------------
 template <int D>
@@ -249,7 +255,7 @@ __global__ void __launch_bounds__(kBlock) k_iota(int* __restrict__ v, int n) {
 __global__ void __launch_bounds__(kBlock) k_fill_sp(const int* __restrict__ ent, const float* __restrict__ bary, int E, int d1,
                           int* __restrict__ sp_pix, float* __restrict__ sp_w) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= E + kOBatch) return;
+  if (t >= E + kSpPad) return;
   if (t >= E) {  // padding entries
     sp_pix[t] = 0;
     sp_w[t] = 0.f;
@@ -556,6 +562,9 @@ constexpr int kStageMinV = 8;
 // batch's sums, and added in pixel order as the per-value loop does.
 struct __attribute__((packed, aligned(4))) Float4u {
   float x, y, z, w;
+};
+struct __attribute__((packed, aligned(4))) Int4u {
+  int x, y, z, w;
 };
 template <SplatIn KIND>
 __device__ inline float4 splat_in4(const SplatArgs& a, int pix, int k, int V, bool whole) {
@@ -872,10 +881,20 @@ __global__ void __launch_bounds__(kBlock) k_onbr(const int* __restrict__ nbr, co
 // One lane per (point chunk, quad of labels): each entry's bins of the four
 // labels are one 4-byte word of its pixel's bins row (rows of LP bytes, LP a
 // multiple of 8; labels past L are row padding, computed and dropped).  The
-// chunk's entries are read in batches of kOBatch (all of a batch's loads in
+// chunk's entries are read in batches of kOSBatch (all of a batch's loads in
 // flight together, the next batch's entry loads in flight during this batch's
-// sums; the entry lists hold kOBatch entries of padding, pixel 0 with weight
-// 0, and entries past the chunk's end get weight 0) and added in entry order.
+// sums; the entry lists hold kOSBatch entries of padding, pixel 0 with weight
+// 0, and entries past the chunk's end get weight 0).
+// Each lane keeps a histogram of its chunk's weights by bin, per label, in its
+// own LDS slots: per (entry, label) the slot of the entry's bin is read,
+// added to and written back, in entry order (the slots are the lane's own, and
+// a wave's LDS operations execute in order, so each read sees the previous
+// write).  The
+// cumulative rows are the histogram's suffix sums (lower pass: bins 1..9 get
+// the weights of bins >= b) or prefix sums (upper pass: bins 0..8 get the
+// weights of bins <= b), taken once per chunk.  Slot (kk, c) of thread t is
+// word (kk * 10 + c) * kBlock + t: the 32 lanes of each half-wave hit 32
+// different banks whatever their bins.
 template <int TAG, class T>  // TAG: lattice dim, +100 for the upper pass
 __global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ bins, int L, int LP, int RS,
                                                    long nlanes, const int* __restrict__ sp_pix,
@@ -887,6 +906,10 @@ __global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ b
   constexpr bool kUpper = TAG >= 100;
   constexpr bool lmaj = label_major(TAG % 100);
   if (kUpper && !edge.any_full()) return;
+  __shared__ float hist[4 * kOHist * kBlock];
+  float* hl = hist + threadIdx.x;  // this lane's slots, stride kBlock
+#pragma unroll
+  for (int i = 0; i < 4 * kOHist; ++i) hl[i * kBlock] = 0.f;
   const int nq = (L + 3) >> 2;
   for (long g = (long)xcd_block() * blockDim.x + threadIdx.x; g < nlanes; g += (long)gridDim.x * blockDim.x) {
     const int c = (int)(g / nq), q = (int)(g - (long)c * nq);
@@ -895,54 +918,71 @@ __global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ b
     const int c0 = chstart[p], nc = chstart[p + 1] - c0;
     const int b = seg[p] + (c - c0) * kChunk;
     const int e = min(seg[p + 1], b + kChunk);
-    float acc[4][kOBins];
+    static_assert(kOSBatch == 8, "entries load as two 16-byte words");
+    int px[kOSBatch];
+    float w[kOSBatch];
+    // kOSBatch consecutive entries: two 16-byte loads per list (4-byte aligned)
+    auto entries = [&](int t) {
+      const Int4u p0 = *reinterpret_cast<const Int4u*>(sp_pix + t), p1 = *reinterpret_cast<const Int4u*>(sp_pix + t + 4);
+      const Float4u w0 = *reinterpret_cast<const Float4u*>(sp_w + t), w1 = *reinterpret_cast<const Float4u*>(sp_w + t + 4);
+      px[0] = p0.x, px[1] = p0.y, px[2] = p0.z, px[3] = p0.w, px[4] = p1.x, px[5] = p1.y, px[6] = p1.z, px[7] = p1.w;
+      w[0] = w0.x, w[1] = w0.y, w[2] = w0.z, w[3] = w0.w, w[4] = w1.x, w[5] = w1.y, w[6] = w1.z, w[7] = w1.w;
+    };
+    entries(b);
+    for (int t0 = b; t0 < e; t0 += kOSBatch) {
+      uint32_t bw[kOSBatch];
+      float wc[kOSBatch];
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk)
-#pragma unroll
-      for (int r = 0; r < kOBins; ++r) acc[kk][r] = 0.f;
-    int px[kOBatch];
-    float w[kOBatch];
-#pragma unroll
-    for (int i = 0; i < kOBatch; ++i) {
-      px[i] = sp_pix[b + i];
-      w[i] = sp_w[b + i];
-    }
-    for (int t0 = b; t0 < e; t0 += kOBatch) {
-      uint32_t bw[kOBatch];
-      float wc[kOBatch];
-#pragma unroll
-      for (int i = 0; i < kOBatch; ++i) {
+      for (int i = 0; i < kOSBatch; ++i) {
         bw[i] = *reinterpret_cast<const uint32_t*>(bins + (size_t)px[i] * LP + 4 * q);
         wc[i] = t0 + i < e ? w[i] : 0.f;
       }
-      if (t0 + kOBatch < e)  // the next batch's entries load during this batch's sums
+      if (t0 + kOSBatch < e) entries(t0 + kOSBatch);  // the next batch's entries load during this batch's sums
 #pragma unroll
-        for (int i = 0; i < kOBatch; ++i) {
-          px[i] = sp_pix[t0 + kOBatch + i];
-          w[i] = sp_w[t0 + kOBatch + i];
-        }
-#pragma unroll
-      for (int i = 0; i < kOBatch; ++i) {
+      for (int i = 0; i < kOSBatch; ++i) {
         const float wi = wc[i];
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
-          const int bv = (bw[i] >> (8 * kk)) & 0xff;
-          const int lo = bv & 15, up = bv >> 4;
-#pragma unroll
-          for (int r = 0; r < kOBins; ++r) {
-            if (kUpper)
-              acc[kk][r] = acc[kk][r] + (r >= up ? wi : 0.f);  // bins 0..8
-            else
-              acc[kk][r] = acc[kk][r] + (r + 1 <= lo ? wi : 0.f);  // bins 1..9
-          }
+          const int c = min(kUpper ? (int)((bw[i] >> (8 * kk + 4)) & 15u) : (int)((bw[i] >> (8 * kk)) & 15u),
+                            kOHist - 1);
+          float* slot = &hl[(kk * kOHist + c) * kBlock];
+          *slot = *slot + wi;
         }
+      }
+    }
+    float acc[4][kOBins];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      float h[kOHist];
+#pragma unroll
+      for (int c = 0; c < kOHist; ++c) {
+        h[c] = hl[(kk * kOHist + c) * kBlock];
+        hl[(kk * kOHist + c) * kBlock] = 0.f;  // the lane's next chunk starts from zero
+      }
+      float sum = kUpper ? h[0] : h[kOHist - 1];
+      if (kUpper) {  // bin r (slot r, r = 0..8): weights of bins 0..r
+        acc[kk][0] = sum;
+#pragma unroll
+        for (int r = 1; r < kOBins; ++r) acc[kk][r] = sum = sum + h[r];
+      } else {  // bin r + 1 (slot r): weights of bins r + 1..9
+        acc[kk][kOBins - 1] = sum;
+#pragma unroll
+        for (int r = kOBins - 2; r >= 0; --r) acc[kk][r] = sum = sum + h[r + 1];
       }
     }
     // label-major rows: bin slot r of label k at k * kOBins + r, so the
     // lane's four labels are 36 consecutive values (nine aligned quads);
-    // bin-major rows: at r * L + k
+    // bin-major rows: at r * L + k, so the lane's four labels of one bin are
+    // one 16-byte store at 4-byte alignment (float rows; bfloat16 rows and a
+    // row's last, partial quad store value by value)
     auto slot = [&](int f) { return acc[f / kOBins][f % kOBins]; };
     const bool quads = lmaj && 4 * q + 4 <= L;
+    const bool bquads = !lmaj && 4 * q + 4 <= L;
+    auto bm_quads = [&](float* dst) {
+#pragma unroll
+      for (int r = 0; r < kOBins; ++r)
+        *reinterpret_cast<Float4u*>(dst + (size_t)r * L + 4 * q) = Float4u{acc[0][r], acc[1][r], acc[2][r], acc[3][r]};
+    };
     auto at = [&](int kk, int r) { return lmaj ? (size_t)(4 * q + kk) * kOBins + r : (size_t)r * L + 4 * q + kk; };
     if (nc == 1) {  // the point's row
       const size_t r0 = (size_t)orow[p] * RS;
@@ -951,6 +991,8 @@ __global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ b
         for (int i = 0; i < kOBins; ++i)
           st_quad(values, r0 / 4 + (size_t)kOBins * q + i,
                   make_float4(slot(4 * i), slot(4 * i + 1), slot(4 * i + 2), slot(4 * i + 3)));
+      } else if (std::is_same<T, float>::value && bquads) {
+        bm_quads(reinterpret_cast<float*>(values) + r0);
       } else {
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk)
@@ -967,6 +1009,8 @@ __global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ b
         for (int i = 0; i < kOBins; ++i)
           st_quad(dst, (size_t)kOBins * q + i,
                   make_float4(slot(4 * i), slot(4 * i + 1), slot(4 * i + 2), slot(4 * i + 3)));
+      } else if (bquads) {
+        bm_quads(dst);
       } else {
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk)
@@ -1367,9 +1411,9 @@ void Lattice::build(const float* d_feat, int N, int d) {
                                                 ent_sorted.get(), E, 0, bits, s));
     DCRF_HIP(hipStreamSynchronize(s));
   }
-  sp_pix_.resize(E + kOBatch);  // padding read by the ordered splat's last batch: pixel 0, weight 0
-  sp_w_.resize(E + kOBatch);
-  k_fill_sp<<<grid_for(E + kOBatch), kBlock, 0, s>>>(ent_sorted.get(), bary_.get(), E, d1, sp_pix_.get(),
+  sp_pix_.resize(E + kSpPad);  // padding read by the splats' last batch: pixel 0, weight 0
+  sp_w_.resize(E + kSpPad);
+  k_fill_sp<<<grid_for(E + kSpPad), kBlock, 0, s>>>(ent_sorted.get(), bary_.get(), E, d1, sp_pix_.get(),
                                                      sp_w_.get());
   check_launch("k_fill_sp");
 
@@ -1787,10 +1831,10 @@ std::unique_ptr<Lattice> Lattice::concat(const std::vector<const Lattice*>& part
   u->nbr_.resize((size_t)d1 * M * 2);
   u->keys_.resize((size_t)M * d);
   u->part_pt_off_.assign(1, 0);
-  u->sp_pix_.resize(E + kOBatch);  // padding read by the ordered splat's last batch: pixel 0, weight 0
-  u->sp_w_.resize(E + kOBatch);
-  DCRF_HIP(hipMemsetAsync(u->sp_pix_.get() + E, 0, kOBatch * sizeof(int), s));
-  DCRF_HIP(hipMemsetAsync(u->sp_w_.get() + E, 0, kOBatch * sizeof(float), s));
+  u->sp_pix_.resize(E + kSpPad);  // padding read by the splats' last batch: pixel 0, weight 0
+  u->sp_w_.resize(E + kSpPad);
+  DCRF_HIP(hipMemsetAsync(u->sp_pix_.get() + E, 0, kSpPad * sizeof(int), s));
+  DCRF_HIP(hipMemsetAsync(u->sp_w_.get() + E, 0, kSpPad * sizeof(float), s));
   u->seg_.resize(M + 1);
   u->chstart_.resize(M + 1);
   u->pstart_.resize(M + 1);
