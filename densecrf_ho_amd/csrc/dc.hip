@@ -192,11 +192,19 @@ static void dc_sorted_slice(Lattice& lat, DcScratch& sc, const float* fs, int d,
   lat.dc_step(col, mn, ml, mh, mx, sc);
 }
 
+// largest N * L of the divide-and-conquer filter (about 70,000 node launches)
+constexpr long long kDcMaxPixelLabels = 1LL << 20;
+
 void potential_apply_uml_dc(Potential& p, const int* ind, float* out, int L) {
   DCRF_REQUIRE(p.feat.get() != nullptr, "apply_upper_minus_lower_dc: the potential keeps no features");
   DCRF_REQUIRE(p.d <= 16, "apply_upper_minus_lower_dc: at most 16 features");
   hipStream_t s = p.lat->stream();
   const int N = p.lat->N(), d = p.d;
+  // the recursion runs the reference's node order, one launch per node: about
+  // N / 15 launches per label, so the form is bounded to small inputs
+  DCRF_REQUIRE((long long)N * L <= kDcMaxPixelLabels,
+               "apply_upper_minus_lower_dc: N * L above 2^20 (one launch per 15 pixels and label; use the ordered "
+               "filter for large inputs)");
   float* tgt = out;
   if (p.ctype != kPotts) {
     p.ctmp.resize((size_t)N * L);

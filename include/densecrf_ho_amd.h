@@ -164,7 +164,8 @@ int dcrf_pairwise_apply_upper_minus_lower_bf(dcrf_pairwise_t pw, const int* ind,
  * src/pairwise.cpp:241-293: per label a lattice on the features in the order ind and
  * the divide-and-conquer over the sorted positions (blocks of <= 30 by brute force with
  * alpha 1/0.6, larger ones halved with the lattice step between the halves); out (L x N)
- * in the sorted order, then the compatibility. */
+ * in the sorted order, then the compatibility.  It runs one launch per node (about N/15
+ * per label), so N * L is limited to 2^20 (DCRF_INVALID_ARGUMENT above). */
 int dcrf_pairwise_apply_upper_minus_lower_dc(dcrf_pairwise_t pw, const int* ind, float* out, int L, int on_device);
 int dcrf_pairwise_apply_upper_minus_lower_bf_ord(dcrf_pairwise_t pw, const int* ind, const float* Q, float* out,
                                                  int L, int on_device);

@@ -16,7 +16,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_SO = os.path.join(_HERE, "liboracle.so")
+# ORACLE_LIB selects another build of the same sources (the sanitizer build of
+# `make -C oracle check-asan`)
+ORACLE_SO = os.environ.get("ORACLE_LIB", os.path.join(_HERE, "liboracle.so"))
 REF_SO = os.path.join(_HERE, "_ref", "libdcrf_ref.so")
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")

@@ -261,6 +261,20 @@ def test_divide_and_conquer_filter_vs_oracle_larger():
     assert rel_err(out, np.float32(-5.0) * O.uml_dc(f, ind)) < 1e-5
 
 
+def test_divide_and_conquer_size_guard():
+    """N * L above 2^20 (one launch per 15 pixels and label) is refused
+    before any launch"""
+    import densecrf_ho_amd as D
+    from densecrf_ho_amd import synthetic as S
+    wl = S.Workload(256, 200, 10, 5, L=21)  # 51,200 pixels x 21 labels > 2^20
+    c = D.DenseCRF2D(256, 200, 21)
+    c.setUnaryEnergy(wl.U)
+    c.addPairwiseEnergy(wl.kernels[0][0], D.PottsCompatibility(3.0))
+    ind = np.tile(np.arange(wl.N, dtype=np.int32)[:, None], (1, 21))
+    with pytest.raises(D.EngineError, match="2\\^20"):
+        c.apply_upper_minus_lower_dc(0, ind)
+
+
 def test_standalone_pairwise_potential_filters(golden):
     """the standalone PairwisePotential (its own lattice and handle): the
     brute-force, divide-and-conquer and restricted filters against the
