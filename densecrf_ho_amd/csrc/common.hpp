@@ -174,6 +174,10 @@ struct DBuf {
 // ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
+// Threads per block of most kernels.  Kernels that are always launched with
+// kBlock threads index with the constant: blockDim.x is a 2-byte read of the
+// dispatch packet, which gfx950 serves through the vector memory path, so it
+// would put one dependent load ahead of every address the kernel forms.
 constexpr int kBlock = 256;
 inline unsigned grid_for(size_t work, int block = kBlock, size_t cap = 1u << 30) {
   size_t g = (work + block - 1) / block;

@@ -76,7 +76,7 @@ struct ElevConst {
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_elevate(const float* __restrict__ feat, int N, int Np, ElevConst c, PKey<D>* __restrict__ ekeys,
                           int* __restrict__ rank_out, float* __restrict__ bary_out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= Np) return;
   constexpr int D1 = D + 1;
   float f[D], elev[D1], rem0[D1], rk[D1], bc[D + 2];
@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(kBlock) k_elevate(const float* __restrict__ fe
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_insert(const PKey<D>* __restrict__ ekeys, int E, int* table, uint32_t mask,
                          int* __restrict__ slot_of, int* __restrict__ err) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = blockIdx.x * kBlock + threadIdx.x;
   if (e >= E) return;
   const PKey<D> k = ekeys[e];
   uint32_t h = key_hash<D>(k) & mask;
@@ -185,7 +185,7 @@ __global__ void __launch_bounds__(kBlock) k_insert(const PKey<D>* __restrict__ e
 
 __global__ void __launch_bounds__(kBlock) k_mark_first(const int* __restrict__ table, const int* __restrict__ slot_of, int E,
                              int* __restrict__ flag) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = blockIdx.x * kBlock + threadIdx.x;
   if (e >= E) return;
   flag[e] = table[slot_of[e]] == e ? 1 : 0;
 }
@@ -194,7 +194,7 @@ template <int D>
 __global__ void __launch_bounds__(kBlock) k_assign(const int* __restrict__ table, const int* __restrict__ slot_of, const int* __restrict__ pos,
                          const int* __restrict__ flag, const PKey<D>* __restrict__ ekeys, int E,
                          int* __restrict__ offset, PKey<D>* __restrict__ ukeys) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = blockIdx.x * kBlock + threadIdx.x;
   if (e >= E) return;
   offset[e] = pos[table[slot_of[e]]];
   if (flag[e]) ukeys[pos[e]] = ekeys[e];
@@ -218,7 +218,7 @@ template <int D>
 __global__ void __launch_bounds__(kBlock) k_neighbors(const PKey<D>* __restrict__ ukeys, int M, const int* __restrict__ table, uint32_t mask,
                             const PKey<D>* __restrict__ ekeys, const int* __restrict__ pos, int* __restrict__ nbr,
                             int16_t* __restrict__ keys16) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
   if (t >= (size_t)M * (D + 1)) return;
   const int j = (int)(t / M), i = (int)(t % M);
   int16_t key[D], n1[D], n2[D];
@@ -245,16 +245,16 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(const PKey<D>* __restrict_
 
 // ---- splat gather lists -------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_count(const int* __restrict__ offset, int E, int* __restrict__ cnt) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = blockIdx.x * kBlock + threadIdx.x;
   if (e < E) atomicAdd(&cnt[offset[e]], 1);
 }
 __global__ void __launch_bounds__(kBlock) k_iota(int* __restrict__ v, int n) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = blockIdx.x * kBlock + threadIdx.x;
   if (e < n) v[e] = e;
 }
 __global__ void __launch_bounds__(kBlock) k_fill_sp(const int* __restrict__ ent, const float* __restrict__ bary, int E, int d1,
                           int* __restrict__ sp_pix, float* __restrict__ sp_w) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= E + kSpPad) return;
   if (t >= E) {  // padding entries
     sp_pix[t] = 0;
@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(kBlock) k_fill_sp(const int* __restrict__ ent,
 // written, also for points that only padding lanes produced)
 __global__ void __launch_bounds__(kBlock) k_nchunks(const int* __restrict__ cnt, int M, int* __restrict__ nch, int* __restrict__ multi_flag,
                           int* __restrict__ pn) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= M) return;
   int n = (cnt[p] + kChunk - 1) / kChunk;
   if (n < 1) n = 1;
@@ -279,7 +279,7 @@ __global__ void __launch_bounds__(kBlock) k_nchunks(const int* __restrict__ cnt,
 }
 __global__ void __launch_bounds__(kBlock) k_chpoint(const int* __restrict__ chstart, const int* __restrict__ mpos,
                           const int* __restrict__ mflag, int M, int* __restrict__ chpoint, int* __restrict__ multi) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= M) return;
   for (int c = chstart[p]; c < chstart[p + 1]; ++c) chpoint[c] = p;
   if (mflag[p]) multi[mpos[p]] = p;
@@ -295,7 +295,7 @@ struct RowMap {
   }
   // same mapping for an explicit (virtual) block index
   __device__ inline void at_block(int V, long blk, long& row, int& k0, int& kstep) const {
-    const long gt = blk * blockDim.x + threadIdx.x;
+    const long gt = blk * kBlock + threadIdx.x;
     const long wave = gt >> 6;
     const int lane = (int)(gt & 63);
     if (V <= 64) {
@@ -531,7 +531,7 @@ __global__ void __launch_bounds__(kBlock) k_combine_if(int V, int RS, RowMap rm,
                                                        const int* __restrict__ orow, T* __restrict__ values,
                                                        EdgeMap edge, int upper) {
   if (upper && !edge.any_full()) return;
-  const long nvb = (long)((nmulti + rm.gpw - 1) / rm.gpw * 64 + blockDim.x - 1) / blockDim.x;
+  const long nvb = (long)((nmulti + rm.gpw - 1) / rm.gpw * 64 + kBlock - 1) / kBlock;
   for (long vb = xcd_block(); vb < nvb; vb += gridDim.x) {
     long m;
     int k0, ks;
@@ -548,7 +548,7 @@ __global__ void __launch_bounds__(kBlock) k_combine_if(int V, int RS, RowMap rm,
   }
 }
 __global__ void __launch_bounds__(kBlock) k_fill_f(float* __restrict__ p, size_t n, float v) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
   if (t < n) p[t] = v;
 }
 
@@ -602,7 +602,7 @@ __global__ void __launch_bounds__(kBlock) k_splat_quad(SplatArgs a, int V, long 
                                                        const int* __restrict__ chpoint, const int* __restrict__ pstart,
                                                        float* __restrict__ values, float* __restrict__ part) {
   const int nq = (V + 3) >> 2;
-  for (long g = (long)xcd_block() * blockDim.x + threadIdx.x; g < nlanes; g += (long)gridDim.x * blockDim.x) {
+  for (long g = (long)xcd_block() * kBlock + threadIdx.x; g < nlanes; g += (long)gridDim.x * kBlock) {
     const int c = (int)(g / nq), q = (int)(g - (long)c * nq);
     const int k = 4 * q;
     const bool whole = k + 4 <= V;
@@ -671,7 +671,7 @@ __global__ void __launch_bounds__(kBlock) k_slice_quad(int V, long nlanes, int N
                                                        const int* __restrict__ offset, const float* __restrict__ bary,
                                                        const float* __restrict__ values, float alpha, SliceArgs o) {
   const int nq = (V + 3) >> 2;
-  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long g = (long)blockIdx.x * kBlock + threadIdx.x;
   const long gc = g < nlanes ? g : nlanes - 1;  // lanes past the end mirror the last one and store nothing
   const int pix = (int)(gc / nq), q = (int)(gc - (long)pix * nq);
   const int k = 4 * q;
@@ -738,8 +738,8 @@ __global__ void __launch_bounds__(kBlock) k_slice_quad(int V, long nlanes, int N
 __global__ void __launch_bounds__(kBlock) k_blur_quad(int V, FastDiv divNQ, uint32_t nl,
                                                       const int* __restrict__ nbr, const float* __restrict__ src,
                                                       float* __restrict__ dst) {
-  const uint32_t nq = divNQ.d, stride = gridDim.x * blockDim.x;
-  for (uint32_t t = xcd_block() * blockDim.x + threadIdx.x; t < nl; t += stride) {
+  const uint32_t nq = divNQ.d, stride = gridDim.x * kBlock;
+  for (uint32_t t = xcd_block() * kBlock + threadIdx.x; t < nl; t += stride) {
     const uint32_t i = divNQ.div(t);
     const int k = 4 * (int)(t - i * nq);
     const bool whole = k + 4 <= V;
@@ -826,7 +826,7 @@ constexpr bool label_major(int tag) { return tag == 2; }
 __global__ void __launch_bounds__(kBlock) k_ovtx(const int* __restrict__ offset, const float* __restrict__ bary,
                                                  const float* __restrict__ Tb, const int* __restrict__ orow, long E,
                                                  float4* __restrict__ vtx) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
   if (t >= E) return;
   const int p = offset[t];
   vtx[t] = make_float4(__int_as_float(orow[p]), bary[t], Tb[p + 1], 0.f);
@@ -837,7 +837,7 @@ __global__ void __launch_bounds__(kBlock) k_ovtx(const int* __restrict__ offset,
 // wrap, which only loosens the locality)
 __global__ void __launch_bounds__(kBlock) k_morton(const int16_t* __restrict__ keys, int M, int d,
                                                    unsigned long long* __restrict__ code, int* __restrict__ ids) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= M) return;
   const int bits = min(64 / d, 30);
   unsigned long long c = 0;
@@ -852,7 +852,7 @@ __global__ void __launch_bounds__(kBlock) k_morton(const int16_t* __restrict__ k
 __global__ void __launch_bounds__(kBlock) k_orow(const int* __restrict__ order, int M,
                                                  const uint16_t* __restrict__ ppimg, int* __restrict__ orow,
                                                  uint16_t* __restrict__ oppart) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.x * kBlock + threadIdx.x;
   if (r >= M) return;
   orow[order[r]] = r + 1;
   if (ppimg) oppart[r] = ppimg[order[r]];
@@ -861,14 +861,14 @@ __global__ void __launch_bounds__(kBlock) k_orow(const int* __restrict__ order, 
 __global__ void __launch_bounds__(kBlock) k_gather_u16(const uint16_t* __restrict__ part,
                                                        const int* __restrict__ order, int M,
                                                        uint16_t* __restrict__ out) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.x * kBlock + threadIdx.x;
   if (r < M) out[r] = part[order[r]];
 }
 // blur neighbours of row r along every axis, as rows (row - 1, -1 if absent)
 __global__ void __launch_bounds__(kBlock) k_onbr(const int* __restrict__ nbr, const int* __restrict__ order,
                                                  const int* __restrict__ orow, int M, int d1,
                                                  int* __restrict__ onbr) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
   if (t >= (long)M * d1) return;
   const int j = (int)(t / M), r = (int)(t - (long)j * M);
   const int i = order[r];
@@ -911,7 +911,7 @@ __global__ void __launch_bounds__(kBlock) k_osplat(const uint8_t* __restrict__ b
 #pragma unroll
   for (int i = 0; i < 4 * kOHist; ++i) hl[i * kBlock] = 0.f;
   const int nq = (L + 3) >> 2;
-  for (long g = (long)xcd_block() * blockDim.x + threadIdx.x; g < nlanes; g += (long)gridDim.x * blockDim.x) {
+  for (long g = (long)xcd_block() * kBlock + threadIdx.x; g < nlanes; g += (long)gridDim.x * kBlock) {
     const int c = (int)(g / nq), q = (int)(g - (long)c * nq);
     const int p = chpoint[c];
     if (kUpper && !edge.point_full(p)) continue;
@@ -1049,7 +1049,7 @@ __global__ void __launch_bounds__(kBlock) k_blur2(int R4, FastDiv divR4, uint32_
                                                   const int* __restrict__ nbr, const T* __restrict__ src,
                                                   T* __restrict__ dst, const float* __restrict__ dsrc,
                                                   float* __restrict__ ddst, EdgeMap edge, int nt) {
-  const uint32_t stride = gridDim.x * blockDim.x, t0 = xcd_block() * blockDim.x + threadIdx.x;
+  const uint32_t stride = gridDim.x * kBlock, t0 = xcd_block() * kBlock + threadIdx.x;
   for (uint32_t t = t0; t < nq; t += stride) {  // lower rows
     const uint32_t i = divR4.div(t);
     blur_quad(nbr, src, dst, 0, i, (uint32_t)R4, t - i * (uint32_t)R4, nt);
@@ -1075,7 +1075,7 @@ __global__ void __launch_bounds__(kBlock) k_obin(const float* __restrict__ Q, co
                                                  int2* __restrict__ elist) {
   // one lane per (pixel, quad of labels): four bins, one 4-byte store (padding labels get 0)
   const uint32_t nq = divNQ.d, n = (uint32_t)N * nq;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
+  for (uint32_t g = blockIdx.x * kBlock + threadIdx.x; g < n; g += gridDim.x * kBlock) {
     const uint32_t pix = divNQ.div(g), q = g - pix * nq;
     const int img = pimg ? pimg[pix] : 0;
     uint32_t word = 0;
@@ -1150,7 +1150,7 @@ __global__ void __launch_bounds__(1024) k_ocombo(int* __restrict__ edge, int par
 __global__ void __launch_bounds__(kBlock) k_odconv(const int* __restrict__ edge, int parts, size_t n,
                                                    unsigned long long* __restrict__ dfix, float* __restrict__ dv) {
   if (!edge[(size_t)parts * kP_STRIDE + kG_CORR]) return;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
     const unsigned long long v = dfix[i];
     dv[i] = (float)((double)v * (1.0 / kDFix));
     if (v) dfix[i] = 0;
@@ -1280,7 +1280,7 @@ struct ZeroRows {
   float* p[4];
 };
 __global__ void __launch_bounds__(kBlock) k_zero_rows(ZeroRows z, size_t n) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
   if (t < n) z.p[blockIdx.y][t] = 0.f;
 }
 
@@ -1793,13 +1793,13 @@ namespace {
 // dst[i] = src[i] + add (add only to entries >= 0 when skip_neg)
 __global__ void __launch_bounds__(kBlock) k_copy_add(const int* __restrict__ src, int* __restrict__ dst, size_t n,
                                                      int add, int skip_neg) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
   if (t >= n) return;
   const int v = src[t];
   dst[t] = (skip_neg && v < 0) ? v : v + add;
 }
 __global__ void __launch_bounds__(kBlock) k_fill_u16(uint16_t* __restrict__ p, size_t n, uint16_t v) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
   if (t < n) p[t] = v;
 }
 }  // namespace
