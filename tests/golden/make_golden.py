@@ -10,6 +10,7 @@ compressed .npz files.  Run from the repository root:
     python tests/golden/make_golden.py restricted  # only ordered_restricted_31x23.npz
     python tests/golden/make_golden.py bf       # only bf_12x9_L4.npz
     python tests/golden/make_golden.py data     # only data_img.json / data_img_labels.npz
+    python tests/golden/make_golden.py data_lp_steps  # only data_img_lp_steps.npz (after data)
     python tests/golden/make_golden.py kgrad    # only kgrad_31x23.npz
     python tests/golden/make_golden.py dc       # only dc_L4.npz
     python tests/golden/make_golden.py eho      # only eho_32x24_L21.npz
@@ -358,6 +359,43 @@ def data_case():
     print("data_img", {k: v for k, v in out.items() if k.startswith("e_")})
 
 
+def data_lp_steps_case():
+    """The LP methods on the reference's own example input, cut to their first
+    one and two proximal steps (src/densecrf.cpp:526-799 with prox_max_iter 1
+    and 2), from the reference's own unary_init: those steps come before the
+    rounding spread of the full ten-step runs sets in.  The initial field is
+    kept as its difference from the oracle's unary_init (the entries where
+    Eigen's vectorised exp rounds differently), so a test rebuilds the
+    reference's field bit for bit; the discretised energy and labels of every
+    cut run are kept."""
+    import densecrf_ho_amd as D
+    src = os.path.join(OUT, "data")
+    p, W, H = O.ref_probimage(os.path.join(src, "img.c_unary"))
+    U = (-np.log(p.astype(np.float64))).astype(np.float32)
+    im, _, _ = D.load_image(os.path.join(src, "img.bmp"))
+    labs = dict(np.load(os.path.join(OUT, "data_img_labels.npz"), allow_pickle=False))
+    out = {}
+    for method, layers in [("lp", False), ("lp_sp", True)]:
+        r = O.RefCRF(W, H, U.shape[1])
+        r.set_unary(U)
+        r.add_gaussian(2, 2, 3)
+        r.add_bilateral(30, 30, 8, 8, 8, im, 2)
+        if layers:
+            r.add_layer(labs["regions"], labs["exp_of_superpixels"], labs["constants"])
+        Q0 = r.unary_init()
+        if method == "lp":
+            q0o = O.unary_init(U)
+            idx = np.flatnonzero(Q0.reshape(-1) != q0o.reshape(-1)).astype(np.int32)
+            out["q0_idx"], out["q0_val"] = idx, Q0.reshape(-1)[idx]
+        for s in (1, 2):
+            lab = r.current_map(r.lp(Q0, O.LPParams(prox_max_iter=s)))
+            out[f"e_{method}_{s}"] = np.float64(r.energy_true(lab))
+            out[f"lab_{method}_{s}"] = lab.astype(np.uint8)
+    np.savez_compressed(os.path.join(OUT, "data_img_lp_steps.npz"), **out)
+    print("data_img_lp_steps", {k: float(v) for k, v in out.items() if k.startswith("e_")},
+          "init entries patched", out["q0_idx"].size)
+
+
 def kgrad_case():
     """kernel (feature) gradients: Permutohedral::gradient on the 31x23
     lattices (d=2 and d=5, V=5), and a standalone PairwisePotential's
@@ -476,6 +514,9 @@ def main():
     if sys.argv[1:] == ["data"]:
         data_case()
         return
+    if sys.argv[1:] == ["data_lp_steps"]:
+        data_lp_steps_case()
+        return
     if sys.argv[1:] == ["bf"]:
         bf_case()
         return
@@ -507,6 +548,7 @@ def main():
     dc_case()
     eho_case()
     data_case()
+    data_lp_steps_case()
     c0_case()
 
 

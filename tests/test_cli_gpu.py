@@ -62,3 +62,44 @@ def test_example_inference_on_reference_data(tmp_path, method):
     agree = float((img == want).all(-1).mean())
     print("label agreement", agree, flush=True)
     assert agree >= (ref[f"agree_{method}_min"] - 0.005 if lp else 0.999)
+
+
+def _reference_data_lp_case():
+    """the reference's data/img.* with its example_inference pairwise terms,
+    its mean-shift layer, and its own unary_init rebuilt bit for bit (the
+    oracle's softmax with the entries where Eigen's exp rounds differently
+    patched from the fixture)"""
+    import densecrf_ho_amd as D
+    import oracle as O
+    g = np.load(os.path.join(GOLDEN, "data_img_lp_steps.npz"), allow_pickle=False)
+    labs = np.load(os.path.join(GOLDEN, "data_img_labels.npz"), allow_pickle=False)
+    U, W, H = D.load_unary(os.path.join(DATA, "img.c_unary"))
+    im, _, _ = D.load_image(os.path.join(DATA, "img.bmp"))
+    Q0 = O.unary_init(U).reshape(-1)
+    Q0[g["q0_idx"]] = g["q0_val"]
+    return g, labs, U, W, H, im, Q0.reshape(U.shape)
+
+
+@pytest.mark.parametrize("method,steps", [("lp", 1), ("lp", 2), ("lp_sp", 1), ("lp_sp", 2)])
+def test_lp_first_steps_on_reference_data(method, steps):
+    """The LP methods on the reference's own input from the reference's own
+    initial field, cut to one and two proximal steps (before the rounding
+    spread of the full runs sets in): the discretised energy within 1e-4 of
+    the compiled reference's and >= 99.9% of the labels equal."""
+    import densecrf_ho_amd as D
+    g, labs, U, W, H, im, Q0 = _reference_data_lp_case()
+    print("case", method, steps, flush=True)
+    c = D.DenseCRF2D(W, H, U.shape[1])
+    c.setUnaryEnergy(U)
+    c.addPairwiseGaussian(2, 2, D.PottsCompatibility(3))
+    c.addPairwiseBilateral(30, 30, 8, 8, 8, im, D.PottsCompatibility(2))
+    if method == "lp_sp":
+        c.addSuperPixelLayer(labs["regions"], labs["exp_of_superpixels"], labs["constants"])
+    Q = c.lp_inference_prox_super_pixels(Q0, D.LP_inf_params(prox_max_iter=steps))
+    lab = np.ascontiguousarray(Q.argmax(1).astype(np.int32))
+    e, er = c.assignment_energy_true(lab), float(g[f"e_{method}_{steps}"])
+    agree = float((lab == g[f"lab_{method}_{steps}"]).mean())
+    print("energy", e, "reference", er, "label agreement", agree, flush=True)
+    assert abs(e - er) <= 1e-4 * abs(er)
+    assert agree >= 0.999
+    c.close()
