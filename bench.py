@@ -36,13 +36,33 @@ PEAK_HBM_GBS = 8000.0
 METRIC = "ms/solve QP & LP (1080p, 21 labels) + filter HBM GB/s; batched images/s"
 
 
-def algorithmic_bytes(name: str, N: int, M: dict, V: int = L, S: int = 4) -> float:
+def algorithmic_bytes(name: str, N: int, M: dict, V: int = L, S: int = 4, prof: dict | None = None) -> float:
     """Bytes a kernel launch must move at minimum (one read of every input it
-    needs, one write of every output), see DESIGN.md "Kernels and rooflines"."""
+    needs, one write of every output), see DESIGN.md "Kernels and rooflines".
+    The coded ordered filter's kernels move the rows of mixed labels only:
+    their figure is the average over the launches of the profiled step, from
+    its counts of stored (row, label) pairs after the splat and each axis."""
     kind, tag = name.split("[")
     d = int(tag.split("d=")[1].split(",")[0])
     m = M[d]
     e = N * (d + 1)
+    if kind in ("csplat", "cblur", "cslice"):
+        if not prof or name not in prof:
+            return float("nan")
+        LC = (V + 3) // 4 * 4
+        launches = prof[name][0]
+        rows = [prof.get(f"rows[{tag}:{j}", (0, float("nan")))[1] for j in range(d + 2)]
+        calls = prof.get(f"rows[{tag}:0", (0, 0.0))[0]
+        if calls == 0:
+            return float("nan")
+        row = 9 * S  # one label's nine bins
+        if kind == "csplat":
+            tot = calls * (N * V + 8 * e + m * LC) + row * rows[0]
+        elif kind == "cblur":
+            tot = calls * (d + 1) * (2 * m * LC + 12 * m) + row * sum(rows[j] + rows[j + 1] for j in range(d + 1))
+        else:
+            tot = calls * (N * V + 8 * e + S * m + m * LC + 2 * S * N * V) + row * rows[d + 1]
+        return tot / launches
     if kind == "splat":
         return S * N * V + 8 * e + S * m * V + 4 * N
     if kind == "blur":
@@ -57,7 +77,7 @@ def algorithmic_bytes(name: str, N: int, M: dict, V: int = L, S: int = 4) -> flo
     if kind == "ogather":
         return 4 * e + 2 * e * V
     if kind == "osplat":
-        return e * V + 4 * e + S * m * 9 * V
+        return N * V + 8 * e + S * m * 9 * V
     if kind == "oblur":
         return 2 * S * m * 9 * V + 8 * m
     if kind == "oslice":
@@ -415,7 +435,7 @@ def main():
         D.profile_enable(False)
 
     def roofline_kernel(pr):
-        lat = {n: v for n, v in pr.items() if "[" in n and math.isfinite(algorithmic_bytes(n, wl.N, Ms))}
+        lat = {n: v for n, v in pr.items() if "[" in n and math.isfinite(algorithmic_bytes(n, wl.N, Ms, prof=pr))}
         return max(lat, key=lambda n: lat[n][1])
 
     dom = roofline_kernel(prof) if prof else ""
@@ -449,6 +469,11 @@ def main():
         D.synchronize()
         pcie_ms = (time.perf_counter() - tp) * 1e3
 
+    # iterations the QP ran: at most QP_ITERS, fewer when the reference's
+    # break on a zero numerator or denominator ends it (src/densecrf.cpp:419)
+    _, qst = crf.qp_inference_super_pixels_non_convex(Q0, max_iter=QP_ITERS, ref_stop=False, out=Qqp, stats=True)
+    qp_iters = int(qst["iters"])
+
     ms_per_step = elapsed / args.steps * 1e3
     # HBM bytes per launch of each kernel, from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE passes over this same workload (scaled by the
@@ -463,8 +488,19 @@ def main():
     lat_k = {n: v for n, v in prof.items() if "[" in n}
     cnt, tot = prof_timed[dom]
     avg_ms = tot / cnt
-    ab = algorithmic_bytes(dom, wl.N, Ms)
+    ab = algorithmic_bytes(dom, wl.N, Ms, prof=prof)
     achieved = ab / (avg_ms * 1e-3) / 1e9
+    # whole ordered filter (one LP step's call of both terms: the shared bin
+    # pass, then per term the bin-edge pass, splat, d+1 blur axes and slice):
+    # the sum of its kernels' algorithmic bytes over the sum of their times
+    ofb = oft = 0.0
+    for n, (c_, ms_) in lat_k.items():
+        if n.split("[")[0] in ("obin", "osplat", "oblur", "oslice", "oedge", "csplat", "cblur", "cslice",
+                               "osplat_up"):
+            b_ = algorithmic_bytes(n, wl.N, Ms, prof=prof)
+            ofb += (b_ if math.isfinite(b_) else 0.0) * c_
+            oft += ms_ * 1e-3
+    ordered_gbs = ofb / oft / 1e9 if oft > 0 else None
     # whole-filter rate of the plain bilateral filter inside the QP
     filt = [n for n in lat_k if n.split("[")[0] in ("splat", "blur", "slice") and "d=5" in n]
     fb = sum(algorithmic_bytes(n, wl.N, Ms) * lat_k[n][0] for n in filt)
@@ -492,14 +528,22 @@ def main():
         "lattice_build_ms": round(build_ms, 2),
         "pcie_inclusive_ms_per_solve": None if pcie_ms is None else round(pcie_ms, 3),
         "lp_iterations": lp_info or None,
+        "qp_iterations": qp_iters,
         "filter_gbps": None if filter_gbs is None else round(filter_gbs, 1),
+        "ordered_filter": None if ordered_gbs is None else {
+            "ms_per_solve": round(oft * 1e3, 3), "achieved": round(ordered_gbs, 1), "unit": "GB/s",
+            "frac": round(ordered_gbs / PEAK_HBM_GBS, 4),
+            "kernels": "obin, oedge, csplat, osplat_up, cblur, cslice (both terms, every LP step)"},
         "images_per_s": None,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": (traffic or {}).get("bytes_per_launch", {}).get(dom),
                      "traffic_source": traffic["source"] if traffic else None,
                      "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": int(ab)},
-        "kernels": {n: {"launches": c, "total_ms": round(m, 3)} for n, (c, m) in sorted(prof.items())},
+        "kernels": {n: {"launches": c, "total_ms": round(m, 3)} for n, (c, m) in sorted(prof.items())
+                    if not n.startswith("rows[")},
+        "ordered_stored_rows": {n: {"calls": c, "rows": int(m)} for n, (c, m) in sorted(prof.items())
+                                if n.startswith("rows[")},
         "autotune": {k: {"grid": g, "ms": round(ms, 4)} for k, (g, ms, _) in sorted(D.autotune_report().items())},
     }
     if not args.qp_only:

@@ -89,6 +89,14 @@ struct Profiler {
     }
     recs.clear();
   }
+  // a counted quantity reported like a kernel: (calls, total) in place of
+  // (launches, ms)
+  void count(const std::string& n, double v) {
+    std::lock_guard<std::mutex> g(mu);
+    auto& d = done[n];
+    d.first += 1;
+    d.second += v;
+  }
   void clear() {
     std::lock_guard<std::mutex> g(mu);
     resolve_locked();

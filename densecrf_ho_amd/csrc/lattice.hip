@@ -1642,12 +1642,18 @@ void Lattice::ensure_ordered_tables() {
   have_otab_ = true;
 }
 
+// Float rows take the coded filter (ordered_coded.hip), whose results equal
+// the dense rows' bit for bit; DCRF_ORDERED_DENSE=1 selects the dense rows.
+// bfloat16 rows are dense.
 void Lattice::ordered(const float* Q, const float* d_minmax, int L, const SliceArgs& out, OrderedBins* shared,
                       bool fresh) {
+  const char* dense = getenv("DCRF_ORDERED_DENSE");
   if (bf16_)
     ordered_t<uint16_t>(Q, d_minmax, L, out, shared, fresh);
-  else
+  else if (dense && dense[0] == '1')
     ordered_t<float>(Q, d_minmax, L, out, shared, fresh);
+  else
+    ordered_coded(Q, d_minmax, L, out, shared, fresh);
 }
 
 // T: storage of the lattice rows (float, or bfloat16 bits)

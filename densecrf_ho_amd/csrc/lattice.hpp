@@ -104,7 +104,9 @@ class Lattice {
                           (const void*)bins_.get(), (const void*)tb_.get(), (const void*)edge_.get(),
                           (const void*)vtx_.get(), (const void*)elist_.get(), (const void*)dfix_.get(),
                           (const void*)dvA_.get(), (const void*)dvB_.get(), (const void*)orow_.get(),
-                          (const void*)onbr_.get(), (const void*)oppart_.get()})
+                          (const void*)onbr_.get(), (const void*)oppart_.get(), (const void*)tbs_.get(),
+                          (const void*)cA_.get(), (const void*)cB_.get(), (const void*)pcode_.get(),
+                          (const void*)ptb_.get()})
       out.push_back(p);
   }
 
@@ -142,6 +144,11 @@ class Lattice {
   template <class T>
   void ordered_t(const float* Q, const float* d_minmax, int L, const SliceArgs& out, OrderedBins* shared,
                  bool fresh);
+  // the same with the lower rows held as per-(point, label) codes and the
+  // rows of mixed labels only (float rows; ordered_coded.hip)
+  void ordered_coded(const float* Q, const float* d_minmax, int L, const SliceArgs& out, OrderedBins* shared,
+                     bool fresh);
+  void ensure_coded_tables(int L);
   bool bf16_ = false;
   void ensure_total_weight();
   hipStream_t stream_;
@@ -188,6 +195,14 @@ class Lattice {
   DBuf<uint16_t> ppimg_;  // part of each lattice point (union lattices only)
   int nparts_ = 1;
   bool have_tb_ = false;
+  // coded ordered filter: the total weight after the splat and after each
+  // blur axis in row order ((d+2) x (M+1)); per-(row, label) codes of the two
+  // row buffers; codes and total weights of the chunk partials
+  DBuf<float> tbs_;
+  bool have_tbs_ = false;
+  DBuf<uint8_t> cA_, cB_, pcode_;
+  DBuf<float> ptb_;
+  size_t ccap_ = 0, pccap_ = 0;
   std::shared_ptr<RestrictedState> rs_;  // restricted ordered filter state
 };
 

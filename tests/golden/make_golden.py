@@ -360,14 +360,18 @@ def data_case():
 
 
 def data_lp_steps_case():
-    """The LP methods on the reference's own example input, cut to their first
-    one and two proximal steps (src/densecrf.cpp:526-799 with prox_max_iter 1
-    and 2), from the reference's own unary_init: those steps come before the
-    rounding spread of the full ten-step runs sets in.  The initial field is
-    kept as its difference from the oracle's unary_init (the entries where
-    Eigen's vectorised exp rounds differently), so a test rebuilds the
-    reference's field bit for bit; the discretised energy and labels of every
-    cut run are kept."""
+    """The LP methods on the reference's own example input from the
+    reference's own unary_init, cut short (src/densecrf.cpp:526-799 with
+    prox_max_iter / fw_max_iter): lp after one and two proximal steps, lp_sp
+    after the first one and two Frank-Wolfe steps of its first proximal step.
+    Those come before the rounding spread of the full runs sets in (with the
+    superpixel layer, computeUCondGrad picks the last value above and below
+    zero per region, src/densecrf.cpp:800-826, and values within rounding of
+    zero decide it from the third step on).  The initial field is kept as its
+    difference from the oracle's unary_init (the entries where Eigen's
+    vectorised exp rounds differently), so a test rebuilds the reference's
+    field bit for bit; the discretised energy and labels of every cut run are
+    kept."""
     import densecrf_ho_amd as D
     src = os.path.join(OUT, "data")
     p, W, H = O.ref_probimage(os.path.join(src, "img.c_unary"))
@@ -387,10 +391,10 @@ def data_lp_steps_case():
             q0o = O.unary_init(U)
             idx = np.flatnonzero(Q0.reshape(-1) != q0o.reshape(-1)).astype(np.int32)
             out["q0_idx"], out["q0_val"] = idx, Q0.reshape(-1)[idx]
-        for s in (1, 2):
-            lab = r.current_map(r.lp(Q0, O.LPParams(prox_max_iter=s)))
-            out[f"e_{method}_{s}"] = np.float64(r.energy_true(lab))
-            out[f"lab_{method}_{s}"] = lab.astype(np.uint8)
+        for prox, fw in ((1, 10), (2, 10)) if method == "lp" else ((1, 1), (1, 2)):
+            lab = r.current_map(r.lp(Q0, O.LPParams(prox_max_iter=prox, fw_max_iter=fw)))
+            out[f"e_{method}_{prox}x{fw}"] = np.float64(r.energy_true(lab))
+            out[f"lab_{method}_{prox}x{fw}"] = lab.astype(np.uint8)
     np.savez_compressed(os.path.join(OUT, "data_img_lp_steps.npz"), **out)
     print("data_img_lp_steps", {k: float(v) for k, v in out.items() if k.startswith("e_")},
           "init entries patched", out["q0_idx"].size)

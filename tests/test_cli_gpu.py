@@ -80,25 +80,28 @@ def _reference_data_lp_case():
     return g, labs, U, W, H, im, Q0.reshape(U.shape)
 
 
-@pytest.mark.parametrize("method,steps", [("lp", 1), ("lp", 2), ("lp_sp", 1), ("lp_sp", 2)])
-def test_lp_first_steps_on_reference_data(method, steps):
+@pytest.mark.parametrize("method,prox,fw", [("lp", 1, 10), ("lp", 2, 10), ("lp_sp", 1, 1), ("lp_sp", 1, 2)])
+def test_lp_first_steps_on_reference_data(method, prox, fw):
     """The LP methods on the reference's own input from the reference's own
-    initial field, cut to one and two proximal steps (before the rounding
-    spread of the full runs sets in): the discretised energy within 1e-4 of
-    the compiled reference's and >= 99.9% of the labels equal."""
+    initial field, cut short before the rounding spread of the full runs sets
+    in (lp: one and two proximal steps; lp_sp: the first one and two
+    Frank-Wolfe steps, since from the third on computeUCondGrad's choice of the
+    last value above and below zero per region follows values within rounding
+    of zero): the discretised energy within 1e-4 of the compiled reference's
+    and >= 99.9% of the labels equal."""
     import densecrf_ho_amd as D
     g, labs, U, W, H, im, Q0 = _reference_data_lp_case()
-    print("case", method, steps, flush=True)
+    print("case", method, prox, fw, flush=True)
     c = D.DenseCRF2D(W, H, U.shape[1])
     c.setUnaryEnergy(U)
     c.addPairwiseGaussian(2, 2, D.PottsCompatibility(3))
     c.addPairwiseBilateral(30, 30, 8, 8, 8, im, D.PottsCompatibility(2))
     if method == "lp_sp":
         c.addSuperPixelLayer(labs["regions"], labs["exp_of_superpixels"], labs["constants"])
-    Q = c.lp_inference_prox_super_pixels(Q0, D.LP_inf_params(prox_max_iter=steps))
+    Q = c.lp_inference_prox_super_pixels(Q0, D.LP_inf_params(prox_max_iter=prox, fw_max_iter=fw))
     lab = np.ascontiguousarray(Q.argmax(1).astype(np.int32))
-    e, er = c.assignment_energy_true(lab), float(g[f"e_{method}_{steps}"])
-    agree = float((lab == g[f"lab_{method}_{steps}"]).mean())
+    e, er = c.assignment_energy_true(lab), float(g[f"e_{method}_{prox}x{fw}"])
+    agree = float((lab == g[f"lab_{method}_{prox}x{fw}"]).mean())
     print("energy", e, "reference", er, "label agreement", agree, flush=True)
     assert abs(e - er) <= 1e-4 * abs(er)
     assert agree >= 0.999

@@ -90,6 +90,56 @@ def test_ordered_filter_bin_edge_modes(ncombo):
     assert rel_err(out, ref.ordered(q, fp64=True)) < 1e-4
 
 
+def _ordered_fields(kind, N, L, rng, cells):
+    """rescaled fields in [0, 1] of the forms the LP produces: one-hot rows
+    (every label of a point uniform), a background level in one bin with the
+    dominant label near 1 and a few spread values, dense random values, and
+    values on bin edges in 1 or 9 (label, bin) pairs (correction and full
+    upper-pass modes)"""
+    lab = cells % L
+    if kind == "onehot":
+        q = np.zeros((N, L), np.float32)
+        q[np.arange(N), lab] = 1.0
+    elif kind == "background":
+        q = (0.1 + 0.08 * rng.random((N, L))).astype(np.float32)
+        q[np.arange(N), lab] = (0.85 + 0.15 * rng.random(N)).astype(np.float32)
+        m = rng.random((N, L)) < 0.03
+        q[m] = rng.random(m.sum()).astype(np.float32)
+    else:
+        q = rng.random((N, L)).astype(np.float32)
+    if kind.startswith("edges"):
+        n = int(kind[5:])
+        combos = [(k, v) for k in range(3) for v in (0.5, 0.7, 0.9)][:n]
+        for k, v in combos:
+            q[rng.choice(N, 40, replace=False), k] = np.float32(v)
+    return q
+
+
+@pytest.mark.parametrize("L", [5, 21, 33])
+@pytest.mark.parametrize("kind", ["onehot", "background", "random", "edges1", "edges9"])
+@pytest.mark.parametrize("d", [2, 5])
+def test_coded_ordered_filter_equals_dense_rows(kind, d, L, monkeypatch):
+    """The coded ordered filter (per-(point, label) codes, rows of mixed
+    labels only) gives the dense rows' results bit for bit, in every bin-edge
+    mode; both against the float oracle within 1e-5."""
+    from densecrf_ho_amd import synthetic as S
+    print("case", kind, "d", d, "L", L, flush=True)
+    wl = S.Workload(320, 213, 40, 6)
+    feat = wl.kernels[0 if d == 2 else 1][0]
+    rng = np.random.default_rng(L * 100 + d)
+    cells = np.asarray(wl.region).reshape(-1)
+    q = _ordered_fields(kind, wl.N, L, rng, cells)
+    lat = gpu_lattice(feat)
+    monkeypatch.delenv("DCRF_ORDERED_DENSE", raising=False)
+    coded = lat.compute_upper_minus_lower_ord(q)
+    coded2 = lat.compute_upper_minus_lower_ord(q)  # a second call over the first call's buffers
+    monkeypatch.setenv("DCRF_ORDERED_DENSE", "1")
+    dense = lat.compute_upper_minus_lower_ord(q)
+    np.testing.assert_array_equal(coded.view(np.uint32), dense.view(np.uint32))
+    np.testing.assert_array_equal(coded2.view(np.uint32), dense.view(np.uint32))
+    assert rel_err(coded, O.Lattice(feat).ordered(q)) < 1e-5
+
+
 def test_ordered_filter_rejects_out_of_range(golden):
     import densecrf_ho_amd as D
     g = golden("lattice_b5_31x23.npz")

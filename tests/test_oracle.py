@@ -27,8 +27,8 @@ How each oracle function is pinned:
       fixtures (bit-exact final Q) and 320x213 energies; feasible_Q against a
       textbook simplex projection; renormalize; ucondgrad (planted 8 fails);
       on the reference's own data/img.* (with and without its mean-shift
-      layer) the runs cut to one and two proximal steps, started from the
-      reference's own unary_init, give its labels and energies exactly.
+      layer) runs cut short, started from the reference's own unary_init,
+      give its labels and energies exactly.
   energies (assignment true / higher order, LP energy, QP value)
       fixtures (exact doubles).
   LP less-confident stop (less_confident_percent, confidence_tol)
@@ -507,11 +507,11 @@ def test_random_lattices_against_compiled_reference(d, N):
     np.testing.assert_array_equal(a.ordered(q), b.ordered(q))
 
 
-@pytest.mark.parametrize("method,steps", [("lp", 1), ("lp_sp", 1), ("lp_sp", 2)])
-def test_lp_first_steps_on_reference_data(method, steps):
+@pytest.mark.parametrize("method,prox,fw", [("lp", 1, 10), ("lp_sp", 1, 1), ("lp_sp", 1, 2)])
+def test_lp_first_steps_on_reference_data(method, prox, fw):
     """the float oracle's LP on data/img.* from the reference's own initial
-    field (rebuilt bit for bit), cut to one or two proximal steps: the
-    compiled reference's labels and discretised energy exactly"""
+    field (rebuilt bit for bit), cut short: the compiled reference's labels and
+    discretised energy exactly"""
     import densecrf_ho_amd as D
     from densecrf_ho_amd import synthetic as S
     g = np.load(os.path.join(GOLDEN, "data_img_lp_steps.npz"), allow_pickle=False)
@@ -526,7 +526,7 @@ def test_lp_first_steps_on_reference_data(method, steps):
     c.add_pairwise(S.bilateral_features(W, H, 30, 30, 8, 8, 8, im), 2.0)
     if method == "lp_sp":
         c.add_layer(labs["regions"], labs["exp_of_superpixels"], labs["constants"])
-    Q, _ = c.lp(Q0.reshape(U.shape), O.LPParams(prox_max_iter=steps))
+    Q, _ = c.lp(Q0.reshape(U.shape), O.LPParams(prox_max_iter=prox, fw_max_iter=fw))
     lab = c.current_map(Q)
-    np.testing.assert_array_equal(lab, g[f"lab_{method}_{steps}"])
-    assert c.energy_true(lab) == float(g[f"e_{method}_{steps}"])
+    np.testing.assert_array_equal(lab, g[f"lab_{method}_{prox}x{fw}"])
+    assert c.energy_true(lab) == float(g[f"e_{method}_{prox}x{fw}"])
