@@ -25,7 +25,10 @@
 // in the ordered row order (Morton order per part, row 0 the zero row);
 // codes are one byte per (row, label) in rows of LC = L rounded up to 4.
 // The upper pass, the bin-edge correction rows and the full upper rows of
-// parts in full mode are those of the dense filter.
+// parts in full mode are those of the dense filter.  With six vertices per
+// pixel (d = 5) the last blur axis writes dense bin-major rows and the dense
+// slice reads them (most labels of a pixel read bin 1, one run per row); with
+// three (d = 2) the slice reads the codes and the stored rows.
 
 namespace dcrf {
 namespace {
@@ -46,9 +49,6 @@ __device__ inline int code3(int co, int ca, int cb) { return code_join(code_join
 // slot r (bin r + 1) of a row with code c (kCodeZero reads Tb = 0)
 __device__ inline float coded_slot(int c, int r, float tb) { return r + 1 <= c ? tb : 0.f; }
 
-struct __attribute__((packed, aligned(4))) F9 {
-  float v[9];
-};
 __device__ inline void ld9(const float* p, float (&v)[9]) {
   const Float4u a = *reinterpret_cast<const Float4u*>(p), b = *reinterpret_cast<const Float4u*>(p + 4);
   v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
@@ -250,6 +250,63 @@ __global__ void __launch_bounds__(kBlock) k_cblur(int L, int RS, int LC, RowMap 
     }
 }
 
+// The last blur axis with its output written as dense bin-major rows (bin b
+// of label k at (b - 1) * L + k, every label: a uniform one from the total
+// weight after the axis, tb_out), read by the dense bin-major slice
+// (k_oslice), which loads only the bins a pixel's value selects.
+__global__ void __launch_bounds__(kBlock) k_cblur_expand(int L, int RS, int LC, RowMap rm, uint32_t M,
+                                                         const int* __restrict__ nbr, const float* __restrict__ src,
+                                                         float* __restrict__ dst, const uint8_t* __restrict__ csrc,
+                                                         const float* __restrict__ tb_in,
+                                                         const float* __restrict__ tb_out,
+                                                         const float* __restrict__ dsrc, float* __restrict__ ddst,
+                                                         int R4, FastDiv divR4, uint32_t nq, size_t half_q,
+                                                         EdgeMap edge) {
+  {
+    long i;
+    int k, ks;
+    rm.at(L, i, k, ks);
+    if (i >= 0 && i < (long)M) {
+      const int n1 = nbr[2 * i] + 1, n2 = nbr[2 * i + 1] + 1;
+      const size_t r = (size_t)i + 1;
+      const int co = csrc[r * LC + k], ca = csrc[(size_t)n1 * LC + k], cb = csrc[(size_t)n2 * LC + k];
+      const int code = code3(co, ca, cb);
+      float o[kOBins];
+      if (code == kCodeRow) {
+        float a[kOBins], bb[kOBins];
+        auto row = [&](size_t rr, int cc, float (&v)[kOBins]) {
+          if (cc == kCodeRow) {
+            ld9(src + rr * RS + (size_t)k * kOBins, v);
+          } else {
+            const float t = tb_in[rr];
+#pragma unroll
+            for (int x = 0; x < kOBins; ++x) v[x] = coded_slot(cc, x, t);
+          }
+        };
+        row(r, co, o);
+        row((size_t)n1, ca, a);
+        row((size_t)n2, cb, bb);
+#pragma unroll
+        for (int x = 0; x < kOBins; ++x) o[x] = o[x] + 0.5f * (a[x] + bb[x]);
+      } else {
+        const float t = tb_out[r];
+#pragma unroll
+        for (int x = 0; x < kOBins; ++x) o[x] = coded_slot(code, x, t);
+      }
+#pragma unroll
+      for (int x = 0; x < kOBins; ++x) __builtin_nontemporal_store(o[x], dst + r * RS + (size_t)x * L + k);
+    }
+  }
+  const uint32_t stride = gridDim.x * kBlock, t0 = xcd_block() * kBlock + threadIdx.x;
+  if (edge.any_corr())  // correction rows: two quads per point
+    for (uint32_t t = t0; t < 2 * M; t += stride) blur_quad(nbr, dsrc, ddst, 0, t >> 1, 2u, t & 1u, false);
+  if (edge.any_full())  // upper rows of the points of parts in full mode
+    for (uint32_t t = t0; t < nq; t += stride) {
+      const uint32_t pt = divR4.div(t);
+      if (edge.point_full(pt)) blur_quad(nbr, src, dst, half_q, pt, (uint32_t)R4, t - pt * (uint32_t)R4, true);
+    }
+}
+
 // Lower slice then upper slice, as k_oslice on label-major rows: a vertex
 // whose code is uniform c gives LO[lo] and LO[up + 1] from its total weight
 // (vt.z) without a row load.  TAG 2 and 5 fix the vertex count.
@@ -334,6 +391,104 @@ __global__ void __launch_bounds__(kBlock) k_cslice(const uint8_t* __restrict__ b
   for (int j = 0; j < NV; ++j)
     if (j < nv) acc = acc + uv[j] * (vt[j].y * alpha);
   slice_store_pre(o, (int)pix, k0, L, acc, pre);
+}
+
+// The same slice with one lane per (pixel, quad of labels) (TAG 2 with at
+// least as many quads as vertices): the pixel's bins of four labels are one
+// 4-byte load, the codes of four labels at a vertex one 4-byte load, the
+// output four values; ten pixels per wave at L = 21.
+template <int TAG>
+__global__ void __launch_bounds__(kBlock) k_cslice4(const uint8_t* __restrict__ bins, int L, int LP, int RS, int LC,
+                                                    RowMap rm, int N, const float4* __restrict__ vtx,
+                                                    const float* __restrict__ lo_v, const uint8_t* __restrict__ codes,
+                                                    const float* __restrict__ up_v, const float* __restrict__ dv,
+                                                    EdgeMap edge, float alpha, SliceArgs o) {
+  constexpr int NV = TAG == 2 ? 3 : 6;
+  const int nq = (L + 3) >> 2;
+  long pix;
+  int q, qs;
+  rm.at(nq, pix, q, qs);
+  if (pix < 0 || pix >= N) return;
+  const int k0 = 4 * q, nk = min(4, L - k0);
+  float* dst = o.out + (size_t)pix * L + k0;
+  const float nm = o.norm ? o.norm[pix] : 1.f;
+  float prev[4] = {0.f, 0.f, 0.f, 0.f};
+  if (o.accumulate) {
+    if (nk == 4) {
+      const Float4u u = *reinterpret_cast<const Float4u*>(dst);
+      prev[0] = u.x, prev[1] = u.y, prev[2] = u.z, prev[3] = u.w;
+    } else {
+      for (int i = 0; i < nk; ++i) prev[i] = dst[i];
+    }
+  }
+  const uint32_t bw = *reinterpret_cast<const uint32_t*>(bins + (size_t)pix * LP + k0);
+  float4 mine = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (q < NV) mine = vtx[(size_t)pix * NV + q];
+  const int g0 = (int)(threadIdx.x & 63) - q;
+  float4 vt[NV];
+#pragma unroll
+  for (int j = 0; j < NV; ++j)
+    vt[j] = make_float4(__shfl(mine.x, g0 + j), __shfl(mine.y, g0 + j), __shfl(mine.z, g0 + j), 0.f);
+  uint32_t cw[NV];
+#pragma unroll
+  for (int j = 0; j < NV; ++j)
+    cw[j] = *reinterpret_cast<const uint32_t*>(codes + (size_t)__float_as_int(vt[j].x) * LC + k0);
+  const int part = edge.ppart(pix);
+  const int mode = edge.mode(part);
+  const bool full = mode > kMaxCombo;
+  float x[4];
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+    x[kk] = 0.f;
+    if (kk >= nk) continue;
+    const int k = k0 + kk;
+    const int bb = (int)((bw >> (8 * kk)) & 255u);
+    const int lo = bb & 15, up = bb >> 4;
+    const int i0 = min(max(lo, 1), 8) - 1;
+    const size_t ip = (size_t)k * kOBins + i0;
+    const bool a_hi = lo - 1 - i0 == 1;
+    const bool u_hi = up - i0 == 1;
+    float a[NV], u[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      const int c = (int)((cw[j] >> (8 * kk)) & 255u);
+      if (c == kCodeRow) {
+        const float2 pr = ld_pair(lo_v, (size_t)__float_as_int(vt[j].x) * RS + ip);
+        a[j] = a_hi ? pr.y : pr.x;
+        u[j] = u_hi ? pr.y : pr.x;
+      } else {
+        a[j] = lo <= c ? vt[j].z : 0.f;
+        u[j] = up + 1 <= c ? vt[j].z : 0.f;
+      }
+    }
+    if (full && up < 9)
+#pragma unroll
+      for (int j = 0; j < NV; ++j) u[j] = up_v[(size_t)__float_as_int(vt[j].x) * RS + (size_t)k * kOBins + up];
+    int ci = -1;
+    if (mode >= 1 && mode <= kMaxCombo && up < 9)
+      for (int c = 0; c < mode; ++c)
+        if (edge.combo(part, c) == k * 16 + up + 1) ci = c;
+    float acc = 0.f;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      const float lv = lo == 0 ? vt[j].z : a[j];
+      acc = acc + lv * (-vt[j].y * alpha);
+    }
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      float uv = full ? (up == 9 ? vt[j].z : u[j]) : vt[j].z - (up < 9 ? u[j] : 0.f);
+      if (ci >= 0) uv = uv + dv[(size_t)__float_as_int(vt[j].x) * kMaxCombo + ci];
+      acc = acc + uv * (vt[j].y * alpha);
+    }
+    if (o.norm) acc = acc * nm;
+    if (o.apply_scale) acc = o.scale * acc;
+    if (o.post_mult != 1.f) acc = o.post_mult * acc;
+    x[kk] = o.accumulate ? prev[kk] + acc : acc;
+  }
+  if (nk == 4)
+    *reinterpret_cast<Float4u*>(dst) = Float4u{x[0], x[1], x[2], x[3]};
+  else
+    for (int i = 0; i < nk; ++i) dst[i] = x[i];
 }
 
 // tbs[orow[p]] = v[p + 1] (one stage of the total weight, point rows to
@@ -499,11 +654,25 @@ void Lattice::ordered_coded(const float* Q, const float* d_minmax, int L, const 
     check_launch("k_csplat");
   }
   count_rows(csrc, 0);
-  {  // the upper pass of parts in full mode (exits at once otherwise), label-major
+  // the slice: d = 5 (six vertices) reads dense bin-major rows that the last
+  // blur axis writes out, where a pixel's labels mostly read one run of the
+  // row (bin 1 of every label); otherwise the coded rows, d = 2 with one lane
+  // per quad of labels when a pixel's vertices fit its quads
+  const int nqc = (L + 3) / 4;
+  const RowMap rq = rowmap(nqc);
+  const bool expand = d_ == 5;
+  const bool quad = d_ == 2 && nqc >= d_ + 1;
+  {  // the upper pass of parts in full mode (exits at once otherwise), in the slice's layout
     KTimer tm("osplat_up" + tag, s);
-    k_osplat<102, float><<<std::min(nvb, 2048u), kBlock, 0, s>>>(
-        bins, L, LP, row, nlanes, sp_pix_.get(), sp_w_.get(), seg_.get(), chstart_.get(), chpoint_.get(),
-        pstart_.get(), nchunks_, orow_.get(), srcU, part_.get(), em);
+    auto up = [&](auto kern) {
+      kern<<<std::min(nvb, 2048u), kBlock, 0, s>>>(bins, L, LP, row, nlanes, sp_pix_.get(), sp_w_.get(), seg_.get(),
+                                                   chstart_.get(), chpoint_.get(), pstart_.get(), nchunks_,
+                                                   orow_.get(), srcU, part_.get(), em);
+    };
+    if (expand)
+      up(k_osplat<105, float>);
+    else
+      up(k_osplat<102, float>);
     if (nmulti_ > 0) {
       const int pv = kOBins * L;
       const RowMap rw = rowmap(pv);
@@ -521,32 +690,45 @@ void Lattice::ordered_coded(const float* Q, const float* d_minmax, int L, const 
     const int* nb = onbr_.get() + (size_t)j * M_ * 2;
     {
       KTimer tm("cblur" + tag, s);
-      k_cblur<<<rows_grid(M_, L), kBlock, 0, s>>>(L, row, LC, rm, (uint32_t)M_, nb, srcL, dstL, csrc, cdst,
-                                                   tbs_.get() + (size_t)j * (M_ + 1), dsrc, ddst, R4, divR4,
-                                                   (uint32_t)nquads, half_q, emr, j == d_);
+      if (expand && j == d_)
+        k_cblur_expand<<<rows_grid(M_, L), kBlock, 0, s>>>(
+            L, row, LC, rm, (uint32_t)M_, nb, srcL, dstL, csrc, tbs_.get() + (size_t)j * (M_ + 1),
+            tbs_.get() + (size_t)(j + 1) * (M_ + 1), dsrc, ddst, R4, divR4, (uint32_t)nquads, half_q, emr);
+      else
+        k_cblur<<<rows_grid(M_, L), kBlock, 0, s>>>(L, row, LC, rm, (uint32_t)M_, nb, srcL, dstL, csrc, cdst,
+                                                     tbs_.get() + (size_t)j * (M_ + 1), dsrc, ddst, R4, divR4,
+                                                     (uint32_t)nquads, half_q, emr, j == d_);
     }
     check_launch("k_cblur");
     std::swap(srcL, dstL);
     std::swap(srcU, dstU);
     std::swap(csrc, cdst);
     std::swap(dsrc, ddst);
-    count_rows(csrc, j + 1);
+    if (!(expand && j == d_)) count_rows(csrc, j + 1);
   }
   if (counting) {
     std::vector<unsigned long long> h(d_ + 2);
     DCRF_HIP(hipMemcpyAsync(h.data(), nrows.get(), h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     DCRF_HIP(hipStreamSynchronize(s));
+    if (expand) h[d_ + 1] = (unsigned long long)M_ * L;  // the last axis writes every row out
     for (int j = 0; j <= d_ + 1; ++j) Profiler::get().count("rows" + tag + ":" + std::to_string(j), (double)h[j]);
   }
-  KTimer tm("cslice" + tag, s);
+  KTimer tm((expand ? "oslice" : "cslice") + tag, s);
   auto slice = [&](auto kern) {
     kern<<<rows_grid(N_, L), kBlock, 0, s>>>(bins, L, LP, row, LC, rm, N_, d_ + 1, vtx_.get(), srcL, csrc, srcU, dsrc,
                                              em, lattice_alpha(d_), out);
   };
-  if (d_ == 2)
+  auto slice4 = [&](auto kern) {
+    kern<<<rows_grid(N_, nqc), kBlock, 0, s>>>(bins, L, LP, row, LC, rq, N_, vtx_.get(), srcL, csrc, srcU, dsrc, em,
+                                               lattice_alpha(d_), out);
+  };
+  if (expand)
+    k_oslice<5, float><<<rows_grid(N_, L), kBlock, 0, s>>>(bins, L, LP, row, rm, N_, d_ + 1, vtx_.get(), srcL, srcU,
+                                                           dsrc, em, lattice_alpha(d_), out);
+  else if (quad)
+    slice4(k_cslice4<2>);
+  else if (d_ == 2)
     slice(k_cslice<2>);
-  else if (d_ == 5)
-    slice(k_cslice<5>);
   else
     slice(k_cslice<0>);
   check_launch("k_cslice");
