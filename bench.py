@@ -46,7 +46,7 @@ def algorithmic_bytes(name: str, N: int, M: dict, V: int = L, S: int = 4, prof: 
     d = int(tag.split("d=")[1].split(",")[0])
     m = M[d]
     e = N * (d + 1)
-    if kind in ("csplat", "cblur", "cslice"):
+    if kind in ("csplat", "cblur", "cblur_last", "cslice"):
         if not prof or name not in prof:
             return float("nan")
         LC = (V + 3) // 4 * 4
@@ -58,8 +58,10 @@ def algorithmic_bytes(name: str, N: int, M: dict, V: int = L, S: int = 4, prof: 
         row = 9 * S  # one label's nine bins
         if kind == "csplat":
             tot = calls * (N * V + 8 * e + m * LC) + row * rows[0]
-        elif kind == "cblur":
-            tot = calls * (d + 1) * (2 * m * LC + 12 * m) + row * sum(rows[j] + rows[j + 1] for j in range(d + 1))
+        elif kind == "cblur":  # axes 0..d-1
+            tot = calls * d * (2 * m * LC + 12 * m) + row * sum(rows[j] + rows[j + 1] for j in range(d))
+        elif kind == "cblur_last":  # axis d (at d = 5 it writes every row out dense)
+            tot = calls * (2 * m * LC + 12 * m) + row * (rows[d] + rows[d + 1])
         else:
             tot = calls * (N * V + 8 * e + S * m + m * LC + 2 * S * N * V) + row * rows[d + 1]
         return tot / launches
@@ -495,8 +497,8 @@ def main():
     # the sum of its kernels' algorithmic bytes over the sum of their times
     ofb = oft = 0.0
     for n, (c_, ms_) in lat_k.items():
-        if n.split("[")[0] in ("obin", "osplat", "oblur", "oslice", "oedge", "csplat", "cblur", "cslice",
-                               "osplat_up"):
+        if n.split("[")[0] in ("obin", "osplat", "oblur", "oslice", "oedge", "csplat", "cblur", "cblur_last",
+                               "cslice", "osplat_up"):
             b_ = algorithmic_bytes(n, wl.N, Ms, prof=prof)
             ofb += (b_ if math.isfinite(b_) else 0.0) * c_
             oft += ms_ * 1e-3
@@ -533,7 +535,7 @@ def main():
         "ordered_filter": None if ordered_gbs is None else {
             "ms_per_solve": round(oft * 1e3, 3), "achieved": round(ordered_gbs, 1), "unit": "GB/s",
             "frac": round(ordered_gbs / PEAK_HBM_GBS, 4),
-            "kernels": "obin, oedge, csplat, osplat_up, cblur, cslice (both terms, every LP step)"},
+            "kernels": "obin, oedge, csplat, osplat_up, cblur, cblur_last, oslice / cslice (both terms, every LP step)"},
         "images_per_s": None,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),

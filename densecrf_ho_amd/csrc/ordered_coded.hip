@@ -197,47 +197,91 @@ __global__ void __launch_bounds__(kBlock) k_ccombine(int L, int RS, int LC, RowM
   st9(values + r * RS + (size_t)k * kOBins, acc, false);
 }
 
-// One blur axis.  Lower rows: one lane per (row, label) (floor(64 / L) rows
-// per wave); the output code from the three rows' codes, and for a mixed
-// label its nine values from the rows that store them or, for a uniform row,
-// from its total weight at this stage (tb_in).  Then, as k_blur2, the
-// correction rows (some part in correction mode) and the upper rows of the
-// points of parts in full mode (dense quads, half_q quads past the lower rows).
-__global__ void __launch_bounds__(kBlock) k_cblur(int L, int RS, int LC, RowMap rm, uint32_t M,
-                                                  const int* __restrict__ nbr, const float* __restrict__ src,
-                                                  float* __restrict__ dst, const uint8_t* __restrict__ csrc,
-                                                  uint8_t* __restrict__ cdst, const float* __restrict__ tb_in,
-                                                  const float* __restrict__ dsrc, float* __restrict__ ddst,
-                                                  int R4, FastDiv divR4, uint32_t nq, size_t half_q, EdgeMap edge,
-                                                  int nt) {
+// One blur axis, one lane per (row, quad of labels): the output codes of four
+// labels from one 4-byte code word of each of the three rows; the wave's
+// mixed (row, label) pairs are then listed in LDS and worked one per lane (a
+// wave covers floor(64 / quads) rows, and a pair's loads cost the same
+// whichever lane works it, so the mixed pairs of ten rows share a few load
+// instructions instead of each row's lanes issuing their own).  Then, as
+// k_blur2, the correction rows and the upper rows of parts in full mode.
+constexpr int kWcItems = 256;  // listed pairs per wave: at most four per lane
+__global__ void __launch_bounds__(kBlock) k_cblur_wc(int L, int RS, int LC, RowMap rm, uint32_t M,
+                                                     const int* __restrict__ nbr, const float* __restrict__ src,
+                                                     float* __restrict__ dst, const uint8_t* __restrict__ csrc,
+                                                     uint8_t* __restrict__ cdst, const float* __restrict__ tb_in,
+                                                     const float* __restrict__ dsrc,
+                                                     float* __restrict__ ddst, int R4, FastDiv divR4, uint32_t nq,
+                                                     size_t half_q, EdgeMap edge, int nt) {
+  __shared__ int4 items[kBlock / 64][kWcItems];
+  const int wv = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  const int nqc = (L + 3) >> 2;
+  long i;
+  int q, qs;
+  rm.at(nqc, i, q, qs);
+  const bool on = i >= 0 && i < (long)M;
+  const int r = (int)i + 1;
+  int n1 = 0, n2 = 0;
+  uint32_t co = 0, ca = 0, cb = 0;
+  if (on) {
+    n1 = nbr[2 * i] + 1;
+    n2 = nbr[2 * i + 1] + 1;
+    co = *reinterpret_cast<const uint32_t*>(csrc + (size_t)r * LC + 4 * q);
+    ca = *reinterpret_cast<const uint32_t*>(csrc + (size_t)n1 * LC + 4 * q);
+    cb = *reinterpret_cast<const uint32_t*>(csrc + (size_t)n2 * LC + 4 * q);
+  }
+  int code[4];
+  uint32_t out = 0, mixed = 0;
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+    code[kk] = code3((int)((co >> (8 * kk)) & 255u), (int)((ca >> (8 * kk)) & 255u), (int)((cb >> (8 * kk)) & 255u));
+    if (4 * q + kk >= L) code[kk] = kCodeZero;
+    out |= (uint32_t)code[kk] << (8 * kk);
+    if (on && code[kk] == kCodeRow) mixed |= 1u << kk;
+  }
+  if (on) *reinterpret_cast<uint32_t*>(cdst + (size_t)r * LC + 4 * q) = out;
+  // the wave's list: this lane's pairs at its exclusive prefix
+  const int n = __popc(mixed);
+  int inc = n;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += v;
+  }
+  const int total = __shfl(inc, 63);
+  const int first = inc - n;
   {
-    long i;
-    int k, ks;
-    rm.at(L, i, k, ks);
-    if (i >= 0 && i < (long)M) {
-      const int n1 = nbr[2 * i] + 1, n2 = nbr[2 * i + 1] + 1;
-      const size_t r = (size_t)i + 1;
-      const int co = csrc[r * LC + k], ca = csrc[(size_t)n1 * LC + k], cb = csrc[(size_t)n2 * LC + k];
-      const int code = code3(co, ca, cb);
-      if (code == kCodeRow) {
-        float o[kOBins], a[kOBins], bb[kOBins];
-        auto row = [&](size_t rr, int cc, float (&v)[kOBins]) {
-          if (cc == kCodeRow) {
-            ld9(src + rr * RS + (size_t)k * kOBins, v);
-          } else {
-            const float t = tb_in[rr];
+    int at = first;
 #pragma unroll
-            for (int x = 0; x < kOBins; ++x) v[x] = coded_slot(cc, x, t);
-          }
-        };
-        row(r, co, o);
-        row((size_t)n1, ca, a);
-        row((size_t)n2, cb, bb);
-#pragma unroll
-        for (int x = 0; x < kOBins; ++x) o[x] = o[x] + 0.5f * (a[x] + bb[x]);
-        st9(dst + r * RS + (size_t)k * kOBins, o, nt != 0);
+    for (int kk = 0; kk < 4; ++kk)
+      if (mixed & (1u << kk)) {
+        const uint32_t cc = ((co >> (8 * kk)) & 255u) | (((ca >> (8 * kk)) & 255u) << 8) |
+                            (((cb >> (8 * kk)) & 255u) << 16);
+        items[wv][at++] = make_int4(r, n1, n2, (int)((uint32_t)(4 * q + kk) | (cc << 8)));
       }
-      cdst[r * LC + k] = (uint8_t)code;
+  }
+  __syncthreads();  // the lists are written
+  for (int base = 0; base < total; base += 64) {
+    const int t = base + lane;
+    if (t < total) {
+      const int4 it = items[wv][t];
+      const int k = it.w & 255;
+      const int c0 = (it.w >> 8) & 255, c1 = (it.w >> 16) & 255, c2 = (int)((uint32_t)it.w >> 24);
+      float o[kOBins], a[kOBins], bb[kOBins];
+      auto row = [&](int rr, int cc, float (&v)[kOBins]) {
+        if (cc == kCodeRow) {
+          ld9(src + (size_t)rr * RS + (size_t)k * kOBins, v);
+        } else {
+          const float tt = tb_in[rr];
+#pragma unroll
+          for (int x = 0; x < kOBins; ++x) v[x] = coded_slot(cc, x, tt);
+        }
+      };
+      row(it.x, c0, o);
+      row(it.y, c1, a);
+      row(it.z, c2, bb);
+#pragma unroll
+      for (int x = 0; x < kOBins; ++x) o[x] = o[x] + 0.5f * (a[x] + bb[x]);
+      st9(dst + (size_t)it.x * RS + (size_t)k * kOBins, o, nt != 0);
     }
   }
   const uint32_t stride = gridDim.x * kBlock, t0 = xcd_block() * kBlock + threadIdx.x;
@@ -245,8 +289,8 @@ __global__ void __launch_bounds__(kBlock) k_cblur(int L, int RS, int LC, RowMap 
     for (uint32_t t = t0; t < 2 * M; t += stride) blur_quad(nbr, dsrc, ddst, 0, t >> 1, 2u, t & 1u, false);
   if (edge.any_full())  // upper rows of the points of parts in full mode
     for (uint32_t t = t0; t < nq; t += stride) {
-      const uint32_t i = divR4.div(t);
-      if (edge.point_full(i)) blur_quad(nbr, src, dst, half_q, i, (uint32_t)R4, t - i * (uint32_t)R4, nt);
+      const uint32_t pt = divR4.div(t);
+      if (edge.point_full(pt)) blur_quad(nbr, src, dst, half_q, pt, (uint32_t)R4, t - pt * (uint32_t)R4, nt);
     }
 }
 
@@ -670,9 +714,9 @@ void Lattice::ordered_coded(const float* Q, const float* d_minmax, int L, const 
                                                    orow_.get(), srcU, part_.get(), em);
     };
     if (expand)
-      up(k_osplat<105, float>);
+      up(k_osplat<100, float>);  // bin-major rows
     else
-      up(k_osplat<102, float>);
+      up(k_osplat<102, float>);  // label-major rows
     if (nmulti_ > 0) {
       const int pv = kOBins * L;
       const RowMap rw = rowmap(pv);
@@ -689,15 +733,15 @@ void Lattice::ordered_coded(const float* Q, const float* d_minmax, int L, const 
   for (int j = 0; j <= d_; ++j) {  // forward order only (:621, :669)
     const int* nb = onbr_.get() + (size_t)j * M_ * 2;
     {
-      KTimer tm("cblur" + tag, s);
+      KTimer tm((j == d_ ? "cblur_last" : "cblur") + tag, s);
       if (expand && j == d_)
         k_cblur_expand<<<rows_grid(M_, L), kBlock, 0, s>>>(
             L, row, LC, rm, (uint32_t)M_, nb, srcL, dstL, csrc, tbs_.get() + (size_t)j * (M_ + 1),
             tbs_.get() + (size_t)(j + 1) * (M_ + 1), dsrc, ddst, R4, divR4, (uint32_t)nquads, half_q, emr);
       else
-        k_cblur<<<rows_grid(M_, L), kBlock, 0, s>>>(L, row, LC, rm, (uint32_t)M_, nb, srcL, dstL, csrc, cdst,
-                                                     tbs_.get() + (size_t)j * (M_ + 1), dsrc, ddst, R4, divR4,
-                                                     (uint32_t)nquads, half_q, emr, j == d_);
+        k_cblur_wc<<<rows_grid(M_, nqc), kBlock, 0, s>>>(L, row, LC, rq, (uint32_t)M_, nb, srcL, dstL, csrc, cdst,
+                                                         tbs_.get() + (size_t)j * (M_ + 1), dsrc, ddst, R4, divR4,
+                                                         (uint32_t)nquads, half_q, emr, j == d_);
     }
     check_launch("k_cblur");
     std::swap(srcL, dstL);
