@@ -504,7 +504,8 @@ std::vector<std::vector<int>> Batch::lp(const float* init, float* out, const LPP
   std::vector<std::vector<int>> sweeps(B);
   float* gcur = gamma_.get();
   float* gnext = gamma2_.get();
-  int prev_sweeps = kSweepBatchB;
+  std::vector<SweepPlan> plans(B);
+  std::vector<double> htot((size_t)B * kSweepBatchB);
   for (int it = 0; it < P.prox_max_iter; ++it) {
     k_fill<<<grid_for(NL), kBlock, 0, s>>>(gcur, NL, 1.f);
     DCRF_HIP(hipMemcpyAsync(curQ_.get(), lQ_.get(), NL * sizeof(float), hipMemcpyDeviceToDevice, s));
@@ -519,9 +520,14 @@ std::vector<std::vector<int>> Batch::lp(const float* init, float* out, const LPP
       // qp-gamma sweeps in batches for all images; images that stopped keep
       // their gamma while the others go on
       k_lp_sweep_reset<<<1, 256, 0, s>>>(fl, B);
+      for (auto& pl : plans) pl.tot.clear();
+      std::vector<char> running(B, 1);
       while (true) {
-        const int nsw = std::min(pit == 1 ? kSweepBatchB : std::max(2, std::min(kSweepBatchB, prev_sweeps)),
-                                 P.qp_max_iter);
+        // the longest planned batch of the images still running
+        int want = 1;
+        for (int b = 0; b < B; ++b)
+          if (running[b]) want = std::max(want, plans[b].next(P.qp_tol));
+        const int nsw = std::min(want, P.qp_max_iter);
         const int parity = gcur == gamma_.get() ? 0 : 1;
         float* gi = gcur;
         float* go = gnext;
@@ -546,17 +552,17 @@ std::vector<std::vector<int>> Batch::lp(const float* init, float* out, const LPP
           }
         });
         std::swap(gcur, gnext);
+        DCRF_HIP(hipMemcpyAsync(htot.data(), swtot_.get(), htot.size() * sizeof(double), hipMemcpyDeviceToHost, s));
         read_flags();
         bool all = true;
-        for (int b = 0; b < B; ++b) all = all && hfl[(size_t)b * kLF_NFL + kLF_SWEEP_DONE] != 0;
+        for (int b = 0; b < B; ++b) {
+          if (running[b]) plans[b].add(htot.data() + (size_t)b * kSweepBatchB, nsw);
+          running[b] = hfl[(size_t)b * kLF_NFL + kLF_SWEEP_DONE] == 0;
+          all = all && !running[b];
+        }
         if (all) break;
       }
-      prev_sweeps = 0;
-      for (int b = 0; b < B; ++b) {
-        const int t = hfl[(size_t)b * kLF_NFL + kLF_TOTAL];
-        sweeps[b].push_back(t);
-        prev_sweeps = std::max(prev_sweeps, t);
-      }
+      for (int b = 0; b < B; ++b) sweeps[b].push_back(hfl[(size_t)b * kLF_NFL + kLF_TOTAL]);
       const int parity = gcur == gamma_.get() ? 0 : 1;
       float* gm = gcur;
       run_graph(kGraphPost + parity, [&]() {
