@@ -71,6 +71,7 @@ __device__ inline void st9(float* p, const float (&v)[9], bool nt) {
 // gives the code.  A one-chunk point writes its codes and its mixed labels'
 // rows; a chunk of a point with several writes its codes, its mixed labels'
 // partial rows and (quad 0) its total weight, for k_ccombine.
+template <int TAG>  // the lattice dimension (2, 5, or 0 for others): kernel names in profiles
 __global__ void __launch_bounds__(kBlock) k_csplat(const uint8_t* __restrict__ bins, int L, int LP, int RS, int LC,
                                                    long nlanes, const int* __restrict__ sp_pix,
                                                    const float* __restrict__ sp_w, const int* __restrict__ seg,
@@ -205,6 +206,7 @@ __global__ void __launch_bounds__(kBlock) k_ccombine(int L, int RS, int LC, RowM
 // instructions instead of each row's lanes issuing their own).  Then, as
 // k_blur2, the correction rows and the upper rows of parts in full mode.
 constexpr int kWcItems = 256;  // listed pairs per wave: at most four per lane
+template <int TAG>  // the lattice dimension (2, 5, or 0 for others): kernel names in profiles
 __global__ void __launch_bounds__(kBlock) k_cblur_wc(int L, int RS, int LC, RowMap rm, uint32_t M,
                                                      const int* __restrict__ nbr, const float* __restrict__ src,
                                                      float* __restrict__ dst, const uint8_t* __restrict__ csrc,
@@ -684,9 +686,17 @@ void Lattice::ordered_coded(const float* Q, const float* d_minmax, int L, const 
   {
     KTimer tm("csplat" + tag, s);
     auto launch = [&](unsigned grid) {
-      k_csplat<<<grid, kBlock, 0, s>>>(bins, L, LP, row, LC, nlanes, sp_pix_.get(), sp_w_.get(), seg_.get(),
-                                       chstart_.get(), chpoint_.get(), pstart_.get(), orow_.get(), srcL, csrc,
-                                       part_.get(), pcode_.get(), ptb_.get());
+      auto go = [&](auto kern) {
+        kern<<<grid, kBlock, 0, s>>>(bins, L, LP, row, LC, nlanes, sp_pix_.get(), sp_w_.get(), seg_.get(),
+                                     chstart_.get(), chpoint_.get(), pstart_.get(), orow_.get(), srcL, csrc,
+                                     part_.get(), pcode_.get(), ptb_.get());
+      };
+      if (d_ == 5)
+        go(k_csplat<5>);
+      else if (d_ == 2)
+        go(k_csplat<2>);
+      else
+        go(k_csplat<0>);
     };
     const unsigned g = Tuner::get().pick(d_ == 5 ? "csplat5" : "csplat", nvb, grid_candidates(nvb), s, launch);
     launch(g);
@@ -739,9 +749,11 @@ void Lattice::ordered_coded(const float* Q, const float* d_minmax, int L, const 
             L, row, LC, rm, (uint32_t)M_, nb, srcL, dstL, csrc, tbs_.get() + (size_t)j * (M_ + 1),
             tbs_.get() + (size_t)(j + 1) * (M_ + 1), dsrc, ddst, R4, divR4, (uint32_t)nquads, half_q, emr);
       else
-        k_cblur_wc<<<rows_grid(M_, nqc), kBlock, 0, s>>>(L, row, LC, rq, (uint32_t)M_, nb, srcL, dstL, csrc, cdst,
-                                                         tbs_.get() + (size_t)j * (M_ + 1), dsrc, ddst, R4, divR4,
-                                                         (uint32_t)nquads, half_q, emr, j == d_);
+        [&](auto kern) {
+          kern<<<rows_grid(M_, nqc), kBlock, 0, s>>>(L, row, LC, rq, (uint32_t)M_, nb, srcL, dstL, csrc, cdst,
+                                                     tbs_.get() + (size_t)j * (M_ + 1), dsrc, ddst, R4, divR4,
+                                                     (uint32_t)nquads, half_q, emr, j == d_);
+        }(d_ == 5 ? k_cblur_wc<5> : d_ == 2 ? k_cblur_wc<2> : k_cblur_wc<0>);
     }
     check_launch("k_cblur");
     std::swap(srcL, dstL);
