@@ -260,6 +260,67 @@ def adjoint_bench(D, S, steps=3):
             "forward_adjoint_ms": round(tb * 1e3, 3), "ratio": round(tb / tf, 3)}
 
 
+def configs_bench(D, S, reps=3):
+    """GPU time of the other configurations' solves (the median of `reps`
+    after one warm-up; inputs resident in HBM, lattice builds untimed):
+    config 0, the QP (10 iterations) at 320x213 with 40 cells; config 1, the QP
+    then the LP 5x10 at 640x480 with 120 cells; config 2, qp_sp (the
+    reference's stopping rule) then lp_sp 5x10 at 1280x720 with 230 cells and
+    300/600/1200-cell superpixel layers (weight 0.5, truncation 0.2)."""
+    P = D.LP_inf_params(prox_max_iter=LP_PROX, fw_max_iter=LP_FW)
+
+    def crf(wl, layers=()):
+        c = D.DenseCRF2D(wl.W, wl.H, L)
+        c.setUnaryEnergy(D.DeviceArray.from_numpy(wl.U))
+        for f, w in wl.kernels:
+            c.addPairwiseEnergy(D.DeviceArray.from_numpy(f), D.PottsCompatibility(w))
+        for region in layers:
+            c.addSuperPixelLayer(region, 0.5, 0.2)
+        return c
+
+    def med(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            D.synchronize()
+            t = time.perf_counter()
+            fn()
+            D.synchronize()
+            ts.append((time.perf_counter() - t) * 1e3)
+        return round(float(np.median(ts)), 3)
+
+    out = {}
+    wl = S.Workload(320, 213, 40, 0, L=L)
+    c = crf(wl)
+    q0 = D.DeviceArray.from_numpy(S.softmax_init(wl.U))
+    q1 = D.DeviceArray((wl.N, L))
+    out["c0_qp_320x213_ms"] = med(lambda: c.qp_inference_super_pixels_non_convex(q0, max_iter=QP_ITERS,
+                                                                                 ref_stop=False, out=q1))
+    c.close()
+    wl = S.Workload(640, 480, 120, 1, L=L)
+    c = crf(wl)
+    q0 = D.DeviceArray.from_numpy(S.softmax_init(wl.U))
+    q1, q2 = D.DeviceArray((wl.N, L)), D.DeviceArray((wl.N, L))
+
+    def c1():
+        c.qp_inference_super_pixels_non_convex(q0, max_iter=QP_ITERS, ref_stop=False, out=q1)
+        c.lp_inference_prox_super_pixels(q1, P, out=q2)
+    out["c1_qp_lp_640x480_ms"] = med(c1)
+    c.close()
+    wl = S.Workload(1280, 720, 230, 2, L=L)
+    layers = [r for r, _ in S.superpixel_layers(1280, 720, cells=(300, 600, 1200), seeds=(21, 22, 23))]
+    c = crf(wl, layers)
+    q0 = D.DeviceArray.from_numpy(S.softmax_init(wl.U))
+    q1, q2 = D.DeviceArray((wl.N, L)), D.DeviceArray((wl.N, L))
+
+    def c2():
+        c.qp_inference_super_pixels_non_convex(q0, out=q1)
+        c.lp_inference_prox_super_pixels(q1, P, out=q2)
+    out["c2_qpsp_lpsp_1280x720_ms"] = med(c2)
+    c.close()
+    return out
+
+
 BATCH_IMAGES = 64
 
 
@@ -309,17 +370,14 @@ def batch_bench(D, imgs):
     e_batch = b.assignment_energy_true(lab)
     qp_batch = Qqp.numpy()
     lp_batch = Qlp.numpy()
-    # the same with the ordered filter's lattice rows in bfloat16
+    # the ordered filter's lattice rows in bfloat16 (opt-in): its error only;
+    # it fails the accuracy bar, so its time is no result
     b.set_value_type(True)
     solve()
-    solve()
-    t0 = time.perf_counter()
-    solve()
-    t_bf16 = time.perf_counter() - t0
     lab16 = np.ascontiguousarray(Qlp.numpy().argmax(1).astype(np.int32))
     e_bf16 = b.assignment_energy_true(lab16)
     rel16 = np.abs(e_bf16 - e_batch) / np.abs(e_batch)
-    _progress(f"batched solve with bfloat16 rows {t_bf16:.3f} s")
+    _progress("batched solve with bfloat16 rows done")
     b.close()
     del Qqp, Qlp
     _progress(f"batched solve {t_batch:.3f} s")
@@ -356,11 +414,10 @@ def batch_bench(D, imgs):
             "batched_s": round(t_batch, 4), "images_per_s": round(len(imgs) / t_batch, 3),
             "one_image_path_s": round(t_single, 4), "one_image_path_images_per_s": round(len(imgs) / t_single, 3),
             "speedup": round(t_single / t_batch, 3),
-            "bf16_rows": {"batched_s": round(t_bf16, 4), "images_per_s": round(len(imgs) / t_bf16, 3),
-                          "speedup_vs_one_image_path": round(t_single / t_bf16, 3),
-                          "lp_energy_rel_diff_vs_float_rows": {"max": float(rel16.max()),
-                                                               "median": float(np.median(rel16))},
-                          "label_agreement_vs_float_rows": float((lab16 == lab).mean())},
+            "bf16_rows_error": {"status": "fails the 1e-4 energy / 99.9% label bar; opt-in only, not timed",
+                                "lp_energy_rel_diff_vs_float_rows": {"max": float(rel16.max()),
+                                                                     "median": float(np.median(rel16))},
+                                "label_agreement_vs_float_rows": float((lab16 == lab).mean())},
             "qp_max_abs_diff_vs_one_image": float(max(qp_diff)),
             "lp_energy_rel_diff_vs_one_image": {"max": float(rel.max()), "median": float(np.median(rel))},
             "lp_label_agreement_vs_one_image": {"min": float(min(lab_agree)), "median": float(np.median(lab_agree))},
@@ -550,6 +607,7 @@ def main():
     }
     if not args.qp_only:
         out["adjoint"] = adjoint_bench(D, S)
+        out["configs"] = configs_bench(D, S)
     if imgs is not None:
         del crf
         out["batch"] = batch_bench(D, imgs)
