@@ -150,11 +150,11 @@ def oracle_parity():
 
 
 # The compiled reference (oracle/_ref, built from the reference's own sources)
-# against this oracle, one thread each on the development container's Intel
-# Xeon @ 2.70 GHz, on the 640x480 configuration (results bit-identical):
-# QP with the reference's stopping rule 1.39 s vs 2.12 s, LP 5x10 107.4 s vs
-# 94.4 s.  Ratios reference / oracle:
-REF_OVER_PORT = {"qp": 0.66, "lp": 1.14}
+# against this oracle, one thread each, on the 640x480 configuration (QP with
+# the reference's stopping rule, then LP 5x10; results bit-identical), timed
+# by oracle/time_configs.py on the development container's Intel Xeon: QP
+# 1.519 s vs 2.477 s, LP 96.0 s vs 108.4 s.  Ratios reference / oracle:
+REF_OVER_PORT = {"qp": 0.61, "lp": 0.89}
 
 
 def cpu_model():
