@@ -115,6 +115,51 @@ inline unsigned rg_grid(long nrows, int L, long cap = 1 << 30) {
   return (unsigned)std::max(1L, std::min(t, cap));
 }
 
+// Row quads: the same tiling with one lane per (row, quad of values): a wave
+// covers floor(64 / ceil(L/4)) rows (10 at L = 21) and each lane moves four
+// values with one 16-byte access (rows of L floats are 4-byte aligned; the last
+// quad of a row, when L % 4 != 0, moves its values one by one).  Per-row
+// reductions go through a wave-private LDS segment of at most 256 floats.
+struct RQ {
+  int g, q;
+  long row;
+  bool valid;
+};
+__device__ inline RQ rq_at(int L, long nrows, long tile) {
+  const int nq = (L + 3) >> 2, gpw = 64 / nq;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  RQ r;
+  r.g = lane / nq;
+  r.q = lane - r.g * nq;
+  r.row = (tile * (kBlock / 64) + wave) * gpw + r.g;
+  r.valid = r.g < gpw && r.row < nrows;
+  return r;
+}
+__host__ __device__ inline long rq_tiles(long nrows, int L) {
+  const long rows_per_tile = (long)(64 / ((L + 3) / 4)) * (kBlock / 64);
+  return (nrows + rows_per_tile - 1) / rows_per_tile;
+}
+inline unsigned rq_grid(long nrows, int L, long cap = 1 << 30) {
+  const long t = rq_tiles(nrows, L);
+  return (unsigned)std::max(1L, std::min(t, cap));
+}
+// four consecutive values of a row starting at value k0 (n of them valid)
+__device__ inline void ld_q4(const float* p, int n, float (&v)[4]) {
+  if (n == 4) {
+    const Float4u u = *reinterpret_cast<const Float4u*>(p);
+    v[0] = u.x, v[1] = u.y, v[2] = u.z, v[3] = u.w;
+  } else {
+    for (int i = 0; i < 4; ++i) v[i] = i < n ? p[i] : 0.f;
+  }
+}
+__device__ inline void st_q4(float* p, int n, const float (&v)[4]) {
+  if (n == 4) {
+    *reinterpret_cast<Float4u*>(p) = Float4u{v[0], v[1], v[2], v[3]};
+  } else {
+    for (int i = 0; i < n; ++i) p[i] = v[i];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Work split of a launch over images.  OneImage spreads every block over all
 // N pixels.  ImageBlocks (a batch) gives each image its own contiguous range

@@ -1078,12 +1078,25 @@ __global__ void __launch_bounds__(kBlock) k_obin(const float* __restrict__ Q, co
   for (uint32_t g = blockIdx.x * kBlock + threadIdx.x; g < n; g += gridDim.x * kBlock) {
     const uint32_t pix = divNQ.div(g), q = g - pix * nq;
     const int img = pimg ? pimg[pix] : 0;
+    const int k0 = 4 * (int)q;
+    // the quad's four values in one 16-byte load (4-byte aligned rows of L floats)
+    float4 v4;
+    if (k0 + 4 <= L) {
+      const Float4u u = *reinterpret_cast<const Float4u*>(Q + (size_t)pix * L + k0);
+      v4 = make_float4(u.x, u.y, u.z, u.w);
+    } else {
+      const float* r = Q + (size_t)pix * L + k0;
+      v4 = make_float4(r[0], k0 + 1 < L ? r[1] : 0.f, k0 + 2 < L ? r[2] : 0.f, 0.f);
+    }
+    const float vq[4] = {v4.x, v4.y, v4.z, v4.w};
+    const float m0 = mm[2 * img], m1 = mm[2 * img + 1];
     uint32_t word = 0;
 #pragma unroll
     for (int kk = 0; kk < 4; ++kk) {
-      const int k = 4 * (int)q + kk;
+      const int k = k0 + kk;
       if (k >= L) break;
-      const float qv = rescaled(Q, (size_t)pix * L + k, mm + 2 * img);
+      float qv = vq[kk] - m0;  // rescaled() on the loaded value
+      if (m1 > 0.f) qv = qv / m1;
       const int lo = bin_lo(qv), up = bin_up(qv);
       if (lo != up) {  // on a bin edge: flag its part and its combo, and list it
         int* P = edge + (size_t)img * kP_STRIDE;

@@ -105,35 +105,50 @@ __global__ void __launch_bounds__(kBlock) k_lp_Y(SEG sg, int L, LPConst c, float
                        const float* __restrict__ sa, const float* __restrict__ su, const double* __restrict__ sc,
                        int apply_update, const float* __restrict__ U, const float* __restrict__ curQ,
                        float* __restrict__ Y) {
-  __shared__ float seg[kBlock];
+  __shared__ float seg[kBlock / 64][256];
   const SegPos qs = sg.pos();
   const float dl = apply_update ? (float)sc[(size_t)qs.img * kL_NSC + kL_DELTA] : 0.f;
-  const long ntiles = rg_tiles(qs.n, L);
+  const long ntiles = rq_tiles(qs.n, L);
   for (long tile = qs.lb; tile < ntiles; tile += qs.nb) {
-    RG r = rg_at(L, qs.n, tile);
+    RQ r = rq_at(L, qs.n, tile);
     r.row += qs.p0;
-    float* row = seg + (threadIdx.x & ~63) + r.g * L;
-    const size_t idx = (size_t)r.row * L + r.k;
-    float t = 0.f, cq = 0.f;
+    float* row = seg[threadIdx.x >> 6] + r.g * L;
+    const int k0 = 4 * r.q, nk = min(4, L - k0);
+    const size_t idx = (size_t)r.row * L + k0;
+    float t[4] = {0.f, 0.f, 0.f, 0.f}, cq[4];
     if (r.valid) {
-      cq = curQ[idx];  // loaded with the other inputs, used after the row sum
-      float a = alpha[idx];
-      float uu = u ? u[idx] : 0.f;
+      float a[4], uu[4] = {0.f, 0.f, 0.f, 0.f}, uv[4];
+      ld_q4(curQ + idx, nk, cq);  // loaded with the other inputs, used after the row sum
+      ld_q4(alpha + idx, nk, a);
+      if (u) ld_q4(u + idx, nk, uu);
+      ld_q4(U + idx, nk, uv);
       if (apply_update) {
-        a = a + dl * (sa[idx] - a);
-        alpha[idx] = a;
+        float sv[4];
+        ld_q4(sa + idx, nk, sv);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = a[i] + dl * (sv[i] - a[i]);
+        st_q4(alpha + idx, nk, a);
         if (u) {
-          uu = uu + dl * (su[idx] - uu);
-          u[idx] = uu;
+          float sw[4];
+          ld_q4(su + idx, nk, sw);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) uu[i] = uu[i] + dl * (sw[i] - uu[i]);
+          st_q4(u + idx, nk, uu);
         }
       }
-      t = (a + uu) - U[idx];
-      row[r.k] = t;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        t[i] = (a[i] + uu[i]) - uv[i];
+        if (i < nk) row[k0 + i] = t[i];
+      }
     }
     __syncthreads();
     if (r.valid) {
       const float S = eigen_sum(L, 0, [&](int i) { return row[i]; });
-      Y[idx] = ((c.c_m1_over_M * S + t) * c.lambda) + cq;
+      float y[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) y[i] = ((c.c_m1_over_M * S + t[i]) * c.lambda) + cq[i];
+      st_q4(Y + idx, nk, y);
     }
     __syncthreads();
   }
@@ -342,34 +357,43 @@ template <class SEG>
 __global__ void __launch_bounds__(kBlock) k_lp_betaQ(SEG sg, int L, float lambda, const float* __restrict__ alpha, const float* __restrict__ u,
                            const float* __restrict__ gamma, const float* __restrict__ U,
                            const float* __restrict__ curQ, float* __restrict__ Q, float* __restrict__ mmparts) {
-  __shared__ float seg[kBlock];
+  __shared__ float seg[kBlock / 64][256];
   __shared__ float smn[kBlock], smx[kBlock];
   float mn = INFINITY, mx = -INFINITY;
   const SegPos qs = sg.pos();
-  const long ntiles = rg_tiles(qs.n, L);
+  const long ntiles = rq_tiles(qs.n, L);
   for (long tile = qs.lb; tile < ntiles; tile += qs.nb) {
-    RG r = rg_at(L, qs.n, tile);
+    RQ r = rq_at(L, qs.n, tile);
     r.row += qs.p0;
-    float* row = seg + (threadIdx.x & ~63) + r.g * L;
-    const size_t idx = (size_t)r.row * L + r.k;
-    float a = 0.f, uu = 0.f, gm = 0.f, uv = 0.f, cq = 0.f;
+    float* row = seg[threadIdx.x >> 6] + r.g * L;
+    const int k0 = 4 * r.q, nk = min(4, L - k0);
+    const size_t idx = (size_t)r.row * L + k0;
+    float a[4], uu[4] = {0.f, 0.f, 0.f, 0.f}, gm[4], uv[4], cq[4];
     if (r.valid) {
-      cq = curQ[idx];  // loaded with the other inputs, used after the row sum
-      a = alpha[idx];
-      uu = u ? u[idx] : 0.f;
-      gm = gamma[idx];
-      uv = U[idx];
-      row[r.k] = ((a + uu) + gm) - uv;
+      ld_q4(curQ + idx, nk, cq);  // loaded with the other inputs, used after the row sum
+      ld_q4(alpha + idx, nk, a);
+      if (u) ld_q4(u + idx, nk, uu);
+      ld_q4(gamma + idx, nk, gm);
+      ld_q4(U + idx, nk, uv);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (i < nk) row[k0 + i] = ((a[i] + uu[i]) + gm[i]) - uv[i];
     }
     __syncthreads();
     if (r.valid) {
       // alignment of the column within its own image's matrix
       float b = -eigen_sum(L, col_astart(r.row - qs.p0, L), [&](int i) { return row[i]; });
       b = b / (float)L;
-      const float q = lambda * ((((a + b) + uu) + gm) - uv) + cq;
-      Q[idx] = q;
-      mn = nmin(mn, q);
-      mx = nmax(mx, q);
+      float qv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        qv[i] = lambda * ((((a[i] + b) + uu[i]) + gm[i]) - uv[i]) + cq[i];
+        if (i < nk) {
+          mn = nmin(mn, qv[i]);
+          mx = nmax(mx, qv[i]);
+        }
+      }
+      st_q4(Q + idx, nk, qv);
     }
     __syncthreads();
   }
@@ -917,9 +941,9 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
         };
         // tuned on the first step (which only writes Y); later steps also
         // update alpha in place and reuse the choice
-        const unsigned gy = pit == 1 ? Tuner::get().pick("lp_Y", N, grid_candidates(rg_grid(N, L)), s,
+        const unsigned gy = pit == 1 ? Tuner::get().pick("lp_Y", N, grid_candidates(rq_grid(N, L)), s,
                                                           [&](unsigned g) { ylaunch(g, 0); })
-                                     : Tuner::get().cached("lp_Y", N, rg_grid(N, L, 8192));
+                                     : Tuner::get().cached("lp_Y", N, rq_grid(N, L, 8192));
         ylaunch(gy, pit > 1 ? 1 : 0);
       }
       // qp-gamma sweeps in batches: pass 1 runs `nsw` sweeps gamma -> gamma2,
@@ -970,7 +994,7 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
           k_lp_betaQ<<<g, kBlock, 0, s>>>(OneImage{N}, L, lambda, st.alpha.get(), u, st.gamma.get(), U_.get(),
                                           st.curQ.get(), st.Q.get(), st.mmparts.get());
         };
-        const unsigned nbq = Tuner::get().pick("lp_betaQ", N, grid_candidates(rg_grid(N, L, 16384)), s, bq);
+        const unsigned nbq = Tuner::get().pick("lp_betaQ", N, grid_candidates(rq_grid(N, L, 16384)), s, bq);
         bq(nbq);
         k_lp_minmax_final<<<1, 1024, 0, s>>>(st.mmparts.get(), nbq, nullptr, st.mm.get(), fl);
       }
