@@ -432,6 +432,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--qp-only", action="store_true", help="time the QP alone (profiling runs)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the adjoint and the other configurations' timings (profiling runs: every "
+                         "launch is then one of the 1080p solve's)")
     ap.add_argument("--batch-images", type=int, default=BATCH_IMAGES)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -605,7 +608,7 @@ def main():
                                 if n.startswith("rows[")},
         "autotune": {k: {"grid": g, "ms": round(ms, 4)} for k, (g, ms, _) in sorted(D.autotune_report().items())},
     }
-    if not args.qp_only:
+    if not args.qp_only and not args.no_extras:
         out["adjoint"] = adjoint_bench(D, S)
         out["configs"] = configs_bench(D, S)
     if imgs is not None:
