@@ -519,7 +519,7 @@ std::vector<std::vector<int>> Batch::lp(const float* init, float* out, const LPP
       });
       // qp-gamma sweeps in batches for all images; images that stopped keep
       // their gamma while the others go on
-      k_lp_sweep_reset<<<1, 256, 0, s>>>(fl, B);
+      k_lp_sweep_reset<<<1, 256, 0, s>>>(fl, B, kSweepFirst);
       for (auto& pl : plans) pl.tot.clear();
       std::vector<char> running(B, 1);
       while (true) {
@@ -540,8 +540,9 @@ std::vector<std::vector<int>> Batch::lp(const float* init, float* out, const LPP
               DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbs, kBlock, 0, s>>>(sm, L, c, gi, go, Y_.get(), nsw, 0,
                                                                              parts_.get(), fl)));
           }
-          k_lp_sweep_sums<<<dim3(B, nsw), 1024, 0, s>>>(parts_.get(), nbs, sweep_.first.get(), swtot_.get(), fl);
-          k_lp_sweep_decide<<<1, 256, 0, s>>>(swtot_.get(), B, nsw, P.qp_tol, P.qp_max_iter, sc, fl);
+          k_lp_sweep_sums<<<dim3(B, nsw), 1024, 0, s>>>(parts_.get(), nbs, sweep_.first.get(), swtot_.get(), fl,
+                                                         0);
+          k_lp_sweep_decide<<<1, 256, 0, s>>>(swtot_.get(), B, nsw, P.qp_tol, P.qp_max_iter, sc, fl, nullptr);
           {
             KTimer tm("lp_sweeps_replay", s);
             if (L == 21)

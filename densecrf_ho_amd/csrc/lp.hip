@@ -23,10 +23,14 @@ namespace dcrf {
 namespace {
 
 // LP scalar slots (doubles) and flags (ints) in LPState::sc / fl
+// kL_H0..kL_H0+2: the last three sweep totals of the inner step (device plan)
 constexpr int kL_GAP = 0, kL_TT = 1, kL_DELTA = 2, kL_QPV = 3, kL_EINT = 4, kL_EHO = 5, kL_EPAIR = 6, kL_TMP = 8,
-              kL_NSC = 16;
-// kLF_TOTAL: sweeps done so far in the current inner step
-constexpr int kLF_SWEEP_DONE = 0, kLF_SWEEPS = 1, kLF_ERR = 2, kLF_TOTAL = 3, kLF_NFL = 4;
+              kL_H0 = 12, kL_NSC = 16;
+// kLF_TOTAL: sweeps done so far in the current inner step.  Batches planned
+// on the device: kLF_NEXT the next batch's length, kLF_CUR the length of the
+// batch being decided, kLF_NB the batches that ran, kLF_NH the totals seen.
+constexpr int kLF_SWEEP_DONE = 0, kLF_SWEEPS = 1, kLF_ERR = 2, kLF_TOTAL = 3, kLF_NEXT = 4, kLF_CUR = 5,
+              kLF_NB = 6, kLF_NH = 7, kLF_NFL = 8;
 constexpr int kSweepBatch = 16;
 
 // sum of the first L entries of a register array in Eigen's order for an
@@ -193,6 +197,9 @@ __device__ inline __attribute__((always_inline)) float qp_sweep(float (&g)[MAXL]
 //   An image whose loop ended in an earlier batch of this inner step only
 //   carries gamma over (gin -> gout) in pass 1.  Block b works on pixels
 //   [lb * kBlock, lb * kBlock + kBlock) of its image.
+//   nsw < 0: the lengths come from the device plan (pass 1 fl[kLF_NEXT], the
+//   replay fl[kLF_CUR]), and a batch after the loop's end does nothing (the
+//   host takes gamma from the buffer of the last batch that ran).
 template <int MAXL, int LT = 0, class SEG>  // LT > 0: the label count is known at compile time
 __global__ void __launch_bounds__(kBlock, 6) k_lp_sweeps(SEG sg, int Lr, LPConst c, const float* __restrict__ gin, float* __restrict__ gout,
                             const float* __restrict__ Y, int nsw, int replay, double* __restrict__ parts,
@@ -201,13 +208,16 @@ __global__ void __launch_bounds__(kBlock, 6) k_lp_sweeps(SEG sg, int Lr, LPConst
   const int L = LT > 0 ? LT : Lr;
   const SegPos qs = sg.pos();
   const int* fl = fl_all + (size_t)qs.img * kLF_NFL;
+  const bool dev = nsw < 0;
+  if (dev) nsw = replay ? fl[kLF_CUR] : fl[kLF_NEXT];
   if (replay && fl[kLF_SWEEPS] >= nsw) return;
   const long p0 = qs.p0 + qs.lb * kBlock;
   const int np = (int)min((long)kBlock, qs.n - qs.lb * kBlock);
   const size_t base = (size_t)p0 * L;
   const int t = threadIdx.x;
   if (!replay && fl[kLF_SWEEP_DONE]) {
-    for (int i = t; i < np * L; i += kBlock) gout[base + i] = gin[base + i];
+    if (!dev)
+      for (int i = t; i < np * L; i += kBlock) gout[base + i] = gin[base + i];
     return;
   }
   const int n = replay ? fl[kLF_SWEEPS] : nsw;
@@ -254,12 +264,14 @@ __global__ void __launch_bounds__(kBlock, 6) k_lp_sweeps(SEG sg, int Lr, LPConst
 // launch (all nblocks when first is null) into tot[image * kSweepBatch +
 // sweep]: each thread's blocks in order, the wave sums, the waves in order.
 // Images whose loop already ended in this inner step are skipped.
+// (dev: the batch's length is fl[kLF_NEXT], the grid covers kSweepBatch)
 __global__ void __launch_bounds__(1024) k_lp_sweep_sums(const double* __restrict__ parts, int nblocks,
                                                         const int* __restrict__ first, double* __restrict__ tot,
-                                                        const int* __restrict__ fl_all) {
+                                                        const int* __restrict__ fl_all, int dev) {
   __shared__ double red[1024 / 64];
   const int img = blockIdx.x, sw = blockIdx.y;
   if (fl_all[(size_t)img * kLF_NFL + kLF_SWEEP_DONE]) return;
+  if (dev && sw >= fl_all[(size_t)img * kLF_NFL + kLF_NEXT]) return;
   const int b0 = first ? first[img] : 0, b1 = first ? first[img + 1] : nblocks;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   double acc = 0.0;
@@ -280,11 +292,21 @@ __global__ void __launch_bounds__(1024) k_lp_sweep_sums(const double* __restrict
 // sc[kL_QPV] carries the previous total across batches.  One thread per
 // image; an image whose loop already ended in this inner step is left as it
 // is (its replay is a no-op).
+__device__ int sweep_plan_next(const double* h, int nh, float qp_tol);
+// nsw < 0: the batch's length is fl[kLF_NEXT] (device plan); the totals then
+// feed the plan of the next batch, and host_status (pinned host memory, may
+// be null; one image) receives {loop ended, sweeps so far, batches run}.
 __global__ void k_lp_sweep_decide(const double* __restrict__ tot, int nimg, int nsw, float qp_tol, int max_total,
-                                  double* __restrict__ sc_all, int* __restrict__ fl_all) {
+                                  double* __restrict__ sc_all, int* __restrict__ fl_all,
+                                  int* __restrict__ host_status) {
+  const bool dev = nsw < 0;
   for (int img = threadIdx.x; img < nimg; img += blockDim.x) {
     double* sc = sc_all + (size_t)img * kL_NSC;
     int* fl = fl_all + (size_t)img * kLF_NFL;
+    if (dev) {
+      nsw = fl[kLF_NEXT];
+      fl[kLF_CUR] = nsw;
+    }
     if (fl[kLF_SWEEP_DONE]) {
       fl[kLF_SWEEPS] = nsw;
       continue;
@@ -307,6 +329,23 @@ __global__ void k_lp_sweep_decide(const double* __restrict__ tot, int nimg, int 
     fl[kLF_SWEEPS] = stop;
     fl[kLF_SWEEP_DONE] = finished;
     fl[kLF_TOTAL] = done_before + stop;
+    fl[kLF_NB] += 1;
+    if (dev && !finished) {  // every total of the batch was used: the plan's history
+      double* h = sc + kL_H0;
+      for (int sw = 0; sw < nsw; ++sw) {
+        h[0] = h[1];
+        h[1] = h[2];
+        h[2] = tt[sw];
+      }
+      fl[kLF_NH] += nsw;
+      fl[kLF_NEXT] = max(1, min(sweep_plan_next(h, fl[kLF_NH], qp_tol), max_total - fl[kLF_TOTAL]));
+    }
+  }
+  if (host_status) {
+    const int* fl = fl_all;
+    host_status[0] = fl[kLF_SWEEP_DONE];
+    host_status[1] = fl[kLF_TOTAL];
+    host_status[2] = fl[kLF_NB];
   }
 }
 
@@ -321,27 +360,37 @@ __global__ void k_lp_sweep_decide(const double* __restrict__ tot, int nimg, int 
 // step's count computed about 960.  Only launch shapes depend on it, never a
 // result.
 constexpr int kSweepFirst = 5;
+// the rule, from the last three totals h[0..2] (oldest first) of the nh seen
+__host__ __device__ inline int sweep_plan_rule(const double* h, int nh, float qp_tol) {
+  if (nh < 3) return kSweepFirst;
+  const double d1 = fabs(h[2] - h[1]), d0 = fabs(h[1] - h[0]);
+  if (d1 < (double)qp_tol) return 1;
+  double r = d0 > 0 ? d1 / d0 : 0.5;
+  if (r >= 0.999) r = 0.95;
+  if (r <= 0.0) r = 0.5;
+  const double m = ceil(log((double)qp_tol / d1) / log(r));
+  return (int)fmax(1.0, fmin((double)kSweepBatch, m));
+}
+__device__ int sweep_plan_next(const double* h, int nh, float qp_tol) { return sweep_plan_rule(h, nh, qp_tol); }
 struct SweepPlan {
   std::vector<double> tot;  // the inner step's sweep totals so far
   void add(const double* t, int n) { tot.insert(tot.end(), t, t + n); }
   int next(float qp_tol) const {
     const size_t n = tot.size();
-    if (n < 3) return kSweepFirst;
-    const double d1 = std::fabs(tot[n - 1] - tot[n - 2]), d0 = std::fabs(tot[n - 2] - tot[n - 3]);
-    if (d1 < (double)qp_tol) return 1;
-    double r = d0 > 0 ? d1 / d0 : 0.5;
-    if (r >= 0.999) r = 0.95;
-    if (r <= 0.0) r = 0.5;
-    const double m = std::ceil(std::log((double)qp_tol / d1) / std::log(r));
-    return (int)std::max(1.0, std::min((double)kSweepBatch, m));
+    return sweep_plan_rule(n >= 3 ? tot.data() + n - 3 : nullptr, (int)n, qp_tol);
   }
 };
 
 // start of an inner step's sweep loop, one thread per image
-__global__ void k_lp_sweep_reset(int* __restrict__ fl_all, int nimg) {
+// (first: the first batch's length under the device plan)
+__global__ void k_lp_sweep_reset(int* __restrict__ fl_all, int nimg, int first) {
   for (int i = threadIdx.x; i < nimg; i += blockDim.x) {
-    fl_all[(size_t)i * kLF_NFL + kLF_SWEEP_DONE] = 0;
-    fl_all[(size_t)i * kLF_NFL + kLF_TOTAL] = 0;
+    int* fl = fl_all + (size_t)i * kLF_NFL;
+    fl[kLF_SWEEP_DONE] = 0;
+    fl[kLF_TOTAL] = 0;
+    fl[kLF_NEXT] = first;
+    fl[kLF_NB] = 0;
+    fl[kLF_NH] = 0;
   }
 }
 
@@ -776,6 +825,12 @@ struct CRF::LPState {
   DBuf<int> fl;
   DBuf<double> st_gap, st_delta;
   int cap = 0;
+  // the sweep decision's status, written by the kernel into pinned host
+  // memory (no copy launch before the host reads it)
+  int* hstatus = nullptr;
+  ~LPState() {
+    if (hstatus) (void)hipHostFree(hstatus);
+  }
 };
 
 // labelling energy: unary + true pairwise (+ superpixel term), into sc[dst]
@@ -872,6 +927,7 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
   st.mmparts.resize(2 * (size_t)std::max<long>(nbp, 16384));  // up to the largest betaQ grid tried
   st.mm.resize(2);
   st.fl.resize(kLF_NFL);
+  if (!st.hstatus) DCRF_HIP(hipHostMalloc((void**)&st.hstatus, 4 * sizeof(int), hipHostMallocCoherent));
   const int cap = std::max(1, std::min(stats_cap, 1 << 16));
   if (cap > st.cap) {
     st.st_gap.resize(cap);
@@ -950,43 +1006,51 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
       // the decision finds where the reference's loop stops, the replay redoes
       // that many sweeps only when the stop comes before the end of the batch.
       // The batch lengths come from SweepPlan.
+      // Batches are planned on the device (k_lp_sweep_decide writes the next
+      // one's length), so the host enqueues kSweepAhead batches before it
+      // waits for the status: one wait covers most inner steps.  A batch
+      // after the loop's end does nothing; gamma is then in the buffer of
+      // the last batch that ran.
+      constexpr int kSweepAhead = 2;
       int done_sweeps = 0;
-      SweepPlan plan;
-      k_lp_sweep_reset<<<1, 64, 0, s>>>(fl, 1);
+      k_lp_sweep_reset<<<1, 64, 0, s>>>(fl, 1, std::min(kSweepFirst, P.qp_max_iter));
+      float* gbuf[2] = {st.gamma.get(), st.gamma2.get()};
+      int enq = 0, nb = 0;
       while (true) {
-        const int nsw = std::min(plan.next(P.qp_tol), P.qp_max_iter - done_sweeps);
-        {
-          KTimer tm("lp_sweeps", s);
-          if (L == 21)
-            k_lp_sweeps<21, 21><<<nbp, kBlock, 0, s>>>(OneImage{N}, L, c, st.gamma.get(), st.gamma2.get(), st.Y.get(),
-                                                       nsw, 0, st.parts.get(), fl);
-          else
-            DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbp, kBlock, 0, s>>>(OneImage{N}, L, c, st.gamma.get(),
-                                                                           st.gamma2.get(), st.Y.get(), nsw, 0,
-                                                                           st.parts.get(), fl)));
+        for (int b = 0; b < kSweepAhead; ++b, ++enq) {
+          float* gin = gbuf[enq & 1];
+          float* gout = gbuf[(enq + 1) & 1];
+          {
+            KTimer tm("lp_sweeps", s);
+            if (L == 21)
+              k_lp_sweeps<21, 21><<<nbp, kBlock, 0, s>>>(OneImage{N}, L, c, gin, gout, st.Y.get(), -1, 0,
+                                                         st.parts.get(), fl);
+            else
+              DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbp, kBlock, 0, s>>>(OneImage{N}, L, c, gin, gout,
+                                                                             st.Y.get(), -1, 0, st.parts.get(),
+                                                                             fl)));
+          }
+          k_lp_sweep_sums<<<dim3(1, kSweepBatch), 1024, 0, s>>>(st.parts.get(), nbp, nullptr, st.swtot.get(), fl,
+                                                                 1);
+          k_lp_sweep_decide<<<1, 64, 0, s>>>(st.swtot.get(), 1, -1, P.qp_tol, P.qp_max_iter, sc, fl, st.hstatus);
+          {
+            KTimer tm("lp_sweeps_replay", s);
+            if (L == 21)
+              k_lp_sweeps<21, 21><<<nbp, kBlock, 0, s>>>(OneImage{N}, L, c, gin, gout, st.Y.get(), -1, 1,
+                                                         st.parts.get(), fl);
+            else
+              DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbp, kBlock, 0, s>>>(OneImage{N}, L, c, gin, gout,
+                                                                             st.Y.get(), -1, 1, st.parts.get(),
+                                                                             fl)));
+          }
         }
-        k_lp_sweep_sums<<<dim3(1, nsw), 1024, 0, s>>>(st.parts.get(), nbp, nullptr, st.swtot.get(), fl);
-        k_lp_sweep_decide<<<1, 64, 0, s>>>(st.swtot.get(), 1, nsw, P.qp_tol, P.qp_max_iter, sc, fl);
-        {
-          KTimer tm("lp_sweeps_replay", s);
-          if (L == 21)
-            k_lp_sweeps<21, 21><<<nbp, kBlock, 0, s>>>(OneImage{N}, L, c, st.gamma.get(), st.gamma2.get(), st.Y.get(),
-                                                       nsw, 1, st.parts.get(), fl);
-          else
-            DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbp, kBlock, 0, s>>>(OneImage{N}, L, c, st.gamma.get(),
-                                                                           st.gamma2.get(), st.Y.get(), nsw, 1,
-                                                                           st.parts.get(), fl)));
-        }
-        std::swap(st.gamma, st.gamma2);
-        int hf[2] = {0, 0};
-        double tt[kSweepBatch];
-        DCRF_HIP(hipMemcpyAsync(hf, fl, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-        DCRF_HIP(hipMemcpyAsync(tt, st.swtot.get(), nsw * sizeof(double), hipMemcpyDeviceToHost, s));
         DCRF_HIP(hipStreamSynchronize(s));
-        plan.add(tt, nsw);
-        done_sweeps += hf[kLF_SWEEPS];
-        if (hf[kLF_SWEEP_DONE] || done_sweeps >= P.qp_max_iter) break;
+        const volatile int* hs = st.hstatus;
+        done_sweeps = hs[1];
+        nb = hs[2];
+        if (hs[0] || done_sweeps >= P.qp_max_iter) break;
       }
+      if (nb & 1) std::swap(st.gamma, st.gamma2);  // gamma after the last batch that ran
       res.qp_sweeps.push_back(done_sweeps);
       {
         KTimer tm("lp_betaQ", s);
