@@ -504,8 +504,7 @@ std::vector<std::vector<int>> Batch::lp(const float* init, float* out, const LPP
   std::vector<std::vector<int>> sweeps(B);
   float* gcur = gamma_.get();
   float* gnext = gamma2_.get();
-  std::vector<SweepPlan> plans(B);
-  std::vector<double> htot((size_t)B * kSweepBatchB);
+  hstatus_.resize(3 + (size_t)B);
   for (int it = 0; it < P.prox_max_iter; ++it) {
     k_fill<<<grid_for(NL), kBlock, 0, s>>>(gcur, NL, 1.f);
     DCRF_HIP(hipMemcpyAsync(curQ_.get(), lQ_.get(), NL * sizeof(float), hipMemcpyDeviceToDevice, s));
@@ -517,53 +516,46 @@ std::vector<std::vector<int>> Batch::lp(const float* init, float* out, const LPP
         k_lp_Y<<<nbr, kBlock, 0, s>>>(rm, L, c, alpha_.get(), nullptr, sa_.get(), nullptr, sc, upd, U_.get(),
                                       curQ_.get(), Y_.get());
       });
-      // qp-gamma sweeps in batches for all images; images that stopped keep
-      // their gamma while the others go on
-      k_lp_sweep_reset<<<1, 256, 0, s>>>(fl, B, kSweepFirst);
-      for (auto& pl : plans) pl.tot.clear();
-      std::vector<char> running(B, 1);
+      // qp-gamma sweeps in batches for all images, each image's batch lengths
+      // planned on the device (k_lp_sweep_decide); images that stopped carry
+      // their gamma over while the others go on.  The host reads the status
+      // from pinned memory after each batch: a batch enqueued ahead would
+      // copy every finished image's gamma once more when the loop has ended.
+      constexpr int kSweepAhead = 1;
+      k_lp_sweep_reset<<<1, 256, 0, s>>>(fl, B, std::min(kSweepFirst, P.qp_max_iter));
       while (true) {
-        // the longest planned batch of the images still running
-        int want = 1;
-        for (int b = 0; b < B; ++b)
-          if (running[b]) want = std::max(want, plans[b].next(P.qp_tol));
-        const int nsw = std::min(want, P.qp_max_iter);
-        const int parity = gcur == gamma_.get() ? 0 : 1;
-        float* gi = gcur;
-        float* go = gnext;
-        run_graph(kGraphSweep + 2 * nsw + parity, [&]() {
-          {
-            KTimer tm("lp_sweeps", s);
-            if (L == 21)
-              k_lp_sweeps<21, 21><<<nbs, kBlock, 0, s>>>(sm, L, c, gi, go, Y_.get(), nsw, 0, parts_.get(), fl);
-            else
-              DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbs, kBlock, 0, s>>>(sm, L, c, gi, go, Y_.get(), nsw, 0,
-                                                                             parts_.get(), fl)));
-          }
-          k_lp_sweep_sums<<<dim3(B, nsw), 1024, 0, s>>>(parts_.get(), nbs, sweep_.first.get(), swtot_.get(), fl,
-                                                         0);
-          k_lp_sweep_decide<<<1, 256, 0, s>>>(swtot_.get(), B, nsw, P.qp_tol, P.qp_max_iter, sc, fl, nullptr);
-          {
-            KTimer tm("lp_sweeps_replay", s);
-            if (L == 21)
-              k_lp_sweeps<21, 21><<<nbs, kBlock, 0, s>>>(sm, L, c, gi, go, Y_.get(), nsw, 1, parts_.get(), fl);
-            else
-              DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbs, kBlock, 0, s>>>(sm, L, c, gi, go, Y_.get(), nsw, 1,
-                                                                             parts_.get(), fl)));
-          }
-        });
-        std::swap(gcur, gnext);
-        DCRF_HIP(hipMemcpyAsync(htot.data(), swtot_.get(), htot.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-        read_flags();
-        bool all = true;
-        for (int b = 0; b < B; ++b) {
-          if (running[b]) plans[b].add(htot.data() + (size_t)b * kSweepBatchB, nsw);
-          running[b] = hfl[(size_t)b * kLF_NFL + kLF_SWEEP_DONE] == 0;
-          all = all && !running[b];
+        for (int ahead = 0; ahead < kSweepAhead; ++ahead) {
+          const int parity = gcur == gamma_.get() ? 0 : 1;
+          float* gi = gcur;
+          float* go = gnext;
+          run_graph(kGraphSweep + parity, [&]() {
+            {
+              KTimer tm("lp_sweeps", s);
+              if (L == 21)
+                k_lp_sweeps<21, 21><<<nbs, kBlock, 0, s>>>(sm, L, c, gi, go, Y_.get(), -1, 0, 1, parts_.get(), fl);
+              else
+                DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbs, kBlock, 0, s>>>(sm, L, c, gi, go, Y_.get(), -1, 0, 1,
+                                                                               parts_.get(), fl)));
+            }
+            k_lp_sweep_sums<<<dim3(B, kSweepBatch), 1024, 0, s>>>(parts_.get(), nbs, sweep_.first.get(),
+                                                                   swtot_.get(), fl, 1);
+            k_lp_sweep_decide<<<1, 256, 0, s>>>(swtot_.get(), B, -1, P.qp_tol, P.qp_max_iter, sc, fl,
+                                                hstatus_.get());
+            {
+              KTimer tm("lp_sweeps_replay", s);
+              if (L == 21)
+                k_lp_sweeps<21, 21><<<nbs, kBlock, 0, s>>>(sm, L, c, gi, go, Y_.get(), -1, 1, 1, parts_.get(), fl);
+              else
+                DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbs, kBlock, 0, s>>>(sm, L, c, gi, go, Y_.get(), -1, 1, 1,
+                                                                               parts_.get(), fl)));
+            }
+          });
+          std::swap(gcur, gnext);
         }
-        if (all) break;
+        DCRF_HIP(hipStreamSynchronize(s));
+        if (hstatus_.get()[0] || P.qp_max_iter <= 0) break;
       }
-      for (int b = 0; b < B; ++b) sweeps[b].push_back(hfl[(size_t)b * kLF_NFL + kLF_TOTAL]);
+      for (int b = 0; b < B; ++b) sweeps[b].push_back((int)((const volatile int*)hstatus_.get())[3 + b]);
       const int parity = gcur == gamma_.get() ? 0 : 1;
       float* gm = gcur;
       run_graph(kGraphPost + parity, [&]() {

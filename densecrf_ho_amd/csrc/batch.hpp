@@ -112,6 +112,7 @@ class Batch {
   DBuf<float> alpha_, sa_, gamma_, gamma2_, Y_, lQ_, curQ_, best_, mmparts_, mm_;
   DBuf<double> lsc_, ost_;
   DBuf<int> lfl_, ofl_;
+  HBuf<int> hstatus_;  // the sweep decision's status (pinned host memory)
 };
 
 }  // namespace dcrf

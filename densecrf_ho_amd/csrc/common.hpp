@@ -179,6 +179,29 @@ struct DBuf {
   size_t bytes() const { return n * sizeof(T); }
 };
 
+// Pinned, coherent host memory that kernels write directly (small status
+// words the host reads after a stream synchronisation, with no copy launch).
+template <class T>
+struct HBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  HBuf() = default;
+  HBuf(const HBuf&) = delete;
+  HBuf& operator=(const HBuf&) = delete;
+  ~HBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  void resize(size_t count) {
+    if (count <= n && p) return;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+    DCRF_HIP(hipHostMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocCoherent));
+    n = count;
+  }
+  T* get() const { return p; }
+};
+
 // ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------

@@ -198,12 +198,13 @@ __device__ inline __attribute__((always_inline)) float qp_sweep(float (&g)[MAXL]
 //   carries gamma over (gin -> gout) in pass 1.  Block b works on pixels
 //   [lb * kBlock, lb * kBlock + kBlock) of its image.
 //   nsw < 0: the lengths come from the device plan (pass 1 fl[kLF_NEXT], the
-//   replay fl[kLF_CUR]), and a batch after the loop's end does nothing (the
-//   host takes gamma from the buffer of the last batch that ran).
+//   replay fl[kLF_CUR]).  Without `carry`, an image whose loop has ended does
+//   nothing in pass 1 (one image: the host takes gamma from the buffer of the
+//   last batch that ran).
 template <int MAXL, int LT = 0, class SEG>  // LT > 0: the label count is known at compile time
 __global__ void __launch_bounds__(kBlock, 6) k_lp_sweeps(SEG sg, int Lr, LPConst c, const float* __restrict__ gin, float* __restrict__ gout,
-                            const float* __restrict__ Y, int nsw, int replay, double* __restrict__ parts,
-                            const int* __restrict__ fl_all) {
+                            const float* __restrict__ Y, int nsw, int replay, int carry,
+                            double* __restrict__ parts, const int* __restrict__ fl_all) {
   __shared__ float tile[kBlock * MAXL];
   const int L = LT > 0 ? LT : Lr;
   const SegPos qs = sg.pos();
@@ -216,7 +217,7 @@ __global__ void __launch_bounds__(kBlock, 6) k_lp_sweeps(SEG sg, int Lr, LPConst
   const size_t base = (size_t)p0 * L;
   const int t = threadIdx.x;
   if (!replay && fl[kLF_SWEEP_DONE]) {
-    if (!dev)
+    if (carry)
       for (int i = t; i < np * L; i += kBlock) gout[base + i] = gin[base + i];
     return;
   }
@@ -292,10 +293,33 @@ __global__ void __launch_bounds__(1024) k_lp_sweep_sums(const double* __restrict
 // sc[kL_QPV] carries the previous total across batches.  One thread per
 // image; an image whose loop already ended in this inner step is left as it
 // is (its replay is a no-op).
-__device__ int sweep_plan_next(const double* h, int nh, float qp_tol);
+// Length of the next batch of qp-gamma sweeps of an inner step.  The loop
+// stops at the first sweep whose total moves by less than qp_tol
+// (src/densecrf.cpp:670-675), and the totals' steps shrink about
+// geometrically: after a first batch of kSweepFirst sweeps the remaining
+// count is predicted from the ratio of the last two steps.  A batch that runs
+// past the stop costs a replay and one that ends short another launch, each
+// about four sweeps' time (1080p); measured on the 1080p LP this plan
+// computes about 530 sweeps for 503 used, where following the previous inner
+// step's count computed about 960.  It runs on the device, in
+// k_lp_sweep_decide, so the host can enqueue batches without reading the
+// totals.  Only launch shapes depend on it, never a result.
+constexpr int kSweepFirst = 5;
+// the rule, from the last three totals h[0..2] (oldest first) of the nh seen
+__device__ inline int sweep_plan_rule(const double* h, int nh, float qp_tol) {
+  if (nh < 3) return kSweepFirst;
+  const double d1 = fabs(h[2] - h[1]), d0 = fabs(h[1] - h[0]);
+  if (d1 < (double)qp_tol) return 1;
+  double r = d0 > 0 ? d1 / d0 : 0.5;
+  if (r >= 0.999) r = 0.95;
+  if (r <= 0.0) r = 0.5;
+  const double m = ceil(log((double)qp_tol / d1) / log(r));
+  return (int)fmax(1.0, fmin((double)kSweepBatch, m));
+}
 // nsw < 0: the batch's length is fl[kLF_NEXT] (device plan); the totals then
-// feed the plan of the next batch, and host_status (pinned host memory, may
-// be null; one image) receives {loop ended, sweeps so far, batches run}.
+// feed the plan of the next batch.  host_status (pinned host memory, may be
+// null) receives {every image's loop ended, image 0's sweeps so far, image
+// 0's batches run, then each image's sweeps so far}.  One block.
 __global__ void k_lp_sweep_decide(const double* __restrict__ tot, int nimg, int nsw, float qp_tol, int max_total,
                                   double* __restrict__ sc_all, int* __restrict__ fl_all,
                                   int* __restrict__ host_status) {
@@ -338,48 +362,23 @@ __global__ void k_lp_sweep_decide(const double* __restrict__ tot, int nimg, int 
         h[2] = tt[sw];
       }
       fl[kLF_NH] += nsw;
-      fl[kLF_NEXT] = max(1, min(sweep_plan_next(h, fl[kLF_NH], qp_tol), max_total - fl[kLF_TOTAL]));
+      fl[kLF_NEXT] = max(1, min(sweep_plan_rule(h, fl[kLF_NH], qp_tol), max_total - fl[kLF_TOTAL]));
     }
   }
   if (host_status) {
-    const int* fl = fl_all;
-    host_status[0] = fl[kLF_SWEEP_DONE];
-    host_status[1] = fl[kLF_TOTAL];
-    host_status[2] = fl[kLF_NB];
+    __syncthreads();
+    for (int img = threadIdx.x; img < nimg; img += blockDim.x)
+      host_status[3 + img] = fl_all[(size_t)img * kLF_NFL + kLF_TOTAL];
+    if (threadIdx.x == 0) {
+      int all = 1;
+      for (int img = 0; img < nimg; ++img) all = all && fl_all[(size_t)img * kLF_NFL + kLF_SWEEP_DONE];
+      host_status[0] = all;
+      host_status[1] = fl_all[kLF_TOTAL];
+      host_status[2] = fl_all[kLF_NB];
+    }
   }
 }
 
-// Length of the next batch of qp-gamma sweeps of an inner step.  The loop
-// stops at the first sweep whose total moves by less than qp_tol
-// (src/densecrf.cpp:670-675), and the totals' steps shrink about
-// geometrically: after a first batch of kSweepFirst sweeps the remaining
-// count is predicted from the ratio of the last two steps.  A batch that runs
-// past the stop costs a replay and one that ends short another launch, each
-// about four sweeps' time (1080p); measured on the 1080p LP this plan
-// computes about 530 sweeps for 503 used, where following the previous inner
-// step's count computed about 960.  Only launch shapes depend on it, never a
-// result.
-constexpr int kSweepFirst = 5;
-// the rule, from the last three totals h[0..2] (oldest first) of the nh seen
-__host__ __device__ inline int sweep_plan_rule(const double* h, int nh, float qp_tol) {
-  if (nh < 3) return kSweepFirst;
-  const double d1 = fabs(h[2] - h[1]), d0 = fabs(h[1] - h[0]);
-  if (d1 < (double)qp_tol) return 1;
-  double r = d0 > 0 ? d1 / d0 : 0.5;
-  if (r >= 0.999) r = 0.95;
-  if (r <= 0.0) r = 0.5;
-  const double m = ceil(log((double)qp_tol / d1) / log(r));
-  return (int)fmax(1.0, fmin((double)kSweepBatch, m));
-}
-__device__ int sweep_plan_next(const double* h, int nh, float qp_tol) { return sweep_plan_rule(h, nh, qp_tol); }
-struct SweepPlan {
-  std::vector<double> tot;  // the inner step's sweep totals so far
-  void add(const double* t, int n) { tot.insert(tot.end(), t, t + n); }
-  int next(float qp_tol) const {
-    const size_t n = tot.size();
-    return sweep_plan_rule(n >= 3 ? tot.data() + n - 3 : nullptr, (int)n, qp_tol);
-  }
-};
 
 // start of an inner step's sweep loop, one thread per image
 // (first: the first batch's length under the device plan)
@@ -825,12 +824,7 @@ struct CRF::LPState {
   DBuf<int> fl;
   DBuf<double> st_gap, st_delta;
   int cap = 0;
-  // the sweep decision's status, written by the kernel into pinned host
-  // memory (no copy launch before the host reads it)
-  int* hstatus = nullptr;
-  ~LPState() {
-    if (hstatus) (void)hipHostFree(hstatus);
-  }
+  HBuf<int> hstatus;  // the sweep decision's status (k_lp_sweep_decide)
 };
 
 // labelling energy: unary + true pairwise (+ superpixel term), into sc[dst]
@@ -927,7 +921,7 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
   st.mmparts.resize(2 * (size_t)std::max<long>(nbp, 16384));  // up to the largest betaQ grid tried
   st.mm.resize(2);
   st.fl.resize(kLF_NFL);
-  if (!st.hstatus) DCRF_HIP(hipHostMalloc((void**)&st.hstatus, 4 * sizeof(int), hipHostMallocCoherent));
+  st.hstatus.resize(4);
   const int cap = std::max(1, std::min(stats_cap, 1 << 16));
   if (cap > st.cap) {
     st.st_gap.resize(cap);
@@ -1005,7 +999,6 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
       // qp-gamma sweeps in batches: pass 1 runs `nsw` sweeps gamma -> gamma2,
       // the decision finds where the reference's loop stops, the replay redoes
       // that many sweeps only when the stop comes before the end of the batch.
-      // The batch lengths come from SweepPlan.
       // Batches are planned on the device (k_lp_sweep_decide writes the next
       // one's length), so the host enqueues kSweepAhead batches before it
       // waits for the status: one wait covers most inner steps.  A batch
@@ -1023,29 +1016,30 @@ LPResult CRF::lp(const float* init, float* out, const LPParams& P, int stats_cap
           {
             KTimer tm("lp_sweeps", s);
             if (L == 21)
-              k_lp_sweeps<21, 21><<<nbp, kBlock, 0, s>>>(OneImage{N}, L, c, gin, gout, st.Y.get(), -1, 0,
+              k_lp_sweeps<21, 21><<<nbp, kBlock, 0, s>>>(OneImage{N}, L, c, gin, gout, st.Y.get(), -1, 0, 0,
                                                          st.parts.get(), fl);
             else
               DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbp, kBlock, 0, s>>>(OneImage{N}, L, c, gin, gout,
-                                                                             st.Y.get(), -1, 0, st.parts.get(),
+                                                                             st.Y.get(), -1, 0, 0, st.parts.get(),
                                                                              fl)));
           }
           k_lp_sweep_sums<<<dim3(1, kSweepBatch), 1024, 0, s>>>(st.parts.get(), nbp, nullptr, st.swtot.get(), fl,
                                                                  1);
-          k_lp_sweep_decide<<<1, 64, 0, s>>>(st.swtot.get(), 1, -1, P.qp_tol, P.qp_max_iter, sc, fl, st.hstatus);
+          k_lp_sweep_decide<<<1, 64, 0, s>>>(st.swtot.get(), 1, -1, P.qp_tol, P.qp_max_iter, sc, fl,
+                                             st.hstatus.get());
           {
             KTimer tm("lp_sweeps_replay", s);
             if (L == 21)
-              k_lp_sweeps<21, 21><<<nbp, kBlock, 0, s>>>(OneImage{N}, L, c, gin, gout, st.Y.get(), -1, 1,
+              k_lp_sweeps<21, 21><<<nbp, kBlock, 0, s>>>(OneImage{N}, L, c, gin, gout, st.Y.get(), -1, 1, 0,
                                                          st.parts.get(), fl);
             else
               DCRF_MAXL_DISPATCH(L, (k_lp_sweeps<MAXL><<<nbp, kBlock, 0, s>>>(OneImage{N}, L, c, gin, gout,
-                                                                             st.Y.get(), -1, 1, st.parts.get(),
+                                                                             st.Y.get(), -1, 1, 0, st.parts.get(),
                                                                              fl)));
           }
         }
         DCRF_HIP(hipStreamSynchronize(s));
-        const volatile int* hs = st.hstatus;
+        const volatile int* hs = st.hstatus.get();
         done_sweeps = hs[1];
         nb = hs[2];
         if (hs[0] || done_sweeps >= P.qp_max_iter) break;
