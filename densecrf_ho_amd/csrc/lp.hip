@@ -225,18 +225,46 @@ __global__ void __launch_bounds__(kBlock, 6) k_lp_sweeps(SEG sg, int Lr, LPConst
   const bool act = t < np;
   __shared__ double red[kSweepBatch][kBlock / 64];  // per sweep and wave: wave sum of the qp values
   float g[MAXL], ph[MAXL], nh[MAXL];
-  for (int i = t; i < np * L; i += kBlock) tile[i] = gin[base + i];
-  __syncthreads();
+  if constexpr (LT > 0 && LT == MAXL) {
+    // the label count fixed at compile time: each thread loads its own two
+    // rows with 16-byte loads, all issued at once (no LDS round trips)
+    const size_t r0 = base + (size_t)t * LT;
+    float y[MAXL];
 #pragma unroll
-  for (int l = 0; l < MAXL; ++l) g[l] = (act && l < L) ? tile[t * L + l] : 0.f;
-  __syncthreads();
-  for (int i = t; i < np * L; i += kBlock) tile[i] = Y[base + i];
-  __syncthreads();
+    for (int l = 0; l < MAXL; ++l) g[l] = y[l] = 0.f;
+    if (act) {
 #pragma unroll
-  for (int l = 0; l < MAXL; ++l) {
-    const float y = (act && l < L) ? tile[t * L + l] : 0.f;
-    ph[l] = fmaxf(-y, 0.f);
-    nh[l] = fmaxf(y, 0.f);
+      for (int l = 0; l + 3 < LT; l += 4) {
+        const Float4u a = *reinterpret_cast<const Float4u*>(gin + r0 + l);
+        const Float4u b = *reinterpret_cast<const Float4u*>(Y + r0 + l);
+        g[l] = a.x, g[l + 1] = a.y, g[l + 2] = a.z, g[l + 3] = a.w;
+        y[l] = b.x, y[l + 1] = b.y, y[l + 2] = b.z, y[l + 3] = b.w;
+      }
+#pragma unroll
+      for (int l = LT & ~3; l < LT; ++l) {
+        g[l] = gin[r0 + l];
+        y[l] = Y[r0 + l];
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < MAXL; ++l) {
+      ph[l] = fmaxf(-y[l], 0.f);
+      nh[l] = fmaxf(y[l], 0.f);
+    }
+  } else {
+    for (int i = t; i < np * L; i += kBlock) tile[i] = gin[base + i];
+    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < MAXL; ++l) g[l] = (act && l < L) ? tile[t * L + l] : 0.f;
+    __syncthreads();
+    for (int i = t; i < np * L; i += kBlock) tile[i] = Y[base + i];
+    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < MAXL; ++l) {
+      const float y = (act && l < L) ? tile[t * L + l] : 0.f;
+      ph[l] = fmaxf(-y, 0.f);
+      nh[l] = fmaxf(y, 0.f);
+    }
   }
   float gsum = eigen_sum_reg<MAXL>(g, L);
   for (int sw = 0; sw < n; ++sw) {
@@ -253,11 +281,22 @@ __global__ void __launch_bounds__(kBlock, 6) k_lp_sweeps(SEG sg, int Lr, LPConst
       for (int w = 0; w < kBlock / 64; ++w) bs += red[sw][w];
       parts[(size_t)sw * gridDim.x + blockIdx.x] = bs;
     }
+  if constexpr (LT > 0 && LT == MAXL) {
+    if (act) {
+      const size_t r0 = base + (size_t)t * LT;
 #pragma unroll
-  for (int l = 0; l < MAXL; ++l)
-    if (act && l < L) tile[t * L + l] = g[l];
-  __syncthreads();
-  for (int i = t; i < np * L; i += kBlock) gout[base + i] = tile[i];
+      for (int l = 0; l + 3 < LT; l += 4)
+        *reinterpret_cast<Float4u*>(gout + r0 + l) = Float4u{g[l], g[l + 1], g[l + 2], g[l + 3]};
+#pragma unroll
+      for (int l = LT & ~3; l < LT; ++l) gout[r0 + l] = g[l];
+    }
+  } else {
+#pragma unroll
+    for (int l = 0; l < MAXL; ++l)
+      if (act && l < L) tile[t * L + l] = g[l];
+    __syncthreads();
+    for (int i = t; i < np * L; i += kBlock) gout[base + i] = tile[i];
+  }
 }
 
 // per-sweep totals of the batch's block partials: block (image, sweep) sums
