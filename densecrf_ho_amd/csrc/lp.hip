@@ -755,24 +755,31 @@ constexpr int pow2_at_least() {
   return n;
 }
 template <int LT, class SEG>
-__global__ void __launch_bounds__(kBlock) k_lp_project_px(SEG sg, const float* __restrict__ Qin,
-                                                          float* __restrict__ Qout, int* __restrict__ lab,
+__global__ void __launch_bounds__(kBlock) k_lp_project_px(SEG sg, const float* Qin, float* Qout,
+                                                          int* __restrict__ lab,
                                                           int do_feasible) {
-  __shared__ float tile[kBlock * LT];
   const SegPos qs = sg.pos();
   const int t = threadIdx.x;
   const long ntiles = (qs.n + kBlock - 1) / kBlock;
   for (long tl = qs.lb; tl < ntiles; tl += qs.nb) {
     const long c0 = tl * kBlock;  // the tile's first column within its image
     const int np = (int)min((long)kBlock, qs.n - c0);
-    const size_t base = (size_t)(qs.p0 + c0) * LT;
-    __syncthreads();  // the previous tile's stores have read the LDS
-    for (int i = t; i < np * LT; i += kBlock) tile[i] = Qin[base + i];
-    __syncthreads();
     const bool act = t < np;
+    // the thread's own row, 16-byte loads (in place when Qin == Qout: a row
+    // is read before it is written, by the same thread)
+    const size_t r0 = (size_t)(qs.p0 + c0 + t) * LT;
     float q[LT], v[LT];
 #pragma unroll
-    for (int l = 0; l < LT; ++l) q[l] = act ? tile[t * LT + l] : 0.f;
+    for (int l = 0; l < LT; ++l) q[l] = 0.f;
+    if (act) {
+#pragma unroll
+      for (int l = 0; l + 3 < LT; l += 4) {
+        const Float4u a4 = *reinterpret_cast<const Float4u*>(Qin + r0 + l);
+        q[l] = a4.x, q[l + 1] = a4.y, q[l + 2] = a4.z, q[l + 3] = a4.w;
+      }
+#pragma unroll
+      for (int l = LT & ~3; l < LT; ++l) q[l] = Qin[r0 + l];
+    }
     const int a = col_astart(c0 + t, LT);
     if (do_feasible) {
       const float cs = eigen_sum_col<LT>(q, a);
@@ -817,10 +824,11 @@ __global__ void __launch_bounds__(kBlock) k_lp_project_px(SEG sg, const float* _
     if (act) {
       lab[qs.p0 + c0 + t] = m;
 #pragma unroll
-      for (int l = 0; l < LT; ++l) tile[t * LT + l] = v[l];  // the thread's own row
+      for (int l = 0; l + 3 < LT; l += 4)
+        *reinterpret_cast<Float4u*>(Qout + r0 + l) = Float4u{v[l], v[l + 1], v[l + 2], v[l + 3]};
+#pragma unroll
+      for (int l = LT & ~3; l < LT; ++l) Qout[r0 + l] = v[l];
     }
-    __syncthreads();
-    for (int i = t; i < np * LT; i += kBlock) Qout[base + i] = tile[i];
   }
 }
 
