@@ -249,6 +249,15 @@ __device__ inline unsigned xcd_block() {
   return x * per + min(x, rem) + i;
 }
 
+// Lanes of one wave exchanging values through LDS: the wave's earlier LDS
+// writes are visible to its later reads, with no barrier across the block
+// (other waves go on with their own rows).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Number of blocks used by grid-stride reduction kernels: a fixed count so the
 // combination order of partial sums (and therefore the result) is
 // deterministic for a given problem size.

@@ -327,7 +327,7 @@ __global__ void __launch_bounds__(kBlock) k_argmin_rg(const float* __restrict__ 
     const RG r = rg_at(L, N, tile);
     float* row = seg + (threadIdx.x & ~63) + r.g * L;
     if (r.valid) row[r.k] = g[(size_t)r.row * L + r.k];
-    __syncthreads();
+    wave_lds_sync();  // a row group stays within its wave
     if (r.valid && r.k == 0) {
       int m = 0;
       float v = row[0];
@@ -338,7 +338,7 @@ __global__ void __launch_bounds__(kBlock) k_argmin_rg(const float* __restrict__ 
         }
       lab[r.row] = m;
     }
-    __syncthreads();
+    wave_lds_sync();
   }
 }
 
@@ -433,7 +433,7 @@ __global__ void __launch_bounds__(kBlock) k_qp_update_rg(const float* __restrict
       e += (double)q * (double)(g + U[idx]);
       row[r.k] = g;
     }
-    __syncthreads();
+    wave_lds_sync();  // a row group stays within its wave
     if (r.valid && r.k == 0) {
       int m = 0;
       float v = row[0];
@@ -444,7 +444,7 @@ __global__ void __launch_bounds__(kBlock) k_qp_update_rg(const float* __restrict
         }
       lab[r.row] = m;
     }
-    __syncthreads();
+    wave_lds_sync();
   }
   double v[1] = {e};
   block_sum<1>(v);

@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(kBlock) k_lp_Y(SEG sg, int L, LPConst c, float
         if (i < nk) row[k0 + i] = t[i];
       }
     }
-    __syncthreads();
+    wave_lds_sync();  // the row sums stay within the wave
     if (r.valid) {
       const float S = eigen_sum(L, 0, [&](int i) { return row[i]; });
       float y[4];
@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(kBlock) k_lp_Y(SEG sg, int L, LPConst c, float
       for (int i = 0; i < 4; ++i) y[i] = ((c.c_m1_over_M * S + t[i]) * c.lambda) + cq[i];
       st_q4(Y + idx, nk, y);
     }
-    __syncthreads();
+    wave_lds_sync();
   }
 }
 
@@ -466,7 +466,7 @@ __global__ void __launch_bounds__(kBlock) k_lp_betaQ(SEG sg, int L, float lambda
       for (int i = 0; i < 4; ++i)
         if (i < nk) row[k0 + i] = ((a[i] + uu[i]) + gm[i]) - uv[i];
     }
-    __syncthreads();
+    wave_lds_sync();  // the row sums stay within the wave
     if (r.valid) {
       // alignment of the column within its own image's matrix
       float b = -eigen_sum(L, col_astart(r.row - qs.p0, L), [&](int i) { return row[i]; });
@@ -482,7 +482,7 @@ __global__ void __launch_bounds__(kBlock) k_lp_betaQ(SEG sg, int L, float lambda
       }
       st_q4(Q + idx, nk, qv);
     }
-    __syncthreads();
+    wave_lds_sync();
   }
   smn[threadIdx.x] = mn;
   smx[threadIdx.x] = mx;
