@@ -452,13 +452,24 @@ def main():
     from densecrf_ho_amd import synthetic as S
 
     wl = S.Workload(W, H, K, 0, L=L)
-    crf = D.DenseCRF2D(W, H, L)
-    crf.setUnaryEnergy(D.DeviceArray.from_numpy(wl.U))
-    t = time.perf_counter()
-    for f, w in wl.kernels:
-        crf.addPairwiseEnergy(D.DeviceArray.from_numpy(f), D.PottsCompatibility(w))
-    D.synchronize()
-    build_ms = (time.perf_counter() - t) * 1e3
+    U_dev = D.DeviceArray.from_numpy(wl.U)
+    feats = [(D.DeviceArray.from_numpy(f), w) for f, w in wl.kernels]
+
+    def build_crf():
+        c = D.DenseCRF2D(W, H, L)
+        c.setUnaryEnergy(U_dev)
+        D.synchronize()
+        t = time.perf_counter()
+        for f, w in feats:
+            c.addPairwiseEnergy(f, D.PottsCompatibility(w))
+        D.synchronize()
+        return c, (time.perf_counter() - t) * 1e3
+
+    # the first build in a process also pays the code object's load and the
+    # first allocations; the second is what each further image costs
+    crf, build_first_ms = build_crf()
+    del crf
+    crf, build_ms = build_crf()
     Ms = {2: None, 5: None}
     for k, (f, _) in enumerate(wl.kernels):
         lat = D.Permutohedral().init(f)
@@ -588,6 +599,7 @@ def main():
         "qp_ms_per_solve": round(qp_t / args.steps * 1e3, 3),
         "lp_ms_per_solve": None if args.qp_only else round(lp_t / args.steps * 1e3, 3),
         "lattice_build_ms": round(build_ms, 2),
+        "lattice_build_first_ms": round(build_first_ms, 2),
         "pcie_inclusive_ms_per_solve": None if pcie_ms is None else round(pcie_ms, 3),
         "lp_iterations": lp_info or None,
         "qp_iterations": qp_iters,

@@ -564,10 +564,22 @@ __global__ void __launch_bounds__(kBlock) k_count_rows(const uint8_t* __restrict
     atomicAdd(cnt, (unsigned long long)s);
   }
 }
-__global__ void __launch_bounds__(kBlock) k_fill_u8(uint8_t* __restrict__ p, size_t n, uint8_t v) {
-  const size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
-  if (t < n) p[t] = v;
+// per call: row 0 of both upper halves (read as zero by the full upper pass's
+// blur; the plain filter's rows share the buffers) and row 0 of both code
+// buffers (the zero row) in one launch
+__global__ void __launch_bounds__(kBlock) k_ocoded_reset(float* __restrict__ u0, float* __restrict__ u1, int nrow,
+                                                         uint8_t* __restrict__ c0, uint8_t* __restrict__ c1, int LC) {
+  const int t = (int)(blockIdx.x * kBlock + threadIdx.x);
+  if (t < nrow) {
+    u0[t] = 0.f;
+    u1[t] = 0.f;
+  }
+  if (t < LC) {
+    c0[t] = kCodeZero;
+    c1[t] = kCodeZero;
+  }
 }
+
 
 }  // namespace
 
@@ -615,9 +627,7 @@ void Lattice::ensure_coded_tables(int L) {
     ptb_.resize(npart_);
     pccap_ = (size_t)npart_ * LC;
   }
-  // row 0 of both code buffers: the zero row (a kernel, so that the call stays capturable)
-  k_fill_u8<<<dim3(grid_for(LC)), kBlock, 0, s>>>(cA_.get(), (size_t)LC, kCodeZero);
-  k_fill_u8<<<dim3(grid_for(LC)), kBlock, 0, s>>>(cB_.get(), (size_t)LC, kCodeZero);
+  // row 0 of both code buffers (the zero row) is set by every call's reset kernel
 }
 
 void Lattice::ordered_coded(const float* Q, const float* d_minmax, int L, const SliceArgs& out, OrderedBins* shared,
@@ -639,8 +649,8 @@ void Lattice::ordered_coded(const float* Q, const float* d_minmax, int L, const 
   float* dstU = dstL + half;
   uint8_t* csrc = cA_.get();
   uint8_t* cdst = cB_.get();
-  // row 0 of the upper halves is read by the full upper pass's blur
-  k_zero_rows<<<dim3(grid_for(row), 2), kBlock, 0, s>>>(ZeroRows{{srcU, dstU, nullptr, nullptr}}, (size_t)row);
+  // row 0 of the upper halves and of the code buffers (a kernel, so that the call stays capturable)
+  k_ocoded_reset<<<grid_for((size_t)std::max(row, LC)), kBlock, 0, s>>>(srcU, dstU, row, csrc, cdst, LC);
   const std::string tag = prof_on() ? "[d=" + std::to_string(d_) + ",L=" + std::to_string(L) + "]" : std::string();
   const int LP = obin_stride(L);
   if (shared) {
@@ -659,7 +669,7 @@ void Lattice::ordered_coded(const float* Q, const float* d_minmax, int L, const 
     KTimer tm("obin" + tag, s);
     k_zero_ints<<<1, kBlock, 0, s>>>(edge, (int)edge_ints(nparts_));
     const uint32_t nq = (uint32_t)(L + 3) / 4;
-    k_obin<<<grid_for((size_t)N_ * nq, kBlock, 16384), kBlock, 0, s>>>(Q, d_minmax, N_, L, FastDiv(nq), bins, LP,
+    k_obin<<<grid_for((size_t)N_ * nq), kBlock, 0, s>>>(Q, d_minmax, N_, L, FastDiv(nq), bins, LP,
                                                                         edge, pimg_.n ? pimg_.get() : nullptr,
                                                                         elist);
   }
@@ -719,7 +729,8 @@ void Lattice::ordered_coded(const float* Q, const float* d_minmax, int L, const 
   {  // the upper pass of parts in full mode (exits at once otherwise), in the slice's layout
     KTimer tm("osplat_up" + tag, s);
     auto up = [&](auto kern) {
-      kern<<<std::min(nvb, 2048u), kBlock, 0, s>>>(bins, L, LP, row, nlanes, sp_pix_.get(), sp_w_.get(), seg_.get(),
+      // a small grid: the launch exits at once unless some image is in full mode
+      kern<<<std::min(nvb, 512u), kBlock, 0, s>>>(bins, L, LP, row, nlanes, sp_pix_.get(), sp_w_.get(), seg_.get(),
                                                    chstart_.get(), chpoint_.get(), pstart_.get(), nchunks_,
                                                    orow_.get(), srcU, part_.get(), em);
     };
@@ -730,7 +741,7 @@ void Lattice::ordered_coded(const float* Q, const float* d_minmax, int L, const 
     if (nmulti_ > 0) {
       const int pv = kOBins * L;
       const RowMap rw = rowmap(pv);
-      k_combine_if<float><<<std::min(rows_grid(nmulti_, pv), 1024u), kBlock, 0, s>>>(
+      k_combine_if<float><<<std::min(rows_grid(nmulti_, pv), 256u), kBlock, 0, s>>>(
           pv, row, rw, multi_.get(), nmulti_, chstart_.get(), pstart_.get(), part_.get(), orow_.get(), srcU, em, 1);
     }
     check_launch("k_osplat upper");
