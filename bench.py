@@ -91,7 +91,8 @@ def oracle_parity():
     """GPU results against the FP64 oracle at sizes the oracle finishes in
     seconds: the plain QP (10 iterations) at 320x213 with 40 cells, the LP
     5x10 at 160x120 from each side's own QP solution, and a 3-image batch
-    (fp32 and bf16 ordered-filter rows) against each image's FP64 LP.  Reports
+    (fp32 and bf16 ordered-filter rows) against each image's FP64 LP (the QP
+    of the headline image itself is checked by _fp64_qp_headline).  Reports
     the relative difference of the integer energy of the final labelling
     (both evaluated by the FP64 oracle) and the label agreement."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -168,6 +169,73 @@ def cpu_model():
     return None
 
 
+def _fp64_qp_headline(conn):
+    """Host process: the FP64 oracle's QP (QP_ITERS iterations, the bench's
+    stopping rule) on the headline image and its QP + LP on the 480x270 image
+    of the same generator; given the GPU's labels of each over the pipe, both
+    labellings' integer energies in FP64 and their agreement.  Each receive
+    comes after the oracle's own solve, so the bench's sends never wait long."""
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        from densecrf_ho_amd import synthetic as S
+
+        def crf(wl):
+            c = O.CRF(wl.N, L)
+            c.set_unary(wl.U)
+            for f, w in wl.kernels:
+                c.add_pairwise(f, w)
+            return c
+
+        def score(o, lg, lo, secs):
+            eg, eo = o.energy_true(lg, fp64=True), o.energy_true(lo, fp64=True)
+            return {"energy_rel_diff": abs(eg - eo) / abs(eo), "label_agreement": float((lg == lo).mean()),
+                    "oracle": "fp64", "oracle_s": round(secs, 1)}
+
+        wl = S.Workload(W, H, K, 0, L=L)
+        o = crf(wl)
+        t0 = time.perf_counter()
+        qo, _ = o.qp(O.unary_init(wl.U), max_iter=QP_ITERS, fp64=True)
+        secs = time.perf_counter() - t0
+        lo = qo.argmax(1).astype(np.int32)
+        del qo
+        res = {f"qp_{W}x{H}": score(o, conn.recv(), lo, secs)}
+        del o
+        sw, sh = W // 4, H // 4
+        wl2 = S.Workload(sw, sh, max(1, round(sw * sh / 4000)), 0, L=L)
+        o2 = crf(wl2)
+        t0 = time.perf_counter()
+        q2, _ = o2.lp(o2.qp(O.unary_init(wl2.U), max_iter=QP_ITERS, fp64=True)[0],
+                      O.LPParams(prox_max_iter=LP_PROX, fw_max_iter=LP_FW), fp64=True)
+        secs2 = time.perf_counter() - t0
+        lo2 = q2.argmax(1).astype(np.int32)
+        lg2, lf2 = conn.recv()  # the GPU's and the float oracle's LP labellings
+        res[f"lp_{sw}x{sh}"] = score(o2, lg2, lo2, secs2)
+        if lf2 is not None:  # the reference's float arithmetic against FP64, for scale
+            res[f"lp_{sw}x{sh}_float_oracle_vs_fp64"] = score(o2, lf2, lo2, secs2)
+        conn.send(res)
+    except Exception as e:  # reported in the bench line instead of a result
+        conn.send({"error": repr(e)})
+
+
+def start_fp64_headline():
+    """starts _fp64_qp_headline in a spawned process (before this process
+    touches the GPU); returns (process, parent end of the pipe)"""
+    import multiprocessing as mp
+    ctx = mp.get_context("spawn")
+    a, b = ctx.Pipe()
+    proc = ctx.Process(target=_fp64_qp_headline, args=(b,), daemon=True)
+    proc.start()
+    return proc, a
+
+
+def _send_labels(fp64, msg):
+    try:
+        fp64[1].send(msg)
+    except OSError:  # the oracle process ended early; its error is read later
+        pass
+
+
 def cpu_baseline():
     """The CPU oracle on one host core: a full 10-iteration QP at 1920x1080
     and a full 5x10 LP (from its own 10-iteration QP) on a 480x270 image of the
@@ -195,8 +263,10 @@ def cpu_baseline():
     c2 = crf(wl2)
     qq, _ = c2.qp(O.unary_init(wl2.U), max_iter=QP_ITERS)
     t0 = time.perf_counter()
-    _, sl = c2.lp(qq, O.LPParams(prox_max_iter=LP_PROX, fw_max_iter=LP_FW))
+    ql, sl = c2.lp(qq, O.LPParams(prox_max_iter=LP_PROX, fw_max_iter=LP_FW))
     lp_small_ms = (time.perf_counter() - t0) * 1e3
+    # the float oracle's LP labelling, which oracle_parity scores the GPU's against
+    cpu_baseline.lp_small = (c2, ql.argmax(1).astype(np.int32))
     lp_ms = lp_small_ms * (W * H) / (sw * sh)
     return {"value": round(qp_ms + lp_ms, 1), "unit": "ms/solve", "cores": 1, "kind": "port",
             "cpu_model": cpu_model(), "qp_ms": round(qp_ms, 1), "qp_iterations": int(sq["iters"]),
@@ -445,6 +515,8 @@ def main():
     # host-side generation of the batch first: worker processes start before
     # this process touches the GPU
     imgs = None if (args.no_batch or args.qp_only) else batch_workload(args.batch_images)
+    # the FP64 oracle's QP on the headline image runs on a host core meanwhile
+    fp64 = None if args.no_cpu_baseline else start_fp64_headline()
     if imgs is not None:
         _progress(f"generated {len(imgs)} batch images")
 
@@ -546,6 +618,8 @@ def main():
     # break on a zero numerator or denominator ends it (src/densecrf.cpp:419)
     _, qst = crf.qp_inference_super_pixels_non_convex(Q0, max_iter=QP_ITERS, ref_stop=False, out=Qqp, stats=True)
     qp_iters = int(qst["iters"])
+    if fp64 is not None:
+        _send_labels(fp64, Qqp.numpy().argmax(1).astype(np.int32))
 
     ms_per_step = elapsed / args.steps * 1e3
     # HBM bytes per launch of each kernel, from the committed rocprofv3
@@ -631,6 +705,26 @@ def main():
         out["cpu_baseline"] = cpu_baseline()
         _progress("oracle parity")
         out["oracle_parity"] = oracle_parity()
+        _progress("fp64 oracle QP on the headline image")
+        sw, sh = W // 4, H // 4
+        wl2 = S.Workload(sw, sh, max(1, round(sw * sh / 4000)), 0, L=L)
+        g2 = D.DenseCRF2D(sw, sh, L)
+        g2.setUnaryEnergy(wl2.U)
+        for f, w in wl2.kernels:
+            g2.addPairwiseEnergy(f, D.PottsCompatibility(w))
+        q2 = g2.lp_inference_prox_super_pixels(
+            g2.qp_inference_super_pixels_non_convex(g2.unary_init(), max_iter=QP_ITERS, ref_stop=False), lp_params)
+        c2, lf = cpu_baseline.lp_small
+        lg2 = q2.argmax(1).astype(np.int32)
+        _send_labels(fp64, (lg2, lf))
+        eg2, ef2 = c2.energy_true(lg2, fp64=True), c2.energy_true(lf, fp64=True)
+        out["oracle_parity"][f"lp_{sw}x{sh}_vs_float_oracle"] = {
+            "energy_rel_diff": abs(eg2 - ef2) / abs(ef2), "label_agreement": float((lg2 == lf).mean()),
+            "oracle": "float (the reference's arithmetic)"}
+        del cpu_baseline.lp_small
+        res = fp64[1].recv() if fp64[1].poll(600) else {"error": "no result within 600 s"}
+        out["oracle_parity"].update(res)
+        fp64[0].join(timeout=10)
     print(json.dumps(out), flush=True)
 
 
