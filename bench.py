@@ -268,12 +268,21 @@ def cpu_baseline():
     # the float oracle's LP labelling, which oracle_parity scores the GPU's against
     cpu_baseline.lp_small = (c2, ql.argmax(1).astype(np.int32))
     lp_ms = lp_small_ms * (W * H) / (sw * sh)
+    # the batched workload on the CPU, one image per host core (at most 16,
+    # the share of one GPU box), from the per-pixel cost measured here: an
+    # estimate, the 64 images would take minutes of CPU time
+    cores = min(os.cpu_count() or 1, 16)
+    batch_px = sum(w * h for w, h in (S.batch_size(i) for i in range(BATCH_IMAGES)))
+    c3_s = (qp_ms + lp_ms) * 1e-3 * batch_px / (W * H) / cores
     return {"value": round(qp_ms + lp_ms, 1), "unit": "ms/solve", "cores": 1, "kind": "port",
             "cpu_model": cpu_model(), "qp_ms": round(qp_ms, 1), "qp_iterations": int(sq["iters"]),
             "lp_ms": round(lp_ms, 1), "lp_measured_ms": round(lp_small_ms, 1), "lp_measured_size": f"{sw}x{sh}",
             "lp_inner_steps": int(len(sl["qp_sweeps"])),
             "reference_over_port": REF_OVER_PORT,
             "reference_estimate_ms": round(qp_ms * REF_OVER_PORT["qp"] + lp_ms * REF_OVER_PORT["lp"], 1),
+            "batch_images_per_s_estimate": {"value": round(BATCH_IMAGES / c3_s, 4), "cores": cores,
+                                            "how": f"{BATCH_IMAGES} images, {batch_px} pixels at the per-pixel "
+                                                   "cost of this 1080p solve, one image per core"},
             "sample": (f"CPU oracle (bit-identical with the compiled reference on the 640x480 LP), 1 thread: a full "
                        f"{QP_ITERS}-iteration QP at {W}x{H} and a full {LP_PROX}x{LP_FW} LP at {sw}x{sh}, the LP "
                        f"time scaled by the pixel count to {W}x{H}")}

@@ -122,13 +122,20 @@ def superpixel_layers(W: int, H: int, cells=(300, 600, 1200), seeds=(21, 22, 23)
     return [(voronoi_regions(W, H, k, s), k) for k, s in zip(cells, seeds)]
 
 
-def batch_image(i: int, L: int = 21):
-    """Image i of the batched workload of 64 uneven images: seed 3000+i,
-    H ~ U[240, 1080], W ~ U[320, 1920], one Voronoi cell per 4000 pixels.
-    Returns (W, H, RGB image, unaries)."""
+def batch_size(i: int):
+    """(W, H) of image i of the batched workload: seed 3000+i,
+    H ~ U[240, 1080], W ~ U[320, 1920]."""
     rng = np.random.default_rng(3000 + i)
     H = int(rng.integers(240, 1081))
     W = int(rng.integers(320, 1921))
+    return W, H
+
+
+def batch_image(i: int, L: int = 21):
+    """Image i of the batched workload of 64 uneven images (sizes from
+    batch_size), one Voronoi cell per 4000 pixels.  Returns (W, H, RGB image,
+    unaries)."""
+    W, H = batch_size(i)
     wl = Workload(W, H, max(1, round(W * H / 4000)), 3000 + i, L=L)
     return W, H, wl.img, wl.U
 
