@@ -657,6 +657,19 @@ def main():
             ofb += (b_ if math.isfinite(b_) else 0.0) * c_
             oft += ms_ * 1e-3
     ordered_gbs = ofb / oft / 1e9 if oft > 0 else None
+    # the same time against the dense form's bytes: one ordered call is the
+    # plain filter's byte count with 20 L lattice values per point (lower and
+    # upper 10-bin histograms) and L per pixel, 2SNL + 8N(d+1) +
+    # S M (20L)(2 + 2(d+1)) + 8M(d+1).  Above the HBM peak where the coded
+    # rows and the upper pass's Tb - LO identity skip most of those bytes.
+    dense_b = 0.0
+    for n, (c_, _) in lat_k.items():
+        if n.split("[")[0] in ("oslice", "cslice"):
+            d_ = int(n.split("d=")[1].split(",")[0])
+            M_ = Ms[d_]
+            dense_b += c_ * (2 * 4 * wl.N * L + 8 * wl.N * (d_ + 1) + 4 * M_ * 20 * L * (2 + 2 * (d_ + 1))
+                             + 8 * M_ * (d_ + 1))
+    dense_gbs = dense_b / oft / 1e9 if oft > 0 else None
     # whole-filter rate of the plain bilateral filter inside the QP
     filt = [n for n in lat_k if n.split("[")[0] in ("splat", "blur", "slice") and "d=5" in n]
     fb = sum(algorithmic_bytes(n, wl.N, Ms) * lat_k[n][0] for n in filt)
@@ -690,6 +703,7 @@ def main():
         "ordered_filter": None if ordered_gbs is None else {
             "ms_per_solve": round(oft * 1e3, 3), "achieved": round(ordered_gbs, 1), "unit": "GB/s",
             "frac": round(ordered_gbs / PEAK_HBM_GBS, 4),
+            "dense_form_equivalent_gbps": None if dense_gbs is None else round(dense_gbs, 1),
             "kernels": "obin, oedge, csplat, osplat_up, cblur, cblur_last, oslice / cslice (both terms, every LP step)"},
         "images_per_s": None,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
