@@ -38,7 +38,7 @@ METRIC = "ms/solve QP & LP (1080p, 21 labels) + filter HBM GB/s; batched images/
 
 def algorithmic_bytes(name: str, N: int, M: dict, V: int = L, S: int = 4, prof: dict | None = None) -> float:
     """Bytes a kernel launch must move at minimum (one read of every input it
-    needs, one write of every output), see DESIGN.md "Kernels and rooflines".
+    needs, one write of every output).
     The coded ordered filter's kernels move the rows of mixed labels only:
     their figure is the average over the launches of the profiled step, from
     its counts of stored (row, label) pairs after the splat and each axis."""
@@ -269,7 +269,7 @@ def cpu_baseline():
     cpu_baseline.lp_small = (c2, ql.argmax(1).astype(np.int32))
     lp_ms = lp_small_ms * (W * H) / (sw * sh)
     # the batched workload on the CPU, one image per host core (at most 16,
-    # the share of one GPU box), from the per-pixel cost measured here: an
+    # one GPU's share of a host), from the per-pixel cost measured here: an
     # estimate, the 64 images would take minutes of CPU time
     cores = min(os.cpu_count() or 1, 16)
     batch_px = sum(w * h for w, h in (S.batch_size(i) for i in range(BATCH_IMAGES)))
@@ -518,7 +518,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(images would shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+              "(images would shard across GPUs, one process per GPU, with no exchange)", file=sys.stderr)
         sys.exit(2)
 
     # host-side generation of the batch first: worker processes start before
