@@ -11,12 +11,13 @@ from .api import (DenseCRF, DenseCRF2D, DeviceArray, LP_inf_params, Permutohedra
                   DiagonalCompatibility, MatrixCompatibility, DenseCRFBatch,
                   NO_NORMALIZATION, NORMALIZE_BEFORE, NORMALIZE_AFTER, NORMALIZE_SYMMETRIC, EngineError,
                   profile_enable, profile_filter, profile_report, synchronize, autotune_enable, autotune_report,
+                  set_reference_path, reference_path,
                   load_unary, load_image, save_map, superpixel_terms, less_confident_pixels,
                   PairwisePotential, CONST_KERNEL, DIAG_KERNEL, FULL_KERNEL, UnaryEnergy, ConstUnaryEnergy,
                   LogisticUnaryEnergy)
 from ._lib import lib_path, load  # noqa: F401
 
-__all__ = ["DenseCRF", "DenseCRF2D", "DenseCRFBatch", "DeviceArray", "profile_enable", "profile_filter", "profile_report", "synchronize", "autotune_enable", "autotune_report", "LP_inf_params", "Permutohedral", "PottsCompatibility", "DiagonalCompatibility",
+__all__ = ["DenseCRF", "DenseCRF2D", "DenseCRFBatch", "DeviceArray", "profile_enable", "profile_filter", "profile_report", "synchronize", "autotune_enable", "autotune_report", "set_reference_path", "reference_path", "LP_inf_params", "Permutohedral", "PottsCompatibility", "DiagonalCompatibility",
            "MatrixCompatibility", "EngineError",
            "NO_NORMALIZATION", "NORMALIZE_BEFORE", "NORMALIZE_AFTER", "NORMALIZE_SYMMETRIC", "load", "lib_path",
            "load_unary", "load_image", "save_map", "superpixel_terms", "less_confident_pixels",

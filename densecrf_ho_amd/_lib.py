@@ -38,6 +38,8 @@ class LPStats(C.Structure):
 SIGNATURES = {
     "dcrf_last_error": (C.c_char_p, []),
     "dcrf_version": (C.c_char_p, []),
+    "dcrf_set_reference_path": (_i, [_i]),
+    "dcrf_reference_path": (_i, []),
     "dcrf_lattice_create": (_i, [_vp, _i, _i, _i, C.POINTER(_vp)]),
     "dcrf_lattice_destroy": (_i, [_vp]),
     "dcrf_lattice_info": (_i, [_vp, _ip, _ip]),

@@ -98,6 +98,18 @@ def profile_report() -> dict:
     return out
 
 
+def set_reference_path(portable: bool):
+    """Follow the reference's portable build (True: scalar Permutohedral::init
+    and seqCompute, src/permutohedral.cpp:334-540) or its SSE2 build (False,
+    the default).  Lattices built afterwards use it; built ones keep theirs."""
+    _check(_lib.load().dcrf_set_reference_path(1 if portable else 0))
+
+
+def reference_path() -> bool:
+    """True while new lattices follow the reference's portable build."""
+    return bool(_lib.load().dcrf_reference_path())
+
+
 def autotune_enable(on: bool = True):
     _check(_lib.load().dcrf_autotune_enable(int(on)))
 

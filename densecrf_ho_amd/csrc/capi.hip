@@ -194,6 +194,14 @@ extern "C" {
 const char* dcrf_last_error(void) { return g_last_error.c_str(); }
 const char* dcrf_version(void) { return "densecrf_ho_amd 0.1 (gfx950)"; }
 
+int dcrf_set_reference_path(int portable) {
+  return guarded([&] {
+    DCRF_REQUIRE(portable == 0 || portable == 1, "dcrf_set_reference_path: 0 (SSE2 build) or 1 (portable build)");
+    reference_path().store(portable);
+  });
+}
+int dcrf_reference_path(void) { return reference_path().load(); }
+
 // ---- lattice ------------------------------------------------------------------
 int dcrf_lattice_create(const float* features, int N, int d, int on_device, dcrf_lattice_t* out) {
   return guarded([&] {

@@ -73,7 +73,11 @@ struct ElevConst {
 // ---- Permutohedral::init, per pixel (src/permutohedral.cpp:203-288) --------
 // Pixels at or past N (the padding of the last block of 4) take a zero
 // feature vector and still produce keys, like the reference's SSE lanes.
-template <int D>
+// SCALAR: the portable init of a build without SSE (:364-446): no padding
+// (Np = N), the nearer of floor and ceil with ties down, an int `sum` that
+// adds each rd * down_factor in float and truncates, and the wrap-around of
+// the barycentric coordinates in double.
+template <int D, bool SCALAR>
 __global__ void __launch_bounds__(kBlock) k_elevate(const float* __restrict__ feat, int N, int Np, ElevConst c, PKey<D>* __restrict__ ekeys,
                           int* __restrict__ rank_out, float* __restrict__ bary_out) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -92,12 +96,25 @@ __global__ void __launch_bounds__(kBlock) k_elevate(const float* __restrict__ fe
   }
   elev[0] = sm;
   float sum = 0.f;
+  if constexpr (SCALAR) {
+    int isum = 0;
 #pragma unroll
-  for (int j = 0; j < D1; ++j) {
-    float v = c.invdp1 * elev[j];
-    v = __builtin_rintf(v);  // round half to even, as _mm_cvtps_epi32
-    rem0[j] = v * c.dp1;
-    sum = sum + v;
+    for (int j = 0; j < D1; ++j) {
+      const float v = c.invdp1 * elev[j];
+      const float up = ceilf(v) * c.dp1, down = floorf(v) * c.dp1;
+      const int rd = (up - elev[j] < elev[j] - down) ? (int)(short)up : (int)(short)down;
+      rem0[j] = (float)rd;
+      isum = (int)((float)isum + (float)rd * c.invdp1);
+    }
+    sum = (float)isum;
+  } else {
+#pragma unroll
+    for (int j = 0; j < D1; ++j) {
+      float v = c.invdp1 * elev[j];
+      v = __builtin_rintf(v);  // round half to even, as _mm_cvtps_epi32
+      rem0[j] = v * c.dp1;
+      sum = sum + v;
+    }
   }
 #pragma unroll
   for (int j = 0; j < D1; ++j) rk[j] = 0.f;
@@ -135,7 +152,10 @@ __global__ void __launch_bounds__(kBlock) k_elevate(const float* __restrict__ fe
     }
     if (p + 1 == D1) bc[D1] = bc[D1] - v;
   }
-  bc[0] = bc[0] + (1.f + bc[D + 1]);
+  if constexpr (SCALAR)
+    bc[0] = (float)((double)bc[0] + (1.0 + (double)bc[D + 1]));
+  else
+    bc[0] = bc[0] + (1.f + bc[D + 1]);
   int rki[D1];
 #pragma unroll
   for (int j = 0; j < D1; ++j) rki[j] = (int)rk[j];
@@ -1319,7 +1339,7 @@ void scan_excl(const int* in, int* out, int n, hipStream_t s) {
 }
 
 template <int D>
-int build_tables(const float* feat, int N, int Np, hipStream_t s, DBuf<int>& offset, DBuf<int>& rank,
+int build_tables(const float* feat, int N, int Np, bool scalar, hipStream_t s, DBuf<int>& offset, DBuf<int>& rank,
                  DBuf<float>& bary, DBuf<int16_t>& keys, DBuf<int>& nbr) {
   constexpr int D1 = D + 1;
   const int E = Np * D1;
@@ -1333,7 +1353,10 @@ int build_tables(const float* feat, int N, int Np, hipStream_t s, DBuf<int>& off
   offset.resize(E);
   rank.resize(E);
   bary.resize(E);
-  k_elevate<D><<<grid_for(Np), kBlock, 0, s>>>(feat, N, Np, c, ekeys.get(), rank.get(), bary.get());
+  if (scalar)
+    k_elevate<D, true><<<grid_for(Np), kBlock, 0, s>>>(feat, N, Np, c, ekeys.get(), rank.get(), bary.get());
+  else
+    k_elevate<D, false><<<grid_for(Np), kBlock, 0, s>>>(feat, N, Np, c, ekeys.get(), rank.get(), bary.get());
   check_launch("k_elevate");
 
   uint32_t cap = 1024;
@@ -1367,8 +1390,8 @@ int build_tables(const float* feat, int N, int Np, hipStream_t s, DBuf<int>& off
   return M;
 }
 
-using BuildFn = int (*)(const float*, int, int, hipStream_t, DBuf<int>&, DBuf<int>&, DBuf<float>&, DBuf<int16_t>&,
-                        DBuf<int>&);
+using BuildFn = int (*)(const float*, int, int, bool, hipStream_t, DBuf<int>&, DBuf<int>&, DBuf<float>&,
+                        DBuf<int16_t>&, DBuf<int>&);
 BuildFn build_fn(int d) {
   switch (d) {
     case 1: return build_tables<1>;
@@ -1389,6 +1412,11 @@ BuildFn build_fn(int d) {
 
 }  // namespace
 
+std::atomic<int>& reference_path() {
+  static std::atomic<int> path{0};
+  return path;
+}
+
 Lattice::Lattice(hipStream_t s) : stream_(s) {}
 Lattice::~Lattice() = default;
 
@@ -1398,10 +1426,11 @@ void Lattice::build(const float* d_feat, int N, int d) {
   DCRF_REQUIRE(fn != nullptr, "lattice: feature dimension must be in [1, 12]");
   N_ = N;
   d_ = d;
-  Np_ = (N + 3) / 4 * 4;
+  scalar_ = reference_path().load() != 0;
+  Np_ = scalar_ ? N : (N + 3) / 4 * 4;  // the SSE build pads to blocks of four pixels
   const int d1 = d + 1;
   hipStream_t s = stream_;
-  M_ = fn(d_feat, N, Np_, s, offset_, rank_, bary_, keys_, nbr_);
+  M_ = fn(d_feat, N, Np_, scalar_, s, offset_, rank_, bary_, keys_, nbr_);
 
   // splat gather lists: entries of real pixels sorted by (point, pixel)
   const int E = N * d1, M = M_;
@@ -1476,7 +1505,7 @@ void Lattice::filter(const SplatArgs& in, const SliceArgs& out, int V, bool reve
   hipStream_t s = stream_;
   const RowMap rm = rowmap(V);
   const std::string tag = prof_on() ? "[d=" + std::to_string(d_) + ",V=" + std::to_string(V) + "]" : std::string();
-  const bool seq = V <= 2;
+  const bool seq = V <= 2 || scalar_;  // seqCompute; every value size in the portable build
   const float* src = splat_blur(in, V, reverse, seq, tag);
   const float alpha = lattice_alpha(d_);
   KTimer tm("slice" + tag, s);
@@ -1840,6 +1869,9 @@ std::unique_ptr<Lattice> Lattice::concat(const std::vector<const Lattice*>& part
   auto u = std::make_unique<Lattice>(s);
   u->N_ = (int)N;
   u->Np_ = (int)N;
+  u->scalar_ = parts[0]->scalar_;
+  for (const Lattice* p : parts)
+    DCRF_REQUIRE(p->scalar_ == u->scalar_, "lattice union: parts built for different reference builds");
   u->d_ = d;
   u->M_ = (int)M;
   u->nchunks_ = (int)nch;

@@ -8,6 +8,8 @@
 // reference's HashTable (src/permutohedral.cpp:100-128).
 #pragma once
 
+#include <atomic>
+
 #include <memory>
 #include <string>
 #include <vector>
@@ -57,6 +59,13 @@ struct SliceArgs {
   bool accumulate = false;      // out = out + x instead of out = x
   float post_mult = 1.f;        // exact multiplier (1 or 2) applied last
 };
+
+// Which build of the reference the lattice follows: the SSE2 build compiled on
+// x86-64 (0, the default: blocks of four pixels with padding lanes, round half
+// to even, sseCompute for more than two values) or the portable build where
+// __SSE__ is not defined (1: src/permutohedral.cpp:334-486, seqCompute for
+// every value size).  Process-wide, read when a lattice is built.
+std::atomic<int>& reference_path();
 
 class Lattice {
  public:
@@ -153,6 +162,7 @@ class Lattice {
   void ensure_total_weight();
   hipStream_t stream_;
   int N_ = 0, d_ = 0, Np_ = 0, M_ = 0;
+  bool scalar_ = false;  // built as the reference's portable build (reference_path() == 1)
   int nchunks_ = 0, nmulti_ = 0, npart_ = 0;
   DBuf<int> offset_, rank_;
   DBuf<float> bary_;

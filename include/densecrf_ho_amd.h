@@ -61,6 +61,21 @@ const char* dcrf_last_error(void);
 /* library version string */
 const char* dcrf_version(void);
 
+/* Which build of the reference the lattices follow.  The reference's
+ * Permutohedral compiles one of two paths (src/permutohedral.cpp:36-40):
+ *   0  the SSE2 build, what x86-64 compilers produce (the default): init in
+ *      blocks of four pixels with padding lanes and round-half-even
+ *      (:152-333), compute() through sseCompute for more than two values
+ *      (:998-1058);
+ *   1  the portable build, where __SSE__ is not defined: init one pixel at a
+ *      time with ties rounded down (:334-486), compute() through seqCompute
+ *      for every value size (:488-540, :1059-1064).
+ * Process-wide; read when a lattice is built (dcrf_lattice_create, and the
+ * potentials of dcrf_pairwise_create / dcrf_crf_add_pairwise*).  A lattice
+ * keeps the path it was built with.  Returns nonzero for other values. */
+int dcrf_set_reference_path(int portable);
+int dcrf_reference_path(void);
+
 /* ------------------------------------------------------------------------ */
 /* Permutohedral lattice (include/permutohedral.h:50-88)                     */
 /* ------------------------------------------------------------------------ */

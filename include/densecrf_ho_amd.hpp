@@ -92,6 +92,12 @@ inline void check(int rc) {
   if (rc != DCRF_OK) throw std::runtime_error(dcrf_last_error());
 }
 
+// Follow the reference's portable build (scalar Permutohedral::init and
+// seqCompute, src/permutohedral.cpp:334-540) instead of its SSE2 build for
+// lattices built from now on.
+inline void set_reference_path(bool portable) { check(dcrf_set_reference_path(portable ? 1 : 0)); }
+inline bool reference_path() { return dcrf_reference_path() != 0; }
+
 enum NormalizationType { NO_NORMALIZATION, NORMALIZE_BEFORE, NORMALIZE_AFTER, NORMALIZE_SYMMETRIC };
 enum KernelType { CONST_KERNEL, DIAG_KERNEL, FULL_KERNEL };
 

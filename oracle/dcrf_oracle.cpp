@@ -37,6 +37,10 @@ namespace orc {
 //   7 superpixel sums over member ids   8 ucondgrad without int truncation
 //   9 restricted ordered filter blurring every point
 static int g_mut = 0;
+// 1: the reference as built where __SSE__ is not defined: the portable init
+// (src/permutohedral.cpp:334-486) and seqCompute for every value size
+// (:1059-1064); 0: the SSE2 build, compiled on x86-64
+static int g_scalar = 0;
 
 // ---------------------------------------------------------------------------
 // Reduction order of Eigen 3.2.7 on SSE2 (include/Eigen/src/Core/Redux.h:193-245,
@@ -228,7 +232,7 @@ struct Lattice {
   void init(const float* feat, int N_, int d_) {
     N = N_;
     d = d_;
-    Npad = g_mut == 1 ? N : (N + 3) / 4 * 4;
+    Npad = (g_mut == 1 || g_scalar) ? N : (N + 3) / 4 * 4;  // the portable init has no lanes
     const int d1 = d + 1;
     offset.assign((size_t)Npad * d1, 0);
     rank.assign((size_t)Npad * d1, 0);
@@ -258,11 +262,25 @@ struct Lattice {
       }
       elev[0] = sm;
       float sum = 0.f;  // round to the 0-coloured simplex (:222-232)
-      for (int i = 0; i <= d; ++i) {
-        float v = invdp1 * elev[i];
-        v = g_mut == 2 ? std::round(v) : std::nearbyint(v);
-        rem0[i] = v * dp1;
-        sum = sum + v;
+      if (g_scalar) {
+        // portable init (:386-402): the nearer of floor and ceil, ties down;
+        // `int sum += rd2 * down_factor` adds in float and truncates
+        int isum = 0;
+        for (int i = 0; i <= d; ++i) {
+          const float v = invdp1 * elev[i];
+          const float up = std::ceil(v) * dp1, down = std::floor(v) * dp1;
+          const int rd = (up - elev[i] < elev[i] - down) ? (int)(short)up : (int)(short)down;
+          rem0[i] = (float)rd;
+          isum = (int)((float)isum + (float)rd * invdp1);
+        }
+        sum = (float)isum;
+      } else {
+        for (int i = 0; i <= d; ++i) {
+          float v = invdp1 * elev[i];
+          v = g_mut == 2 ? std::round(v) : std::nearbyint(v);
+          rem0[i] = v * dp1;
+          sum = sum + v;
+        }
       }
       for (int i = 0; i <= d; ++i) rk[i] = 0.f;  // rank (:235-245)
       for (int i = 0; i < d; ++i) {
@@ -289,7 +307,10 @@ struct Lattice {
         bc[p] = bc[p] + v;
         bc[p + 1] = bc[p + 1] - v;
       }
-      bc[0] = bc[0] + (1.f + bc[d + 1]);
+      if (g_scalar)
+        bc[0] = (float)((double)bc[0] + (1.0 + (double)bc[d + 1]));  // :438, a double expression
+      else
+        bc[0] = bc[0] + (1.f + bc[d + 1]);
       for (int r = 0; r <= d; ++r) {  // keys (:280-287)
         for (int i = 0; i < d; ++i)
           key[i] = (int16_t)(int)(rem0[i] + (float)canonical[r * d1 + (int)rk[i]]);
@@ -323,6 +344,7 @@ struct Lattice {
   // value buffer is the zero pad for absent neighbours.
   void filter_f(const float* in, float* out, int V, bool reverse) const {
     const int d1 = d + 1;
+    const bool seq = V <= 2 || g_scalar;  // seqCompute (every value size in the portable build)
     std::vector<float> a((size_t)(M + 2) * V, 0.f), b((size_t)(M + 2) * V, 0.f);
     float* values = a.data();
     float* nv = b.data();
@@ -342,7 +364,7 @@ struct Lattice {
         const float* v2 = values + (size_t)n2 * V;
         for (int k = 0; k < V; ++k) {
           float s = v1[k] + v2[k];
-          if (V <= 2 && g_mut != 4)
+          if (seq && g_mut != 4)
             nw[k] = (float)((double)ov[k] + 0.5 * (double)s);  // :518 (double expression)
           else
             nw[k] = ov[k] + 0.5f * s;  // :1035
@@ -357,7 +379,7 @@ struct Lattice {
       for (int j = 0; j <= d; ++j) {
         int o = offset[(size_t)i * d1 + j] + 1;
         float w = bary[(size_t)i * d1 + j];
-        if (V <= 2) {
+        if (seq) {
           for (int k = 0; k < V; ++k) acc[k] = acc[k] + (w * values[(size_t)o * V + k]) * alpha;  // :533
         } else {
           float wa = w * alpha;  // :1048
@@ -1567,6 +1589,7 @@ extern "C" {
 
 const char* orc_last_error() { return g_err.c_str(); }
 void orc_set_mutation(int m) { orc::g_mut = m; }
+void orc_set_reference_path(int scalar) { orc::g_scalar = scalar ? 1 : 0; }
 
 void* orc_lattice_new(const float* feat, int N, int d) {
   Lattice* L = new Lattice();

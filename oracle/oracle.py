@@ -20,6 +20,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # `make -C oracle check-asan`)
 ORACLE_SO = os.environ.get("ORACLE_LIB", os.path.join(_HERE, "liboracle.so"))
 REF_SO = os.path.join(_HERE, "_ref", "libdcrf_ref.so")
+# the reference's Permutohedral compiled without its SSE code (portable build)
+REFS_SO = os.path.join(_HERE, "_ref", "libdcrf_ref_scalar.so")
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -98,6 +100,7 @@ def lib():
         _sig(L, "orc_feasible", None, _f32p, _f32p, _i, _i)
         _sig(L, "orc_renormalize", None, _f32p, _i, _i)
         _sig(L, "orc_set_mutation", None, _i)
+        _sig(L, "orc_set_reference_path", None, _i)
         _orc = L
     return _orc
 
@@ -107,7 +110,8 @@ def _c(a, dt):
 
 
 class Lattice:
-    """Oracle lattice built with the reference's SSE2 semantics."""
+    """Oracle lattice built with the reference's SSE2 semantics, or its
+    portable build's after set_reference_path(True)."""
 
     def __init__(self, feat: np.ndarray):
         feat = _c(feat, np.float32)
@@ -398,6 +402,14 @@ class CRF:
 def set_mutation(m: int):
     """Plant deliberate mistake `m` in the oracle (tests only; 0 restores it)."""
     lib().orc_set_mutation(int(m))
+
+
+def set_reference_path(portable: bool):
+    """Restate the reference's portable build (scalar init with ties rounded
+    down, no padding lanes, seqCompute for every value size;
+    src/permutohedral.cpp:334-540) instead of its SSE2 build for lattices
+    built and filtered from now on."""
+    lib().orc_set_reference_path(1 if portable else 0)
 
 
 def unary_init(U):
@@ -732,3 +744,62 @@ def ref_renormalize(Q):
     Q = _c(Q, np.float32).copy()
     reflib().ref_renormalize(Q, Q.shape[0], Q.shape[1])
     return Q
+
+
+_refs = None
+
+
+def refs_available() -> bool:
+    return os.path.exists(REFS_SO)
+
+
+class RefScalarLattice(RefLattice):
+    """the reference's Permutohedral as its portable build computes it"""
+
+    def __init__(self, feat):
+        global _refs
+        if _refs is None:
+            if not refs_available():
+                raise RuntimeError("portable reference build missing (make -C oracle ref-scalar)")
+            R = C.CDLL(REFS_SO)
+            _sig(R, "refs_lattice_new", _vp, _f32p, _i, _i)
+            _sig(R, "refs_lattice_free", None, _vp)
+            _sig(R, "refs_lattice_M", _i, _vp)
+            _sig(R, "refs_lattice_get", None, _vp, _i32p, _i32p, _f32p, _i32p, _i)
+            _sig(R, "refs_lattice_compute", None, _vp, _f32p, _f32p, _i, _i, _i)
+            _sig(R, "refs_lattice_ordered", None, _vp, _f32p, _f32p, _i, _i)
+            _refs = R
+        feat = _c(feat, np.float32)
+        self.N, self.d = feat.shape
+        self._S = _refs
+        self.h = self._S.refs_lattice_new(feat, self.N, self.d)
+        self.M = self._S.refs_lattice_M(self.h)
+        self.Npad = self.N
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._S.refs_lattice_free(self.h)
+            self.h = None
+
+    def arrays(self):
+        d1 = self.d + 1
+        off = np.zeros(self.N * d1, np.int32)
+        rk = np.zeros(self.N * d1, np.int32)
+        bc = np.zeros(self.N * d1, np.float32)
+        nbr = np.zeros(max(d1 * self.M * 2, 1), np.int32)
+        self._S.refs_lattice_get(self.h, off, rk, bc, nbr, self.N)
+        return dict(offset=off.reshape(self.N, d1), rank=rk.reshape(self.N, d1),
+                    bary=bc.reshape(self.N, d1), nbr=nbr[: d1 * self.M * 2].reshape(d1, self.M, 2))
+
+    def filter(self, x, reverse=False):
+        x = _c(x, np.float32)
+        V = x.shape[1] if x.ndim == 2 else 1
+        out = np.zeros_like(x)
+        self._S.refs_lattice_compute(self.h, x, out, V, self.N, int(reverse))
+        return out
+
+    def ordered(self, q):
+        q = _c(q, np.float32)
+        out = np.zeros_like(q)
+        self._S.refs_lattice_ordered(self.h, q, out, q.shape[1], self.N)
+        return out

@@ -15,6 +15,7 @@ compressed .npz files.  Run from the repository root:
     python tests/golden/make_golden.py dc       # only dc_L4.npz
     python tests/golden/make_golden.py eho      # only eho_32x24_L21.npz
     python tests/golden/make_golden.py logistic # only logistic_unary.npz
+    python tests/golden/make_golden.py portable # only lattice_portable.npz (make -C oracle ref-scalar)
 """
 
 from __future__ import annotations
@@ -502,7 +503,33 @@ def logistic_case():
     print("logistic_unary")
 
 
+def portable_case():
+    """lattices of the reference's portable build (no __SSE__): tables, the
+    filter at V=1 and V=21 both ways and the ordered filter, for features with
+    exact rounding ties and two image kernels"""
+    wl = S.Workload(31, 23, 8, 1)
+    rng = np.random.default_rng(21)
+    cases = {"ties_d1": tie_features(61, 7), "b5_31x23": wl.kernels[1][0], "g2_31x23": wl.kernels[0][0],
+             "r3_101": (rng.standard_normal((101, 3)) * 3).astype(np.float32)}
+    d = {}
+    for k, feat in cases.items():
+        lat = O.RefScalarLattice(feat)
+        A = lat.arrays()
+        x = rng.standard_normal((lat.N, 21)).astype(np.float32)
+        x1 = (rng.random((lat.N, 1)) + 0.5).astype(np.float32)
+        q = edge_q(lat.N, 21, 22)
+        d.update({f"{k}_feat": feat, f"{k}_M": np.int32(lat.M), f"{k}_offset": A["offset"], f"{k}_rank": A["rank"],
+                  f"{k}_bary": A["bary"], f"{k}_nbr": A["nbr"], f"{k}_x": x, f"{k}_out": lat.filter(x),
+                  f"{k}_out_rev": lat.filter(x, reverse=True), f"{k}_x1": x1, f"{k}_out1": lat.filter(x1),
+                  f"{k}_out1_rev": lat.filter(x1, reverse=True), f"{k}_q": q, f"{k}_ord": lat.ordered(q)})
+        print(f"portable {k}: N={lat.N} d={feat.shape[1]} M={lat.M}")
+    np.savez_compressed(os.path.join(OUT, "lattice_portable.npz"), names=np.array(list(cases)), **d)
+
+
 def main():
+    if sys.argv[1:] == ["portable"]:
+        portable_case()
+        return
     if sys.argv[1:] == ["logistic"]:
         logistic_case()
         return
@@ -554,6 +581,7 @@ def main():
     data_case()
     data_lp_steps_case()
     c0_case()
+    portable_case()
 
 
 if __name__ == "__main__":

@@ -44,6 +44,39 @@ def test_filters_vs_reference(golden, name):
     assert rel_err(lat.compute_upper_minus_lower_ord(g["q"]), g["ord"]) < 1e-5
 
 
+@pytest.mark.parametrize("name", ["ties_d1", "b5_31x23", "g2_31x23", "r3_101"])
+def test_portable_build_vs_reference(golden, name):
+    """set_reference_path(True): tables bit-identical to the reference's
+    portable build, filters at V=1 and V=21 both ways within 1e-5 (the splat
+    and the double blur of seqCompute), the ordered filter within 1e-5; the
+    default path returns afterwards"""
+    import densecrf_ho_amd as D
+    g = golden("lattice_portable.npz")
+    print("case", name, flush=True)
+    D.set_reference_path(True)
+    try:
+        assert D.reference_path()
+        lat = gpu_lattice(g[f"{name}_feat"])
+    finally:
+        D.set_reference_path(False)
+    assert not D.reference_path()
+    T = lat.tables()
+    assert lat.M == int(g[f"{name}_M"])
+    for k in ("offset", "rank", "bary", "nbr"):
+        np.testing.assert_array_equal(T[k], g[f"{name}_{k}"], err_msg=k)
+    for x, o in (("x", "out"), ("x1", "out1")):
+        assert rel_err(lat.compute(g[f"{name}_{x}"]), g[f"{name}_{o}"]) < 1e-5
+        assert rel_err(lat.compute(g[f"{name}_{x}"], reverse=True), g[f"{name}_{o}_rev"]) < 1e-5
+    assert rel_err(lat.compute_upper_minus_lower_ord(g[f"{name}_q"]), g[f"{name}_ord"]) < 1e-5
+
+
+def test_reference_path_rejects_other_values():
+    import densecrf_ho_amd as D
+    from densecrf_ho_amd import _lib
+    assert _lib.load().dcrf_set_reference_path(2) != 0
+    assert not D.reference_path()
+
+
 @pytest.mark.parametrize("W,H,K,d", [(320, 213, 40, 5), (641, 479, 120, 2), (641, 479, 120, 5)])
 def test_image_lattice_vs_oracle(W, H, K, d):
     """Voronoi images (641x479 has N % 4 == 3: padding lanes)."""

@@ -99,6 +99,46 @@ def test_filter_matches_reference(golden, name):
     np.testing.assert_array_equal(lat.ordered(g["q"]), g["ord"])
 
 
+PORTABLE = ["ties_d1", "b5_31x23", "g2_31x23", "r3_101"]
+
+
+@pytest.mark.parametrize("name", PORTABLE)
+def test_portable_build_matches_reference(golden, name):
+    """the oracle's restatement of the reference's portable build (no __SSE__:
+    ties rounded down, no padding lanes, seqCompute at every value size)
+    against lattice_portable.npz, written by that build of the reference"""
+    g = golden("lattice_portable.npz")
+    O.set_reference_path(True)
+    try:
+        lat = O.Lattice(g[f"{name}_feat"])
+        A = lat.arrays()
+        assert lat.M == int(g[f"{name}_M"]) and lat.Npad == lat.N
+        for k in ("offset", "rank", "bary", "nbr"):
+            np.testing.assert_array_equal(A[k], g[f"{name}_{k}"], err_msg=k)
+        np.testing.assert_array_equal(lat.filter(g[f"{name}_x"]), g[f"{name}_out"])
+        np.testing.assert_array_equal(lat.filter(g[f"{name}_x"], reverse=True), g[f"{name}_out_rev"])
+        np.testing.assert_array_equal(lat.filter(g[f"{name}_x1"]), g[f"{name}_out1"])
+        np.testing.assert_array_equal(lat.filter(g[f"{name}_x1"], reverse=True), g[f"{name}_out1_rev"])
+        np.testing.assert_array_equal(lat.ordered(g[f"{name}_q"]), g[f"{name}_ord"])
+    finally:
+        O.set_reference_path(False)
+
+
+def test_portable_build_differs_where_it_should(golden):
+    """the two builds differ exactly where the fixtures say: exact .5 ties of
+    ties_d1 round the other way (different tables), and at V=21 the SSE
+    build's float blur differs from seqCompute's double blur; the SSE-mode
+    oracle still reproduces the SSE fixture afterwards"""
+    g = golden("lattice_portable.npz")
+    sse = O.Lattice(g["ties_d1_feat"])
+    assert not np.array_equal(sse.arrays()["offset"][: sse.N], g["ties_d1_offset"]) or \
+        not np.array_equal(sse.arrays()["bary"][: sse.N], g["ties_d1_bary"])
+    lat = O.Lattice(g["b5_31x23_feat"])
+    assert not np.array_equal(lat.filter(g["b5_31x23_x"]), g["b5_31x23_out"])
+    ref = golden("lattice_ties_d1.npz")
+    np.testing.assert_array_equal(O.Lattice(ref["feat"]).arrays()["offset"], ref["offset"])
+
+
 @pytest.mark.parametrize("name", LATTICES)
 def test_lattice_invariants(golden, name):
     g, lat = _lat(golden, name)
