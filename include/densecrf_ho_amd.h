@@ -72,7 +72,9 @@ const char* dcrf_version(void);
  *      for every value size (:488-540, :1059-1064).
  * Process-wide; read when a lattice is built (dcrf_lattice_create, and the
  * potentials of dcrf_pairwise_create / dcrf_crf_add_pairwise*).  A lattice
- * keeps the path it was built with.  Returns nonzero for other values. */
+ * keeps the path it was built with.  Only the lattice follows the switch:
+ * column sums (renormalize, feasible_Q, qp-gamma) keep the order of Eigen's
+ * 4-float packets either way.  Returns nonzero for other values. */
 int dcrf_set_reference_path(int portable);
 int dcrf_reference_path(void);
 
