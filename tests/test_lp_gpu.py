@@ -65,6 +65,39 @@ def test_lp_vs_reference(golden, case):
     assert rel_err(Q, g["Qlp"]) < 1e-3
 
 
+def test_portable_build_qp_lp_vs_oracle(golden):
+    """QP and LP on lattices of the reference's portable build (engine and
+    oracle both switched): the same iteration and sweep counts, energies within
+    1e-4, labels >= 99.9 %; 21 labels, so every filter takes seqCompute's
+    double blur"""
+    import densecrf_ho_amd as D
+    g = golden("crf_32x24_L21_ho.npz")
+    D.set_reference_path(True)
+    O.set_reference_path(True)
+    try:
+        c = gpu_crf(g)
+        o = orc_crf(g)
+        Q0 = O.unary_init(g["U"])
+        Qq, stq = c.qp_inference_super_pixels_non_convex(Q0, max_iter=10, ref_stop=False, stats=True)
+        Qqo, stqo = o.qp(Q0, max_iter=10)
+        print("qp energies gpu", stq["energy"], "oracle", stqo["energy"], flush=True)
+        assert stq["iters"] == stqo["iters"]
+        np.testing.assert_allclose(stq["energy"], stqo["energy"], rtol=1e-4)
+        assert (Qq.argmax(1) == Qqo.argmax(1)).mean() >= 0.999
+        P = D.LP_inf_params(prox_max_iter=int(g["lp_outer_cap"]))
+        Q, st = c.lp_inference_prox_super_pixels(Qqo, P, stats=True)
+        Qo, sto = o.lp(Qqo, O.LPParams(prox_max_iter=int(g["lp_outer_cap"])))
+        print("lp sweeps gpu", st["qp_sweeps"][:20], "oracle", sto["qp_sweeps"][:20], flush=True)
+        np.testing.assert_array_equal(st["qp_sweeps"], sto["qp_sweeps"])
+        assert st["outer"] == sto["outer"]
+        np.testing.assert_allclose(st["int_energy"], sto["int_energy"], rtol=1e-4)
+        assert (Q.argmax(1) == Qo.argmax(1)).mean() >= 0.999
+        assert rel_err(Q, Qo) < 1e-3
+    finally:
+        D.set_reference_path(False)
+        O.set_reference_path(False)
+
+
 def test_lp_less_confident_stop_vs_reference(golden):
     """the less-confident stop (src/densecrf.cpp:784-794) ends the GPU LP at
     the oracle's outer step for each setting, with the reference's result"""
